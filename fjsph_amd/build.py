@@ -1,0 +1,54 @@
+"""Build the engine library fjsph_amd/libfjsph_engine.so for gfx950.
+
+hipcc compiles the kernels and the host C-ABI in one shared object.  Packed
+FP32 instructions are disabled (-packed-fp32-ops) on purpose; the FP32 paths
+are scalar.  The CPU restatement used as the test oracle is built separately
+by oracle/Makefile.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libfjsph_engine.so")
+SOURCES = ["kernels.hip", "engine.cpp", "settings.cpp"]
+ARCH = os.environ.get("FJSPH_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False):
+    objs = []
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    common = [
+        f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wall", "-Wno-unused-function",
+    ]
+    for src in SOURCES:
+        obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
+        lang = ["-x", "hip"] if src.endswith((".hip", ".cpp")) and src != "settings.cpp" else []
+        extra = ["-ffp-contract=off"] if src == "settings.cpp" else []
+        cmd = [hipcc()] + common + extra + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

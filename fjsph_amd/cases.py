@@ -1,0 +1,279 @@
+"""Seeded initial conditions for the benchmark and parity workloads.
+
+Each builder returns a Case: host-order arrays (boundary particles first, then
+fluid, as Init_Particles orders them, src/Init.cpp:288-484), the settings and
+the block table.  Lattices follow the reference's conventions: a cubic
+lattice clipped to a sphere for droplets (src/shapes/circle.cpp:163-189),
+square/cube blocks indexed i fastest (src/shapes/square.cpp:112-138) and
+layered walls (src/shapes/line.cpp:140-154).  Choices the reference leaves to
+the user (wall placement, jitter, unstated constants) are stated per builder.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class Case:
+    name: str
+    settings: abi.Settings
+    x: np.ndarray
+    v: np.ndarray
+    rho: np.ndarray
+    p: np.ndarray
+    m: np.ndarray
+    b: np.ndarray
+    n_bound: int
+    blocks: list = field(default_factory=list)
+    notes: str = ""
+
+    @property
+    def n(self):
+        return int(self.x.shape[0])
+
+    @property
+    def dim(self):
+        return int(self.settings.dim)
+
+
+def _mass(s):
+    return s.rho_rest * s.particle_step ** s.dim
+
+
+def _finish(name, s, xb, xf, vf, notes, bound_solver=abi.PRESSURE_G, fluid_type=abi.FREE):
+    dim = s.dim
+    xb = np.asarray(xb, dtype=np.float64).reshape(-1, dim)
+    xf = np.asarray(xf, dtype=np.float64).reshape(-1, dim)
+    nb, nf = xb.shape[0], xf.shape[0]
+    x = np.concatenate([xb, xf], axis=0)
+    v = np.zeros_like(x)
+    v[nb:] = np.asarray(vf, dtype=np.float64).reshape(-1, dim) if np.ndim(vf) > 1 else np.asarray(vf, dtype=np.float64)
+    rho = np.full(nb + nf, s.rho_rest)
+    p = np.zeros(nb + nf)
+    m = np.full(nb + nf, _mass(s))
+    b = np.full(nb + nf, fluid_type, dtype=np.int32)
+    b[:nb] = abi.BOUND
+    blocks = []
+    if nb:
+        blocks.append(dict(first=0, last=nb, is_fluid=0, bound_solver=bound_solver))
+    blocks.append(dict(first=nb, last=nb + nf, is_fluid=1))
+    return Case(name, s, np.ascontiguousarray(x), np.ascontiguousarray(v), rho, p, m, b, nb, blocks, notes)
+
+
+def dam_break_2d(dx=0.02, precision=abi.PREC_FP64, width=1.0, height=2.0, tank=(4.0, 3.0), layers=3):
+    """Plain 2D WCSPH dam break: a 1.0 x 2.0 m column in a 4.0 x 3.0 m tank.
+
+    dx = 0.02 gives 50 x 100 = 5,000 fluid particles.  Wendland C2 with
+    H = 1.3 dx, Cole EOS with rho0 = 1000 and c0 = 10 sqrt(2 g H_col),
+    delta = 0.1, alpha = 0.1, g = -9.81, mu = sigma = 0 (as in
+    Examples/Dam_2D/para), Adami boundary pressure (Examples/Dam_2D/
+    boundary.bmap:9), fixed CFL 0.3.  Fluid particles sit at cell centres
+    (i + 1/2) dx inside [0, W] x [0, H]; the walls are `layers` rows of
+    particles outside the tank at -(k + 1/2) dx, corners included.  No
+    jitter (the reference's own jitter is below 1e-17 dx).
+    """
+    s = abi.default_settings(2)
+    s.particle_step = dx
+    s.H_fac = 1.3
+    s.rho_rest = 1000.0
+    s.speed_sound = 10.0 * math.sqrt(2.0 * 9.81 * height)
+    s.mu = 0.0
+    s.sig = 0.0
+    s.visc_alpha = 0.1
+    s.dsph_delta = 0.1
+    s.cfl = s.cfl_min = s.cfl_max = 0.3
+    s.precision = precision
+    s.aero_case = abi.NO_AERO
+    nx, ny = int(round(width / dx)), int(round(height / dx))
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    xf = np.stack([(i.ravel() + 0.5) * dx, (j.ravel() + 0.5) * dx], axis=1)
+    tw, th = tank
+    ntx = int(round(tw / dx))
+    nty = int(round(th / dx))
+    walls = []
+    for k in range(layers):
+        off = (k + 0.5) * dx
+        # bottom row spans the corners
+        xs = (np.arange(-layers, ntx + layers) + 0.5) * dx
+        walls.append(np.stack([xs, np.full_like(xs, -off)], axis=1))
+        ys = (np.arange(0, nty) + 0.5) * dx
+        walls.append(np.stack([np.full_like(ys, -off), ys], axis=1))
+        walls.append(np.stack([np.full_like(ys, tw + off), ys], axis=1))
+    xb = np.concatenate(walls, axis=0)
+    return _finish("dam_break_2d", s, xb, xf, 0.0,
+                   f"2D dam break dx={dx}: {xf.shape[0]} fluid + {xb.shape[0]} boundary")
+
+
+def _sphere_lattice(radius, dx, centre, rng, jitter):
+    n = int(math.ceil(2.0 * radius / dx))
+    n = max(n, 1)
+    g = np.arange(n, dtype=np.float64) * dx
+    k, j, i = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1)
+    pts += np.asarray(centre) - radius
+    if jitter:
+        pts += rng.uniform(-jitter * dx, jitter * dx, size=pts.shape)
+    keep = ((pts - np.asarray(centre)) ** 2).sum(axis=1) <= radius * radius
+    return pts[keep]
+
+
+def _gas(s, speed, rho_g):
+    """Constant-velocity gas of speed `speed` along +x with density rho_g:
+    p_ref is set so that gamma p / sos^2 = rho_g (the density the Gissler
+    force uses, src/Aero.h:97)."""
+    s.v_inf[0], s.v_inf[1], s.v_inf[2] = speed, 0.0, 0.0
+    s.rho_g = rho_g
+    sos2 = s.temp_g * s.R_g * s.gamma_g
+    s.p_ref = rho_g * sos2 / s.gamma_g
+
+
+def droplet_3d(diameter=4e-3, dx=5e-5, seed=1234, precision=abi.PREC_MIXED, speed=20.0, rho_g=1.2, H_fac=2.0):
+    """3D single droplet in uniform crossflow.
+
+    A sphere of the given diameter on a cubic lattice of spacing dx (clipped as
+    in src/shapes/circle.cpp:163-189), jittered by a seeded U(-0.01dx, +0.01dx)
+    per component (the reference jitters by U(0, eps dx), src/shapes/
+    square.cpp:103: a deliberate input change).  D = 4 mm, dx = 0.05 mm gives
+    about 268k particles.  rho0 = 1000, sigma = 0.0728 with pairwise surface
+    tension, Gissler aero in a 20 m/s stream of density 1.2.  Unstated
+    constants keep the reference defaults: H_fac = 2, c0 = 300, mu = 8.94e-4,
+    alpha = 0.1, delta = 0.1, gravity (0, 0, -9.81).
+    """
+    s = abi.default_settings(3)
+    s.particle_step = dx
+    s.H_fac = H_fac
+    s.rho_rest = 1000.0
+    s.sig = 0.0728
+    s.precision = precision
+    s.aero_case = abi.GISSLER
+    _gas(s, speed, rho_g)
+    rng = np.random.default_rng(seed)
+    xf = _sphere_lattice(diameter / 2.0, dx, (0.0, 0.0, 0.0), rng, 0.01)
+    return _finish("droplet_3d", s, np.zeros((0, 3)), xf, 0.0,
+                   f"3D droplet D={diameter} dx={dx}: {xf.shape[0]} particles, seed {seed}")
+
+
+def jet_column_3d(n_target=4_000_000, diameter=2e-3, dx=4e-5, seed=7, precision=abi.PREC_MIXED,
+                  jet_speed=10.0, coflow=30.0, rho_g=1.2, ale=1, H_fac=2.0):
+    """Large 3D coflow jet column: the 4M-particle single-GPU workload.
+
+    A liquid column of the given diameter on a cubic lattice of spacing dx,
+    long enough to hold about n_target particles, moving at jet_speed along
+    +x inside a constant coflow of `coflow` m/s (relative gas speed
+    coflow - jet_speed).  ALE shifting on; Gissler aero from the constant
+    coflow velocity.  Jitter U(-0.01dx, 0.01dx), seeded.  Unstated constants
+    keep the reference defaults (H_fac = 2, c0 = 300, mu = 8.94e-4,
+    alpha = 0.1, delta = 0.1); gravity is off along the jet axis only in the
+    sense that the reference default (0, 0, -9.81) is kept.
+    """
+    s = abi.default_settings(3)
+    s.particle_step = dx
+    s.H_fac = H_fac
+    s.rho_rest = 1000.0
+    s.sig = 0.0728
+    s.precision = precision
+    s.aero_case = abi.GISSLER
+    s.ale = ale
+    _gas(s, coflow, rho_g)
+    r = diameter / 2.0
+    nr = int(math.ceil(2.0 * r / dx))
+    g = (np.arange(nr) * dx) - r
+    yy, zz = np.meshgrid(g, g, indexing="ij")
+    keep = (yy ** 2 + zz ** 2) <= r * r
+    yz = np.stack([yy[keep], zz[keep]], axis=1)
+    per_layer = yz.shape[0]
+    nlayers = max(1, int(round(n_target / per_layer)))
+    xs = np.arange(nlayers, dtype=np.float64) * dx
+    X = np.repeat(xs, per_layer)
+    YZ = np.tile(yz, (nlayers, 1))
+    pts = np.concatenate([X[:, None], YZ], axis=1)
+    rng = np.random.default_rng(seed)
+    pts += rng.uniform(-0.01 * dx, 0.01 * dx, size=pts.shape)
+    vel = np.zeros_like(pts)
+    vel[:, 0] = jet_speed
+    return _finish("jet_column_3d", s, np.zeros((0, 3)), pts, vel,
+                   f"3D coflow jet column D={diameter} dx={dx}: {pts.shape[0]} particles "
+                   f"({nlayers} layers of {per_layer}), seed {seed}")
+
+
+def fluid_block(dim, n_side, dx=1e-3, jitter=0.0, seed=0, precision=abi.PREC_FP64, H_fac=1.3,
+                with_walls=False, layers=3, speed=0.0, aero=abi.NO_AERO, sig=0.0, mu=8.94e-4):
+    """Small square/cube of fluid (i fastest, src/shapes/square.cpp:112-138),
+    optionally in an open box of `layers` wall rows: the unit-test workload."""
+    s = abi.default_settings(dim)
+    s.particle_step = dx
+    s.H_fac = H_fac
+    s.precision = precision
+    s.sig = sig
+    s.mu = mu
+    s.aero_case = aero
+    if aero != abi.NO_AERO:
+        _gas(s, speed, 1.2)
+    rng = np.random.default_rng(seed)
+    g = (np.arange(n_side) + 0.5) * dx
+    mesh = np.meshgrid(*([g] * dim), indexing="ij")
+    xf = np.stack([m.ravel() for m in reversed(mesh)], axis=1)
+    if jitter:
+        xf += rng.uniform(-jitter * dx, jitter * dx, size=xf.shape)
+    xb = np.zeros((0, dim))
+    if with_walls:
+        L = n_side * dx
+        walls = []
+        span = (np.arange(-layers, n_side + layers) + 0.5) * dx
+        for k in range(layers):
+            off = (k + 0.5) * dx
+            if dim == 2:
+                walls.append(np.stack([span, np.full_like(span, -off)], axis=1))
+                ys = (np.arange(n_side + layers) + 0.5) * dx
+                walls.append(np.stack([np.full_like(ys, -off), ys], axis=1))
+                walls.append(np.stack([np.full_like(ys, L + off), ys], axis=1))
+            else:
+                a, bb = np.meshgrid(span, span, indexing="ij")
+                walls.append(np.stack([a.ravel(), bb.ravel(), np.full(a.size, -off)], axis=1))
+        xb = np.concatenate(walls, axis=0)
+    return _finish(f"block_{dim}d", s, xb, xf, 0.0, f"{dim}D block {n_side}^{dim}")
+
+
+def batched_droplets(n_drops=256, dx=5e-5, seed=99, precision=abi.PREC_MIXED, d_range=(0.5e-3, 2e-3),
+                     u_range=(5.0, 40.0), spacing_factor=1.5):
+    """Many independent 3D droplets of uneven size in one particle set.
+
+    The stated radius range [0.5, 2] mm at dx = 0.05 mm gives 4k-268k
+    particles per drop, not the stated 2k-60k; this builder treats the range
+    as DIAMETERS (log-uniform), which gives about 0.5k-33k particles per
+    drop.  Drops are laid out on a grid with gaps of spacing_factor x the
+    largest diameter so no two interact.  Each drop gets its own crossflow
+    speed U[5, 40] m/s through its initial velocity relative to a common gas
+    at rest (the relative velocity is what the Gissler force sees).
+    """
+    s = abi.default_settings(3)
+    s.particle_step = dx
+    s.rho_rest = 1000.0
+    s.sig = 0.0728
+    s.precision = precision
+    s.aero_case = abi.GISSLER
+    _gas(s, 0.0, 1.2)
+    rng = np.random.default_rng(seed)
+    logd = rng.uniform(math.log(d_range[0]), math.log(d_range[1]), size=n_drops)
+    diams = np.exp(logd)
+    speeds = rng.uniform(u_range[0], u_range[1], size=n_drops)
+    side = int(math.ceil(n_drops ** (1.0 / 3.0)))
+    pitch = spacing_factor * d_range[1] + 8 * s.H_fac * dx
+    pts, vels, owner = [], [], []
+    for k in range(n_drops):
+        c = np.array([k % side, (k // side) % side, k // (side * side)], dtype=np.float64) * pitch
+        p = _sphere_lattice(diams[k] / 2.0, dx, c, rng, 0.01)
+        pts.append(p)
+        v = np.zeros_like(p)
+        v[:, 0] = -speeds[k]
+        vels.append(v)
+        owner.append(np.full(p.shape[0], k, dtype=np.int32))
+    xf = np.concatenate(pts)
+    vf = np.concatenate(vels)
+    case = _finish("batched_droplets", s, np.zeros((0, 3)), xf, vf,
+                   f"{n_drops} droplets, diameters log-U{d_range}, {xf.shape[0]} particles")
+    case.owner = np.concatenate(owner)
+    return case

@@ -1,0 +1,1189 @@
+/*
+ * C-ABI of the WCSPH step engine (include/fjsph_engine.h): device memory,
+ * host<->device transfer, neighbour build with cell-order reordering, and
+ * the host control flow of Integrator / Newmark_Beta.
+ *
+ * Host control mirrors src/Integration.cpp:27-303 and
+ * src/Newmark_Beta.cpp:10-331 line for line in its decisions; every
+ * particle operation runs in kernels.hip.  The host reads back one scalar
+ * per Newmark sub-iteration (the residual), the bounding box and the list
+ * length per neighbour build, and the find_timestep reduction per step.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine_internal.hpp"
+
+using namespace fj;
+
+namespace {
+
+constexpr int kNumFields = FJSPH_F_COUNT;
+
+struct FieldDesc {
+    int width_kind; /* 0 scalar, 1 vector (dim), 2 matrix (dim*dim) */
+    bool is_int;
+};
+
+FieldDesc field_desc(int f)
+{
+    switch (f) {
+    case FJSPH_F_X: case FJSPH_F_V: case FJSPH_F_ACC: case FJSPH_F_AF: case FJSPH_F_AVISC:
+    case FJSPH_F_GRADRHO: case FJSPH_F_NORM: case FJSPH_F_CELLV: case FJSPH_F_VPERT:
+        return {1, false};
+    case FJSPH_F_L:
+        return {2, false};
+    case FJSPH_F_SURF: case FJSPH_F_CELLID: case FJSPH_F_SURFZONE: case FJSPH_F_B:
+        return {0, true};
+    default:
+        return {0, false};
+    }
+}
+
+void** field_slot(DevState& s, int f)
+{
+    switch (f) {
+    case FJSPH_F_X: return (void**)&s.x;
+    case FJSPH_F_V: return (void**)&s.v;
+    case FJSPH_F_ACC: return (void**)&s.acc;
+    case FJSPH_F_AF: return (void**)&s.Af;
+    case FJSPH_F_AVISC: return (void**)&s.aVisc;
+    case FJSPH_F_RHO: return (void**)&s.rho;
+    case FJSPH_F_P: return (void**)&s.p;
+    case FJSPH_F_RRHO: return (void**)&s.Rrho;
+    case FJSPH_F_M: return (void**)&s.m;
+    case FJSPH_F_DELTAD: return (void**)&s.deltaD;
+    case FJSPH_F_L: return (void**)&s.L;
+    case FJSPH_F_GRADRHO: return (void**)&s.gradRho;
+    case FJSPH_F_NORM: return (void**)&s.norm;
+    case FJSPH_F_LAM: return (void**)&s.lam;
+    case FJSPH_F_LAM_NB: return (void**)&s.lam_nb;
+    case FJSPH_F_COLOURG: return (void**)&s.colourG;
+    case FJSPH_F_COLOUR: return (void**)&s.colour;
+    case FJSPH_F_KERNSUM: return (void**)&s.kernsum;
+    case FJSPH_F_SURF: return (void**)&s.surf;
+    case FJSPH_F_CURVE: return (void**)&s.curve;
+    case FJSPH_F_NORM_CURVE: return (void**)&s.norm_curve;
+    case FJSPH_F_WOCCL: return (void**)&s.woccl;
+    case FJSPH_F_PDIST: return (void**)&s.pDist;
+    case FJSPH_F_CELLV: return (void**)&s.cellV;
+    case FJSPH_F_CELLP: return (void**)&s.cellP;
+    case FJSPH_F_CELLRHO: return (void**)&s.cellRho;
+    case FJSPH_F_CELLID: return (void**)&s.cellID;
+    case FJSPH_F_VPERT: return (void**)&s.vPert;
+    case FJSPH_F_SURFZONE: return (void**)&s.surfzone;
+    case FJSPH_F_B: return (void**)&s.b;
+    default: return nullptr;
+    }
+}
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+struct HostBlock {
+    fjsph_block b;
+    std::vector<double> times, vels;
+};
+
+struct TimerRec {
+    int pass;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct fjsph_ctx {
+    fjsph_settings s{};
+    fjsph_derived d{};
+    Consts c{};
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+
+    int64_t n = 0, nb = 0;
+    bool have_state = false, have_nbrs = false;
+
+    DevState pn{}, pnp1{};
+    double *x_nxt = nullptr, *v_nxt = nullptr, *rho_nxt = nullptr, *p_nxt = nullptr;
+    int32_t *part_id = nullptr, *blk = nullptr, *near_inlet = nullptr;
+    double* norms_tmp = nullptr;
+    double* partial = nullptr;
+    int nparts = 0;
+    double* gather_tmp = nullptr; /* n * 9 doubles */
+    int32_t* gather_tmp_i = nullptr;
+
+    /* cell grid */
+    int32_t *cell = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
+    int32_t *order = nullptr, *tile_sums = nullptr;
+    int64_t ncell_cap = 0, ncell = 0;
+    int32_t dims[3] = {1, 1, 1};
+
+    NbrList nl{};
+    DevScalars* sc = nullptr;
+    DevScalars* sc_host = nullptr; /* pinned */
+    DevBlocks bl{};
+    std::vector<HostBlock> blocks;
+    std::vector<int32_t> slot_host; /* host copy of part_id when needed */
+
+    /* integrator state (INTEG_SETT + Integrator members) */
+    double current_time = 0.0, last_frame_time = 0.0, delta_t = 2e-10, cfl = 1.0, safe_dt = 0.0;
+    unsigned n_stable = 0, n_unstable = 0, iteration = 0;
+    int rollbacks = 0;
+    double maxf = 0, maxAf = 0, maxRho_pc = 0, maxRhoi = 0, maxdrho = 0, minST = 0, maxU = 0, maxShift = 0;
+    double last_npd = 1.0;
+
+    /* timers */
+    bool timing = false;
+    std::vector<TimerRec> pending;
+    std::vector<hipEvent_t> event_pool;
+    double t_ms[FJSPH_T_COUNT] = {0};
+    int64_t t_launch[FJSPH_T_COUNT] = {0};
+
+    int64_t pairs = 0;
+    int32_t max_count = 0;
+};
+
+namespace {
+
+void set_err(fjsph_ctx* ctx, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+}
+
+#define HIPCHK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) throw HipError{e_, #call};                                    \
+    } while (0)
+
+template <typename T>
+void dalloc(T*& p, size_t count)
+{
+    if (p) { (void)hipFree(p); p = nullptr; }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+        p = nullptr;
+        throw HipError{e, "hipMalloc"};
+    }
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int width_of(const fjsph_ctx* ctx, int f)
+{
+    const FieldDesc fd = field_desc(f);
+    return fd.width_kind == 0 ? 1 : (fd.width_kind == 1 ? ctx->s.dim : ctx->s.dim * ctx->s.dim);
+}
+
+void alloc_state(fjsph_ctx* ctx, DevState& s)
+{
+    for (int f = 0; f < kNumFields; ++f) {
+        void** slot = field_slot(s, f);
+        const size_t cnt = size_t(ctx->n) * width_of(ctx, f);
+        if (field_desc(f).is_int) dalloc(*(int32_t**)slot, cnt);
+        else dalloc(*(double**)slot, cnt);
+        HIPCHK(hipMemsetAsync(*slot, 0, cnt * (field_desc(f).is_int ? 4 : 8), ctx->st));
+    }
+}
+
+void free_state(DevState& s)
+{
+    for (int f = 0; f < kNumFields; ++f) {
+        void** slot = field_slot(s, f);
+        if (*slot) (void)hipFree(*slot);
+        *slot = nullptr;
+    }
+}
+
+void free_all(fjsph_ctx* ctx)
+{
+    free_state(ctx->pn);
+    free_state(ctx->pnp1);
+    dfree(ctx->x_nxt); dfree(ctx->v_nxt); dfree(ctx->rho_nxt); dfree(ctx->p_nxt);
+    dfree(ctx->part_id); dfree(ctx->blk); dfree(ctx->near_inlet); dfree(ctx->norms_tmp);
+    dfree(ctx->partial); dfree(ctx->gather_tmp); dfree(ctx->gather_tmp_i);
+    dfree(ctx->cell); dfree(ctx->cell_count); dfree(ctx->cell_start); dfree(ctx->cell_fill);
+    dfree(ctx->order); dfree(ctx->tile_sums);
+    dfree(ctx->nl.idx); dfree(ctx->nl.rr); dfree(ctx->nl.count);
+    dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
+    ctx->ncell_cap = 0;
+    ctx->nl.cap = 0;
+}
+
+void fill_consts(fjsph_ctx* ctx)
+{
+    const fjsph_settings& s = ctx->s;
+    const fjsph_derived& d = ctx->d;
+    Consts& c = ctx->c;
+    c.H = d.H; c.H_sq = d.H_sq; c.sr = d.sr; c.Wc = d.W_correc; c.W_dx = d.W_dx; c.B = d.B; c.gam = s.gam;
+    c.rho0 = s.rho_rest; c.c0_sq = s.speed_sound * s.speed_sound; c.press_back = s.press_back; c.nu = d.nu;
+    c.visc_alpha = s.visc_alpha; c.dsph_cont = d.dsph_cont; c.rho_min = d.rho_min; c.rho_max = d.rho_max;
+    c.dx = d.dx; c.particle_step = s.particle_step; c.lam_cutoff = s.lam_cutoff; c.interp_fac = d.interp_fac;
+    c.i_n_full = d.i_n_full; c.A_plate = d.A_plate; c.A_sphere = d.A_sphere; c.L = d.L; c.sos = d.sos;
+    c.gamma_g = s.gamma_g; c.mu_g = s.mu_g; c.rho_g = s.rho_g; c.ycoef = d.ycoef; c.Cb = s.Cb;
+    c.max_shift_vel = s.max_shift_vel; c.lam_st = d.lam_st; c.sig = s.sig;
+    for (int k = 0; k < 3; ++k) { c.grav[k] = s.grav[k]; c.v_inf[k] = s.v_inf[k]; }
+    c.eos = s.eos; c.aero_case = s.aero_case; c.aero_source = s.aero_source; c.use_lam = s.use_lam;
+    c.use_TAB_def = s.use_TAB_def; c.ale = s.ale;
+    c.n_bound = ctx->nb;
+}
+
+/* ---------------------------------------------------------------- timers */
+
+hipEvent_t take_event(fjsph_ctx* ctx)
+{
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+}
+
+struct PassTimer {
+    fjsph_ctx* ctx;
+    int pass;
+    hipEvent_t a = nullptr;
+    PassTimer(fjsph_ctx* c, int p) : ctx(c), pass(p)
+    {
+        if (ctx->timing) {
+            a = take_event(ctx);
+            HIPCHK(hipEventRecord(a, ctx->st));
+        }
+    }
+    ~PassTimer() noexcept(false)
+    {
+        if (ctx->timing && a) {
+            hipEvent_t b = take_event(ctx);
+            HIPCHK(hipEventRecord(b, ctx->st));
+            ctx->pending.push_back({pass, a, b});
+        }
+    }
+};
+
+void drain_timers(fjsph_ctx* ctx)
+{
+    if (ctx->pending.empty()) return;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (auto& r : ctx->pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        ctx->t_ms[r.pass] += ms;
+        ctx->t_launch[r.pass] += 1;
+        ctx->event_pool.push_back(r.a);
+        ctx->event_pool.push_back(r.b);
+    }
+    ctx->pending.clear();
+}
+
+/* ----------------------------------------------------------- state moves */
+
+void sync(fjsph_ctx* ctx) { HIPCHK(hipStreamSynchronize(ctx->st)); }
+
+void copy_state(fjsph_ctx* ctx, DevState& dst, DevState& src)
+{
+    for (int f = 0; f < kNumFields; ++f) {
+        void* d = *field_slot(dst, f);
+        void* s = *field_slot(src, f);
+        const size_t bytes = size_t(ctx->n) * width_of(ctx, f) * (field_desc(f).is_int ? 4 : 8);
+        HIPCHK(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, ctx->st));
+    }
+}
+
+/* Permute every per-slot array by perm (new slot s <- old slot perm[s]). */
+void permute_all(fjsph_ctx* ctx, const int32_t* perm)
+{
+    const int64_t n = ctx->n;
+    for (DevState* st : {&ctx->pn, &ctx->pnp1}) {
+        for (int f = 0; f < kNumFields; ++f) {
+            void** slot = field_slot(*st, f);
+            if (field_desc(f).is_int) {
+                launch_gather_i((int32_t*)*slot, ctx->gather_tmp_i, perm, n, ctx->st);
+                std::swap(*(int32_t**)slot, ctx->gather_tmp_i);
+            } else {
+                const int w = width_of(ctx, f);
+                launch_gather_d((double*)*slot, ctx->gather_tmp, perm, n, w, ctx->st);
+                /* gather_tmp is sized for the widest field; swap buffers only
+                 * when the widths match, otherwise copy back. */
+                HIPCHK(hipMemcpyAsync(*slot, ctx->gather_tmp, size_t(n) * w * 8, hipMemcpyDeviceToDevice, ctx->st));
+            }
+        }
+    }
+    for (int32_t** arr : {&ctx->part_id, &ctx->blk, &ctx->cell}) {
+        launch_gather_i(*arr, ctx->gather_tmp_i, perm, n, ctx->st);
+        std::swap(*arr, ctx->gather_tmp_i);
+    }
+}
+
+/* update_neighbours (src/Neighbours.cpp:7-31) on pnp1.  When `reorder` is
+ * set the particles are first moved into cell order (a permutation only;
+ * the neighbour sets and every result are unchanged). */
+void build_neighbours(fjsph_ctx* ctx, bool reorder)
+{
+    PassTimer t(ctx, FJSPH_T_NEIGHBOUR);
+    const int D = ctx->s.dim;
+    const int64_t n = ctx->n;
+    launch_bbox(D, ctx->pnp1.x, n, ctx->partial, ctx->nparts, ctx->sc->bbox, ctx->st);
+    HIPCHK(hipMemcpyAsync(ctx->sc_host->bbox, ctx->sc->bbox, sizeof(double) * 6, hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) { lo[k] = ctx->sc_host->bbox[k]; hi[k] = ctx->sc_host->bbox[D + k]; }
+    for (int k = 0; k < D; ++k)
+        if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]))
+            throw std::string("non-finite particle position");
+    const double cs = std::sqrt(ctx->d.sr) * (1.0 + 1e-7);
+    double ncell_d = 1.0;
+    for (int k = 0; k < 3; ++k) {
+        ctx->dims[k] = 1;
+        if (k < D) {
+            const double span = (hi[k] - lo[k]) / cs;
+            ctx->dims[k] = int32_t(std::floor(span)) + 1;
+        }
+        ncell_d *= ctx->dims[k];
+    }
+    const double cell_limit = std::max(4.0 * double(n), 1.0e6) + 64.0;
+    if (ncell_d > cell_limit || ncell_d > 2.0e9)
+        throw std::string("particle cloud too sparse for the dense cell grid (") + std::to_string(ncell_d) + " cells)";
+    const int64_t ncell = int64_t(ncell_d);
+    if (ncell > ctx->ncell_cap) {
+        const int64_t cap = ncell + ncell / 4 + 64;
+        dalloc(ctx->cell_count, cap);
+        dalloc(ctx->cell_fill, cap);
+        dalloc(ctx->cell_start, cap + 1);
+        dalloc(ctx->tile_sums, cap / 2048 + 2);
+        ctx->ncell_cap = cap;
+    }
+    ctx->ncell = ncell;
+    const double inv_cs = 1.0 / cs;
+    HIPCHK(hipMemsetAsync(ctx->cell_count, 0, sizeof(int32_t) * ncell, ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->cell_fill, 0, sizeof(int32_t) * ncell, ctx->st));
+    launch_cells(D, ctx->pnp1.x, n, lo, inv_cs, ctx->dims, ctx->cell, ctx->cell_count, ctx->st);
+    launch_scan(ctx->cell_count, ncell, ctx->cell_start, ctx->tile_sums, ctx->st);
+    launch_scatter(ctx->cell, n, ctx->cell_start, ctx->cell_fill, ctx->order, ctx->st);
+    launch_sort_cells(ctx->cell_start, ncell, ctx->order, ctx->st);
+    if (reorder) {
+        permute_all(ctx, ctx->order);
+        launch_iota(ctx->order, n, ctx->st);
+    }
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        HIPCHK(hipMemsetAsync(&ctx->sc->max_count, 0, sizeof(int32_t), ctx->st));
+        launch_build_lists(D, ctx->pnp1.x, n, ctx->cell, ctx->dims, ctx->cell_start, ctx->order, ctx->d.sr, ctx->nl,
+                           ctx->sc, ctx->st);
+        HIPCHK(hipMemcpyAsync(&ctx->sc_host->max_count, &ctx->sc->max_count, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->st));
+        sync(ctx);
+        const int32_t mc = ctx->sc_host->max_count;
+        ctx->max_count = mc;
+        if (mc <= ctx->nl.cap) break;
+        const int32_t cap = ((mc + mc / 8 + 8) + 7) / 8 * 8;
+        dalloc(ctx->nl.idx, size_t(cap) * ctx->nl.npad);
+        dalloc(ctx->nl.rr, size_t(cap) * ctx->nl.npad);
+        ctx->nl.cap = cap;
+    }
+    ctx->have_nbrs = true;
+}
+
+/* ------------------------------------------------------------ the passes */
+
+void require(fjsph_ctx* ctx, bool nbrs)
+{
+    if (!ctx->have_state) throw std::string("no particle state uploaded");
+    if (nbrs && !ctx->have_nbrs) throw std::string("neighbour lists not built");
+}
+
+double dsph(fjsph_ctx* ctx, bool with_aero)
+{
+    PassTimer t(ctx, FJSPH_T_DSPH);
+    launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, with_aero ? 1 : 0, ctx->partial,
+                ctx->nparts, ctx->sc, ctx->st);
+    return 0.0;
+}
+
+void surface_and_dissipation(fjsph_ctx* ctx, bool with_diss)
+{
+    {
+        PassTimer t(ctx, FJSPH_T_SURFACE);
+        launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n,
+                        ctx->norms_tmp, with_diss ? 1 : 0, ctx->st);
+        launch_surface2(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->norms_tmp, ctx->st);
+    }
+}
+
+void shift(fjsph_ctx* ctx)
+{
+    PassTimer t(ctx, FJSPH_T_SHIFT);
+    launch_shift(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->st);
+}
+
+/* dSPH_PreStep, get_aero_velocity, Detect_Surface, dissipation_terms,
+ * particle_shift (src/Integration.cpp:45-61 and 80-96). */
+void prepasses(fjsph_ctx* ctx)
+{
+    dsph(ctx, true);
+    surface_and_dissipation(ctx, true);
+    if (ctx->s.ale) shift(ctx);
+}
+
+void upload_block_velocities(fjsph_ctx* ctx, bool rk)
+{
+    bool moving = false;
+    for (auto& hb : ctx->blocks) moving |= hb.b.n_times != 0;
+    if (!moving) return;
+    std::vector<double> vel(ctx->blocks.size() * 3, 0.0);
+    for (size_t k = 0; k < ctx->blocks.size(); ++k) {
+        const HostBlock& hb = ctx->blocks[k];
+        if (hb.b.n_times == 0) {
+            for (int a = 0; a < 3; ++a) vel[k * 3 + a] = hb.b.vel[a];
+            continue;
+        }
+        for (int t = 0; t < hb.b.n_times; ++t) {
+            const bool cond = rk ? (hb.times[t] > ctx->current_time) : (ctx->current_time > hb.times[t]);
+            if (cond) for (int a = 0; a < 3; ++a) vel[k * 3 + a] = hb.vels[t * 3 + a];
+        }
+    }
+    HIPCHK(hipMemcpyAsync(ctx->bl.vel, vel.data(), sizeof(double) * vel.size(), hipMemcpyHostToDevice, ctx->st));
+    sync(ctx);
+}
+
+void boundary_update(fjsph_ctx* ctx)
+{
+    if (ctx->nb == 0) return;
+    PassTimer t(ctx, FJSPH_T_BOUNDARY);
+    upload_block_velocities(ctx, false);
+    for (size_t k = 0; k < ctx->blocks.size(); ++k) {
+        if (ctx->blocks[k].b.is_fluid) continue;
+        launch_boundary(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->blk, ctx->bl, int32_t(k), ctx->nl, ctx->n,
+                        ctx->near_inlet, ctx->st);
+    }
+}
+
+/* Do_NB_Iter + the residual of Check_Error; returns log10 rms. */
+double nb_iter(fjsph_ctx* ctx)
+{
+    boundary_update(ctx);
+    {
+        PassTimer t(ctx, FJSPH_T_FORCE_UPDATE);
+        launch_force(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->pn, nullptr, ctx->x_nxt, ctx->v_nxt,
+                     ctx->rho_nxt, ctx->p_nxt, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->nl, ctx->n,
+                     ctx->delta_t, ctx->sc, ctx->partial, ctx->nparts, ctx->st);
+    }
+    std::swap(ctx->pnp1.x, ctx->x_nxt);
+    std::swap(ctx->pnp1.v, ctx->v_nxt);
+    std::swap(ctx->pnp1.rho, ctx->rho_nxt);
+    std::swap(ctx->pnp1.p, ctx->p_nxt);
+    {
+        PassTimer t(ctx, FJSPH_T_REDUCE);
+        launch_sum_partials(ctx->partial, ctx->nparts, 1, &ctx->sc->residual, ctx->st);
+        HIPCHK(hipMemcpyAsync(&ctx->sc_host->residual, &ctx->sc->residual, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->st));
+    }
+    sync(ctx);
+    const double nf = double(ctx->n - ctx->nb);
+    return std::log10(std::sqrt(ctx->sc_host->residual / nf));
+}
+
+/* Integrator::find_timestep (src/Integration.cpp:370-443). */
+double find_timestep(fjsph_ctx* ctx)
+{
+    {
+        PassTimer t(ctx, FJSPH_T_REDUCE);
+        launch_timestep_reduce(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->n, ctx->partial, ctx->nparts,
+                               ctx->sc->red, ctx->st);
+        HIPCHK(hipMemcpyAsync(ctx->sc_host->red, ctx->sc->red, sizeof(double) * 7, hipMemcpyDeviceToHost, ctx->st));
+    }
+    sync(ctx);
+    const double* r = ctx->sc_host->red;
+    const fjsph_settings& s = ctx->s;
+    const fjsph_derived& d = ctx->d;
+    ctx->maxf = r[0];
+    ctx->maxAf = r[1];
+    ctx->maxdrho = r[2];
+    ctx->maxRhoi = r[3];
+    ctx->maxU = r[4];
+    ctx->maxShift = r[5];
+    ctx->minST = r[6];
+    ctx->maxRho_pc = 100 * ctx->maxRhoi / s.rho_rest;
+    double mn = 0.25 * std::sqrt(d.H / ctx->maxf);
+    mn = std::min(mn, 2 * d.H / ctx->maxU);
+    if (s.mu != 0.0) mn = std::min(mn, 0.125 * d.H_sq * s.rho_rest / s.mu);
+    mn = std::min(mn, 0.067 * ctx->minST);
+    mn = std::min(mn, 0.5 * std::sqrt(d.H / ctx->maxdrho));
+    mn = std::min(mn, 1.5 * d.H / s.speed_sound);
+    ctx->safe_dt = 0.75 * mn;
+    double dt = ctx->cfl * ctx->safe_dt;
+    if (dt < s.delta_t_min) dt = s.delta_t_min;
+    else if (dt > s.delta_t_max) dt = s.delta_t_max;
+    if (dt > ctx->last_frame_time + s.frame_time_interval - ctx->current_time)
+        dt = ctx->last_frame_time + s.frame_time_interval - ctx->current_time + s.delta_t_min;
+    return dt;
+}
+
+void rollback(fjsph_ctx* ctx)
+{
+    copy_state(ctx, ctx->pnp1, ctx->pn);
+    build_neighbours(ctx, false);
+    ctx->delta_t = 0.5 * ctx->delta_t;
+    ctx->rollbacks++;
+}
+
+/* Newmark_Beta::Newmark_Beta + Check_Error (src/Newmark_Beta.cpp:10-52, 303-331). */
+double newmark_loop(fjsph_ctx* ctx, double& logbase)
+{
+    double rms = 0.0;
+    while (rms > ctx->s.min_residual) {
+        const double le = nb_iter(ctx);
+        if (ctx->iteration == 0) logbase = le;
+        rms = le - logbase;
+        if (ctx->iteration > unsigned(ctx->s.max_subits)) {
+            if (rms > 0.0) {
+                rollback(ctx);
+                ctx->iteration = 0;
+                rms = 0.0;
+                continue;
+            }
+            break;
+        }
+        ctx->iteration++;
+    }
+    return rms;
+}
+
+/* integrate_no_update (src/Integration.cpp:27-107). */
+double integrate_no_update(fjsph_ctx* ctx)
+{
+    if (ctx->s.integrator != FJSPH_NEWMARK_BETA)
+        throw std::string("Runge-Kutta integration is not built into this engine yet");
+    ctx->iteration = 0;
+    ctx->rollbacks = 0;
+    double logbase = 0.0;
+    ctx->delta_t = find_timestep(ctx);
+    build_neighbours(ctx, true);
+    prepasses(ctx);
+    const double le0 = nb_iter(ctx);
+    logbase = le0; /* Check_Error at iteration 0 */
+    ctx->iteration++;
+    build_neighbours(ctx, false);
+    prepasses(ctx);
+    const double rms = newmark_loop(ctx, logbase);
+    HIPCHK(hipMemcpyAsync(&ctx->sc_host->npd, &ctx->sc->npd, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    ctx->last_npd = ctx->sc_host->npd;
+    drain_timers(ctx);
+    return rms;
+}
+
+/* update_data (src/Integration.cpp:109-226) without inlets or deletion. */
+void update_data(fjsph_ctx* ctx)
+{
+    launch_fix_rho(ctx->pnp1, ctx->part_id, ctx->nb, ctx->n, ctx->s.rho_rest, ctx->st);
+    copy_state(ctx, ctx->pn, ctx->pnp1);
+}
+
+/* integrate (src/Integration.cpp:233-303). */
+double integrate(fjsph_ctx* ctx)
+{
+    const double step_error = integrate_no_update(ctx);
+    update_data(ctx);
+    ctx->current_time += ctx->delta_t;
+    const fjsph_settings& s = ctx->s;
+    if (step_error > s.min_residual || ctx->maxRho_pc > s.rho_max_iter) {
+        if (step_error > 0.6 * s.min_residual) {
+            ctx->cfl = std::max(s.cfl_min, ctx->cfl - s.cfl_step);
+            ctx->n_unstable = 0;
+        } else if (ctx->n_unstable > unsigned(s.n_unstable_limit)) {
+            ctx->cfl = std::max(s.cfl_min, ctx->cfl - s.cfl_step);
+            ctx->n_unstable = 0;
+        } else
+            ctx->n_unstable++;
+    } else
+        ctx->n_unstable = 0;
+    if (ctx->iteration < s.subits_factor * s.max_subits && ctx->n_unstable == 0) {
+        if (ctx->n_stable > unsigned(s.n_stable_limit)) {
+            ctx->cfl = std::min(s.cfl_max, ctx->cfl + s.cfl_step);
+            ctx->n_stable = 0;
+        } else
+            ctx->n_stable++;
+    } else
+        ctx->n_stable = 0;
+    sync(ctx);
+    return step_error;
+}
+
+void fill_info(fjsph_ctx* ctx, double rms, fjsph_step_info* info)
+{
+    if (!info) return;
+    info->rms_error = rms;
+    info->dt = ctx->delta_t;
+    info->cfl = ctx->cfl;
+    info->time = ctx->current_time;
+    info->max_rho_pc = ctx->maxRho_pc;
+    info->maxf = ctx->maxf;
+    info->maxAf = ctx->maxAf;
+    info->npd = ctx->last_npd;
+    info->iterations = int32_t(ctx->iteration);
+    info->rollbacks = ctx->rollbacks;
+}
+
+template <typename F>
+fjsph_status guarded(fjsph_ctx* ctx, F&& f)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    try {
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) throw HipError{e, "hipSetDevice"};
+        f();
+        return FJSPH_OK;
+    } catch (const HipError& he) {
+        set_err(ctx, "%s failed: %s", he.what, hipGetErrorString(he.e));
+        return he.e == hipErrorOutOfMemory ? FJSPH_ERR_OOM : FJSPH_ERR_HIP;
+    } catch (const std::string& msg) {
+        ctx->err = msg;
+        return FJSPH_ERR_STATE;
+    } catch (const std::exception& ex) {
+        ctx->err = ex.what();
+        return FJSPH_ERR_STATE;
+    }
+}
+
+std::vector<int32_t>& slot_to_host(fjsph_ctx* ctx)
+{
+    ctx->slot_host.resize(size_t(ctx->n));
+    HIPCHK(hipMemcpyAsync(ctx->slot_host.data(), ctx->part_id, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost,
+                          ctx->st));
+    sync(ctx);
+    return ctx->slot_host;
+}
+
+}  // namespace
+
+/* =========================================================== C entry points */
+
+extern "C" {
+
+int32_t fjsph_abi_version(void) { return FJSPH_ABI_VERSION; }
+
+fjsph_status fjsph_create(const fjsph_settings* s, int32_t device, fjsph_ctx** out)
+{
+    if (!s || !out) return FJSPH_ERR_ARG;
+    *out = nullptr;
+    fjsph_derived d;
+    if (fjsph_derive(s, &d) != FJSPH_OK) return FJSPH_ERR_ARG;
+    if (s->precision != FJSPH_PREC_FP64 && s->precision != FJSPH_PREC_MIXED) return FJSPH_ERR_ARG;
+    if (s->aero_case != FJSPH_NO_AERO && s->aero_case != FJSPH_GISSLER) return FJSPH_ERR_UNSUPPORTED;
+    if (s->aero_source != FJSPH_CONST_VEL) return FJSPH_ERR_UNSUPPORTED;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FJSPH_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return FJSPH_ERR_ARG;
+    fjsph_ctx* ctx = new fjsph_ctx();
+    ctx->s = *s;
+    ctx->d = d;
+    ctx->device = device;
+    ctx->cfl = s->cfl;
+    fjsph_status rc = guarded(ctx, [&] {
+        HIPCHK(hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking));
+        HIPCHK(hipMalloc((void**)&ctx->sc, sizeof(DevScalars)));
+        HIPCHK(hipMemsetAsync(ctx->sc, 0, sizeof(DevScalars), ctx->st));
+        HIPCHK(hipHostMalloc((void**)&ctx->sc_host, sizeof(DevScalars), hipHostMallocDefault));
+        fill_consts(ctx);
+    });
+    if (rc != FJSPH_OK) {
+        fjsph_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return FJSPH_OK;
+}
+
+void fjsph_destroy(fjsph_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    free_all(ctx);
+    for (auto& r : ctx->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->sc) (void)hipFree(ctx->sc);
+    if (ctx->sc_host) (void)hipHostFree(ctx->sc_host);
+    if (ctx->st) (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+}
+
+const char* fjsph_last_error(const fjsph_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound, const double* x, const double* v,
+                                const double* rho, const double* p, const double* m, const int32_t* b)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    if (n_total <= 0 || n_bound < 0 || n_bound >= n_total || n_total >= (int64_t(1) << 31) || !x || !v || !rho ||
+        !p || !m || !b) {
+        ctx->err = "upload_state: bad sizes or null arrays (need 0 <= n_bound < n_total < 2^31)";
+        return FJSPH_ERR_ARG;
+    }
+    return guarded(ctx, [&] {
+        free_all(ctx);
+        ctx->n = n_total;
+        ctx->nb = n_bound;
+        const int D = ctx->s.dim;
+        alloc_state(ctx, ctx->pn);
+        alloc_state(ctx, ctx->pnp1);
+        dalloc(ctx->x_nxt, size_t(n_total) * D);
+        dalloc(ctx->v_nxt, size_t(n_total) * D);
+        dalloc(ctx->rho_nxt, size_t(n_total));
+        dalloc(ctx->p_nxt, size_t(n_total));
+        dalloc(ctx->part_id, size_t(n_total));
+        dalloc(ctx->blk, size_t(n_total));
+        dalloc(ctx->near_inlet, size_t(n_total));
+        dalloc(ctx->norms_tmp, size_t(n_total) * D);
+        ctx->nparts = reduction_parts(n_total);
+        dalloc(ctx->partial, size_t(ctx->nparts) * 8);
+        dalloc(ctx->gather_tmp, size_t(n_total) * D * D);
+        dalloc(ctx->gather_tmp_i, size_t(n_total));
+        dalloc(ctx->cell, size_t(n_total));
+        dalloc(ctx->order, size_t(n_total));
+        ctx->nl.npad = (n_total + 63) / 64 * 64;
+        dalloc(ctx->nl.count, size_t(ctx->nl.npad));
+        ctx->nl.cap = 0;
+        HIPCHK(hipMemsetAsync(ctx->near_inlet, 0, sizeof(int32_t) * n_total, ctx->st));
+        sync(ctx); /* the blocking copies below run on the null stream */
+        /* host-side staging of the initial state (src/Init.cpp:288-493,
+         * src/FJSPH.cpp:115-126) */
+        std::vector<int32_t> ids(n_total), blk(n_total, -1);
+        for (int64_t i = 0; i < n_total; ++i) ids[i] = int32_t(i);
+        for (size_t k = 0; k < ctx->blocks.size(); ++k)
+            for (int64_t i = std::max<int64_t>(0, ctx->blocks[k].b.first);
+                 i < std::min<int64_t>(n_total, ctx->blocks[k].b.last); ++i)
+                blk[i] = int32_t(k);
+        std::vector<double> cellV(size_t(n_total) * D), cellP(n_total, ctx->s.p_ref), cellRho(n_total, ctx->s.rho_g);
+        for (int64_t i = 0; i < n_total; ++i)
+            for (int k = 0; k < D; ++k) cellV[i * D + k] = ctx->s.v_inf[k];
+        std::vector<int32_t> cellID(n_total, -3);
+        for (DevState* st : {&ctx->pn, &ctx->pnp1}) {
+            HIPCHK(hipMemcpy(st->x, x, sizeof(double) * n_total * D, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(st->v, v, sizeof(double) * n_total * D, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(st->rho, rho, sizeof(double) * n_total, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(st->p, p, sizeof(double) * n_total, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(st->m, m, sizeof(double) * n_total, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(st->b, b, sizeof(int32_t) * n_total, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(st->cellID, cellID.data(), sizeof(int32_t) * n_total, hipMemcpyHostToDevice));
+            if (ctx->s.aero_source != FJSPH_MESH_INFL) {
+                HIPCHK(hipMemcpy(st->cellV, cellV.data(), sizeof(double) * n_total * D, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(st->cellP, cellP.data(), sizeof(double) * n_total, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(st->cellRho, cellRho.data(), sizeof(double) * n_total, hipMemcpyHostToDevice));
+            }
+        }
+        HIPCHK(hipMemcpy(ctx->part_id, ids.data(), sizeof(int32_t) * n_total, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->blk, blk.data(), sizeof(int32_t) * n_total, hipMemcpyHostToDevice));
+        ctx->c.n_bound = n_bound;
+        ctx->have_state = true;
+        ctx->have_nbrs = false;
+        ctx->current_time = 0.0;
+        ctx->last_frame_time = 0.0;
+        ctx->cfl = ctx->s.cfl;
+        ctx->delta_t = ctx->s.delta_t_min > 0 ? ctx->s.delta_t_min : 2e-10;
+        ctx->n_stable = ctx->n_unstable = 0;
+        /* blocks may have been set before the state: push them */
+        if (!ctx->blocks.empty()) {
+            const int nbk = int(ctx->blocks.size());
+            std::vector<int32_t> solver(nbk), ns(nbk);
+            std::vector<double> vel(size_t(nbk) * 3);
+            for (int k = 0; k < nbk; ++k) {
+                solver[k] = ctx->blocks[k].b.bound_solver;
+                ns[k] = ctx->blocks[k].b.no_slip;
+                for (int a = 0; a < 3; ++a) vel[k * 3 + a] = ctx->blocks[k].b.vel[a];
+            }
+            dalloc(ctx->bl.solver, nbk);
+            dalloc(ctx->bl.no_slip, nbk);
+            dalloc(ctx->bl.vel, size_t(nbk) * 3);
+            HIPCHK(hipMemcpy(ctx->bl.solver, solver.data(), sizeof(int32_t) * nbk, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(ctx->bl.no_slip, ns.data(), sizeof(int32_t) * nbk, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(ctx->bl.vel, vel.data(), sizeof(double) * nbk * 3, hipMemcpyHostToDevice));
+            ctx->bl.n = nbk;
+        }
+        sync(ctx);
+    });
+}
+
+fjsph_status fjsph_set_blocks(fjsph_ctx* ctx, int32_t n_blocks, const fjsph_block* blocks)
+{
+    if (!ctx || n_blocks < 0 || (n_blocks > 0 && !blocks)) return FJSPH_ERR_ARG;
+    for (int k = 0; k < n_blocks; ++k) {
+        const fjsph_block& b = blocks[k];
+        if (b.first < 0 || b.last < b.first || (b.bound_solver < 0 || b.bound_solver > 2)) {
+            ctx->err = "set_blocks: bad block range or solver";
+            return FJSPH_ERR_ARG;
+        }
+    }
+    ctx->blocks.clear();
+    for (int k = 0; k < n_blocks; ++k) {
+        HostBlock hb;
+        hb.b = blocks[k];
+        hb.b.n_times = 0;
+        ctx->blocks.push_back(hb);
+    }
+    if (!ctx->have_state) return FJSPH_OK;
+    /* re-derive per-slot block ids in the current slot order */
+    return guarded(ctx, [&] {
+        std::vector<int32_t>& sh = slot_to_host(ctx);
+        std::vector<int32_t> blk(size_t(ctx->n), -1);
+        for (int64_t s = 0; s < ctx->n; ++s) {
+            const int64_t h = sh[s];
+            for (int k = 0; k < n_blocks; ++k)
+                if (h >= blocks[k].first && h < blocks[k].last) { blk[s] = k; break; }
+        }
+        HIPCHK(hipMemcpy(ctx->blk, blk.data(), sizeof(int32_t) * ctx->n, hipMemcpyHostToDevice));
+        const int nbk = n_blocks;
+        std::vector<int32_t> solver(std::max(nbk, 1)), ns(std::max(nbk, 1));
+        std::vector<double> vel(size_t(std::max(nbk, 1)) * 3, 0.0);
+        for (int k = 0; k < nbk; ++k) {
+            solver[k] = blocks[k].bound_solver;
+            ns[k] = blocks[k].no_slip;
+            for (int a = 0; a < 3; ++a) vel[k * 3 + a] = blocks[k].vel[a];
+        }
+        dalloc(ctx->bl.solver, solver.size());
+        dalloc(ctx->bl.no_slip, ns.size());
+        dalloc(ctx->bl.vel, vel.size());
+        HIPCHK(hipMemcpy(ctx->bl.solver, solver.data(), sizeof(int32_t) * solver.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->bl.no_slip, ns.data(), sizeof(int32_t) * ns.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->bl.vel, vel.data(), sizeof(double) * vel.size(), hipMemcpyHostToDevice));
+        ctx->bl.n = nbk;
+    });
+}
+
+fjsph_status fjsph_set_block_motion(fjsph_ctx* ctx, int32_t block, int32_t n_times, const double* times,
+                                    const double* vels)
+{
+    if (!ctx || block < 0 || block >= int32_t(ctx->blocks.size()) || n_times < 0 || (n_times > 0 && (!times || !vels)))
+        return FJSPH_ERR_ARG;
+    HostBlock& hb = ctx->blocks[block];
+    hb.b.n_times = n_times;
+    hb.times.assign(times, times + n_times);
+    hb.vels.assign(vels, vels + size_t(n_times) * 3);
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_download_field(fjsph_ctx* ctx, int32_t which, int32_t field, double* out)
+{
+    if (!ctx || !out || field < 0 || field >= FJSPH_F_COUNT || (which != 0 && which != 1)) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        require(ctx, false);
+        DevState& st = which == FJSPH_STATE_N ? ctx->pn : ctx->pnp1;
+        const int w = width_of(ctx, field);
+        const size_t cnt = size_t(ctx->n) * w;
+        std::vector<int32_t>& sh = slot_to_host(ctx);
+        if (field_desc(field).is_int) {
+            std::vector<int32_t> buf(cnt);
+            HIPCHK(hipMemcpy(buf.data(), *field_slot(st, field), cnt * 4, hipMemcpyDeviceToHost));
+            for (int64_t s = 0; s < ctx->n; ++s) out[sh[s]] = double(buf[s]);
+        } else {
+            std::vector<double> buf(cnt);
+            HIPCHK(hipMemcpy(buf.data(), *field_slot(st, field), cnt * 8, hipMemcpyDeviceToHost));
+            for (int64_t s = 0; s < ctx->n; ++s)
+                for (int k = 0; k < w; ++k) out[size_t(sh[s]) * w + k] = buf[size_t(s) * w + k];
+        }
+    });
+}
+
+fjsph_status fjsph_upload_field(fjsph_ctx* ctx, int32_t which, int32_t field, const double* in)
+{
+    if (!ctx || !in || field < 0 || field >= FJSPH_F_COUNT || (which != 0 && which != 1)) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        require(ctx, false);
+        DevState& st = which == FJSPH_STATE_N ? ctx->pn : ctx->pnp1;
+        const int w = width_of(ctx, field);
+        const size_t cnt = size_t(ctx->n) * w;
+        std::vector<int32_t>& sh = slot_to_host(ctx);
+        if (field_desc(field).is_int) {
+            std::vector<int32_t> buf(cnt);
+            for (int64_t s = 0; s < ctx->n; ++s) buf[s] = int32_t(in[sh[s]]);
+            HIPCHK(hipMemcpy(*field_slot(st, field), buf.data(), cnt * 4, hipMemcpyHostToDevice));
+        } else {
+            std::vector<double> buf(cnt);
+            for (int64_t s = 0; s < ctx->n; ++s)
+                for (int k = 0; k < w; ++k) buf[size_t(s) * w + k] = in[size_t(sh[s]) * w + k];
+            HIPCHK(hipMemcpy(*field_slot(st, field), buf.data(), cnt * 8, hipMemcpyHostToDevice));
+        }
+    });
+}
+
+fjsph_status fjsph_build_neighbours(fjsph_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        require(ctx, false);
+        build_neighbours(ctx, true);
+        drain_timers(ctx);
+    });
+}
+
+fjsph_status fjsph_get_neighbour_counts(fjsph_ctx* ctx, int32_t* counts)
+{
+    if (!counts) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        sync(ctx);
+        std::vector<int32_t> c(static_cast<size_t>(ctx->n));
+        HIPCHK(hipMemcpy(c.data(), ctx->nl.count, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
+        std::vector<int32_t>& sh = slot_to_host(ctx);
+        for (int64_t s = 0; s < ctx->n; ++s) counts[sh[s]] = c[s] + 1;
+    });
+}
+
+fjsph_status fjsph_get_neighbours(fjsph_ctx* ctx, int64_t* offsets, int64_t* idx, int64_t idx_capacity)
+{
+    if (!offsets || !idx) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        sync(ctx);
+        const int64_t n = ctx->n;
+        std::vector<int32_t> cnt(static_cast<size_t>(n));
+        HIPCHK(hipMemcpy(cnt.data(), ctx->nl.count, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        std::vector<int32_t> ids(size_t(ctx->nl.cap) * ctx->nl.npad);
+        if (!ids.empty())
+            HIPCHK(hipMemcpy(ids.data(), ctx->nl.idx, sizeof(int32_t) * ids.size(), hipMemcpyDeviceToHost));
+        std::vector<int32_t>& sh = slot_to_host(ctx);
+        std::vector<int64_t> hcount(static_cast<size_t>(n));
+        for (int64_t s = 0; s < n; ++s) hcount[sh[s]] = cnt[s] + 1;
+        offsets[0] = 0;
+        for (int64_t h = 0; h < n; ++h) offsets[h + 1] = offsets[h] + hcount[h];
+        if (offsets[n] > idx_capacity) throw std::string("get_neighbours: idx_capacity too small");
+        for (int64_t s = 0; s < n; ++s) {
+            const int64_t h = sh[s];
+            int64_t* o = idx + offsets[h];
+            o[0] = h;
+            for (int k = 0; k < cnt[s]; ++k) o[k + 1] = sh[ids[size_t(k) * ctx->nl.npad + s]];
+            std::sort(o, o + cnt[s] + 1);
+        }
+    });
+}
+
+fjsph_status fjsph_dsph_prestep(fjsph_ctx* ctx, double* npd)
+{
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        dsph(ctx, false);
+        HIPCHK(hipMemcpyAsync(&ctx->sc_host->npd, &ctx->sc->npd, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+        sync(ctx);
+        if (npd) *npd = ctx->sc_host->npd;
+        drain_timers(ctx);
+    });
+}
+
+fjsph_status fjsph_aero_velocity(fjsph_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        launch_aero_velocity(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->st);
+        sync(ctx);
+    });
+}
+
+fjsph_status fjsph_detect_surface(fjsph_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        surface_and_dissipation(ctx, false);
+        sync(ctx);
+        drain_timers(ctx);
+    });
+}
+
+fjsph_status fjsph_dissipation_terms(fjsph_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        PassTimer t(ctx, FJSPH_T_DISSIPATION);
+        launch_dissipation(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->st);
+        sync(ctx);
+    });
+}
+
+fjsph_status fjsph_particle_shift(fjsph_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        if (!ctx->s.ale) throw std::string("particle_shift needs settings.ale = 1");
+        shift(ctx);
+        sync(ctx);
+        drain_timers(ctx);
+    });
+}
+
+fjsph_status fjsph_boundary_update(fjsph_ctx* ctx)
+{
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        boundary_update(ctx);
+        sync(ctx);
+        drain_timers(ctx);
+    });
+}
+
+fjsph_status fjsph_acc_rrho(fjsph_ctx* ctx, double npd)
+{
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        HIPCHK(hipMemcpyAsync(&ctx->sc->npd, &npd, sizeof(double), hipMemcpyHostToDevice, ctx->st));
+        launch_force(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->pn, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->nl, ctx->n, ctx->delta_t, ctx->sc,
+                     ctx->partial, ctx->nparts, ctx->st);
+        sync(ctx);
+    });
+}
+
+fjsph_status fjsph_nb_iter(fjsph_ctx* ctx, double npd, double* log_error)
+{
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        HIPCHK(hipMemcpyAsync(&ctx->sc->npd, &npd, sizeof(double), hipMemcpyHostToDevice, ctx->st));
+        const double le = nb_iter(ctx);
+        if (log_error) *log_error = le;
+        drain_timers(ctx);
+    });
+}
+
+fjsph_status fjsph_find_timestep(fjsph_ctx* ctx, double* dt)
+{
+    return guarded(ctx, [&] {
+        require(ctx, false);
+        const double v = find_timestep(ctx);
+        if (dt) *dt = v;
+        drain_timers(ctx);
+    });
+}
+
+fjsph_status fjsph_integrate_no_update(fjsph_ctx* ctx, fjsph_step_info* info)
+{
+    return guarded(ctx, [&] {
+        require(ctx, false);
+        const double rms = integrate_no_update(ctx);
+        fill_info(ctx, rms, info);
+    });
+}
+
+fjsph_status fjsph_integrate_step(fjsph_ctx* ctx, fjsph_step_info* info)
+{
+    return guarded(ctx, [&] {
+        require(ctx, false);
+        const double rms = integrate(ctx);
+        fill_info(ctx, rms, info);
+        drain_timers(ctx);
+    });
+}
+
+fjsph_status fjsph_advance_frame(fjsph_ctx* ctx)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    ctx->last_frame_time += ctx->s.frame_time_interval;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_get_time(fjsph_ctx* ctx, double* time, double* delta_t, double* cfl)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    if (time) *time = ctx->current_time;
+    if (delta_t) *delta_t = ctx->delta_t;
+    if (cfl) *cfl = ctx->cfl;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_set_time(fjsph_ctx* ctx, double time, double delta_t, double cfl)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    ctx->current_time = time;
+    ctx->delta_t = delta_t;
+    ctx->cfl = cfl;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_enable_timers(fjsph_ctx* ctx, int32_t on)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        drain_timers(ctx);
+        ctx->timing = on != 0;
+        if (ctx->timing)
+            for (int k = 0; k < FJSPH_T_COUNT; ++k) { ctx->t_ms[k] = 0.0; ctx->t_launch[k] = 0; }
+    });
+}
+
+fjsph_status fjsph_get_timers(fjsph_ctx* ctx, double* ms, int64_t* launches)
+{
+    if (!ctx || !ms || !launches) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        drain_timers(ctx);
+        for (int k = 0; k < FJSPH_T_COUNT; ++k) { ms[k] = ctx->t_ms[k]; launches[k] = ctx->t_launch[k]; }
+    });
+}
+
+fjsph_status fjsph_pass_bytes(fjsph_ctx* ctx, int32_t pass, double* bytes_per_particle)
+{
+    if (!ctx || !bytes_per_particle || pass < 0 || pass >= FJSPH_T_COUNT) return FJSPH_ERR_ARG;
+    const double D = ctx->s.dim;
+    const double ale = ctx->s.ale ? 1.0 : 0.0;
+    double b = 0.0;
+    switch (pass) {
+    case FJSPH_T_FORCE_UPDATE:
+        /* reads: x, v (2*8D), rho, p, m (24), b, part_id, blk, cellID, count
+         * (20), cellV (8D), cellP, cellRho, lam_nb, woccl, deltaD (40),
+         * aVisc (8D), pn x, v, acc (3*8D), pn rho, Rrho (16), ALE vPert
+         * (8D) + surfzone (4); writes: acc, Af (2*8D), Rrho (8), next x, v
+         * (2*8D), next rho, p (16). */
+        b = 8 * D * (2 + 1 + 1 + 3 + 2 + 2) + 24 + 20 + 40 + 16 + 8 + 16 + ale * (8 * D + 4);
+        break;
+    case FJSPH_T_DSPH:
+        /* reads x (8D), rho, m, b, part_id, count; writes L (8D^2), gradRho,
+         * norm (16D), lam, lam_nb, colourG, kernsum, colour (40), cellV (8D),
+         * cellID (4) */
+        b = 8 * D + 16 + 12 + 8 * D * D + 16 * D + 40 + 8 * D + 4;
+        break;
+    case FJSPH_T_SURFACE:
+        b = 2 * (8 * D + 8 * D * D + 8 * D + 48) + 8 * D * 3 + 24;
+        break;
+    case FJSPH_T_NEIGHBOUR:
+        b = 8 * D + 4 * 4 + 4;
+        break;
+    default:
+        b = 48;
+    }
+    *bytes_per_particle = b;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_neighbour_stats(fjsph_ctx* ctx, int64_t* pairs, int32_t* max_count)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        require(ctx, true);
+        sync(ctx);
+        std::vector<int32_t> c(static_cast<size_t>(ctx->n));
+        HIPCHK(hipMemcpy(c.data(), ctx->nl.count, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
+        int64_t t = 0;
+        int32_t mx = 0;
+        for (int32_t v : c) { t += v; mx = std::max(mx, v); }
+        if (pairs) *pairs = t;
+        if (max_count) *max_count = mx;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+/*
+ * Internal types of the MI355X WCSPH engine: device state layout, constants
+ * passed to kernels, and the kernel launch wrappers (kernels.hip).
+ *
+ * Device layout (all arrays in cell-sorted slot order):
+ *   - one DevState per reference state (pn, pnp1): every SPHPart field the
+ *     step reads, FP64, vectors interleaved per particle (x[s*D + k]);
+ *   - x, v, rho, p of pnp1 exist twice (cur / next) so the fused
+ *     force + Newmark kernel reads iterate k and writes k+1 without races;
+ *   - neighbour lists column-major: idx[k*npad + s], frozen squared distance
+ *     rr[k*npad + s] (FP64), count[s] (self excluded).
+ */
+#ifndef FJSPH_ENGINE_INTERNAL_HPP
+#define FJSPH_ENGINE_INTERNAL_HPP
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fjsph_engine.h"
+
+namespace fj {
+
+/* Constants of one launch (derived once per context). */
+struct Consts {
+    double H, H_sq, sr, Wc, W_dx, B, gam, rho0, c0_sq, press_back, nu, visc_alpha, dsph_cont;
+    double rho_min, rho_max, dx, particle_step, lam_cutoff, interp_fac, i_n_full;
+    double A_plate, A_sphere, L, sos, gamma_g, mu_g, rho_g, ycoef, Cb, max_shift_vel, lam_st, sig;
+    double grav[3];
+    double v_inf[3];
+    int32_t eos, aero_case, aero_source, use_lam, use_TAB_def, ale;
+    int64_t n_bound;
+};
+
+struct DevState {
+    double *x, *v, *acc, *Af, *aVisc, *Rrho, *rho, *p, *m, *deltaD, *L, *gradRho, *norm;
+    double *lam, *lam_nb, *colourG, *colour, *kernsum, *curve, *norm_curve, *woccl, *pDist;
+    double *cellV, *cellP, *cellRho, *vPert;
+    int32_t *surf, *cellID, *surfzone, *b;
+};
+
+struct NbrList {
+    int32_t* idx;
+    double* rr;
+    int32_t* count;
+    int64_t npad;
+    int32_t cap;
+};
+
+/* Per-block boundary settings visible to kernels. */
+struct DevBlocks {
+    int32_t* solver;   /* per block */
+    int32_t* no_slip;  /* per block */
+    double* vel;       /* per block, 3 each (current velocity) */
+    int32_t n;
+};
+
+/* Scalars kept on the device between kernels. */
+struct DevScalars {
+    double npd;       /* numerical particle density of the last dSPH pass */
+    double residual;  /* sum |x' - x_prev|^2 of the last Newmark iterate */
+    double red[8];    /* find_timestep reduction results */
+    double bbox[6];   /* lo[3], hi[3] */
+    int32_t max_count;
+    int32_t overflow;
+};
+
+/* ---- launch wrappers (kernels.hip); every one enqueues on `st` ---------- */
+
+void launch_bbox(int dim, const double* x, int64_t n, double* partial, int nparts, double* out, hipStream_t st);
+void launch_cells(int dim, const double* x, int64_t n, const double* lo, double inv_cs, const int32_t* dims,
+                  int32_t* cell, int32_t* cell_count, hipStream_t st);
+void launch_scan(const int32_t* count, int64_t ncell, int32_t* start, int32_t* block_sums, hipStream_t st);
+void launch_scatter(const int32_t* cell, int64_t n, const int32_t* start, int32_t* fill, int32_t* order,
+                    hipStream_t st);
+void launch_sort_cells(const int32_t* start, int64_t ncell, int32_t* order, hipStream_t st);
+void launch_build_lists(int dim, const double* x, int64_t n, const int32_t* cell, const int32_t* dims,
+                        const int32_t* start, const int32_t* order, double sr, NbrList nl, DevScalars* sc,
+                        hipStream_t st);
+void launch_gather_d(const double* src, double* dst, const int32_t* perm, int64_t n, int width, hipStream_t st);
+void launch_gather_i(const int32_t* src, int32_t* dst, const int32_t* perm, int64_t n, hipStream_t st);
+void launch_iota(int32_t* a, int64_t n, hipStream_t st);
+
+/* dSPH pass; npd (sum / n) lands in sc->npd. */
+void launch_dsph(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
+                 int with_aero_vel, double* partial, int nparts, DevScalars* sc, hipStream_t st);
+void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
+                          hipStream_t st);
+void launch_surface1(int dim, int prec, const Consts& c, DevState s, const int32_t* part_id, NbrList nl,
+                     int64_t n, double* norms_tmp, int with_dissipation, hipStream_t st);
+void launch_dissipation(int dim, int prec, const Consts& c, DevState s, const int32_t* part_id, NbrList nl,
+                        int64_t n, hipStream_t st);
+void launch_surface2(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
+                     const double* norms_tmp, hipStream_t st);
+void launch_shift(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
+                  hipStream_t st);
+void launch_boundary(int dim, const Consts& c, DevState s, const int32_t* part_id, const int32_t* blk,
+                     DevBlocks bl, int32_t block, NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st);
+/* Force pass on `s` (x/v/rho/p read from s); when `next` is non-null the
+ * Newmark update of Do_NB_Iter is fused and writes next.x/v/rho/p for every
+ * slot, plus per-block residual partial sums. */
+void launch_force(int dim, int prec, const Consts& c, DevState s, DevState pn, const double* next_x,
+                  double* next_x_w, double* next_v, double* next_rho, double* next_p, const int32_t* part_id,
+                  const int32_t* blk, DevBlocks bl, const int32_t* near_inlet, NbrList nl, int64_t n,
+                  double dt, const DevScalars* sc, double* partial, int nparts, hipStream_t st);
+void launch_sum_partials(const double* partial, int nparts, int width, double* out, hipStream_t st);
+void launch_timestep_reduce(int dim, const Consts& c, DevState s, const int32_t* part_id, int64_t n,
+                            double* partial, int nparts, double* out, hipStream_t st);
+void launch_fix_rho(DevState s, const int32_t* part_id, int64_t nb, int64_t n, double rho0, hipStream_t st);
+
+/* Number of blocks used for per-block partial reductions. */
+int reduction_parts(int64_t n);
+
+}  // namespace fj
+
+#endif

@@ -1,0 +1,276 @@
+"""Python host side of the WCSPH step engine: a ctypes binding of the C-ABI
+in include/fjsph_engine.h, exposing the reference's per-step units
+(update_neighbours, dSPH_PreStep, Detect_Surface, dissipation_terms,
+particle_shift, get_acc_and_Rrho, Do_NB_Iter, find_timestep,
+Integrator::integrate) on a HIP device.
+
+There is no CPU fallback: if libfjsph_engine.so is missing or no HIP device
+is visible, construction raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfjsph_engine.so")
+_LIB = None
+
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+_VP = C.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in include/fjsph_engine.h
+SIGNATURES = {
+    "fjsph_abi_version": (C.c_int32, []),
+    "fjsph_settings_default": (None, [C.POINTER(abi.Settings), C.c_int32]),
+    "fjsph_derive": (C.c_int, [C.POINTER(abi.Settings), C.POINTER(abi.Derived)]),
+    "fjsph_settings_parse_file": (C.c_int, [C.POINTER(abi.Settings), C.c_char_p]),
+    "fjsph_create": (C.c_int, [C.POINTER(abi.Settings), C.c_int32, C.POINTER(_VP)]),
+    "fjsph_destroy": (None, [_VP]),
+    "fjsph_last_error": (C.c_char_p, [_VP]),
+    "fjsph_upload_state": (C.c_int, [_VP, C.c_int64, C.c_int64, _D, _D, _D, _D, _D, _I32]),
+    "fjsph_set_blocks": (C.c_int, [_VP, C.c_int32, C.POINTER(abi.Block)]),
+    "fjsph_set_block_motion": (C.c_int, [_VP, C.c_int32, C.c_int32, _D, _D]),
+    "fjsph_download_field": (C.c_int, [_VP, C.c_int32, C.c_int32, _D]),
+    "fjsph_upload_field": (C.c_int, [_VP, C.c_int32, C.c_int32, _D]),
+    "fjsph_build_neighbours": (C.c_int, [_VP]),
+    "fjsph_get_neighbour_counts": (C.c_int, [_VP, _I32]),
+    "fjsph_get_neighbours": (C.c_int, [_VP, _I64, _I64, C.c_int64]),
+    "fjsph_dsph_prestep": (C.c_int, [_VP, _D]),
+    "fjsph_aero_velocity": (C.c_int, [_VP]),
+    "fjsph_detect_surface": (C.c_int, [_VP]),
+    "fjsph_dissipation_terms": (C.c_int, [_VP]),
+    "fjsph_particle_shift": (C.c_int, [_VP]),
+    "fjsph_boundary_update": (C.c_int, [_VP]),
+    "fjsph_acc_rrho": (C.c_int, [_VP, C.c_double]),
+    "fjsph_nb_iter": (C.c_int, [_VP, C.c_double, _D]),
+    "fjsph_find_timestep": (C.c_int, [_VP, _D]),
+    "fjsph_integrate_no_update": (C.c_int, [_VP, C.POINTER(abi.StepInfo)]),
+    "fjsph_integrate_step": (C.c_int, [_VP, C.POINTER(abi.StepInfo)]),
+    "fjsph_advance_frame": (C.c_int, [_VP]),
+    "fjsph_get_time": (C.c_int, [_VP, _D, _D, _D]),
+    "fjsph_set_time": (C.c_int, [_VP, C.c_double, C.c_double, C.c_double]),
+    "fjsph_enable_timers": (C.c_int, [_VP, C.c_int32]),
+    "fjsph_get_timers": (C.c_int, [_VP, _D, _I64]),
+    "fjsph_pass_bytes": (C.c_int, [_VP, C.c_int32, _D]),
+    "fjsph_neighbour_stats": (C.c_int, [_VP, _I64, _I32]),
+}
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libfjsph_engine.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise EngineError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if lib.fjsph_abi_version() != abi.ABI_VERSION:
+        raise EngineError("engine ABI version mismatch")
+    _LIB = lib
+    return lib
+
+
+def _ptr(a, t=_D):
+    return a.ctypes.data_as(t)
+
+
+def settings_default(dim):
+    s = abi.Settings()
+    load().fjsph_settings_default(C.byref(s), dim)
+    return s
+
+
+def derive(settings):
+    d = abi.Derived()
+    rc = load().fjsph_derive(C.byref(settings), C.byref(d))
+    if rc != abi.OK:
+        raise EngineError(f"fjsph_derive failed ({rc})")
+    return d
+
+
+def parse_para(path, dim):
+    s = settings_default(dim)
+    rc = load().fjsph_settings_parse_file(C.byref(s), path.encode())
+    if rc != abi.OK:
+        raise EngineError(f"could not parse {path} ({rc})")
+    return s
+
+
+class Engine:
+    """One engine context on a HIP device; methods follow the reference units."""
+
+    def __init__(self, case_or_settings, device=0):
+        self.lib = load()
+        settings = case_or_settings.settings if hasattr(case_or_settings, "settings") else case_or_settings
+        self.settings = settings.copy()
+        self.dim = int(settings.dim)
+        h = _VP()
+        rc = self.lib.fjsph_create(C.byref(self.settings), int(device), C.byref(h))
+        if rc != abi.OK:
+            raise EngineError(f"fjsph_create failed with status {rc}")
+        self.h = h
+        self.n = 0
+        if hasattr(case_or_settings, "x"):
+            self.upload_case(case_or_settings)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fjsph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != abi.OK:
+            msg = self.lib.fjsph_last_error(self.h)
+            raise EngineError(f"{what}: status {rc}: {msg.decode() if msg else ''}")
+
+    # ---- state
+    def upload_case(self, case):
+        self.set_blocks(case.blocks)
+        self.upload_state(case.x, case.v, case.rho, case.p, case.m, case.b, case.n_bound)
+
+    def upload_state(self, x, v, rho, p, m, b, n_bound):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        rho = np.ascontiguousarray(rho, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        m = np.ascontiguousarray(m, np.float64)
+        b = np.ascontiguousarray(b, np.int32)
+        self.n = x.shape[0]
+        self._chk(self.lib.fjsph_upload_state(self.h, self.n, int(n_bound), _ptr(x), _ptr(v), _ptr(rho),
+                                              _ptr(p), _ptr(m), _ptr(b, _I32)), "upload_state")
+
+    def set_blocks(self, specs):
+        arr = abi.make_blocks(specs)
+        self._chk(self.lib.fjsph_set_blocks(self.h, len(specs), arr), "set_blocks")
+
+    def set_block_motion(self, block, times, vels):
+        t = np.ascontiguousarray(times, np.float64)
+        v = np.ascontiguousarray(vels, np.float64).reshape(-1, 3)
+        self._chk(self.lib.fjsph_set_block_motion(self.h, int(block), len(t), _ptr(t), _ptr(v)), "set_block_motion")
+
+    def field(self, name, which=abi.STATE_NP1):
+        w = abi.field_width(name, self.dim)
+        out = np.zeros(self.n * w)
+        self._chk(self.lib.fjsph_download_field(self.h, which, abi.F[name], _ptr(out)), f"download {name}")
+        return out.reshape(self.n, w) if w > 1 else out
+
+    def set_field(self, name, values, which=abi.STATE_NP1):
+        a = np.ascontiguousarray(values, np.float64).ravel()
+        self._chk(self.lib.fjsph_upload_field(self.h, which, abi.F[name], _ptr(a)), f"upload {name}")
+
+    # ---- passes
+    def build_neighbours(self):
+        self._chk(self.lib.fjsph_build_neighbours(self.h), "build_neighbours")
+
+    def neighbour_counts(self):
+        out = np.zeros(self.n, np.int32)
+        self._chk(self.lib.fjsph_get_neighbour_counts(self.h, _ptr(out, _I32)), "neighbour_counts")
+        return out
+
+    def neighbours(self):
+        counts = self.neighbour_counts()
+        total = int(counts.astype(np.int64).sum())
+        off = np.zeros(self.n + 1, np.int64)
+        idx = np.zeros(total, np.int64)
+        self._chk(self.lib.fjsph_get_neighbours(self.h, _ptr(off, _I64), _ptr(idx, _I64), total), "neighbours")
+        return off, idx
+
+    def neighbour_stats(self):
+        pairs = np.zeros(1, np.int64)
+        mx = np.zeros(1, np.int32)
+        self._chk(self.lib.fjsph_neighbour_stats(self.h, _ptr(pairs, _I64), _ptr(mx, _I32)), "neighbour_stats")
+        return int(pairs[0]), int(mx[0])
+
+    def dsph_prestep(self):
+        v = np.zeros(1)
+        self._chk(self.lib.fjsph_dsph_prestep(self.h, _ptr(v)), "dsph_prestep")
+        return float(v[0])
+
+    def aero_velocity(self):
+        self._chk(self.lib.fjsph_aero_velocity(self.h), "aero_velocity")
+
+    def detect_surface(self):
+        self._chk(self.lib.fjsph_detect_surface(self.h), "detect_surface")
+
+    def dissipation_terms(self):
+        self._chk(self.lib.fjsph_dissipation_terms(self.h), "dissipation_terms")
+
+    def particle_shift(self):
+        self._chk(self.lib.fjsph_particle_shift(self.h), "particle_shift")
+
+    def boundary_update(self):
+        self._chk(self.lib.fjsph_boundary_update(self.h), "boundary_update")
+
+    def acc_rrho(self, npd):
+        self._chk(self.lib.fjsph_acc_rrho(self.h, float(npd)), "acc_rrho")
+
+    def nb_iter(self, npd):
+        v = np.zeros(1)
+        self._chk(self.lib.fjsph_nb_iter(self.h, float(npd), _ptr(v)), "nb_iter")
+        return float(v[0])
+
+    def find_timestep(self):
+        v = np.zeros(1)
+        self._chk(self.lib.fjsph_find_timestep(self.h, _ptr(v)), "find_timestep")
+        return float(v[0])
+
+    def integrate_no_update(self):
+        info = abi.StepInfo()
+        self._chk(self.lib.fjsph_integrate_no_update(self.h, C.byref(info)), "integrate_no_update")
+        return info.as_dict()
+
+    def integrate_step(self):
+        info = abi.StepInfo()
+        self._chk(self.lib.fjsph_integrate_step(self.h, C.byref(info)), "integrate_step")
+        return info.as_dict()
+
+    def advance_frame(self):
+        self._chk(self.lib.fjsph_advance_frame(self.h), "advance_frame")
+
+    def get_time(self):
+        t, dt, cfl = np.zeros(1), np.zeros(1), np.zeros(1)
+        self._chk(self.lib.fjsph_get_time(self.h, _ptr(t), _ptr(dt), _ptr(cfl)), "get_time")
+        return float(t[0]), float(dt[0]), float(cfl[0])
+
+    def set_time(self, t, dt, cfl):
+        self._chk(self.lib.fjsph_set_time(self.h, float(t), float(dt), float(cfl)), "set_time")
+
+    # ---- measurement
+    def enable_timers(self, on=True):
+        self._chk(self.lib.fjsph_enable_timers(self.h, int(bool(on))), "enable_timers")
+
+    def timers(self):
+        ms = np.zeros(len(abi.TIMER_NAMES))
+        cnt = np.zeros(len(abi.TIMER_NAMES), np.int64)
+        self._chk(self.lib.fjsph_get_timers(self.h, _ptr(ms), _ptr(cnt, _I64)), "get_timers")
+        return {name: (float(ms[k]), int(cnt[k])) for k, name in enumerate(abi.TIMER_NAMES)}
+
+    def pass_bytes(self, pass_id):
+        v = np.zeros(1)
+        self._chk(self.lib.fjsph_pass_bytes(self.h, int(pass_id), _ptr(v)), "pass_bytes")
+        return float(v[0])

@@ -1,0 +1,290 @@
+/*
+ * fjsph_engine.h -- C-ABI of the MI355X WCSPH step engine.
+ *
+ * The engine replaces the per-step particle operators of FJSPH
+ * (src/Neighbours.h, src/Shifting.h, src/Geometry.h, src/Resid.h,
+ * src/Newmark_Beta.h, src/Runge_Kutta.h, src/Integration.h) with hand-written
+ * gfx950 kernels.  A host driver owns the settings and calls these entry points
+ * in the order the reference's Integrator does; every call is synchronous on
+ * the context's HIP stream unless stated otherwise.
+ *
+ * Conventions
+ *  - Plain C types only; no exceptions cross this boundary.  Every entry point
+ *    returns an fjsph_status; fjsph_last_error(ctx) holds the message of the
+ *    last failure on that context.  The reference prints and calls exit(-1)
+ *    on fatal input errors (src/IO.cpp:708-712); here the call fails instead.
+ *  - Host arrays are in HOST ORDER: the order of the upload, boundary
+ *    particles first ([0, n_bound)), then fluid ([n_bound, n_total)), as
+ *    Init_Particles lays them out (src/Init.cpp:288-484).  The engine keeps
+ *    particles sorted by cell on the device and maps back on download.
+ *  - Vectors are interleaved per particle: x[i*dim + k].
+ *  - One context is used by one host thread at a time.
+ */
+#ifndef FJSPH_ENGINE_H
+#define FJSPH_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FJSPH_ABI_VERSION 1
+
+typedef enum fjsph_status {
+    FJSPH_OK = 0,
+    FJSPH_ERR_ARG = 1,         /* bad argument (null pointer, size, enum) */
+    FJSPH_ERR_HIP = 2,         /* a HIP runtime call failed */
+    FJSPH_ERR_OOM = 3,         /* device allocation failed */
+    FJSPH_ERR_STATE = 4,       /* call out of order (e.g. no state uploaded) */
+    FJSPH_ERR_UNSUPPORTED = 5, /* option not built into this engine */
+    FJSPH_ERR_NO_DEVICE = 6    /* no HIP device visible */
+} fjsph_status;
+
+/* Particle type flag b (src/VarDefs.h:92-102). */
+enum {
+    FJSPH_BOUND = 0,
+    FJSPH_PISTON = 1,
+    FJSPH_BUFFER = 2,
+    FJSPH_BACK = 3,
+    FJSPH_PIPE = 4,
+    FJSPH_FREE = 5,
+    FJSPH_OUTLET = 6,
+    FJSPH_LOST = 7
+};
+
+/* Arithmetic of the pair passes. */
+enum {
+    FJSPH_PREC_FP64 = 0,  /* every pair term in FP64 (parity path)               */
+    FJSPH_PREC_MIXED = 1  /* FP32 pair terms, FP64 accumulation, FP64 x/t/residual */
+};
+
+enum { FJSPH_NEWMARK_BETA = 0, FJSPH_RUNGE_KUTTA = 1 };     /* src/VarDefs.h:129-133 */
+enum { FJSPH_DBC = 0, FJSPH_PRESSURE_G = 1, FJSPH_GHOST = 2 }; /* src/VarDefs.h:135-140 */
+enum { FJSPH_COLE_EOS = 0, FJSPH_ISO_EOS = 1 };            /* src/VarDefs.h:142-146 */
+enum { FJSPH_NO_AERO = 0, FJSPH_GISSLER = 1, FJSPH_INDUCED_PRESSURE = 2, FJSPH_SKIN_FRIC = 3 };
+enum { FJSPH_CONST_VEL = 0, FJSPH_MESH_INFL = 1 };          /* src/VarDefs.h:168-173 */
+
+/*
+ * Settings: the fields of FLUID / AERO / INTEG_SETT / SIM that the hot path
+ * reads (src/Var.h:63-92, 160-237, 240-310, 337-394), under the para-file keys
+ * of GetInput (src/IO.cpp:349-453).  fjsph_settings_default() fills the
+ * reference's defaults; derived constants (Set_Values, src/IO.cpp:26-128) are
+ * computed by the engine and can be read back with fjsph_derive().
+ */
+typedef struct fjsph_settings {
+    double particle_step;       /* "SPH initial spacing"                       */
+    double H_fac;               /* "SPH smoothing length factor"    (2.0)      */
+    double rho_rest;            /* "Reference dispersed density"    (1000)     */
+    double speed_sound;         /* "SPH speed of sound"             (300)      */
+    double gam;                 /* Tait gamma                       (7)        */
+    double mu;                  /* "Reference dispersed viscosity"  (8.94e-4)  */
+    double sig;                 /* "Reference surface tension"      (0.0708)   */
+    double visc_alpha;          /* "SPH artificial viscosity factor"(0.1)      */
+    double dsph_delta;          /* "SPH delta coefficient"          (0.1)      */
+    double press_back;          /* "SPH background pressure"        (0)        */
+    double press_pipe;          /* "SPH starting pressure"          (0)        */
+    double rho_var;             /* "SPH maximum absolute density variation (%)" (50) */
+    double rho_max;             /* "SPH maximum density" (1500 = unset)        */
+    double rho_min;             /* "SPH minimum density" (500 = unset)         */
+    double rho_max_iter;        /* "SPH density variation to reduce timestep (%)" (1) */
+    double grav[3];             /* "SPH gravity vector"                        */
+    double cfl;                 /* "SPH CFL condition"              (1.0)      */
+    double cfl_min;             /* "SPH minimum CFL"                (0.1)      */
+    double cfl_max;             /* "SPH maximum CFL"                (2.0)      */
+    double cfl_step;            /* "SPH unstable CFL step"          (0.05)     */
+    double subits_factor;       /* "SPH stable CFL count iteration factor" (0.333) */
+    double min_residual;        /* "SPH solver minimum residual"    (-7)       */
+    double delta_t_max;         /* "SPH maximum timestep"           (1.0)      */
+    double delta_t_min;         /* "SPH minimum timestep"           (0)        */
+    double frame_time_interval; /* "SPH frame time interval"                   */
+    double max_shift_vel;       /* "SPH maximum shifting velocity"  (9999999)  */
+    double v_inf[3];            /* "SPH freestream velocity"                   */
+    double p_ref;               /* "Reference pressure"             (101353)   */
+    double rho_g;               /* "Reference density"              (1.29251)  */
+    double mu_g;                /* "Sutherland reference viscosity" (1.716e-5) */
+    double temp_g;              /* "Reference temperature"          (298)      */
+    double R_g;                 /* specific gas constant            (287)      */
+    double gamma_g;             /* "Gas constant gamma"             (1.403)    */
+    double lam_cutoff;          /* "SPH aerodynamic cutoff value"   (0.75)     */
+    double i_interp_fac;        /* "SPH aerodynamic interpolation factor" (0.5) */
+    double Cf, Ck, Cd, Cb;      /* TAB constants (1/3, 8, 5, 0.5)              */
+    int32_t dim;                /* SIMDIM: 2 or 3                              */
+    int32_t precision;          /* FJSPH_PREC_*                                */
+    int32_t integrator;         /* "SPH integration solver": FJSPH_NEWMARK_BETA / FJSPH_RUNGE_KUTTA */
+    int32_t eos;                /* "SPH equation of state (0=Cole/1=Isothermal)" */
+    int32_t aero_case;          /* "SPH aerodynamic case": FJSPH_NO_AERO ...   */
+    int32_t aero_source;        /* FJSPH_CONST_VEL / FJSPH_MESH_INFL           */
+    int32_t use_lam;            /* "SPH interpolation factor (0=ncount/1=lambda)" (1) */
+    int32_t use_TAB_def;        /* "SPH use TAB deformation (0/1)"  (0)        */
+    int32_t ale;                /* ALE particle shifting (the reference's -DALE build) */
+    int32_t max_subits;         /* "SPH Newmark-Beta iteration limit" (20)     */
+    int32_t n_stable_limit;     /* "SPH stable CFL count limit"     (10)       */
+    int32_t n_unstable_limit;   /* "SPH unstable CFL count limit"   (3)        */
+} fjsph_settings;
+
+/* Constants derived by Set_Values (src/IO.cpp:26-128). */
+typedef struct fjsph_derived {
+    double H, H_sq, sr, W_correc, W_dx, B, dsph_cont, nu;
+    double sim_mass, bnd_mass, rho_max, rho_min, dx;
+    double n_full, i_n_full, interp_fac, A_plate, A_sphere, L;
+    double sos, i_sos_sq, td, omega, tmax, Cdef, ycoef;
+    double lam_st;  /* pairwise surface-tension normaliser (src/Resid.cpp:437-439) */
+} fjsph_derived;
+
+/* A boundary or fluid block: the bound_block fields the step reads
+ * (src/Var.h:820-881). Blocks are given in host order and must tile
+ * [0, n_total) with boundary blocks first. */
+typedef struct fjsph_block {
+    int64_t first;        /* index.first  */
+    int64_t last;         /* index.second (exclusive) */
+    int32_t is_fluid;     /* 0 = boundary block, 1 = fluid block */
+    int32_t bound_solver; /* FJSPH_DBC / FJSPH_PRESSURE_G / FJSPH_GHOST */
+    int32_t no_slip;      /* "Wall is no slip (0/1)" */
+    int32_t n_times;      /* 0: constant velocity vel; else see fjsph_set_block_motion */
+    double vel[3];        /* vels[0] */
+} fjsph_block;
+
+/* Per-step report of Integrator::integrate (src/Integration.cpp:233-303). */
+typedef struct fjsph_step_info {
+    double rms_error;   /* log-residual of the last sub-iteration        */
+    double dt;          /* timestep used                                  */
+    double cfl;         /* CFL after the adaptation                       */
+    double time;        /* simulation time after the step                 */
+    double max_rho_pc;  /* maxRho_pc                                      */
+    double maxf;        /* max |acc|                                      */
+    double maxAf;       /* max |Af|                                       */
+    double npd;         /* numerical particle density of the step         */
+    int32_t iterations; /* sub-iteration counter `iteration`              */
+    int32_t rollbacks;  /* rejected sub-iteration sequences (dt halvings) */
+} fjsph_step_info;
+
+/* Field identifiers for fjsph_download_field (names follow SPHPart,
+ * src/Var.h:600-641).  Vector fields have dim components, L has dim*dim
+ * (row-major), scalars one. */
+enum {
+    FJSPH_F_X = 0, FJSPH_F_V, FJSPH_F_ACC, FJSPH_F_AF, FJSPH_F_AVISC,
+    FJSPH_F_RHO, FJSPH_F_P, FJSPH_F_RRHO, FJSPH_F_M, FJSPH_F_DELTAD,
+    FJSPH_F_L, FJSPH_F_GRADRHO, FJSPH_F_NORM, FJSPH_F_LAM, FJSPH_F_LAM_NB,
+    FJSPH_F_COLOURG, FJSPH_F_COLOUR, FJSPH_F_KERNSUM, FJSPH_F_SURF,
+    FJSPH_F_CURVE, FJSPH_F_NORM_CURVE, FJSPH_F_WOCCL, FJSPH_F_PDIST,
+    FJSPH_F_CELLV, FJSPH_F_CELLP, FJSPH_F_CELLRHO, FJSPH_F_CELLID,
+    FJSPH_F_VPERT, FJSPH_F_SURFZONE, FJSPH_F_B,
+    FJSPH_F_COUNT
+};
+enum { FJSPH_STATE_N = 0, FJSPH_STATE_NP1 = 1 }; /* pn / pnp1 */
+
+/* ---- settings ------------------------------------------------------------ */
+
+/* Fill the reference defaults (src/Var.h:63-310) for dimension dim. */
+void fjsph_settings_default(fjsph_settings* s, int32_t dim);
+
+/* Set_Values (src/IO.cpp:26-128) plus get_n_full (src/Geometry.cpp:282-308). */
+fjsph_status fjsph_derive(const fjsph_settings* s, fjsph_derived* out);
+
+/* Parse a para file in the reference format ("key: value # comment",
+ * GetInput src/IO.cpp:305-723, Get_Number/Get_Vector src/IOFunctions.h:32-230)
+ * into s (which should hold defaults first). Unknown keys are ignored, as in
+ * the reference. */
+fjsph_status fjsph_settings_parse_file(fjsph_settings* s, const char* path);
+
+/* ---- context ------------------------------------------------------------- */
+
+typedef struct fjsph_ctx fjsph_ctx;
+
+int32_t fjsph_abi_version(void);
+
+/* Create an engine on HIP device `device`. Replaces the driver-side SIM/pn/
+ * pnp1/outlist locals of main (src/FJSPH.cpp:49-58). */
+fjsph_status fjsph_create(const fjsph_settings* s, int32_t device, fjsph_ctx** out);
+void fjsph_destroy(fjsph_ctx* ctx);
+const char* fjsph_last_error(const fjsph_ctx* ctx);
+
+/* Upload the initial state (pn = pnp1 = state, src/Init.cpp:493) in host
+ * order. x, v: n_total*dim; rho, p, m: n_total; b: particle types. For the
+ * constant-velocity aero source, cellV/cellP/cellRho are initialised as in
+ * src/FJSPH.cpp:115-126. Time is reset to 0. */
+fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound,
+                                const double* x, const double* v, const double* rho,
+                                const double* p, const double* m, const int32_t* b);
+
+/* Boundary/fluid blocks (LIMITS, src/Var.h:883). */
+fjsph_status fjsph_set_blocks(fjsph_ctx* ctx, int32_t n_blocks, const fjsph_block* blocks);
+/* Time-velocity table of a moving boundary block (bound_block::times/vels). */
+fjsph_status fjsph_set_block_motion(fjsph_ctx* ctx, int32_t block, int32_t n_times,
+                                    const double* times, const double* vels);
+
+/* Download one field of pn or pnp1 in host order (see FJSPH_F_*). */
+fjsph_status fjsph_download_field(fjsph_ctx* ctx, int32_t which, int32_t field, double* out);
+/* Overwrite one field of pn or pnp1 from host order (test and restart use). */
+fjsph_status fjsph_upload_field(fjsph_ctx* ctx, int32_t which, int32_t field, const double* in);
+
+/* ---- per-pass entry points (same units as the reference's functions) ------ */
+
+/* update_neighbours (src/Neighbours.cpp:7-31) on pnp1: the set of j with
+ * |x_i - x_j|^2 < 4H^2 in FP64, self included. */
+fjsph_status fjsph_build_neighbours(fjsph_ctx* ctx);
+/* Neighbour counts (self included) in host order. */
+fjsph_status fjsph_get_neighbour_counts(fjsph_ctx* ctx, int32_t* counts);
+/* CSR dump of the lists in host order: offsets[n_total+1], idx[offsets[n]];
+ * each list sorted ascending, self included. */
+fjsph_status fjsph_get_neighbours(fjsph_ctx* ctx, int64_t* offsets, int64_t* idx, int64_t idx_capacity);
+
+/* dSPH_PreStep (src/Shifting.cpp:12-123) over all particles; returns npd. */
+fjsph_status fjsph_dsph_prestep(fjsph_ctx* ctx, double* npd);
+/* get_aero_velocity, constant-velocity source (src/Resid.cpp:569-611). */
+fjsph_status fjsph_aero_velocity(fjsph_ctx* ctx);
+/* Detect_Surface (src/Geometry.cpp:14-280). */
+fjsph_status fjsph_detect_surface(fjsph_ctx* ctx);
+/* dissipation_terms (src/Shifting.cpp:126-186). */
+fjsph_status fjsph_dissipation_terms(fjsph_ctx* ctx);
+/* particle_shift (src/Shifting.cpp:189-290); requires settings.ale. */
+fjsph_status fjsph_particle_shift(fjsph_ctx* ctx);
+/* Boundary velocity, no-slip and boundary solver of Do_NB_Iter
+ * (src/Newmark_Beta.cpp:69-132, src/Resid.cpp:21-186). */
+fjsph_status fjsph_boundary_update(fjsph_ctx* ctx);
+/* get_acc_and_Rrho (src/Resid.cpp:426-469) on pnp1 with the given npd. */
+fjsph_status fjsph_acc_rrho(fjsph_ctx* ctx, double npd);
+/* Newmark_Beta::Do_NB_Iter (src/Newmark_Beta.cpp:54-301) followed by the
+ * residual of Check_Error (src/Newmark_Beta.cpp:16-25): returns
+ * log10(sqrt(sum|x'-x_prev|^2 / n_fluid)). */
+fjsph_status fjsph_nb_iter(fjsph_ctx* ctx, double npd, double* log_error);
+/* Integrator::find_timestep (src/Integration.cpp:370-443) on pnp1. */
+fjsph_status fjsph_find_timestep(fjsph_ctx* ctx, double* dt);
+
+/* Integrator::integrate_no_update (src/Integration.cpp:27-107): the start-up
+ * pass the driver makes before its frame loop (src/FJSPH.cpp:183). */
+fjsph_status fjsph_integrate_no_update(fjsph_ctx* ctx, fjsph_step_info* info);
+/* Integrator::integrate (src/Integration.cpp:233-303): one full step,
+ * including update_data's pn <- pnp1 and the adaptive CFL. */
+fjsph_status fjsph_integrate_step(fjsph_ctx* ctx, fjsph_step_info* info);
+/* Advance the frame clock: last_frame_time += frame_time_interval
+ * (src/FJSPH.cpp:322-323). */
+fjsph_status fjsph_advance_frame(fjsph_ctx* ctx);
+/* Simulation clock and timestep (INTEG_SETT current_time / delta_t). */
+fjsph_status fjsph_get_time(fjsph_ctx* ctx, double* time, double* delta_t, double* cfl);
+fjsph_status fjsph_set_time(fjsph_ctx* ctx, double time, double delta_t, double cfl);
+
+/* ---- measurement ---------------------------------------------------------- */
+
+/* Pass ids of the engine's own timers (HIP events on the context stream). */
+enum {
+    FJSPH_T_NEIGHBOUR = 0, FJSPH_T_DSPH, FJSPH_T_SURFACE, FJSPH_T_DISSIPATION,
+    FJSPH_T_SHIFT, FJSPH_T_BOUNDARY, FJSPH_T_FORCE_UPDATE, FJSPH_T_REDUCE,
+    FJSPH_T_COUNT
+};
+/* Enable (1) / disable (0) per-pass timing; enabling clears the totals. */
+fjsph_status fjsph_enable_timers(fjsph_ctx* ctx, int32_t on);
+/* Total milliseconds and launch counts per pass since timers were enabled. */
+fjsph_status fjsph_get_timers(fjsph_ctx* ctx, double* ms, int64_t* launches);
+/* Number of particles processed per launch of a pass id, and the
+ * algorithmic bytes per particle the engine moves in that pass. */
+fjsph_status fjsph_pass_bytes(fjsph_ctx* ctx, int32_t pass, double* bytes_per_particle);
+/* Current neighbour statistics: total pairs (self excluded) and max count. */
+fjsph_status fjsph_neighbour_stats(fjsph_ctx* ctx, int64_t* pairs, int32_t* max_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FJSPH_ENGINE_H */

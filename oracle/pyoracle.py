@@ -1,0 +1,222 @@
+"""ctypes front end of the FP64 oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  It drives liboracle.so (or libbaseline.so for timing) with the
+same arrays and settings the engine takes.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from fjsph_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+
+
+def _ptr(a, t=_D):
+    return a.ctypes.data_as(t)
+
+
+def load(kind="oracle"):
+    """kind 'oracle' (the checker build) or 'baseline' (reference flags)."""
+    if kind in _LIBS:
+        return _LIBS[kind]
+    path = os.path.join(_HERE, "liboracle.so" if kind == "oracle" else "libbaseline.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() or make -C oracle")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    sig = {
+        "orc_create": (vp, [C.POINTER(abi.Settings), C.c_int]),
+        "orc_destroy": (None, [vp]),
+        "orc_derive": (C.c_int, [C.POINTER(abi.Settings), C.POINTER(abi.Derived)]),
+        "orc_set_serial_dissipation": (C.c_int, [vp, C.c_int]),
+        "orc_upload_state": (C.c_int, [vp, C.c_int64, C.c_int64, _D, _D, _D, _D, _D, _I32]),
+        "orc_set_blocks": (C.c_int, [vp, C.c_int, C.POINTER(abi.Block)]),
+        "orc_set_block_motion": (C.c_int, [vp, C.c_int, C.c_int, _D, _D]),
+        "orc_download_field": (C.c_int, [vp, C.c_int, C.c_int, _D]),
+        "orc_upload_field": (C.c_int, [vp, C.c_int, C.c_int, _D]),
+        "orc_build_neighbours": (C.c_int, [vp, C.c_int]),
+        "orc_neighbour_total": (C.c_int64, [vp]),
+        "orc_get_neighbours": (C.c_int, [vp, _I64, _I64, _D]),
+        "orc_dsph_prestep": (C.c_int, [vp, _D]),
+        "orc_aero_velocity": (C.c_int, [vp]),
+        "orc_detect_surface": (C.c_int, [vp]),
+        "orc_dissipation_terms": (C.c_int, [vp]),
+        "orc_particle_shift": (C.c_int, [vp]),
+        "orc_boundary_update": (C.c_int, [vp]),
+        "orc_acc_rrho": (C.c_int, [vp, C.c_double]),
+        "orc_nb_iter": (C.c_int, [vp, C.c_double, _D]),
+        "orc_find_timestep": (C.c_int, [vp, _D]),
+        "orc_integrate_no_update": (C.c_int, [vp, C.POINTER(abi.StepInfo)]),
+        "orc_integrate_step": (C.c_int, [vp, C.POINTER(abi.StepInfo)]),
+        "orc_advance_frame": (C.c_int, [vp]),
+        "orc_get_time": (C.c_int, [vp, _D, _D, _D]),
+        "orc_set_time": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+        "orc_qr_inverse": (C.c_int, [C.c_int, _D, _D]),
+        "orc_min_eig": (C.c_double, [C.c_int, _D]),
+        "orc_kernel": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+        "orc_gradk_scalar": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _LIBS[kind] = lib
+    return lib
+
+
+def derive(settings):
+    lib = load()
+    d = abi.Derived()
+    lib.orc_derive(C.byref(settings), C.byref(d))
+    return d
+
+
+def qr_inverse(m):
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    dim = m.shape[0]
+    out = np.zeros_like(m)
+    ok = load().orc_qr_inverse(dim, _ptr(m), _ptr(out))
+    return bool(ok), out
+
+
+def min_eig(m):
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    return float(load().orc_min_eig(m.shape[0], _ptr(m)))
+
+
+class Oracle:
+    """One oracle simulation; method names follow the engine wrapper."""
+
+    def __init__(self, case_or_settings, nthreads=None, kind="oracle"):
+        self.lib = load(kind)
+        settings = case_or_settings.settings if hasattr(case_or_settings, "settings") else case_or_settings
+        self.settings = settings.copy()
+        self.dim = int(settings.dim)
+        if nthreads is None:
+            nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        self.h = self.lib.orc_create(C.byref(self.settings), int(nthreads))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+        self.n = 0
+        if hasattr(case_or_settings, "x"):
+            self.upload_case(case_or_settings)
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"oracle {what} failed with {rc}")
+
+    def upload_case(self, case):
+        self.upload_state(case.x, case.v, case.rho, case.p, case.m, case.b, case.n_bound)
+        self.set_blocks(case.blocks)
+
+    def upload_state(self, x, v, rho, p, m, b, n_bound):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        rho = np.ascontiguousarray(rho, np.float64)
+        p = np.ascontiguousarray(p, np.float64)
+        m = np.ascontiguousarray(m, np.float64)
+        b = np.ascontiguousarray(b, np.int32)
+        self.n = x.shape[0]
+        self._chk(self.lib.orc_upload_state(self.h, self.n, int(n_bound), _ptr(x), _ptr(v), _ptr(rho),
+                                            _ptr(p), _ptr(m), _ptr(b, _I32)), "upload")
+
+    def set_blocks(self, specs):
+        arr = abi.make_blocks(specs)
+        self._chk(self.lib.orc_set_blocks(self.h, len(specs), arr), "set_blocks")
+
+    def set_serial_dissipation(self, on):
+        self.lib.orc_set_serial_dissipation(self.h, int(on))
+
+    def field(self, name, which=abi.STATE_NP1):
+        w = abi.field_width(name, self.dim)
+        out = np.zeros(self.n * w)
+        self._chk(self.lib.orc_download_field(self.h, which, abi.F[name], _ptr(out)), f"download {name}")
+        return out.reshape(self.n, w) if w > 1 else out
+
+    def set_field(self, name, values, which=abi.STATE_NP1):
+        a = np.ascontiguousarray(values, np.float64).ravel()
+        self._chk(self.lib.orc_upload_field(self.h, which, abi.F[name], _ptr(a)), f"upload {name}")
+
+    def build_neighbours(self, brute=False):
+        self._chk(self.lib.orc_build_neighbours(self.h, int(brute)), "neighbours")
+
+    def neighbours(self):
+        total = self.lib.orc_neighbour_total(self.h)
+        off = np.zeros(self.n + 1, np.int64)
+        idx = np.zeros(total, np.int64)
+        d2 = np.zeros(total)
+        self._chk(self.lib.orc_get_neighbours(self.h, _ptr(off, _I64), _ptr(idx, _I64), _ptr(d2)), "get_neighbours")
+        return off, idx, d2
+
+    def dsph_prestep(self):
+        v = np.zeros(1)
+        self._chk(self.lib.orc_dsph_prestep(self.h, _ptr(v)), "dsph")
+        return float(v[0])
+
+    def aero_velocity(self):
+        self._chk(self.lib.orc_aero_velocity(self.h), "aero_velocity")
+
+    def detect_surface(self):
+        self._chk(self.lib.orc_detect_surface(self.h), "detect_surface")
+
+    def dissipation_terms(self):
+        self._chk(self.lib.orc_dissipation_terms(self.h), "dissipation")
+
+    def particle_shift(self):
+        self._chk(self.lib.orc_particle_shift(self.h), "shift")
+
+    def boundary_update(self):
+        self._chk(self.lib.orc_boundary_update(self.h), "boundary")
+
+    def acc_rrho(self, npd):
+        self._chk(self.lib.orc_acc_rrho(self.h, float(npd)), "acc_rrho")
+
+    def nb_iter(self, npd):
+        v = np.zeros(1)
+        self._chk(self.lib.orc_nb_iter(self.h, float(npd), _ptr(v)), "nb_iter")
+        return float(v[0])
+
+    def find_timestep(self):
+        v = np.zeros(1)
+        self._chk(self.lib.orc_find_timestep(self.h, _ptr(v)), "find_timestep")
+        return float(v[0])
+
+    def integrate_no_update(self):
+        info = abi.StepInfo()
+        self._chk(self.lib.orc_integrate_no_update(self.h, C.byref(info)), "integrate_no_update")
+        return info.as_dict()
+
+    def integrate_step(self):
+        info = abi.StepInfo()
+        self._chk(self.lib.orc_integrate_step(self.h, C.byref(info)), "integrate")
+        return info.as_dict()
+
+    def advance_frame(self):
+        self.lib.orc_advance_frame(self.h)
+
+    def get_time(self):
+        t, dt, cfl = np.zeros(1), np.zeros(1), np.zeros(1)
+        self.lib.orc_get_time(self.h, _ptr(t), _ptr(dt), _ptr(cfl))
+        return float(t[0]), float(dt[0]), float(cfl[0])
+
+    def set_time(self, t, dt, cfl):
+        self.lib.orc_set_time(self.h, float(t), float(dt), float(cfl))

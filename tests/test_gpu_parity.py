@@ -1,0 +1,180 @@
+"""Engine (HIP, through the C-ABI) versus the FP64 oracle, pass by pass and
+over whole steps.
+
+Tolerances (field-normwise: max_i |engine_i - oracle_i| / max_i |oracle_i|):
+  FP64 path    1e-9 per pass, 1e-8 over a few steps (summation order of the
+               neighbour loops differs from the oracle's index order);
+  mixed path   1e-5 per step (FP32 pair terms, FP64 accumulation), as the
+               target states; 1e-3 drift over the multi-step runs.
+Neighbour sets must match exactly.
+"""
+import numpy as np
+import pytest
+
+from fjsph_amd import abi, cases
+from tests.parity import assert_close, same_neighbour_sets
+
+pytestmark = pytest.mark.gpu
+
+FP64_PASS = 1e-9
+FP64_STEP = 1e-8
+MIXED_STEP = 1e-5
+
+
+def _engine(case):
+    from fjsph_amd.engine import Engine
+    return Engine(case)
+
+
+def _oracle(case):
+    from oracle.pyoracle import Oracle
+    return Oracle(case, nthreads=8)
+
+
+def _perturb(case, seed=5, dv=0.05, drho=0.005):
+    """Give the lattice a random velocity and density field so every pair
+    term is exercised; p follows the Cole EOS."""
+    rng = np.random.default_rng(seed)
+    nb = case.n_bound
+    case.v = case.v.copy()
+    case.v[nb:] += rng.uniform(-dv, dv, size=case.v[nb:].shape)
+    case.rho = case.rho.copy()
+    case.rho[nb:] *= 1.0 + rng.uniform(-drho, drho, size=case.n - nb)
+    s = case.settings
+    B = s.rho_rest * s.speed_sound ** 2 / s.gam
+    case.p = B * ((case.rho / s.rho_rest) ** s.gam - 1.0)
+    return case
+
+
+def _small_cases():
+    dam = _perturb(cases.dam_break_2d(dx=0.05), seed=1)
+    drop = _perturb(cases.droplet_3d(diameter=1.0e-3, dx=5e-5, precision=abi.PREC_FP64), seed=2)
+    jet = _perturb(cases.jet_column_3d(n_target=6000, diameter=0.6e-3, dx=5e-5, precision=abi.PREC_FP64), seed=3)
+    return [("dam2d", dam), ("drop3d", drop), ("jet3d_ale", jet)]
+
+
+@pytest.mark.parametrize("name,case", _small_cases(), ids=lambda v: v if isinstance(v, str) else "")
+def test_pass_parity_fp64(name, case):
+    print(f"[pass parity] {name}: n={case.n} nb={case.n_bound} dim={case.dim}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    nb = case.n_bound
+    fluid = slice(nb, case.n)
+    eng.build_neighbours()
+    orc.build_neighbours()
+    off_e, idx_e = eng.neighbours()
+    off_o, idx_o, _ = orc.neighbours()
+    assert same_neighbour_sets(off_e, idx_e, off_o, idx_o), "neighbour sets differ"
+
+    npd_e, npd_o = eng.dsph_prestep(), orc.dsph_prestep()
+    assert abs(npd_e - npd_o) <= 1e-12 * abs(npd_o)
+    for f in ("L", "GRADRHO", "NORM", "LAM", "LAM_NB", "COLOURG", "KERNSUM", "COLOUR"):
+        assert_close(f, eng.field(f), orc.field(f), FP64_PASS)
+
+    eng.aero_velocity()
+    orc.aero_velocity()
+    assert np.array_equal(eng.field("CELLID"), orc.field("CELLID"))
+    assert_close("CELLV", eng.field("CELLV"), orc.field("CELLV"), 0.0)
+
+    eng.detect_surface()
+    orc.detect_surface()
+    assert np.array_equal(eng.field("SURF")[fluid], orc.field("SURF")[fluid]), "surface flags differ"
+    for f in ("NORM", "CURVE", "WOCCL", "NORM_CURVE"):
+        assert_close(f, eng.field(f)[fluid], orc.field(f)[fluid], FP64_PASS)
+    if case.settings.ale:
+        assert np.array_equal(eng.field("SURFZONE"), orc.field("SURFZONE"))
+
+    eng.dissipation_terms()
+    orc.dissipation_terms()
+    for f in ("AVISC", "DELTAD"):
+        assert_close(f, eng.field(f)[fluid], orc.field(f)[fluid], FP64_PASS)
+
+    if case.settings.ale:
+        eng.particle_shift()
+        orc.particle_shift()
+        assert_close("VPERT", eng.field("VPERT")[fluid], orc.field("VPERT")[fluid], FP64_PASS)
+
+    dt_e, dt_o = eng.find_timestep(), orc.find_timestep()
+    assert abs(dt_e - dt_o) <= 1e-12 * dt_o
+    eng.set_time(0.0, dt_o, case.settings.cfl)
+    orc.set_time(0.0, dt_o, case.settings.cfl)
+
+    if nb:
+        eng.boundary_update()
+        orc.boundary_update()
+        for f in ("P", "RHO"):
+            assert_close(f + "_bound", eng.field(f)[:nb], orc.field(f)[:nb], FP64_PASS)
+
+    eng.acc_rrho(npd_o)
+    orc.acc_rrho(npd_o)
+    for f in ("ACC", "RRHO", "AF"):
+        assert_close(f, eng.field(f)[fluid], orc.field(f)[fluid], FP64_PASS)
+
+    le_e, le_o = eng.nb_iter(npd_o), orc.nb_iter(npd_o)
+    assert abs(le_e - le_o) <= 1e-9 * abs(le_o)
+    for f in ("X", "V", "RHO", "P"):
+        assert_close(f, eng.field(f), orc.field(f), FP64_PASS)
+
+
+@pytest.mark.parametrize("name,case", _small_cases(), ids=lambda v: v if isinstance(v, str) else "")
+def test_step_parity_fp64(name, case):
+    print(f"[step parity] {name}: n={case.n}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    ie, io = eng.integrate_no_update(), orc.integrate_no_update()
+    assert ie["iterations"] == io["iterations"]
+    for step in range(3):
+        ie, io = eng.integrate_step(), orc.integrate_step()
+        assert ie["iterations"] == io["iterations"], f"step {step}: sub-iterations {ie} vs {io}"
+        assert abs(ie["dt"] - io["dt"]) <= 1e-10 * io["dt"]
+        # rms_error = log10 of a difference of nearly equal iterates: its
+        # conditioning is |x| / |dx| ~ 1e11, so compare in log units.
+        assert abs(ie["rms_error"] - io["rms_error"]) <= 1e-3
+    for f in ("X", "V", "RHO", "P", "ACC", "RRHO"):
+        assert_close(f, eng.field(f), orc.field(f), FP64_STEP)
+
+
+def test_step_parity_mixed_droplet():
+    case = _perturb(cases.droplet_3d(diameter=1.0e-3, dx=5e-5, precision=abi.PREC_MIXED), seed=4)
+    print(f"[mixed step] droplet n={case.n}", flush=True)
+    eng = _engine(case)
+    orc = _oracle(case)
+    eng.integrate_no_update()
+    orc.integrate_no_update()
+    eng.integrate_step()
+    orc.integrate_step()
+    rho0 = case.settings.rho_rest
+    pairs = {
+        "X-x0": (eng.field("X") - case.x, orc.field("X") - case.x),
+        "V": (eng.field("V"), orc.field("V")),
+        "RHO-rho0": (eng.field("RHO") - rho0, orc.field("RHO") - rho0),
+        "P": (eng.field("P"), orc.field("P")),
+        "ACC": (eng.field("ACC"), orc.field("ACC")),
+        "RRHO": (eng.field("RRHO"), orc.field("RRHO")),
+    }
+    errs = {k: assert_close(k, g, r, MIXED_STEP) for k, (g, r) in pairs.items()}
+    print("mixed per-step errors:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+
+
+def test_neighbours_full_droplet():
+    """Bit-exact sorted neighbour sets on the 268k-particle droplet IC."""
+    case = cases.droplet_3d()
+    print(f"[neighbours] droplet n={case.n}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    eng.build_neighbours()
+    orc.build_neighbours()
+    off_e, idx_e = eng.neighbours()
+    off_o, idx_o, _ = orc.neighbours()
+    assert same_neighbour_sets(off_e, idx_e, off_o, idx_o)
+
+
+def test_dam_break_drift_fp64():
+    """2D dam break (dx = 0.04): 40 steps, FP64 engine vs FP64 oracle."""
+    case = cases.dam_break_2d(dx=0.04)
+    eng, orc = _engine(case), _oracle(case)
+    eng.integrate_no_update()
+    orc.integrate_no_update()
+    for _ in range(40):
+        ie, io = eng.integrate_step(), orc.integrate_step()
+        assert ie["iterations"] == io["iterations"]
+    for f in ("X", "V", "RHO", "P"):
+        e = assert_close(f, eng.field(f), orc.field(f), 1e-7)
+        print(f"drift {f}: {e:.2e}", flush=True)
