@@ -128,6 +128,9 @@ struct fjsph_ctx {
     int32_t dims[3] = {1, 1, 1};
 
     NbrList nl{};
+    Aux aux{};
+    double* xs = nullptr;      /* positions in cell order for the list build */
+    bool x_at_build = false;   /* pnp1.x unchanged since the last build     */
     DevScalars* sc = nullptr;
     DevScalars* sc_host = nullptr; /* pinned */
     DevBlocks bl{};
@@ -224,7 +227,11 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->partial); dfree(ctx->gather_tmp); dfree(ctx->gather_tmp_i);
     dfree(ctx->cell); dfree(ctx->cell_count); dfree(ctx->cell_start); dfree(ctx->cell_fill);
     dfree(ctx->order); dfree(ctx->tile_sums);
-    dfree(ctx->nl.idx); dfree(ctx->nl.rr); dfree(ctx->nl.count);
+    dfree(ctx->nl.idx); dfree(ctx->nl.count);
+    if (ctx->nl.r) (void)hipFree(ctx->nl.r);
+    ctx->nl.r = nullptr;
+    dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
+    dfree(ctx->aux.nhat); dfree(ctx->xs);
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
     ctx->ncell_cap = 0;
     ctx->nl.cap = 0;
@@ -387,10 +394,11 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         permute_all(ctx, ctx->order);
         launch_iota(ctx->order, n, ctx->st);
     }
+    launch_gather_x(D, ctx->pnp1.x, ctx->order, n, ctx->xs, ctx->st);
     for (int attempt = 0; attempt < 2; ++attempt) {
         HIPCHK(hipMemsetAsync(&ctx->sc->max_count, 0, sizeof(int32_t), ctx->st));
-        launch_build_lists(D, ctx->pnp1.x, n, ctx->cell, ctx->dims, ctx->cell_start, ctx->order, ctx->d.sr, ctx->nl,
-                           ctx->sc, ctx->st);
+        launch_build_lists(D, ctx->pnp1.x, ctx->xs, n, ctx->cell, ctx->dims, ctx->cell_start, ctx->order, ctx->d.sr,
+                           ctx->nl, ctx->sc, ctx->st);
         HIPCHK(hipMemcpyAsync(&ctx->sc_host->max_count, &ctx->sc->max_count, sizeof(int32_t), hipMemcpyDeviceToHost,
                               ctx->st));
         sync(ctx);
@@ -399,10 +407,12 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         if (mc <= ctx->nl.cap) break;
         const int32_t cap = ((mc + mc / 8 + 8) + 7) / 8 * 8;
         dalloc(ctx->nl.idx, size_t(cap) * ctx->nl.npad);
-        dalloc(ctx->nl.rr, size_t(cap) * ctx->nl.npad);
+        if (ctx->nl.f32) dalloc(*(float**)&ctx->nl.r, size_t(cap) * ctx->nl.npad);
+        else dalloc(*(double**)&ctx->nl.r, size_t(cap) * ctx->nl.npad);
         ctx->nl.cap = cap;
     }
     ctx->have_nbrs = true;
+    ctx->x_at_build = true;
 }
 
 /* ------------------------------------------------------------ the passes */
@@ -413,28 +423,34 @@ void require(fjsph_ctx* ctx, bool nbrs)
     if (nbrs && !ctx->have_nbrs) throw std::string("neighbour lists not built");
 }
 
-double dsph(fjsph_ctx* ctx, bool with_aero)
+void refresh_aux(fjsph_ctx* ctx)
+{
+    launch_aux(ctx->s.dim, ctx->pnp1, ctx->aux, ctx->n, ctx->s.ale, ctx->s.precision == FJSPH_PREC_MIXED, ctx->st);
+}
+
+int r_from_x(const fjsph_ctx* ctx) { return ctx->x_at_build ? 1 : 0; }
+
+void dsph(fjsph_ctx* ctx, bool with_aero)
 {
     PassTimer t(ctx, FJSPH_T_DSPH);
-    launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, with_aero ? 1 : 0, ctx->partial,
-                ctx->nparts, ctx->sc, ctx->st);
-    return 0.0;
+    refresh_aux(ctx);
+    launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n, r_from_x(ctx),
+                with_aero ? 1 : 0, ctx->partial, ctx->nparts, ctx->sc, ctx->st);
 }
 
 void surface_and_dissipation(fjsph_ctx* ctx, bool with_diss)
 {
-    {
-        PassTimer t(ctx, FJSPH_T_SURFACE);
-        launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n,
-                        ctx->norms_tmp, with_diss ? 1 : 0, ctx->st);
-        launch_surface2(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->norms_tmp, ctx->st);
-    }
+    PassTimer t(ctx, FJSPH_T_SURFACE);
+    launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n,
+                    r_from_x(ctx), ctx->norms_tmp, with_diss ? 1 : 0, ctx->st);
+    launch_surface2(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n, r_from_x(ctx),
+                    ctx->norms_tmp, ctx->st);
 }
 
 void shift(fjsph_ctx* ctx)
 {
     PassTimer t(ctx, FJSPH_T_SHIFT);
-    launch_shift(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->st);
+    launch_shift(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n, r_from_x(ctx), ctx->st);
 }
 
 /* dSPH_PreStep, get_aero_velocity, Detect_Surface, dissipation_terms,
@@ -472,27 +488,33 @@ void boundary_update(fjsph_ctx* ctx)
     if (ctx->nb == 0) return;
     PassTimer t(ctx, FJSPH_T_BOUNDARY);
     upload_block_velocities(ctx, false);
+    refresh_aux(ctx);
     for (size_t k = 0; k < ctx->blocks.size(); ++k) {
         if (ctx->blocks[k].b.is_fluid) continue;
-        launch_boundary(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->blk, ctx->bl, int32_t(k), ctx->nl, ctx->n,
+        launch_boundary(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->blk, ctx->bl, int32_t(k), ctx->nl, ctx->n,
                         ctx->near_inlet, ctx->st);
     }
+}
+
+void force(fjsph_ctx* ctx, bool update)
+{
+    PassTimer t(ctx, FJSPH_T_FORCE_UPDATE);
+    refresh_aux(ctx);
+    launch_force(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->pn, update ? ctx->x_nxt : nullptr,
+                 ctx->v_nxt, ctx->rho_nxt, ctx->p_nxt, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->nl,
+                 ctx->n, ctx->delta_t, ctx->sc, ctx->partial, ctx->nparts, ctx->st);
 }
 
 /* Do_NB_Iter + the residual of Check_Error; returns log10 rms. */
 double nb_iter(fjsph_ctx* ctx)
 {
     boundary_update(ctx);
-    {
-        PassTimer t(ctx, FJSPH_T_FORCE_UPDATE);
-        launch_force(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->pn, nullptr, ctx->x_nxt, ctx->v_nxt,
-                     ctx->rho_nxt, ctx->p_nxt, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->nl, ctx->n,
-                     ctx->delta_t, ctx->sc, ctx->partial, ctx->nparts, ctx->st);
-    }
+    force(ctx, true);
     std::swap(ctx->pnp1.x, ctx->x_nxt);
     std::swap(ctx->pnp1.v, ctx->v_nxt);
     std::swap(ctx->pnp1.rho, ctx->rho_nxt);
     std::swap(ctx->pnp1.p, ctx->p_nxt);
+    ctx->x_at_build = false;
     {
         PassTimer t(ctx, FJSPH_T_REDUCE);
         launch_sum_partials(ctx->partial, ctx->nparts, 1, &ctx->sc->residual, ctx->st);
@@ -762,6 +784,15 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
         dalloc(ctx->gather_tmp_i, size_t(n_total));
         dalloc(ctx->cell, size_t(n_total));
         dalloc(ctx->order, size_t(n_total));
+        dalloc(ctx->xs, size_t(n_total) * D);
+        dalloc(ctx->aux.vol, size_t(n_total));
+        dalloc(ctx->aux.P, size_t(n_total));
+        dalloc(ctx->aux.invrho, size_t(n_total));
+        dalloc(ctx->aux.fq, size_t(n_total));
+        dalloc(ctx->aux.fale, size_t(n_total));
+        dalloc(ctx->aux.nhat, size_t(n_total) * D);
+        HIPCHK(hipMemsetAsync(ctx->aux.nhat, 0, sizeof(double) * n_total * D, ctx->st));
+        ctx->nl.f32 = ctx->s.precision == FJSPH_PREC_MIXED ? 1 : 0;
         ctx->nl.npad = (n_total + 63) / 64 * 64;
         dalloc(ctx->nl.count, size_t(ctx->nl.npad));
         ctx->nl.cap = 0;
@@ -911,6 +942,7 @@ fjsph_status fjsph_upload_field(fjsph_ctx* ctx, int32_t which, int32_t field, co
     return guarded(ctx, [&] {
         require(ctx, false);
         DevState& st = which == FJSPH_STATE_N ? ctx->pn : ctx->pnp1;
+        if (which == FJSPH_STATE_NP1 && field == FJSPH_F_X) ctx->x_at_build = false;
         const int w = width_of(ctx, field);
         const size_t cnt = size_t(ctx->n) * w;
         std::vector<int32_t>& sh = slot_to_host(ctx);
@@ -1002,6 +1034,7 @@ fjsph_status fjsph_detect_surface(fjsph_ctx* ctx)
 {
     return guarded(ctx, [&] {
         require(ctx, true);
+        refresh_aux(ctx);
         surface_and_dissipation(ctx, false);
         sync(ctx);
         drain_timers(ctx);
@@ -1013,7 +1046,9 @@ fjsph_status fjsph_dissipation_terms(fjsph_ctx* ctx)
     return guarded(ctx, [&] {
         require(ctx, true);
         PassTimer t(ctx, FJSPH_T_DISSIPATION);
-        launch_dissipation(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->st);
+        refresh_aux(ctx);
+        launch_dissipation(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n,
+                           r_from_x(ctx), ctx->st);
         sync(ctx);
     });
 }
@@ -1023,6 +1058,7 @@ fjsph_status fjsph_particle_shift(fjsph_ctx* ctx)
     return guarded(ctx, [&] {
         require(ctx, true);
         if (!ctx->s.ale) throw std::string("particle_shift needs settings.ale = 1");
+        refresh_aux(ctx);
         shift(ctx);
         sync(ctx);
         drain_timers(ctx);
@@ -1044,10 +1080,9 @@ fjsph_status fjsph_acc_rrho(fjsph_ctx* ctx, double npd)
     return guarded(ctx, [&] {
         require(ctx, true);
         HIPCHK(hipMemcpyAsync(&ctx->sc->npd, &npd, sizeof(double), hipMemcpyHostToDevice, ctx->st));
-        launch_force(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->pn, nullptr, nullptr, nullptr, nullptr,
-                     nullptr, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->nl, ctx->n, ctx->delta_t, ctx->sc,
-                     ctx->partial, ctx->nparts, ctx->st);
+        force(ctx, false);
         sync(ctx);
+        drain_timers(ctx);
     });
 }
 

@@ -38,12 +38,27 @@ struct DevState {
     int32_t *surf, *cellID, *surfzone, *b;
 };
 
+/* Neighbour lists.  The payload is the distance frozen at build time
+ * (src/Resid.cpp:288-289 reads it from the list): FP64 d^2 on the parity
+ * path, FP32 r on the mixed path. */
 struct NbrList {
     int32_t* idx;
-    double* rr;
+    void* r;
     int32_t* count;
     int64_t npad;
     int32_t cap;
+    int32_t f32; /* payload is float r (1) or double d^2 (0) */
+};
+
+/* Per-particle quantities derived from the current rho, p, m, b (and vPert),
+ * refreshed by launch_aux before every pass that reads them. */
+struct Aux {
+    double* vol;     /* m / rho            */
+    double* P;       /* p / rho^2          */
+    double* invrho;  /* 1 / rho            */
+    float4* fq;      /* {m (negative for BOUND), p/rho^2, 1/rho, m/rho} in FP32 */
+    float4* fale;    /* {vPert, rho} in FP32 (ALE)                          */
+    double* nhat;    /* normalised surface normal (after Detect_Surface)     */
 };
 
 /* Per-block boundary settings visible to kernels. */
@@ -73,35 +88,39 @@ void launch_scan(const int32_t* count, int64_t ncell, int32_t* start, int32_t* b
 void launch_scatter(const int32_t* cell, int64_t n, const int32_t* start, int32_t* fill, int32_t* order,
                     hipStream_t st);
 void launch_sort_cells(const int32_t* start, int64_t ncell, int32_t* order, hipStream_t st);
-void launch_build_lists(int dim, const double* x, int64_t n, const int32_t* cell, const int32_t* dims,
-                        const int32_t* start, const int32_t* order, double sr, NbrList nl, DevScalars* sc,
-                        hipStream_t st);
+void launch_gather_x(int dim, const double* x, const int32_t* order, int64_t n, double* xs, hipStream_t st);
+void launch_build_lists(int dim, const double* x, const double* xs, int64_t n, const int32_t* cell,
+                        const int32_t* dims, const int32_t* start, const int32_t* order, double sr, NbrList nl,
+                        DevScalars* sc, hipStream_t st);
+void launch_aux(int dim, DevState s, Aux a, int64_t n, int ale, int mixed, hipStream_t st);
 void launch_gather_d(const double* src, double* dst, const int32_t* perm, int64_t n, int width, hipStream_t st);
 void launch_gather_i(const int32_t* src, int32_t* dst, const int32_t* perm, int64_t n, hipStream_t st);
 void launch_iota(int32_t* a, int64_t n, hipStream_t st);
 
-/* dSPH pass; npd (sum / n) lands in sc->npd. */
-void launch_dsph(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
-                 int with_aero_vel, double* partial, int nparts, DevScalars* sc, hipStream_t st);
+/* Pre-passes.  r_from_x = 1 when the positions are those of the last
+ * neighbour build (then r is recomputed bit-identically from x instead of
+ * being read from the list). */
+void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl, int64_t n,
+                 int r_from_x, int with_aero_vel, double* partial, int nparts, DevScalars* sc, hipStream_t st);
 void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
                           hipStream_t st);
-void launch_surface1(int dim, int prec, const Consts& c, DevState s, const int32_t* part_id, NbrList nl,
-                     int64_t n, double* norms_tmp, int with_dissipation, hipStream_t st);
-void launch_dissipation(int dim, int prec, const Consts& c, DevState s, const int32_t* part_id, NbrList nl,
-                        int64_t n, hipStream_t st);
-void launch_surface2(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
-                     const double* norms_tmp, hipStream_t st);
-void launch_shift(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
-                  hipStream_t st);
-void launch_boundary(int dim, const Consts& c, DevState s, const int32_t* part_id, const int32_t* blk,
-                     DevBlocks bl, int32_t block, NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st);
-/* Force pass on `s` (x/v/rho/p read from s); when `next` is non-null the
- * Newmark update of Do_NB_Iter is fused and writes next.x/v/rho/p for every
+void launch_surface1(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl,
+                     int64_t n, int r_from_x, double* norms_tmp, int with_dissipation, hipStream_t st);
+void launch_dissipation(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl,
+                        int64_t n, int r_from_x, hipStream_t st);
+void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl, int64_t n,
+                     int r_from_x, const double* norms_tmp, hipStream_t st);
+void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl, int64_t n,
+                  int r_from_x, hipStream_t st);
+void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t* blk, DevBlocks bl, int32_t block,
+                     NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st);
+/* Force pass on `s` (x/v/rho/p read from s); when next_x is non-null the
+ * Newmark update of Do_NB_Iter is fused and writes next x/v/rho/p for every
  * slot, plus per-block residual partial sums. */
-void launch_force(int dim, int prec, const Consts& c, DevState s, DevState pn, const double* next_x,
-                  double* next_x_w, double* next_v, double* next_rho, double* next_p, const int32_t* part_id,
-                  const int32_t* blk, DevBlocks bl, const int32_t* near_inlet, NbrList nl, int64_t n,
-                  double dt, const DevScalars* sc, double* partial, int nparts, hipStream_t st);
+void launch_force(int dim, int prec, const Consts& c, DevState s, Aux a, DevState pn, double* next_x,
+                  double* next_v, double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk,
+                  DevBlocks bl, const int32_t* near_inlet, NbrList nl, int64_t n, double dt, const DevScalars* sc,
+                  double* partial, int nparts, hipStream_t st);
 void launch_sum_partials(const double* partial, int nparts, int width, double* out, hipStream_t st);
 void launch_timestep_reduce(int dim, const Consts& c, DevState s, const int32_t* part_id, int64_t n,
                             double* partial, int nparts, double* out, hipStream_t st);

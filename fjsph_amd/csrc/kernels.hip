@@ -507,35 +507,72 @@ void launch_scatter(const int32_t* cell, int64_t n, const int32_t* start, int32_
     hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(TPB), 0, st, cell, n, start, fill, order);
 }
 
-/* Sort each cell's slot list ascending so list order is deterministic. */
-__global__ __launch_bounds__(TPB) void k_sort_cells(const int32_t* __restrict__ start, int64_t ncell, int32_t* order)
+/* Sort each cell's slot list ascending so list order is deterministic: one
+ * workgroup per cell, segment staged in LDS, each lane writes its elements
+ * at their rank.  Segments longer than the LDS tile fall back to an
+ * insertion sort in one lane. */
+constexpr int SORT_TILE = 1024;
+
+__global__ __launch_bounds__(64) void k_sort_cells(const int32_t* __restrict__ start, int64_t ncell, int32_t* order)
 {
-    const int64_t c = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    __shared__ int32_t seg[SORT_TILE];
+    const int64_t c = blockIdx.x;
     if (c >= ncell) return;
     const int32_t a = start[c], b = start[c + 1];
-    for (int32_t i = a + 1; i < b; ++i) {
-        const int32_t v = order[i];
-        int32_t j = i - 1;
-        while (j >= a && order[j] > v) {
-            order[j + 1] = order[j];
-            --j;
+    const int32_t k = b - a;
+    if (k <= 1) return;
+    if (k > SORT_TILE) {
+        if (threadIdx.x == 0) {
+            for (int32_t i = a + 1; i < b; ++i) {
+                const int32_t v = order[i];
+                int32_t j = i - 1;
+                while (j >= a && order[j] > v) { order[j + 1] = order[j]; --j; }
+                order[j + 1] = v;
+            }
         }
-        order[j + 1] = v;
+        return;
+    }
+    for (int t = threadIdx.x; t < k; t += 64) seg[t] = order[a + t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < k; t += 64) {
+        const int32_t v = seg[t];
+        int rank = 0;
+        for (int u = 0; u < k; ++u) rank += seg[u] < v;
+        order[a + rank] = v;
     }
 }
 
 void launch_sort_cells(const int32_t* start, int64_t ncell, int32_t* order, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sort_cells, dim3(grid_of(ncell)), dim3(TPB), 0, st, start, ncell, order);
+    hipLaunchKernelGGL(k_sort_cells, dim3((unsigned)ncell), dim3(64), 0, st, start, ncell, order);
+}
+
+/* Positions in cell order (xs[s] = x[order[s]]) so the list build streams
+ * candidate strips contiguously. */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_gather_x(const double* __restrict__ x, const int32_t* __restrict__ order,
+                                                  int64_t n, double* xs)
+{
+    const int64_t s = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (s >= n) return;
+    const int64_t j = order[s];
+#pragma unroll
+    for (int k = 0; k < D; ++k) xs[s * D + k] = x[j * D + k];
+}
+
+void launch_gather_x(int dim, const double* x, const int32_t* order, int64_t n, double* xs, hipStream_t st)
+{
+    if (dim == 2) hipLaunchKernelGGL(k_gather_x<2>, dim3(grid_of(n)), dim3(TPB), 0, st, x, order, n, xs);
+    else hipLaunchKernelGGL(k_gather_x<3>, dim3(grid_of(n)), dim3(TPB), 0, st, x, order, n, xs);
 }
 
 /* Neighbour lists: for each slot, the 3^(D-1) x-strips of the stencil
- * (3 x-adjacent cells are contiguous in `order`), exact FP64 predicate
+ * (3 x-adjacent cells are contiguous in cell order), exact FP64 predicate
  * d2 < sr, self excluded.  Counts beyond the capacity are still counted so
  * the host can grow the lists and rebuild. */
-template <int D>
-__global__ __launch_bounds__(TPB) void k_build_lists(const double* __restrict__ x, int64_t n,
-                                                     const int32_t* __restrict__ cell, Grid3 g,
+template <int D, typename RT>
+__global__ __launch_bounds__(TPB) void k_build_lists(const double* __restrict__ x, const double* __restrict__ xs,
+                                                     int64_t n, const int32_t* __restrict__ cell, Grid3 g,
                                                      const int32_t* __restrict__ start,
                                                      const int32_t* __restrict__ order, double sr, NbrList nl,
                                                      DevScalars* sc)
@@ -549,6 +586,7 @@ __global__ __launch_bounds__(TPB) void k_build_lists(const double* __restrict__ 
     const int32_t cy = (lin / g.dims[0]) % g.dims[1];
     const int32_t cz = (D == 3) ? lin / (g.dims[0] * g.dims[1]) : 0;
     const int32_t x0 = max(cx - 1, 0), x1 = min(cx + 1, g.dims[0] - 1);
+    RT* pay = (RT*)nl.r;
     int cnt = 0;
     const int zlo = (D == 3) ? max(cz - 1, 0) : 0, zhi = (D == 3) ? min(cz + 1, g.dims[2] - 1) : 0;
     for (int z = zlo; z <= zhi; ++z) {
@@ -556,15 +594,15 @@ __global__ __launch_bounds__(TPB) void k_build_lists(const double* __restrict__ 
             const int32_t row = (D == 3) ? (z * g.dims[1] + y) * g.dims[0] : y * g.dims[0];
             const int32_t a = start[row + x0], b = start[row + x1 + 1];
             for (int32_t s = a; s < b; ++s) {
-                const int32_t j = order[s];
-                if (j == (int32_t)i) continue;
                 double xj[D];
-                load_vec<D>(x, j, xj);
+                load_vec<D>(xs, s, xj);
                 const double d2 = exact_dist2<D>(xi, xj);
                 if (d2 < sr) {
+                    const int32_t j = order[s];
+                    if (j == (int32_t)i) continue;
                     if (cnt < nl.cap) {
                         nl.idx[(int64_t)cnt * nl.npad + i] = j;
-                        nl.rr[(int64_t)cnt * nl.npad + i] = d2;
+                        pay[(int64_t)cnt * nl.npad + i] = (sizeof(RT) == 4) ? RT(sqrt(d2)) : RT(d2);
                     }
                     ++cnt;
                 }
@@ -575,17 +613,50 @@ __global__ __launch_bounds__(TPB) void k_build_lists(const double* __restrict__ 
     atomicMax(&sc->max_count, cnt);
 }
 
-void launch_build_lists(int dim, const double* x, int64_t n, const int32_t* cell, const int32_t* dims,
-                        const int32_t* start, const int32_t* order, double sr, NbrList nl, DevScalars* sc,
-                        hipStream_t st)
+void launch_build_lists(int dim, const double* x, const double* xs, int64_t n, const int32_t* cell,
+                        const int32_t* dims, const int32_t* start, const int32_t* order, double sr, NbrList nl,
+                        DevScalars* sc, hipStream_t st)
 {
     Grid3 g;
     for (int k = 0; k < 3; ++k) { g.lo[k] = 0.0; g.dims[k] = dims[k]; }
     g.inv_cs = 0.0;
-    if (dim == 2)
-        hipLaunchKernelGGL(k_build_lists<2>, dim3(grid_of(n)), dim3(TPB), 0, st, x, n, cell, g, start, order, sr, nl, sc);
-    else
-        hipLaunchKernelGGL(k_build_lists<3>, dim3(grid_of(n)), dim3(TPB), 0, st, x, n, cell, g, start, order, sr, nl, sc);
+    const dim3 gr(grid_of(n)), bl(TPB);
+    if (dim == 2) {
+        if (nl.f32) hipLaunchKernelGGL((k_build_lists<2, float>), gr, bl, 0, st, x, xs, n, cell, g, start, order, sr, nl, sc);
+        else hipLaunchKernelGGL((k_build_lists<2, double>), gr, bl, 0, st, x, xs, n, cell, g, start, order, sr, nl, sc);
+    } else {
+        if (nl.f32) hipLaunchKernelGGL((k_build_lists<3, float>), gr, bl, 0, st, x, xs, n, cell, g, start, order, sr, nl, sc);
+        else hipLaunchKernelGGL((k_build_lists<3, double>), gr, bl, 0, st, x, xs, n, cell, g, start, order, sr, nl, sc);
+    }
+}
+
+/* Per-particle derived quantities (see Aux). */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_aux(DevState s, Aux a, int64_t n, int ale, int mixed)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n) return;
+    const double rho = s.rho[i], m = s.m[i], p = s.p[i];
+    const double vol = m / rho, P = p / (rho * rho), ir = 1.0 / rho;
+    a.vol[i] = vol;
+    a.P[i] = P;
+    a.invrho[i] = ir;
+    if (mixed) {
+        const float ms = s.b[i] == FJSPH_BOUND ? -float(m) : float(m);
+        a.fq[i] = make_float4(ms, float(P), float(ir), float(vol));
+        if (ale) {
+            float vp[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < D; ++k) vp[k] = float(s.vPert[i * D + k]);
+            a.fale[i] = make_float4(vp[0], vp[1], vp[2], float(rho));
+        }
+    }
+}
+
+void launch_aux(int dim, DevState s, Aux a, int64_t n, int ale, int mixed, hipStream_t st)
+{
+    if (dim == 2) hipLaunchKernelGGL(k_aux<2>, dim3(grid_of(n)), dim3(TPB), 0, st, s, a, n, ale, mixed);
+    else hipLaunchKernelGGL(k_aux<3>, dim3(grid_of(n)), dim3(TPB), 0, st, s, a, n, ale, mixed);
 }
 
 __global__ __launch_bounds__(TPB) void k_gather_d(const double* __restrict__ src, double* __restrict__ dst,
@@ -627,6 +698,33 @@ void launch_gather_i(const int32_t* src, int32_t* dst, const int32_t* perm, int6
     hipLaunchKernelGGL(k_gather_i, dim3(grid_of(n)), dim3(TPB), 0, st, src, dst, perm, n);
 }
 
+/* ------------------------------------------------------------ pair helpers */
+
+/* The distance of pair k of slot i as the reference uses it: from the d^2
+ * frozen at the last neighbour build (src/Resid.cpp:288-289).  When the
+ * positions are still those of the build (r_from_x) it is recomputed from x
+ * with the build's own arithmetic, which gives the same bits. */
+template <int D>
+__device__ __forceinline__ void pair_dist(int r_from_x, const double* xi, const double* xj, const NbrList& nl,
+                                          int k, int64_t i, double& d2, double& r)
+{
+    if (r_from_x) {
+        d2 = exact_dist2<D>(xi, xj);
+        r = sqrt(d2);
+    } else if (nl.f32) {
+        r = double(((const float*)nl.r)[(int64_t)k * nl.npad + i]);
+        d2 = r * r;
+    } else {
+        d2 = ((const double*)nl.r)[(int64_t)k * nl.npad + i];
+        r = sqrt(d2);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T fast_rcp(T x) { return T(1) / x; }
+template <>
+__device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 /* ------------------------------------------------------------ dSPH_PreStep */
 
 template <int D>
@@ -641,8 +739,8 @@ __device__ __forceinline__ void aero_velocity_one(const Consts& c, DevState& s, 
 }
 
 template <int D>
-__global__ __launch_bounds__(TPB) void k_dsph(Consts c, DevState s, const int32_t* __restrict__ part_id, NbrList nl,
-                                              int64_t n, int with_aero, double* partial)
+__global__ __launch_bounds__(TPB) void k_dsph(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                              NbrList nl, int64_t n, int r_from_x, int with_aero, double* partial)
 {
     __shared__ double lds[NWAVE];
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
@@ -664,11 +762,13 @@ __global__ __launch_bounds__(TPB) void k_dsph(Consts c, DevState s, const int32_
         const int cnt = nl.count[i];
         for (int k = 0; k < cnt; ++k) {
             const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
-            const double r = sqrt(nl.rr[(int64_t)k * nl.npad + i]);
+            double xj[D], d2, r;
+            load_vec<D>(s.x, j, xj);
+            pair_dist<D>(r_from_x, xi, xj, nl, k, i, d2, r);
             double Rji[D];
 #pragma unroll
-            for (int a = 0; a < D; ++a) Rji[a] = s.x[(int64_t)j * D + a] - xi[a];
-            const double volj = s.m[j] / s.rho[j];
+            for (int a = 0; a < D; ++a) Rji[a] = xj[a] - xi[a];
+            const double volj = ax.vol[j];
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
             double G[D];
@@ -757,13 +857,13 @@ void launch_sum_partials(const double* partial, int nparts, int width, double* o
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(TPB), 0, st, partial, nparts, width, 1.0, out);
 }
 
-void launch_dsph(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
-                 int with_aero_vel, double* partial, int nparts, DevScalars* sc, hipStream_t st)
+void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl, int64_t n,
+                 int r_from_x, int with_aero_vel, double* partial, int nparts, DevScalars* sc, hipStream_t st)
 {
     if (dim == 2)
-        hipLaunchKernelGGL(k_dsph<2>, dim3(nparts), dim3(TPB), 0, st, c, s, part_id, nl, n, with_aero_vel, partial);
+        hipLaunchKernelGGL(k_dsph<2>, dim3(nparts), dim3(TPB), 0, st, c, s, a, part_id, nl, n, r_from_x, with_aero_vel, partial);
     else
-        hipLaunchKernelGGL(k_dsph<3>, dim3(nparts), dim3(TPB), 0, st, c, s, part_id, nl, n, with_aero_vel, partial);
+        hipLaunchKernelGGL(k_dsph<3>, dim3(nparts), dim3(TPB), 0, st, c, s, a, part_id, nl, n, r_from_x, with_aero_vel, partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(TPB), 0, st, partial, nparts, 1, 1.0 / double(n), &sc->npd);
 }
 
@@ -786,43 +886,37 @@ void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* p
 
 /* -------------------------------------- Detect_Surface + dissipation_terms */
 
+/* One dissipation_terms pair term (src/Shifting.cpp:141-173) in T. */
 template <int D, typename T>
-__device__ __forceinline__ void dissipation_one(const Consts& c, DevState& s, int64_t i, const NbrList& nl)
-{
-    double xi[D], vi[D], gri[D];
-    load_vec<D>(s.x, i, xi);
-    load_vec<D>(s.v, i, vi);
-    load_vec<D>(s.gradRho, i, gri);
-    const double rhoi = s.rho[i];
-    const T inv_H = T(1.0 / c.H), H = T(c.H), Wc = T(c.Wc);
-    const T eps2 = T(0.0001 * c.H_sq);
-    const T cfac = T(sqrt(c.B * c.gam / rhoi));
-    const T Bg = T(c.B * c.gam);
+struct DissAcc {
     double av[D];
+    double rd;
+    T Bg, cfac, H, eps2, alpha;
+    __device__ void init(const Consts& c, double rhoi)
+    {
 #pragma unroll
-    for (int a = 0; a < D; ++a) av[a] = 0.0;
-    double rd = 0.0;
-    const int cnt = nl.count[i];
-    for (int k = 0; k < cnt; ++k) {
-        const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
-        const double rrd = nl.rr[(int64_t)k * nl.npad + i];
-        T Rji[D], Vji[D];
+        for (int a = 0; a < D; ++a) av[a] = 0.0;
+        rd = 0.0;
+        Bg = T(c.B * c.gam);
+        cfac = T(sqrt(c.B * c.gam / rhoi));
+        H = T(c.H);
+        eps2 = T(0.0001 * c.H_sq);
+        alpha = T(c.visc_alpha);
+    }
+    __device__ void add(const DevState& s, const Aux& ax, int32_t j, const double* xi, const double* xj,
+                        const double* vi, const double* gri, double rhoi, double d2, T r, T fg)
+    {
+        T Rji[D], Vji[D], g[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            Rji[a] = T(s.x[(int64_t)j * D + a] - xi[a]);
+            Rji[a] = T(xj[a] - xi[a]);
             Vji[a] = T(s.v[(int64_t)j * D + a] - vi[a]);
+            g[a] = fg * Rji[a];
         }
-        const T rr = T(rrd);
-        const T r = T(sqrt(rrd));
-        const T idist2 = T(1) / (rr + eps2);
+        (void)r;
+        const T idist2 = fast_rcp<T>(T(d2) + eps2);
         const double rhoj = s.rho[j];
-        const T mj = T(s.m[j]);
-        const T volj = T(s.m[j] / rhoj);
-        T W, fg;
-        wendland<T>(r, H, inv_H, Wc, W, fg);
-        T g[D];
-#pragma unroll
-        for (int a = 0; a < D; ++a) g[a] = fg * Rji[a];
+        const T volj = T(ax.vol[j]);
         T rdotg = Rji[0] * g[0];
 #pragma unroll
         for (int a = 1; a < D; ++a) rdotg += Rji[a] * g[a];
@@ -833,9 +927,8 @@ __device__ __forceinline__ void dissipation_one(const Consts& c, DevState& s, in
             for (int a = 1; a < D; ++a) vdotr += Vji[a] * Rji[a];
             if (!(vdotr > T(0))) {
                 const T muij = H * vdotr * idist2;
-                const T rhoij = T(0.5 * (rhoi + rhoj));
-                const T cbar = T(0.5) * (cfac + sqrt(Bg / T(rhoj)));
-                const T f = mj * T(c.visc_alpha) * cbar * muij / rhoij;
+                const T cbar = T(0.5) * (cfac + sqrt(Bg * T(ax.invrho[j])));
+                const T f = T(s.m[j]) * alpha * cbar * muij * fast_rcp<T>(T(0.5 * (rhoi + rhoj)));
 #pragma unroll
                 for (int a = 0; a < D; ++a) av[a] += double(g[a] * f);
             }
@@ -847,14 +940,18 @@ __device__ __forceinline__ void dissipation_one(const Consts& c, DevState& s, in
             rd += double(volj * drho * rdotg * idist2);
         }
     }
+    __device__ void store(const Consts& c, DevState& s, int64_t i)
+    {
 #pragma unroll
-    for (int a = 0; a < D; ++a) s.aVisc[i * D + a] = av[a];
-    s.deltaD[i] = c.dsph_cont * rd;
-}
+        for (int a = 0; a < D; ++a) s.aVisc[i * D + a] = av[a];
+        s.deltaD[i] = c.dsph_cont * rd;
+    }
+};
 
 template <int D, typename T>
-__global__ __launch_bounds__(TPB) void k_surface1(Consts c, DevState s, const int32_t* __restrict__ part_id,
-                                                  NbrList nl, int64_t n, double* norms_tmp, int with_diss)
+__global__ __launch_bounds__(TPB) void k_surface1(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                  NbrList nl, int64_t n, int r_from_x, double* norms_tmp,
+                                                  int with_diss)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n) return;
@@ -879,6 +976,7 @@ __global__ __launch_bounds__(TPB) void k_surface1(Consts c, DevState s, const in
     } else if (lam_nb < 0.2) {
         s.surf[i] = 1;
     } else if (lam_nb < 0.75) {
+        /* Marrone et al. (2010) point-T test, src/Geometry.cpp:43-98 */
         double pT[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) pT[a] = xi[a] + h * nhat[a];
@@ -893,9 +991,9 @@ __global__ __launch_bounds__(TPB) void k_surface1(Consts c, DevState s, const in
         }
         for (int k = 0; k < cnt; ++k) {
             const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
-            const double r = sqrt(nl.rr[(int64_t)k * nl.npad + i]);
-            double xj[D], xjT[D];
+            double xj[D], xjT[D], d2, r;
             load_vec<D>(s.x, j, xj);
+            pair_dist<D>(r_from_x, xi, xj, nl, k, i, d2, r);
 #pragma unroll
             for (int a = 0; a < D; ++a) xjT[a] = xj[a] - pT[a];
             if (r >= r_lim) {
@@ -915,20 +1013,36 @@ __global__ __launch_bounds__(TPB) void k_surface1(Consts c, DevState s, const in
     } else {
         s.surf[i] = 0;
     }
-    /* normal from the eigenvalue gradient */
+    /* normal from the eigenvalue gradient (src/Geometry.cpp:109-141), with
+     * the dissipation terms in the same neighbour loop */
     double nrm[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) nrm[a] = 0.0;
     const bool high = lami > 0.7;
+    double vi[D], gri[D];
+    DissAcc<D, T> dacc;
+    const double rhoi = s.rho[i];
+    if (with_diss) {
+        load_vec<D>(s.v, i, vi);
+        load_vec<D>(s.gradRho, i, gri);
+        dacc.init(c, rhoi);
+    }
     for (int k = 0; k < cnt; ++k) {
         const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
-        const double r = sqrt(nl.rr[(int64_t)k * nl.npad + i]);
+        double xj[D], d2, r;
+        load_vec<D>(s.x, j, xj);
+        pair_dist<D>(r_from_x, xi, xj, nl, k, i, d2, r);
         double W, fg;
         wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
-        const double volj = s.m[j] / s.rho[j];
-        const double f = volj * (high ? (s.lam[j] - lami) : s.lam[j]) * fg;
+        const double f = ax.vol[j] * (high ? (s.lam[j] - lami) : s.lam[j]) * fg;
 #pragma unroll
-        for (int a = 0; a < D; ++a) nrm[a] += f * (xi[a] - s.x[(int64_t)j * D + a]);
+        for (int a = 0; a < D; ++a) nrm[a] += f * (xi[a] - xj[a]);
+        if (with_diss) {
+            T Wt, fgt;
+            if (sizeof(T) == 8) { Wt = T(W); fgt = T(fg); }
+            else wendland<T>(T(r), T(c.H), T(inv_H), T(c.Wc), Wt, fgt);
+            dacc.add(s, ax, j, xi, xj, vi, gri, rhoi, d2, T(r), fgt);
+        }
     }
     double Ln[D];
 #pragma unroll
@@ -946,47 +1060,67 @@ __global__ __launch_bounds__(TPB) void k_surface1(Consts c, DevState s, const in
     }
 #pragma unroll
     for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = out[a];
-    if (with_diss) dissipation_one<D, T>(c, s, i, nl);
+    if (with_diss) dacc.store(c, s, i);
 }
 
 template <int D, typename T>
-__global__ __launch_bounds__(TPB) void k_dissipation(Consts c, DevState s, const int32_t* __restrict__ part_id,
-                                                     NbrList nl, int64_t n)
+__global__ __launch_bounds__(TPB) void k_dissipation(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                     NbrList nl, int64_t n, int r_from_x)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n || part_id[i] < c.n_bound) return;
-    dissipation_one<D, T>(c, s, i, nl);
+    double xi[D], vi[D], gri[D];
+    load_vec<D>(s.x, i, xi);
+    load_vec<D>(s.v, i, vi);
+    load_vec<D>(s.gradRho, i, gri);
+    const double rhoi = s.rho[i];
+    DissAcc<D, T> dacc;
+    dacc.init(c, rhoi);
+    const T inv_H = T(1.0 / c.H);
+    const int cnt = nl.count[i];
+    for (int k = 0; k < cnt; ++k) {
+        const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
+        double xj[D], d2, r;
+        load_vec<D>(s.x, j, xj);
+        pair_dist<D>(r_from_x, xi, xj, nl, k, i, d2, r);
+        T W, fg;
+        wendland<T>(T(r), T(c.H), inv_H, T(c.Wc), W, fg);
+        dacc.add(s, ax, j, xi, xj, vi, gri, rhoi, d2, T(r), fg);
+    }
+    dacc.store(c, s, i);
 }
 
-void launch_surface1(int dim, int prec, const Consts& c, DevState s, const int32_t* part_id, NbrList nl,
-                     int64_t n, double* norms_tmp, int with_dissipation, hipStream_t st)
+void launch_surface1(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl,
+                     int64_t n, int r_from_x, double* norms_tmp, int with_dissipation, hipStream_t st)
 {
     const dim3 g(grid_of(n)), b(TPB);
     if (dim == 2) {
-        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_surface1<2, float>), g, b, 0, st, c, s, part_id, nl, n, norms_tmp, with_dissipation);
-        else hipLaunchKernelGGL((k_surface1<2, double>), g, b, 0, st, c, s, part_id, nl, n, norms_tmp, with_dissipation);
+        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_surface1<2, float>), g, b, 0, st, c, s, a, part_id, nl, n, r_from_x, norms_tmp, with_dissipation);
+        else hipLaunchKernelGGL((k_surface1<2, double>), g, b, 0, st, c, s, a, part_id, nl, n, r_from_x, norms_tmp, with_dissipation);
     } else {
-        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_surface1<3, float>), g, b, 0, st, c, s, part_id, nl, n, norms_tmp, with_dissipation);
-        else hipLaunchKernelGGL((k_surface1<3, double>), g, b, 0, st, c, s, part_id, nl, n, norms_tmp, with_dissipation);
+        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_surface1<3, float>), g, b, 0, st, c, s, a, part_id, nl, n, r_from_x, norms_tmp, with_dissipation);
+        else hipLaunchKernelGGL((k_surface1<3, double>), g, b, 0, st, c, s, a, part_id, nl, n, r_from_x, norms_tmp, with_dissipation);
     }
 }
 
-void launch_dissipation(int dim, int prec, const Consts& c, DevState s, const int32_t* part_id, NbrList nl,
-                        int64_t n, hipStream_t st)
+void launch_dissipation(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl,
+                        int64_t n, int r_from_x, hipStream_t st)
 {
     const dim3 g(grid_of(n)), b(TPB);
     if (dim == 2) {
-        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_dissipation<2, float>), g, b, 0, st, c, s, part_id, nl, n);
-        else hipLaunchKernelGGL((k_dissipation<2, double>), g, b, 0, st, c, s, part_id, nl, n);
+        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_dissipation<2, float>), g, b, 0, st, c, s, a, part_id, nl, n, r_from_x);
+        else hipLaunchKernelGGL((k_dissipation<2, double>), g, b, 0, st, c, s, a, part_id, nl, n, r_from_x);
     } else {
-        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_dissipation<3, float>), g, b, 0, st, c, s, part_id, nl, n);
-        else hipLaunchKernelGGL((k_dissipation<3, double>), g, b, 0, st, c, s, part_id, nl, n);
+        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_dissipation<3, float>), g, b, 0, st, c, s, a, part_id, nl, n, r_from_x);
+        else hipLaunchKernelGGL((k_dissipation<3, double>), g, b, 0, st, c, s, a, part_id, nl, n, r_from_x);
     }
 }
 
+/* Curvature, Gissler occlusion and surfzone (src/Geometry.cpp:145-277). */
 template <int D>
-__global__ __launch_bounds__(TPB) void k_surface2(Consts c, DevState s, const int32_t* __restrict__ part_id,
-                                                  NbrList nl, int64_t n, const double* __restrict__ norms_tmp)
+__global__ __launch_bounds__(TPB) void k_surface2(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                  NbrList nl, int64_t n, int r_from_x,
+                                                  const double* __restrict__ norms_tmp)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n || part_id[i] < c.n_bound) return;
@@ -994,76 +1128,86 @@ __global__ __launch_bounds__(TPB) void k_surface2(Consts c, DevState s, const in
     load_vec<D>(s.x, i, xi);
     load_vec<D>(norms_tmp, i, ni);
     const int32_t bi = s.b[i];
-    const bool gis = (c.aero_case == FJSPH_GISSLER);
+    const bool occl = (c.aero_case == FJSPH_GISSLER) && bi == FJSPH_FREE;
 #pragma unroll
     for (int a = 0; a < D; ++a) Vd[a] = 0.0;
-    if (s.cellID[i] != -1 && bi == FJSPH_FREE && gis) {
+    if (s.cellID[i] != -1 && occl) {
 #pragma unroll
         for (int a = 0; a < D; ++a)
             Vd[a] = (c.aero_source == FJSPH_MESH_INFL ? s.cellV[i * D + a] : c.v_inf[a]) - s.v[i * D + a];
     }
     const double Vdn = sqrt(dotd<D>(Vd, Vd));
-    const bool ni_nz = sqrt(dotd<D>(ni, ni)) > 0.0;
+    const bool ni_nz = dotd<D>(ni, ni) > 0.0;
     double Li[D][D];
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
         for (int b2 = 0; b2 < D; ++b2) Li[a][b2] = s.L[i * D * D + a * D + b2];
     const double inv_H = 1.0 / c.H;
-    double curve = 0.0, wo = 0.0;
+    double curve = 0.0, qmax = 0.0;
     int zone = s.surf[i] == 1 ? 1 : 0;
     const int cnt = nl.count[i];
     for (int k = 0; k < cnt; ++k) {
         const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
-        const double r = sqrt(nl.rr[(int64_t)k * nl.npad + i]);
-        double Rij[D], nj[D];
+        double xj[D], d2, r, nj[D];
+        load_vec<D>(s.x, j, xj);
+        pair_dist<D>(r_from_x, xi, xj, nl, k, i, d2, r);
+        double Rij[D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) Rij[a] = s.x[(int64_t)j * D + a] - xi[a];
+        for (int a = 0; a < D; ++a) Rij[a] = xj[a] - xi[a];
         load_vec<D>(norms_tmp, j, nj);
-        if (ni_nz && sqrt(dotd<D>(nj, nj)) > 0.0) {
+        if (ni_nz && dotd<D>(nj, nj) > 0.0) {
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
-            const double volj = s.m[j] / s.rho[j];
-            double dn[D];
-#pragma unroll
-            for (int a = 0; a < D; ++a) dn[a] = nj[a] - ni[a];
             double t = 0.0;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 double la = 0.0;
 #pragma unroll
-                for (int b2 = 0; b2 < D; ++b2) la += Li[a][b2] * dn[b2];
+                for (int b2 = 0; b2 < D; ++b2) la += Li[a][b2] * (nj[b2] - ni[b2]);
                 t += la * (fg * Rij[a]);
             }
-            curve += volj * t;
+            curve += ax.vol[j] * t;
         }
-        if (bi == FJSPH_FREE && gis) {
-            const double frac = -dotd<D>(Rij, Vd) / (Vdn * r);
-            if (frac > wo) wo = frac;
+        if (occl) {
+            /* frac = -Rij.Vd / (|Vd| r); the max is taken over -Rij.Vd / r
+             * and divided by |Vd| once */
+            const double q = -dotd<D>(Rij, Vd) / r;
+            if (q > qmax) qmax = q;
         }
         if (c.ale && s.surf[j] == 1) zone = 1;
     }
+    const double wo = (occl && Vdn > 0.0) ? qmax / Vdn : 0.0;
     s.woccl[i] = s.lam_nb[i] < c.lam_cutoff ? fmax(0.0, fmin(wo, 1.0)) : 1.0;
+    double nh[D];
+    normalize<D>(ni, nh);
 #pragma unroll
-    for (int a = 0; a < D; ++a) s.norm[i * D + a] = ni[a];
+    for (int a = 0; a < D; ++a) {
+        s.norm[i * D + a] = ni[a];
+        ax.nhat[i * D + a] = nh[a];
+    }
     s.curve[i] = curve;
     s.norm_curve[i] = c.dx * curve;
     s.pDist[i] = s.lam[i];
     if (c.ale) s.surfzone[i] = zone;
 }
 
-void launch_surface2(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
-                     const double* norms_tmp, hipStream_t st)
+void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl, int64_t n,
+                     int r_from_x, const double* norms_tmp, hipStream_t st)
 {
-    if (dim == 2) hipLaunchKernelGGL(k_surface2<2>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, part_id, nl, n, norms_tmp);
-    else hipLaunchKernelGGL(k_surface2<3>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, part_id, nl, n, norms_tmp);
+    if (dim == 2) hipLaunchKernelGGL(k_surface2<2>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, a, part_id, nl, n, r_from_x, norms_tmp);
+    else hipLaunchKernelGGL(k_surface2<3>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, a, part_id, nl, n, r_from_x, norms_tmp);
 }
 
 /* ------------------------------------------------------------ particle_shift */
 
+/* src/Shifting.cpp:189-290.  max_j acos(n_i.n_j) is taken as acos(min_j
+ * n_i.n_j) over the dot products acos accepts ([-1, 1]); NaN angles never
+ * win the reference's max, and acos is decreasing, so the result is the
+ * same.  max_j |v_j - v_i| is taken on squared norms. */
 template <int D>
-__global__ __launch_bounds__(TPB) void k_shift(Consts c, DevState s, const int32_t* __restrict__ part_id, NbrList nl,
-                                               int64_t n)
+__global__ __launch_bounds__(TPB) void k_shift(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                               NbrList nl, int64_t n, int r_from_x)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n || part_id[i] < c.n_bound) return;
@@ -1073,47 +1217,49 @@ __global__ __launch_bounds__(TPB) void k_shift(Consts c, DevState s, const int32
         for (int a = 0; a < D; ++a) s.vPert[i * D + a] = 0.0;
         return;
     }
-    double xi[D], vi[D], ni[D], nhat[D];
+    double xi[D], vi[D], nhat[D];
     load_vec<D>(s.x, i, xi);
     load_vec<D>(s.v, i, vi);
-    load_vec<D>(s.norm, i, ni);
-    normalize<D>(ni, nhat);
+    load_vec<D>(ax.nhat, i, nhat);
     const double inv_H = 1.0 / c.H;
+    const double inv_Wdx = 1.0 / c.W_dx;
     double du[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) du[a] = 0.0;
-    double maxUij = 0.0, wo = 0.0;
+    double maxU2 = 0.0, dmin = 2.0;
     const int cnt = nl.count[i];
     for (int k = 0; k < cnt; ++k) {
         const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
-        const double r = sqrt(nl.rr[(int64_t)k * nl.npad + i]);
-        const double volj = s.m[j] / s.rho[j];
+        double xj[D], d2, r;
+        load_vec<D>(s.x, j, xj);
+        pair_dist<D>(r_from_x, xi, xj, nl, k, i, d2, r);
         double W, fg;
         wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
-        const double w4 = W / c.W_dx;
-        const double f = (1.0 + 0.2 * (w4 * w4) * (w4 * w4)) * fg * volj;
-        double dv[D];
+        const double w = W * inv_Wdx;
+        const double w2 = w * w;
+        const double f = (1.0 + 0.2 * (w2 * w2)) * fg * ax.vol[j];
+        double dv2 = 0.0;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            du[a] += f * (s.x[(int64_t)j * D + a] - xi[a]);
-            dv[a] = s.v[(int64_t)j * D + a] - vi[a];
+            du[a] += f * (xj[a] - xi[a]);
+            const double dv = s.v[(int64_t)j * D + a] - vi[a];
+            dv2 += dv * dv;
         }
         if (s.b[j] > FJSPH_PISTON) {
-            double nj[D], njh[D];
-            load_vec<D>(s.norm, j, nj);
-            normalize<D>(nj, njh);
-            const double th = acos(dotd<D>(nhat, njh));
-            if (th > wo) wo = th;
+            double nj[D];
+            load_vec<D>(ax.nhat, j, nj);
+            const double d = dotd<D>(nhat, nj);
+            if (d >= -1.0 && d <= 1.0 && d < dmin) dmin = d;
         }
-        const double dvn = sqrt(dotd<D>(dv, dv));
-        if (dvn > maxUij) maxUij = dvn;
+        if (dv2 > maxU2) maxU2 = dv2;
     }
+    const double wo = dmin <= 1.0 ? acos(dmin) : 0.0;
     const double fac = -2.0 * c.H * sqrt(dotd<D>(vi, vi));
 #pragma unroll
     for (int a = 0; a < D; ++a) du[a] *= fac;
     double dun[D];
     normalize<D>(du, dun);
-    const double mag = fmin(sqrt(dotd<D>(du, du)), fmin(maxUij / 2.0, c.max_shift_vel));
+    const double mag = fmin(sqrt(dotd<D>(du, du)), fmin(sqrt(maxU2) / 2.0, c.max_shift_vel));
 #pragma unroll
     for (int a = 0; a < D; ++a) du[a] = mag * dun[a];
     double out[D];
@@ -1121,14 +1267,11 @@ __global__ __launch_bounds__(TPB) void k_shift(Consts c, DevState s, const int32
 #pragma unroll
         for (int a = 0; a < D; ++a) out[a] = du[a];
     } else if (wo < kPi / 12.0) {
-        double nm[D];
-#pragma unroll
-        for (int a = 0; a < D; ++a) nm[a] = -nhat[a];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             double t = 0.0;
 #pragma unroll
-            for (int b2 = 0; b2 < D; ++b2) t += (((a == b2) ? 1.0 : 0.0) - nm[a] * nm[b2]) * du[b2];
+            for (int b2 = 0; b2 < D; ++b2) t += (((a == b2) ? 1.0 : 0.0) - nhat[a] * nhat[b2]) * du[b2];
             out[a] = t;
         }
     } else {
@@ -1139,11 +1282,11 @@ __global__ __launch_bounds__(TPB) void k_shift(Consts c, DevState s, const int32
     for (int a = 0; a < D; ++a) s.vPert[i * D + a] = out[a];
 }
 
-void launch_shift(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
-                  hipStream_t st)
+void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl, int64_t n,
+                  int r_from_x, hipStream_t st)
 {
-    if (dim == 2) hipLaunchKernelGGL(k_shift<2>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, part_id, nl, n);
-    else hipLaunchKernelGGL(k_shift<3>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, part_id, nl, n);
+    if (dim == 2) hipLaunchKernelGGL(k_shift<2>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, a, part_id, nl, n, r_from_x);
+    else hipLaunchKernelGGL(k_shift<3>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, a, part_id, nl, n, r_from_x);
 }
 
 /* ------------------------------------------------------- boundary solvers */
@@ -1160,7 +1303,8 @@ __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int
     for (int a = 0; a < D; ++a) v[a] = bl.vel[block * 3 + a];
     if (bl.no_slip[block]) {
         const double inv_H = 1.0 / c.H;
-        double vs[D];
+        double xi[D], vs[D];
+        load_vec<D>(s.x, i, xi);
 #pragma unroll
         for (int a = 0; a < D; ++a) vs[a] = 0.0;
         double ks = 0.0;
@@ -1168,7 +1312,9 @@ __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int
         for (int k = 0; k < cnt; ++k) {
             const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
             if (s.b[j] <= FJSPH_PISTON) continue;
-            const double r = sqrt(nl.rr[(int64_t)k * nl.npad + i]);
+            double xj[D], d2, r;
+            load_vec<D>(s.x, j, xj);
+            pair_dist<D>(0, xi, xj, nl, k, i, d2, r);
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
             ks += W;
@@ -1184,8 +1330,10 @@ __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int
     for (int a = 0; a < D; ++a) s.v[i * D + a] = v[a];
 }
 
+/* Boundary solver of one block (src/Resid.cpp:21-160). The list distance is
+ * the frozen one, as in the reference. */
 template <int D>
-__global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, const int32_t* __restrict__ blk,
+__global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax, const int32_t* __restrict__ blk,
                                                    DevBlocks bl, int32_t block, NbrList nl, int64_t n,
                                                    int32_t* near_inlet)
 {
@@ -1207,15 +1355,17 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, const i
         for (int k = 0; k < cnt; ++k) {
             const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
             if (s.b[j] <= FJSPH_PISTON) continue;
-            const double r = sqrt(nl.rr[(int64_t)k * nl.npad + i]);
+            double xj[D], d2, r;
+            load_vec<D>(s.x, j, xj);
+            pair_dist<D>(0, xi, xj, nl, k, i, d2, r);
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
-            const double kern = s.m[j] / s.rho[j] * W;
+            const double kern = ax.vol[j] * W;
             ks += kern;
             pk += s.p[j] * kern;
             const double kr = kern * s.rho[j];
 #pragma unroll
-            for (int a = 0; a < D; ++a) ak[a] += kr * (xi[a] - s.x[(int64_t)j * D + a]);
+            for (int a = 0; a < D; ++a) ak[a] += kr * (xi[a] - xj[a]);
             if (s.surfzone[j]) near = 1;
         }
         double pres = 0.0;
@@ -1236,32 +1386,31 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, const i
             const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
             const int32_t bj = s.b[j];
             if (bj == FJSPH_PIPE || bj == FJSPH_FREE) ni = 0;
-            const double r = sqrt(nl.rr[(int64_t)k * nl.npad + i]);
+            double xj[D], d2, r;
+            load_vec<D>(s.x, j, xj);
+            pair_dist<D>(0, xi, xj, nl, k, i, d2, r);
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
-            const double volj = s.m[j] / s.rho[j];
             double t = 0.0;
 #pragma unroll
-            for (int a = 0; a < D; ++a)
-                t += (s.v[(int64_t)j * D + a] - vi[a]) * (fg * (s.x[(int64_t)j * D + a] - xi[a]));
-            rr -= volj * t;
+            for (int a = 0; a < D; ++a) t += (s.v[(int64_t)j * D + a] - vi[a]) * (fg * (xj[a] - xi[a]));
+            rr -= ax.vol[j] * t;
         }
         near_inlet[i] = ni;
         s.Rrho[i] = rr * s.rho[i];
     }
 }
 
-void launch_boundary(int dim, const Consts& c, DevState s, const int32_t* part_id, const int32_t* blk,
-                     DevBlocks bl, int32_t block, NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st)
+void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t* blk, DevBlocks bl, int32_t block,
+                     NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st)
 {
-    (void)part_id;
     const dim3 g(grid_of(n)), b(TPB);
     if (dim == 2) {
         hipLaunchKernelGGL(k_bnd_vel<2>, g, b, 0, st, c, s, blk, bl, block, nl, n);
-        hipLaunchKernelGGL(k_bnd_solve<2>, g, b, 0, st, c, s, blk, bl, block, nl, n, near_inlet);
+        hipLaunchKernelGGL(k_bnd_solve<2>, g, b, 0, st, c, s, a, blk, bl, block, nl, n, near_inlet);
     } else {
         hipLaunchKernelGGL(k_bnd_vel<3>, g, b, 0, st, c, s, blk, bl, block, nl, n);
-        hipLaunchKernelGGL(k_bnd_solve<3>, g, b, 0, st, c, s, blk, bl, block, nl, n, near_inlet);
+        hipLaunchKernelGGL(k_bnd_solve<3>, g, b, 0, st, c, s, a, blk, bl, block, nl, n, near_inlet);
     }
 }
 
@@ -1303,13 +1452,19 @@ struct NextState {
     double *x, *v, *rho, *p;
 };
 
+/* get_acc_and_Rrho_on_i (src/Resid.cpp:243-418) for fluid slots, fused with
+ * the Newmark-Beta update of Do_NB_Iter (src/Newmark_Beta.cpp:137-240) when
+ * do_update.  Pair terms are formed in T; every sum is accumulated in FP64.
+ * The pressure, viscous and surface-tension terms of a pair are combined
+ * before accumulation: acc_i = sum_j (-m_j (P_i + P_j) g + visc + ST / m_i). */
 template <int D, typename T>
-__global__ __launch_bounds__(TPB) void k_force(Consts c, DevState s, DevState pn, NextState nx, int do_update,
+__global__ __launch_bounds__(TPB) void k_force(Consts c, DevState s, Aux ax, DevState pn, NextState nx, int do_update,
                                                const int32_t* __restrict__ part_id, const int32_t* __restrict__ blk,
                                                DevBlocks bl, const int32_t* __restrict__ near_inlet, NbrList nl,
                                                int64_t n, double dt, const DevScalars* sc, double* partial)
 {
     __shared__ double lds[NWAVE];
+    constexpr bool F32 = sizeof(T) == 4;
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     double resid = 0.0;
     if (i < n) {
@@ -1319,19 +1474,23 @@ __global__ __launch_bounds__(TPB) void k_force(Consts c, DevState s, DevState pn
             double xi[D], vi[D];
             load_vec<D>(s.x, i, xi);
             load_vec<D>(s.v, i, vi);
-            const double rhoi = s.rho[i], pi_ = s.p[i], mi = s.m[i];
+            const double rhoi = s.rho[i], mi = s.m[i];
             const int32_t bi = s.b[i];
             const double npd = sc->npd;
-            const double npdm2 = (0.5 * c.sig / c.lam_st) / (npd * npd);
+            const T ste = T(-(0.5 * c.sig / c.lam_st) / (npd * npd));
             const T pi3o4 = T(3.0 * kPi / 4.0);
             const T inv_H = T(1.0 / c.H), H = T(c.H), Wc = T(c.Wc);
             const T eps2 = T(0.001 * c.H_sq);
-            const double Pi = pi_ / (rhoi * rhoi);
+            const T Pi = T(ax.P[i]);
+            const T iri = T(ax.invrho[i]);
+            const T nu = T(c.nu);
+            const T inv_mi = T(1.0 / mi);
             const bool st_on = (!c.ale) || (s.surfzone[i] == 1);
+            const bool bound_i = bi == FJSPH_BOUND;
             const T contang_fac = T(1.0 + 0.5 * cos(0.5 * kPi * 7.0 / 9.0));
-            double acc[D], visc[D], surf[D], ale_acc[D];
+            double acc[D];
 #pragma unroll
-            for (int a = 0; a < D; ++a) { acc[a] = 0.0; visc[a] = 0.0; surf[a] = 0.0; ale_acc[a] = 0.0; }
+            for (int a = 0; a < D; ++a) acc[a] = 0.0;
             double Rrho_ = 0.0, Rrhoc_ = 0.0;
             const int cnt = nl.count[i];
             if (s.cellID[i] != -1) {
@@ -1347,67 +1506,91 @@ __global__ __launch_bounds__(TPB) void k_force(Consts c, DevState s, DevState pn
 #pragma unroll
                 for (int a = 0; a < D; ++a) { acc[a] += aero[a]; s.Af[i * D + a] = aero[a]; }
             }
-            double vpi[D];
-            if (c.ale) load_vec<D>(s.vPert, i, vpi);
-            const T ste = T(-npdm2);
+            T vpi[D];
+            if (c.ale) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) vpi[a] = T(s.vPert[i * D + a]);
+            }
             for (int k = 0; k < cnt; ++k) {
-                const int32_t j = nl.idx[(int64_t)k * nl.npad + i];
-                const double rrd = nl.rr[(int64_t)k * nl.npad + i];
+                const int64_t e = (int64_t)k * nl.npad + i;
+                const int32_t j = nl.idx[e];
+                T r, rr;
+                if (F32) {
+                    r = T(((const float*)nl.r)[e]);
+                    rr = r * r;
+                } else {
+                    const double d2 = ((const double*)nl.r)[e];
+                    rr = T(d2);
+                    r = T(sqrt(d2));
+                }
                 T Rji[D], Vji[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     Rji[a] = T(s.x[(int64_t)j * D + a] - xi[a]);
                     Vji[a] = T(s.v[(int64_t)j * D + a] - vi[a]);
                 }
-                const T rr = T(rrd);
-                const T r = T(sqrt(rrd));
-                const T idist2 = T(1) / (rr + eps2);
-                const double rhoj = s.rho[j];
-                const double mjd = s.m[j];
-                const T mj = T(mjd);
-                const T volj = T(mjd / rhoj);
+                T mj, Pj, irj, volj;
+                bool bound_j;
+                if (F32) {
+                    const float4 q = ax.fq[j];
+                    bound_j = q.x < 0.f;
+                    mj = T(fabsf(q.x));
+                    Pj = T(q.y);
+                    irj = T(q.z);
+                    volj = T(q.w);
+                } else {
+                    mj = T(s.m[j]);
+                    Pj = T(ax.P[j]);
+                    irj = T(ax.invrho[j]);
+                    volj = T(ax.vol[j]);
+                    bound_j = s.b[j] == FJSPH_BOUND;
+                }
+                const T idist2 = fast_rcp<T>(rr + eps2);
                 T W, fg;
                 wendland<T>(r, H, inv_H, Wc, W, fg);
                 T g[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) g[a] = fg * Rji[a];
-                /* pressure, Monaghan 1994 (BasePos) */
-                const T pc = mj * T(Pi + s.p[j] / (rhoj * rhoj));
-                /* laminar viscosity, Morris */
+                const T pc = mj * (Pi + Pj);
                 T rdotg = Rji[0] * g[0];
 #pragma unroll
                 for (int a = 1; a < D; ++a) rdotg += Rji[a] * g[a];
-                const T vf = mj * T(c.nu * (rhoi + rhoj) / (rhoi * rhoj)) * rdotg * idist2;
-                /* pairwise surface tension, Nair & Poeschel */
+                const T vf = mj * nu * (iri + irj) * rdotg * idist2;
                 T sf = T(0);
                 if (st_on) {
-                    const int32_t bj = s.b[j];
-                    const T fac = (bi == FJSPH_BOUND || bj == FJSPH_BOUND) ? contang_fac : T(1);
-                    sf = ste * fac * cos(pi3o4 * r * inv_H) / r;
+                    const T fac = (bound_i || bound_j) ? contang_fac : T(1);
+                    sf = ste * fac * cos(pi3o4 * r * inv_H) * fast_rcp<T>(r) * inv_mi;
                 }
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    acc[a] -= double(pc * g[a]);
-                    visc[a] += double(vf * Vji[a]);
-                    surf[a] += double(sf * Rji[a]);
-                }
+                for (int a = 0; a < D; ++a) acc[a] += double(vf * Vji[a] + sf * Rji[a] - pc * g[a]);
                 if (c.ale) {
-                    double vpj[D], vj[D];
-                    load_vec<D>(s.vPert, j, vpj);
-                    load_vec<D>(s.v, j, vj);
+                    T vpj[D], rhoj;
+                    if (F32) {
+                        const float4 q = ax.fale[j];
+                        vpj[0] = q.x;
+                        if (D > 1) vpj[1 % D] = q.y;
+                        if (D > 2) vpj[2 % D] = q.z;
+                        rhoj = q.w;
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < D; ++a) vpj[a] = T(s.vPert[(int64_t)j * D + a]);
+                        rhoj = T(s.rho[j]);
+                    }
                     T aj = T(0), ai = T(0), cd = T(0), cont = T(0), c2 = T(0);
-                    const double rvj = rhoj;
+                    const T rhoit = T(rhoi);
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
-                        aj += T(vpj[a]) * g[a];
-                        ai += T(vpi[a]) * g[a];
-                        cd += T(vpj[a] - vpi[a]) * g[a];
-                        cont += T((vj[a] + vpj[a]) - (vi[a] + vpi[a])) * g[a];
-                        c2 += T(rvj * vpj[a] + rhoi * vpi[a]) * g[a];
+                        aj += vpj[a] * g[a];
+                        ai += vpi[a] * g[a];
+                        cd += (vpj[a] - vpi[a]) * g[a];
+                        cont += (Vji[a] + (vpj[a] - vpi[a])) * g[a];
+                        c2 += (rhoj * vpj[a] + rhoit * vpi[a]) * g[a];
                     }
 #pragma unroll
-                    for (int a = 0; a < D; ++a)
-                        ale_acc[a] += double((T(vj[a]) * aj + T(vi[a]) * ai - T(vi[a]) * cd) * volj);
+                    for (int a = 0; a < D; ++a) {
+                        const T vja = T(s.v[(int64_t)j * D + a]), via = T(vi[a]);
+                        acc[a] += double((vja * aj + via * ai - via * cd) * volj);
+                    }
                     Rrho_ -= double(cont * volj);
                     Rrhoc_ += double(c2 * volj);
                 } else {
@@ -1419,9 +1602,7 @@ __global__ __launch_bounds__(TPB) void k_force(Consts c, DevState s, DevState pn
             }
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                double t = acc[a];
-                if (c.ale) t += ale_acc[a];
-                acc_new[a] = t + s.aVisc[i * D + a] + visc[a] + surf[a] / mi + c.grav[a];
+                acc_new[a] = acc[a] + s.aVisc[i * D + a] + c.grav[a];
                 s.acc[i * D + a] = acc_new[a];
             }
             Rrho_new = c.ale ? (Rrho_ * rhoi + Rrhoc_ + s.deltaD[i]) : (Rrho_ * rhoi + s.deltaD[i]);
@@ -1489,25 +1670,24 @@ __global__ __launch_bounds__(TPB) void k_force(Consts c, DevState s, DevState pn
     }
 }
 
-void launch_force(int dim, int prec, const Consts& c, DevState s, DevState pn, const double* next_x,
-                  double* next_x_w, double* next_v, double* next_rho, double* next_p, const int32_t* part_id,
-                  const int32_t* blk, DevBlocks bl, const int32_t* near_inlet, NbrList nl, int64_t n,
-                  double dt, const DevScalars* sc, double* partial, int nparts, hipStream_t st)
+void launch_force(int dim, int prec, const Consts& c, DevState s, Aux a, DevState pn, double* next_x,
+                  double* next_v, double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk,
+                  DevBlocks bl, const int32_t* near_inlet, NbrList nl, int64_t n, double dt, const DevScalars* sc,
+                  double* partial, int nparts, hipStream_t st)
 {
-    (void)next_x;
-    NextState nx{next_x_w, next_v, next_rho, next_p};
-    const int upd = next_x_w != nullptr;
+    NextState nx{next_x, next_v, next_rho, next_p};
+    const int upd = next_x != nullptr;
     const dim3 g(nparts), b(TPB);
     if (dim == 2) {
         if (prec == FJSPH_PREC_MIXED)
-            hipLaunchKernelGGL((k_force<2, float>), g, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl, n, dt, sc, partial);
+            hipLaunchKernelGGL((k_force<2, float>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, nl, n, dt, sc, partial);
         else
-            hipLaunchKernelGGL((k_force<2, double>), g, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl, n, dt, sc, partial);
+            hipLaunchKernelGGL((k_force<2, double>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, nl, n, dt, sc, partial);
     } else {
         if (prec == FJSPH_PREC_MIXED)
-            hipLaunchKernelGGL((k_force<3, float>), g, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl, n, dt, sc, partial);
+            hipLaunchKernelGGL((k_force<3, float>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, nl, n, dt, sc, partial);
         else
-            hipLaunchKernelGGL((k_force<3, double>), g, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl, n, dt, sc, partial);
+            hipLaunchKernelGGL((k_force<3, double>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, nl, n, dt, sc, partial);
     }
 }
 
