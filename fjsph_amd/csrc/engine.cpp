@@ -129,7 +129,9 @@ struct fjsph_ctx {
 
     NbrList nl{};
     Aux aux{};
-    double* xs = nullptr;      /* positions in cell order for the list build */
+    TileGrid tg{};
+    int64_t partial_cap = 0;
+    int32_t* export_buf = nullptr;
     bool x_at_build = false;   /* pnp1.x unchanged since the last build     */
     DevScalars* sc = nullptr;
     DevScalars* sc_host = nullptr; /* pinned */
@@ -227,11 +229,11 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->partial); dfree(ctx->gather_tmp); dfree(ctx->gather_tmp_i);
     dfree(ctx->cell); dfree(ctx->cell_count); dfree(ctx->cell_start); dfree(ctx->cell_fill);
     dfree(ctx->order); dfree(ctx->tile_sums);
-    dfree(ctx->nl.idx); dfree(ctx->nl.count);
+    dfree(ctx->nl.lu); dfree(ctx->nl.count); dfree(ctx->export_buf);
     if (ctx->nl.r) (void)hipFree(ctx->nl.r);
     ctx->nl.r = nullptr;
     dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
-    dfree(ctx->aux.nhat); dfree(ctx->xs);
+    dfree(ctx->aux.nhat);
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
     ctx->ncell_cap = 0;
     ctx->nl.cap = 0;
@@ -394,23 +396,44 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         permute_all(ctx, ctx->order);
         launch_iota(ctx->order, n, ctx->st);
     }
-    launch_gather_x(D, ctx->pnp1.x, ctx->order, n, ctx->xs, ctx->st);
+    /* tile grid: chunks of C cells sized for about one workgroup of targets */
+    {
+        TileGrid& tg = ctx->tg;
+        const double occ = std::pow(cs / ctx->s.particle_step, D);
+        int32_t C = int32_t(std::lround(256.0 / std::max(occ, 1.0)));
+        C = std::max(1, std::min(C, ctx->dims[0]));
+        for (int k = 0; k < 3; ++k) tg.dims[k] = ctx->dims[k];
+        tg.C = C;
+        tg.nchunk = (ctx->dims[0] + C - 1) / C;
+        tg.nblocks = int64_t(ctx->dims[1]) * ctx->dims[2] * tg.nchunk;
+        tg.start = ctx->cell_start;
+        tg.order = ctx->order;
+        tg.cell = ctx->cell;
+        if (tg.nblocks > 0x7fffffff) throw std::string("tile grid too large");
+        if (tg.nblocks > ctx->partial_cap) {
+            ctx->partial_cap = tg.nblocks + tg.nblocks / 4 + 64;
+            dalloc(ctx->partial, size_t(std::max<int64_t>(ctx->partial_cap, int64_t(ctx->nparts) * 8)));
+        }
+    }
     for (int attempt = 0; attempt < 2; ++attempt) {
-        HIPCHK(hipMemsetAsync(&ctx->sc->max_count, 0, sizeof(int32_t), ctx->st));
-        launch_build_lists(D, ctx->pnp1.x, ctx->xs, n, ctx->cell, ctx->dims, ctx->cell_start, ctx->order, ctx->d.sr,
-                           ctx->nl, ctx->sc, ctx->st);
-        HIPCHK(hipMemcpyAsync(&ctx->sc_host->max_count, &ctx->sc->max_count, sizeof(int32_t), hipMemcpyDeviceToHost,
-                              ctx->st));
+        HIPCHK(hipMemsetAsync(&ctx->sc->max_count, 0, 2 * sizeof(int32_t), ctx->st));
+        launch_build_lists(D, ctx->pnp1.x, n, ctx->tg, ctx->d.sr, ctx->nl, ctx->sc, ctx->st);
+        HIPCHK(hipMemcpyAsync(&ctx->sc_host->max_count, &ctx->sc->max_count, 2 * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, ctx->st));
         sync(ctx);
+        if (ctx->sc_host->overflow)
+            throw std::string("neighbour candidate union of a tile exceeds 65535 particles (") +
+                std::to_string(ctx->sc_host->overflow) + "); particles are too clustered for the tile grid";
         const int32_t mc = ctx->sc_host->max_count;
         ctx->max_count = mc;
         if (mc <= ctx->nl.cap) break;
         const int32_t cap = ((mc + mc / 8 + 8) + 7) / 8 * 8;
-        dalloc(ctx->nl.idx, size_t(cap) * ctx->nl.npad);
+        dalloc(ctx->nl.lu, size_t(cap) * ctx->nl.npad);
         if (ctx->nl.f32) dalloc(*(float**)&ctx->nl.r, size_t(cap) * ctx->nl.npad);
         else dalloc(*(double**)&ctx->nl.r, size_t(cap) * ctx->nl.npad);
         ctx->nl.cap = cap;
     }
+    if (ctx->max_count > ctx->nl.cap) throw std::string("neighbour list capacity could not be established");
     ctx->have_nbrs = true;
     ctx->x_at_build = true;
 }
@@ -434,23 +457,24 @@ void dsph(fjsph_ctx* ctx, bool with_aero)
 {
     PassTimer t(ctx, FJSPH_T_DSPH);
     refresh_aux(ctx);
-    launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n, r_from_x(ctx),
-                with_aero ? 1 : 0, ctx->partial, ctx->nparts, ctx->sc, ctx->st);
+    launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
+                with_aero ? 1 : 0, ctx->partial, ctx->sc, ctx->st);
 }
 
 void surface_and_dissipation(fjsph_ctx* ctx, bool with_diss)
 {
     PassTimer t(ctx, FJSPH_T_SURFACE);
-    launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n,
-                    r_from_x(ctx), ctx->norms_tmp, with_diss ? 1 : 0, ctx->st);
-    launch_surface2(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n, r_from_x(ctx),
+    launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl,
+                    ctx->n, r_from_x(ctx), ctx->norms_tmp, with_diss ? 1 : 0, ctx->st);
+    launch_surface2(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
                     ctx->norms_tmp, ctx->st);
 }
 
 void shift(fjsph_ctx* ctx)
 {
     PassTimer t(ctx, FJSPH_T_SHIFT);
-    launch_shift(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n, r_from_x(ctx), ctx->st);
+    launch_shift(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
+                 ctx->st);
 }
 
 /* dSPH_PreStep, get_aero_velocity, Detect_Surface, dissipation_terms,
@@ -491,8 +515,8 @@ void boundary_update(fjsph_ctx* ctx)
     refresh_aux(ctx);
     for (size_t k = 0; k < ctx->blocks.size(); ++k) {
         if (ctx->blocks[k].b.is_fluid) continue;
-        launch_boundary(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->blk, ctx->bl, int32_t(k), ctx->nl, ctx->n,
-                        ctx->near_inlet, ctx->st);
+        launch_boundary(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->blk, ctx->bl, int32_t(k), ctx->tg, ctx->nl,
+                        ctx->n, ctx->near_inlet, ctx->st);
     }
 }
 
@@ -501,8 +525,8 @@ void force(fjsph_ctx* ctx, bool update)
     PassTimer t(ctx, FJSPH_T_FORCE_UPDATE);
     refresh_aux(ctx);
     launch_force(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->pn, update ? ctx->x_nxt : nullptr,
-                 ctx->v_nxt, ctx->rho_nxt, ctx->p_nxt, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->nl,
-                 ctx->n, ctx->delta_t, ctx->sc, ctx->partial, ctx->nparts, ctx->st);
+                 ctx->v_nxt, ctx->rho_nxt, ctx->p_nxt, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->tg,
+                 ctx->nl, ctx->n, ctx->delta_t, ctx->sc, ctx->partial, ctx->st);
 }
 
 /* Do_NB_Iter + the residual of Check_Error; returns log10 rms. */
@@ -517,7 +541,7 @@ double nb_iter(fjsph_ctx* ctx)
     ctx->x_at_build = false;
     {
         PassTimer t(ctx, FJSPH_T_REDUCE);
-        launch_sum_partials(ctx->partial, ctx->nparts, 1, &ctx->sc->residual, ctx->st);
+        launch_sum_partials(ctx->partial, int(ctx->tg.nblocks), 1, &ctx->sc->residual, ctx->st);
         HIPCHK(hipMemcpyAsync(&ctx->sc_host->residual, &ctx->sc->residual, sizeof(double), hipMemcpyDeviceToHost,
                               ctx->st));
     }
@@ -780,11 +804,11 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
         dalloc(ctx->norms_tmp, size_t(n_total) * D);
         ctx->nparts = reduction_parts(n_total);
         dalloc(ctx->partial, size_t(ctx->nparts) * 8);
+        ctx->partial_cap = int64_t(ctx->nparts) * 8;
         dalloc(ctx->gather_tmp, size_t(n_total) * D * D);
         dalloc(ctx->gather_tmp_i, size_t(n_total));
         dalloc(ctx->cell, size_t(n_total));
         dalloc(ctx->order, size_t(n_total));
-        dalloc(ctx->xs, size_t(n_total) * D);
         dalloc(ctx->aux.vol, size_t(n_total));
         dalloc(ctx->aux.P, size_t(n_total));
         dalloc(ctx->aux.invrho, size_t(n_total));
@@ -991,8 +1015,13 @@ fjsph_status fjsph_get_neighbours(fjsph_ctx* ctx, int64_t* offsets, int64_t* idx
         std::vector<int32_t> cnt(static_cast<size_t>(n));
         HIPCHK(hipMemcpy(cnt.data(), ctx->nl.count, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
         std::vector<int32_t> ids(size_t(ctx->nl.cap) * ctx->nl.npad);
-        if (!ids.empty())
-            HIPCHK(hipMemcpy(ids.data(), ctx->nl.idx, sizeof(int32_t) * ids.size(), hipMemcpyDeviceToHost));
+        if (!ids.empty()) {
+            dalloc(ctx->export_buf, ids.size());
+            launch_export_lists(ctx->s.dim, ctx->tg, ctx->nl, n, ctx->export_buf, ctx->st);
+            sync(ctx);
+            HIPCHK(hipMemcpy(ids.data(), ctx->export_buf, sizeof(int32_t) * ids.size(), hipMemcpyDeviceToHost));
+            dfree(ctx->export_buf);
+        }
         std::vector<int32_t>& sh = slot_to_host(ctx);
         std::vector<int64_t> hcount(static_cast<size_t>(n));
         for (int64_t s = 0; s < n; ++s) hcount[sh[s]] = cnt[s] + 1;
@@ -1047,8 +1076,8 @@ fjsph_status fjsph_dissipation_terms(fjsph_ctx* ctx)
         require(ctx, true);
         PassTimer t(ctx, FJSPH_T_DISSIPATION);
         refresh_aux(ctx);
-        launch_dissipation(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->nl, ctx->n,
-                           r_from_x(ctx), ctx->st);
+        launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl,
+                        ctx->n, r_from_x(ctx), ctx->norms_tmp, 2, ctx->st);
         sync(ctx);
     });
 }

@@ -38,16 +38,30 @@ struct DevState {
     int32_t *surf, *cellID, *surfzone, *b;
 };
 
-/* Neighbour lists.  The payload is the distance frozen at build time
+/* Neighbour lists, column-major by slot (entry k of slot i at k*npad + i).
+ * lu holds the index of the neighbour in the candidate union of the slot's
+ * tile block (ascending); the payload is the distance frozen at build time
  * (src/Resid.cpp:288-289 reads it from the list): FP64 d^2 on the parity
  * path, FP32 r on the mixed path. */
 struct NbrList {
-    int32_t* idx;
+    uint16_t* lu;
     void* r;
     int32_t* count;
     int64_t npad;
     int32_t cap;
     int32_t f32; /* payload is float r (1) or double d^2 (0) */
+};
+
+/* Tile decomposition of the last neighbour build: block b covers cells
+ * [q*C, q*C + C) of cell row b / nchunk, q = b % nchunk. */
+struct TileGrid {
+    int32_t dims[3];
+    int32_t C;
+    int32_t nchunk;
+    int64_t nblocks;
+    const int32_t* start; /* cell_start, ncell + 1 */
+    const int32_t* order; /* slots in cell order */
+    const int32_t* cell;  /* cell of every slot */
 };
 
 /* Per-particle quantities derived from the current rho, p, m, b (and vPert),
@@ -76,7 +90,7 @@ struct DevScalars {
     double red[8];    /* find_timestep reduction results */
     double bbox[6];   /* lo[3], hi[3] */
     int32_t max_count;
-    int32_t overflow;
+    int32_t overflow; /* largest candidate union beyond 65535 (0 if none) */
 };
 
 /* ---- launch wrappers (kernels.hip); every one enqueues on `st` ---------- */
@@ -88,10 +102,9 @@ void launch_scan(const int32_t* count, int64_t ncell, int32_t* start, int32_t* b
 void launch_scatter(const int32_t* cell, int64_t n, const int32_t* start, int32_t* fill, int32_t* order,
                     hipStream_t st);
 void launch_sort_cells(const int32_t* start, int64_t ncell, int32_t* order, hipStream_t st);
-void launch_gather_x(int dim, const double* x, const int32_t* order, int64_t n, double* xs, hipStream_t st);
-void launch_build_lists(int dim, const double* x, const double* xs, int64_t n, const int32_t* cell,
-                        const int32_t* dims, const int32_t* start, const int32_t* order, double sr, NbrList nl,
-                        DevScalars* sc, hipStream_t st);
+void launch_build_lists(int dim, const double* x, int64_t n, TileGrid tg, double sr, NbrList nl, DevScalars* sc,
+                        hipStream_t st);
+void launch_export_lists(int dim, TileGrid tg, NbrList nl, int64_t n, int32_t* out, hipStream_t st);
 void launch_aux(int dim, DevState s, Aux a, int64_t n, int ale, int mixed, hipStream_t st);
 void launch_gather_d(const double* src, double* dst, const int32_t* perm, int64_t n, int width, hipStream_t st);
 void launch_gather_i(const int32_t* src, int32_t* dst, const int32_t* perm, int64_t n, hipStream_t st);
@@ -100,28 +113,29 @@ void launch_iota(int32_t* a, int64_t n, hipStream_t st);
 /* Pre-passes.  r_from_x = 1 when the positions are those of the last
  * neighbour build (then r is recomputed bit-identically from x instead of
  * being read from the list). */
-void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl, int64_t n,
-                 int r_from_x, int with_aero_vel, double* partial, int nparts, DevScalars* sc, hipStream_t st);
+void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                 int64_t n, int r_from_x, int with_aero_vel, double* partial, DevScalars* sc, hipStream_t st);
 void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
                           hipStream_t st);
-void launch_surface1(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl,
-                     int64_t n, int r_from_x, double* norms_tmp, int with_dissipation, hipStream_t st);
-void launch_dissipation(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl,
-                        int64_t n, int r_from_x, hipStream_t st);
-void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl, int64_t n,
-                     int r_from_x, const double* norms_tmp, hipStream_t st);
-void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, NbrList nl, int64_t n,
-                  int r_from_x, hipStream_t st);
+/* mode 0: Detect_Surface loop 1; 1: + dissipation_terms; 2: dissipation only */
+void launch_surface1(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg,
+                     NbrList nl, int64_t n, int r_from_x, double* norms_tmp, int mode, hipStream_t st);
+void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                     int64_t n, int r_from_x, const double* norms_tmp, hipStream_t st);
+void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                  int64_t n, int r_from_x, hipStream_t st);
 void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t* blk, DevBlocks bl, int32_t block,
-                     NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st);
-/* Force pass on `s` (x/v/rho/p read from s); when next_x is non-null the
- * Newmark update of Do_NB_Iter is fused and writes next x/v/rho/p for every
- * slot, plus per-block residual partial sums. */
+                     TileGrid tg, NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st);
+/* Force pass on `s`; when next_x is non-null the Newmark update of
+ * Do_NB_Iter is fused and writes next x/v/rho/p for every slot, plus one
+ * residual partial sum per tile block. */
 void launch_force(int dim, int prec, const Consts& c, DevState s, Aux a, DevState pn, double* next_x,
                   double* next_v, double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk,
-                  DevBlocks bl, const int32_t* near_inlet, NbrList nl, int64_t n, double dt, const DevScalars* sc,
-                  double* partial, int nparts, hipStream_t st);
+                  DevBlocks bl, const int32_t* near_inlet, TileGrid tg, NbrList nl, int64_t n, double dt,
+                  const DevScalars* sc, double* partial, hipStream_t st);
 void launch_sum_partials(const double* partial, int nparts, int width, double* out, hipStream_t st);
+void launch_sum_partials_scaled(const double* partial, int nparts, int width, double scale, double* out,
+                                hipStream_t st);
 void launch_timestep_reduce(int dim, const Consts& c, DevState s, const int32_t* part_id, int64_t n,
                             double* partial, int nparts, double* out, hipStream_t st);
 void launch_fix_rho(DevState s, const int32_t* part_id, int64_t nb, int64_t n, double rho0, hipStream_t st);

@@ -1,0 +1,1304 @@
+/*
+ * Block-tiled pair passes of the WCSPH step.
+ *
+ * Work decomposition: a block is a chunk of C consecutive cells of one cell
+ * row; its targets are the particles of those cells (enumerated through the
+ * cell-sorted order[] array) and its candidate union is the 3^(D-1)
+ * neighbour rows over cells [cx0 - 1, cx1].  The union is staged in LDS in
+ * tiles with coalesced loads; each lane walks its own neighbour list, which
+ * holds union-local 16-bit indices in ascending order, tile by tile.  Pair
+ * terms never scatter: every pass is owner-computes.
+ *
+ * Reference correspondence (file:line):
+ *   k_build_t      update_neighbours            src/Neighbours.cpp:7-47
+ *   k_dsph_t       dSPH_PreStep                 src/Shifting.cpp:12-123
+ *   k_surface1_t   Detect_Surface loop 1        src/Geometry.cpp:26-143
+ *                  + dissipation_terms          src/Shifting.cpp:126-186
+ *   k_surface2_t   Detect_Surface loops 2, 3    src/Geometry.cpp:145-277
+ *   k_shift_t      particle_shift               src/Shifting.cpp:189-290
+ *   k_bnd_*        boundary solvers             src/Resid.cpp:21-186, src/Newmark_Beta.cpp:69-132
+ *   k_force_t      get_acc_and_Rrho_on_i        src/Resid.cpp:243-418
+ *                  + Do_NB_Iter update          src/Newmark_Beta.cpp:137-240
+ */
+#include "device_common.hpp"
+
+namespace fj {
+
+constexpr int MAXSEG = 9;
+constexpr int BIG_U = 0x7fffffff;
+
+/* Union of block b (see file header). seg_a[k] is the order[] start of
+ * segment k, pre[k] its first union index; the target positions in order[]
+ * are [p0, p1). */
+template <int D>
+__device__ void union_of_block(const TileGrid& tg, int64_t b, int32_t* seg_a, int32_t* pre, int& nseg, int& p0,
+                               int& p1)
+{
+    const int64_t row = b / tg.nchunk;
+    const int q = (int)(b - row * tg.nchunk);
+    const int cx0 = q * tg.C;
+    const int cx1 = min(cx0 + tg.C, tg.dims[0]);
+    const int64_t rbase = row * tg.dims[0];
+    p0 = tg.start[rbase + cx0];
+    p1 = tg.start[rbase + cx1];
+    const int cy = (int)(row % tg.dims[1]);
+    const int cz = (D == 3) ? (int)(row / tg.dims[1]) : 0;
+    const int xa = max(cx0 - 1, 0), xb = min(cx1, tg.dims[0] - 1);
+    nseg = 0;
+    pre[0] = 0;
+    const int zlo = (D == 3) ? max(cz - 1, 0) : 0, zhi = (D == 3) ? min(cz + 1, tg.dims[2] - 1) : 0;
+    for (int z = zlo; z <= zhi; ++z)
+        for (int y = max(cy - 1, 0); y <= min(cy + 1, tg.dims[1] - 1); ++y) {
+            const int64_t r2 = ((int64_t)z * tg.dims[1] + y) * tg.dims[0];
+            const int32_t a = tg.start[r2 + xa], e = tg.start[r2 + xb + 1];
+            seg_a[nseg] = a;
+            pre[nseg + 1] = pre[nseg] + (e - a);
+            ++nseg;
+        }
+}
+
+__device__ __forceinline__ int32_t union_slot(const TileGrid& tg, const int32_t* seg_a, const int32_t* pre, int nseg,
+                                              int u)
+{
+    int k = 0;
+    while (k + 1 < nseg && u >= pre[k + 1]) ++k;
+    return tg.order[seg_a[k] + (u - pre[k])];
+}
+
+/* Block id of the block that owns slot i at the last build. */
+__device__ __forceinline__ int64_t block_of_slot(const TileGrid& tg, int64_t i)
+{
+    const int32_t lin = tg.cell[i];
+    const int64_t row = lin / tg.dims[0];
+    const int cx = lin - (int)(row * tg.dims[0]);
+    return row * tg.nchunk + cx / tg.C;
+}
+
+/* Shared block prologue: union table in LDS, uniform p0/p1/U. */
+struct BlockHead {
+    int32_t seg_a[MAXSEG];
+    int32_t pre[MAXSEG + 1];
+    int32_t nseg, p0, p1;
+};
+
+template <int D>
+__device__ __forceinline__ void block_head(const TileGrid& tg, BlockHead& h)
+{
+    if (threadIdx.x == 0) {
+        int nseg, p0, p1;
+        union_of_block<D>(tg, blockIdx.x, h.seg_a, h.pre, nseg, p0, p1);
+        h.nseg = nseg;
+        h.p0 = p0;
+        h.p1 = p1;
+    }
+    __syncthreads();
+}
+
+/* List entry e of slot i: union index and frozen distance payload. */
+struct ListCursor {
+    int k, cnt, u;
+    __device__ __forceinline__ void init(const NbrList& nl, int64_t i, bool active)
+    {
+        k = 0;
+        cnt = active ? nl.count[i] : 0;
+        u = cnt > 0 ? int(nl.lu[i]) : BIG_U;
+    }
+    __device__ __forceinline__ int next_u(const NbrList& nl, int64_t i) const
+    {
+        return (k + 1 < cnt) ? int(nl.lu[(int64_t)(k + 1) * nl.npad + i]) : BIG_U;
+    }
+};
+
+__device__ __forceinline__ double payload_r(const NbrList& nl, int k, int64_t i, double& d2)
+{
+    const int64_t e = (int64_t)k * nl.npad + i;
+    if (nl.f32) {
+        const double r = double(((const float*)nl.r)[e]);
+        d2 = r * r;
+        return r;
+    }
+    d2 = ((const double*)nl.r)[e];
+    return sqrt(d2);
+}
+
+/* ------------------------------------------------------------ list build */
+
+constexpr int BUILD_TILE = 1024;
+
+template <int D, typename RT>
+__global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, TileGrid tg, double sr, NbrList nl,
+                                                 DevScalars* sc)
+{
+    __shared__ double tp[D][BUILD_TILE];
+    __shared__ int32_t tsl[BUILD_TILE];
+    __shared__ BlockHead h;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    if (h.p0 == h.p1) return;
+    if (U > 65535) {
+        if (threadIdx.x == 0) atomicMax(&sc->overflow, U);
+        return;
+    }
+    RT* pay = (RT*)nl.r;
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.order[p] : 0;
+        double xi[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) xi[a] = active ? x[i * D + a] : 0.0;
+        int cnt = 0;
+        for (int base = 0; base < U; base += BUILD_TILE) {
+            const int nt = min(BUILD_TILE, U - base);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+                tsl[t] = sl;
+#pragma unroll
+                for (int a = 0; a < D; ++a) tp[a][t] = x[(int64_t)sl * D + a];
+            }
+            __syncthreads();
+            if (active) {
+                for (int t = 0; t < nt; ++t) {
+                    double xj[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
+                    const double d2 = exact_dist2<D>(xi, xj);
+                    if (d2 < sr && tsl[t] != (int32_t)i) {
+                        if (cnt < nl.cap) {
+                            const int64_t e = (int64_t)cnt * nl.npad + i;
+                            nl.lu[e] = (uint16_t)(base + t);
+                            pay[e] = (sizeof(RT) == 4) ? RT(sqrt(d2)) : RT(d2);
+                        }
+                        ++cnt;
+                    }
+                }
+            }
+        }
+        if (active) {
+            nl.count[i] = cnt;
+            atomicMax(&sc->max_count, cnt);
+        }
+    }
+}
+
+void launch_build_lists(int dim, const double* x, int64_t n, TileGrid tg, double sr, NbrList nl, DevScalars* sc,
+                        hipStream_t st)
+{
+    (void)n;
+    const dim3 g((unsigned)tg.nblocks), b(TPB);
+    if (dim == 2) {
+        if (nl.f32) hipLaunchKernelGGL((k_build_t<2, float>), g, b, 0, st, x, tg, sr, nl, sc);
+        else hipLaunchKernelGGL((k_build_t<2, double>), g, b, 0, st, x, tg, sr, nl, sc);
+    } else {
+        if (nl.f32) hipLaunchKernelGGL((k_build_t<3, float>), g, b, 0, st, x, tg, sr, nl, sc);
+        else hipLaunchKernelGGL((k_build_t<3, double>), g, b, 0, st, x, tg, sr, nl, sc);
+    }
+}
+
+/* Global slot of every list entry (for the host dump of the lists). */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_export(TileGrid tg, NbrList nl, int64_t n, int32_t* out)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n) return;
+    int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
+    int nseg, p0, p1;
+    union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
+    const int cnt = min(nl.count[i], nl.cap);
+    for (int k = 0; k < cnt; ++k) {
+        const int64_t e = (int64_t)k * nl.npad + i;
+        out[e] = union_slot(tg, seg_a, pre, nseg, nl.lu[e]);
+    }
+}
+
+void launch_export_lists(int dim, TileGrid tg, NbrList nl, int64_t n, int32_t* out, hipStream_t st)
+{
+    if (dim == 2) hipLaunchKernelGGL(k_export<2>, dim3(grid_of(n)), dim3(TPB), 0, st, tg, nl, n, out);
+    else hipLaunchKernelGGL(k_export<3>, dim3(grid_of(n)), dim3(TPB), 0, st, tg, nl, n, out);
+}
+
+/* ------------------------------------------------------------ dSPH_PreStep */
+
+constexpr int DSPH_TILE = 1024;
+
+template <int D>
+__device__ __forceinline__ void aero_velocity_one(const Consts& c, DevState& s, int64_t i, double lam_nb, int cnt_with_self)
+{
+    const int32_t b = s.b[i];
+    const bool cond = c.use_lam ? (lam_nb < c.lam_cutoff && b == FJSPH_FREE)
+                                : (double(cnt_with_self) * c.i_n_full < c.lam_cutoff && b == FJSPH_FREE);
+#pragma unroll
+    for (int k = 0; k < D; ++k) s.cellV[i * D + k] = cond ? c.v_inf[k] : 0.0;
+    s.cellID[i] = cond ? 1 : -3;
+}
+
+template <int D>
+__global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                TileGrid tg, NbrList nl, int r_from_x, int with_aero, double* partial)
+{
+    __shared__ double tp[D][DSPH_TILE];
+    __shared__ double tvol[DSPH_TILE], trho[DSPH_TILE];
+    __shared__ int32_t tnb[DSPH_TILE];
+    __shared__ BlockHead h;
+    __shared__ double lds[NWAVE];
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    double npd_acc = 0.0;
+    const double inv_H = 1.0 / c.H;
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.order[p] : 0;
+        double xi[D], Lm[D][D], Lnb[D][D], gR[D], nr[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xi[a] = active ? s.x[i * D + a] : 0.0;
+            gR[a] = 0.0;
+            nr[a] = 0.0;
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) { Lm[a][b2] = 0.0; Lnb[a][b2] = 0.0; }
+        }
+        const double rhoi = active ? s.rho[i] : 0.0;
+        double kernsum = c.Wc, colour = 0.0;
+        ListCursor lc;
+        lc.init(nl, i, active);
+        for (int base = 0; base < U; base += DSPH_TILE) {
+            const int nt = min(DSPH_TILE, U - base);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+#pragma unroll
+                for (int a = 0; a < D; ++a) tp[a][t] = s.x[(int64_t)sl * D + a];
+                tvol[t] = ax.vol[sl];
+                trho[t] = s.rho[sl];
+                tnb[t] = s.b[sl] > FJSPH_PISTON;
+            }
+            __syncthreads();
+            const int lim = base + DSPH_TILE;
+            while (lc.u < lim) {
+                const int t = lc.u - base;
+                const int u_next = lc.next_u(nl, i);
+                double xj[D], Rji[D], d2, r;
+#pragma unroll
+                for (int a = 0; a < D; ++a) { xj[a] = tp[a][t]; Rji[a] = xj[a] - xi[a]; }
+                if (r_from_x) { d2 = exact_dist2<D>(xi, xj); r = sqrt(d2); }
+                else r = payload_r(nl, lc.k, i, d2);
+                const double volj = tvol[t];
+                double W, fg;
+                wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
+                double G[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) G[a] = -fg * Rji[a];
+                const double drho = volj * (trho[t] - rhoi);
+                const bool nb = tnb[t] != 0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const double vr = volj * Rji[a];
+#pragma unroll
+                    for (int b2 = 0; b2 < D; ++b2) {
+                        const double tt = vr * G[b2];
+                        Lm[a][b2] -= tt;
+                        if (nb) Lnb[a][b2] -= tt;
+                    }
+                    gR[a] += drho * G[a];
+                }
+                if (nb) {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) nr[a] += volj * G[a];
+                    kernsum += W;
+                    colour += volj * W;
+                    npd_acc += W;
+                }
+                ++lc.k;
+                lc.u = u_next;
+            }
+        }
+        if (active) {
+            double Lw[D][D], Linv[D][D], tmp[D][D];
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b2 = 0; b2 < D; ++b2) { Lw[a][b2] = Lm[a][b2]; Linv[a][b2] = (a == b2) ? 1.0 : 0.0; }
+            if (qr_inverse<D>(Lw, tmp)) {
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int b2 = 0; b2 < D; ++b2) Linv[a][b2] = tmp[a][b2];
+            }
+            double lam, lam_nb;
+            if (s.b[i] == FJSPH_BOUND) {
+                lam = 1.0;
+                lam_nb = 1.0;
+            } else {
+                lam = min_eig(Lm);
+                lam_nb = min_eig(Lnb);
+            }
+            s.lam[i] = lam;
+            s.lam_nb[i] = lam_nb;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                double g = 0.0, q = 0.0;
+#pragma unroll
+                for (int b2 = 0; b2 < D; ++b2) {
+                    g += Linv[a][b2] * gR[b2];
+                    q += Linv[a][b2] * nr[b2];
+                    s.L[i * D * D + a * D + b2] = Linv[a][b2];
+                }
+                s.gradRho[i * D + a] = g;
+                s.norm[i * D + a] = q;
+            }
+            s.colourG[i] = lam > 0.7 ? 2.0 : 2.0 * fmax(1.0, 1.0 / (2.0 * colour));
+            s.kernsum[i] = kernsum;
+            s.colour[i] = colour;
+            if (with_aero && c.aero_source == FJSPH_CONST_VEL && part_id[i] >= c.n_bound)
+                aero_velocity_one<D>(c, s, i, lam_nb, lc.cnt + 1);
+        }
+    }
+    const double bs = block_reduce(npd_acc, OpSum(), lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+
+void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                 int64_t n, int r_from_x, int with_aero_vel, double* partial, DevScalars* sc, hipStream_t st)
+{
+    const dim3 g((unsigned)tg.nblocks), b(TPB);
+    if (dim == 2) hipLaunchKernelGGL(k_dsph_t<2>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, with_aero_vel, partial);
+    else hipLaunchKernelGGL(k_dsph_t<3>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, with_aero_vel, partial);
+    launch_sum_partials_scaled(partial, (int)tg.nblocks, 1, 1.0 / double(n), &sc->npd, st);
+}
+
+template <int D>
+__global__ __launch_bounds__(TPB) void k_aero_vel(Consts c, DevState s, const int32_t* __restrict__ part_id,
+                                                  NbrList nl, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n || part_id[i] < c.n_bound) return;
+    aero_velocity_one<D>(c, s, i, s.lam_nb[i], nl.count[i] + 1);
+}
+
+void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
+                          hipStream_t st)
+{
+    if (c.aero_source != FJSPH_CONST_VEL) return;
+    if (dim == 2) hipLaunchKernelGGL(k_aero_vel<2>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, part_id, nl, n);
+    else hipLaunchKernelGGL(k_aero_vel<3>, dim3(grid_of(n)), dim3(TPB), 0, st, c, s, part_id, nl, n);
+}
+
+/* -------------------------------------- Detect_Surface + dissipation_terms */
+
+constexpr int SURF1_TILE = 512;
+
+/* mode 0: Detect_Surface loop 1 only; 1: with dissipation_terms;
+ * 2: dissipation_terms only. */
+template <int D, typename T>
+__global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                    TileGrid tg, NbrList nl, int r_from_x, double* norms_tmp,
+                                                    int mode)
+{
+    const bool with_surf = mode != 2;
+    const bool with_diss = mode != 0;
+    __shared__ double tp[D][SURF1_TILE], tv[D][SURF1_TILE], tg_[D][SURF1_TILE];
+    __shared__ double tvol[SURF1_TILE], tlam[SURF1_TILE], trho[SURF1_TILE], tir[SURF1_TILE], tm[SURF1_TILE];
+    __shared__ int32_t tnb[SURF1_TILE];
+    __shared__ BlockHead h;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    const double inv_H = 1.0 / c.H;
+    const double h133 = 1.33 * c.particle_step;
+    const double r_lim = sqrt(2.0) * h133;
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.order[p] : 0;
+        const bool fluid = active && part_id[i] >= c.n_bound;
+        double xi[D], ni[D], nhat[D], tau_hat[D], pT[D], vi[D], gri[D], nrm[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xi[a] = fluid ? s.x[i * D + a] : 0.0;
+            ni[a] = fluid ? s.norm[i * D + a] : 0.0;
+            vi[a] = fluid ? s.v[i * D + a] : 0.0;
+            gri[a] = fluid ? s.gradRho[i * D + a] : 0.0;
+            nrm[a] = 0.0;
+            tau_hat[a] = 0.0;
+        }
+        normalize<D>(ni, nhat);
+        if (D == 2) {
+            double tau[D];
+            tau[0] = ni[1 % D];
+            tau[1 % D] = -ni[0];
+            normalize<D>(tau, tau_hat);
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) pT[a] = xi[a] + h133 * nhat[a];
+        const int32_t bi = fluid ? s.b[i] : 0;
+        const double lam_nb = fluid ? s.lam_nb[i] : 1.0;
+        const double lami = fluid ? s.lam[i] : 1.0;
+        const double rhoi = fluid ? s.rho[i] : 1.0;
+        const bool testing = with_surf && fluid && bi >= FJSPH_PIPE && lam_nb >= 0.2 && lam_nb < 0.75;
+        bool found = false;
+        const bool high = lami > 0.7;
+        /* dissipation accumulators (src/Shifting.cpp:135-183) */
+        double av[D], rd = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) av[a] = 0.0;
+        const T Bg = T(c.B * c.gam), cfac = T(sqrt(c.B * c.gam / rhoi)), Ht = T(c.H);
+        const T eps2 = T(0.0001 * c.H_sq), alpha = T(c.visc_alpha);
+        ListCursor lc;
+        lc.init(nl, i, fluid);
+        for (int base = 0; base < U; base += SURF1_TILE) {
+            const int nt = min(SURF1_TILE, U - base);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    tp[a][t] = s.x[(int64_t)sl * D + a];
+                    tv[a][t] = s.v[(int64_t)sl * D + a];
+                    tg_[a][t] = s.gradRho[(int64_t)sl * D + a];
+                }
+                tvol[t] = ax.vol[sl];
+                tlam[t] = s.lam[sl];
+                trho[t] = s.rho[sl];
+                tir[t] = ax.invrho[sl];
+                tm[t] = s.m[sl];
+                tnb[t] = s.b[sl] > FJSPH_PISTON;
+            }
+            __syncthreads();
+            const int lim = base + SURF1_TILE;
+            while (lc.u < lim) {
+                const int t = lc.u - base;
+                const int u_next = lc.next_u(nl, i);
+                double xj[D], d2, r;
+#pragma unroll
+                for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
+                if (r_from_x) { d2 = exact_dist2<D>(xi, xj); r = sqrt(d2); }
+                else r = payload_r(nl, lc.k, i, d2);
+                if (testing && !found) {
+                    /* Marrone et al. (2010) point-T test, src/Geometry.cpp:43-98;
+                     * the outcome does not depend on the order of the pairs */
+                    double xjT[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) xjT[a] = xj[a] - pT[a];
+                    if (r >= r_lim) {
+                        if (sqrt(dotd<D>(xjT, xjT)) < h133) found = true;
+                    } else if (D == 2) {
+                        if (fabs(dotd<D>(nhat, xjT)) + fabs(dotd<D>(tau_hat, xjT)) < h133) found = true;
+                    } else {
+                        double mr[D];
+#pragma unroll
+                        for (int a = 0; a < D; ++a) mr[a] = -(xi[a] - xj[a]) / r;
+                        if (acos(dotd<D>(nhat, mr)) < kPi / 4.0) found = true;
+                    }
+                }
+                double W, fg;
+                wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
+                if (with_surf) {
+                    const double f = tvol[t] * (high ? (tlam[t] - lami) : tlam[t]) * fg;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) nrm[a] += f * (xi[a] - xj[a]);
+                }
+                if (with_diss) {
+                    T Rji[D], Vji[D], g[D];
+                    T fgt;
+                    if (sizeof(T) == 8) fgt = T(fg);
+                    else { T Wt; wendland<T>(T(r), Ht, T(inv_H), T(c.Wc), Wt, fgt); }
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        Rji[a] = T(xj[a] - xi[a]);
+                        Vji[a] = T(tv[a][t] - vi[a]);
+                        g[a] = fgt * Rji[a];
+                    }
+                    const T idist2 = fast_rcp<T>(T(d2) + eps2);
+                    const double rhoj = trho[t];
+                    const T volj = T(tvol[t]);
+                    T rdotg = Rji[0] * g[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rdotg += Rji[a] * g[a];
+                    const T drho = T(rhoj - rhoi);
+                    if (tnb[t]) {
+                        T vdotr = Vji[0] * Rji[0];
+#pragma unroll
+                        for (int a = 1; a < D; ++a) vdotr += Vji[a] * Rji[a];
+                        if (!(vdotr > T(0))) {
+                            const T muij = Ht * vdotr * idist2;
+                            const T cbar = T(0.5) * (cfac + sqrt(Bg * T(tir[t])));
+                            const T ff = T(tm[t]) * alpha * cbar * muij * fast_rcp<T>(T(0.5 * (rhoi + rhoj)));
+#pragma unroll
+                            for (int a = 0; a < D; ++a) av[a] += double(g[a] * ff);
+                        }
+                        T gsum = T(0);
+#pragma unroll
+                        for (int a = 0; a < D; ++a) gsum += T(gri[a] + tg_[a][t]) * Rji[a];
+                        rd += double(volj * (drho + T(0.5) * gsum) * rdotg * idist2);
+                    } else {
+                        rd += double(volj * drho * rdotg * idist2);
+                    }
+                }
+                ++lc.k;
+                lc.u = u_next;
+            }
+        }
+        if (!active) continue;
+        if (!fluid) {
+            if (with_surf) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = 0.0;
+            }
+            continue;
+        }
+        if (with_diss) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) s.aVisc[i * D + a] = av[a];
+            s.deltaD[i] = c.dsph_cont * rd;
+        }
+        if (!with_surf) continue;
+        int surf;
+        if (bi < FJSPH_PIPE) surf = 0;
+        else if (lam_nb < 0.2) surf = 1;
+        else if (lam_nb < 0.75) surf = found ? 0 : 1;
+        else surf = 0;
+        s.surf[i] = surf;
+        if (bi < FJSPH_PIPE) s.woccl[i] = 1.0;
+        double Ln[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            double t = 0.0;
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) t += s.L[i * D * D + a * D + b2] * nrm[b2];
+            Ln[a] = t;
+        }
+        double out[D];
+        if (sqrt(dotd<D>(Ln, Ln)) > 0.1 * lami / c.H) normalize<D>(Ln, out);
+        else {
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[a] = 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = out[a];
+    }
+}
+
+void launch_surface1(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg,
+                     NbrList nl, int64_t n, int r_from_x, double* norms_tmp, int mode, hipStream_t st)
+{
+    (void)n;
+    const dim3 g((unsigned)tg.nblocks), b(TPB);
+    if (dim == 2) {
+        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_surface1_t<2, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode);
+        else hipLaunchKernelGGL((k_surface1_t<2, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode);
+    } else {
+        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_surface1_t<3, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode);
+        else hipLaunchKernelGGL((k_surface1_t<3, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode);
+    }
+}
+
+/* Curvature, Gissler occlusion and surfzone (src/Geometry.cpp:145-277). */
+constexpr int SURF2_TILE = 512;
+
+template <int D>
+__global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                    TileGrid tg, NbrList nl, int r_from_x,
+                                                    const double* __restrict__ norms_tmp)
+{
+    __shared__ double tp[D][SURF2_TILE], tn[D][SURF2_TILE];
+    __shared__ double tvol[SURF2_TILE];
+    __shared__ int32_t tsurf[SURF2_TILE];
+    __shared__ BlockHead h;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    const double inv_H = 1.0 / c.H;
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.order[p] : 0;
+        const bool fluid = active && part_id[i] >= c.n_bound;
+        double xi[D], ni[D], Vd[D], Li[D][D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xi[a] = fluid ? s.x[i * D + a] : 0.0;
+            ni[a] = fluid ? norms_tmp[i * D + a] : 0.0;
+            Vd[a] = 0.0;
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) Li[a][b2] = fluid ? s.L[i * D * D + a * D + b2] : 0.0;
+        }
+        const int32_t bi = fluid ? s.b[i] : 0;
+        const bool occl = fluid && (c.aero_case == FJSPH_GISSLER) && bi == FJSPH_FREE;
+        if (occl && s.cellID[i] != -1) {
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+                Vd[a] = (c.aero_source == FJSPH_MESH_INFL ? s.cellV[i * D + a] : c.v_inf[a]) - s.v[i * D + a];
+        }
+        const double Vdn = sqrt(dotd<D>(Vd, Vd));
+        const bool ni_nz = dotd<D>(ni, ni) > 0.0;
+        double curve = 0.0, qmax = 0.0;
+        int zone = (fluid && s.surf[i] == 1) ? 1 : 0;
+        ListCursor lc;
+        lc.init(nl, i, fluid);
+        for (int base = 0; base < U; base += SURF2_TILE) {
+            const int nt = min(SURF2_TILE, U - base);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    tp[a][t] = s.x[(int64_t)sl * D + a];
+                    tn[a][t] = norms_tmp[(int64_t)sl * D + a];
+                }
+                tvol[t] = ax.vol[sl];
+                tsurf[t] = s.surf[sl];
+            }
+            __syncthreads();
+            const int lim = base + SURF2_TILE;
+            while (lc.u < lim) {
+                const int t = lc.u - base;
+                const int u_next = lc.next_u(nl, i);
+                double xj[D], d2, r, Rij[D], nj[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) { xj[a] = tp[a][t]; Rij[a] = xj[a] - xi[a]; nj[a] = tn[a][t]; }
+                if (r_from_x) { d2 = exact_dist2<D>(xi, xj); r = sqrt(d2); }
+                else r = payload_r(nl, lc.k, i, d2);
+                if (ni_nz && dotd<D>(nj, nj) > 0.0) {
+                    double W, fg;
+                    wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
+                    double tt = 0.0;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        double la = 0.0;
+#pragma unroll
+                        for (int b2 = 0; b2 < D; ++b2) la += Li[a][b2] * (nj[b2] - ni[b2]);
+                        tt += la * (fg * Rij[a]);
+                    }
+                    curve += tvol[t] * tt;
+                }
+                if (occl) {
+                    /* frac = -Rij.Vd / (|Vd| r): the max is taken over
+                     * -Rij.Vd / r and divided by |Vd| once */
+                    const double q = -dotd<D>(Rij, Vd) / r;
+                    if (q > qmax) qmax = q;
+                }
+                if (tsurf[t] == 1) zone = 1;
+                ++lc.k;
+                lc.u = u_next;
+            }
+        }
+        if (!fluid) continue;
+        const double wo = (occl && Vdn > 0.0) ? qmax / Vdn : 0.0;
+        s.woccl[i] = s.lam_nb[i] < c.lam_cutoff ? fmax(0.0, fmin(wo, 1.0)) : 1.0;
+        double nh[D];
+        normalize<D>(ni, nh);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            s.norm[i * D + a] = ni[a];
+            ax.nhat[i * D + a] = nh[a];
+        }
+        s.curve[i] = curve;
+        s.norm_curve[i] = c.dx * curve;
+        s.pDist[i] = s.lam[i];
+        if (c.ale) s.surfzone[i] = zone;
+    }
+}
+
+void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                     int64_t n, int r_from_x, const double* norms_tmp, hipStream_t st)
+{
+    (void)n;
+    const dim3 g((unsigned)tg.nblocks), b(TPB);
+    if (dim == 2) hipLaunchKernelGGL(k_surface2_t<2>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp);
+    else hipLaunchKernelGGL(k_surface2_t<3>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp);
+}
+
+/* ------------------------------------------------------------ particle_shift */
+
+/* src/Shifting.cpp:189-290.  max_j acos(n_i.n_j) is taken as acos(min_j
+ * n_i.n_j) over the dot products acos accepts ([-1, 1]): NaN angles never
+ * win the reference's max and acos is decreasing, so the result is the same.
+ * max_j |v_j - v_i| is taken on squared norms. */
+constexpr int SHIFT_TILE = 512;
+
+template <int D>
+__global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                 TileGrid tg, NbrList nl, int r_from_x)
+{
+    __shared__ double tp[D][SHIFT_TILE], tv[D][SHIFT_TILE], tn[D][SHIFT_TILE];
+    __shared__ double tvol[SHIFT_TILE];
+    __shared__ int32_t tnb[SHIFT_TILE];
+    __shared__ BlockHead h;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    const double inv_H = 1.0 / c.H;
+    const double inv_Wdx = 1.0 / c.W_dx;
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.order[p] : 0;
+        const bool fluid = active && part_id[i] >= c.n_bound;
+        const double lam_nb = fluid ? s.lam_nb[i] : 0.0;
+        const bool work = fluid && !(lam_nb < 0.55 || s.b[i] == FJSPH_BUFFER);
+        double xi[D], vi[D], nhat[D], du[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xi[a] = work ? s.x[i * D + a] : 0.0;
+            vi[a] = work ? s.v[i * D + a] : 0.0;
+            nhat[a] = work ? ax.nhat[i * D + a] : 0.0;
+            du[a] = 0.0;
+        }
+        double maxU2 = 0.0, dmin = 2.0;
+        ListCursor lc;
+        lc.init(nl, i, work);
+        for (int base = 0; base < U; base += SHIFT_TILE) {
+            const int nt = min(SHIFT_TILE, U - base);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    tp[a][t] = s.x[(int64_t)sl * D + a];
+                    tv[a][t] = s.v[(int64_t)sl * D + a];
+                    tn[a][t] = ax.nhat[(int64_t)sl * D + a];
+                }
+                tvol[t] = ax.vol[sl];
+                tnb[t] = s.b[sl] > FJSPH_PISTON;
+            }
+            __syncthreads();
+            const int lim = base + SHIFT_TILE;
+            while (lc.u < lim) {
+                const int t = lc.u - base;
+                const int u_next = lc.next_u(nl, i);
+                double xj[D], d2, r;
+#pragma unroll
+                for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
+                if (r_from_x) { d2 = exact_dist2<D>(xi, xj); r = sqrt(d2); }
+                else r = payload_r(nl, lc.k, i, d2);
+                double W, fg;
+                wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
+                const double w = W * inv_Wdx;
+                const double w2 = w * w;
+                const double f = (1.0 + 0.2 * (w2 * w2)) * fg * tvol[t];
+                double dv2 = 0.0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    du[a] += f * (xj[a] - xi[a]);
+                    const double dv = tv[a][t] - vi[a];
+                    dv2 += dv * dv;
+                }
+                if (tnb[t]) {
+                    double d = 0.0;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) d += nhat[a] * tn[a][t];
+                    if (d >= -1.0 && d <= 1.0 && d < dmin) dmin = d;
+                }
+                if (dv2 > maxU2) maxU2 = dv2;
+                ++lc.k;
+                lc.u = u_next;
+            }
+        }
+        if (!fluid) continue;
+        double out[D];
+        if (!work) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[a] = 0.0;
+        } else {
+            const double wo = dmin <= 1.0 ? acos(dmin) : 0.0;
+            const double fac = -2.0 * c.H * sqrt(dotd<D>(vi, vi));
+#pragma unroll
+            for (int a = 0; a < D; ++a) du[a] *= fac;
+            double dun[D];
+            normalize<D>(du, dun);
+            const double mag = fmin(sqrt(dotd<D>(du, du)), fmin(sqrt(maxU2) / 2.0, c.max_shift_vel));
+#pragma unroll
+            for (int a = 0; a < D; ++a) du[a] = mag * dun[a];
+            if (s.surfzone[i] == 0 && lam_nb > 0.55) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) out[a] = du[a];
+            } else if (wo < kPi / 12.0) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int b2 = 0; b2 < D; ++b2) t += (((a == b2) ? 1.0 : 0.0) - nhat[a] * nhat[b2]) * du[b2];
+                    out[a] = t;
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < D; ++a) out[a] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) s.vPert[i * D + a] = out[a];
+    }
+}
+
+void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                  int64_t n, int r_from_x, hipStream_t st)
+{
+    (void)n;
+    const dim3 g((unsigned)tg.nblocks), b(TPB);
+    if (dim == 2) hipLaunchKernelGGL(k_shift_t<2>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x);
+    else hipLaunchKernelGGL(k_shift_t<3>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x);
+}
+
+/* ------------------------------------------------------- boundary solvers */
+
+/* The boundary passes touch only boundary particles (a thin layer), so they
+ * gather per lane through the union table instead of staging tiles. */
+template <int D>
+__device__ __forceinline__ void lane_union(const TileGrid& tg, int64_t i, int32_t* seg_a, int32_t* pre, int& nseg)
+{
+    int p0, p1;
+    union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
+}
+
+/* Block velocity + no-slip (src/Newmark_Beta.cpp:69-102, src/Resid.cpp:162-186). */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int32_t* __restrict__ blk, DevBlocks bl,
+                                                 int32_t block, TileGrid tg, NbrList nl, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n || blk[i] != block) return;
+    double v[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) v[a] = bl.vel[block * 3 + a];
+    if (bl.no_slip[block]) {
+        int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
+        int nseg;
+        lane_union<D>(tg, i, seg_a, pre, nseg);
+        const double inv_H = 1.0 / c.H;
+        double vs[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) vs[a] = 0.0;
+        double ks = 0.0;
+        const int cnt = nl.count[i];
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t j = union_slot(tg, seg_a, pre, nseg, nl.lu[(int64_t)k * nl.npad + i]);
+            if (s.b[j] <= FJSPH_PISTON) continue;
+            double d2;
+            const double r = payload_r(nl, k, i, d2);
+            double W, fg;
+            wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
+            ks += W;
+#pragma unroll
+            for (int a = 0; a < D; ++a) vs[a] += s.v[(int64_t)j * D + a] * W;
+        }
+        if (ks > 0.0) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) v[a] = 2.0 * v[a] - vs[a] / ks;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) s.v[i * D + a] = v[a];
+}
+
+/* Boundary solver of one block (src/Resid.cpp:21-160). */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax, const int32_t* __restrict__ blk,
+                                                   DevBlocks bl, int32_t block, TileGrid tg, NbrList nl, int64_t n,
+                                                   int32_t* near_inlet)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n || blk[i] != block) return;
+    const int solver = bl.solver[block];
+    const double inv_H = 1.0 / c.H;
+    const int cnt = nl.count[i];
+    double xi[D];
+    load_vec<D>(s.x, i, xi);
+    int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
+    int nseg;
+    lane_union<D>(tg, i, seg_a, pre, nseg);
+    if (solver == FJSPH_DBC) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) s.acc[i * D + a] = 0.0;
+    } else if (solver == FJSPH_PRESSURE_G) {
+        int near = 0;
+        double ks = 0.0, pk = 0.0, ak[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) ak[a] = 0.0;
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t j = union_slot(tg, seg_a, pre, nseg, nl.lu[(int64_t)k * nl.npad + i]);
+            if (s.b[j] <= FJSPH_PISTON) continue;
+            double d2;
+            const double r = payload_r(nl, k, i, d2);
+            double W, fg;
+            wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
+            const double kern = ax.vol[j] * W;
+            ks += kern;
+            pk += s.p[j] * kern;
+            const double kr = kern * s.rho[j];
+#pragma unroll
+            for (int a = 0; a < D; ++a) ak[a] += kr * (xi[a] - s.x[(int64_t)j * D + a]);
+            if (s.surfzone[j]) near = 1;
+        }
+        double pres = 0.0;
+        if (ks > 0.0) {
+            double ga = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) ga += (c.grav[a] - s.acc[i * D + a]) * ak[a];
+            const double pp = (pk + ga) / ks;
+            pres = near ? fmax(0.0, pp) : pp;
+        }
+        s.p[i] = pres;
+        s.rho[i] = eos_density(c, pres);
+    } else if (solver == FJSPH_GHOST) {
+        int ni = 1;
+        double rr = 0.0, vi[D];
+        load_vec<D>(s.v, i, vi);
+        for (int k = 0; k < cnt; ++k) {
+            const int32_t j = union_slot(tg, seg_a, pre, nseg, nl.lu[(int64_t)k * nl.npad + i]);
+            const int32_t bj = s.b[j];
+            if (bj == FJSPH_PIPE || bj == FJSPH_FREE) ni = 0;
+            double d2;
+            const double r = payload_r(nl, k, i, d2);
+            double W, fg;
+            wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+                t += (s.v[(int64_t)j * D + a] - vi[a]) * (fg * (s.x[(int64_t)j * D + a] - xi[a]));
+            rr -= ax.vol[j] * t;
+        }
+        near_inlet[i] = ni;
+        s.Rrho[i] = rr * s.rho[i];
+    }
+}
+
+void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t* blk, DevBlocks bl, int32_t block,
+                     TileGrid tg, NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st)
+{
+    const dim3 g(grid_of(n)), b(TPB);
+    if (dim == 2) {
+        hipLaunchKernelGGL(k_bnd_vel<2>, g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
+        hipLaunchKernelGGL(k_bnd_solve<2>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+    } else {
+        hipLaunchKernelGGL(k_bnd_vel<3>, g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
+        hipLaunchKernelGGL(k_bnd_solve<3>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+    }
+}
+
+/* ----------------------------------------------------- force + Newmark */
+
+/* Gissler drag (src/Aero.h:37-98) in FP64 for one particle. */
+template <int D>
+__device__ void gissler(const Consts& c, const double* Vd, double rho, double press, double mass, double lam,
+                        double nneigh, double woccl, double* out)
+{
+    const double Vn = sqrt(dotd<D>(Vd, Vd));
+    const double Re = 2.0 * rho * Vn * c.L / c.mu_g;
+    const double frac2 = c.use_lam ? fmin(c.interp_fac * lam, 1.0) : fmin(c.interp_fac * nneigh * c.i_n_full, 1.0);
+    const double frac1 = 1.0 - frac2;
+    const double Cds = (1.0 + 0.197 * pow(Re, 0.63) + 2.6e-04 * pow(Re, 1.38)) * (24.0 / (Re + 0.00001));
+    double Cdl, Adrop;
+    if (c.use_TAB_def) {
+        double ymax = dotd<D>(Vd, Vd) * c.ycoef;
+        if (ymax > 1.0) ymax = 1.0;
+        Cdl = Cds * (1 + 2.632 * ymax);
+        if (D == 3) {
+            const double t = c.L + c.Cb * c.L * ymax;
+            Adrop = kPi * t * t;
+        } else {
+            Adrop = c.A_sphere + 2 * (c.Cb * c.L * ymax);
+        }
+    } else {
+        Cdl = Cds;
+        Adrop = c.A_sphere;
+    }
+    const double Cdi = frac1 * Cdl + frac2;
+    const double Ai = (1.0 - woccl) * (frac1 * Adrop + frac2 * c.A_plate);
+    const double f = 0.5 * Vn / (c.sos * c.sos) * c.gamma_g * press * Cdi * Ai / mass;
+#pragma unroll
+    for (int a = 0; a < D; ++a) out[a] = f * Vd[a];
+}
+
+struct NextState {
+    double *x, *v, *rho, *p;
+};
+
+/* LDS tile of the force pass: positions and velocities relative to the
+ * block's first target in T (exact FP64 differences in the FP64 path), and
+ * m (negative for BOUND), p/rho^2, 1/rho, m/rho; ALE adds vPert and rho. */
+template <int D, typename T, int TILE>
+struct ForceTile {
+    T px[D][TILE], vx[D][TILE];
+    T m[TILE], P[TILE], ir[TILE], vol[TILE];
+    T vp[D][TILE], rho[TILE];
+};
+
+/* get_acc_and_Rrho_on_i (src/Resid.cpp:243-418) for fluid slots, fused with
+ * the Newmark-Beta update of Do_NB_Iter (src/Newmark_Beta.cpp:137-240).
+ * Pair terms are formed in T; every sum is accumulated in FP64.  The
+ * pressure, viscous and surface-tension terms of a pair are combined before
+ * accumulation: acc_i = sum_j (visc + ST / m_i - m_j (P_i + P_j) g). */
+template <int D, typename T, int TILE>
+__global__ __launch_bounds__(TPB) void k_force_t(Consts c, DevState s, Aux ax, DevState pn, NextState nx,
+                                                 int do_update, const int32_t* __restrict__ part_id,
+                                                 const int32_t* __restrict__ blk, DevBlocks bl,
+                                                 const int32_t* __restrict__ near_inlet, TileGrid tg, NbrList nl,
+                                                 double dt, const DevScalars* sc, double* partial)
+{
+    __shared__ ForceTile<D, T, TILE> tl;
+    __shared__ BlockHead h;
+    __shared__ double lds[NWAVE];
+    constexpr bool F32 = sizeof(T) == 4;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    double resid = 0.0;
+    /* reference point of the block (first target) for the FP32 relative
+     * coordinates */
+    double xref[D], vref[D];
+    {
+        const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xref[a] = F32 ? s.x[i0 * D + a] : 0.0;
+            vref[a] = F32 ? s.v[i0 * D + a] : 0.0;
+        }
+    }
+    const double npd = sc->npd;
+    const T ste = T(-(0.5 * c.sig / c.lam_st) / (npd * npd));
+    const T pi3o4H = T(3.0 * kPi / 4.0 / c.H);
+    const T inv_H = T(1.0 / c.H), Ht = T(c.H), Wc = T(c.Wc);
+    const T eps2 = T(0.001 * c.H_sq);
+    const T nu = T(c.nu);
+    const T contang_fac = T(1.0 + 0.5 * cos(0.5 * kPi * 7.0 / 9.0));
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.order[p] : 0;
+        const bool fluid = active && part_id[i] >= c.n_bound;
+        double xi[D], vi[D];
+        T xit[D], vit[D], vpi[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xi[a] = active ? s.x[i * D + a] : 0.0;
+            vi[a] = active ? s.v[i * D + a] : 0.0;
+            xit[a] = T(xi[a] - xref[a]);
+            vit[a] = T(vi[a] - vref[a]);
+            vpi[a] = (fluid && c.ale) ? T(s.vPert[i * D + a]) : T(0);
+        }
+        const double rhoi = fluid ? s.rho[i] : 1.0, mi = fluid ? s.m[i] : 1.0;
+        const int32_t bi = fluid ? s.b[i] : 0;
+        const T Pi = fluid ? T(ax.P[i]) : T(0);
+        const T iri = fluid ? T(ax.invrho[i]) : T(0);
+        const T inv_mi = T(1.0 / mi);
+        const T rhoit = T(rhoi);
+        const bool st_on = (!c.ale) || (fluid && s.surfzone[i] == 1);
+        const bool bound_i = bi == FJSPH_BOUND;
+        double acc[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) acc[a] = 0.0;
+        double Rrho_ = 0.0, Rrhoc_ = 0.0;
+        ListCursor lc;
+        lc.init(nl, i, fluid);
+        for (int base = 0; base < U; base += TILE) {
+            const int nt = min(TILE, U - base);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    tl.px[a][t] = T(s.x[(int64_t)sl * D + a] - xref[a]);
+                    tl.vx[a][t] = T(s.v[(int64_t)sl * D + a] - vref[a]);
+                }
+                if (F32) {
+                    const float4 q = ax.fq[sl];
+                    tl.m[t] = T(q.x);
+                    tl.P[t] = T(q.y);
+                    tl.ir[t] = T(q.z);
+                    tl.vol[t] = T(q.w);
+                } else {
+                    tl.m[t] = T(s.b[sl] == FJSPH_BOUND ? -s.m[sl] : s.m[sl]);
+                    tl.P[t] = T(ax.P[sl]);
+                    tl.ir[t] = T(ax.invrho[sl]);
+                    tl.vol[t] = T(ax.vol[sl]);
+                }
+                if (c.ale) {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) tl.vp[a][t] = T(s.vPert[(int64_t)sl * D + a]);
+                    tl.rho[t] = T(s.rho[sl]);
+                }
+            }
+            __syncthreads();
+            const int lim = base + TILE;
+            while (lc.u < lim) {
+                const int t = lc.u - base;
+                const int u_next = lc.next_u(nl, i);
+                T r, rr;
+                {
+                    const int64_t e = (int64_t)lc.k * nl.npad + i;
+                    if (F32) {
+                        r = T(((const float*)nl.r)[e]);
+                        rr = r * r;
+                    } else {
+                        const double d2 = ((const double*)nl.r)[e];
+                        rr = T(d2);
+                        r = T(sqrt(d2));
+                    }
+                }
+                T Rji[D], Vji[D];
+                if (F32) {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) { Rji[a] = tl.px[a][t] - xit[a]; Vji[a] = tl.vx[a][t] - vit[a]; }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) { Rji[a] = tl.px[a][t] - T(xi[a]); Vji[a] = tl.vx[a][t] - T(vi[a]); }
+                }
+                const T msg = tl.m[t];
+                const bool bound_j = msg < T(0);
+                const T mj = bound_j ? -msg : msg;
+                const T Pj = tl.P[t], irj = tl.ir[t], volj = tl.vol[t];
+                const T idist2 = fast_rcp<T>(rr + eps2);
+                T W, fg;
+                wendland<T>(r, Ht, inv_H, Wc, W, fg);
+                T g[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) g[a] = fg * Rji[a];
+                const T pc = mj * (Pi + Pj);
+                T rdotg = Rji[0] * g[0];
+#pragma unroll
+                for (int a = 1; a < D; ++a) rdotg += Rji[a] * g[a];
+                const T vf = mj * nu * (iri + irj) * rdotg * idist2;
+                T sf = T(0);
+                if (st_on) {
+                    const T fac = (bound_i || bound_j) ? contang_fac : T(1);
+                    sf = ste * fac * cos(pi3o4H * r) * fast_rcp<T>(r) * inv_mi;
+                }
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc[a] += double(vf * Vji[a] + sf * Rji[a] - pc * g[a]);
+                if (c.ale) {
+                    T aj = T(0), ai = T(0), cd = T(0), cont = T(0), c2 = T(0);
+                    const T rhoj = tl.rho[t];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        const T vpj = tl.vp[a][t];
+                        aj += vpj * g[a];
+                        ai += vpi[a] * g[a];
+                        cd += (vpj - vpi[a]) * g[a];
+                        cont += (Vji[a] + (vpj - vpi[a])) * g[a];
+                        c2 += (rhoj * vpj + rhoit * vpi[a]) * g[a];
+                    }
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        const T via = F32 ? (T(vref[a]) + vit[a]) : T(vi[a]);
+                        const T vja = F32 ? (T(vref[a]) + tl.vx[a][t]) : tl.vx[a][t];
+                        acc[a] += double((vja * aj + via * ai - via * cd) * volj);
+                    }
+                    Rrho_ -= double(cont * volj);
+                    Rrhoc_ += double(c2 * volj);
+                } else {
+                    T vg = Vji[0] * g[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) vg += Vji[a] * g[a];
+                    Rrho_ -= double(volj * vg);
+                }
+                ++lc.k;
+                lc.u = u_next;
+            }
+        }
+        if (!active) continue;
+        double acc_new[D], Rrho_new = 0.0;
+        if (fluid) {
+            if (s.cellID[i] != -1) {
+                double Vd[D], aero[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) Vd[a] = s.cellV[i * D + a] - vi[a];
+                if (c.aero_case == FJSPH_GISSLER) {
+                    gissler<D>(c, Vd, s.cellRho[i], s.cellP[i], mi, s.lam_nb[i], double(lc.cnt + 1), s.woccl[i], aero);
+                } else {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) aero[a] = 0.0;
+                }
+#pragma unroll
+                for (int a = 0; a < D; ++a) { acc[a] += aero[a]; s.Af[i * D + a] = aero[a]; }
+            }
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                acc_new[a] = acc[a] + s.aVisc[i * D + a] + c.grav[a];
+                s.acc[i * D + a] = acc_new[a];
+            }
+            Rrho_new = c.ale ? (Rrho_ * rhoi + Rrhoc_ + s.deltaD[i]) : (Rrho_ * rhoi + s.deltaD[i]);
+            s.Rrho[i] = Rrho_new;
+        }
+        if (do_update) {
+            /* Newmark-Beta update (beta = 1/4, gamma = 1/2) */
+            const double g1 = 0.5, g2 = 0.5, b1 = 0.25, b2 = 0.25;
+            const double dt2 = dt * dt;
+            double xo[D], vo[D], rho_o = s.rho[i], p_o = s.p[i];
+#pragma unroll
+            for (int a = 0; a < D; ++a) { xo[a] = xi[a]; vo[a] = vi[a]; }
+            const int32_t bcur = s.b[i];
+            if (fluid) {
+                if (bcur > FJSPH_BUFFER && bcur != FJSPH_OUTLET) {
+                    double xn[D], vn[D], an[D];
+                    load_vec<D>(pn.x, i, xn);
+                    load_vec<D>(pn.v, i, vn);
+                    load_vec<D>(pn.acc, i, an);
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        double xnew;
+                        if (c.ale)
+                            xnew = xn[a] + dt * (vn[a] + s.vPert[i * D + a]) + dt2 * (b1 * acc_new[a] + b2 * an[a]);
+                        else
+                            xnew = xn[a] + dt * vn[a] + dt2 * (b2 * an[a] + b1 * acc_new[a]);
+                        const double d = xnew - xo[a];
+                        resid += d * d;
+                        xo[a] = xnew;
+                        vo[a] = vn[a] + dt * (g1 * acc_new[a] + g2 * an[a]);
+                    }
+                    rho_o = fmax(c.rho_min, fmin(c.rho_max, pn.rho[i] + dt * (g1 * Rrho_new + g2 * pn.Rrho[i])));
+                    p_o = eos_pressure(c, rho_o);
+                } else if (bcur == FJSPH_OUTLET) {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        const double xnew = pn.x[i * D + a] + dt * vo[a];
+                        const double d = xnew - xo[a];
+                        resid += d * d;
+                        xo[a] = xnew;
+                    }
+                }
+            } else {
+                const int bk = blk[i];
+                const int solver = bk >= 0 ? bl.solver[bk] : -1;
+                if (solver == FJSPH_DBC) {
+                    rho_o = fmax(c.rho_min, fmin(c.rho_max, pn.rho[i] + dt * (g1 * s.Rrho[i] + g2 * pn.Rrho[i])));
+                    p_o = eos_pressure(c, rho_o);
+                } else if (solver == FJSPH_GHOST) {
+                    const double lo = near_inlet[i] ? c.rho0 : c.rho_min;
+                    rho_o = fmax(lo, fmin(c.rho_max, pn.rho[i] + dt * (g1 * s.Rrho[i] + g2 * pn.Rrho[i])));
+                    p_o = eos_pressure(c, rho_o);
+                    if (near_inlet[i]) s.Rrho[i] = fmax(0.0, s.Rrho[i]);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < D; ++a) { nx.x[i * D + a] = xo[a]; nx.v[i * D + a] = vo[a]; }
+            nx.rho[i] = rho_o;
+            nx.p[i] = p_o;
+        }
+    }
+    if (do_update) {
+        const double bs = block_reduce(resid, OpSum(), lds);
+        if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+    }
+}
+
+void launch_force(int dim, int prec, const Consts& c, DevState s, Aux a, DevState pn, double* next_x,
+                  double* next_v, double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk,
+                  DevBlocks bl, const int32_t* near_inlet, TileGrid tg, NbrList nl, int64_t n, double dt,
+                  const DevScalars* sc, double* partial, hipStream_t st)
+{
+    (void)n;
+    NextState nx{next_x, next_v, next_rho, next_p};
+    const int upd = next_x != nullptr;
+    const dim3 g((unsigned)tg.nblocks), b(TPB);
+    if (dim == 2) {
+        if (prec == FJSPH_PREC_MIXED)
+            hipLaunchKernelGGL((k_force_t<2, float, 1024>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, tg, nl, dt, sc, partial);
+        else
+            hipLaunchKernelGGL((k_force_t<2, double, 512>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, tg, nl, dt, sc, partial);
+    } else {
+        if (prec == FJSPH_PREC_MIXED)
+            hipLaunchKernelGGL((k_force_t<3, float, 768>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, tg, nl, dt, sc, partial);
+        else
+            hipLaunchKernelGGL((k_force_t<3, double, 384>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, tg, nl, dt, sc, partial);
+    }
+}
+
+}  // namespace fj
