@@ -24,30 +24,38 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    objs = []
-    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+def build(verbose=False, lib=LIB, build_dir=None, defines=()):
+    """Compile every source (in parallel) and link `lib`.  `defines` are
+    extra -D options for schedule variants."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    build_dir = build_dir or os.path.join(HERE, "_build")
+    os.makedirs(build_dir, exist_ok=True)
     common = [
         f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wall", "-Wno-unused-function",
-    ]
+    ] + [f"-D{d}" for d in defines]
+    cmds, objs = [], []
     for src in SOURCES:
-        obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith((".hip", ".cpp")) and src != "settings.cpp" else []
         extra = ["-ffp-contract=off"] if src == "settings.cpp" else []
-        cmd = [hipcc()] + common + extra + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmds.append([hipcc()] + common + extra + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        for f in [ex.submit(run, c) for c in cmds]:
+            f.result()
+    tmp = lib + ".tmp"
+    run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -229,11 +229,11 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->partial); dfree(ctx->gather_tmp); dfree(ctx->gather_tmp_i);
     dfree(ctx->cell); dfree(ctx->cell_count); dfree(ctx->cell_start); dfree(ctx->cell_fill);
     dfree(ctx->order); dfree(ctx->tile_sums);
-    dfree(ctx->nl.lu); dfree(ctx->nl.count); dfree(ctx->export_buf);
+    dfree(ctx->nl.lu4); dfree(ctx->nl.count); dfree(ctx->nl.nnb); dfree(ctx->export_buf);
     if (ctx->nl.r) (void)hipFree(ctx->nl.r);
     ctx->nl.r = nullptr;
     dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
-    dfree(ctx->aux.nhat);
+    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb);
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
     ctx->ncell_cap = 0;
     ctx->nl.cap = 0;
@@ -427,8 +427,9 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         const int32_t mc = ctx->sc_host->max_count;
         ctx->max_count = mc;
         if (mc <= ctx->nl.cap) break;
+        /* capacity in entries, a multiple of the 4-entry batch */
         const int32_t cap = ((mc + mc / 8 + 8) + 7) / 8 * 8;
-        dalloc(ctx->nl.lu, size_t(cap) * ctx->nl.npad);
+        dalloc(ctx->nl.lu4, size_t(cap / 4) * ctx->nl.npad);
         if (ctx->nl.f32) dalloc(*(float**)&ctx->nl.r, size_t(cap) * ctx->nl.npad);
         else dalloc(*(double**)&ctx->nl.r, size_t(cap) * ctx->nl.npad);
         ctx->nl.cap = cap;
@@ -461,11 +462,24 @@ void dsph(fjsph_ctx* ctx, bool with_aero)
                 with_aero ? 1 : 0, ctx->partial, ctx->sc, ctx->st);
 }
 
+/* Detect_Surface loop 1 and dissipation_terms share one staged pass
+ * (measured faster than two separate passes at 4M particles). */
+#ifndef FJSPH_SURFACE_FUSED
+#define FJSPH_SURFACE_FUSED 1
+#endif
+
 void surface_and_dissipation(fjsph_ctx* ctx, bool with_diss)
 {
     PassTimer t(ctx, FJSPH_T_SURFACE);
-    launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl,
-                    ctx->n, r_from_x(ctx), ctx->norms_tmp, with_diss ? 1 : 0, ctx->st);
+    if (FJSPH_SURFACE_FUSED || !with_diss) {
+        launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl,
+                        ctx->n, r_from_x(ctx), ctx->norms_tmp, with_diss ? 1 : 0, ctx->st);
+    } else {
+        launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl,
+                        ctx->n, r_from_x(ctx), ctx->norms_tmp, 0, ctx->st);
+        launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl,
+                        ctx->n, r_from_x(ctx), ctx->norms_tmp, 2, ctx->st);
+    }
     launch_surface2(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
                     ctx->norms_tmp, ctx->st);
 }
@@ -541,7 +555,8 @@ double nb_iter(fjsph_ctx* ctx)
     ctx->x_at_build = false;
     {
         PassTimer t(ctx, FJSPH_T_REDUCE);
-        launch_sum_partials(ctx->partial, int(ctx->tg.nblocks), 1, &ctx->sc->residual, ctx->st);
+        /* one residual partial per TPB slots (k_nb_update) */
+        launch_sum_partials(ctx->partial, reduction_parts(ctx->n), 1, &ctx->sc->residual, ctx->st);
         HIPCHK(hipMemcpyAsync(&ctx->sc_host->residual, &ctx->sc->residual, sizeof(double), hipMemcpyDeviceToHost,
                               ctx->st));
     }
@@ -815,10 +830,12 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
         dalloc(ctx->aux.fq, size_t(n_total));
         dalloc(ctx->aux.fale, size_t(n_total));
         dalloc(ctx->aux.nhat, size_t(n_total) * D);
+        dalloc(ctx->aux.lnb, size_t(n_total) * D * D);
         HIPCHK(hipMemsetAsync(ctx->aux.nhat, 0, sizeof(double) * n_total * D, ctx->st));
         ctx->nl.f32 = ctx->s.precision == FJSPH_PREC_MIXED ? 1 : 0;
         ctx->nl.npad = (n_total + 63) / 64 * 64;
         dalloc(ctx->nl.count, size_t(ctx->nl.npad));
+        dalloc(ctx->nl.nnb, size_t(ctx->nl.npad));
         ctx->nl.cap = 0;
         HIPCHK(hipMemsetAsync(ctx->near_inlet, 0, sizeof(int32_t) * n_total, ctx->st));
         sync(ctx); /* the blocking copies below run on the null stream */
@@ -999,7 +1016,7 @@ fjsph_status fjsph_get_neighbour_counts(fjsph_ctx* ctx, int32_t* counts)
         require(ctx, true);
         sync(ctx);
         std::vector<int32_t> c(static_cast<size_t>(ctx->n));
-        HIPCHK(hipMemcpy(c.data(), ctx->nl.count, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(c.data(), ctx->nl.nnb, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
         std::vector<int32_t>& sh = slot_to_host(ctx);
         for (int64_t s = 0; s < ctx->n; ++s) counts[sh[s]] = c[s] + 1;
     });
@@ -1013,7 +1030,7 @@ fjsph_status fjsph_get_neighbours(fjsph_ctx* ctx, int64_t* offsets, int64_t* idx
         sync(ctx);
         const int64_t n = ctx->n;
         std::vector<int32_t> cnt(static_cast<size_t>(n));
-        HIPCHK(hipMemcpy(cnt.data(), ctx->nl.count, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cnt.data(), ctx->nl.nnb, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
         std::vector<int32_t> ids(size_t(ctx->nl.cap) * ctx->nl.npad);
         if (!ids.empty()) {
             dalloc(ctx->export_buf, ids.size());
@@ -1241,7 +1258,7 @@ fjsph_status fjsph_neighbour_stats(fjsph_ctx* ctx, int64_t* pairs, int32_t* max_
         require(ctx, true);
         sync(ctx);
         std::vector<int32_t> c(static_cast<size_t>(ctx->n));
-        HIPCHK(hipMemcpy(c.data(), ctx->nl.count, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(c.data(), ctx->nl.nnb, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
         int64_t t = 0;
         int32_t mx = 0;
         for (int32_t v : c) { t += v; mx = std::max(mx, v); }

@@ -38,19 +38,27 @@ struct DevState {
     int32_t *surf, *cellID, *surfzone, *b;
 };
 
-/* Neighbour lists, column-major by slot (entry k of slot i at k*npad + i).
- * lu holds the index of the neighbour in the candidate union of the slot's
- * tile block (ascending); the payload is the distance frozen at build time
- * (src/Resid.cpp:288-289 reads it from the list): FP64 d^2 on the parity
- * path, FP32 r on the mixed path. */
+/* Neighbour lists in batches of 4 entries, column-major by slot: batch b of
+ * slot i is lu4[b*npad + i] (four 16-bit indices into the candidate union
+ * of the slot's tile block, ascending) and r[b*npad + i] (four payloads).
+ * Batches never straddle a TILE of the union; unused entries hold 0xFFFF.
+ * The payload is the distance frozen at build time (src/Resid.cpp:288-289
+ * reads it from the list): FP64 d^2 on the parity path, FP32 r on the
+ * mixed path.  count[i] counts entries including padding, nnb[i] the
+ * neighbours (self excluded). */
 struct NbrList {
-    uint16_t* lu;
+    uint64_t* lu4;
     void* r;
     int32_t* count;
+    int32_t* nnb;
     int64_t npad;
-    int32_t cap;
-    int32_t f32; /* payload is float r (1) or double d^2 (0) */
+    int32_t cap;  /* entries per slot (multiple of 4) */
+    int32_t f32;  /* payload is float r (1) or double d^2 (0) */
 };
+
+/* Candidate-union tile shared by the list build and every pair pass. */
+constexpr int TILE = 512;
+constexpr uint32_t NO_ENTRY = 0xFFFF;
 
 /* Tile decomposition of the last neighbour build: block b covers cells
  * [q*C, q*C + C) of cell row b / nchunk, q = b % nchunk. */
@@ -73,6 +81,8 @@ struct Aux {
     float4* fq;      /* {m (negative for BOUND), p/rho^2, 1/rho, m/rho} in FP32 */
     float4* fale;    /* {vPert, rho} in FP32 (ALE)                          */
     double* nhat;    /* normalised surface normal (after Detect_Surface)     */
+    double* lnb;     /* dSPH_PreStep: correction matrix over non-boundary
+                        neighbours (D x D per slot), for its eigenvalue     */
 };
 
 /* Per-block boundary settings visible to kernels. */

@@ -94,43 +94,118 @@ __device__ __forceinline__ void block_head(const TileGrid& tg, BlockHead& h)
     __syncthreads();
 }
 
-/* List entry e of slot i: union index and frozen distance payload. */
-struct ListCursor {
-    int k, cnt, u;
+/* ------------------------------------------------- batched list access */
+
+template <typename RT>
+__device__ __forceinline__ void load_batch(const NbrList& nl, int64_t bidx, int64_t i, uint64_t& u, RT* r)
+{
+    const int64_t e = bidx * nl.npad + i;
+    u = nl.lu4[e];
+    if constexpr (sizeof(RT) == 4) {
+        const float4 q = ((const float4*)nl.r)[e];
+        r[0] = q.x; r[1] = q.y; r[2] = q.z; r[3] = q.w;
+    } else {
+        const double2* p = (const double2*)nl.r + 2 * e;
+        const double2 a = p[0], c = p[1];
+        r[0] = a.x; r[1] = a.y; r[2] = c.x; r[3] = c.y;
+    }
+}
+
+template <typename RT>
+__device__ __forceinline__ void store_batch(const NbrList& nl, int64_t bidx, int64_t i, uint64_t u, const RT* r)
+{
+    const int64_t e = bidx * nl.npad + i;
+    nl.lu4[e] = u;
+    if constexpr (sizeof(RT) == 4) {
+        ((float4*)nl.r)[e] = make_float4(r[0], r[1], r[2], r[3]);
+    } else {
+        double2* p = (double2*)nl.r + 2 * e;
+        p[0] = make_double2(r[0], r[1]);
+        p[1] = make_double2(r[2], r[3]);
+    }
+}
+
+/* Walks one slot's list batch by batch with the next batch in flight. */
+template <typename RT>
+struct Cursor {
+    int b, nb;
+    uint64_t u, un;
+    RT r[4], rn[4];
     __device__ __forceinline__ void init(const NbrList& nl, int64_t i, bool active)
     {
-        k = 0;
-        cnt = active ? nl.count[i] : 0;
-        u = cnt > 0 ? int(nl.lu[i]) : BIG_U;
+        nb = active ? (nl.count[i] >> 2) : 0;
+        b = 0;
+        u = ~0ull;
+        un = ~0ull;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { r[q] = RT(0); rn[q] = RT(0); }
+        if (nb > 0) load_batch<RT>(nl, 0, i, u, r);
+        if (nb > 1) load_batch<RT>(nl, 1, i, un, rn);
     }
-    __device__ __forceinline__ int next_u(const NbrList& nl, int64_t i) const
+    __device__ __forceinline__ int first() const { return b < nb ? int(u & 0xFFFFull) : BIG_U; }
+    __device__ __forceinline__ void advance(const NbrList& nl, int64_t i)
     {
-        return (k + 1 < cnt) ? int(nl.lu[(int64_t)(k + 1) * nl.npad + i]) : BIG_U;
+        ++b;
+        u = un;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = rn[q];
+        if (b + 1 < nb) load_batch<RT>(nl, b + 1, i, un, rn);
     }
 };
 
-__device__ __forceinline__ double payload_r(const NbrList& nl, int k, int64_t i, double& d2)
-{
-    const int64_t e = (int64_t)k * nl.npad + i;
-    if (nl.f32) {
-        const double r = double(((const float*)nl.r)[e]);
-        d2 = r * r;
-        return r;
+/* FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base) { body } FJ_TILE_ENTRIES_END
+ * runs body for every list entry of slot i that lies in the union tile
+ * [base, base + TILE), with `t` its tile-local index and `pay` its payload.
+ * All entries of a batch lie in one tile, so a batch is taken whole; its 4
+ * entries go through one copy of the body (a rolled loop rotating the batch
+ * through the first register slot).  The body is written inline, not as a
+ * closure, so per-lane accumulator arrays stay in registers. */
+#define FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base)                                   \
+    while ((cu).first() < (base) + TILE) {                                             \
+        uint64_t fj_u = (cu).u;                                                        \
+        RT fj_r0 = (cu).r[0], fj_r1 = (cu).r[1], fj_r2 = (cu).r[2], fj_r3 = (cu).r[3]; \
+        _Pragma("unroll 1") for (int fj_q = 0; fj_q < 4; ++fj_q)                      \
+        {                                                                              \
+            const uint32_t fj_uq = uint32_t(fj_u) & 0xFFFFu;                           \
+            const int t = int(fj_uq) - (base);                                         \
+            const RT pay = fj_r0;                                                      \
+            fj_u >>= 16;                                                               \
+            fj_r0 = fj_r1;                                                             \
+            fj_r1 = fj_r2;                                                             \
+            fj_r2 = fj_r3;                                                             \
+            if (fj_uq == NO_ENTRY) continue;
+
+#define FJ_TILE_ENTRIES_END(cu, nl, i) \
+        }                              \
+        (cu).advance(nl, i);           \
     }
-    d2 = ((const double*)nl.r)[e];
-    return sqrt(d2);
+
+/* Distance of a pair as the reference uses it: sqrt of the d^2 frozen at the
+ * last build; recomputed bit-identically from x while the positions are
+ * still those of the build (r_from_x). */
+template <int D, typename RT>
+__device__ __forceinline__ void pair_r(int r_from_x, const double* xi, const double* xj, RT pay, double& d2, double& r)
+{
+    if (r_from_x) {
+        d2 = exact_dist2<D>(xi, xj);
+        r = sqrt(d2);
+    } else if (sizeof(RT) == 4) {
+        r = double(pay);
+        d2 = r * r;
+    } else {
+        d2 = double(pay);
+        r = sqrt(d2);
+    }
 }
 
 /* ------------------------------------------------------------ list build */
-
-constexpr int BUILD_TILE = 1024;
 
 template <int D, typename RT>
 __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, TileGrid tg, double sr, NbrList nl,
                                                  DevScalars* sc)
 {
-    __shared__ double tp[D][BUILD_TILE];
-    __shared__ int32_t tsl[BUILD_TILE];
+    __shared__ double tp[D][TILE];
+    __shared__ int32_t tsl[TILE];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -139,7 +214,7 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
         if (threadIdx.x == 0) atomicMax(&sc->overflow, U);
         return;
     }
-    RT* pay = (RT*)nl.r;
+    const int bcap = nl.cap >> 2;
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
@@ -147,9 +222,11 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
         double xi[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) xi[a] = active ? x[i * D + a] : 0.0;
-        int cnt = 0;
-        for (int base = 0; base < U; base += BUILD_TILE) {
-            const int nt = min(BUILD_TILE, U - base);
+        int nbat = 0, q = 0, nreal = 0;
+        uint64_t cu = ~0ull;
+        RT cr[4] = {RT(0), RT(0), RT(0), RT(0)};
+        for (int base = 0; base < U; base += TILE) {
+            const int nt = min(TILE, U - base);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
@@ -165,19 +242,36 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
                     for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
                     const double d2 = exact_dist2<D>(xi, xj);
                     if (d2 < sr && tsl[t] != (int32_t)i) {
-                        if (cnt < nl.cap) {
-                            const int64_t e = (int64_t)cnt * nl.npad + i;
-                            nl.lu[e] = (uint16_t)(base + t);
-                            pay[e] = (sizeof(RT) == 4) ? RT(sqrt(d2)) : RT(d2);
+                        cu = (cu >> 16) | (uint64_t(base + t) << 48);
+                        cr[0] = cr[1]; cr[1] = cr[2]; cr[2] = cr[3];
+                        cr[3] = (sizeof(RT) == 4) ? RT(sqrt(d2)) : RT(d2);
+                        ++q;
+                        ++nreal;
+                        if (q == 4) {
+                            if (nbat < bcap) store_batch<RT>(nl, nbat, i, cu, cr);
+                            ++nbat;
+                            q = 0;
+                            cu = ~0ull;
                         }
-                        ++cnt;
                     }
+                }
+                /* close the tile: pad the open batch so none straddles a tile */
+                if (q > 0) {
+                    for (int k = q; k < 4; ++k) {
+                        cu = (cu >> 16) | (uint64_t(NO_ENTRY) << 48);
+                        cr[0] = cr[1]; cr[1] = cr[2]; cr[2] = cr[3]; cr[3] = RT(0);
+                    }
+                    if (nbat < bcap) store_batch<RT>(nl, nbat, i, cu, cr);
+                    ++nbat;
+                    q = 0;
+                    cu = ~0ull;
                 }
             }
         }
         if (active) {
-            nl.count[i] = cnt;
-            atomicMax(&sc->max_count, cnt);
+            nl.count[i] = nbat * 4;
+            nl.nnb[i] = nreal;
+            atomicMax(&sc->max_count, nbat * 4);
         }
     }
 }
@@ -196,7 +290,8 @@ void launch_build_lists(int dim, const double* x, int64_t n, TileGrid tg, double
     }
 }
 
-/* Global slot of every list entry (for the host dump of the lists). */
+/* Global slots of every slot's neighbours, compacted: entry k of slot i at
+ * out[k*npad + i] for k < nnb[i] (for the host dump of the lists). */
 template <int D>
 __global__ __launch_bounds__(TPB) void k_export(TileGrid tg, NbrList nl, int64_t n, int32_t* out)
 {
@@ -205,10 +300,16 @@ __global__ __launch_bounds__(TPB) void k_export(TileGrid tg, NbrList nl, int64_t
     int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
     int nseg, p0, p1;
     union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
-    const int cnt = min(nl.count[i], nl.cap);
-    for (int k = 0; k < cnt; ++k) {
-        const int64_t e = (int64_t)k * nl.npad + i;
-        out[e] = union_slot(tg, seg_a, pre, nseg, nl.lu[e]);
+    const int nbat = min(nl.count[i], nl.cap) >> 2;
+    int k = 0;
+    for (int bb = 0; bb < nbat; ++bb) {
+        const uint64_t u = nl.lu4[(int64_t)bb * nl.npad + i];
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t uq = uint32_t(u >> (16 * q)) & 0xFFFFu;
+            if (uq == NO_ENTRY) continue;
+            out[(int64_t)k * nl.npad + i] = union_slot(tg, seg_a, pre, nseg, (int)uq);
+            ++k;
+        }
     }
 }
 
@@ -219,8 +320,6 @@ void launch_export_lists(int dim, TileGrid tg, NbrList nl, int64_t n, int32_t* o
 }
 
 /* ------------------------------------------------------------ dSPH_PreStep */
-
-constexpr int DSPH_TILE = 1024;
 
 template <int D>
 __device__ __forceinline__ void aero_velocity_one(const Consts& c, DevState& s, int64_t i, double lam_nb, int cnt_with_self)
@@ -233,13 +332,26 @@ __device__ __forceinline__ void aero_velocity_one(const Consts& c, DevState& s, 
     s.cellID[i] = cond ? 1 : -3;
 }
 
-template <int D>
-__global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
-                                                TileGrid tg, NbrList nl, int r_from_x, int with_aero, double* partial)
+/* Dispatch on the list payload type (float on the mixed path). */
+#define FJ_PAYLOAD_DISPATCH(nl, KERNEL_FLOAT, KERNEL_DOUBLE) \
+    do {                                                     \
+        if ((nl).f32) { KERNEL_FLOAT; } else { KERNEL_DOUBLE; } \
+    } while (0)
+
+/* Pair sums of dSPH_PreStep (src/Shifting.cpp:12-123): the correction
+ * matrix L over all neighbours and over non-boundary neighbours (both
+ * symmetric, sum_j V_j W'(r)/r R_ji R_ji^T, accumulated on the upper
+ * triangle), the density-gradient and normal sums, the kernel sum and the
+ * colour function, and the per-block partial of npd.  k_dsph_fin turns
+ * them into L^-1, the eigenvalues and the corrected gradients. */
+template <int D, typename RT>
+__global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl, int r_from_x,
+                                                double* partial)
 {
-    __shared__ double tp[D][DSPH_TILE];
-    __shared__ double tvol[DSPH_TILE], trho[DSPH_TILE];
-    __shared__ int32_t tnb[DSPH_TILE];
+    constexpr int NS = D * (D + 1) / 2;
+    __shared__ double tp[D][TILE];
+    __shared__ double tvol[TILE], trho[TILE];
+    __shared__ int32_t tnb[TILE];
     __shared__ BlockHead h;
     __shared__ double lds[NWAVE];
     block_head<D>(tg, h);
@@ -250,21 +362,21 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, co
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
         const int64_t i = active ? tg.order[p] : 0;
-        double xi[D], Lm[D][D], Lnb[D][D], gR[D], nr[D];
+        double xi[D], Lm[NS], Lnb[NS], gR[D], nr[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             xi[a] = active ? s.x[i * D + a] : 0.0;
             gR[a] = 0.0;
             nr[a] = 0.0;
-#pragma unroll
-            for (int b2 = 0; b2 < D; ++b2) { Lm[a][b2] = 0.0; Lnb[a][b2] = 0.0; }
         }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) { Lm[k] = 0.0; Lnb[k] = 0.0; }
         const double rhoi = active ? s.rho[i] : 0.0;
         double kernsum = c.Wc, colour = 0.0;
-        ListCursor lc;
-        lc.init(nl, i, active);
-        for (int base = 0; base < U; base += DSPH_TILE) {
-            const int nt = min(DSPH_TILE, U - base);
+        Cursor<RT> cu;
+        cu.init(nl, i, active);
+        for (int base = 0; base < U; base += TILE) {
+            const int nt = min(TILE, U - base);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
@@ -275,15 +387,11 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, co
                 tnb[t] = s.b[sl] > FJSPH_PISTON;
             }
             __syncthreads();
-            const int lim = base + DSPH_TILE;
-            while (lc.u < lim) {
-                const int t = lc.u - base;
-                const int u_next = lc.next_u(nl, i);
+            FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base)
                 double xj[D], Rji[D], d2, r;
 #pragma unroll
                 for (int a = 0; a < D; ++a) { xj[a] = tp[a][t]; Rji[a] = xj[a] - xi[a]; }
-                if (r_from_x) { d2 = exact_dist2<D>(xi, xj); r = sqrt(d2); }
-                else r = payload_r(nl, lc.k, i, d2);
+                pair_r<D, RT>(r_from_x, xi, xj, pay, d2, r);
                 const double volj = tvol[t];
                 double W, fg;
                 wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
@@ -292,14 +400,15 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, co
                 for (int a = 0; a < D; ++a) G[a] = -fg * Rji[a];
                 const double drho = volj * (trho[t] - rhoi);
                 const bool nb = tnb[t] != 0;
+                int k = 0;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     const double vr = volj * Rji[a];
 #pragma unroll
-                    for (int b2 = 0; b2 < D; ++b2) {
+                    for (int b2 = a; b2 < D; ++b2, ++k) {
                         const double tt = vr * G[b2];
-                        Lm[a][b2] -= tt;
-                        if (nb) Lnb[a][b2] -= tt;
+                        Lm[k] -= tt;
+                        if (nb) Lnb[k] -= tt;
                     }
                     gR[a] += drho * G[a];
                 }
@@ -310,62 +419,110 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, co
                     colour += volj * W;
                     npd_acc += W;
                 }
-                ++lc.k;
-                lc.u = u_next;
-            }
+            FJ_TILE_ENTRIES_END(cu, nl, i)
         }
         if (active) {
-            double Lw[D][D], Linv[D][D], tmp[D][D];
+            int k = 0;
 #pragma unroll
             for (int a = 0; a < D; ++a)
 #pragma unroll
-                for (int b2 = 0; b2 < D; ++b2) { Lw[a][b2] = Lm[a][b2]; Linv[a][b2] = (a == b2) ? 1.0 : 0.0; }
-            if (qr_inverse<D>(Lw, tmp)) {
-#pragma unroll
-                for (int a = 0; a < D; ++a)
-#pragma unroll
-                    for (int b2 = 0; b2 < D; ++b2) Linv[a][b2] = tmp[a][b2];
-            }
-            double lam, lam_nb;
-            if (s.b[i] == FJSPH_BOUND) {
-                lam = 1.0;
-                lam_nb = 1.0;
-            } else {
-                lam = min_eig(Lm);
-                lam_nb = min_eig(Lnb);
-            }
-            s.lam[i] = lam;
-            s.lam_nb[i] = lam_nb;
+                for (int b2 = a; b2 < D; ++b2, ++k) {
+                    s.L[i * D * D + a * D + b2] = Lm[k];
+                    s.L[i * D * D + b2 * D + a] = Lm[k];
+                    ax.lnb[i * D * D + a * D + b2] = Lnb[k];
+                    ax.lnb[i * D * D + b2 * D + a] = Lnb[k];
+                }
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                double g = 0.0, q = 0.0;
-#pragma unroll
-                for (int b2 = 0; b2 < D; ++b2) {
-                    g += Linv[a][b2] * gR[b2];
-                    q += Linv[a][b2] * nr[b2];
-                    s.L[i * D * D + a * D + b2] = Linv[a][b2];
-                }
-                s.gradRho[i * D + a] = g;
-                s.norm[i * D + a] = q;
+                s.gradRho[i * D + a] = gR[a];
+                s.norm[i * D + a] = nr[a];
             }
-            s.colourG[i] = lam > 0.7 ? 2.0 : 2.0 * fmax(1.0, 1.0 / (2.0 * colour));
             s.kernsum[i] = kernsum;
             s.colour[i] = colour;
-            if (with_aero && c.aero_source == FJSPH_CONST_VEL && part_id[i] >= c.n_bound)
-                aero_velocity_one<D>(c, s, i, lam_nb, lc.cnt + 1);
         }
     }
     const double bs = block_reduce(npd_acc, OpSum(), lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;
 }
 
+/* Per-slot tail of dSPH_PreStep: L^-1 (column-pivoting QR, identity when
+ * L is rank deficient), the minimum eigenvalues, the corrected density
+ * gradient and normal, the colour gradient factor, and get_aero_velocity
+ * for the constant-velocity source. */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_dsph_fin(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                  const int32_t* __restrict__ nnb, int64_t n, int with_aero)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n) return;
+    double Lm[D][D], Lw[D][D], Linv[D][D], tmp[D][D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b2 = 0; b2 < D; ++b2) {
+            Lm[a][b2] = s.L[i * D * D + a * D + b2];
+            Lw[a][b2] = Lm[a][b2];
+            Linv[a][b2] = (a == b2) ? 1.0 : 0.0;
+        }
+    if (qr_inverse<D>(Lw, tmp)) {
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) Linv[a][b2] = tmp[a][b2];
+    }
+    double lam, lam_nb;
+    if (s.b[i] == FJSPH_BOUND) {
+        lam = 1.0;
+        lam_nb = 1.0;
+    } else {
+        double Lnb[D][D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) Lnb[a][b2] = ax.lnb[i * D * D + a * D + b2];
+        lam = min_eig(Lm);
+        lam_nb = min_eig(Lnb);
+    }
+    s.lam[i] = lam;
+    s.lam_nb[i] = lam_nb;
+    double gR[D], nr[D];
+    load_vec<D>(s.gradRho, i, gR);
+    load_vec<D>(s.norm, i, nr);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double g = 0.0, q = 0.0;
+#pragma unroll
+        for (int b2 = 0; b2 < D; ++b2) {
+            g += Linv[a][b2] * gR[b2];
+            q += Linv[a][b2] * nr[b2];
+            s.L[i * D * D + a * D + b2] = Linv[a][b2];
+        }
+        s.gradRho[i * D + a] = g;
+        s.norm[i * D + a] = q;
+    }
+    const double colour = s.colour[i];
+    s.colourG[i] = lam > 0.7 ? 2.0 : 2.0 * fmax(1.0, 1.0 / (2.0 * colour));
+    if (with_aero && c.aero_source == FJSPH_CONST_VEL && part_id[i] >= c.n_bound)
+        aero_velocity_one<D>(c, s, i, lam_nb, nnb[i] + 1);
+}
+
 void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
                  int64_t n, int r_from_x, int with_aero_vel, double* partial, DevScalars* sc, hipStream_t st)
 {
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2) hipLaunchKernelGGL(k_dsph_t<2>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, with_aero_vel, partial);
-    else hipLaunchKernelGGL(k_dsph_t<3>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, with_aero_vel, partial);
+    if (dim == 2)
+        FJ_PAYLOAD_DISPATCH(nl,
+            hipLaunchKernelGGL((k_dsph_t<2, float>), g, b, 0, st, c, s, a, tg, nl, r_from_x, partial),
+            hipLaunchKernelGGL((k_dsph_t<2, double>), g, b, 0, st, c, s, a, tg, nl, r_from_x, partial));
+    else
+        FJ_PAYLOAD_DISPATCH(nl,
+            hipLaunchKernelGGL((k_dsph_t<3, float>), g, b, 0, st, c, s, a, tg, nl, r_from_x, partial),
+            hipLaunchKernelGGL((k_dsph_t<3, double>), g, b, 0, st, c, s, a, tg, nl, r_from_x, partial));
     launch_sum_partials_scaled(partial, (int)tg.nblocks, 1, 1.0 / double(n), &sc->npd, st);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_dsph_fin<2>, dim3(grid_of(n)), b, 0, st, c, s, a, part_id, nl.nnb, n, with_aero_vel);
+    else
+        hipLaunchKernelGGL(k_dsph_fin<3>, dim3(grid_of(n)), b, 0, st, c, s, a, part_id, nl.nnb, n, with_aero_vel);
 }
 
 template <int D>
@@ -374,7 +531,7 @@ __global__ __launch_bounds__(TPB) void k_aero_vel(Consts c, DevState s, const in
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n || part_id[i] < c.n_bound) return;
-    aero_velocity_one<D>(c, s, i, s.lam_nb[i], nl.count[i] + 1);
+    aero_velocity_one<D>(c, s, i, s.lam_nb[i], nl.nnb[i] + 1);
 }
 
 void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
@@ -387,20 +544,19 @@ void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* p
 
 /* -------------------------------------- Detect_Surface + dissipation_terms */
 
-constexpr int SURF1_TILE = 512;
-
 /* mode 0: Detect_Surface loop 1 only; 1: with dissipation_terms;
- * 2: dissipation_terms only. */
-template <int D, typename T>
+ * 2: dissipation_terms only.  T is the arithmetic of the dissipation terms
+ * and the list payload type. */
+template <int D, typename T, int MODE>
 __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
-                                                    TileGrid tg, NbrList nl, int r_from_x, double* norms_tmp,
-                                                    int mode)
+                                                    TileGrid tg, NbrList nl, int r_from_x, double* norms_tmp)
 {
-    const bool with_surf = mode != 2;
-    const bool with_diss = mode != 0;
-    __shared__ double tp[D][SURF1_TILE], tv[D][SURF1_TILE], tg_[D][SURF1_TILE];
-    __shared__ double tvol[SURF1_TILE], tlam[SURF1_TILE], trho[SURF1_TILE], tir[SURF1_TILE], tm[SURF1_TILE];
-    __shared__ int32_t tnb[SURF1_TILE];
+    constexpr bool with_surf = MODE != 2;
+    constexpr bool with_diss = MODE != 0;
+    constexpr int TS = with_surf ? TILE : 1, TD = with_diss ? TILE : 1;
+    __shared__ double tp[D][TILE], tv[D][TD], tg_[D][TD];
+    __shared__ double tvol[TILE], tlam[TS], trho[TD], tir[TD], tm[TD];
+    __shared__ int32_t tnb[TD];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -438,42 +594,40 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
         const bool testing = with_surf && fluid && bi >= FJSPH_PIPE && lam_nb >= 0.2 && lam_nb < 0.75;
         bool found = false;
         const bool high = lami > 0.7;
-        /* dissipation accumulators (src/Shifting.cpp:135-183) */
         double av[D], rd = 0.0;
 #pragma unroll
         for (int a = 0; a < D; ++a) av[a] = 0.0;
         const T Bg = T(c.B * c.gam), cfac = T(sqrt(c.B * c.gam / rhoi)), Ht = T(c.H);
         const T eps2 = T(0.0001 * c.H_sq), alpha = T(c.visc_alpha);
-        ListCursor lc;
-        lc.init(nl, i, fluid);
-        for (int base = 0; base < U; base += SURF1_TILE) {
-            const int nt = min(SURF1_TILE, U - base);
+        Cursor<T> cu;
+        cu.init(nl, i, fluid);
+        for (int base = 0; base < U; base += TILE) {
+            const int nt = min(TILE, U - base);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    tp[a][t] = s.x[(int64_t)sl * D + a];
-                    tv[a][t] = s.v[(int64_t)sl * D + a];
-                    tg_[a][t] = s.gradRho[(int64_t)sl * D + a];
-                }
+                for (int a = 0; a < D; ++a) tp[a][t] = s.x[(int64_t)sl * D + a];
                 tvol[t] = ax.vol[sl];
-                tlam[t] = s.lam[sl];
-                trho[t] = s.rho[sl];
-                tir[t] = ax.invrho[sl];
-                tm[t] = s.m[sl];
-                tnb[t] = s.b[sl] > FJSPH_PISTON;
+                if constexpr (with_surf) tlam[t] = s.lam[sl];
+                if constexpr (with_diss) {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        tv[a][t] = s.v[(int64_t)sl * D + a];
+                        tg_[a][t] = s.gradRho[(int64_t)sl * D + a];
+                    }
+                    trho[t] = s.rho[sl];
+                    tir[t] = ax.invrho[sl];
+                    tm[t] = s.m[sl];
+                    tnb[t] = s.b[sl] > FJSPH_PISTON;
+                }
             }
             __syncthreads();
-            const int lim = base + SURF1_TILE;
-            while (lc.u < lim) {
-                const int t = lc.u - base;
-                const int u_next = lc.next_u(nl, i);
+            FJ_TILE_ENTRIES_BEGIN(T, cu, nl, i, base)
                 double xj[D], d2, r;
 #pragma unroll
                 for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
-                if (r_from_x) { d2 = exact_dist2<D>(xi, xj); r = sqrt(d2); }
-                else r = payload_r(nl, lc.k, i, d2);
+                pair_r<D, T>(r_from_x, xi, xj, pay, d2, r);
                 if (testing && !found) {
                     /* Marrone et al. (2010) point-T test, src/Geometry.cpp:43-98;
                      * the outcome does not depend on the order of the pairs */
@@ -493,12 +647,13 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
                 }
                 double W, fg;
                 wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
-                if (with_surf) {
+                if constexpr (with_surf) {
                     const double f = tvol[t] * (high ? (tlam[t] - lami) : tlam[t]) * fg;
 #pragma unroll
                     for (int a = 0; a < D; ++a) nrm[a] += f * (xi[a] - xj[a]);
                 }
-                if (with_diss) {
+                if constexpr (with_diss) {
+                    /* dissipation_terms pair term (src/Shifting.cpp:141-173) */
                     T Rji[D], Vji[D], g[D];
                     T fgt;
                     if (sizeof(T) == 8) fgt = T(fg);
@@ -535,19 +690,17 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
                         rd += double(volj * drho * rdotg * idist2);
                     }
                 }
-                ++lc.k;
-                lc.u = u_next;
-            }
+            FJ_TILE_ENTRIES_END(cu, nl, i)
         }
         if (!active) continue;
         if (!fluid) {
-            if (with_surf) {
+            if constexpr (with_surf) {
 #pragma unroll
                 for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = 0.0;
             }
             continue;
         }
-        if (with_diss) {
+        if constexpr (with_diss) {
 #pragma unroll
             for (int a = 0; a < D; ++a) s.aVisc[i * D + a] = av[a];
             s.deltaD[i] = c.dsph_cont * rd;
@@ -579,31 +732,43 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
     }
 }
 
+template <int D, int MODE>
+static void launch_surface1_m(const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                              int r_from_x, double* norms_tmp, hipStream_t st)
+{
+    const dim3 g((unsigned)tg.nblocks), b(TPB);
+    FJ_PAYLOAD_DISPATCH(nl,
+        hipLaunchKernelGGL((k_surface1_t<D, float, MODE>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp),
+        hipLaunchKernelGGL((k_surface1_t<D, double, MODE>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp));
+}
+
+template <int D>
+static void launch_surface1_d(const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                              int r_from_x, double* norms_tmp, int mode, hipStream_t st)
+{
+    if (mode == 0) launch_surface1_m<D, 0>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
+    else if (mode == 1) launch_surface1_m<D, 1>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
+    else launch_surface1_m<D, 2>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
+}
+
 void launch_surface1(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg,
                      NbrList nl, int64_t n, int r_from_x, double* norms_tmp, int mode, hipStream_t st)
 {
     (void)n;
-    const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2) {
-        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_surface1_t<2, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode);
-        else hipLaunchKernelGGL((k_surface1_t<2, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode);
-    } else {
-        if (prec == FJSPH_PREC_MIXED) hipLaunchKernelGGL((k_surface1_t<3, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode);
-        else hipLaunchKernelGGL((k_surface1_t<3, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode);
-    }
+    (void)prec;
+    if (dim == 2) launch_surface1_d<2>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
+    else launch_surface1_d<3>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
 }
 
 /* Curvature, Gissler occlusion and surfzone (src/Geometry.cpp:145-277). */
-constexpr int SURF2_TILE = 512;
-
-template <int D>
+template <int D, typename RT>
 __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
                                                     TileGrid tg, NbrList nl, int r_from_x,
                                                     const double* __restrict__ norms_tmp)
 {
-    __shared__ double tp[D][SURF2_TILE], tn[D][SURF2_TILE];
-    __shared__ double tvol[SURF2_TILE];
-    __shared__ int32_t tsurf[SURF2_TILE];
+    __shared__ double tp[D][TILE], tn[D][TILE];
+    __shared__ double tvol[TILE];
+    __shared__ int32_t tsurf[TILE];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -633,10 +798,10 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
         const bool ni_nz = dotd<D>(ni, ni) > 0.0;
         double curve = 0.0, qmax = 0.0;
         int zone = (fluid && s.surf[i] == 1) ? 1 : 0;
-        ListCursor lc;
-        lc.init(nl, i, fluid);
-        for (int base = 0; base < U; base += SURF2_TILE) {
-            const int nt = min(SURF2_TILE, U - base);
+        Cursor<RT> cu;
+        cu.init(nl, i, fluid);
+        for (int base = 0; base < U; base += TILE) {
+            const int nt = min(TILE, U - base);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
@@ -649,15 +814,11 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
                 tsurf[t] = s.surf[sl];
             }
             __syncthreads();
-            const int lim = base + SURF2_TILE;
-            while (lc.u < lim) {
-                const int t = lc.u - base;
-                const int u_next = lc.next_u(nl, i);
+            FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base)
                 double xj[D], d2, r, Rij[D], nj[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) { xj[a] = tp[a][t]; Rij[a] = xj[a] - xi[a]; nj[a] = tn[a][t]; }
-                if (r_from_x) { d2 = exact_dist2<D>(xi, xj); r = sqrt(d2); }
-                else r = payload_r(nl, lc.k, i, d2);
+                pair_r<D, RT>(r_from_x, xi, xj, pay, d2, r);
                 if (ni_nz && dotd<D>(nj, nj) > 0.0) {
                     double W, fg;
                     wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
@@ -678,9 +839,7 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
                     if (q > qmax) qmax = q;
                 }
                 if (tsurf[t] == 1) zone = 1;
-                ++lc.k;
-                lc.u = u_next;
-            }
+            FJ_TILE_ENTRIES_END(cu, nl, i)
         }
         if (!fluid) continue;
         const double wo = (occl && Vdn > 0.0) ? qmax / Vdn : 0.0;
@@ -704,8 +863,14 @@ void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t*
 {
     (void)n;
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2) hipLaunchKernelGGL(k_surface2_t<2>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp);
-    else hipLaunchKernelGGL(k_surface2_t<3>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp);
+    if (dim == 2)
+        FJ_PAYLOAD_DISPATCH(nl,
+            hipLaunchKernelGGL((k_surface2_t<2, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp),
+            hipLaunchKernelGGL((k_surface2_t<2, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp));
+    else
+        FJ_PAYLOAD_DISPATCH(nl,
+            hipLaunchKernelGGL((k_surface2_t<3, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp),
+            hipLaunchKernelGGL((k_surface2_t<3, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp));
 }
 
 /* ------------------------------------------------------------ particle_shift */
@@ -714,15 +879,13 @@ void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t*
  * n_i.n_j) over the dot products acos accepts ([-1, 1]): NaN angles never
  * win the reference's max and acos is decreasing, so the result is the same.
  * max_j |v_j - v_i| is taken on squared norms. */
-constexpr int SHIFT_TILE = 512;
-
-template <int D>
+template <int D, typename RT>
 __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
                                                  TileGrid tg, NbrList nl, int r_from_x)
 {
-    __shared__ double tp[D][SHIFT_TILE], tv[D][SHIFT_TILE], tn[D][SHIFT_TILE];
-    __shared__ double tvol[SHIFT_TILE];
-    __shared__ int32_t tnb[SHIFT_TILE];
+    __shared__ double tp[D][TILE], tv[D][TILE], tn[D][TILE];
+    __shared__ double tvol[TILE];
+    __shared__ int32_t tnb[TILE];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -744,10 +907,10 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
             du[a] = 0.0;
         }
         double maxU2 = 0.0, dmin = 2.0;
-        ListCursor lc;
-        lc.init(nl, i, work);
-        for (int base = 0; base < U; base += SHIFT_TILE) {
-            const int nt = min(SHIFT_TILE, U - base);
+        Cursor<RT> cu;
+        cu.init(nl, i, work);
+        for (int base = 0; base < U; base += TILE) {
+            const int nt = min(TILE, U - base);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
@@ -761,15 +924,11 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
                 tnb[t] = s.b[sl] > FJSPH_PISTON;
             }
             __syncthreads();
-            const int lim = base + SHIFT_TILE;
-            while (lc.u < lim) {
-                const int t = lc.u - base;
-                const int u_next = lc.next_u(nl, i);
+            FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base)
                 double xj[D], d2, r;
 #pragma unroll
                 for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
-                if (r_from_x) { d2 = exact_dist2<D>(xi, xj); r = sqrt(d2); }
-                else r = payload_r(nl, lc.k, i, d2);
+                pair_r<D, RT>(r_from_x, xi, xj, pay, d2, r);
                 double W, fg;
                 wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
                 const double w = W * inv_Wdx;
@@ -789,9 +948,7 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
                     if (d >= -1.0 && d <= 1.0 && d < dmin) dmin = d;
                 }
                 if (dv2 > maxU2) maxU2 = dv2;
-                ++lc.k;
-                lc.u = u_next;
-            }
+            FJ_TILE_ENTRIES_END(cu, nl, i)
         }
         if (!fluid) continue;
         double out[D];
@@ -834,14 +991,20 @@ void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
 {
     (void)n;
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2) hipLaunchKernelGGL(k_shift_t<2>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x);
-    else hipLaunchKernelGGL(k_shift_t<3>, g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x);
+    if (dim == 2)
+        FJ_PAYLOAD_DISPATCH(nl,
+            hipLaunchKernelGGL((k_shift_t<2, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x),
+            hipLaunchKernelGGL((k_shift_t<2, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x));
+    else
+        FJ_PAYLOAD_DISPATCH(nl,
+            hipLaunchKernelGGL((k_shift_t<3, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x),
+            hipLaunchKernelGGL((k_shift_t<3, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x));
 }
 
 /* ------------------------------------------------------- boundary solvers */
 
 /* The boundary passes touch only boundary particles (a thin layer), so they
- * gather per lane through the union table instead of staging tiles. */
+ * walk their lists per lane through the union table instead of staging. */
 template <int D>
 __device__ __forceinline__ void lane_union(const TileGrid& tg, int64_t i, int32_t* seg_a, int32_t* pre, int& nseg)
 {
@@ -849,8 +1012,32 @@ __device__ __forceinline__ void lane_union(const TileGrid& tg, int64_t i, int32_
     union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
 }
 
+/* f(j, r) for every neighbour j of slot i with its frozen distance r. */
+template <int D, typename RT, typename F>
+__device__ __forceinline__ void walk_lane(const TileGrid& tg, const NbrList& nl, int64_t i, F&& f)
+{
+    int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
+    int nseg;
+    lane_union<D>(tg, i, seg_a, pre, nseg);
+    const int nbat = nl.count[i] >> 2;
+    for (int bb = 0; bb < nbat; ++bb) {
+        uint64_t u;
+        RT r[4];
+        load_batch<RT>(nl, bb, i, u, r);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t uq = uint32_t(u >> (16 * q)) & 0xFFFFu;
+            if (uq == NO_ENTRY) continue;
+            const int32_t j = union_slot(tg, seg_a, pre, nseg, (int)uq);
+            double d2, rr;
+            pair_r<D, RT>(0, nullptr, nullptr, r[q], d2, rr);
+            f(j, rr);
+        }
+    }
+}
+
 /* Block velocity + no-slip (src/Newmark_Beta.cpp:69-102, src/Resid.cpp:162-186). */
-template <int D>
+template <int D, typename RT>
 __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int32_t* __restrict__ blk, DevBlocks bl,
                                                  int32_t block, TileGrid tg, NbrList nl, int64_t n)
 {
@@ -860,26 +1047,19 @@ __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int
 #pragma unroll
     for (int a = 0; a < D; ++a) v[a] = bl.vel[block * 3 + a];
     if (bl.no_slip[block]) {
-        int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
-        int nseg;
-        lane_union<D>(tg, i, seg_a, pre, nseg);
         const double inv_H = 1.0 / c.H;
         double vs[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) vs[a] = 0.0;
         double ks = 0.0;
-        const int cnt = nl.count[i];
-        for (int k = 0; k < cnt; ++k) {
-            const int32_t j = union_slot(tg, seg_a, pre, nseg, nl.lu[(int64_t)k * nl.npad + i]);
-            if (s.b[j] <= FJSPH_PISTON) continue;
-            double d2;
-            const double r = payload_r(nl, k, i, d2);
+        walk_lane<D, RT>(tg, nl, i, [&](int32_t j, double r) {
+            if (s.b[j] <= FJSPH_PISTON) return;
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
             ks += W;
 #pragma unroll
             for (int a = 0; a < D; ++a) vs[a] += s.v[(int64_t)j * D + a] * W;
-        }
+        });
         if (ks > 0.0) {
 #pragma unroll
             for (int a = 0; a < D; ++a) v[a] = 2.0 * v[a] - vs[a] / ks;
@@ -890,7 +1070,7 @@ __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int
 }
 
 /* Boundary solver of one block (src/Resid.cpp:21-160). */
-template <int D>
+template <int D, typename RT>
 __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax, const int32_t* __restrict__ blk,
                                                    DevBlocks bl, int32_t block, TileGrid tg, NbrList nl, int64_t n,
                                                    int32_t* near_inlet)
@@ -899,12 +1079,8 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
     if (i >= n || blk[i] != block) return;
     const int solver = bl.solver[block];
     const double inv_H = 1.0 / c.H;
-    const int cnt = nl.count[i];
     double xi[D];
     load_vec<D>(s.x, i, xi);
-    int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
-    int nseg;
-    lane_union<D>(tg, i, seg_a, pre, nseg);
     if (solver == FJSPH_DBC) {
 #pragma unroll
         for (int a = 0; a < D; ++a) s.acc[i * D + a] = 0.0;
@@ -913,11 +1089,8 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
         double ks = 0.0, pk = 0.0, ak[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) ak[a] = 0.0;
-        for (int k = 0; k < cnt; ++k) {
-            const int32_t j = union_slot(tg, seg_a, pre, nseg, nl.lu[(int64_t)k * nl.npad + i]);
-            if (s.b[j] <= FJSPH_PISTON) continue;
-            double d2;
-            const double r = payload_r(nl, k, i, d2);
+        walk_lane<D, RT>(tg, nl, i, [&](int32_t j, double r) {
+            if (s.b[j] <= FJSPH_PISTON) return;
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
             const double kern = ax.vol[j] * W;
@@ -927,7 +1100,7 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
 #pragma unroll
             for (int a = 0; a < D; ++a) ak[a] += kr * (xi[a] - s.x[(int64_t)j * D + a]);
             if (s.surfzone[j]) near = 1;
-        }
+        });
         double pres = 0.0;
         if (ks > 0.0) {
             double ga = 0.0;
@@ -942,12 +1115,9 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
         int ni = 1;
         double rr = 0.0, vi[D];
         load_vec<D>(s.v, i, vi);
-        for (int k = 0; k < cnt; ++k) {
-            const int32_t j = union_slot(tg, seg_a, pre, nseg, nl.lu[(int64_t)k * nl.npad + i]);
+        walk_lane<D, RT>(tg, nl, i, [&](int32_t j, double r) {
             const int32_t bj = s.b[j];
             if (bj == FJSPH_PIPE || bj == FJSPH_FREE) ni = 0;
-            double d2;
-            const double r = payload_r(nl, k, i, d2);
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
             double t = 0.0;
@@ -955,7 +1125,7 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
             for (int a = 0; a < D; ++a)
                 t += (s.v[(int64_t)j * D + a] - vi[a]) * (fg * (s.x[(int64_t)j * D + a] - xi[a]));
             rr -= ax.vol[j] * t;
-        }
+        });
         near_inlet[i] = ni;
         s.Rrho[i] = rr * s.rho[i];
     }
@@ -965,12 +1135,18 @@ void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t*
                      TileGrid tg, NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st)
 {
     const dim3 g(grid_of(n)), b(TPB);
-    if (dim == 2) {
-        hipLaunchKernelGGL(k_bnd_vel<2>, g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
-        hipLaunchKernelGGL(k_bnd_solve<2>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+    if (dim == 2 && nl.f32) {
+        hipLaunchKernelGGL((k_bnd_vel<2, float>), g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
+        hipLaunchKernelGGL((k_bnd_solve<2, float>), g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+    } else if (dim == 2) {
+        hipLaunchKernelGGL((k_bnd_vel<2, double>), g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
+        hipLaunchKernelGGL((k_bnd_solve<2, double>), g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+    } else if (nl.f32) {
+        hipLaunchKernelGGL((k_bnd_vel<3, float>), g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
+        hipLaunchKernelGGL((k_bnd_solve<3, float>), g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
     } else {
-        hipLaunchKernelGGL(k_bnd_vel<3>, g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
-        hipLaunchKernelGGL(k_bnd_solve<3>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+        hipLaunchKernelGGL((k_bnd_vel<3, double>), g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
+        hipLaunchKernelGGL((k_bnd_solve<3, double>), g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
     }
 }
 
@@ -1012,37 +1188,43 @@ struct NextState {
     double *x, *v, *rho, *p;
 };
 
-/* LDS tile of the force pass: positions and velocities relative to the
- * block's first target in T (exact FP64 differences in the FP64 path), and
- * m (negative for BOUND), p/rho^2, 1/rho, m/rho; ALE adds vPert and rho. */
-template <int D, typename T, int TILE>
+/* Occupancy request for the force pass: 5 waves per SIMD (at most 102
+ * VGPRs) measured faster than the compiler's 4 at 97 VGPRs. */
+#ifndef FJ_FORCE_OCC
+#define FJ_FORCE_OCC __attribute__((amdgpu_waves_per_eu(5, 8)))
+#endif
+
+/* LDS tile of the force pass, one record per union slot in four arrays:
+ * A {x - xref, m (negative for BOUND)}, B {v - vref, p/rho^2},
+ * C {1/rho, m/rho}, E (ALE) {vPert, rho}.  FP32: float4/float2, so a pair
+ * costs four LDS reads.  FP64: the same values as doubles, xref = vref = 0. */
+template <typename T>
 struct ForceTile {
-    T px[D][TILE], vx[D][TILE];
-    T m[TILE], P[TILE], ir[TILE], vol[TILE];
-    T vp[D][TILE], rho[TILE];
+    T A[TILE][4], B[TILE][4], E[TILE][4], C[TILE][2];
+};
+template <>
+struct ForceTile<float> {
+    float4 A[TILE], B[TILE], E[TILE];
+    float2 C[TILE];
 };
 
-/* get_acc_and_Rrho_on_i (src/Resid.cpp:243-418) for fluid slots, fused with
- * the Newmark-Beta update of Do_NB_Iter (src/Newmark_Beta.cpp:137-240).
- * Pair terms are formed in T; every sum is accumulated in FP64.  The
- * pressure, viscous and surface-tension terms of a pair are combined before
- * accumulation: acc_i = sum_j (visc + ST / m_i - m_j (P_i + P_j) g). */
-template <int D, typename T, int TILE>
-__global__ __launch_bounds__(TPB) void k_force_t(Consts c, DevState s, Aux ax, DevState pn, NextState nx,
-                                                 int do_update, const int32_t* __restrict__ part_id,
-                                                 const int32_t* __restrict__ blk, DevBlocks bl,
-                                                 const int32_t* __restrict__ near_inlet, TileGrid tg, NbrList nl,
-                                                 double dt, const DevScalars* sc, double* partial)
+/* Pair sums of get_acc_and_Rrho_on_i (src/Resid.cpp:243-418) for fluid
+ * slots: writes the pair sum of the acceleration to acc (the aerodynamic,
+ * artificial-viscosity and gravity terms are added by k_nb_update) and the
+ * full Rrho.  Pair terms are formed in T; every sum is accumulated in FP64.
+ * The pressure, viscous and surface-tension terms of a pair are combined
+ * before accumulation: acc_i = sum_j (visc + ST / m_i - m_j (P_i + P_j) g). */
+template <int D, typename T>
+__global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                 TileGrid tg, NbrList nl, const DevScalars* sc)
 {
-    __shared__ ForceTile<D, T, TILE> tl;
+    __shared__ ForceTile<T> ft;
     __shared__ BlockHead h;
-    __shared__ double lds[NWAVE];
     constexpr bool F32 = sizeof(T) == 4;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
-    double resid = 0.0;
-    /* reference point of the block (first target) for the FP32 relative
-     * coordinates */
+    /* reference point of the block (its first target) for the FP32
+     * relative coordinates */
     double xref[D], vref[D];
     {
         const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
@@ -1086,65 +1268,77 @@ __global__ __launch_bounds__(TPB) void k_force_t(Consts c, DevState s, Aux ax, D
 #pragma unroll
         for (int a = 0; a < D; ++a) acc[a] = 0.0;
         double Rrho_ = 0.0, Rrhoc_ = 0.0;
-        ListCursor lc;
-        lc.init(nl, i, fluid);
+        Cursor<T> cu;
+        cu.init(nl, i, fluid);
         for (int base = 0; base < U; base += TILE) {
             const int nt = min(TILE, U - base);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+                T pa[3] = {T(0), T(0), T(0)}, va[3] = {T(0), T(0), T(0)};
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    tl.px[a][t] = T(s.x[(int64_t)sl * D + a] - xref[a]);
-                    tl.vx[a][t] = T(s.v[(int64_t)sl * D + a] - vref[a]);
+                    pa[a] = T(s.x[(int64_t)sl * D + a] - xref[a]);
+                    va[a] = T(s.v[(int64_t)sl * D + a] - vref[a]);
                 }
+                T m, P, ir, vol;
                 if (F32) {
                     const float4 q = ax.fq[sl];
-                    tl.m[t] = T(q.x);
-                    tl.P[t] = T(q.y);
-                    tl.ir[t] = T(q.z);
-                    tl.vol[t] = T(q.w);
+                    m = T(q.x); P = T(q.y); ir = T(q.z); vol = T(q.w);
                 } else {
-                    tl.m[t] = T(s.b[sl] == FJSPH_BOUND ? -s.m[sl] : s.m[sl]);
-                    tl.P[t] = T(ax.P[sl]);
-                    tl.ir[t] = T(ax.invrho[sl]);
-                    tl.vol[t] = T(ax.vol[sl]);
+                    m = T(s.b[sl] == FJSPH_BOUND ? -s.m[sl] : s.m[sl]);
+                    P = T(ax.P[sl]); ir = T(ax.invrho[sl]); vol = T(ax.vol[sl]);
+                }
+                if constexpr (F32) {
+                    ft.A[t] = make_float4(pa[0], pa[1], pa[2], m);
+                    ft.B[t] = make_float4(va[0], va[1], va[2], P);
+                    ft.C[t] = make_float2(ir, vol);
+                } else {
+                    ft.A[t][0] = pa[0]; ft.A[t][1] = pa[1]; ft.A[t][2] = pa[2]; ft.A[t][3] = m;
+                    ft.B[t][0] = va[0]; ft.B[t][1] = va[1]; ft.B[t][2] = va[2]; ft.B[t][3] = P;
+                    ft.C[t][0] = ir; ft.C[t][1] = vol;
                 }
                 if (c.ale) {
+                    T vp[3] = {T(0), T(0), T(0)};
 #pragma unroll
-                    for (int a = 0; a < D; ++a) tl.vp[a][t] = T(s.vPert[(int64_t)sl * D + a]);
-                    tl.rho[t] = T(s.rho[sl]);
+                    for (int a = 0; a < D; ++a) vp[a] = T(s.vPert[(int64_t)sl * D + a]);
+                    if constexpr (F32) {
+                        ft.E[t] = make_float4(vp[0], vp[1], vp[2], float(s.rho[sl]));
+                    } else {
+                        ft.E[t][0] = vp[0]; ft.E[t][1] = vp[1]; ft.E[t][2] = vp[2]; ft.E[t][3] = T(s.rho[sl]);
+                    }
                 }
             }
             __syncthreads();
-            const int lim = base + TILE;
-            while (lc.u < lim) {
-                const int t = lc.u - base;
-                const int u_next = lc.next_u(nl, i);
+            FJ_TILE_ENTRIES_BEGIN(T, cu, nl, i, base)
                 T r, rr;
-                {
-                    const int64_t e = (int64_t)lc.k * nl.npad + i;
-                    if (F32) {
-                        r = T(((const float*)nl.r)[e]);
-                        rr = r * r;
-                    } else {
-                        const double d2 = ((const double*)nl.r)[e];
-                        rr = T(d2);
-                        r = T(sqrt(d2));
-                    }
+                if (F32) {
+                    r = pay;
+                    rr = r * r;
+                } else {
+                    rr = pay;
+                    r = T(sqrt(double(pay)));
+                }
+                T pj[3], vj[3], msg, Pj, irj, volj;
+                if constexpr (F32) {
+                    const float4 A = ft.A[t], B = ft.B[t];
+                    const float2 Cc = ft.C[t];
+                    pj[0] = A.x; pj[1] = A.y; pj[2] = A.z; msg = A.w;
+                    vj[0] = B.x; vj[1] = B.y; vj[2] = B.z; Pj = B.w;
+                    irj = Cc.x; volj = Cc.y;
+                } else {
+                    pj[0] = ft.A[t][0]; pj[1] = ft.A[t][1]; pj[2] = ft.A[t][2]; msg = ft.A[t][3];
+                    vj[0] = ft.B[t][0]; vj[1] = ft.B[t][1]; vj[2] = ft.B[t][2]; Pj = ft.B[t][3];
+                    irj = ft.C[t][0]; volj = ft.C[t][1];
                 }
                 T Rji[D], Vji[D];
-                if (F32) {
 #pragma unroll
-                    for (int a = 0; a < D; ++a) { Rji[a] = tl.px[a][t] - xit[a]; Vji[a] = tl.vx[a][t] - vit[a]; }
-                } else {
-#pragma unroll
-                    for (int a = 0; a < D; ++a) { Rji[a] = tl.px[a][t] - T(xi[a]); Vji[a] = tl.vx[a][t] - T(vi[a]); }
+                for (int a = 0; a < D; ++a) {
+                    Rji[a] = F32 ? (pj[a] - xit[a]) : T(double(pj[a]) - xi[a]);
+                    Vji[a] = F32 ? (vj[a] - vit[a]) : T(double(vj[a]) - vi[a]);
                 }
-                const T msg = tl.m[t];
                 const bool bound_j = msg < T(0);
                 const T mj = bound_j ? -msg : msg;
-                const T Pj = tl.P[t], irj = tl.ir[t], volj = tl.vol[t];
                 const T idist2 = fast_rcp<T>(rr + eps2);
                 T W, fg;
                 wendland<T>(r, Ht, inv_H, Wc, W, fg);
@@ -1165,10 +1359,17 @@ __global__ __launch_bounds__(TPB) void k_force_t(Consts c, DevState s, Aux ax, D
                 for (int a = 0; a < D; ++a) acc[a] += double(vf * Vji[a] + sf * Rji[a] - pc * g[a]);
                 if (c.ale) {
                     T aj = T(0), ai = T(0), cd = T(0), cont = T(0), c2 = T(0);
-                    const T rhoj = tl.rho[t];
+                    T ej[4];
+                    if constexpr (F32) {
+                        const float4 E = ft.E[t];
+                        ej[0] = E.x; ej[1] = E.y; ej[2] = E.z; ej[3] = E.w;
+                    } else {
+                        ej[0] = ft.E[t][0]; ej[1] = ft.E[t][1]; ej[2] = ft.E[t][2]; ej[3] = ft.E[t][3];
+                    }
+                    const T rhoj = ej[3];
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
-                        const T vpj = tl.vp[a][t];
+                        const T vpj = ej[a];
                         aj += vpj * g[a];
                         ai += vpi[a] * g[a];
                         cd += (vpj - vpi[a]) * g[a];
@@ -1177,8 +1378,8 @@ __global__ __launch_bounds__(TPB) void k_force_t(Consts c, DevState s, Aux ax, D
                     }
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
-                        const T via = F32 ? (T(vref[a]) + vit[a]) : T(vi[a]);
-                        const T vja = F32 ? (T(vref[a]) + tl.vx[a][t]) : tl.vx[a][t];
+                        const T via = T(vi[a]);
+                        const T vja = F32 ? (T(vref[a]) + vj[a]) : vj[a];
                         acc[a] += double((vja * aj + via * ai - via * cd) * volj);
                     }
                     Rrho_ -= double(cont * volj);
@@ -1189,19 +1390,45 @@ __global__ __launch_bounds__(TPB) void k_force_t(Consts c, DevState s, Aux ax, D
                     for (int a = 1; a < D; ++a) vg += Vji[a] * g[a];
                     Rrho_ -= double(volj * vg);
                 }
-                ++lc.k;
-                lc.u = u_next;
-            }
+            FJ_TILE_ENTRIES_END(cu, nl, i)
         }
-        if (!active) continue;
-        double acc_new[D], Rrho_new = 0.0;
+        if (!fluid) continue;
+#pragma unroll
+        for (int a = 0; a < D; ++a) s.acc[i * D + a] = acc[a];
+        s.Rrho[i] = c.ale ? (Rrho_ * rhoi + Rrhoc_ + s.deltaD[i]) : (Rrho_ * rhoi + s.deltaD[i]);
+    }
+}
+
+/* Per-slot tail of get_acc_and_Rrho_on_i (aerodynamic force, artificial
+ * viscosity and gravity added to the pair sum, src/Resid.cpp:381-418) and
+ * the Newmark-Beta update of Do_NB_Iter (src/Newmark_Beta.cpp:137-240) with
+ * the per-block partial of the squared position change. */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_nb_update(Consts c, DevState s, DevState pn, NextState nx, int do_update,
+                                                   const int32_t* __restrict__ part_id,
+                                                   const int32_t* __restrict__ blk, DevBlocks bl,
+                                                   const int32_t* __restrict__ near_inlet,
+                                                   const int32_t* __restrict__ nnb, int64_t n, double dt,
+                                                   double* partial)
+{
+    __shared__ double lds[NWAVE];
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    double resid = 0.0;
+    if (i < n) {
+        const bool fluid = part_id[i] >= c.n_bound;
+        double xi[D], vi[D], acc_new[D];
+        load_vec<D>(s.x, i, xi);
+        load_vec<D>(s.v, i, vi);
         if (fluid) {
+            double acc[D];
+            load_vec<D>(s.acc, i, acc);
             if (s.cellID[i] != -1) {
                 double Vd[D], aero[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) Vd[a] = s.cellV[i * D + a] - vi[a];
                 if (c.aero_case == FJSPH_GISSLER) {
-                    gissler<D>(c, Vd, s.cellRho[i], s.cellP[i], mi, s.lam_nb[i], double(lc.cnt + 1), s.woccl[i], aero);
+                    gissler<D>(c, Vd, s.cellRho[i], s.cellP[i], s.m[i], s.lam_nb[i], double(nnb[i] + 1), s.woccl[i],
+                               aero);
                 } else {
 #pragma unroll
                     for (int a = 0; a < D; ++a) aero[a] = 0.0;
@@ -1214,8 +1441,6 @@ __global__ __launch_bounds__(TPB) void k_force_t(Consts c, DevState s, Aux ax, D
                 acc_new[a] = acc[a] + s.aVisc[i * D + a] + c.grav[a];
                 s.acc[i * D + a] = acc_new[a];
             }
-            Rrho_new = c.ale ? (Rrho_ * rhoi + Rrhoc_ + s.deltaD[i]) : (Rrho_ * rhoi + s.deltaD[i]);
-            s.Rrho[i] = Rrho_new;
         }
         if (do_update) {
             /* Newmark-Beta update (beta = 1/4, gamma = 1/2) */
@@ -1243,7 +1468,7 @@ __global__ __launch_bounds__(TPB) void k_force_t(Consts c, DevState s, Aux ax, D
                         xo[a] = xnew;
                         vo[a] = vn[a] + dt * (g1 * acc_new[a] + g2 * an[a]);
                     }
-                    rho_o = fmax(c.rho_min, fmin(c.rho_max, pn.rho[i] + dt * (g1 * Rrho_new + g2 * pn.Rrho[i])));
+                    rho_o = fmax(c.rho_min, fmin(c.rho_max, pn.rho[i] + dt * (g1 * s.Rrho[i] + g2 * pn.Rrho[i])));
                     p_o = eos_pressure(c, rho_o);
                 } else if (bcur == FJSPH_OUTLET) {
 #pragma unroll
@@ -1284,21 +1509,25 @@ void launch_force(int dim, int prec, const Consts& c, DevState s, Aux a, DevStat
                   DevBlocks bl, const int32_t* near_inlet, TileGrid tg, NbrList nl, int64_t n, double dt,
                   const DevScalars* sc, double* partial, hipStream_t st)
 {
-    (void)n;
+    (void)prec;
     NextState nx{next_x, next_v, next_rho, next_p};
     const int upd = next_x != nullptr;
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2) {
-        if (prec == FJSPH_PREC_MIXED)
-            hipLaunchKernelGGL((k_force_t<2, float, 1024>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, tg, nl, dt, sc, partial);
-        else
-            hipLaunchKernelGGL((k_force_t<2, double, 512>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, tg, nl, dt, sc, partial);
-    } else {
-        if (prec == FJSPH_PREC_MIXED)
-            hipLaunchKernelGGL((k_force_t<3, float, 768>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, tg, nl, dt, sc, partial);
-        else
-            hipLaunchKernelGGL((k_force_t<3, double, 384>), g, b, 0, st, c, s, a, pn, nx, upd, part_id, blk, bl, near_inlet, tg, nl, dt, sc, partial);
-    }
+    if (dim == 2)
+        FJ_PAYLOAD_DISPATCH(nl,
+            hipLaunchKernelGGL((k_force_t<2, float>), g, b, 0, st, c, s, a, part_id, tg, nl, sc),
+            hipLaunchKernelGGL((k_force_t<2, double>), g, b, 0, st, c, s, a, part_id, tg, nl, sc));
+    else
+        FJ_PAYLOAD_DISPATCH(nl,
+            hipLaunchKernelGGL((k_force_t<3, float>), g, b, 0, st, c, s, a, part_id, tg, nl, sc),
+            hipLaunchKernelGGL((k_force_t<3, double>), g, b, 0, st, c, s, a, part_id, tg, nl, sc));
+    const dim3 gu(grid_of(n));
+    if (dim == 2)
+        hipLaunchKernelGGL(k_nb_update<2>, gu, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl.nnb, n,
+                           dt, partial);
+    else
+        hipLaunchKernelGGL(k_nb_update<3>, gu, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl.nnb, n,
+                           dt, partial);
 }
 
 }  // namespace fj
