@@ -9,9 +9,13 @@ per second = particles x steps / wall time, with the particle state resident
 in HBM when the timed region starts.
 
 Prints ONE JSON line with the metric, the roofline of the dominant kernel
-(the fused force + Newmark update) and the CPU baseline (the FP64 oracle
-restatement built with the reference's compiler flags, timed on the host
-cores on a bounded sample of the same workload).
+(k_force_t, the pair sums of get_acc_and_Rrho), per-pass timings and the CPU
+baseline (the FP64 restatement built with the reference's compiler flags,
+timed on the host cores on a bounded sample of the same workload).
+
+`--copy-probe BYTES` instead copies a device buffer of BYTES bytes a few
+times and exits: the known-size kernel that scales the HBM counters of
+rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE).
 """
 import argparse
 import json
@@ -28,6 +32,7 @@ from fjsph_amd import abi, cases  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s and %HBM roofline at 4M 3D particles; surrogate vs mesh aero"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_t_jet4m_traffic.json")
 
 
 def workload(name, n_target):
@@ -44,23 +49,23 @@ def workload(name, n_target):
     raise SystemExit(f"unknown workload {name}")
 
 
-def load_traffic():
-    """HBM bytes per force launch measured with rocprofv3 --pmc (FETCH_SIZE,
-    WRITE_SIZE) and committed as profiles/force_update_traffic.json."""
-    path = os.path.join(ROOT, "profiles", "force_update_traffic.json")
-    if not os.path.exists(path):
-        return None
+def load_traffic(n):
+    """HBM bytes per k_force_t launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
+    rescaled to n particles."""
     try:
-        with open(path) as f:
-            return json.load(f)
-    except (OSError, ValueError):
-        return None
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["bytes_per_particle"] * n, t
+    except (OSError, ValueError, KeyError):
+        return None, None
 
 
 def cpu_baseline(args):
-    """FP64 oracle restatement (reference flags -O3 -ffast-math, OpenMP) on a
+    """FP64 restatement (reference flags -O3 -ffast-math, OpenMP) on a
     bounded sample of the same workload: start-up pass + 2 untimed steps,
-    then `cpu_steps` timed steps."""
+    then `cpu_steps` timed steps with the reference's serial
+    dissipation_terms, then `cpu_steps` more with it parallelised."""
     from oracle.pyoracle import Oracle
     nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if args.workload == "jet4m":
@@ -71,22 +76,49 @@ def cpu_baseline(args):
     o.integrate_no_update()
     for _ in range(2):
         o.integrate_step()
-    its = []
-    t0 = time.perf_counter()
-    for _ in range(args.cpu_steps):
-        its.append(o.integrate_step()["iterations"])
-    dt = time.perf_counter() - t0
+    res = {}
+    for serial in (True, False):
+        o.set_serial_dissipation(serial)
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            its.append(o.integrate_step()["iterations"])
+        res[serial] = (c.n * args.cpu_steps / (time.perf_counter() - t0), float(np.mean(its)))
     o.close()
     return {
-        "value": c.n * args.cpu_steps / dt,
+        "value": res[True][0],
         "unit": "particle-steps/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": (f"{c.name} with {c.n} particles (same dx, H and physics), FP64, steps 3-{2 + args.cpu_steps} "
-                   f"after start-up, {args.cpu_steps} timed steps, mean {np.mean(its):.1f} sub-iterations/step; "
-                   "cell-list neighbour search, serial dissipation_terms as in the reference"),
-        "seconds": dt,
+        "sample": (f"{c.name} with {c.n} particles (same dx, H and physics), FP64, {args.cpu_steps} timed steps "
+                   f"after start-up + 2 steps, mean {res[True][1]:.1f} sub-iterations/step; cell-list neighbour "
+                   "search, serial dissipation_terms as in the reference"),
+        "value_parallel_dissipation": res[False][0],
+        "cpu_model": _cpu_model(),
     }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(nbytes):
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
 
 
 def main():
@@ -99,12 +131,16 @@ def main():
     ap.add_argument("--cpu-particles", type=int, default=60_000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path shards by "
               "spatial domain decomposition, which is described but not built.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     from fjsph_amd.engine import Engine
 
@@ -127,26 +163,40 @@ def main():
     pairs, max_count = eng.neighbour_stats()
     value = case.n * args.steps / elapsed
 
-    # roofline of the dominant kernel: fused force + Newmark update
-    ms, launches = timers["force_update"]
+    passes = {}
+    for name, (ms, launches) in timers.items():
+        if launches == 0:
+            continue
+        avg_s = ms / 1e3 / launches
+        bpp = eng.pass_bytes(abi.TIMER_NAMES.index(name))
+        passes[name] = {
+            "ms_per_step": round(ms / args.steps, 3),
+            "ms_per_launch": round(avg_s * 1e3, 3),
+            "launches_per_step": round(launches / args.steps, 2),
+            "alg_bytes_per_particle": bpp,
+            "alg_GBs": round(bpp * case.n / avg_s / 1e9, 1),
+        }
+
+    # roofline of the dominant kernel: k_force_t (HIP events on the engine stream)
+    ms, launches = timers["force"]
     avg_s = (ms / 1e3) / max(launches, 1)
-    bpp = eng.pass_bytes(abi.TIMER_NAMES.index("force_update"))
-    alg_bytes = bpp * case.n
-    achieved = alg_bytes / avg_s / 1e9
-    traffic = load_traffic()
+    bpp = eng.pass_bytes(abi.TIMER_NAMES.index("force"))
+    achieved = bpp * case.n / avg_s / 1e9
+    traffic, tmeta = load_traffic(case.n)
     roofline = {
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": (traffic or {}).get("bytes_per_launch"),
-        "kernel": "k_force (fused get_acc_and_Rrho + Newmark update)",
-        "bytes_per_particle": bpp,
+        "traffic": traffic,
+        "kernel": "k_force_t<3,float> (pair sums of get_acc_and_Rrho_on_i)",
+        "alg_bytes_per_particle": bpp,
         "avg_launch_ms": avg_s * 1e3,
         "launches": launches,
+        "pairs_per_launch": pairs + case.n,
+        "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tmeta else None,
     }
-    pass_ms = {k: round(v[0] / args.steps, 3) for k, v in timers.items()}
 
     cpu = None
     if not args.no_cpu:
@@ -172,7 +222,7 @@ def main():
             "mean_neighbours": pairs / case.n + 1,
             "max_neighbours": max_count + 1,
             "subiterations_per_step": float(np.mean(its)),
-            "pass_ms_per_step": pass_ms,
+            "passes": passes,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

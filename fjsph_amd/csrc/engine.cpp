@@ -536,11 +536,15 @@ void boundary_update(fjsph_ctx* ctx)
 
 void force(fjsph_ctx* ctx, bool update)
 {
-    PassTimer t(ctx, FJSPH_T_FORCE_UPDATE);
     refresh_aux(ctx);
-    launch_force(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->pn, update ? ctx->x_nxt : nullptr,
-                 ctx->v_nxt, ctx->rho_nxt, ctx->p_nxt, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->tg,
-                 ctx->nl, ctx->n, ctx->delta_t, ctx->sc, ctx->partial, ctx->st);
+    {
+        PassTimer t(ctx, FJSPH_T_FORCE);
+        launch_force(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->sc, ctx->st);
+    }
+    PassTimer t(ctx, FJSPH_T_UPDATE);
+    launch_nb_update(ctx->s.dim, ctx->c, ctx->pnp1, ctx->pn, update ? ctx->x_nxt : nullptr, ctx->v_nxt,
+                     ctx->rho_nxt, ctx->p_nxt, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->nl, ctx->n,
+                     ctx->delta_t, ctx->partial, ctx->st);
 }
 
 /* Do_NB_Iter + the residual of Check_Error; returns log10 rms. */
@@ -1224,13 +1228,20 @@ fjsph_status fjsph_pass_bytes(fjsph_ctx* ctx, int32_t pass, double* bytes_per_pa
     const double ale = ctx->s.ale ? 1.0 : 0.0;
     double b = 0.0;
     switch (pass) {
-    case FJSPH_T_FORCE_UPDATE:
-        /* reads: x, v (2*8D), rho, p, m (24), b, part_id, blk, cellID, count
-         * (20), cellV (8D), cellP, cellRho, lam_nb, woccl, deltaD (40),
-         * aVisc (8D), pn x, v, acc (3*8D), pn rho, Rrho (16), ALE vPert
-         * (8D) + surfzone (4); writes: acc, Af (2*8D), Rrho (8), next x, v
-         * (2*8D), next rho, p (16). */
-        b = 8 * D * (2 + 1 + 1 + 3 + 2 + 2) + 24 + 20 + 40 + 16 + 8 + 16 + ale * (8 * D + 4);
+    case FJSPH_T_FORCE:
+        /* k_force_t, once per slot: reads order (4), x, v (2*8D), the FP32
+         * record {m, p/rho^2, 1/rho, m/rho} (16), rho, m, p/rho^2, 1/rho,
+         * deltaD (40), part_id, b (8), ALE vPert (8D) + surfzone (4);
+         * writes acc (8D), Rrho (8). */
+        b = 4 + 16 * D + 16 + 40 + 8 + 8 * D + 8 + ale * (8 * D + 4);
+        break;
+    case FJSPH_T_UPDATE:
+        /* k_nb_update: reads part_id, b, cellID, neighbour count (16),
+         * x, v, acc, aVisc, cellV (5*8D), cellRho, cellP, m, lam_nb, woccl
+         * (40), pn x, v, acc (3*8D), Rrho, rho, p, pn rho, pn Rrho (40),
+         * ALE vPert (8D); writes Af, acc (2*8D), next x, v (2*8D), next
+         * rho, p (16). */
+        b = 16 + 40 * D + 40 + 24 * D + 40 + ale * 8 * D + 32 * D + 16;
         break;
     case FJSPH_T_DSPH:
         /* reads x (8D), rho, m, b, part_id, count; writes L (8D^2), gradRho,

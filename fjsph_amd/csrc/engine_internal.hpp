@@ -139,10 +139,12 @@ void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t*
 /* Force pass on `s`; when next_x is non-null the Newmark update of
  * Do_NB_Iter is fused and writes next x/v/rho/p for every slot, plus one
  * residual partial sum per tile block. */
-void launch_force(int dim, int prec, const Consts& c, DevState s, Aux a, DevState pn, double* next_x,
-                  double* next_v, double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk,
-                  DevBlocks bl, const int32_t* near_inlet, TileGrid tg, NbrList nl, int64_t n, double dt,
-                  const DevScalars* sc, double* partial, hipStream_t st);
+void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                  const DevScalars* sc, hipStream_t st);
+/* next_x == nullptr: only the acceleration tail, no Newmark update. */
+void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double* next_x, double* next_v,
+                      double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk, DevBlocks bl,
+                      const int32_t* near_inlet, NbrList nl, int64_t n, double dt, double* partial, hipStream_t st);
 void launch_sum_partials(const double* partial, int nparts, int width, double* out, hipStream_t st);
 void launch_sum_partials_scaled(const double* partial, int nparts, int width, double scale, double* out,
                                 hipStream_t st);

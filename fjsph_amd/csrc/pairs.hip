@@ -1504,14 +1504,9 @@ __global__ __launch_bounds__(TPB) void k_nb_update(Consts c, DevState s, DevStat
     }
 }
 
-void launch_force(int dim, int prec, const Consts& c, DevState s, Aux a, DevState pn, double* next_x,
-                  double* next_v, double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk,
-                  DevBlocks bl, const int32_t* near_inlet, TileGrid tg, NbrList nl, int64_t n, double dt,
-                  const DevScalars* sc, double* partial, hipStream_t st)
+void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
+                  const DevScalars* sc, hipStream_t st)
 {
-    (void)prec;
-    NextState nx{next_x, next_v, next_rho, next_p};
-    const int upd = next_x != nullptr;
     const dim3 g((unsigned)tg.nblocks), b(TPB);
     if (dim == 2)
         FJ_PAYLOAD_DISPATCH(nl,
@@ -1521,13 +1516,21 @@ void launch_force(int dim, int prec, const Consts& c, DevState s, Aux a, DevStat
         FJ_PAYLOAD_DISPATCH(nl,
             hipLaunchKernelGGL((k_force_t<3, float>), g, b, 0, st, c, s, a, part_id, tg, nl, sc),
             hipLaunchKernelGGL((k_force_t<3, double>), g, b, 0, st, c, s, a, part_id, tg, nl, sc));
-    const dim3 gu(grid_of(n));
+}
+
+void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double* next_x, double* next_v,
+                      double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk, DevBlocks bl,
+                      const int32_t* near_inlet, NbrList nl, int64_t n, double dt, double* partial, hipStream_t st)
+{
+    NextState nx{next_x, next_v, next_rho, next_p};
+    const int upd = next_x != nullptr;
+    const dim3 g(grid_of(n)), b(TPB);
     if (dim == 2)
-        hipLaunchKernelGGL(k_nb_update<2>, gu, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl.nnb, n,
-                           dt, partial);
+        hipLaunchKernelGGL(k_nb_update<2>, g, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl.nnb, n, dt,
+                           partial);
     else
-        hipLaunchKernelGGL(k_nb_update<3>, gu, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl.nnb, n,
-                           dt, partial);
+        hipLaunchKernelGGL(k_nb_update<3>, g, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl.nnb, n, dt,
+                           partial);
 }
 
 }  // namespace fj

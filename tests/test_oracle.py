@@ -375,3 +375,110 @@ def test_detect_surface_square_block(po):
     deep = np.all((x > lo + 3.5 * dx) & (x < hi - 3.5 * dx), axis=1)
     assert np.all(surf[edge] == 1)
     assert np.all(surf[deep] == 0)
+
+
+# ------------------------------------------------------- boundary pressure
+
+def test_boundary_pressure_hydrostatic(po):
+    """Adami boundary pressure (src/Resid.cpp:21-80) reproduces a hydrostatic
+    column exactly: p_w = sum_f V W (p_f + rho g.(x_w - x_f)) / sum_f V W is
+    a weighted mean of the exact wall value when p_f is linear in depth."""
+    c = cases.fluid_block(2, 12, with_walls=True, layers=3, H_fac=1.3, mu=0.0)
+    s = c.settings
+    g = float(s.grav[1])
+    y_top = c.x[c.n_bound:, 1].max() + 0.5 * s.particle_step
+    c.p = s.rho_rest * g * (c.x[:, 1] - y_top)
+    c.p[:c.n_bound] = 0.0
+    o = po.Oracle(c, nthreads=2)
+    o.build_neighbours()
+    o.dsph_prestep()
+    o.boundary_update()
+    pw = o.field("P")[:c.n_bound]
+    exact = s.rho_rest * g * (c.x[:c.n_bound, 1] - y_top)
+    off, idx, _ = o.neighbours()
+    has_fluid = np.array([np.any(idx[off[k]:off[k + 1]] >= c.n_bound) for k in range(c.n_bound)])
+    assert has_fluid.sum() > 30
+    assert np.allclose(pw[has_fluid], exact[has_fluid], rtol=1e-10, atol=0)
+    assert np.all(pw[~has_fluid] == 0.0)
+
+
+# ---------------------------------------------------------- particle shift
+
+def _shift_case(displace):
+    c = cases.fluid_block(3, 9, H_fac=1.3, mu=0.0)
+    s = c.settings
+    s.ale = 1
+    dx = s.particle_step
+    c.v = np.zeros_like(c.x)
+    c.v[:, 0] = 1.0 + 50.0 * c.x[:, 1]          # sheared, so max |v_j - v_i| > 0
+    centre = np.argmin(np.linalg.norm(c.x - c.x.mean(axis=0), axis=1))
+    c.x[centre, 0] += displace * dx
+    return c, centre
+
+
+@pytest.mark.parametrize("displace", [0.0, 0.1, -0.1])
+def test_particle_shift_restores_lattice(po, displace):
+    """particle_shift (src/Shifting.cpp:189-290) is a Fickian shift down the
+    concentration gradient: zero on a perfect interior lattice, and pointing
+    back to the lattice site for a displaced particle, capped at half the
+    largest neighbour velocity difference."""
+    c, i = _shift_case(displace)
+    o = po.Oracle(c, nthreads=2)
+    o.build_neighbours()
+    o.dsph_prestep()
+    o.detect_surface()
+    o.particle_shift()
+    vp = o.field("VPERT")
+    v = c.v
+    off, idx, _ = o.neighbours()
+    maxU = max(np.linalg.norm(v[j] - v[i]) for j in idx[off[i]:off[i + 1]])
+    assert np.linalg.norm(vp[i]) <= maxU / 2 * (1 + 1e-12)
+    if displace == 0.0:
+        assert np.linalg.norm(vp[i]) < 1e-12 * np.linalg.norm(v[i])
+    else:
+        assert np.sign(vp[i][0]) == -np.sign(displace)
+        assert abs(vp[i][0]) > 10 * (abs(vp[i][1]) + abs(vp[i][2]))
+
+
+# ------------------------------------------------------------ Gissler drag
+
+def _sphere_cd_standard(re):
+    """Standard drag curve of a sphere (Clift, Grace & Weber 1978 tabulation,
+    Re = 1, 10, 100): independent of the correlation under test."""
+    return {1.0: 26.5, 10.0: 4.15, 100.0: 1.09}[re]
+
+
+@pytest.mark.parametrize("re_target", [1.0, 10.0, 100.0])
+def test_gissler_isolated_droplet_drag(po, re_target):
+    """Gissler force (src/Aero.h:37-98) on an isolated particle (lambda = 0,
+    so the sphere branch with no occlusion): a = 1/2 rho_g |U| U Cd A / m
+    with A the cross-section of the volume-equivalent sphere and Re =
+    rho_g |U| 2L / mu_g; the correlation's Cd lies within 12% of the
+    standard drag curve."""
+    s = abi.default_settings(3)
+    s.particle_step = 1e-4
+    s.H_fac = 1.3
+    s.aero_case = abi.GISSLER
+    for k in range(3):
+        s.grav[k] = 0.0
+    d = po.derive(s)
+    L = s.particle_step * (3 / (4 * math.pi)) ** (1 / 3)
+    U = re_target * s.mu_g / (1.2 * 2 * L)
+    cases._gas(s, U, 1.2)
+    o = po.Oracle(s, nthreads=1)
+    m = 1000.0 * s.particle_step ** 3
+    o.upload_state(np.zeros((1, 3)), np.zeros((1, 3)), np.array([1000.0]), np.array([0.0]), np.array([m]),
+                   np.array([abi.FREE], np.int32), 0)
+    o.set_blocks([dict(first=0, last=1, is_fluid=1)])
+    o.build_neighbours()
+    npd = o.dsph_prestep()
+    o.aero_velocity()
+    o.detect_surface()
+    o.dissipation_terms()
+    o.acc_rrho(npd if npd > 0 else 1.0)
+    a = o.field("AF")[0]
+    assert a[1] == 0.0 and a[2] == 0.0
+    cd = a[0] * m / (0.5 * 1.2 * U * U * math.pi * L * L)
+    assert cd == pytest.approx(_sphere_cd_standard(re_target), rel=0.12)
+    assert o.field("ACC")[0][0] == pytest.approx(a[0], rel=1e-14)
+    del d
