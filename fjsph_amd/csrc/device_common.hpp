@@ -312,6 +312,19 @@ __device__ __forceinline__ T fast_rcp(T x) { return T(1) / x; }
 template <>
 __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
+/* Square root and cosine of the FP32 pair terms: the hardware instructions
+ * (v_sqrt_f32, about 1 ulp; v_cos_f32, absolute error near 1e-6 for the
+ * arguments in [0, 3pi/2] used here) instead of the correctly rounded
+ * library sequences.  FP64: the library functions. */
+template <typename T>
+__device__ __forceinline__ T fast_sqrt(T x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <typename T>
+__device__ __forceinline__ T fast_cos(T x) { return cos(x); }
+template <>
+__device__ __forceinline__ float fast_cos<float>(float x) { return __cosf(x); }
+
 }  // namespace fj
 
 #endif

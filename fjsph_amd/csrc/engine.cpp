@@ -123,7 +123,7 @@ struct fjsph_ctx {
 
     /* cell grid */
     int32_t *cell = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
-    int32_t *order = nullptr, *tile_sums = nullptr;
+    int32_t *order = nullptr, *pos = nullptr, *tile_sums = nullptr;
     int64_t ncell_cap = 0, ncell = 0;
     int32_t dims[3] = {1, 1, 1};
 
@@ -228,12 +228,10 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->part_id); dfree(ctx->blk); dfree(ctx->near_inlet); dfree(ctx->norms_tmp);
     dfree(ctx->partial); dfree(ctx->gather_tmp); dfree(ctx->gather_tmp_i);
     dfree(ctx->cell); dfree(ctx->cell_count); dfree(ctx->cell_start); dfree(ctx->cell_fill);
-    dfree(ctx->order); dfree(ctx->tile_sums);
+    dfree(ctx->order); dfree(ctx->pos); dfree(ctx->tile_sums);
     dfree(ctx->nl.lu4); dfree(ctx->nl.count); dfree(ctx->nl.nnb); dfree(ctx->export_buf);
-    if (ctx->nl.r) (void)hipFree(ctx->nl.r);
-    ctx->nl.r = nullptr;
     dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
-    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb);
+    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb);
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
     ctx->ncell_cap = 0;
     ctx->nl.cap = 0;
@@ -396,6 +394,8 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         permute_all(ctx, ctx->order);
         launch_iota(ctx->order, n, ctx->st);
     }
+    launch_positions(ctx->order, n, ctx->pos, ctx->st);
+    HIPCHK(hipMemcpyAsync(ctx->aux.xb, ctx->pnp1.x, sizeof(double) * n * D, hipMemcpyDeviceToDevice, ctx->st));
     /* tile grid: chunks of C cells sized for about one workgroup of targets */
     {
         TileGrid& tg = ctx->tg;
@@ -408,6 +408,7 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         tg.nblocks = int64_t(ctx->dims[1]) * ctx->dims[2] * tg.nchunk;
         tg.start = ctx->cell_start;
         tg.order = ctx->order;
+        tg.pos = ctx->pos;
         tg.cell = ctx->cell;
         if (tg.nblocks > 0x7fffffff) throw std::string("tile grid too large");
         if (tg.nblocks > ctx->partial_cap) {
@@ -430,8 +431,6 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         /* capacity in entries, a multiple of the 4-entry batch */
         const int32_t cap = ((mc + mc / 8 + 8) + 7) / 8 * 8;
         dalloc(ctx->nl.lu4, size_t(cap / 4) * ctx->nl.npad);
-        if (ctx->nl.f32) dalloc(*(float**)&ctx->nl.r, size_t(cap) * ctx->nl.npad);
-        else dalloc(*(double**)&ctx->nl.r, size_t(cap) * ctx->nl.npad);
         ctx->nl.cap = cap;
     }
     if (ctx->max_count > ctx->nl.cap) throw std::string("neighbour list capacity could not be established");
@@ -539,7 +538,8 @@ void force(fjsph_ctx* ctx, bool update)
     refresh_aux(ctx);
     {
         PassTimer t(ctx, FJSPH_T_FORCE);
-        launch_force(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->sc, ctx->st);
+        launch_force(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, r_from_x(ctx), ctx->sc,
+                     ctx->st);
     }
     PassTimer t(ctx, FJSPH_T_UPDATE);
     launch_nb_update(ctx->s.dim, ctx->c, ctx->pnp1, ctx->pn, update ? ctx->x_nxt : nullptr, ctx->v_nxt,
@@ -828,6 +828,7 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
         dalloc(ctx->gather_tmp_i, size_t(n_total));
         dalloc(ctx->cell, size_t(n_total));
         dalloc(ctx->order, size_t(n_total));
+        dalloc(ctx->pos, size_t(n_total));
         dalloc(ctx->aux.vol, size_t(n_total));
         dalloc(ctx->aux.P, size_t(n_total));
         dalloc(ctx->aux.invrho, size_t(n_total));
@@ -835,6 +836,7 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
         dalloc(ctx->aux.fale, size_t(n_total));
         dalloc(ctx->aux.nhat, size_t(n_total) * D);
         dalloc(ctx->aux.lnb, size_t(n_total) * D * D);
+        dalloc(ctx->aux.xb, size_t(n_total) * D);
         HIPCHK(hipMemsetAsync(ctx->aux.nhat, 0, sizeof(double) * n_total * D, ctx->st));
         ctx->nl.f32 = ctx->s.precision == FJSPH_PREC_MIXED ? 1 : 0;
         ctx->nl.npad = (n_total + 63) / 64 * 64;

@@ -7,8 +7,9 @@
  *     step reads, FP64, vectors interleaved per particle (x[s*D + k]);
  *   - x, v, rho, p of pnp1 exist twice (cur / next) so the fused
  *     force + Newmark kernel reads iterate k and writes k+1 without races;
- *   - neighbour lists column-major: idx[k*npad + s], frozen squared distance
- *     rr[k*npad + s] (FP64), count[s] (self excluded).
+ *   - neighbour lists as rows of four 16-bit union indices per target,
+ *     row-major by row and column-major by the target's position in cell
+ *     order (see NbrList).
  */
 #ifndef FJSPH_ENGINE_INTERNAL_HPP
 #define FJSPH_ENGINE_INTERNAL_HPP
@@ -38,22 +39,21 @@ struct DevState {
     int32_t *surf, *cellID, *surfzone, *b;
 };
 
-/* Neighbour lists in batches of 4 entries, column-major by slot: batch b of
- * slot i is lu4[b*npad + i] (four 16-bit indices into the candidate union
- * of the slot's tile block, ascending) and r[b*npad + i] (four payloads).
- * Batches never straddle a TILE of the union; unused entries hold 0xFFFF.
- * The payload is the distance frozen at build time (src/Resid.cpp:288-289
- * reads it from the list): FP64 d^2 on the parity path, FP32 r on the
- * mixed path.  count[i] counts entries including padding, nnb[i] the
- * neighbours (self excluded). */
+/* Neighbour lists in rows of 4 entries: row r of the target at position p
+ * of the cell order is lu4[r*npad + p], four 16-bit indices into the
+ * candidate union of the target's tile block, ascending, all in one TILE
+ * of the union; unused entries hold NO_ENTRY.  The targets of one wave
+ * hold the same number of rows per tile (padding rows are all NO_ENTRY).
+ * count[i] is the row count of slot i, nnb[i] its neighbours (self
+ * excluded).  The frozen distance of src/Resid.cpp:288-289 is not stored:
+ * it is recomputed from the build positions Aux::xb. */
 struct NbrList {
     uint64_t* lu4;
-    void* r;
     int32_t* count;
     int32_t* nnb;
     int64_t npad;
-    int32_t cap;  /* entries per slot (multiple of 4) */
-    int32_t f32;  /* payload is float r (1) or double d^2 (0) */
+    int32_t cap;  /* entries per slot (4 per row) */
+    int32_t f32;  /* mixed precision: FP32 pair terms */
 };
 
 /* Candidate-union tile shared by the list build and every pair pass. */
@@ -70,6 +70,7 @@ struct TileGrid {
     const int32_t* start; /* cell_start, ncell + 1 */
     const int32_t* order; /* slots in cell order */
     const int32_t* cell;  /* cell of every slot */
+    const int32_t* pos;   /* position of every slot in order[] */
 };
 
 /* Per-particle quantities derived from the current rho, p, m, b (and vPert),
@@ -83,6 +84,7 @@ struct Aux {
     double* nhat;    /* normalised surface normal (after Detect_Surface)     */
     double* lnb;     /* dSPH_PreStep: correction matrix over non-boundary
                         neighbours (D x D per slot), for its eigenvalue     */
+    double* xb;      /* positions at the last neighbour build (D per slot)   */
 };
 
 /* Per-block boundary settings visible to kernels. */
@@ -115,14 +117,14 @@ void launch_sort_cells(const int32_t* start, int64_t ncell, int32_t* order, hipS
 void launch_build_lists(int dim, const double* x, int64_t n, TileGrid tg, double sr, NbrList nl, DevScalars* sc,
                         hipStream_t st);
 void launch_export_lists(int dim, TileGrid tg, NbrList nl, int64_t n, int32_t* out, hipStream_t st);
+void launch_positions(const int32_t* order, int64_t n, int32_t* pos, hipStream_t st);
 void launch_aux(int dim, DevState s, Aux a, int64_t n, int ale, int mixed, hipStream_t st);
 void launch_gather_d(const double* src, double* dst, const int32_t* perm, int64_t n, int width, hipStream_t st);
 void launch_gather_i(const int32_t* src, int32_t* dst, const int32_t* perm, int64_t n, hipStream_t st);
 void launch_iota(int32_t* a, int64_t n, hipStream_t st);
 
-/* Pre-passes.  r_from_x = 1 when the positions are those of the last
- * neighbour build (then r is recomputed bit-identically from x instead of
- * being read from the list). */
+/* Pair passes.  r_from_x = 1 when the positions are those of the last
+ * neighbour build (r is then formed from x); otherwise from Aux::xb. */
 void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
                  int64_t n, int r_from_x, int with_aero_vel, double* partial, DevScalars* sc, hipStream_t st);
 void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* part_id, NbrList nl, int64_t n,
@@ -140,7 +142,7 @@ void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t*
  * Do_NB_Iter is fused and writes next x/v/rho/p for every slot, plus one
  * residual partial sum per tile block. */
 void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
-                  const DevScalars* sc, hipStream_t st);
+                  int r_from_x, const DevScalars* sc, hipStream_t st);
 /* next_x == nullptr: only the acceleration tail, no Newmark update. */
 void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double* next_x, double* next_v,
                       double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk, DevBlocks bl,

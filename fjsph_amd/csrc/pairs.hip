@@ -2,12 +2,22 @@
  * Block-tiled pair passes of the WCSPH step.
  *
  * Work decomposition: a block is a chunk of C consecutive cells of one cell
- * row; its targets are the particles of those cells (enumerated through the
- * cell-sorted order[] array) and its candidate union is the 3^(D-1)
+ * row; its targets are the particles of those cells (positions [p0, p1) of
+ * the cell-sorted order[] array) and its candidate union is the 3^(D-1)
  * neighbour rows over cells [cx0 - 1, cx1].  The union is staged in LDS in
- * tiles with coalesced loads; each lane walks its own neighbour list, which
- * holds union-local 16-bit indices in ascending order, tile by tile.  Pair
- * terms never scatter: every pass is owner-computes.
+ * tiles of TILE records with coalesced loads; each lane walks its own
+ * neighbour list tile by tile.  Pair terms never scatter: every pass is
+ * owner-computes.
+ *
+ * Neighbour lists (built by k_build_t, read by every pass): rows of four
+ * 16-bit union indices, row r of the target at position p stored at
+ * lu4[r*npad + p].  Within a row the indices ascend and lie in one union
+ * tile; unused entries hold NO_ENTRY.  The 64 targets of a wave hold the
+ * same number of rows for every tile (rows padded to the wave's maximum), so
+ * a wave walks its lists in lock-step: every row load is one coalesced
+ * 512-byte read.  Distances are not stored: the reference's frozen distance
+ * (sqrt of the d^2 of the last build, src/Resid.cpp:288-289) is recomputed
+ * from the positions of the last build (Aux::xb).
  *
  * Reference correspondence (file:line):
  *   k_build_t      update_neighbours            src/Neighbours.cpp:7-47
@@ -18,14 +28,13 @@
  *   k_shift_t      particle_shift               src/Shifting.cpp:189-290
  *   k_bnd_*        boundary solvers             src/Resid.cpp:21-186, src/Newmark_Beta.cpp:69-132
  *   k_force_t      get_acc_and_Rrho_on_i        src/Resid.cpp:243-418
- *                  + Do_NB_Iter update          src/Newmark_Beta.cpp:137-240
+ *   k_nb_update    Do_NB_Iter update            src/Newmark_Beta.cpp:137-240
  */
 #include "device_common.hpp"
 
 namespace fj {
 
 constexpr int MAXSEG = 9;
-constexpr int BIG_U = 0x7fffffff;
 
 /* Union of block b (see file header). seg_a[k] is the order[] start of
  * segment k, pre[k] its first union index; the target positions in order[]
@@ -94,113 +103,69 @@ __device__ __forceinline__ void block_head(const TileGrid& tg, BlockHead& h)
     __syncthreads();
 }
 
-/* ------------------------------------------------- batched list access */
+/* ------------------------------------------------------------ list rows */
 
-template <typename RT>
-__device__ __forceinline__ void load_batch(const NbrList& nl, int64_t bidx, int64_t i, uint64_t& u, RT* r)
-{
-    const int64_t e = bidx * nl.npad + i;
-    u = nl.lu4[e];
-    if constexpr (sizeof(RT) == 4) {
-        const float4 q = ((const float4*)nl.r)[e];
-        r[0] = q.x; r[1] = q.y; r[2] = q.z; r[3] = q.w;
-    } else {
-        const double2* p = (const double2*)nl.r + 2 * e;
-        const double2 a = p[0], c = p[1];
-        r[0] = a.x; r[1] = a.y; r[2] = c.x; r[3] = c.y;
-    }
-}
-
-template <typename RT>
-__device__ __forceinline__ void store_batch(const NbrList& nl, int64_t bidx, int64_t i, uint64_t u, const RT* r)
-{
-    const int64_t e = bidx * nl.npad + i;
-    nl.lu4[e] = u;
-    if constexpr (sizeof(RT) == 4) {
-        ((float4*)nl.r)[e] = make_float4(r[0], r[1], r[2], r[3]);
-    } else {
-        double2* p = (double2*)nl.r + 2 * e;
-        p[0] = make_double2(r[0], r[1]);
-        p[1] = make_double2(r[2], r[3]);
-    }
-}
-
-/* Walks one slot's list batch by batch with the next batch in flight. */
-template <typename RT>
-struct Cursor {
-    int b, nb;
+/* Lock-step walk of one target's rows: `row` advances in every lane of the
+ * wave together; the next row is in flight while the current one is used.
+ * Rows past nrows read as all-NO_ENTRY. */
+struct Rows {
+    int64_t p;
+    int row, nrows;
     uint64_t u, un;
-    RT r[4], rn[4];
-    __device__ __forceinline__ void init(const NbrList& nl, int64_t i, bool active)
+    __device__ __forceinline__ void init(const NbrList& nl, int64_t p_, int nrows_)
     {
-        nb = active ? (nl.count[i] >> 2) : 0;
-        b = 0;
-        u = ~0ull;
-        un = ~0ull;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { r[q] = RT(0); rn[q] = RT(0); }
-        if (nb > 0) load_batch<RT>(nl, 0, i, u, r);
-        if (nb > 1) load_batch<RT>(nl, 1, i, un, rn);
+        p = p_;
+        row = 0;
+        nrows = nrows_;
+        u = nrows > 0 ? nl.lu4[p] : ~0ull;
+        un = nrows > 1 ? nl.lu4[nl.npad + p] : ~0ull;
     }
-    __device__ __forceinline__ int first() const { return b < nb ? int(u & 0xFFFFull) : BIG_U; }
-    __device__ __forceinline__ void advance(const NbrList& nl, int64_t i)
+    __device__ __forceinline__ void advance(const NbrList& nl)
     {
-        ++b;
+        ++row;
         u = un;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) r[q] = rn[q];
-        if (b + 1 < nb) load_batch<RT>(nl, b + 1, i, un, rn);
+        un = (row + 1 < nrows) ? nl.lu4[(int64_t)(row + 1) * nl.npad + p] : ~0ull;
     }
 };
 
-/* FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base) { body } FJ_TILE_ENTRIES_END
- * runs body for every list entry of slot i that lies in the union tile
- * [base, base + TILE), with `t` its tile-local index and `pay` its payload.
- * All entries of a batch lie in one tile, so a batch is taken whole; its 4
- * entries go through one copy of the body (a rolled loop rotating the batch
- * through the first register slot).  The body is written inline, not as a
- * closure, so per-lane accumulator arrays stay in registers. */
-#define FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base)                                   \
-    while ((cu).first() < (base) + TILE) {                                             \
-        uint64_t fj_u = (cu).u;                                                        \
-        RT fj_r0 = (cu).r[0], fj_r1 = (cu).r[1], fj_r2 = (cu).r[2], fj_r3 = (cu).r[3]; \
-        _Pragma("unroll 1") for (int fj_q = 0; fj_q < 4; ++fj_q)                      \
-        {                                                                              \
-            const uint32_t fj_uq = uint32_t(fj_u) & 0xFFFFu;                           \
-            const int t = int(fj_uq) - (base);                                         \
-            const RT pay = fj_r0;                                                      \
-            fj_u >>= 16;                                                               \
-            fj_r0 = fj_r1;                                                             \
-            fj_r1 = fj_r2;                                                             \
-            fj_r2 = fj_r3;                                                             \
-            if (fj_uq == NO_ENTRY) continue;
+/* FJ_TILE_ENTRIES_BEGIN(cu, nl, base) { body } FJ_TILE_ENTRIES_END(cu, nl)
+ * runs body for every list entry of the lane's target in the union tile
+ * [base, base + TILE), with `t` the tile-local index.  A row belongs to the
+ * tile when its first index does; padding rows read NO_ENTRY first, so a
+ * lane is idle in them.  The wave leaves the loop at the first row where
+ * no lane has an entry of the tile, which is the first row of the next
+ * tile in every lane.  The 4 entries of a row go through one copy of the
+ * body (a rolled loop); the body is written inline, not as a closure, so
+ * per-lane accumulator arrays stay in registers. */
+#define FJ_TILE_ENTRIES_BEGIN(cu, nl, base)                                               \
+    for (;;) {                                                                            \
+        const bool fj_act = (uint32_t((cu).u) & 0xFFFFu) < uint32_t((base) + TILE);        \
+        if (!__any(fj_act)) break;                                                        \
+        if (fj_act) {                                                                     \
+            uint64_t fj_u = (cu).u;                                                       \
+            _Pragma("unroll 1") for (int fj_q = 0; fj_q < 4; ++fj_q)                     \
+            {                                                                             \
+                const uint32_t fj_uq = uint32_t(fj_u) & 0xFFFFu;                          \
+                fj_u >>= 16;                                                              \
+                if (fj_uq == NO_ENTRY) continue;                                          \
+                const int t = int(fj_uq) - (base);
 
-#define FJ_TILE_ENTRIES_END(cu, nl, i) \
-        }                              \
-        (cu).advance(nl, i);           \
+#define FJ_TILE_ENTRIES_END(cu, nl) \
+            }                       \
+        }                           \
+        (cu).advance(nl);           \
     }
 
-/* Distance of a pair as the reference uses it: sqrt of the d^2 frozen at the
- * last build; recomputed bit-identically from x while the positions are
- * still those of the build (r_from_x). */
-template <int D, typename RT>
-__device__ __forceinline__ void pair_r(int r_from_x, const double* xi, const double* xj, RT pay, double& d2, double& r)
+__device__ __forceinline__ int wave_max(int v)
 {
-    if (r_from_x) {
-        d2 = exact_dist2<D>(xi, xj);
-        r = sqrt(d2);
-    } else if (sizeof(RT) == 4) {
-        r = double(pay);
-        d2 = r * r;
-    } else {
-        d2 = double(pay);
-        r = sqrt(d2);
-    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
 /* ------------------------------------------------------------ list build */
 
-template <int D, typename RT>
+template <int D>
 __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, TileGrid tg, double sr, NbrList nl,
                                                  DevScalars* sc)
 {
@@ -210,11 +175,11 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
     if (h.p0 == h.p1) return;
-    if (U > 65535) {
+    if (U >= int(NO_ENTRY)) {
         if (threadIdx.x == 0) atomicMax(&sc->overflow, U);
         return;
     }
-    const int bcap = nl.cap >> 2;
+    const int rcap = nl.cap >> 2;
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
@@ -222,9 +187,7 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
         double xi[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) xi[a] = active ? x[i * D + a] : 0.0;
-        int nbat = 0, q = 0, nreal = 0;
-        uint64_t cu = ~0ull;
-        RT cr[4] = {RT(0), RT(0), RT(0), RT(0)};
+        int nrow = 0, nreal = 0;
         for (int base = 0; base < U; base += TILE) {
             const int nt = min(TILE, U - base);
             __syncthreads();
@@ -235,7 +198,10 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
                 for (int a = 0; a < D; ++a) tp[a][t] = x[(int64_t)sl * D + a];
             }
             __syncthreads();
+            int rows_t = 0;
             if (active) {
+                int q = 0;
+                uint64_t cu = ~0ull;
                 for (int t = 0; t < nt; ++t) {
                     double xj[D];
 #pragma unroll
@@ -243,37 +209,50 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
                     const double d2 = exact_dist2<D>(xi, xj);
                     if (d2 < sr && tsl[t] != (int32_t)i) {
                         cu = (cu >> 16) | (uint64_t(base + t) << 48);
-                        cr[0] = cr[1]; cr[1] = cr[2]; cr[2] = cr[3];
-                        cr[3] = (sizeof(RT) == 4) ? RT(sqrt(d2)) : RT(d2);
-                        ++q;
                         ++nreal;
-                        if (q == 4) {
-                            if (nbat < bcap) store_batch<RT>(nl, nbat, i, cu, cr);
-                            ++nbat;
+                        if (++q == 4) {
+                            if (nrow < rcap) nl.lu4[(int64_t)nrow * nl.npad + p] = cu;
+                            ++nrow;
+                            ++rows_t;
                             q = 0;
                             cu = ~0ull;
                         }
                     }
                 }
-                /* close the tile: pad the open batch so none straddles a tile */
                 if (q > 0) {
-                    for (int k = q; k < 4; ++k) {
-                        cu = (cu >> 16) | (uint64_t(NO_ENTRY) << 48);
-                        cr[0] = cr[1]; cr[1] = cr[2]; cr[2] = cr[3]; cr[3] = RT(0);
-                    }
-                    if (nbat < bcap) store_batch<RT>(nl, nbat, i, cu, cr);
-                    ++nbat;
-                    q = 0;
-                    cu = ~0ull;
+                    /* close the tile's last row: entries to the low slots */
+                    cu = (cu >> (16 * (4 - q))) | (~0ull << (16 * q));
+                    if (nrow < rcap) nl.lu4[(int64_t)nrow * nl.npad + p] = cu;
+                    ++nrow;
+                    ++rows_t;
+                }
+            }
+            /* pad every lane of the wave to the wave's row count of this tile */
+            const int wrows = wave_max(rows_t);
+            if (active) {
+                for (int k = rows_t; k < wrows; ++k) {
+                    if (nrow < rcap) nl.lu4[(int64_t)nrow * nl.npad + p] = ~0ull;
+                    ++nrow;
                 }
             }
         }
         if (active) {
-            nl.count[i] = nbat * 4;
+            nl.count[i] = nrow;
             nl.nnb[i] = nreal;
-            atomicMax(&sc->max_count, nbat * 4);
+            atomicMax(&sc->max_count, nrow * 4);
         }
     }
+}
+
+__global__ __launch_bounds__(TPB) void k_positions(const int32_t* __restrict__ order, int64_t n, int32_t* pos)
+{
+    const int64_t p = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (p < n) pos[order[p]] = (int32_t)p;
+}
+
+void launch_positions(const int32_t* order, int64_t n, int32_t* pos, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_positions, dim3(grid_of(n)), dim3(TPB), 0, st, order, n, pos);
 }
 
 void launch_build_lists(int dim, const double* x, int64_t n, TileGrid tg, double sr, NbrList nl, DevScalars* sc,
@@ -281,12 +260,27 @@ void launch_build_lists(int dim, const double* x, int64_t n, TileGrid tg, double
 {
     (void)n;
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2) {
-        if (nl.f32) hipLaunchKernelGGL((k_build_t<2, float>), g, b, 0, st, x, tg, sr, nl, sc);
-        else hipLaunchKernelGGL((k_build_t<2, double>), g, b, 0, st, x, tg, sr, nl, sc);
-    } else {
-        if (nl.f32) hipLaunchKernelGGL((k_build_t<3, float>), g, b, 0, st, x, tg, sr, nl, sc);
-        else hipLaunchKernelGGL((k_build_t<3, double>), g, b, 0, st, x, tg, sr, nl, sc);
+    if (dim == 2) hipLaunchKernelGGL(k_build_t<2>, g, b, 0, st, x, tg, sr, nl, sc);
+    else hipLaunchKernelGGL(k_build_t<3>, g, b, 0, st, x, tg, sr, nl, sc);
+}
+
+/* f(j) for every neighbour slot j of slot i, one lane per slot (the passes
+ * over the thin boundary layer and the host dump walk lists this way). */
+template <int D, typename F>
+__device__ __forceinline__ void walk_lane(const TileGrid& tg, const NbrList& nl, int64_t i, F&& f)
+{
+    int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
+    int nseg, p0, p1;
+    union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
+    const int64_t p = tg.pos[i];
+    const int nrows = min(nl.count[i], nl.cap >> 2);
+    for (int r = 0; r < nrows; ++r) {
+        const uint64_t u = nl.lu4[(int64_t)r * nl.npad + p];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t uq = uint32_t(u >> (16 * q)) & 0xFFFFu;
+            if (uq != NO_ENTRY) f(union_slot(tg, seg_a, pre, nseg, (int)uq));
+        }
     }
 }
 
@@ -297,26 +291,35 @@ __global__ __launch_bounds__(TPB) void k_export(TileGrid tg, NbrList nl, int64_t
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n) return;
-    int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
-    int nseg, p0, p1;
-    union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
-    const int nbat = min(nl.count[i], nl.cap) >> 2;
     int k = 0;
-    for (int bb = 0; bb < nbat; ++bb) {
-        const uint64_t u = nl.lu4[(int64_t)bb * nl.npad + i];
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t uq = uint32_t(u >> (16 * q)) & 0xFFFFu;
-            if (uq == NO_ENTRY) continue;
-            out[(int64_t)k * nl.npad + i] = union_slot(tg, seg_a, pre, nseg, (int)uq);
-            ++k;
-        }
-    }
+    walk_lane<D>(tg, nl, i, [&](int32_t j) {
+        out[(int64_t)k * nl.npad + i] = j;
+        ++k;
+    });
 }
 
 void launch_export_lists(int dim, TileGrid tg, NbrList nl, int64_t n, int32_t* out, hipStream_t st)
 {
     if (dim == 2) hipLaunchKernelGGL(k_export<2>, dim3(grid_of(n)), dim3(TPB), 0, st, tg, nl, n, out);
     else hipLaunchKernelGGL(k_export<3>, dim3(grid_of(n)), dim3(TPB), 0, st, tg, nl, n, out);
+}
+
+/* Distance of a pair as the reference uses it: sqrt of the d^2 of the last
+ * neighbour build.  FR = false: the positions are still those of the build
+ * (xi, xj); FR = true: the build positions of both (xbi, and xb of slot j). */
+template <int D, bool FR>
+__device__ __forceinline__ void frozen_r(const double* xi, const double* xj, const double* xbi,
+                                         const double* __restrict__ xb, int32_t j, double& d2, double& r)
+{
+    if constexpr (FR) {
+        double xbj[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) xbj[a] = xb[(int64_t)j * D + a];
+        d2 = exact_dist2<D>(xbi, xbj);
+    } else {
+        d2 = exact_dist2<D>(xi, xj);
+    }
+    r = sqrt(d2);
 }
 
 /* ------------------------------------------------------------ dSPH_PreStep */
@@ -332,26 +335,19 @@ __device__ __forceinline__ void aero_velocity_one(const Consts& c, DevState& s, 
     s.cellID[i] = cond ? 1 : -3;
 }
 
-/* Dispatch on the list payload type (float on the mixed path). */
-#define FJ_PAYLOAD_DISPATCH(nl, KERNEL_FLOAT, KERNEL_DOUBLE) \
-    do {                                                     \
-        if ((nl).f32) { KERNEL_FLOAT; } else { KERNEL_DOUBLE; } \
-    } while (0)
-
 /* Pair sums of dSPH_PreStep (src/Shifting.cpp:12-123): the correction
  * matrix L over all neighbours and over non-boundary neighbours (both
  * symmetric, sum_j V_j W'(r)/r R_ji R_ji^T, accumulated on the upper
  * triangle), the density-gradient and normal sums, the kernel sum and the
  * colour function, and the per-block partial of npd.  k_dsph_fin turns
  * them into L^-1, the eigenvalues and the corrected gradients. */
-template <int D, typename RT>
-__global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl, int r_from_x,
-                                                double* partial)
+template <int D, bool FR>
+__global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl, double* partial)
 {
     constexpr int NS = D * (D + 1) / 2;
     __shared__ double tp[D][TILE];
     __shared__ double tvol[TILE], trho[TILE];
-    __shared__ int32_t tnb[TILE];
+    __shared__ int32_t tnb[TILE], tsl[FR ? TILE : 1];
     __shared__ BlockHead h;
     __shared__ double lds[NWAVE];
     block_head<D>(tg, h);
@@ -362,10 +358,11 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
         const int64_t i = active ? tg.order[p] : 0;
-        double xi[D], Lm[NS], Lnb[NS], gR[D], nr[D];
+        double xi[D], xbi[D], Lm[NS], Lnb[NS], gR[D], nr[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             xi[a] = active ? s.x[i * D + a] : 0.0;
+            xbi[a] = (FR && active) ? ax.xb[i * D + a] : 0.0;
             gR[a] = 0.0;
             nr[a] = 0.0;
         }
@@ -373,8 +370,8 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
         for (int k = 0; k < NS; ++k) { Lm[k] = 0.0; Lnb[k] = 0.0; }
         const double rhoi = active ? s.rho[i] : 0.0;
         double kernsum = c.Wc, colour = 0.0;
-        Cursor<RT> cu;
-        cu.init(nl, i, active);
+        Rows cu;
+        cu.init(nl, p, active ? nl.count[i] : 0);
         for (int base = 0; base < U; base += TILE) {
             const int nt = min(TILE, U - base);
             __syncthreads();
@@ -385,13 +382,14 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
                 tvol[t] = ax.vol[sl];
                 trho[t] = s.rho[sl];
                 tnb[t] = s.b[sl] > FJSPH_PISTON;
+                if constexpr (FR) tsl[t] = sl;
             }
             __syncthreads();
-            FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base)
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 double xj[D], Rji[D], d2, r;
 #pragma unroll
                 for (int a = 0; a < D; ++a) { xj[a] = tp[a][t]; Rji[a] = xj[a] - xi[a]; }
-                pair_r<D, RT>(r_from_x, xi, xj, pay, d2, r);
+                frozen_r<D, FR>(xi, xj, xbi, ax.xb, FR ? tsl[FR ? t : 0] : 0, d2, r);
                 const double volj = tvol[t];
                 double W, fg;
                 wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
@@ -419,7 +417,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
                     colour += volj * W;
                     npd_acc += W;
                 }
-            FJ_TILE_ENTRIES_END(cu, nl, i)
+            FJ_TILE_ENTRIES_END(cu, nl)
         }
         if (active) {
             int k = 0;
@@ -510,14 +508,13 @@ void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* par
                  int64_t n, int r_from_x, int with_aero_vel, double* partial, DevScalars* sc, hipStream_t st)
 {
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2)
-        FJ_PAYLOAD_DISPATCH(nl,
-            hipLaunchKernelGGL((k_dsph_t<2, float>), g, b, 0, st, c, s, a, tg, nl, r_from_x, partial),
-            hipLaunchKernelGGL((k_dsph_t<2, double>), g, b, 0, st, c, s, a, tg, nl, r_from_x, partial));
-    else
-        FJ_PAYLOAD_DISPATCH(nl,
-            hipLaunchKernelGGL((k_dsph_t<3, float>), g, b, 0, st, c, s, a, tg, nl, r_from_x, partial),
-            hipLaunchKernelGGL((k_dsph_t<3, double>), g, b, 0, st, c, s, a, tg, nl, r_from_x, partial));
+    if (dim == 2) {
+        if (r_from_x) hipLaunchKernelGGL((k_dsph_t<2, false>), g, b, 0, st, c, s, a, tg, nl, partial);
+        else hipLaunchKernelGGL((k_dsph_t<2, true>), g, b, 0, st, c, s, a, tg, nl, partial);
+    } else {
+        if (r_from_x) hipLaunchKernelGGL((k_dsph_t<3, false>), g, b, 0, st, c, s, a, tg, nl, partial);
+        else hipLaunchKernelGGL((k_dsph_t<3, true>), g, b, 0, st, c, s, a, tg, nl, partial);
+    }
     launch_sum_partials_scaled(partial, (int)tg.nblocks, 1, 1.0 / double(n), &sc->npd, st);
     if (dim == 2)
         hipLaunchKernelGGL(k_dsph_fin<2>, dim3(grid_of(n)), b, 0, st, c, s, a, part_id, nl.nnb, n, with_aero_vel);
@@ -544,19 +541,19 @@ void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* p
 
 /* -------------------------------------- Detect_Surface + dissipation_terms */
 
-/* mode 0: Detect_Surface loop 1 only; 1: with dissipation_terms;
+/* MODE 0: Detect_Surface loop 1 only; 1: with dissipation_terms;
  * 2: dissipation_terms only.  T is the arithmetic of the dissipation terms
- * and the list payload type. */
-template <int D, typename T, int MODE>
+ * (FP32 on the mixed path). */
+template <int D, typename T, int MODE, bool FR>
 __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
-                                                    TileGrid tg, NbrList nl, int r_from_x, double* norms_tmp)
+                                                    TileGrid tg, NbrList nl, double* norms_tmp)
 {
     constexpr bool with_surf = MODE != 2;
     constexpr bool with_diss = MODE != 0;
     constexpr int TS = with_surf ? TILE : 1, TD = with_diss ? TILE : 1;
     __shared__ double tp[D][TILE], tv[D][TD], tg_[D][TD];
     __shared__ double tvol[TILE], tlam[TS], trho[TD], tir[TD], tm[TD];
-    __shared__ int32_t tnb[TD];
+    __shared__ int32_t tnb[TD], tsl[FR ? TILE : 1];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -568,10 +565,11 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
         const bool active = p < h.p1;
         const int64_t i = active ? tg.order[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
-        double xi[D], ni[D], nhat[D], tau_hat[D], pT[D], vi[D], gri[D], nrm[D];
+        double xi[D], xbi[D], ni[D], nhat[D], tau_hat[D], pT[D], vi[D], gri[D], nrm[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             xi[a] = fluid ? s.x[i * D + a] : 0.0;
+            xbi[a] = (FR && fluid) ? ax.xb[i * D + a] : 0.0;
             ni[a] = fluid ? s.norm[i * D + a] : 0.0;
             vi[a] = fluid ? s.v[i * D + a] : 0.0;
             gri[a] = fluid ? s.gradRho[i * D + a] : 0.0;
@@ -599,8 +597,8 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
         for (int a = 0; a < D; ++a) av[a] = 0.0;
         const T Bg = T(c.B * c.gam), cfac = T(sqrt(c.B * c.gam / rhoi)), Ht = T(c.H);
         const T eps2 = T(0.0001 * c.H_sq), alpha = T(c.visc_alpha);
-        Cursor<T> cu;
-        cu.init(nl, i, fluid);
+        Rows cu;
+        cu.init(nl, p, fluid ? nl.count[i] : 0);
         for (int base = 0; base < U; base += TILE) {
             const int nt = min(TILE, U - base);
             __syncthreads();
@@ -609,6 +607,7 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
 #pragma unroll
                 for (int a = 0; a < D; ++a) tp[a][t] = s.x[(int64_t)sl * D + a];
                 tvol[t] = ax.vol[sl];
+                if constexpr (FR) tsl[t] = sl;
                 if constexpr (with_surf) tlam[t] = s.lam[sl];
                 if constexpr (with_diss) {
 #pragma unroll
@@ -623,11 +622,11 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
                 }
             }
             __syncthreads();
-            FJ_TILE_ENTRIES_BEGIN(T, cu, nl, i, base)
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 double xj[D], d2, r;
 #pragma unroll
                 for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
-                pair_r<D, T>(r_from_x, xi, xj, pay, d2, r);
+                frozen_r<D, FR>(xi, xj, xbi, ax.xb, FR ? tsl[FR ? t : 0] : 0, d2, r);
                 if (testing && !found) {
                     /* Marrone et al. (2010) point-T test, src/Geometry.cpp:43-98;
                      * the outcome does not depend on the order of the pairs */
@@ -677,7 +676,7 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
                         for (int a = 1; a < D; ++a) vdotr += Vji[a] * Rji[a];
                         if (!(vdotr > T(0))) {
                             const T muij = Ht * vdotr * idist2;
-                            const T cbar = T(0.5) * (cfac + sqrt(Bg * T(tir[t])));
+                            const T cbar = T(0.5) * (cfac + fast_sqrt<T>(Bg * T(tir[t])));
                             const T ff = T(tm[t]) * alpha * cbar * muij * fast_rcp<T>(T(0.5 * (rhoi + rhoj)));
 #pragma unroll
                             for (int a = 0; a < D; ++a) av[a] += double(g[a] * ff);
@@ -690,7 +689,7 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
                         rd += double(volj * drho * rdotg * idist2);
                     }
                 }
-            FJ_TILE_ENTRIES_END(cu, nl, i)
+            FJ_TILE_ENTRIES_END(cu, nl)
         }
         if (!active) continue;
         if (!fluid) {
@@ -732,43 +731,48 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
     }
 }
 
-template <int D, int MODE>
+template <int D, typename T, int MODE>
 static void launch_surface1_m(const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
                               int r_from_x, double* norms_tmp, hipStream_t st)
 {
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    FJ_PAYLOAD_DISPATCH(nl,
-        hipLaunchKernelGGL((k_surface1_t<D, float, MODE>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp),
-        hipLaunchKernelGGL((k_surface1_t<D, double, MODE>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp));
+    if (r_from_x)
+        hipLaunchKernelGGL((k_surface1_t<D, T, MODE, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+    else
+        hipLaunchKernelGGL((k_surface1_t<D, T, MODE, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
 }
 
-template <int D>
+template <int D, typename T>
 static void launch_surface1_d(const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
                               int r_from_x, double* norms_tmp, int mode, hipStream_t st)
 {
-    if (mode == 0) launch_surface1_m<D, 0>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
-    else if (mode == 1) launch_surface1_m<D, 1>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
-    else launch_surface1_m<D, 2>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
+    if (mode == 0) launch_surface1_m<D, T, 0>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
+    else if (mode == 1) launch_surface1_m<D, T, 1>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
+    else launch_surface1_m<D, T, 2>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
 }
 
 void launch_surface1(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg,
                      NbrList nl, int64_t n, int r_from_x, double* norms_tmp, int mode, hipStream_t st)
 {
     (void)n;
-    (void)prec;
-    if (dim == 2) launch_surface1_d<2>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
-    else launch_surface1_d<3>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
+    const bool f32 = prec == FJSPH_PREC_MIXED;
+    if (dim == 2) {
+        if (f32) launch_surface1_d<2, float>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
+        else launch_surface1_d<2, double>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
+    } else {
+        if (f32) launch_surface1_d<3, float>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
+        else launch_surface1_d<3, double>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
+    }
 }
 
 /* Curvature, Gissler occlusion and surfzone (src/Geometry.cpp:145-277). */
-template <int D, typename RT>
+template <int D, bool FR>
 __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
-                                                    TileGrid tg, NbrList nl, int r_from_x,
-                                                    const double* __restrict__ norms_tmp)
+                                                    TileGrid tg, NbrList nl, const double* __restrict__ norms_tmp)
 {
     __shared__ double tp[D][TILE], tn[D][TILE];
     __shared__ double tvol[TILE];
-    __shared__ int32_t tsurf[TILE];
+    __shared__ int32_t tsurf[TILE], tsl[FR ? TILE : 1];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -778,10 +782,11 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
         const bool active = p < h.p1;
         const int64_t i = active ? tg.order[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
-        double xi[D], ni[D], Vd[D], Li[D][D];
+        double xi[D], xbi[D], ni[D], Vd[D], Li[D][D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             xi[a] = fluid ? s.x[i * D + a] : 0.0;
+            xbi[a] = (FR && fluid) ? ax.xb[i * D + a] : 0.0;
             ni[a] = fluid ? norms_tmp[i * D + a] : 0.0;
             Vd[a] = 0.0;
 #pragma unroll
@@ -798,8 +803,8 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
         const bool ni_nz = dotd<D>(ni, ni) > 0.0;
         double curve = 0.0, qmax = 0.0;
         int zone = (fluid && s.surf[i] == 1) ? 1 : 0;
-        Cursor<RT> cu;
-        cu.init(nl, i, fluid);
+        Rows cu;
+        cu.init(nl, p, fluid ? nl.count[i] : 0);
         for (int base = 0; base < U; base += TILE) {
             const int nt = min(TILE, U - base);
             __syncthreads();
@@ -812,13 +817,14 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
                 }
                 tvol[t] = ax.vol[sl];
                 tsurf[t] = s.surf[sl];
+                if constexpr (FR) tsl[t] = sl;
             }
             __syncthreads();
-            FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base)
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 double xj[D], d2, r, Rij[D], nj[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) { xj[a] = tp[a][t]; Rij[a] = xj[a] - xi[a]; nj[a] = tn[a][t]; }
-                pair_r<D, RT>(r_from_x, xi, xj, pay, d2, r);
+                frozen_r<D, FR>(xi, xj, xbi, ax.xb, FR ? tsl[FR ? t : 0] : 0, d2, r);
                 if (ni_nz && dotd<D>(nj, nj) > 0.0) {
                     double W, fg;
                     wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
@@ -839,7 +845,7 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
                     if (q > qmax) qmax = q;
                 }
                 if (tsurf[t] == 1) zone = 1;
-            FJ_TILE_ENTRIES_END(cu, nl, i)
+            FJ_TILE_ENTRIES_END(cu, nl)
         }
         if (!fluid) continue;
         const double wo = (occl && Vdn > 0.0) ? qmax / Vdn : 0.0;
@@ -863,14 +869,13 @@ void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t*
 {
     (void)n;
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2)
-        FJ_PAYLOAD_DISPATCH(nl,
-            hipLaunchKernelGGL((k_surface2_t<2, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp),
-            hipLaunchKernelGGL((k_surface2_t<2, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp));
-    else
-        FJ_PAYLOAD_DISPATCH(nl,
-            hipLaunchKernelGGL((k_surface2_t<3, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp),
-            hipLaunchKernelGGL((k_surface2_t<3, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x, norms_tmp));
+    if (dim == 2) {
+        if (r_from_x) hipLaunchKernelGGL((k_surface2_t<2, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+        else hipLaunchKernelGGL((k_surface2_t<2, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+    } else {
+        if (r_from_x) hipLaunchKernelGGL((k_surface2_t<3, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+        else hipLaunchKernelGGL((k_surface2_t<3, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+    }
 }
 
 /* ------------------------------------------------------------ particle_shift */
@@ -879,13 +884,13 @@ void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t*
  * n_i.n_j) over the dot products acos accepts ([-1, 1]): NaN angles never
  * win the reference's max and acos is decreasing, so the result is the same.
  * max_j |v_j - v_i| is taken on squared norms. */
-template <int D, typename RT>
+template <int D, bool FR>
 __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
-                                                 TileGrid tg, NbrList nl, int r_from_x)
+                                                 TileGrid tg, NbrList nl)
 {
     __shared__ double tp[D][TILE], tv[D][TILE], tn[D][TILE];
     __shared__ double tvol[TILE];
-    __shared__ int32_t tnb[TILE];
+    __shared__ int32_t tnb[TILE], tsl[FR ? TILE : 1];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -898,17 +903,18 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
         const bool fluid = active && part_id[i] >= c.n_bound;
         const double lam_nb = fluid ? s.lam_nb[i] : 0.0;
         const bool work = fluid && !(lam_nb < 0.55 || s.b[i] == FJSPH_BUFFER);
-        double xi[D], vi[D], nhat[D], du[D];
+        double xi[D], xbi[D], vi[D], nhat[D], du[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             xi[a] = work ? s.x[i * D + a] : 0.0;
+            xbi[a] = (FR && work) ? ax.xb[i * D + a] : 0.0;
             vi[a] = work ? s.v[i * D + a] : 0.0;
             nhat[a] = work ? ax.nhat[i * D + a] : 0.0;
             du[a] = 0.0;
         }
         double maxU2 = 0.0, dmin = 2.0;
-        Cursor<RT> cu;
-        cu.init(nl, i, work);
+        Rows cu;
+        cu.init(nl, p, work ? nl.count[i] : 0);
         for (int base = 0; base < U; base += TILE) {
             const int nt = min(TILE, U - base);
             __syncthreads();
@@ -922,13 +928,14 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
                 }
                 tvol[t] = ax.vol[sl];
                 tnb[t] = s.b[sl] > FJSPH_PISTON;
+                if constexpr (FR) tsl[t] = sl;
             }
             __syncthreads();
-            FJ_TILE_ENTRIES_BEGIN(RT, cu, nl, i, base)
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 double xj[D], d2, r;
 #pragma unroll
                 for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
-                pair_r<D, RT>(r_from_x, xi, xj, pay, d2, r);
+                frozen_r<D, FR>(xi, xj, xbi, ax.xb, FR ? tsl[FR ? t : 0] : 0, d2, r);
                 double W, fg;
                 wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
                 const double w = W * inv_Wdx;
@@ -948,7 +955,7 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
                     if (d >= -1.0 && d <= 1.0 && d < dmin) dmin = d;
                 }
                 if (dv2 > maxU2) maxU2 = dv2;
-            FJ_TILE_ENTRIES_END(cu, nl, i)
+            FJ_TILE_ENTRIES_END(cu, nl)
         }
         if (!fluid) continue;
         double out[D];
@@ -991,55 +998,36 @@ void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
 {
     (void)n;
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2)
-        FJ_PAYLOAD_DISPATCH(nl,
-            hipLaunchKernelGGL((k_shift_t<2, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x),
-            hipLaunchKernelGGL((k_shift_t<2, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x));
-    else
-        FJ_PAYLOAD_DISPATCH(nl,
-            hipLaunchKernelGGL((k_shift_t<3, float>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x),
-            hipLaunchKernelGGL((k_shift_t<3, double>), g, b, 0, st, c, s, a, part_id, tg, nl, r_from_x));
+    if (dim == 2) {
+        if (r_from_x) hipLaunchKernelGGL((k_shift_t<2, false>), g, b, 0, st, c, s, a, part_id, tg, nl);
+        else hipLaunchKernelGGL((k_shift_t<2, true>), g, b, 0, st, c, s, a, part_id, tg, nl);
+    } else {
+        if (r_from_x) hipLaunchKernelGGL((k_shift_t<3, false>), g, b, 0, st, c, s, a, part_id, tg, nl);
+        else hipLaunchKernelGGL((k_shift_t<3, true>), g, b, 0, st, c, s, a, part_id, tg, nl);
+    }
 }
 
 /* ------------------------------------------------------- boundary solvers */
 
-/* The boundary passes touch only boundary particles (a thin layer), so they
- * walk their lists per lane through the union table instead of staging. */
-template <int D>
-__device__ __forceinline__ void lane_union(const TileGrid& tg, int64_t i, int32_t* seg_a, int32_t* pre, int& nseg)
+/* f(j, r) for every neighbour j of slot i with the distance frozen at the
+ * last build. */
+template <int D, typename F>
+__device__ __forceinline__ void walk_lane_r(const TileGrid& tg, const NbrList& nl, const double* __restrict__ xb,
+                                            int64_t i, F&& f)
 {
-    int p0, p1;
-    union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
-}
-
-/* f(j, r) for every neighbour j of slot i with its frozen distance r. */
-template <int D, typename RT, typename F>
-__device__ __forceinline__ void walk_lane(const TileGrid& tg, const NbrList& nl, int64_t i, F&& f)
-{
-    int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
-    int nseg;
-    lane_union<D>(tg, i, seg_a, pre, nseg);
-    const int nbat = nl.count[i] >> 2;
-    for (int bb = 0; bb < nbat; ++bb) {
-        uint64_t u;
-        RT r[4];
-        load_batch<RT>(nl, bb, i, u, r);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t uq = uint32_t(u >> (16 * q)) & 0xFFFFu;
-            if (uq == NO_ENTRY) continue;
-            const int32_t j = union_slot(tg, seg_a, pre, nseg, (int)uq);
-            double d2, rr;
-            pair_r<D, RT>(0, nullptr, nullptr, r[q], d2, rr);
-            f(j, rr);
-        }
-    }
+    double xbi[D];
+    load_vec<D>(xb, i, xbi);
+    walk_lane<D>(tg, nl, i, [&](int32_t j) {
+        double xbj[D];
+        load_vec<D>(xb, j, xbj);
+        f(j, sqrt(exact_dist2<D>(xbi, xbj)));
+    });
 }
 
 /* Block velocity + no-slip (src/Newmark_Beta.cpp:69-102, src/Resid.cpp:162-186). */
-template <int D, typename RT>
-__global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int32_t* __restrict__ blk, DevBlocks bl,
-                                                 int32_t block, TileGrid tg, NbrList nl, int64_t n)
+template <int D>
+__global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, Aux ax, const int32_t* __restrict__ blk,
+                                                 DevBlocks bl, int32_t block, TileGrid tg, NbrList nl, int64_t n)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n || blk[i] != block) return;
@@ -1052,7 +1040,7 @@ __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int
 #pragma unroll
         for (int a = 0; a < D; ++a) vs[a] = 0.0;
         double ks = 0.0;
-        walk_lane<D, RT>(tg, nl, i, [&](int32_t j, double r) {
+        walk_lane_r<D>(tg, nl, ax.xb, i, [&](int32_t j, double r) {
             if (s.b[j] <= FJSPH_PISTON) return;
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
@@ -1070,7 +1058,7 @@ __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, const int
 }
 
 /* Boundary solver of one block (src/Resid.cpp:21-160). */
-template <int D, typename RT>
+template <int D>
 __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax, const int32_t* __restrict__ blk,
                                                    DevBlocks bl, int32_t block, TileGrid tg, NbrList nl, int64_t n,
                                                    int32_t* near_inlet)
@@ -1089,7 +1077,7 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
         double ks = 0.0, pk = 0.0, ak[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) ak[a] = 0.0;
-        walk_lane<D, RT>(tg, nl, i, [&](int32_t j, double r) {
+        walk_lane_r<D>(tg, nl, ax.xb, i, [&](int32_t j, double r) {
             if (s.b[j] <= FJSPH_PISTON) return;
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
@@ -1115,7 +1103,7 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
         int ni = 1;
         double rr = 0.0, vi[D];
         load_vec<D>(s.v, i, vi);
-        walk_lane<D, RT>(tg, nl, i, [&](int32_t j, double r) {
+        walk_lane_r<D>(tg, nl, ax.xb, i, [&](int32_t j, double r) {
             const int32_t bj = s.b[j];
             if (bj == FJSPH_PIPE || bj == FJSPH_FREE) ni = 0;
             double W, fg;
@@ -1135,18 +1123,12 @@ void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t*
                      TileGrid tg, NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st)
 {
     const dim3 g(grid_of(n)), b(TPB);
-    if (dim == 2 && nl.f32) {
-        hipLaunchKernelGGL((k_bnd_vel<2, float>), g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
-        hipLaunchKernelGGL((k_bnd_solve<2, float>), g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
-    } else if (dim == 2) {
-        hipLaunchKernelGGL((k_bnd_vel<2, double>), g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
-        hipLaunchKernelGGL((k_bnd_solve<2, double>), g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
-    } else if (nl.f32) {
-        hipLaunchKernelGGL((k_bnd_vel<3, float>), g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
-        hipLaunchKernelGGL((k_bnd_solve<3, float>), g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+    if (dim == 2) {
+        hipLaunchKernelGGL(k_bnd_vel<2>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n);
+        hipLaunchKernelGGL(k_bnd_solve<2>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
     } else {
-        hipLaunchKernelGGL((k_bnd_vel<3, double>), g, b, 0, st, c, s, blk, bl, block, tg, nl, n);
-        hipLaunchKernelGGL((k_bnd_solve<3, double>), g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+        hipLaunchKernelGGL(k_bnd_vel<3>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n);
+        hipLaunchKernelGGL(k_bnd_solve<3>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
     }
 }
 
@@ -1194,18 +1176,23 @@ struct NextState {
 #define FJ_FORCE_OCC __attribute__((amdgpu_waves_per_eu(5, 8)))
 #endif
 
-/* LDS tile of the force pass, one record per union slot in four arrays:
- * A {x - xref, m (negative for BOUND)}, B {v - vref, p/rho^2},
- * C {1/rho, m/rho}, E (ALE) {vPert, rho}.  FP32: float4/float2, so a pair
- * costs four LDS reads.  FP64: the same values as doubles, xref = vref = 0. */
-template <typename T>
+/* LDS tile of the force pass, one record per union slot.
+ * FP32 (mixed path), four float4 arrays, four LDS reads per pair:
+ *   A {x - xref, m (negative for BOUND)}, B {v - vref, p/rho^2},
+ *   X {xb - xbref (build positions, FR only), 1/rho}, E {vPert, rho};
+ *   V_j = |m_j| / rho_j is formed as |m_j| * (1/rho_j).
+ * FP64 (parity path), one array per value: x, m, v, p/rho^2, xb (FR), vPert,
+ * rho; 1/rho and m/rho are recomputed per pair with the same operations as
+ * launch_aux, so they are bit-identical to the stored ones. */
+template <typename T, bool FR>
 struct ForceTile {
-    T A[TILE][4], B[TILE][4], E[TILE][4], C[TILE][2];
+    double P[8][TILE];
+    double XB[3][FR ? TILE : 1];
+    double EA[4][TILE];
 };
-template <>
-struct ForceTile<float> {
-    float4 A[TILE], B[TILE], E[TILE];
-    float2 C[TILE];
+template <bool FR>
+struct ForceTile<float, FR> {
+    float4 A[TILE], B[TILE], X[TILE], E[TILE];
 };
 
 /* Pair sums of get_acc_and_Rrho_on_i (src/Resid.cpp:243-418) for fluid
@@ -1213,25 +1200,28 @@ struct ForceTile<float> {
  * artificial-viscosity and gravity terms are added by k_nb_update) and the
  * full Rrho.  Pair terms are formed in T; every sum is accumulated in FP64.
  * The pressure, viscous and surface-tension terms of a pair are combined
- * before accumulation: acc_i = sum_j (visc + ST / m_i - m_j (P_i + P_j) g). */
-template <int D, typename T>
-__global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
-                                                 TileGrid tg, NbrList nl, const DevScalars* sc)
+ * before accumulation: acc_i = sum_j (visc + ST / m_i - m_j (P_i + P_j) g).
+ * FR: the positions have moved since the build, r comes from Aux::xb. */
+template <int D, typename T, bool FR>
+__global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState s, Aux ax,
+                                                              const int32_t* __restrict__ part_id, TileGrid tg,
+                                                              NbrList nl, const DevScalars* sc)
 {
-    __shared__ ForceTile<T> ft;
+    __shared__ ForceTile<T, FR> ft;
     __shared__ BlockHead h;
     constexpr bool F32 = sizeof(T) == 4;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
     /* reference point of the block (its first target) for the FP32
      * relative coordinates */
-    double xref[D], vref[D];
+    double xref[D], vref[D], xbref[D];
     {
         const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             xref[a] = F32 ? s.x[i0 * D + a] : 0.0;
             vref[a] = F32 ? s.v[i0 * D + a] : 0.0;
+            xbref[a] = (F32 && FR) ? ax.xb[i0 * D + a] : 0.0;
         }
     }
     const double npd = sc->npd;
@@ -1246,14 +1236,16 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
         const bool active = p < h.p1;
         const int64_t i = active ? tg.order[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
-        double xi[D], vi[D];
-        T xit[D], vit[D], vpi[D];
+        double xi[D], vi[D], xbi[D];
+        T xit[D], vit[D], xbit[D], vpi[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             xi[a] = active ? s.x[i * D + a] : 0.0;
             vi[a] = active ? s.v[i * D + a] : 0.0;
+            xbi[a] = (FR && active) ? ax.xb[i * D + a] : 0.0;
             xit[a] = T(xi[a] - xref[a]);
             vit[a] = T(vi[a] - vref[a]);
+            xbit[a] = T(xbi[a] - xbref[a]);
             vpi[a] = (fluid && c.ale) ? T(s.vPert[i * D + a]) : T(0);
         }
         const double rhoi = fluid ? s.rho[i] : 1.0, mi = fluid ? s.m[i] : 1.0;
@@ -1268,74 +1260,102 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
 #pragma unroll
         for (int a = 0; a < D; ++a) acc[a] = 0.0;
         double Rrho_ = 0.0, Rrhoc_ = 0.0;
-        Cursor<T> cu;
-        cu.init(nl, i, fluid);
+        Rows cu;
+        cu.init(nl, p, fluid ? nl.count[i] : 0);
         for (int base = 0; base < U; base += TILE) {
             const int nt = min(TILE, U - base);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
-                T pa[3] = {T(0), T(0), T(0)}, va[3] = {T(0), T(0), T(0)};
-#pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    pa[a] = T(s.x[(int64_t)sl * D + a] - xref[a]);
-                    va[a] = T(s.v[(int64_t)sl * D + a] - vref[a]);
-                }
-                T m, P, ir, vol;
-                if (F32) {
-                    const float4 q = ax.fq[sl];
-                    m = T(q.x); P = T(q.y); ir = T(q.z); vol = T(q.w);
-                } else {
-                    m = T(s.b[sl] == FJSPH_BOUND ? -s.m[sl] : s.m[sl]);
-                    P = T(ax.P[sl]); ir = T(ax.invrho[sl]); vol = T(ax.vol[sl]);
-                }
                 if constexpr (F32) {
-                    ft.A[t] = make_float4(pa[0], pa[1], pa[2], m);
-                    ft.B[t] = make_float4(va[0], va[1], va[2], P);
-                    ft.C[t] = make_float2(ir, vol);
-                } else {
-                    ft.A[t][0] = pa[0]; ft.A[t][1] = pa[1]; ft.A[t][2] = pa[2]; ft.A[t][3] = m;
-                    ft.B[t][0] = va[0]; ft.B[t][1] = va[1]; ft.B[t][2] = va[2]; ft.B[t][3] = P;
-                    ft.C[t][0] = ir; ft.C[t][1] = vol;
-                }
-                if (c.ale) {
-                    T vp[3] = {T(0), T(0), T(0)};
+                    float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-                    for (int a = 0; a < D; ++a) vp[a] = T(s.vPert[(int64_t)sl * D + a]);
-                    if constexpr (F32) {
-                        ft.E[t] = make_float4(vp[0], vp[1], vp[2], float(s.rho[sl]));
-                    } else {
-                        ft.E[t][0] = vp[0]; ft.E[t][1] = vp[1]; ft.E[t][2] = vp[2]; ft.E[t][3] = T(s.rho[sl]);
+                    for (int a = 0; a < D; ++a) {
+                        pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
+                        va[a] = float(s.v[(int64_t)sl * D + a] - vref[a]);
+                        if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
                     }
+                    const float4 q = ax.fq[sl];
+                    ft.A[t] = make_float4(pa[0], pa[1], pa[2], q.x);
+                    ft.B[t] = make_float4(va[0], va[1], va[2], q.y);
+                    ft.X[t] = make_float4(xa[0], xa[1], xa[2], q.z);
+                    if (c.ale) ft.E[t] = ax.fale[sl];
+                } else {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        ft.P[a][t] = s.x[(int64_t)sl * D + a];
+                        ft.P[4 + a][t] = s.v[(int64_t)sl * D + a];
+                        if constexpr (FR) ft.XB[FR ? a : 0][FR ? t : 0] = ax.xb[(int64_t)sl * D + a];
+                        ft.EA[a][t] = c.ale ? s.vPert[(int64_t)sl * D + a] : 0.0;
+                    }
+                    ft.P[3][t] = s.b[sl] == FJSPH_BOUND ? -s.m[sl] : s.m[sl];
+                    ft.P[7][t] = ax.P[sl];
+                    ft.EA[3][t] = s.rho[sl];
                 }
             }
             __syncthreads();
-            FJ_TILE_ENTRIES_BEGIN(T, cu, nl, i, base)
-                T r, rr;
-                if (F32) {
-                    r = pay;
-                    rr = r * r;
-                } else {
-                    rr = pay;
-                    r = T(sqrt(double(pay)));
-                }
-                T pj[3], vj[3], msg, Pj, irj, volj;
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
+                T Rji[D], Vji[D], vjr[3], msg, Pj, irj, volj, r, rr, rhoj, vpj[3];
                 if constexpr (F32) {
-                    const float4 A = ft.A[t], B = ft.B[t];
-                    const float2 Cc = ft.C[t];
-                    pj[0] = A.x; pj[1] = A.y; pj[2] = A.z; msg = A.w;
-                    vj[0] = B.x; vj[1] = B.y; vj[2] = B.z; Pj = B.w;
-                    irj = Cc.x; volj = Cc.y;
-                } else {
-                    pj[0] = ft.A[t][0]; pj[1] = ft.A[t][1]; pj[2] = ft.A[t][2]; msg = ft.A[t][3];
-                    vj[0] = ft.B[t][0]; vj[1] = ft.B[t][1]; vj[2] = ft.B[t][2]; Pj = ft.B[t][3];
-                    irj = ft.C[t][0]; volj = ft.C[t][1];
-                }
-                T Rji[D], Vji[D];
+                    const float4 A = ft.A[t], B = ft.B[t], X = ft.X[t];
+                    const float pj[3] = {A.x, A.y, A.z}, vj[3] = {B.x, B.y, B.z}, xbj[3] = {X.x, X.y, X.z};
+                    msg = A.w;
+                    Pj = B.w;
+                    irj = X.w;
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    Rji[a] = F32 ? (pj[a] - xit[a]) : T(double(pj[a]) - xi[a]);
-                    Vji[a] = F32 ? (vj[a] - vit[a]) : T(double(vj[a]) - vi[a]);
+                    for (int a = 0; a < D; ++a) {
+                        Rji[a] = pj[a] - xit[a];
+                        Vji[a] = vj[a] - vit[a];
+                        vjr[a] = vj[a];
+                    }
+                    if constexpr (FR) {
+                        T Rb[D];
+#pragma unroll
+                        for (int a = 0; a < D; ++a) Rb[a] = xbj[a] - xbit[a];
+                        rr = Rb[0] * Rb[0];
+#pragma unroll
+                        for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
+                    } else {
+                        rr = Rji[0] * Rji[0];
+#pragma unroll
+                        for (int a = 1; a < D; ++a) rr += Rji[a] * Rji[a];
+                    }
+                    r = fast_sqrt<T>(rr);
+                    volj = fabsf(msg) * irj;
+                    if (c.ale) {
+                        const float4 E = ft.E[t];
+                        vpj[0] = E.x; vpj[1] = E.y; vpj[2] = E.z;
+                        rhoj = E.w;
+                    } else {
+                        vpj[0] = vpj[1] = vpj[2] = T(0);
+                        rhoj = T(0);
+                    }
+                } else {
+                    double xj[D], d2;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        xj[a] = ft.P[a][t];
+                        Rji[a] = T(xj[a] - xi[a]);
+                        Vji[a] = T(ft.P[4 + a][t] - vi[a]);
+                        vjr[a] = T(ft.P[4 + a][t]);
+                        vpj[a] = T(ft.EA[a][t]);
+                    }
+                    if constexpr (FR) {
+                        double xbj[D];
+#pragma unroll
+                        for (int a = 0; a < D; ++a) xbj[a] = ft.XB[FR ? a : 0][FR ? t : 0];
+                        d2 = exact_dist2<D>(xbi, xbj);
+                    } else {
+                        d2 = exact_dist2<D>(xi, xj);
+                    }
+                    rr = T(d2);
+                    r = T(sqrt(d2));
+                    msg = T(ft.P[3][t]);
+                    Pj = T(ft.P[7][t]);
+                    const double rj = ft.EA[3][t];
+                    rhoj = T(rj);
+                    irj = T(1.0 / rj);
+                    volj = T(fabs(double(msg)) / rj);
                 }
                 const bool bound_j = msg < T(0);
                 const T mj = bound_j ? -msg : msg;
@@ -1353,33 +1373,24 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
                 T sf = T(0);
                 if (st_on) {
                     const T fac = (bound_i || bound_j) ? contang_fac : T(1);
-                    sf = ste * fac * cos(pi3o4H * r) * fast_rcp<T>(r) * inv_mi;
+                    sf = ste * fac * fast_cos<T>(pi3o4H * r) * fast_rcp<T>(r) * inv_mi;
                 }
 #pragma unroll
                 for (int a = 0; a < D; ++a) acc[a] += double(vf * Vji[a] + sf * Rji[a] - pc * g[a]);
                 if (c.ale) {
                     T aj = T(0), ai = T(0), cd = T(0), cont = T(0), c2 = T(0);
-                    T ej[4];
-                    if constexpr (F32) {
-                        const float4 E = ft.E[t];
-                        ej[0] = E.x; ej[1] = E.y; ej[2] = E.z; ej[3] = E.w;
-                    } else {
-                        ej[0] = ft.E[t][0]; ej[1] = ft.E[t][1]; ej[2] = ft.E[t][2]; ej[3] = ft.E[t][3];
-                    }
-                    const T rhoj = ej[3];
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
-                        const T vpj = ej[a];
-                        aj += vpj * g[a];
+                        aj += vpj[a] * g[a];
                         ai += vpi[a] * g[a];
-                        cd += (vpj - vpi[a]) * g[a];
-                        cont += (Vji[a] + (vpj - vpi[a])) * g[a];
-                        c2 += (rhoj * vpj + rhoit * vpi[a]) * g[a];
+                        cd += (vpj[a] - vpi[a]) * g[a];
+                        cont += (Vji[a] + (vpj[a] - vpi[a])) * g[a];
+                        c2 += (rhoj * vpj[a] + rhoit * vpi[a]) * g[a];
                     }
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
                         const T via = T(vi[a]);
-                        const T vja = F32 ? (T(vref[a]) + vj[a]) : vj[a];
+                        const T vja = F32 ? (T(vref[a]) + vjr[a]) : vjr[a];
                         acc[a] += double((vja * aj + via * ai - via * cd) * volj);
                     }
                     Rrho_ -= double(cont * volj);
@@ -1390,7 +1401,7 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
                     for (int a = 1; a < D; ++a) vg += Vji[a] * g[a];
                     Rrho_ -= double(volj * vg);
                 }
-            FJ_TILE_ENTRIES_END(cu, nl, i)
+            FJ_TILE_ENTRIES_END(cu, nl)
         }
         if (!fluid) continue;
 #pragma unroll
@@ -1505,17 +1516,19 @@ __global__ __launch_bounds__(TPB) void k_nb_update(Consts c, DevState s, DevStat
 }
 
 void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
-                  const DevScalars* sc, hipStream_t st)
+                  int r_from_x, const DevScalars* sc, hipStream_t st)
 {
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2)
-        FJ_PAYLOAD_DISPATCH(nl,
-            hipLaunchKernelGGL((k_force_t<2, float>), g, b, 0, st, c, s, a, part_id, tg, nl, sc),
-            hipLaunchKernelGGL((k_force_t<2, double>), g, b, 0, st, c, s, a, part_id, tg, nl, sc));
-    else
-        FJ_PAYLOAD_DISPATCH(nl,
-            hipLaunchKernelGGL((k_force_t<3, float>), g, b, 0, st, c, s, a, part_id, tg, nl, sc),
-            hipLaunchKernelGGL((k_force_t<3, double>), g, b, 0, st, c, s, a, part_id, tg, nl, sc));
+#define FJ_FORCE_LAUNCH(DD, TT, FF) \
+    hipLaunchKernelGGL((k_force_t<DD, TT, FF>), g, b, 0, st, c, s, a, part_id, tg, nl, sc)
+    if (dim == 2) {
+        if (nl.f32) { if (r_from_x) FJ_FORCE_LAUNCH(2, float, false); else FJ_FORCE_LAUNCH(2, float, true); }
+        else { if (r_from_x) FJ_FORCE_LAUNCH(2, double, false); else FJ_FORCE_LAUNCH(2, double, true); }
+    } else {
+        if (nl.f32) { if (r_from_x) FJ_FORCE_LAUNCH(3, float, false); else FJ_FORCE_LAUNCH(3, float, true); }
+        else { if (r_from_x) FJ_FORCE_LAUNCH(3, double, false); else FJ_FORCE_LAUNCH(3, double, true); }
+    }
+#undef FJ_FORCE_LAUNCH
 }
 
 void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double* next_x, double* next_v,
