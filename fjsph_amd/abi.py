@@ -19,7 +19,7 @@ NEWMARK_BETA, RUNGE_KUTTA = 0, 1
 DBC, PRESSURE_G, GHOST = 0, 1, 2
 COLE_EOS, ISO_EOS = 0, 1
 NO_AERO, GISSLER, INDUCED_PRESSURE, SKIN_FRIC = 0, 1, 2, 3
-CONST_VEL, MESH_INFL = 0, 1
+CONST_VEL, MESH_INFL, VLM_INFL, SURROGATE = 0, 1, 2, 3
 STATE_N, STATE_NP1 = 0, 1
 
 FIELDS = [

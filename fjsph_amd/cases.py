@@ -29,6 +29,7 @@ class Case:
     n_bound: int
     blocks: list = field(default_factory=list)
     notes: str = ""
+    mesh: object = None
 
     @property
     def n(self):
@@ -276,4 +277,117 @@ def batched_droplets(n_drops=256, dx=5e-5, seed=99, precision=abi.PREC_MIXED, d_
     case = _finish("batched_droplets", s, np.zeros((0, 3)), xf, vf,
                    f"{n_drops} droplets, diameters log-U{d_range}, {xf.shape[0]} particles")
     case.owner = np.concatenate(owner)
+    return case
+
+
+@dataclass
+class Mesh:
+    """Aerodynamic mesh in the flat layout of fjsph_set_mesh (MESH of
+    src/Var.h:396-451): CSR faces with left/right cells (-2 marks the outer
+    boundary), CSR cells, and the gas solution at the vertices."""
+    verts: np.ndarray
+    face_off: np.ndarray
+    face_verts: np.ndarray
+    face_lr: np.ndarray
+    cell_off: np.ndarray
+    cell_faces: np.ndarray
+    vert_vel: np.ndarray
+    vert_p: np.ndarray
+    vert_rho: np.ndarray
+    simplices: np.ndarray = None
+
+    @property
+    def nverts(self):
+        return self.verts.shape[0]
+
+    @property
+    def nfaces(self):
+        return self.face_off.size - 1
+
+    @property
+    def ncells(self):
+        return self.cell_off.size - 1
+
+
+def cylinder_flow(x, speed, radius, centre, rho_g, p_inf):
+    """Potential flow past a circular cylinder (axis z) of the given radius
+    and centre in a uniform stream `speed` along +x: velocity, Bernoulli
+    pressure, constant density.  Points inside the cylinder get the surface
+    value of their direction."""
+    dim = x.shape[1]
+    dx = x[:, 0] - centre[0]
+    dy = x[:, 1] - centre[1]
+    r2 = np.maximum(dx * dx + dy * dy, radius * radius)
+    a2 = radius * radius
+    u = speed * (1.0 - a2 * (dx * dx - dy * dy) / (r2 * r2))
+    v = -speed * 2.0 * a2 * dx * dy / (r2 * r2)
+    vel = np.zeros((x.shape[0], dim))
+    vel[:, 0] = u
+    vel[:, 1] = v
+    p = p_inf + 0.5 * rho_g * (speed * speed - (u * u + v * v))
+    return vel, p, np.full(x.shape[0], rho_g)
+
+
+def delaunay_mesh(lo, hi, n_points, seed=7, speed=20.0, rho_g=1.2, p_inf=101325.0, cyl_radius=None,
+                  cyl_centre=None):
+    """Seeded Delaunay mesh (scipy) of random points plus the box corners in
+    [lo, hi], tetrahedra (3D) or triangles (2D), with the potential flow of
+    `cylinder_flow` at the vertices.  Faces are the simplices' facets, each
+    stored once; the first cell seen is left, a facet on the hull gets -2."""
+    from itertools import combinations
+
+    from scipy.spatial import Delaunay
+
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    dim = lo.size
+    rng = np.random.default_rng(seed)
+    corners = np.array(np.meshgrid(*[[lo[k], hi[k]] for k in range(dim)], indexing="ij")).reshape(dim, -1).T
+    pts = np.concatenate([corners, rng.uniform(lo, hi, size=(n_points, dim))], axis=0)
+    tri = Delaunay(pts)
+    simp = tri.simplices.astype(np.int64)
+    nc = simp.shape[0]
+    combos = list(combinations(range(dim + 1), dim))
+    facets = np.stack([simp[:, list(cmb)] for cmb in combos], axis=1).reshape(-1, dim)
+    key = np.sort(facets, axis=1)
+    uniq, inv = np.unique(key, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    nf = uniq.shape[0]
+    owner = np.repeat(np.arange(nc), dim + 1)
+    lr = np.full((nf, 2), -2, dtype=np.int64)
+    first = np.full(nf, -1, dtype=np.int64)
+    order = np.argsort(inv, kind="stable")
+    inv_sorted = inv[order]
+    starts = np.searchsorted(inv_sorted, np.arange(nf))
+    lr[:, 0] = owner[order[starts]]
+    second = starts + 1
+    has2 = (second < inv_sorted.size) & (inv_sorted[np.minimum(second, inv_sorted.size - 1)] == np.arange(nf))
+    lr[has2, 1] = owner[order[second[has2]]]
+    del first
+    face_off = np.arange(nf + 1, dtype=np.int64) * dim
+    face_verts = uniq.reshape(-1).astype(np.int64)
+    cell_off = np.arange(nc + 1, dtype=np.int64) * (dim + 1)
+    cell_faces = inv.astype(np.int64)
+    if cyl_radius is None:
+        cyl_radius = 0.25 * float(hi[1] - lo[1])
+    if cyl_centre is None:
+        cyl_centre = (float(lo[0]) - 1.5 * cyl_radius, 0.5 * float(lo[1] + hi[1]))
+    vel, p, rho = cylinder_flow(pts, speed, cyl_radius, cyl_centre, rho_g, p_inf)
+    return Mesh(np.ascontiguousarray(pts), face_off, face_verts, np.ascontiguousarray(lr.reshape(-1)), cell_off,
+                cell_faces, np.ascontiguousarray(vel), p, rho, simplices=simp)
+
+
+def with_mesh(case, n_points=20000, margin=4.0, seed=7, speed=None):
+    """Switch a case to the mesh aero source with a Delaunay mesh around its
+    particles (`margin` particle spacings beyond their bounding box)."""
+    s = case.settings
+    dx = s.particle_step
+    lo = case.x.min(axis=0) - margin * dx
+    hi = case.x.max(axis=0) + margin * dx
+    sp = float(speed if speed is not None else s.v_inf[0])
+    rho_g = float(s.rho_g)
+    p_inf = float(s.p_ref)
+    case.mesh = delaunay_mesh(lo, hi, n_points, seed=seed, speed=sp, rho_g=rho_g, p_inf=p_inf)
+    s.aero_source = abi.MESH_INFL
+    case.name = case.name + "_mesh"
     return case

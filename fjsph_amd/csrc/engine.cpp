@@ -113,6 +113,15 @@ struct fjsph_ctx {
     bool have_state = false, have_nbrs = false;
 
     DevState pn{}, pnp1{};
+    DevState rk1{}, rk2{}, rk3{}; /* Runge-Kutta stage states, allocated on first use */
+    bool rk_alloc = false;
+
+    /* aerodynamic mesh: host copies of the cell data, device arrays */
+    bool have_mesh = false;
+    int64_t mesh_ncells = 0;
+    std::vector<double> mcentre, mvel, mp, mrho;
+    DevMesh dm{};
+    std::vector<void*> mesh_bufs;
     double *x_nxt = nullptr, *v_nxt = nullptr, *rho_nxt = nullptr, *p_nxt = nullptr;
     int32_t *part_id = nullptr, *blk = nullptr, *near_inlet = nullptr;
     double* norms_tmp = nullptr;
@@ -123,7 +132,7 @@ struct fjsph_ctx {
 
     /* cell grid */
     int32_t *cell = nullptr, *cell_count = nullptr, *cell_start = nullptr, *cell_fill = nullptr;
-    int32_t *order = nullptr, *pos = nullptr, *tile_sums = nullptr;
+    int32_t *order = nullptr, *vorder = nullptr, *pos = nullptr, *tile_sums = nullptr;
     int64_t ncell_cap = 0, ncell = 0;
     int32_t dims[3] = {1, 1, 1};
 
@@ -224,11 +233,18 @@ void free_all(fjsph_ctx* ctx)
 {
     free_state(ctx->pn);
     free_state(ctx->pnp1);
+    free_state(ctx->rk1);
+    free_state(ctx->rk2);
+    free_state(ctx->rk3);
+    ctx->rk_alloc = false;
+    for (void* p : ctx->mesh_bufs) (void)hipFree(p);
+    ctx->mesh_bufs.clear();
+    ctx->have_mesh = false;
     dfree(ctx->x_nxt); dfree(ctx->v_nxt); dfree(ctx->rho_nxt); dfree(ctx->p_nxt);
     dfree(ctx->part_id); dfree(ctx->blk); dfree(ctx->near_inlet); dfree(ctx->norms_tmp);
     dfree(ctx->partial); dfree(ctx->gather_tmp); dfree(ctx->gather_tmp_i);
     dfree(ctx->cell); dfree(ctx->cell_count); dfree(ctx->cell_start); dfree(ctx->cell_fill);
-    dfree(ctx->order); dfree(ctx->pos); dfree(ctx->tile_sums);
+    dfree(ctx->order); dfree(ctx->vorder); dfree(ctx->pos); dfree(ctx->tile_sums);
     dfree(ctx->nl.lu4); dfree(ctx->nl.count); dfree(ctx->nl.nnb); dfree(ctx->export_buf);
     dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
     dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb);
@@ -394,7 +410,6 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         permute_all(ctx, ctx->order);
         launch_iota(ctx->order, n, ctx->st);
     }
-    launch_positions(ctx->order, n, ctx->pos, ctx->st);
     HIPCHK(hipMemcpyAsync(ctx->aux.xb, ctx->pnp1.x, sizeof(double) * n * D, hipMemcpyDeviceToDevice, ctx->st));
     /* tile grid: chunks of C cells sized for about one workgroup of targets */
     {
@@ -408,7 +423,10 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         tg.nblocks = int64_t(ctx->dims[1]) * ctx->dims[2] * tg.nchunk;
         tg.start = ctx->cell_start;
         tg.order = ctx->order;
+        tg.vorder = ctx->vorder;
         tg.pos = ctx->pos;
+        for (int k = 0; k < 3; ++k) tg.lo[k] = lo[k];
+        tg.inv_cs = inv_cs;
         tg.cell = ctx->cell;
         if (tg.nblocks > 0x7fffffff) throw std::string("tile grid too large");
         if (tg.nblocks > ctx->partial_cap) {
@@ -453,12 +471,21 @@ void refresh_aux(fjsph_ctx* ctx)
 
 int r_from_x(const fjsph_ctx* ctx) { return ctx->x_at_build ? 1 : 0; }
 
+/* get_aero_velocity for the mesh source (src/Resid.cpp:479-523): FindCell. */
+void mesh_lookup(fjsph_ctx* ctx)
+{
+    if (!ctx->have_mesh) throw std::string("aero_source is the mesh but no mesh was set (fjsph_set_mesh)");
+    HIPCHK(hipMemsetAsync(&ctx->sc->mesh_lost, 0, sizeof(int32_t), ctx->st));
+    launch_find_cell(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->dm, ctx->n, &ctx->sc->mesh_lost, ctx->st);
+}
+
 void dsph(fjsph_ctx* ctx, bool with_aero)
 {
     PassTimer t(ctx, FJSPH_T_DSPH);
     refresh_aux(ctx);
     launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
                 with_aero ? 1 : 0, ctx->partial, ctx->sc, ctx->st);
+    if (with_aero && ctx->s.aero_source == FJSPH_MESH_INFL) mesh_lookup(ctx);
 }
 
 /* Detect_Surface loop 1 and dissipation_terms share one staged pass
@@ -635,11 +662,92 @@ double newmark_loop(fjsph_ctx* ctx, double& logbase)
     return rms;
 }
 
+/* ---- Runge-Kutta 4 (src/Runge_Kutta.cpp:10-515, src/Integration.cpp:305-365).
+ * Stage states are full copies of the previous stage, as in the reference;
+ * every stage's force pass uses the distances frozen at the last build. */
+void rk_stage(fjsph_ctx* ctx, DevState& dst, DevState& prev, double dt, bool final_stage)
+{
+    const int D = ctx->s.dim;
+    const int mixed = ctx->s.precision == FJSPH_PREC_MIXED;
+    if (&dst != &prev) copy_state(ctx, dst, prev);
+    if (ctx->nb > 0) {
+        PassTimer t(ctx, FJSPH_T_BOUNDARY);
+        upload_block_velocities(ctx, true);
+        launch_aux(D, dst, ctx->aux, ctx->n, ctx->s.ale, mixed, ctx->st);
+        for (size_t k = 0; k < ctx->blocks.size(); ++k) {
+            if (ctx->blocks[k].b.is_fluid) continue;
+            launch_boundary(D, ctx->c, dst, ctx->aux, ctx->blk, ctx->bl, int32_t(k), ctx->tg, ctx->nl, ctx->n,
+                            ctx->near_inlet, ctx->st);
+        }
+        launch_rk_bnd_rho(D, ctx->c, dst, ctx->pn, ctx->rk1, ctx->rk2, ctx->rk3, ctx->blk, ctx->bl, ctx->near_inlet,
+                          ctx->n, dt, final_stage ? 1 : 0, ctx->st);
+    }
+    launch_aux(D, dst, ctx->aux, ctx->n, ctx->s.ale, mixed, ctx->st);
+    {
+        PassTimer t(ctx, FJSPH_T_FORCE);
+        launch_force(D, ctx->c, dst, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, 0, ctx->sc, ctx->st);
+    }
+    PassTimer t(ctx, FJSPH_T_UPDATE);
+    launch_nb_update(D, ctx->c, dst, ctx->pn, nullptr, nullptr, nullptr, nullptr, ctx->part_id, ctx->blk, ctx->bl,
+                     ctx->near_inlet, ctx->nl, ctx->n, dt, ctx->partial, ctx->st);
+    launch_rk_update(D, ctx->c, dst, ctx->pn, ctx->rk1, ctx->rk2, ctx->rk3, ctx->part_id, ctx->n, dt,
+                     final_stage ? 1 : 0, ctx->st);
+}
+
+/* Check_RK_Error (src/Runge_Kutta.cpp:10-26): log10 rms of b.x - a.x over
+ * the fluid, minus logbase. */
+double rk_error(fjsph_ctx* ctx, const DevState& a, const DevState& b, double logbase)
+{
+    {
+        PassTimer t(ctx, FJSPH_T_REDUCE);
+        launch_sqdiff(ctx->s.dim, a.x, b.x, ctx->part_id, ctx->nb, ctx->n, ctx->partial, ctx->st);
+        launch_sum_partials(ctx->partial, reduction_parts(ctx->n), 1, &ctx->sc->residual, ctx->st);
+        HIPCHK(hipMemcpyAsync(&ctx->sc_host->residual, &ctx->sc->residual, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->st));
+    }
+    sync(ctx);
+    const double nf = double(ctx->n - ctx->nb);
+    return std::log10(std::sqrt(ctx->sc_host->residual / nf)) - logbase;
+}
+
+/* integrate_no_update with the Runge-Kutta solver: Get_First_RK on the
+ * state of the first build, then Runge_Kutta4 after the second. */
+double integrate_no_update_rk(fjsph_ctx* ctx)
+{
+    if (!ctx->rk_alloc) {
+        alloc_state(ctx, ctx->rk1);
+        alloc_state(ctx, ctx->rk2);
+        alloc_state(ctx, ctx->rk3);
+        ctx->rk_alloc = true;
+    }
+    ctx->iteration = 0;
+    ctx->rollbacks = 0;
+    ctx->delta_t = find_timestep(ctx);
+    build_neighbours(ctx, true);
+    prepasses(ctx);
+    rk_stage(ctx, ctx->pnp1, ctx->pn, 0.5 * ctx->delta_t, false);
+    ctx->x_at_build = false;
+    const double logbase = rk_error(ctx, ctx->pn, ctx->pnp1, 0.0);
+    build_neighbours(ctx, false);
+    prepasses(ctx);
+    copy_state(ctx, ctx->rk1, ctx->pnp1);
+    rk_stage(ctx, ctx->rk2, ctx->rk1, 0.5 * ctx->delta_t, false);
+    rk_stage(ctx, ctx->rk3, ctx->rk2, ctx->delta_t, false);
+    rk_stage(ctx, ctx->pnp1, ctx->rk3, ctx->delta_t, true);
+    ctx->x_at_build = false;
+    const double rms = rk_error(ctx, ctx->rk3, ctx->pnp1, logbase);
+    HIPCHK(hipMemcpyAsync(&ctx->sc_host->npd, &ctx->sc->npd, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    ctx->last_npd = ctx->sc_host->npd;
+    drain_timers(ctx);
+    return rms;
+}
+
 /* integrate_no_update (src/Integration.cpp:27-107). */
 double integrate_no_update(fjsph_ctx* ctx)
 {
-    if (ctx->s.integrator != FJSPH_NEWMARK_BETA)
-        throw std::string("Runge-Kutta integration is not built into this engine yet");
+    if (ctx->s.integrator == FJSPH_RUNGE_KUTTA) return integrate_no_update_rk(ctx);
+    if (ctx->s.integrator != FJSPH_NEWMARK_BETA) throw std::string("unknown integrator");
     ctx->iteration = 0;
     ctx->rollbacks = 0;
     double logbase = 0.0;
@@ -757,7 +865,7 @@ fjsph_status fjsph_create(const fjsph_settings* s, int32_t device, fjsph_ctx** o
     if (fjsph_derive(s, &d) != FJSPH_OK) return FJSPH_ERR_ARG;
     if (s->precision != FJSPH_PREC_FP64 && s->precision != FJSPH_PREC_MIXED) return FJSPH_ERR_ARG;
     if (s->aero_case != FJSPH_NO_AERO && s->aero_case != FJSPH_GISSLER) return FJSPH_ERR_UNSUPPORTED;
-    if (s->aero_source != FJSPH_CONST_VEL) return FJSPH_ERR_UNSUPPORTED;
+    if (s->aero_source != FJSPH_CONST_VEL && s->aero_source != FJSPH_MESH_INFL) return FJSPH_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FJSPH_ERR_NO_DEVICE;
     if (device < 0 || device >= ndev) return FJSPH_ERR_ARG;
@@ -829,6 +937,7 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
         dalloc(ctx->cell, size_t(n_total));
         dalloc(ctx->order, size_t(n_total));
         dalloc(ctx->pos, size_t(n_total));
+        dalloc(ctx->vorder, size_t(n_total));
         dalloc(ctx->aux.vol, size_t(n_total));
         dalloc(ctx->aux.P, size_t(n_total));
         dalloc(ctx->aux.invrho, size_t(n_total));
@@ -1077,8 +1186,195 @@ fjsph_status fjsph_aero_velocity(fjsph_ctx* ctx)
 {
     return guarded(ctx, [&] {
         require(ctx, true);
-        launch_aero_velocity(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->st);
+        if (ctx->s.aero_source == FJSPH_MESH_INFL) mesh_lookup(ctx);
+        else launch_aero_velocity(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->st);
         sync(ctx);
+    });
+}
+
+fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts, int64_t nfaces,
+                            const int64_t* face_off, const int64_t* face_verts, const int64_t* face_lr,
+                            int64_t ncells, const int64_t* cell_off, const int64_t* cell_faces,
+                            const double* vert_vel, const double* vert_p, const double* vert_rho)
+{
+    if (!ctx || nverts <= 0 || nfaces <= 0 || ncells <= 0 || !verts || !face_off || !face_verts || !face_lr ||
+        !cell_off || !cell_faces || !vert_vel || !vert_p || !vert_rho)
+        return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        const int D = ctx->s.dim;
+        const int64_t nfv = face_off[nfaces], ncf = cell_off[ncells];
+        if (face_off[0] != 0 || cell_off[0] != 0 || nfv > INT32_MAX || ncf > INT32_MAX || nverts > INT32_MAX ||
+            ncells > INT32_MAX)
+            throw std::string("set_mesh: offsets must start at 0 and sizes fit 32-bit indices");
+        for (int64_t f = 0; f < nfaces; ++f) {
+            const int64_t nv = face_off[f + 1] - face_off[f];
+            if (nv < D || (D == 2 && nv != 2)) throw std::string("set_mesh: a face has too few vertices");
+        }
+        for (int64_t k = 0; k < nfv; ++k)
+            if (face_verts[k] < 0 || face_verts[k] >= nverts) throw std::string("set_mesh: face vertex out of range");
+        for (int64_t k = 0; k < ncf; ++k)
+            if (cell_faces[k] < 0 || cell_faces[k] >= nfaces) throw std::string("set_mesh: cell face out of range");
+        (void)face_lr; /* boundary markers: used by deletion, which is not built */
+
+        /* Average_Point_Data_to_Cell (src/CDFIO.cpp:112-155): compensated sums
+         * over the cell's unique vertices in ascending order */
+        ctx->mcentre.assign(size_t(ncells) * D, 0.0);
+        ctx->mvel.assign(size_t(ncells) * D, 0.0);
+        ctx->mp.assign(size_t(ncells), 0.0);
+        ctx->mrho.assign(size_t(ncells), 0.0);
+        std::vector<double> cbox(size_t(ncells) * 6);
+        std::vector<int64_t> uv;
+        auto kahan = [&](auto&& val) {
+            double sum = 0.0, cc = 0.0;
+            for (int64_t v : uv) {
+                const double y = val(v) - cc;
+                const double t = sum + y;
+                cc = (t - sum) - y;
+                sum = t;
+            }
+            return sum;
+        };
+        for (int64_t c = 0; c < ncells; ++c) {
+            uv.clear();
+            for (int64_t k = cell_off[c]; k < cell_off[c + 1]; ++k) {
+                const int64_t f = cell_faces[k];
+                for (int64_t q = face_off[f]; q < face_off[f + 1]; ++q) uv.push_back(face_verts[q]);
+            }
+            std::sort(uv.begin(), uv.end());
+            uv.erase(std::unique(uv.begin(), uv.end()), uv.end());
+            const double nv = double(uv.size());
+            for (int a = 0; a < D; ++a) {
+                ctx->mcentre[c * D + a] = kahan([&](int64_t v) { return verts[v * D + a]; }) / nv;
+                ctx->mvel[c * D + a] = kahan([&](int64_t v) { return vert_vel[v * D + a]; }) / nv;
+            }
+            ctx->mp[c] = kahan([&](int64_t v) { return vert_p[v]; }) / nv;
+            ctx->mrho[c] = kahan([&](int64_t v) { return vert_rho[v]; }) / nv;
+            for (int a = 0; a < 3; ++a) {
+                double lo = 0.0, hi = 0.0;
+                if (a < D) {
+                    lo = hi = verts[uv[0] * D + a];
+                    for (int64_t v : uv) { lo = std::min(lo, verts[v * D + a]); hi = std::max(hi, verts[v * D + a]); }
+                }
+                cbox[c * 6 + a] = lo;
+                cbox[c * 6 + 3 + a] = hi;
+            }
+        }
+
+        /* voxel candidate grid, about 4 cells per voxel */
+        double glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+        for (int a = 0; a < D; ++a) {
+            glo[a] = verts[a];
+            ghi[a] = verts[a];
+            for (int64_t v = 0; v < nverts; ++v) {
+                glo[a] = std::min(glo[a], verts[v * D + a]);
+                ghi[a] = std::max(ghi[a], verts[v * D + a]);
+            }
+        }
+        double vol = 1.0;
+        for (int a = 0; a < D; ++a) vol *= std::max(ghi[a] - glo[a], 1e-300);
+        double h = std::pow(vol / std::max(1.0, double(ncells) / 4.0), 1.0 / D);
+        int32_t gd[3] = {1, 1, 1};
+        for (;;) {
+            double tot = 1.0;
+            for (int a = 0; a < D; ++a) {
+                gd[a] = int32_t(std::min(1.0e6, std::floor((ghi[a] - glo[a]) / h) + 1.0));
+                tot *= gd[a];
+            }
+            if (tot <= 6.4e7) break;
+            h *= 1.25;
+        }
+        const double ginv = 1.0 / h;
+        const int64_t nvox = int64_t(gd[0]) * gd[1] * gd[2];
+        auto vrange = [&](int64_t c, int a, int32_t& g0, int32_t& g1) {
+            g0 = 0;
+            g1 = 0;
+            if (a >= D) return;
+            g0 = int32_t(std::max(0.0, std::floor((cbox[c * 6 + a] - glo[a]) * ginv)));
+            g1 = int32_t(std::min(double(gd[a] - 1), std::floor((cbox[c * 6 + 3 + a] - glo[a]) * ginv)));
+        };
+        std::vector<int64_t> gcount(size_t(nvox) + 1, 0);
+        std::vector<int32_t> gcells;
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<int64_t> fill;
+            if (pass == 1) {
+                fill.assign(gcount.begin(), gcount.end() - 1);
+                gcells.assign(size_t(gcount[nvox]), 0);
+            }
+            for (int64_t c = 0; c < ncells; ++c) {
+                int32_t r[3][2];
+                for (int a = 0; a < 3; ++a) vrange(c, a, r[a][0], r[a][1]);
+                for (int32_t z = r[2][0]; z <= r[2][1]; ++z)
+                    for (int32_t y = r[1][0]; y <= r[1][1]; ++y)
+                        for (int32_t x = r[0][0]; x <= r[0][1]; ++x) {
+                            const int64_t v = (int64_t(z) * gd[1] + y) * gd[0] + x;
+                            if (pass == 0) ++gcount[v + 1];
+                            else gcells[fill[v]++] = int32_t(c);
+                        }
+            }
+            if (pass == 0) {
+                for (int64_t v = 0; v < nvox; ++v) gcount[v + 1] += gcount[v];
+                if (gcount[nvox] > INT32_MAX) throw std::string("set_mesh: voxel lists too long");
+                continue;
+            }
+            /* upload */
+            for (void* p : ctx->mesh_bufs) (void)hipFree(p);
+            ctx->mesh_bufs.clear();
+            auto up = [&](const void* src, size_t bytes) {
+                void* d = nullptr;
+                HIPCHK(hipMalloc(&d, std::max<size_t>(bytes, 4)));
+                ctx->mesh_bufs.push_back(d);
+                if (bytes) HIPCHK(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+                return d;
+            };
+            auto to32 = [](const int64_t* a, int64_t n) { return std::vector<int32_t>(a, a + n); };
+            std::vector<double> v3(size_t(nverts) * 3, 0.0);
+            for (int64_t v = 0; v < nverts; ++v)
+                for (int a = 0; a < D; ++a) v3[v * 3 + a] = verts[v * D + a];
+            std::vector<int32_t> fo = to32(face_off, nfaces + 1), fv = to32(face_verts, nfv);
+            std::vector<int32_t> co = to32(cell_off, ncells + 1), cf = to32(cell_faces, ncf);
+            std::vector<int32_t> gs(size_t(nvox) + 1);
+            for (int64_t v = 0; v <= nvox; ++v) gs[v] = int32_t(gcount[v]);
+            sync(ctx);
+            DevMesh& m = ctx->dm;
+            m.ncells = int32_t(ncells);
+            m.verts = (const double*)up(v3.data(), v3.size() * 8);
+            m.face_off = (const int32_t*)up(fo.data(), fo.size() * 4);
+            m.face_v = (const int32_t*)up(fv.data(), fv.size() * 4);
+            m.cell_off = (const int32_t*)up(co.data(), co.size() * 4);
+            m.cell_f = (const int32_t*)up(cf.data(), cf.size() * 4);
+            m.cVel = (const double*)up(ctx->mvel.data(), ctx->mvel.size() * 8);
+            m.cP = (const double*)up(ctx->mp.data(), ctx->mp.size() * 8);
+            m.cRho = (const double*)up(ctx->mrho.data(), ctx->mrho.size() * 8);
+            m.gstart = (const int32_t*)up(gs.data(), gs.size() * 4);
+            m.gcells = (const int32_t*)up(gcells.data(), gcells.size() * 4);
+            for (int a = 0; a < 3; ++a) { m.glo[a] = glo[a]; m.gdims[a] = gd[a]; }
+            m.ginv = ginv;
+        }
+        ctx->mesh_ncells = ncells;
+        ctx->have_mesh = true;
+    });
+}
+
+fjsph_status fjsph_get_mesh_cells(fjsph_ctx* ctx, double* centre, double* vel, double* p, double* rho)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->have_mesh) throw std::string("no mesh set");
+        if (centre) std::copy(ctx->mcentre.begin(), ctx->mcentre.end(), centre);
+        if (vel) std::copy(ctx->mvel.begin(), ctx->mvel.end(), vel);
+        if (p) std::copy(ctx->mp.begin(), ctx->mp.end(), p);
+        if (rho) std::copy(ctx->mrho.begin(), ctx->mrho.end(), rho);
+    });
+}
+
+fjsph_status fjsph_mesh_lost(fjsph_ctx* ctx, int64_t* lost)
+{
+    if (!ctx || !lost) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        HIPCHK(hipMemcpyAsync(&ctx->sc_host->mesh_lost, &ctx->sc->mesh_lost, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->st));
+        sync(ctx);
+        *lost = ctx->sc_host->mesh_lost;
     });
 }
 

@@ -40,7 +40,7 @@ struct DevState {
 };
 
 /* Neighbour lists in rows of 4 entries: row r of the target at position p
- * of the cell order is lu4[r*npad + p], four 16-bit indices into the
+ * of the target order (TileGrid::vorder) is lu4[r*npad + p], four 16-bit indices into the
  * candidate union of the target's tile block, ascending, all in one TILE
  * of the union; unused entries hold NO_ENTRY.  The targets of one wave
  * hold the same number of rows per tile (padding rows are all NO_ENTRY).
@@ -68,9 +68,12 @@ struct TileGrid {
     int32_t nchunk;
     int64_t nblocks;
     const int32_t* start; /* cell_start, ncell + 1 */
-    const int32_t* order; /* slots in cell order */
+    const int32_t* order; /* slots in cell order (the candidate unions) */
     const int32_t* cell;  /* cell of every slot */
-    const int32_t* pos;   /* position of every slot in order[] */
+    int32_t* vorder;      /* target order: order[] regrouped per block chunk */
+    int32_t* pos;         /* position of every slot in vorder[] */
+    double lo[3];         /* grid origin */
+    double inv_cs;        /* 1 / cell edge */
 };
 
 /* Per-particle quantities derived from the current rho, p, m, b (and vPert),
@@ -95,6 +98,22 @@ struct DevBlocks {
     int32_t n;
 };
 
+/* Aerodynamic mesh on the device (vertices with 3 components in 2D too,
+ * z = 0), CSR faces and cells, cell values, and the voxel candidate grid:
+ * voxel v = (z*gdims[1] + y)*gdims[0] + x lists gcells[gstart[v] ..
+ * gstart[v+1]), the cells whose bounding box overlaps it, ascending. */
+struct DevMesh {
+    int32_t ncells;
+    const double* verts;
+    const int32_t *face_off, *face_v;
+    const int32_t *cell_off, *cell_f;
+    const double *cVel, *cP, *cRho;
+    double glo[3];
+    double ginv;
+    int32_t gdims[3];
+    const int32_t *gstart, *gcells;
+};
+
 /* Scalars kept on the device between kernels. */
 struct DevScalars {
     double npd;       /* numerical particle density of the last dSPH pass */
@@ -103,6 +122,7 @@ struct DevScalars {
     double bbox[6];   /* lo[3], hi[3] */
     int32_t max_count;
     int32_t overflow; /* largest candidate union beyond 65535 (0 if none) */
+    int32_t mesh_lost; /* aero particles in no mesh cell at the last lookup */
 };
 
 /* ---- launch wrappers (kernels.hip); every one enqueues on `st` ---------- */
@@ -117,7 +137,6 @@ void launch_sort_cells(const int32_t* start, int64_t ncell, int32_t* order, hipS
 void launch_build_lists(int dim, const double* x, int64_t n, TileGrid tg, double sr, NbrList nl, DevScalars* sc,
                         hipStream_t st);
 void launch_export_lists(int dim, TileGrid tg, NbrList nl, int64_t n, int32_t* out, hipStream_t st);
-void launch_positions(const int32_t* order, int64_t n, int32_t* pos, hipStream_t st);
 void launch_aux(int dim, DevState s, Aux a, int64_t n, int ale, int mixed, hipStream_t st);
 void launch_gather_d(const double* src, double* dst, const int32_t* perm, int64_t n, int width, hipStream_t st);
 void launch_gather_i(const int32_t* src, int32_t* dst, const int32_t* perm, int64_t n, hipStream_t st);
@@ -147,6 +166,19 @@ void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
 void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double* next_x, double* next_v,
                       double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk, DevBlocks bl,
                       const int32_t* near_inlet, NbrList nl, int64_t n, double dt, double* partial, hipStream_t st);
+/* FindCell on the aero particles of s (mesh.hip). */
+void launch_find_cell(int dim, const Consts& c, DevState s, const int32_t* part_id, DevMesh m, int64_t n,
+                      int32_t* lost, hipStream_t st);
+/* Runge-Kutta 4 (pairs.hip): stage update of s from pn (final_stage: the
+ * weighted combination with stages s1..s3), boundary densities, and the
+ * squared position change partials of Check_RK_Error. */
+void launch_rk_update(int dim, const Consts& c, DevState s, DevState pn, DevState s1, DevState s2, DevState s3,
+                      const int32_t* part_id, int64_t n, double dt, int final_stage, hipStream_t st);
+void launch_rk_bnd_rho(int dim, const Consts& c, DevState s, DevState pn, DevState s1, DevState s2, DevState s3,
+                       const int32_t* blk, DevBlocks bl, const int32_t* near_inlet, int64_t n, double dt,
+                       int final_stage, hipStream_t st);
+void launch_sqdiff(int dim, const double* xa, const double* xb, const int32_t* part_id, int64_t n_bound, int64_t n,
+                   double* partial, hipStream_t st);
 void launch_sum_partials(const double* partial, int nparts, int width, double* out, hipStream_t st);
 void launch_sum_partials_scaled(const double* partial, int nparts, int width, double scale, double* out,
                                 hipStream_t st);

@@ -3,15 +3,17 @@
  *
  * Work decomposition: a block is a chunk of C consecutive cells of one cell
  * row; its targets are the particles of those cells (positions [p0, p1) of
- * the cell-sorted order[] array) and its candidate union is the 3^(D-1)
+ * the cell-sorted order[] array, taken in the target order vorder[], which
+ * is order[] itself unless FJ_GROUP_TARGETS regroups them) and its candidate
+ * union is the 3^(D-1)
  * neighbour rows over cells [cx0 - 1, cx1].  The union is staged in LDS in
  * tiles of TILE records with coalesced loads; each lane walks its own
  * neighbour list tile by tile.  Pair terms never scatter: every pass is
  * owner-computes.
  *
  * Neighbour lists (built by k_build_t, read by every pass): rows of four
- * 16-bit union indices, row r of the target at position p stored at
- * lu4[r*npad + p].  Within a row the indices ascend and lie in one union
+ * 16-bit union indices, row r of the target at position p of vorder[]
+ * stored at lu4[r*npad + p].  Within a row the indices ascend and lie in one union
  * tile; unused entries hold NO_ENTRY.  The 64 targets of a wave hold the
  * same number of rows for every tile (rows padded to the wave's maximum), so
  * a wave walks its lists in lock-step: every row load is one coalesced
@@ -156,6 +158,14 @@ struct Rows {
         (cu).advance(nl);           \
     }
 
+/* Regroup each chunk's targets by the (y, z) half of their cell (1) or keep
+ * the cell order (0).  Measured on the 4M jet column: regrouping made
+ * dSPH 8% faster but Detect_Surface, shifting and the build slower (their
+ * per-target loads coalesce less), 1% slower overall, so it is off. */
+#ifndef FJ_GROUP_TARGETS
+#define FJ_GROUP_TARGETS 0
+#endif
+
 __device__ __forceinline__ int wave_max(int v)
 {
 #pragma unroll
@@ -171,6 +181,7 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
 {
     __shared__ double tp[D][TILE];
     __shared__ int32_t tsl[TILE];
+    __shared__ uint64_t scan[TPB];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -180,10 +191,54 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
         return;
     }
     const int rcap = nl.cap >> 2;
+    const int64_t brow = blockIdx.x / tg.nchunk;
+    const int cy = (int)(brow % tg.dims[1]);
+    const int cz = (D == 3) ? (int)(brow / tg.dims[1]) : 0;
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
-        const int64_t i = active ? tg.order[p] : 0;
+        /* target order of this chunk: stable regrouping by the (y, z) half of
+         * the cell the target sits in, so a wave's targets reach the same
+         * neighbour rows and their per-tile list lengths are alike */
+        if (!FJ_GROUP_TARGETS) {
+            if (active) {
+                const int32_t sl = tg.order[p];
+                tg.vorder[p] = sl;
+                tg.pos[sl] = p;
+            }
+            __syncthreads();
+        } else {
+            const int64_t sl = active ? tg.order[p] : 0;
+            int key = 0;
+            if (active) {
+                const double fy = (x[sl * D + 1] - tg.lo[1]) * tg.inv_cs - cy;
+                key = fy >= 0.5 ? 1 : 0;
+                if (D == 3) {
+                    const double fz = (x[sl * D + 2] - tg.lo[2]) * tg.inv_cs - cz;
+                    key += fz >= 0.5 ? 2 : 0;
+                }
+            }
+            const uint64_t mine = active ? (1ull << (16 * key)) : 0ull;
+            uint64_t incl = mine;
+            scan[threadIdx.x] = incl;
+            __syncthreads();
+            for (int off = 1; off < TPB; off <<= 1) {
+                const uint64_t o = threadIdx.x >= off ? scan[threadIdx.x - off] : 0ull;
+                __syncthreads();
+                incl += o;
+                scan[threadIdx.x] = incl;
+                __syncthreads();
+            }
+            const uint64_t tot = scan[TPB - 1], excl = incl - mine;
+            int rank = int((excl >> (16 * key)) & 0xFFFFull);
+            for (int k = 0; k < key; ++k) rank += int((tot >> (16 * k)) & 0xFFFFull);
+            if (active) {
+                tg.vorder[pb + rank] = (int32_t)sl;
+                tg.pos[sl] = pb + rank;
+            }
+            __syncthreads();
+        }
+        const int64_t i = active ? tg.vorder[p] : 0;
         double xi[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) xi[a] = active ? x[i * D + a] : 0.0;
@@ -242,17 +297,6 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
             atomicMax(&sc->max_count, nrow * 4);
         }
     }
-}
-
-__global__ __launch_bounds__(TPB) void k_positions(const int32_t* __restrict__ order, int64_t n, int32_t* pos)
-{
-    const int64_t p = blockIdx.x * (int64_t)TPB + threadIdx.x;
-    if (p < n) pos[order[p]] = (int32_t)p;
-}
-
-void launch_positions(const int32_t* order, int64_t n, int32_t* pos, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_positions, dim3(grid_of(n)), dim3(TPB), 0, st, order, n, pos);
 }
 
 void launch_build_lists(int dim, const double* x, int64_t n, TileGrid tg, double sr, NbrList nl, DevScalars* sc,
@@ -357,7 +401,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
-        const int64_t i = active ? tg.order[p] : 0;
+        const int64_t i = active ? tg.vorder[p] : 0;
         double xi[D], xbi[D], Lm[NS], Lnb[NS], gR[D], nr[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
@@ -563,7 +607,7 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
-        const int64_t i = active ? tg.order[p] : 0;
+        const int64_t i = active ? tg.vorder[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
         double xi[D], xbi[D], ni[D], nhat[D], tau_hat[D], pT[D], vi[D], gri[D], nrm[D];
 #pragma unroll
@@ -780,7 +824,7 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
-        const int64_t i = active ? tg.order[p] : 0;
+        const int64_t i = active ? tg.vorder[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
         double xi[D], xbi[D], ni[D], Vd[D], Li[D][D];
 #pragma unroll
@@ -899,7 +943,7 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
-        const int64_t i = active ? tg.order[p] : 0;
+        const int64_t i = active ? tg.vorder[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
         const double lam_nb = fluid ? s.lam_nb[i] : 0.0;
         const bool work = fluid && !(lam_nb < 0.55 || s.b[i] == FJSPH_BUFFER);
@@ -1234,7 +1278,7 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
-        const int64_t i = active ? tg.order[p] : 0;
+        const int64_t i = active ? tg.vorder[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
         double xi[D], vi[D], xbi[D];
         T xit[D], vit[D], xbit[D], vpi[D];
@@ -1513,6 +1557,136 @@ __global__ __launch_bounds__(TPB) void k_nb_update(Consts c, DevState s, DevStat
         const double bs = block_reduce(resid, OpSum(), lds);
         if (threadIdx.x == 0) partial[blockIdx.x] = bs;
     }
+}
+
+/* ------------------------------------------------------- Runge-Kutta 4 */
+
+/* Fluid update of do_runge_kutta_intermediate_step (final = 0) and
+ * do_runge_kutta_final_step (final = 1) (src/Runge_Kutta.cpp:28-457) on the
+ * stage state s, whose acc and Rrho were just computed; the x update reads
+ * the velocity s held before this stage.  The final step combines the
+ * previous step's acceleration (pn.acc) with stages 1-3, as the reference
+ * does. */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_rk_update(Consts c, DevState s, DevState pn, DevState s1, DevState s2,
+                                                   DevState s3, const int32_t* __restrict__ part_id, int64_t n,
+                                                   double dt, int final_stage)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n || part_id[i] < c.n_bound) return;
+    const int32_t b = s.b[i];
+    if (!final_stage) {
+        if (b <= FJSPH_BUFFER) return;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const int64_t k = i * D + a;
+            const double vel = c.ale ? (s.v[k] + s.vPert[k]) : s.v[k];
+            s.x[k] = pn.x[k] + dt * vel;
+            s.v[k] = pn.v[k] + dt * s.acc[k];
+        }
+        const double rho = fmax(c.rho_min, fmin(c.rho_max, pn.rho[i] + dt * s.Rrho[i]));
+        s.rho[i] = rho;
+        s.p[i] = eos_pressure(c, rho);
+        return;
+    }
+    const double h6 = dt / 6.0;
+    if (b > FJSPH_BUFFER && b != FJSPH_OUTLET) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const int64_t k = i * D + a;
+            double vs;
+            if (c.ale)
+                vs = (((pn.v[k] + pn.vPert[k]) + 2.0 * (s1.v[k] + s1.vPert[k])) + 2.0 * (s2.v[k] + s2.vPert[k])) +
+                     (s3.v[k] + s3.vPert[k]);
+            else
+                vs = ((pn.v[k] + 2.0 * s1.v[k]) + 2.0 * s2.v[k]) + s3.v[k];
+            s.x[k] = pn.x[k] + h6 * vs;
+            s.v[k] = pn.v[k] + h6 * (((pn.acc[k] + 2.0 * s1.acc[k]) + 2.0 * s2.acc[k]) + s3.acc[k]);
+        }
+        const double dr = ((pn.Rrho[i] + 2.0 * s1.Rrho[i]) + 2.0 * s2.Rrho[i]) + s3.Rrho[i];
+        const double rho = fmax(c.rho_min, fmin(c.rho_max, pn.rho[i] + h6 * dr));
+        s.rho[i] = rho;
+        s.p[i] = eos_pressure(c, rho);
+    } else if (b == FJSPH_OUTLET) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const int64_t k = i * D + a;
+            s.x[k] = pn.x[k] + dt * s.v[k];
+        }
+    }
+}
+
+/* Density of DBC and ghost boundary particles in a Runge-Kutta stage (the
+ * solver part of rk_boundary, src/Runge_Kutta.cpp:69-160 and 281-380). */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_rk_bnd_rho(Consts c, DevState s, DevState pn, DevState s1, DevState s2,
+                                                    DevState s3, const int32_t* __restrict__ blk, DevBlocks bl,
+                                                    const int32_t* __restrict__ near_inlet, int64_t n, double dt,
+                                                    int final_stage)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n) return;
+    const int bk = blk[i];
+    if (bk < 0) return;
+    const int solver = bl.solver[bk];
+    if (solver != FJSPH_DBC && solver != FJSPH_GHOST) return;
+    const double drho = final_stage ? (((pn.Rrho[i] + 2.0 * s1.Rrho[i]) + 2.0 * s2.Rrho[i]) + s3.Rrho[i]) : s.Rrho[i];
+    const double fac = final_stage ? (dt / 6.0) : dt;
+    const bool near = solver == FJSPH_GHOST && near_inlet[i];
+    const double lo = near ? c.rho0 : c.rho_min;
+    const double rho = fmax(lo, fmin(c.rho_max, pn.rho[i] + fac * drho));
+    s.rho[i] = rho;
+    s.p[i] = eos_pressure(c, rho);
+    if (solver == FJSPH_DBC && final_stage) s.Rrho[i] = s3.Rrho[i];
+    if (near) s.Rrho[i] = fmax(0.0, s.Rrho[i]);
+}
+
+/* Per-block partials of sum |xb - xa|^2 over fluid slots (Check_RK_Error,
+ * src/Runge_Kutta.cpp:10-26). */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_sqdiff(const double* __restrict__ xa, const double* __restrict__ xb,
+                                                const int32_t* __restrict__ part_id, int64_t n_bound, int64_t n,
+                                                double* partial)
+{
+    __shared__ double lds[NWAVE];
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    double e = 0.0;
+    if (i < n && part_id[i] >= n_bound) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const double d = xb[i * D + a] - xa[i * D + a];
+            e += d * d;
+        }
+    }
+    const double bs = block_reduce(e, OpSum(), lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+
+void launch_rk_update(int dim, const Consts& c, DevState s, DevState pn, DevState s1, DevState s2, DevState s3,
+                      const int32_t* part_id, int64_t n, double dt, int final_stage, hipStream_t st)
+{
+    const dim3 g(grid_of(n)), b(TPB);
+    if (dim == 2) hipLaunchKernelGGL(k_rk_update<2>, g, b, 0, st, c, s, pn, s1, s2, s3, part_id, n, dt, final_stage);
+    else hipLaunchKernelGGL(k_rk_update<3>, g, b, 0, st, c, s, pn, s1, s2, s3, part_id, n, dt, final_stage);
+}
+
+void launch_rk_bnd_rho(int dim, const Consts& c, DevState s, DevState pn, DevState s1, DevState s2, DevState s3,
+                       const int32_t* blk, DevBlocks bl, const int32_t* near_inlet, int64_t n, double dt,
+                       int final_stage, hipStream_t st)
+{
+    const dim3 g(grid_of(n)), b(TPB);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_rk_bnd_rho<2>, g, b, 0, st, c, s, pn, s1, s2, s3, blk, bl, near_inlet, n, dt, final_stage);
+    else
+        hipLaunchKernelGGL(k_rk_bnd_rho<3>, g, b, 0, st, c, s, pn, s1, s2, s3, blk, bl, near_inlet, n, dt, final_stage);
+}
+
+void launch_sqdiff(int dim, const double* xa, const double* xb, const int32_t* part_id, int64_t n_bound, int64_t n,
+                   double* partial, hipStream_t st)
+{
+    const dim3 g(grid_of(n)), b(TPB);
+    if (dim == 2) hipLaunchKernelGGL(k_sqdiff<2>, g, b, 0, st, xa, xb, part_id, n_bound, n, partial);
+    else hipLaunchKernelGGL(k_sqdiff<3>, g, b, 0, st, xa, xb, part_id, n_bound, n, partial);
 }
 
 void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,

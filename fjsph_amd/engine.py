@@ -58,6 +58,10 @@ SIGNATURES = {
     "fjsph_get_timers": (C.c_int, [_VP, _D, _I64]),
     "fjsph_pass_bytes": (C.c_int, [_VP, C.c_int32, _D]),
     "fjsph_neighbour_stats": (C.c_int, [_VP, _I64, _I32]),
+    "fjsph_set_mesh": (C.c_int, [_VP, C.c_int64, _D, C.c_int64, _I64, _I64, _I64, C.c_int64, _I64, _I64,
+                                 _D, _D, _D]),
+    "fjsph_get_mesh_cells": (C.c_int, [_VP, _D, _D, _D, _D]),
+    "fjsph_mesh_lost": (C.c_int, [_VP, _I64]),
 }
 
 
@@ -152,6 +156,8 @@ class Engine:
     def upload_case(self, case):
         self.set_blocks(case.blocks)
         self.upload_state(case.x, case.v, case.rho, case.p, case.m, case.b, case.n_bound)
+        if getattr(case, "mesh", None) is not None:
+            self.set_mesh(case.mesh)
 
     def upload_state(self, x, v, rho, p, m, b, n_bound):
         x = np.ascontiguousarray(x, np.float64)
@@ -199,6 +205,30 @@ class Engine:
         idx = np.zeros(total, np.int64)
         self._chk(self.lib.fjsph_get_neighbours(self.h, _ptr(off, _I64), _ptr(idx, _I64), total), "neighbours")
         return off, idx
+
+    def set_mesh(self, mesh):
+        """fjsph_set_mesh with a cases.Mesh."""
+        self._mesh = mesh
+        a = lambda x, t=np.float64: np.ascontiguousarray(x, dtype=t)
+        self._mesh_arrays = [a(mesh.verts), a(mesh.face_off, np.int64), a(mesh.face_verts, np.int64),
+                             a(mesh.face_lr, np.int64), a(mesh.cell_off, np.int64), a(mesh.cell_faces, np.int64),
+                             a(mesh.vert_vel), a(mesh.vert_p), a(mesh.vert_rho)]
+        v, fo, fv, lr, co, cf, vv, vp, vr = self._mesh_arrays
+        self._chk(self.lib.fjsph_set_mesh(self.h, mesh.nverts, _ptr(v), mesh.nfaces, _ptr(fo, _I64), _ptr(fv, _I64),
+                                          _ptr(lr, _I64), mesh.ncells, _ptr(co, _I64), _ptr(cf, _I64), _ptr(vv),
+                                          _ptr(vp), _ptr(vr)), "set_mesh")
+
+    def mesh_cells(self):
+        nc = self._mesh.ncells
+        c, v = np.zeros((nc, self.dim)), np.zeros((nc, self.dim))
+        p, r = np.zeros(nc), np.zeros(nc)
+        self._chk(self.lib.fjsph_get_mesh_cells(self.h, _ptr(c), _ptr(v), _ptr(p), _ptr(r)), "get_mesh_cells")
+        return c, v, p, r
+
+    def mesh_lost(self):
+        v = np.zeros(1, np.int64)
+        self._chk(self.lib.fjsph_mesh_lost(self.h, _ptr(v, _I64)), "mesh_lost")
+        return int(v[0])
 
     def neighbour_stats(self):
         pairs = np.zeros(1, np.int64)

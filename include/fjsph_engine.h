@@ -63,7 +63,10 @@ enum { FJSPH_NEWMARK_BETA = 0, FJSPH_RUNGE_KUTTA = 1 };     /* src/VarDefs.h:129
 enum { FJSPH_DBC = 0, FJSPH_PRESSURE_G = 1, FJSPH_GHOST = 2 }; /* src/VarDefs.h:135-140 */
 enum { FJSPH_COLE_EOS = 0, FJSPH_ISO_EOS = 1 };            /* src/VarDefs.h:142-146 */
 enum { FJSPH_NO_AERO = 0, FJSPH_GISSLER = 1, FJSPH_INDUCED_PRESSURE = 2, FJSPH_SKIN_FRIC = 3 };
-enum { FJSPH_CONST_VEL = 0, FJSPH_MESH_INFL = 1 };          /* src/VarDefs.h:168-173 */
+/* Aerodynamic velocity source (src/VarDefs.h:168-173: constVel, meshInfl,
+ * VLMInfl).  FJSPH_SURROGATE has no counterpart in the reference: the mesh
+ * field replaced by a small neural network (fjsph_set_surrogate). */
+enum { FJSPH_CONST_VEL = 0, FJSPH_MESH_INFL = 1, FJSPH_VLM_INFL = 2, FJSPH_SURROGATE = 3 };
 
 /*
  * Settings: the fields of FLUID / AERO / INTEG_SETT / SIM that the hot path
@@ -264,6 +267,34 @@ fjsph_status fjsph_advance_frame(fjsph_ctx* ctx);
 /* Simulation clock and timestep (INTEG_SETT current_time / delta_t). */
 fjsph_status fjsph_get_time(fjsph_ctx* ctx, double* time, double* delta_t, double* cfl);
 fjsph_status fjsph_set_time(fjsph_ctx* ctx, double time, double delta_t, double cfl);
+
+/* ---- aerodynamic mesh (aero_source = FJSPH_MESH_INFL) --------------------- */
+
+/* The MESH of src/Var.h:396-451 as flat arrays (host order, 0-based):
+ *   verts[nverts*dim];
+ *   face_off[nfaces+1], face_verts[face_off[nfaces]]: the vertices of each
+ *     face (3D: polygons, tested as src/Geometry.cpp:485-575 does, through
+ *     vertices 0, 1, 2 and the last; 2D: edges of two vertices);
+ *   face_lr[2*nfaces]: left cell, right cell or a negative boundary marker
+ *     (leftright, -1 inner wall, -2 outer boundary);
+ *   cell_off[ncells+1], cell_faces[cell_off[ncells]]: the faces of each cell;
+ *   vert_vel[nverts*dim], vert_p[nverts], vert_rho[nverts]: the gas solution
+ *     at the vertices, averaged to the cells as Average_Point_Data_to_Cell
+ *     (src/CDFIO.cpp:112-155) does.
+ * Replaces the mesh readers of src/FJSPH.cpp:66-90 and the cell KD-tree.
+ * The lookup (FindCell, src/Containment.cpp:706-820) runs on every
+ * get_aero_velocity: FREE particles with lam_nb <= lam_cutoff get the
+ * values of the cell that contains them (the previous cell first, then the
+ * cells near their position); particles found in no cell keep their values
+ * and are counted (deletion is not built). */
+fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts, int64_t nfaces,
+                            const int64_t* face_off, const int64_t* face_verts, const int64_t* face_lr,
+                            int64_t ncells, const int64_t* cell_off, const int64_t* cell_faces,
+                            const double* vert_vel, const double* vert_p, const double* vert_rho);
+/* Cell centres and values as averaged: centre, vel (ncells*dim), p, rho (ncells). */
+fjsph_status fjsph_get_mesh_cells(fjsph_ctx* ctx, double* centre, double* vel, double* p, double* rho);
+/* Aero particles found in no cell by the last lookup. */
+fjsph_status fjsph_mesh_lost(fjsph_ctx* ctx, int64_t* lost);
 
 /* ---- measurement ---------------------------------------------------------- */
 

@@ -46,6 +46,7 @@
 #include <string>
 #include <utility>
 #include <vector>
+#include <set>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -695,6 +696,156 @@ struct Block {
 
 /* --------------------------------------------------------------- the sim */
 
+/* ----------------------------------------- aerodynamic mesh (row a14)
+ * MESH (src/Var.h:396-451) as flat arrays: faces are vertex lists (CSR),
+ * leftright[f] = (left cell, right cell or a negative boundary marker),
+ * cFaces[c] the faces of cell c (CSR); cell centre, velocity, pressure and
+ * density from Average_Point_Data_to_Cell. */
+struct MeshData {
+    int dim = 3;
+    std::vector<double> verts;                 /* nverts * dim */
+    std::vector<int64_t> face_off, face_v;     /* CSR faces -> vertices */
+    std::vector<int64_t> face_l, face_r;       /* leftright */
+    std::vector<int64_t> cell_off, cell_f;     /* CSR cells -> faces */
+    std::vector<double> cCentre, cVel, cP, cRho;
+    size_t ncells() const { return cell_off.empty() ? 0 : cell_off.size() - 1; }
+    const double* vert(int64_t v) const { return &verts[size_t(v) * dim]; }
+};
+
+/* Determinant of the 4x4 matrix with rows (p_k, 1), k = 0..3, by Laplace
+ * expansion along the first two columns (2x2 minors of columns 0-1 times
+ * complementary 2x2 minors of columns 2-3), the structure of Eigen 3.4's
+ * fixed-size 4x4 determinant.  The reference only uses its sign. */
+inline double det4_rows(const double* p0, const double* p1, const double* p2, const double* p3)
+{
+    const double* r[4] = {p0, p1, p2, p3};
+    auto m = [&](int i, int j) { return j < 3 ? r[i][j] : 1.0; };
+    auto d2 = [&](int i0, int i1) { return m(i0, 0) * m(i1, 1) - m(i1, 0) * m(i0, 1); };
+    const double d2_01 = d2(0, 1), d2_02 = d2(0, 2), d2_03 = d2(0, 3);
+    const double d2_12 = d2(1, 2), d2_13 = d2(1, 3), d2_23 = d2(2, 3);
+    auto d3 = [&](int i0, double a0, int i1, double a1, int i2, double a2) {
+        return m(i0, 2) * a0 + (-m(i1, 2) * a1 + m(i2, 2) * a2);
+    };
+    const double d3_0 = d3(1, d2_23, 2, d2_13, 3, d2_12);
+    const double d3_1 = d3(0, d2_23, 2, d2_03, 3, d2_02);
+    const double d3_2 = d3(0, d2_13, 1, d2_03, 3, d2_01);
+    const double d3_3 = d3(0, d2_12, 1, d2_02, 2, d2_01);
+    return (-m(0, 3) * d3_0 + m(1, 3) * d3_1) + (-m(2, 3) * d3_2 + m(3, 3) * d3_3);
+}
+
+/* Crossings3D (src/Geometry.cpp:485-575): does the segment testp-rayp cross
+ * the face?  Signed volumes of tetrahedra; uses face vertices 0, 1, 2 for the
+ * plane and the edges (last, 0), (0, 1), (1, 2). */
+inline int crossings3d(const MeshData& M, int64_t f, const double* testp, const double* rayp)
+{
+    const int64_t* fv = &M.face_v[size_t(M.face_off[size_t(f)])];
+    const int64_t nv = M.face_off[size_t(f) + 1] - M.face_off[size_t(f)];
+    const int flag1 = det4_rows(testp, M.vert(fv[0]), M.vert(fv[1]), M.vert(fv[2])) < 0.0;
+    const int flag2 = det4_rows(rayp, M.vert(fv[0]), M.vert(fv[1]), M.vert(fv[2])) < 0.0;
+    if (flag1 == flag2) return 0;
+    const double* vtx0 = M.vert(fv[nv - 1]);
+    const double* vtx1 = M.vert(fv[0]);
+    const int flag3 = det4_rows(testp, vtx0, vtx1, rayp) < 0.0;
+    for (int ii = 1; ii < 3; ++ii) {
+        vtx0 = vtx1;
+        vtx1 = M.vert(fv[ii]);
+        const int flag4 = det4_rows(testp, vtx0, vtx1, rayp) < 0.0;
+        if (flag4 != flag3) return 0;
+    }
+    return 1;
+}
+
+/* Crossings2D (src/Geometry.cpp:354-395): does the +x ray from the point
+ * cross the edge? */
+inline int crossings2d(const MeshData& M, int64_t f, const double* point)
+{
+    const int64_t* fv = &M.face_v[size_t(M.face_off[size_t(f)])];
+    const double tx = point[0], ty = point[1];
+    const double* vtx0 = M.vert(fv[0]);
+    const double* vtx1 = M.vert(fv[1]);
+    const int yflag0 = vtx0[1] >= ty, yflag1 = vtx1[1] >= ty;
+    if (yflag0 != yflag1)
+        if (((vtx1[1] - ty) * (vtx1[0] - vtx0[0]) >= (vtx1[0] - tx) * (vtx1[1] - vtx0[1])) == yflag1) return 1;
+    return 0;
+}
+
+/* CheckCell (src/Containment.cpp:385-421): parity of the face crossings of
+ * the +x ray (3D: to testp + 1e5 x), stopping after the second hit. */
+inline int check_cell(const MeshData& M, int64_t cell, const double* testp)
+{
+    double rayp[3] = {testp[0] + 1e+5, testp[1], M.dim == 3 ? testp[2] : 0.0};
+    int line_flag = 0, inside_flag = 0;
+    for (int64_t k = M.cell_off[size_t(cell)]; k < M.cell_off[size_t(cell) + 1]; ++k) {
+        const int64_t f = M.cell_f[size_t(k)];
+        const int hit = M.dim == 3 ? crossings3d(M, f, testp, rayp) : crossings2d(M, f, testp);
+        if (hit) {
+            inside_flag = !inside_flag;
+            if (line_flag) break;
+            line_flag = 1;
+        }
+    }
+    return inside_flag;
+}
+
+/* Average_Point_Data_to_Cell (src/CDFIO.cpp:112-155): over the cell's unique
+ * vertices (ascending index), compensated (Kahan) sums divided by their
+ * count, for the centre, velocity, pressure and density. */
+inline void average_point_data_to_cell(MeshData& M, const double* vvel, const double* vp, const double* vrho)
+{
+    const size_t nc = M.ncells();
+    const int D = M.dim;
+    M.cCentre.assign(nc * D, 0.0);
+    M.cVel.assign(nc * D, 0.0);
+    M.cP.assign(nc, 0.0);
+    M.cRho.assign(nc, 0.0);
+    auto kahan = [](const std::set<int64_t>& idx, auto&& val) {
+        double sum = 0.0, c = 0.0;
+        for (int64_t ii : idx) {
+            const double y = val(ii) - c;
+            const double t = sum + y;
+            c = (t - sum) - y;
+            sum = t;
+        }
+        return sum;
+    };
+    for (size_t ii = 0; ii < nc; ++ii) {
+        std::set<int64_t> elem;
+        for (int64_t k = M.cell_off[ii]; k < M.cell_off[ii + 1]; ++k) {
+            const int64_t f = M.cell_f[size_t(k)];
+            for (int64_t q = M.face_off[size_t(f)]; q < M.face_off[size_t(f) + 1]; ++q) elem.insert(M.face_v[size_t(q)]);
+        }
+        const double nv = double(elem.size());
+        for (int a = 0; a < D; ++a) {
+            M.cCentre[ii * D + a] = kahan(elem, [&](int64_t v) { return M.verts[size_t(v) * D + a]; }) / nv;
+            M.cVel[ii * D + a] = kahan(elem, [&](int64_t v) { return vvel[size_t(v) * D + a]; }) / nv;
+        }
+        M.cP[ii] = kahan(elem, [&](int64_t v) { return vp[size_t(v)]; }) / nv;
+        M.cRho[ii] = kahan(elem, [&](int64_t v) { return vrho[size_t(v)]; }) / nv;
+    }
+}
+
+/* The k nearest cell centres of a point, nearest first (the result order
+ * of nanoflann's KNNResultSet; equal distances by cell index), by brute
+ * force. */
+inline std::vector<int64_t> knn_cells(const MeshData& M, const double* p, size_t k)
+{
+    const size_t nc = M.ncells();
+    std::vector<std::pair<double, int64_t>> dd(nc);
+    for (size_t c = 0; c < nc; ++c) {
+        double d2 = 0.0;
+        for (int a = 0; a < M.dim; ++a) {
+            const double t = M.cCentre[c * M.dim + a] - p[a];
+            d2 += t * t;
+        }
+        dd[c] = {d2, int64_t(c)};
+    }
+    k = std::min(k, nc);
+    std::partial_sort(dd.begin(), dd.begin() + long(k), dd.end());
+    std::vector<int64_t> out(k);
+    for (size_t i = 0; i < k; ++i) out[i] = dd[i].second;
+    return out;
+}
+
 template <int D>
 struct Sim {
     fjsph_settings s;
@@ -717,6 +868,51 @@ struct Sim {
     int rollbacks = 0;
     double last_npd = 1.0;
     std::vector<Vec<D>> xih;
+    MeshData mesh;
+    bool have_mesh = false;
+    int64_t mesh_lost = 0; /* aero particles in no cell at the last FindCell */
+
+    /* FindCell (src/Containment.cpp:706-820) for the aero particles of pnp1
+     * (FREE with lam_nb <= lam_cutoff; the others get no cell and keep their
+     * cell values): the previous cell, then the 5 and the 500 nearest cell
+     * centres.  A particle whose previous cell is unset (negative) skips the
+     * first test.  Particles in no cell keep their values and are counted
+     * (deletion at outer boundaries is not built). */
+    void find_cell(size_t start, size_t end)
+    {
+        int64_t lost = 0;
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : lost)
+        for (long ii = long(start); ii < long(end); ++ii) {
+            Part<D>& q = pnp1[ii];
+            if (q.b != FJSPH_FREE || q.lam_nb > s.lam_cutoff) {
+                q.cellID = kNoCell;
+                continue;
+            }
+            const double* testp = q.xi.c;
+            auto take = [&](int64_t cell) {
+                q.cellID = long(cell);
+                for (int a = 0; a < D; ++a) q.cellV.c[a] = mesh.cVel[size_t(cell) * D + a];
+                q.cellP = mesh.cP[size_t(cell)];
+                q.cellRho = mesh.cRho[size_t(cell)];
+            };
+            if (q.cellID >= 0 && size_t(q.cellID) < mesh.ncells() && check_cell(mesh, q.cellID, testp)) {
+                take(q.cellID);
+                continue;
+            }
+            bool found = false;
+            for (size_t k : {size_t(5), size_t(500)}) {
+                for (int64_t cell : knn_cells(mesh, testp, k))
+                    if (check_cell(mesh, cell, testp)) {
+                        take(cell);
+                        found = true;
+                        break;
+                    }
+                if (found) break;
+            }
+            if (!found) ++lost;
+        }
+        mesh_lost = lost;
+    }
 
     /* -------------------------------------------------------- neighbours */
     void update_neighbours() { outlist = find_neighbours<D>(pnp1, d.sr, nthreads); }
@@ -782,6 +978,10 @@ struct Sim {
      * src/Resid.cpp:569-611 */
     void get_aero_velocity(size_t start, size_t end)
     {
+        if (s.aero_source == FJSPH_MESH_INFL) {
+            if (have_mesh) find_cell(start, end);
+            return;
+        }
         if (s.aero_source != FJSPH_CONST_VEL) return;
         Vec<D> vinf;
         for (int k = 0; k < D; ++k) vinf.c[k] = s.v_inf[k];

@@ -180,6 +180,27 @@ int step(Handle* h, int kind, fjsph_step_info* info)
     return FJSPH_OK;
 }
 
+template <int D>
+int set_mesh_t(Handle* h, int64_t nverts, const double* verts, int64_t nfaces, const int64_t* face_off,
+               const int64_t* face_verts, const int64_t* face_lr, int64_t ncells, const int64_t* cell_off,
+               const int64_t* cell_faces, const double* vvel, const double* vp, const double* vrho)
+{
+    Sim<D>& S = sim<D>(h);
+    MeshData& M = S.mesh;
+    M.dim = D;
+    M.verts.assign(verts, verts + nverts * D);
+    M.face_off.assign(face_off, face_off + nfaces + 1);
+    M.face_v.assign(face_verts, face_verts + face_off[nfaces]);
+    M.face_l.resize(size_t(nfaces));
+    M.face_r.resize(size_t(nfaces));
+    for (int64_t f = 0; f < nfaces; ++f) { M.face_l[size_t(f)] = face_lr[2 * f]; M.face_r[size_t(f)] = face_lr[2 * f + 1]; }
+    M.cell_off.assign(cell_off, cell_off + ncells + 1);
+    M.cell_f.assign(cell_faces, cell_faces + cell_off[ncells]);
+    average_point_data_to_cell(M, vvel, vp, vrho);
+    S.have_mesh = true;
+    return FJSPH_OK;
+}
+
 }  // namespace
 
 #define DISPATCH(h, call) ((h)->dim == 2 ? call<2> : call<3>)
@@ -318,6 +339,43 @@ int orc_aero_velocity(void* hp)
     Handle* h = static_cast<Handle*>(hp);
     SIMCALL(h, S.get_aero_velocity(S.bound_points, S.total_points), S.get_aero_velocity(S.bound_points, S.total_points));
     return FJSPH_OK;
+}
+
+int orc_set_mesh(void* hp, int64_t nverts, const double* verts, int64_t nfaces, const int64_t* face_off,
+                 const int64_t* face_verts, const int64_t* face_lr, int64_t ncells, const int64_t* cell_off,
+                 const int64_t* cell_faces, const double* vvel, const double* vp, const double* vrho)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    return h->dim == 2 ? set_mesh_t<2>(h, nverts, verts, nfaces, face_off, face_verts, face_lr, ncells, cell_off,
+                                       cell_faces, vvel, vp, vrho)
+                       : set_mesh_t<3>(h, nverts, verts, nfaces, face_off, face_verts, face_lr, ncells, cell_off,
+                                       cell_faces, vvel, vp, vrho);
+}
+
+int orc_get_mesh_cells(void* hp, double* centre, double* vel, double* p, double* rho)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    const MeshData& M = h->dim == 2 ? h->s2->mesh : h->s3->mesh;
+    std::copy(M.cCentre.begin(), M.cCentre.end(), centre);
+    std::copy(M.cVel.begin(), M.cVel.end(), vel);
+    std::copy(M.cP.begin(), M.cP.end(), p);
+    std::copy(M.cRho.begin(), M.cRho.end(), rho);
+    return FJSPH_OK;
+}
+
+int orc_mesh_lost(void* hp, int64_t* lost)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    *lost = h->dim == 2 ? h->s2->mesh_lost : h->s3->mesh_lost;
+    return FJSPH_OK;
+}
+
+/* CheckCell of one point against one cell (for pinning tests). */
+int orc_check_cell(void* hp, int64_t cell, const double* point)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    const MeshData& M = h->dim == 2 ? h->s2->mesh : h->s3->mesh;
+    return check_cell(M, cell, point);
 }
 
 int orc_detect_surface(void* hp)

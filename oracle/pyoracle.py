@@ -63,6 +63,11 @@ def load(kind="oracle"):
         "orc_min_eig": (C.c_double, [C.c_int, _D]),
         "orc_kernel": (C.c_double, [C.c_double, C.c_double, C.c_double]),
         "orc_gradk_scalar": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+        "orc_set_mesh": (C.c_int, [vp, C.c_int64, _D, C.c_int64, _I64, _I64, _I64, C.c_int64, _I64, _I64,
+                                   _D, _D, _D]),
+        "orc_get_mesh_cells": (C.c_int, [vp, _D, _D, _D, _D]),
+        "orc_mesh_lost": (C.c_int, [vp, _I64]),
+        "orc_check_cell": (C.c_int, [vp, C.c_int64, _D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -127,7 +132,36 @@ class Oracle:
     def upload_case(self, case):
         self.upload_state(case.x, case.v, case.rho, case.p, case.m, case.b, case.n_bound)
         self.set_blocks(case.blocks)
+        if getattr(case, "mesh", None) is not None:
+            self.set_mesh(case.mesh)
 
+    def set_mesh(self, mesh):
+        self._mesh = mesh
+        a = lambda x, t=np.float64: np.ascontiguousarray(x, dtype=t)
+        self._mesh_arrays = [a(mesh.verts), a(mesh.face_off, np.int64), a(mesh.face_verts, np.int64),
+                             a(mesh.face_lr, np.int64), a(mesh.cell_off, np.int64), a(mesh.cell_faces, np.int64),
+                             a(mesh.vert_vel), a(mesh.vert_p), a(mesh.vert_rho)]
+        v, fo, fv, lr, co, cf, vv, vp, vr = self._mesh_arrays
+        self._chk(self.lib.orc_set_mesh(self.h, mesh.nverts, _ptr(v), mesh.nfaces, _ptr(fo, _I64), _ptr(fv, _I64),
+                                        _ptr(lr, _I64), mesh.ncells, _ptr(co, _I64), _ptr(cf, _I64), _ptr(vv),
+                                        _ptr(vp), _ptr(vr)), "set_mesh")
+
+    def mesh_cells(self):
+        nc = self._mesh.ncells
+        c, v = np.zeros((nc, self.dim)), np.zeros((nc, self.dim))
+        p, r = np.zeros(nc), np.zeros(nc)
+        self._chk(self.lib.orc_get_mesh_cells(self.h, _ptr(c), _ptr(v), _ptr(p), _ptr(r)), "get_mesh_cells")
+        return c, v, p, r
+
+    def mesh_lost(self):
+        v = np.zeros(1, np.int64)
+        self._chk(self.lib.orc_mesh_lost(self.h, _ptr(v, _I64)), "mesh_lost")
+        return int(v[0])
+
+    def check_cell(self, cell, point):
+        p = np.zeros(3)
+        p[:len(point)] = point
+        return int(self.lib.orc_check_cell(self.h, int(cell), _ptr(p)))
     def upload_state(self, x, v, rho, p, m, b, n_bound):
         x = np.ascontiguousarray(x, np.float64)
         v = np.ascontiguousarray(v, np.float64)

@@ -132,6 +132,56 @@ def test_step_parity_fp64(name, case):
         assert_close(f, eng.field(f), orc.field(f), FP64_STEP)
 
 
+def _rk(case):
+    case.settings.integrator = abi.RUNGE_KUTTA
+    return case
+
+
+@pytest.mark.parametrize("name,case", [(n, _rk(c)) for n, c in _small_cases() if n != "drop3d"],
+                         ids=lambda v: v if isinstance(v, str) else "")
+def test_step_parity_rk4_fp64(name, case):
+    """Runge-Kutta 4 (src/Runge_Kutta.cpp) on the GPU against the oracle."""
+    print(f"[rk4 step parity] {name}: n={case.n}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    ie, io = eng.integrate_no_update(), orc.integrate_no_update()
+    assert abs(ie["rms_error"] - io["rms_error"]) <= 1e-3
+    for step in range(3):
+        ie, io = eng.integrate_step(), orc.integrate_step()
+        assert abs(ie["dt"] - io["dt"]) <= 1e-10 * io["dt"]
+        assert abs(ie["rms_error"] - io["rms_error"]) <= 1e-3
+    for f in ("X", "V", "RHO", "P", "ACC", "RRHO"):
+        assert_close(f, eng.field(f), orc.field(f), FP64_STEP)
+
+
+def test_mesh_aero_fp64():
+    """Mesh aero source (FindCell + Gissler) on a 3D droplet in a Delaunay
+    mesh: cell ids, cell values and two full steps against the oracle."""
+    case = cases.with_mesh(_perturb(cases.droplet_3d(diameter=1.0e-3, dx=5e-5, precision=abi.PREC_FP64), seed=6),
+                           n_points=20000)
+    print(f"[mesh aero] droplet n={case.n}, mesh cells={case.mesh.ncells}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    ce, co = eng.mesh_cells(), orc.mesh_cells()
+    for a, b in zip(ce, co):
+        assert np.array_equal(a, b)
+    for o in (eng, orc):
+        o.build_neighbours()
+        o.dsph_prestep()
+        o.aero_velocity()
+    assert eng.mesh_lost() == 0 and orc.mesh_lost() == 0
+    cid = eng.field("CELLID")
+    assert np.array_equal(cid, orc.field("CELLID"))
+    assert (cid >= 0).sum() > 100
+    for f in ("CELLV", "CELLP", "CELLRHO"):
+        assert np.array_equal(eng.field(f), orc.field(f)), f
+    eng.integrate_no_update()
+    orc.integrate_no_update()
+    for _ in range(2):
+        ie, io = eng.integrate_step(), orc.integrate_step()
+        assert ie["iterations"] == io["iterations"]
+    for f in ("X", "V", "RHO", "ACC", "AF"):
+        assert_close(f, eng.field(f), orc.field(f), FP64_STEP)
+
+
 def test_step_parity_mixed_droplet():
     case = _perturb(cases.droplet_3d(diameter=1.0e-3, dx=5e-5, precision=abi.PREC_MIXED), seed=4)
     print(f"[mixed step] droplet n={case.n}", flush=True)

@@ -337,6 +337,38 @@ def test_free_particle_parabola(po):
     assert np.allclose(o.field("V"), v1 + g * tau, rtol=0, atol=1e-13)
 
 
+def test_free_particle_rk4(po):
+    """Runge-Kutta 4 (src/Runge_Kutta.cpp:28-515) is exact for constant
+    acceleration once the previous step's acceleration is g; on the first
+    step the final combination uses acc = 0 from the initial state (the
+    reference combines a_n + 2k1 + 2k2 + k3), so v1 = v0 + 5/6 g dt."""
+    s = abi.default_settings(3)
+    s.particle_step = 1e-3
+    s.H_fac = 1.3
+    s.mu = 0.0
+    s.integrator = abi.RUNGE_KUTTA
+    o = po.Oracle(s, nthreads=1)
+    x0 = np.array([[0.1, 0.2, 0.3]])
+    v0 = np.array([[1.0, 0.5, 2.0]])
+    o.upload_state(x0, v0, np.array([1000.0]), np.array([0.0]), np.array([1e-6]),
+                   np.array([abi.FREE], np.int32), 0)
+    o.set_blocks([dict(first=0, last=1, is_fluid=1)])
+    o.integrate_no_update()
+    g = np.array(s.grav[:3])
+    o.integrate_step()
+    t1, dt, _ = o.get_time()
+    x1 = x0 + dt * v0 + 0.5 * dt * dt * g
+    v1 = v0 + dt * g * 5.0 / 6.0
+    assert np.allclose(o.field("X"), x1, rtol=0, atol=1e-14)
+    assert np.allclose(o.field("V"), v1, rtol=0, atol=1e-14)
+    for _ in range(5):
+        o.integrate_step()
+    t, _, _ = o.get_time()
+    tau = t - t1
+    assert np.allclose(o.field("X"), x1 + v1 * tau + 0.5 * g * tau * tau, rtol=0, atol=1e-13)
+    assert np.allclose(o.field("V"), v1 + g * tau, rtol=0, atol=1e-13)
+
+
 def test_find_timestep_closed_form(po):
     c = cases.fluid_block(2, 6, H_fac=1.3, sig=0.05, mu=2e-3)
     o = po.Oracle(c, nthreads=1)
@@ -482,3 +514,100 @@ def test_gissler_isolated_droplet_drag(po, re_target):
     assert cd == pytest.approx(_sphere_cd_standard(re_target), rel=0.12)
     assert o.field("ACC")[0][0] == pytest.approx(a[0], rel=1e-14)
     del d
+
+
+# ------------------------------------------------------- aerodynamic mesh
+
+def _mesh_case(dim, n_part=300, n_mesh=3000, seed=21):
+    """Isolated FREE particles (no neighbours, so lambda_nb = 0 and all get
+    a cell) scattered in a Delaunay box mesh."""
+    s = abi.default_settings(dim)
+    s.particle_step = 1e-4
+    s.H_fac = 1.3
+    s.aero_case = abi.GISSLER
+    cases._gas(s, 20.0, 1.2)
+    rng = np.random.default_rng(seed)
+    lo, hi = np.zeros(dim), np.ones(dim) * 1e-2
+    x = rng.uniform(lo + 1e-4, hi - 1e-4, size=(n_part, dim))
+    n = x.shape[0]
+    c = cases._finish(f"mesh_{dim}d", s, np.zeros((0, dim)), x, 0.0, "isolated particles in a mesh")
+    c.mesh = cases.delaunay_mesh(lo, hi, n_mesh, seed=seed, speed=20.0, rho_g=1.2, p_inf=s.p_ref)
+    s.aero_source = abi.MESH_INFL
+    assert n == c.n
+    return c
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mesh_find_cell_matches_scipy(po, dim):
+    """FindCell (src/Containment.cpp:706-820, ray-parity CheckCell) returns
+    the simplex scipy's Delaunay.find_simplex locates, for every particle."""
+    from scipy.spatial import Delaunay
+
+    c = _mesh_case(dim)
+    o = po.Oracle(c, nthreads=4)
+    o.build_neighbours()
+    o.dsph_prestep()
+    o.aero_velocity()
+    cell = o.field("CELLID").astype(np.int64)
+    ref = Delaunay(c.mesh.verts).find_simplex(c.x)
+    # the mesh's cells are Delaunay(c.mesh.verts)'s simplices only if qhull
+    # reproduces the same triangulation; compare through the vertex sets
+    got_sets = [frozenset(c.mesh.simplices[k]) for k in cell]
+    tri = Delaunay(c.mesh.verts)
+    ref_sets = [frozenset(tri.simplices[k]) for k in ref]
+    assert np.all(cell >= 0) and o.mesh_lost() == 0
+    assert got_sets == ref_sets
+    _, cvel, cp, crho = o.mesh_cells()
+    assert np.array_equal(o.field("CELLV"), cvel[cell])
+    assert np.array_equal(o.field("CELLP"), cp[cell])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mesh_cell_average_of_linear_field(po, dim):
+    """Average_Point_Data_to_Cell (src/CDFIO.cpp:112-155): the vertex mean of
+    a linear field is the field at the vertex centroid; the centre is the
+    mean of the cell's vertices."""
+    c = _mesh_case(dim, n_part=10, n_mesh=400)
+    m = c.mesh
+    a = np.arange(1, dim + 1) * 3.0
+    m.vert_p = 5.0 + m.verts @ a
+    m.vert_rho = 1.0 + 2.0 * m.verts[:, 0]
+    m.vert_vel = np.stack([m.verts[:, 0] * 7.0 - 1.0] * dim, axis=1)
+    o = po.Oracle(c, nthreads=1)
+    centre, vel, p, rho = o.mesh_cells()
+    simp = m.simplices
+    assert np.allclose(centre, m.verts[simp].mean(axis=1), rtol=0, atol=1e-15)
+    assert np.allclose(p, 5.0 + centre @ a, rtol=1e-13, atol=0)
+    assert np.allclose(rho, 1.0 + 2.0 * centre[:, 0], rtol=1e-13, atol=0)
+    assert np.allclose(vel[:, 0], centre[:, 0] * 7.0 - 1.0, rtol=0, atol=1e-13)
+
+
+def test_check_cell_matches_barycentric(po):
+    """CheckCell's ray parity agrees with barycentric containment for random
+    points against random tetrahedra of the mesh."""
+    c = _mesh_case(3, n_part=5, n_mesh=500)
+    m = c.mesh
+    o = po.Oracle(c, nthreads=1)
+    rng = np.random.default_rng(3)
+    agree = 0
+    for _ in range(400):
+        k = int(rng.integers(m.ncells))
+        T = m.verts[m.simplices[k]]
+        # half the points inside (convex combinations), half near the cell
+        if rng.random() < 0.5:
+            w = rng.dirichlet(np.ones(4))
+        else:
+            w = rng.dirichlet(np.ones(4)) * 1.6 - 0.15
+        pt = w @ T
+        A = np.vstack([(T[1:] - T[0]).T])
+        lam = np.linalg.solve(A, pt - T[0])
+        inside = lam.min() > 1e-9 and lam.sum() < 1 - 1e-9
+        outside = lam.min() < -1e-9 or lam.sum() > 1 + 1e-9
+        got = o.check_cell(k, pt)
+        if inside:
+            assert got == 1
+            agree += 1
+        elif outside:
+            assert got == 0
+            agree += 1
+    assert agree > 350
