@@ -122,6 +122,10 @@ struct fjsph_ctx {
     std::vector<double> mcentre, mvel, mp, mrho;
     DevMesh dm{};
     std::vector<void*> mesh_bufs;
+    /* surrogate */
+    bool have_surrogate = false;
+    Surrogate sg{};
+    std::vector<void*> sg_bufs;
     double *x_nxt = nullptr, *v_nxt = nullptr, *rho_nxt = nullptr, *p_nxt = nullptr;
     int32_t *part_id = nullptr, *blk = nullptr, *near_inlet = nullptr;
     double* norms_tmp = nullptr;
@@ -240,6 +244,9 @@ void free_all(fjsph_ctx* ctx)
     for (void* p : ctx->mesh_bufs) (void)hipFree(p);
     ctx->mesh_bufs.clear();
     ctx->have_mesh = false;
+    for (void* p : ctx->sg_bufs) (void)hipFree(p);
+    ctx->sg_bufs.clear();
+    ctx->have_surrogate = false;
     dfree(ctx->x_nxt); dfree(ctx->v_nxt); dfree(ctx->rho_nxt); dfree(ctx->p_nxt);
     dfree(ctx->part_id); dfree(ctx->blk); dfree(ctx->near_inlet); dfree(ctx->norms_tmp);
     dfree(ctx->partial); dfree(ctx->gather_tmp); dfree(ctx->gather_tmp_i);
@@ -479,6 +486,13 @@ void mesh_lookup(fjsph_ctx* ctx)
     launch_find_cell(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->dm, ctx->n, &ctx->sc->mesh_lost, ctx->st);
 }
 
+/* get_aero_velocity for the surrogate source: the MLP in place of FindCell. */
+void surrogate_lookup(fjsph_ctx* ctx)
+{
+    if (!ctx->have_surrogate) throw std::string("aero_source is the surrogate but none was set (fjsph_set_surrogate)");
+    launch_surrogate(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->sg, ctx->n, 0, nullptr, nullptr, ctx->st);
+}
+
 void dsph(fjsph_ctx* ctx, bool with_aero)
 {
     PassTimer t(ctx, FJSPH_T_DSPH);
@@ -486,6 +500,7 @@ void dsph(fjsph_ctx* ctx, bool with_aero)
     launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
                 with_aero ? 1 : 0, ctx->partial, ctx->sc, ctx->st);
     if (with_aero && ctx->s.aero_source == FJSPH_MESH_INFL) mesh_lookup(ctx);
+    if (with_aero && ctx->s.aero_source == FJSPH_SURROGATE) surrogate_lookup(ctx);
 }
 
 /* Detect_Surface loop 1 and dissipation_terms share one staged pass
@@ -865,7 +880,8 @@ fjsph_status fjsph_create(const fjsph_settings* s, int32_t device, fjsph_ctx** o
     if (fjsph_derive(s, &d) != FJSPH_OK) return FJSPH_ERR_ARG;
     if (s->precision != FJSPH_PREC_FP64 && s->precision != FJSPH_PREC_MIXED) return FJSPH_ERR_ARG;
     if (s->aero_case != FJSPH_NO_AERO && s->aero_case != FJSPH_GISSLER) return FJSPH_ERR_UNSUPPORTED;
-    if (s->aero_source != FJSPH_CONST_VEL && s->aero_source != FJSPH_MESH_INFL) return FJSPH_ERR_UNSUPPORTED;
+    if (s->aero_source != FJSPH_CONST_VEL && s->aero_source != FJSPH_MESH_INFL && s->aero_source != FJSPH_SURROGATE)
+        return FJSPH_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FJSPH_ERR_NO_DEVICE;
     if (device < 0 || device >= ndev) return FJSPH_ERR_ARG;
@@ -1187,6 +1203,7 @@ fjsph_status fjsph_aero_velocity(fjsph_ctx* ctx)
     return guarded(ctx, [&] {
         require(ctx, true);
         if (ctx->s.aero_source == FJSPH_MESH_INFL) mesh_lookup(ctx);
+        else if (ctx->s.aero_source == FJSPH_SURROGATE) surrogate_lookup(ctx);
         else launch_aero_velocity(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->st);
         sync(ctx);
     });
@@ -1352,6 +1369,92 @@ fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts,
         }
         ctx->mesh_ncells = ncells;
         ctx->have_mesh = true;
+    });
+}
+
+fjsph_status fjsph_set_surrogate(fjsph_ctx* ctx, int32_t n_hidden, const float* weights, const float* biases,
+                                 const double* in_shift, const double* in_scale, const double* out_shift,
+                                 const double* out_scale)
+{
+    if (!ctx || n_hidden < 1 || n_hidden > 8 || !weights || !biases || !in_shift || !in_scale || !out_shift ||
+        !out_scale)
+        return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        const int D = ctx->s.dim, NO = D + 2;
+        for (int k = 0; k < D; ++k)
+            if (!(in_scale[k] != 0.0)) throw std::string("set_surrogate: zero input scale");
+        /* FP32 -> BF16, round to nearest even */
+        auto bf16 = [](float f) {
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t(u >> 16);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            return uint16_t(u >> 16);
+        };
+        /* A-operand tiles of a layer: W[out][in] padded to MT*16 x KC*32;
+         * lane l of tile (mt, kc) holds W[mt*16 + l%16][kc*32 + 8*(l/16) + e] */
+        std::vector<uint16_t> tiles;
+        const float* wp = weights;
+        auto add_layer = [&](int nout, int nin, int MT, int KC) {
+            for (int mt = 0; mt < MT; ++mt)
+                for (int kc = 0; kc < KC; ++kc)
+                    for (int l = 0; l < 64; ++l)
+                        for (int e = 0; e < 8; ++e) {
+                            const int r = mt * 16 + l % 16, k = kc * 32 + 8 * (l / 16) + e;
+                            tiles.push_back(r < nout && k < nin ? bf16(wp[size_t(r) * nin + k]) : uint16_t(0));
+                        }
+            wp += size_t(nout) * nin;
+        };
+        add_layer(64, D, 4, 1);
+        for (int l = 1; l < n_hidden; ++l) add_layer(64, 64, 4, 2);
+        add_layer(NO, 64, 1, 2);
+        std::vector<float> bh(biases, biases + size_t(n_hidden) * 64);
+        std::vector<float> bo(16, 0.0f);
+        for (int k = 0; k < NO; ++k) bo[k] = biases[size_t(n_hidden) * 64 + k];
+        for (void* p : ctx->sg_bufs) (void)hipFree(p);
+        ctx->sg_bufs.clear();
+        auto up = [&](const void* src, size_t bytes) {
+            void* d = nullptr;
+            HIPCHK(hipMalloc(&d, bytes));
+            ctx->sg_bufs.push_back(d);
+            HIPCHK(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+            return d;
+        };
+        sync(ctx);
+        Surrogate& sg = ctx->sg;
+        sg.w = up(tiles.data(), tiles.size() * 2);
+        sg.b = (const float*)up(bh.data(), bh.size() * 4);
+        sg.bo = (const float*)up(bo.data(), bo.size() * 4);
+        sg.n_hidden = n_hidden;
+        for (int k = 0; k < 3; ++k) {
+            sg.in_shift[k] = k < D ? in_shift[k] : 0.0;
+            sg.in_scale_inv[k] = k < D ? 1.0 / in_scale[k] : 0.0;
+        }
+        for (int k = 0; k < 5; ++k) {
+            sg.out_shift[k] = k < NO ? out_shift[k] : 0.0;
+            sg.out_scale[k] = k < NO ? out_scale[k] : 0.0;
+        }
+        ctx->have_surrogate = true;
+    });
+}
+
+fjsph_status fjsph_surrogate_eval(fjsph_ctx* ctx, int64_t n, const double* points, double* out)
+{
+    if (!ctx || n < 0 || (n > 0 && (!points || !out))) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->have_surrogate) throw std::string("no surrogate set");
+        if (n == 0) return;
+        const int D = ctx->s.dim;
+        double *dp = nullptr, *dout = nullptr;
+        HIPCHK(hipMalloc((void**)&dp, sizeof(double) * n * D));
+        HIPCHK(hipMalloc((void**)&dout, sizeof(double) * n * (D + 2)));
+        sync(ctx);
+        HIPCHK(hipMemcpy(dp, points, sizeof(double) * n * D, hipMemcpyHostToDevice));
+        launch_surrogate(D, ctx->c, DevState{}, nullptr, ctx->sg, n, 1, dp, dout, ctx->st);
+        sync(ctx);
+        HIPCHK(hipMemcpy(out, dout, sizeof(double) * n * (D + 2), hipMemcpyDeviceToHost));
+        (void)hipFree(dp);
+        (void)hipFree(dout);
     });
 }
 

@@ -114,6 +114,17 @@ struct DevMesh {
     const int32_t *gstart, *gcells;
 };
 
+/* MLP surrogate of the mesh field (surrogate.hip): BF16 weights already in
+ * the MFMA A-operand layout, FP32 biases, input/output scaling. */
+struct Surrogate {
+    const void* w;   /* bf16 tiles: layer 1 (4 tiles), hidden layers (8 each), output (2) */
+    const float* b;  /* n_hidden x 64 */
+    const float* bo; /* 16 (D + 2 used) */
+    int32_t n_hidden;
+    double in_shift[3], in_scale_inv[3];
+    double out_shift[5], out_scale[5];
+};
+
 /* Scalars kept on the device between kernels. */
 struct DevScalars {
     double npd;       /* numerical particle density of the last dSPH pass */
@@ -166,6 +177,9 @@ void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
 void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double* next_x, double* next_v,
                       double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk, DevBlocks bl,
                       const int32_t* near_inlet, NbrList nl, int64_t n, double dt, double* partial, hipStream_t st);
+/* Surrogate on the aero particles of s (mode 0) or on points (mode 1). */
+void launch_surrogate(int dim, const Consts& c, DevState s, const int32_t* part_id, Surrogate sg, int64_t n, int mode,
+                      const double* pts, double* out, hipStream_t st);
 /* FindCell on the aero particles of s (mesh.hip). */
 void launch_find_cell(int dim, const Consts& c, DevState s, const int32_t* part_id, DevMesh m, int64_t n,
                       int32_t* lost, hipStream_t st);

@@ -1,0 +1,155 @@
+/*
+ * Neural surrogate of the aerodynamic mesh field (aero_source =
+ * FJSPH_SURROGATE; it has no counterpart in the reference): an MLP
+ *     h_1 = gelu(W_1 xn + b_1), h_k = gelu(W_k h_{k-1} + b_k), k = 2..L,
+ *     y = W_o h_L + b_o,
+ * width 64, GELU in its erf form, xn = (x - in_shift) / in_scale, and the
+ * outputs (gas velocity, pressure, density) = y * out_scale + out_shift.
+ *
+ * Each wave evaluates 64 points.  A layer is a 64 x 64 (features x points)
+ * product formed from 16 x 16 output tiles with v_mfma_f32_16x16x32_bf16:
+ * BF16 operands, FP32 accumulation.  Operand layout of that instruction
+ * (lane l = 0..63):
+ *     A[i][k]  (16 x 32): lane i + 16*(k/8), element k%8
+ *     B[k][j]  (32 x 16): lane j + 16*(k/8), element k%8
+ *     D[i][j]  (16 x 16): lane j + 16*(i/4), element i%4
+ * The weights are stored on the host already in the A layout (8 BF16 per
+ * lane per tile, one 16-byte load); activations sit in LDS as BF16 rows per
+ * point, so a lane reads its 8 B-operand values with one 16-byte LDS read.
+ */
+#include "device_common.hpp"
+
+namespace fj {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int SW = 64;       /* hidden width */
+constexpr int SROW = SW + 8; /* LDS row of one point, padded against bank conflicts */
+
+__device__ __forceinline__ float gelu_erf(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+
+/* layer input: KC chunks of 32 features from the LDS rows; output: MT tiles
+ * of 16 features (registers acc[MT][4 column tiles]) */
+template <int MT, int KC>
+__device__ __forceinline__ void mlp_layer(const bf16x8* __restrict__ wA, const __bf16* act, f32x4 (&acc)[4][4],
+                                          int lane)
+{
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < KC; ++kc) {
+        bf16x8 b[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+            b[nt] = *(const bf16x8*)(act + (nt * 16 + (lane & 15)) * SROW + kc * 32 + 8 * (lane >> 4));
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const bf16x8 a = wA[(mt * KC + kc) * 64 + lane];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
+        }
+    }
+}
+
+/* bias + GELU, back to the LDS rows (after every lane has read them) */
+__device__ __forceinline__ void store_hidden(const f32x4 (&acc)[4][4], const float* __restrict__ bias, __bf16* act,
+                                             int lane)
+{
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int f = mt * 16 + 4 * (lane >> 4) + e;
+                const int pt = nt * 16 + (lane & 15);
+                act[pt * SROW + f] = __bf16(gelu_erf(acc[mt][nt][e] + bias[f]));
+            }
+    __builtin_amdgcn_wave_barrier();
+}
+
+/* mode 0: the aero particles of state s (FREE with lam_nb <= lam_cutoff;
+ * the others get no cell, as FindCell does); mode 1: the points pts[n*D],
+ * outputs out[n*(D+2)]. */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_surrogate(Consts c, DevState s, const int32_t* __restrict__ part_id,
+                                                   Surrogate sg, int64_t n, int mode, const double* __restrict__ pts,
+                                                   double* __restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 lds_act[NWAVE][64 * SROW];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __bf16* act = lds_act[wv];
+    const int64_t base = (blockIdx.x * (int64_t)TPB) + wv * 64;
+    const int64_t i = base + lane;
+    bool use = i < n;
+    double x[3] = {0.0, 0.0, 0.0};
+    if (use && mode == 0) {
+        use = part_id[i] >= c.n_bound;
+        if (use && (s.b[i] != FJSPH_FREE || s.lam_nb[i] > c.lam_cutoff)) {
+            s.cellID[i] = -3;
+            use = false;
+        }
+    }
+    if (use) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) x[a] = mode == 0 ? s.x[i * D + a] : pts[i * D + a];
+    }
+    /* input row: normalised position, zero-padded to 32 features */
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        float v = 0.f;
+        if (k < D) v = float((x[k] - sg.in_shift[k]) * sg.in_scale_inv[k]);
+        act[lane * SROW + k] = __bf16(v);
+    }
+    __builtin_amdgcn_wave_barrier();
+    f32x4 acc[4][4];
+    const bf16x8* W = (const bf16x8*)sg.w;
+    mlp_layer<4, 1>(W, act, acc, lane);
+    store_hidden(acc, sg.b, act, lane);
+    for (int l = 1; l < sg.n_hidden; ++l) {
+        mlp_layer<4, 2>(W + (size_t)(4 * 1 + (l - 1) * 8) * 64, act, acc, lane);
+        store_hidden(acc, sg.b + l * SW, act, lane);
+    }
+    mlp_layer<1, 2>(W + (size_t)(4 * 1 + (sg.n_hidden - 1) * 8) * 64, act, acc, lane);
+    /* output tile: feature 4*(lane/16) + e of point 16*nt + lane%16; gather
+     * the point's D+2 outputs through LDS */
+    __builtin_amdgcn_wave_barrier();
+    float* res = (float*)act; /* 64 points x 8 floats, reuses the rows */
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int f = 4 * (lane >> 4) + e;
+            if (f < 8) res[(nt * 16 + (lane & 15)) * 8 + f] = acc[0][nt][e];
+        }
+    __builtin_amdgcn_wave_barrier();
+    if (!use) return;
+    double y[D + 2];
+#pragma unroll
+    for (int k = 0; k < D + 2; ++k) y[k] = double(res[lane * 8 + k] + sg.bo[k]) * sg.out_scale[k] + sg.out_shift[k];
+    if (mode == 1) {
+#pragma unroll
+        for (int k = 0; k < D + 2; ++k) out[i * (D + 2) + k] = y[k];
+        return;
+    }
+    s.cellID[i] = 1;
+#pragma unroll
+    for (int a = 0; a < D; ++a) s.cellV[i * D + a] = y[a];
+    s.cellP[i] = y[D];
+    s.cellRho[i] = y[D + 1];
+}
+
+void launch_surrogate(int dim, const Consts& c, DevState s, const int32_t* part_id, Surrogate sg, int64_t n, int mode,
+                      const double* pts, double* out, hipStream_t st)
+{
+    const dim3 g(grid_of(n)), b(TPB);
+    if (dim == 2) hipLaunchKernelGGL(k_surrogate<2>, g, b, 0, st, c, s, part_id, sg, n, mode, pts, out);
+    else hipLaunchKernelGGL(k_surrogate<3>, g, b, 0, st, c, s, part_id, sg, n, mode, pts, out);
+}
+
+}  // namespace fj

@@ -21,6 +21,7 @@ _LIB = None
 _D = C.POINTER(C.c_double)
 _I32 = C.POINTER(C.c_int32)
 _I64 = C.POINTER(C.c_int64)
+_F32 = C.POINTER(C.c_float)
 _VP = C.c_void_p
 
 # name -> (restype, argtypes); every symbol declared in include/fjsph_engine.h
@@ -62,6 +63,8 @@ SIGNATURES = {
                                  _D, _D, _D]),
     "fjsph_get_mesh_cells": (C.c_int, [_VP, _D, _D, _D, _D]),
     "fjsph_mesh_lost": (C.c_int, [_VP, _I64]),
+    "fjsph_set_surrogate": (C.c_int, [_VP, C.c_int32, _F32, _F32, _D, _D, _D, _D]),
+    "fjsph_surrogate_eval": (C.c_int, [_VP, C.c_int64, _D, _D]),
 }
 
 
@@ -229,6 +232,25 @@ class Engine:
         v = np.zeros(1, np.int64)
         self._chk(self.lib.fjsph_mesh_lost(self.h, _ptr(v, _I64)), "mesh_lost")
         return int(v[0])
+
+    def set_surrogate(self, sg):
+        """fjsph_set_surrogate with a surrogate.Surrogate (or dict of the same fields)."""
+        g = (lambda k: getattr(sg, k)) if not isinstance(sg, dict) else sg.__getitem__
+        self._sg_arrays = [np.ascontiguousarray(g("weights"), dtype=np.float32),
+                           np.ascontiguousarray(g("biases"), dtype=np.float32),
+                           np.ascontiguousarray(g("in_shift"), dtype=np.float64),
+                           np.ascontiguousarray(g("in_scale"), dtype=np.float64),
+                           np.ascontiguousarray(g("out_shift"), dtype=np.float64),
+                           np.ascontiguousarray(g("out_scale"), dtype=np.float64)]
+        w, b, ish, isc, osh, osc = self._sg_arrays
+        self._chk(self.lib.fjsph_set_surrogate(self.h, int(g("n_hidden")), _ptr(w, _F32), _ptr(b, _F32), _ptr(ish),
+                                               _ptr(isc), _ptr(osh), _ptr(osc)), "set_surrogate")
+
+    def surrogate_eval(self, points):
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        out = np.zeros((pts.shape[0], self.dim + 2))
+        self._chk(self.lib.fjsph_surrogate_eval(self.h, pts.shape[0], _ptr(pts), _ptr(out)), "surrogate_eval")
+        return out
 
     def neighbour_stats(self):
         pairs = np.zeros(1, np.int64)

@@ -296,6 +296,22 @@ fjsph_status fjsph_get_mesh_cells(fjsph_ctx* ctx, double* centre, double* vel, d
 /* Aero particles found in no cell by the last lookup. */
 fjsph_status fjsph_mesh_lost(fjsph_ctx* ctx, int64_t* lost);
 
+/* ---- surrogate of the mesh field (aero_source = FJSPH_SURROGATE) ----------- */
+
+/* No counterpart in the reference: replaces the mesh lookup by an MLP of
+ * n_hidden (1..8) layers of width 64 with GELU (erf form), evaluated with
+ * BF16 MFMA and FP32 accumulation.  Input: the position normalised as
+ * (x - in_shift) / in_scale (dim values); output: (vel[dim], p, rho) =
+ * y * out_scale + out_shift (dim + 2 values).  weights: FP32, each layer
+ * [out][in] row-major, in order W_1 [64][dim], W_2 .. W_n [64][64],
+ * W_out [dim+2][64]; biases: b_1 .. b_n [64], b_out [dim+2].  Applied to the
+ * same particles as the mesh lookup (FREE, lam_nb <= lam_cutoff). */
+fjsph_status fjsph_set_surrogate(fjsph_ctx* ctx, int32_t n_hidden, const float* weights, const float* biases,
+                                 const double* in_shift, const double* in_scale, const double* out_shift,
+                                 const double* out_scale);
+/* Evaluate the surrogate at n host points (n*dim) into out (n*(dim+2)). */
+fjsph_status fjsph_surrogate_eval(fjsph_ctx* ctx, int64_t n, const double* points, double* out);
+
 /* ---- measurement ---------------------------------------------------------- */
 
 /* Pass ids of the engine's own timers (HIP events on the context stream).
