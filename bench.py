@@ -39,8 +39,7 @@ def workload(name, n_target):
     if name == "jet4m":
         c = cases.jet_column_3d(n_target=n_target, precision=abi.PREC_MIXED)
         desc = ("3D coflow jet column, D=2 mm liquid jet at 10 m/s in 30 m/s coflow, dx=0.04 mm, "
-                "H=2dx, ALE shifting on, Gissler aero from the constant coflow velocity "
-                "(mesh and surrogate aero sources not built yet)")
+                "H=2dx, ALE shifting on, Gissler drag")
         return c, desc
     if name == "droplet":
         return cases.droplet_3d(precision=abi.PREC_MIXED), "3D droplet D=4 mm, dx=0.05 mm, 20 m/s crossflow"
@@ -121,6 +120,41 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
 
 
+def run_engine(case, args, label):
+    """Warm up, then time `steps` steps; returns the measurements."""
+    from fjsph_amd.engine import Engine
+
+    print(f"[bench] {label}: {case.n} particles; building engine", file=sys.stderr, flush=True)
+    eng = Engine(case)
+    if getattr(case, "surrogate", None) is not None:
+        eng.set_surrogate(case.surrogate)
+    eng.integrate_no_update()
+    for _ in range(args.warmup):
+        eng.integrate_step()
+    eng.enable_timers(True)
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        its.append(eng.integrate_step()["iterations"])
+    elapsed = time.perf_counter() - t0
+    r = {"elapsed": elapsed, "value": case.n * args.steps / elapsed, "its": float(np.mean(its)),
+         "timers": eng.timers(), "stats": eng.neighbour_stats(),
+         "force_bpp": eng.pass_bytes(abi.TIMER_NAMES.index("force")),
+         "pass_bytes": {k: eng.pass_bytes(abi.TIMER_NAMES.index(k)) for k in abi.TIMER_NAMES}}
+    if case.settings.aero_source in (abi.MESH_INFL, abi.SURROGATE):
+        cid = eng.field("CELLID")
+        aero = cid >= 0
+        r["aero_particles"] = int(aero.sum())
+        if case.settings.aero_source == abi.MESH_INFL:
+            r["mesh_lost"] = eng.mesh_lost()
+            r["aero_x"] = eng.field("X")[aero]
+            r["aero_vals"] = np.concatenate([eng.field("CELLV")[aero], eng.field("CELLP")[aero, None],
+                                             eng.field("CELLRHO")[aero, None]], axis=1)
+    eng.close()
+    print(f"[bench] {label}: {r['value'] / 1e6:.2f}M particle-steps/s", file=sys.stderr, flush=True)
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +162,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="jet4m")
     ap.add_argument("--particles", type=int, default=4_000_000)
+    ap.add_argument("--aero", default="surrogate", choices=["surrogate", "mesh", "const"],
+                    help="gas velocity source: MLP surrogate of the mesh (timed, with the mesh lookup timed "
+                         "beside it), the mesh lookup alone, or the constant coflow velocity")
+    ap.add_argument("--mesh-points", type=int, default=300_000)
+    ap.add_argument("--train-samples", type=int, default=1_000_000)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-device", default=None, help="torch device of the surrogate training (default: GPU)")
     ap.add_argument("--cpu-particles", type=int, default=60_000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -142,33 +183,61 @@ def main():
         copy_probe(args.copy_probe)
         return
 
-    from fjsph_amd.engine import Engine
-
     case, desc = workload(args.workload, args.particles)
-    print(f"[bench] workload {case.name}: {case.n} particles; building engine", file=sys.stderr, flush=True)
-    eng = Engine(case)
-    eng.integrate_no_update()
-    for k in range(args.warmup):
-        eng.integrate_step()
-    print(f"[bench] warm-up done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    aero_info = None
+    if args.aero == "const":
+        desc += ", gas velocity: the constant coflow velocity"
+        run = run_engine(case, args, "constant-velocity aero")
+    else:
+        from fjsph_amd import surrogate
+        t0 = time.perf_counter()
+        cases.with_mesh(case, n_points=args.mesh_points, speed=float(case.settings.v_inf[0]))
+        t_mesh = time.perf_counter() - t0
+        print(f"[bench] mesh: {case.mesh.ncells} cells ({t_mesh:.0f} s)", file=sys.stderr, flush=True)
+        mesh_run = run_engine(case, args, "mesh aero")
+        aero_info = {"mesh_cells": int(case.mesh.ncells), "mesh_points": int(case.mesh.nverts),
+                     "mesh_aero_value": mesh_run["value"], "mesh_aero_ms_per_step": mesh_run["elapsed"] / args.steps * 1e3,
+                     "mesh_lost": mesh_run.get("mesh_lost"), "aero_particles": mesh_run.get("aero_particles")}
+        run = mesh_run
+        if args.aero == "surrogate":
+            t0 = time.perf_counter()
+            from fjsph_amd.engine import Engine
+            lab = Engine(case.settings)
+            lab.set_mesh(case.mesh)
+            sg = surrogate.train(case.mesh, n_samples=args.train_samples, steps=args.train_steps, engine=lab,
+                                 device=args.train_device)
+            lab.close()
+            aero_info["train_seconds"] = time.perf_counter() - t0
+            case.settings.aero_source = abi.SURROGATE
+            case.surrogate = sg
+            run = run_engine(case, args, "surrogate aero")
+            chk = Engine(case.settings)
+            chk.set_surrogate(sg)
+            pred = chk.surrogate_eval(mesh_run["aero_x"])
+            chk.close()
+            err = pred - mesh_run["aero_vals"]
+            dim = case.settings.dim
+            aero_info.update({
+                "surrogate_value": run["value"],
+                "surrogate_vs_mesh_speedup": run["value"] / mesh_run["value"],
+                "surrogate_rms_error": {"vel_m_s": float(np.sqrt(np.mean(np.sum(err[:, :dim] ** 2, axis=1)))),
+                                        "p_pa": float(np.sqrt(np.mean(err[:, dim] ** 2))),
+                                        "rho": float(np.sqrt(np.mean(err[:, dim + 1] ** 2)))},
+                "surrogate": f"MLP 3-64x{sg.n_hidden}-5, GELU, BF16 MFMA with FP32 accumulation, trained on "
+                             f"{args.train_samples} mesh-lookup samples",
+            })
+        desc += (f", gas velocity from {'the MLP surrogate of ' if args.aero == 'surrogate' else ''}a "
+                 f"{case.mesh.ncells}-cell Delaunay mesh (potential flow past a cylinder, "
+                 f"{case.settings.v_inf[0]:.0f} m/s)")
 
-    eng.enable_timers(True)
-    its = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        info = eng.integrate_step()
-        its.append(info["iterations"])
-    elapsed = time.perf_counter() - t0
-    timers = eng.timers()
-    pairs, max_count = eng.neighbour_stats()
-    value = case.n * args.steps / elapsed
-
+    timers = run["timers"]
+    pairs, max_count = run["stats"]
     passes = {}
     for name, (ms, launches) in timers.items():
         if launches == 0:
             continue
         avg_s = ms / 1e3 / launches
-        bpp = eng.pass_bytes(abi.TIMER_NAMES.index(name))
+        bpp = run["pass_bytes"][name]
         passes[name] = {
             "ms_per_step": round(ms / args.steps, 3),
             "ms_per_launch": round(avg_s * 1e3, 3),
@@ -180,7 +249,7 @@ def main():
     # roofline of the dominant kernel: k_force_t (HIP events on the engine stream)
     ms, launches = timers["force"]
     avg_s = (ms / 1e3) / max(launches, 1)
-    bpp = eng.pass_bytes(abi.TIMER_NAMES.index("force"))
+    bpp = run["force_bpp"]
     achieved = bpp * case.n / avg_s / 1e9
     traffic, tmeta = load_traffic(case.n)
     roofline = {
@@ -203,6 +272,7 @@ def main():
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args)
 
+    value = run["value"]
     out = {
         "metric": METRIC,
         "value": value,
@@ -210,7 +280,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": run["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -221,7 +291,8 @@ def main():
             "particles": case.n,
             "mean_neighbours": pairs / case.n + 1,
             "max_neighbours": max_count + 1,
-            "subiterations_per_step": float(np.mean(its)),
+            "subiterations_per_step": run["its"],
+            "aero": aero_info,
             "passes": passes,
         },
         "roofline": roofline,
@@ -230,7 +301,6 @@ def main():
     if cpu:
         out["config"]["speedup_vs_cpu"] = value / cpu["value"]
     print(json.dumps(out), flush=True)
-    eng.close()
 
 
 if __name__ == "__main__":

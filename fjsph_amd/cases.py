@@ -295,6 +295,7 @@ class Mesh:
     vert_p: np.ndarray
     vert_rho: np.ndarray
     simplices: np.ndarray = None
+    tri: object = None  # the scipy triangulation the cells come from (point location for sampling)
 
     @property
     def nverts(self):
@@ -374,7 +375,7 @@ def delaunay_mesh(lo, hi, n_points, seed=7, speed=20.0, rho_g=1.2, p_inf=101325.
         cyl_centre = (float(lo[0]) - 1.5 * cyl_radius, 0.5 * float(lo[1] + hi[1]))
     vel, p, rho = cylinder_flow(pts, speed, cyl_radius, cyl_centre, rho_g, p_inf)
     return Mesh(np.ascontiguousarray(pts), face_off, face_verts, np.ascontiguousarray(lr.reshape(-1)), cell_off,
-                cell_faces, np.ascontiguousarray(vel), p, rho, simplices=simp)
+                cell_faces, np.ascontiguousarray(vel), p, rho, simplices=simp, tri=tri)
 
 
 def with_mesh(case, n_points=20000, margin=4.0, seed=7, speed=None):

@@ -1470,6 +1470,30 @@ fjsph_status fjsph_get_mesh_cells(fjsph_ctx* ctx, double* centre, double* vel, d
     });
 }
 
+fjsph_status fjsph_mesh_eval(fjsph_ctx* ctx, int64_t n, const double* points, int32_t* cells, double* vals)
+{
+    if (!ctx || n < 0 || (n > 0 && (!points || !cells || !vals))) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->have_mesh) throw std::string("no mesh set");
+        if (n == 0) return;
+        const int D = ctx->s.dim;
+        double *dp = nullptr, *dv = nullptr;
+        int32_t* dc = nullptr;
+        HIPCHK(hipMalloc((void**)&dp, sizeof(double) * n * D));
+        HIPCHK(hipMalloc((void**)&dv, sizeof(double) * n * (D + 2)));
+        HIPCHK(hipMalloc((void**)&dc, sizeof(int32_t) * n));
+        sync(ctx);
+        HIPCHK(hipMemcpy(dp, points, sizeof(double) * n * D, hipMemcpyHostToDevice));
+        launch_mesh_eval(D, ctx->dm, n, dp, dc, dv, ctx->st);
+        sync(ctx);
+        HIPCHK(hipMemcpy(cells, dc, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(vals, dv, sizeof(double) * n * (D + 2), hipMemcpyDeviceToHost));
+        (void)hipFree(dp);
+        (void)hipFree(dv);
+        (void)hipFree(dc);
+    });
+}
+
 fjsph_status fjsph_mesh_lost(fjsph_ctx* ctx, int64_t* lost)
 {
     if (!ctx || !lost) return FJSPH_ERR_ARG;

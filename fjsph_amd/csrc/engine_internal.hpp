@@ -180,6 +180,8 @@ void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double*
 /* Surrogate on the aero particles of s (mode 0) or on points (mode 1). */
 void launch_surrogate(int dim, const Consts& c, DevState s, const int32_t* part_id, Surrogate sg, int64_t n, int mode,
                       const double* pts, double* out, hipStream_t st);
+/* The mesh field at points: containing cell (-1 if none) and (vel, p, rho). */
+void launch_mesh_eval(int dim, DevMesh m, int64_t n, const double* pts, int32_t* cell, double* vals, hipStream_t st);
 /* FindCell on the aero particles of s (mesh.hip). */
 void launch_find_cell(int dim, const Consts& c, DevState s, const int32_t* part_id, DevMesh m, int64_t n,
                       int32_t* lost, hipStream_t st);

@@ -145,6 +145,50 @@ __global__ __launch_bounds__(TPB) void k_find_cell(Consts c, DevState s, const i
     s.cellRho[i] = M.cRho[found];
 }
 
+/* The mesh field at arbitrary points: the containing cell (-1 if none, with
+ * zero values) found over the voxel candidates, vals = (vel[D], p, rho). */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_mesh_eval(DevMesh M, int64_t n, const double* __restrict__ pts,
+                                                   int32_t* __restrict__ cell_out, double* __restrict__ vals)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n) return;
+    double testp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < D; ++a) testp[a] = pts[i * D + a];
+    int32_t found = -1;
+    int64_t vox = 0;
+    bool inside = true;
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) {
+        const double q = (testp[a] - M.glo[a]) * M.ginv;
+        if (!(q >= 0.0) || q >= double(M.gdims[a])) inside = false;
+        const int32_t g = inside ? int32_t(q) : 0;
+        vox = vox * M.gdims[a] + g;
+    }
+    if (inside) {
+        for (int32_t k = M.gstart[vox]; k < M.gstart[vox + 1]; ++k) {
+            const int32_t cell = M.gcells[k];
+            if (check_cell<D>(M, cell, testp)) {
+                found = cell;
+                break;
+            }
+        }
+    }
+    cell_out[i] = found;
+#pragma unroll
+    for (int a = 0; a < D; ++a) vals[i * (D + 2) + a] = found >= 0 ? M.cVel[(int64_t)found * D + a] : 0.0;
+    vals[i * (D + 2) + D] = found >= 0 ? M.cP[found] : 0.0;
+    vals[i * (D + 2) + D + 1] = found >= 0 ? M.cRho[found] : 0.0;
+}
+
+void launch_mesh_eval(int dim, DevMesh m, int64_t n, const double* pts, int32_t* cell, double* vals, hipStream_t st)
+{
+    const dim3 g(grid_of(n)), b(TPB);
+    if (dim == 2) hipLaunchKernelGGL(k_mesh_eval<2>, g, b, 0, st, m, n, pts, cell, vals);
+    else hipLaunchKernelGGL(k_mesh_eval<3>, g, b, 0, st, m, n, pts, cell, vals);
+}
+
 void launch_find_cell(int dim, const Consts& c, DevState s, const int32_t* part_id, DevMesh m, int64_t n,
                       int32_t* lost, hipStream_t st)
 {

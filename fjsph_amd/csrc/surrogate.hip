@@ -95,6 +95,9 @@ __global__ __launch_bounds__(TPB) void k_surrogate(Consts c, DevState s, const i
             use = false;
         }
     }
+    /* the aero particles are surface particles, clustered in cell order: a
+     * wave with none skips the network (the test is uniform over the wave) */
+    if (!__any(use)) return;
     if (use) {
 #pragma unroll
         for (int a = 0; a < D; ++a) x[a] = mode == 0 ? s.x[i * D + a] : pts[i * D + a];

@@ -63,6 +63,7 @@ SIGNATURES = {
                                  _D, _D, _D]),
     "fjsph_get_mesh_cells": (C.c_int, [_VP, _D, _D, _D, _D]),
     "fjsph_mesh_lost": (C.c_int, [_VP, _I64]),
+    "fjsph_mesh_eval": (C.c_int, [_VP, C.c_int64, _D, _I32, _D]),
     "fjsph_set_surrogate": (C.c_int, [_VP, C.c_int32, _F32, _F32, _D, _D, _D, _D]),
     "fjsph_surrogate_eval": (C.c_int, [_VP, C.c_int64, _D, _D]),
 }
@@ -227,6 +228,15 @@ class Engine:
         p, r = np.zeros(nc), np.zeros(nc)
         self._chk(self.lib.fjsph_get_mesh_cells(self.h, _ptr(c), _ptr(v), _ptr(p), _ptr(r)), "get_mesh_cells")
         return c, v, p, r
+
+    def mesh_eval(self, points):
+        """The mesh field at points: (cells, vals)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        cells = np.zeros(pts.shape[0], np.int32)
+        vals = np.zeros((pts.shape[0], self.dim + 2))
+        self._chk(self.lib.fjsph_mesh_eval(self.h, pts.shape[0], _ptr(pts), _ptr(cells, _I32), _ptr(vals)),
+                  "mesh_eval")
+        return cells, vals
 
     def mesh_lost(self):
         v = np.zeros(1, np.int64)

@@ -38,13 +38,23 @@ def mesh_cell_values(mesh):
     return np.concatenate([vel, p[:, None], rho[:, None]], axis=1)
 
 
-def sample_mesh(mesh, n, seed=0):
-    """n uniform points in the mesh box and the values of their cells."""
+def sample_mesh(mesh, n, seed=0, engine=None):
+    """n uniform points in the mesh box and the values of their cells,
+    located by the engine's mesh lookup when an engine holding the mesh is
+    given, else by scipy."""
     from scipy.spatial import Delaunay
 
     rng = np.random.default_rng(seed)
     lo, hi = mesh.verts.min(axis=0), mesh.verts.max(axis=0)
     x = rng.uniform(lo, hi, size=(n, lo.size))
+    if engine is not None:
+        cell, vals = engine.mesh_eval(x)
+        keep = cell >= 0
+        return x[keep], vals[keep]
+    if getattr(mesh, "tri", None) is not None:
+        cell = mesh.tri.find_simplex(x)
+        keep = cell >= 0
+        return x[keep], mesh_cell_values(mesh)[cell[keep]]
     tri = Delaunay(mesh.verts)
     cell = tri.find_simplex(x)
     keep = cell >= 0
@@ -76,11 +86,13 @@ def torch_forward(sg, x):
     return y.double().numpy() * sg.out_scale + sg.out_shift
 
 
-def train(mesh, n_samples=1_000_000, n_hidden=4, steps=3000, batch=16384, lr=3e-3, seed=0, device=None):
-    """Fit the surrogate to the mesh field; returns a Surrogate."""
+def train(mesh, n_samples=1_000_000, n_hidden=4, steps=3000, batch=16384, lr=3e-3, seed=0, device=None,
+          engine=None):
+    """Fit the surrogate to the mesh field (samples labelled by `engine`'s
+    mesh lookup when given); returns a Surrogate."""
     import torch
 
-    x, y = sample_mesh(mesh, n_samples, seed=seed)
+    x, y = sample_mesh(mesh, n_samples, seed=seed, engine=engine)
     dim = x.shape[1]
     lo, hi = mesh.verts.min(axis=0), mesh.verts.max(axis=0)
     in_shift, in_scale = 0.5 * (lo + hi), 0.5 * (hi - lo)

@@ -182,6 +182,21 @@ def test_mesh_aero_fp64():
         assert_close(f, eng.field(f), orc.field(f), FP64_STEP)
 
 
+def test_batched_droplets_fp64():
+    """Many independent droplets of uneven size in one particle set (the
+    batched-droplet configuration, scaled down): steps against the oracle."""
+    case = cases.batched_droplets(n_drops=12, dx=5e-5, d_range=(0.2e-3, 0.5e-3), precision=abi.PREC_FP64)
+    print(f"[batched droplets] n={case.n}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    eng.integrate_no_update()
+    orc.integrate_no_update()
+    for _ in range(3):
+        ie, io = eng.integrate_step(), orc.integrate_step()
+        assert ie["iterations"] == io["iterations"]
+    for f in ("X", "V", "RHO", "ACC"):
+        assert_close(f, eng.field(f), orc.field(f), FP64_STEP)
+
+
 def test_step_parity_mixed_droplet():
     case = _perturb(cases.droplet_3d(diameter=1.0e-3, dx=5e-5, precision=abi.PREC_MIXED), seed=4)
     print(f"[mixed step] droplet n={case.n}", flush=True)

@@ -91,3 +91,28 @@ def test_surrogate_aero_step():
         info = eng_s.integrate_step()
         assert np.isfinite(info["rms_error"])
     assert np.all(np.isfinite(eng_s.field("X")))
+
+
+def test_mesh_eval_matches_scipy():
+    """fjsph_mesh_eval locates points in the same simplices as scipy's
+    Delaunay.find_simplex and returns those cells' averaged values."""
+    from scipy.spatial import Delaunay
+
+    from fjsph_amd.engine import Engine
+    s = abi.default_settings(3)
+    s.particle_step = 1e-4
+    eng = Engine(s)
+    mesh = cases.delaunay_mesh([0, 0, 0], [1e-2, 2e-3, 2e-3], 20000, seed=3)
+    eng.set_mesh(mesh)
+    x = np.random.default_rng(4).uniform([0, 0, 0], [1e-2, 2e-3, 2e-3], size=(20000, 3))
+    cell, vals = eng.mesh_eval(x)
+    ref = mesh.tri.find_simplex(x)
+    assert np.array_equal(cell >= 0, ref >= 0)
+    ok = cell >= 0
+    assert ok.sum() > 19000
+    assert np.array_equal(cell[ok], ref[ok])
+    _, cvel, cp, crho = eng.mesh_cells()
+    assert np.array_equal(vals[ok, :3], cvel[cell[ok]])
+    assert np.array_equal(vals[ok, 3], cp[cell[ok]])
+    del Delaunay
+    eng.close()
