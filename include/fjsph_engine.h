@@ -295,6 +295,11 @@ fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts,
 fjsph_status fjsph_get_mesh_cells(fjsph_ctx* ctx, double* centre, double* vel, double* p, double* rho);
 /* Aero particles found in no cell by the last lookup. */
 fjsph_status fjsph_mesh_lost(fjsph_ctx* ctx, int64_t* lost);
+/* The mesh field at n host points (n*dim): the containing cell (-1 if none)
+ * and its (vel[dim], p, rho) into vals (n*(dim+2), zeros if none).  No
+ * counterpart in the reference; used to label the surrogate's training
+ * samples. */
+fjsph_status fjsph_mesh_eval(fjsph_ctx* ctx, int64_t n, const double* points, int32_t* cells, double* vals);
 
 /* ---- surrogate of the mesh field (aero_source = FJSPH_SURROGATE) ----------- */
 
