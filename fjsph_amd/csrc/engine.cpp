@@ -448,7 +448,7 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
                               hipMemcpyDeviceToHost, ctx->st));
         sync(ctx);
         if (ctx->sc_host->overflow)
-            throw std::string("neighbour candidate union of a tile exceeds 65535 particles (") +
+            throw std::string("neighbour candidate union of a tile exceeds 127 LDS tiles (65024 particles; union of ") +
                 std::to_string(ctx->sc_host->overflow) + "); particles are too clustered for the tile grid";
         const int32_t mc = ctx->sc_host->max_count;
         ctx->max_count = mc;

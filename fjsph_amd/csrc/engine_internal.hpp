@@ -132,7 +132,7 @@ struct DevScalars {
     double red[8];    /* find_timestep reduction results */
     double bbox[6];   /* lo[3], hi[3] */
     int32_t max_count;
-    int32_t overflow; /* largest candidate union beyond 65535 (0 if none) */
+    int32_t overflow; /* largest candidate union beyond 127 tiles (0 if none) */
     int32_t mesh_lost; /* aero particles in no mesh cell at the last lookup */
 };
 

@@ -76,6 +76,36 @@ __device__ __forceinline__ int32_t union_slot(const TileGrid& tg, const int32_t*
     return tg.order[seg_a[k] + (u - pre[k])];
 }
 
+/* Union indices are dealt to the LDS tiles in groups of TGRP consecutive
+ * indices, round robin: tile k holds groups k, k + NT, k + 2 NT, ... of the
+ * NT = ceil(U / TILE) tiles.  Every tile then samples the whole union
+ * evenly, so each target's neighbours spread evenly over the tiles and the
+ * lock-step rows keep most lanes of a wave busy in every tile (with tiles
+ * of consecutive indices a tile covers one slab of the union, and a wave's
+ * per-tile list lengths differ several-fold between targets near and far
+ * from it).  Tile-local slot t of tile k is union index
+ * ((t / TGRP) NT + k) TGRP + t % TGRP; a list entry stores k TILE + t, so
+ * entries ascend with the tile. */
+constexpr int TGRP = 4;
+
+__device__ __forceinline__ int n_tiles(int U) { return (U + TILE - 1) / TILE; }
+
+__device__ __forceinline__ int tile_union_index(int k, int t, int NT) { return ((t / TGRP) * NT + k) * TGRP + t % TGRP; }
+
+/* slots of tile k (the last group of the union may be partial: its slots
+ * past U are padding) */
+__device__ __forceinline__ int tile_len(int k, int U, int NT)
+{
+    const int ng = (U + TGRP - 1) / TGRP;
+    return k < ng ? ((ng - 1 - k) / NT + 1) * TGRP : 0;
+}
+
+/* union index of a list entry */
+__device__ __forceinline__ int entry_union_index(uint32_t e, int NT)
+{
+    return tile_union_index(int(e / TILE), int(e % TILE), NT);
+}
+
 /* Block id of the block that owns slot i at the last build. */
 __device__ __forceinline__ int64_t block_of_slot(const TileGrid& tg, int64_t i)
 {
@@ -185,8 +215,9 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
     if (h.p0 == h.p1) return;
-    if (U >= int(NO_ENTRY)) {
+    if (NT * TILE > int(NO_ENTRY)) {
         if (threadIdx.x == 0) atomicMax(&sc->overflow, U);
         return;
     }
@@ -243,11 +274,16 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
 #pragma unroll
         for (int a = 0; a < D; ++a) xi[a] = active ? x[i * D + a] : 0.0;
         int nrow = 0, nreal = 0;
-        for (int base = 0; base < U; base += TILE) {
-            const int nt = min(TILE, U - base);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) {
+                    tsl[t] = -1;
+                    continue;
+                }
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
                 tsl[t] = sl;
 #pragma unroll
                 for (int a = 0; a < D; ++a) tp[a][t] = x[(int64_t)sl * D + a];
@@ -262,7 +298,7 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
 #pragma unroll
                     for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
                     const double d2 = exact_dist2<D>(xi, xj);
-                    if (d2 < sr && tsl[t] != (int32_t)i) {
+                    if (d2 < sr && tsl[t] != (int32_t)i && tsl[t] >= 0) {
                         cu = (cu >> 16) | (uint64_t(base + t) << 48);
                         ++nreal;
                         if (++q == 4) {
@@ -316,6 +352,7 @@ __device__ __forceinline__ void walk_lane(const TileGrid& tg, const NbrList& nl,
     int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
     int nseg, p0, p1;
     union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
+    const int NT = n_tiles(pre[nseg]);
     const int64_t p = tg.pos[i];
     const int nrows = min(nl.count[i], nl.cap >> 2);
     for (int r = 0; r < nrows; ++r) {
@@ -323,7 +360,7 @@ __device__ __forceinline__ void walk_lane(const TileGrid& tg, const NbrList& nl,
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint32_t uq = uint32_t(u >> (16 * q)) & 0xFFFFu;
-            if (uq != NO_ENTRY) f(union_slot(tg, seg_a, pre, nseg, (int)uq));
+            if (uq != NO_ENTRY) f(union_slot(tg, seg_a, pre, nseg, entry_union_index(uq, NT)));
         }
     }
 }
@@ -396,6 +433,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
     __shared__ double lds[NWAVE];
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
     double npd_acc = 0.0;
     const double inv_H = 1.0 / c.H;
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
@@ -416,11 +454,13 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
         double kernsum = c.Wc, colour = 0.0;
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0);
-        for (int base = 0; base < U; base += TILE) {
-            const int nt = min(TILE, U - base);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
 #pragma unroll
                 for (int a = 0; a < D; ++a) tp[a][t] = s.x[(int64_t)sl * D + a];
                 tvol[t] = ax.vol[sl];
@@ -601,6 +641,7 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
     const double inv_H = 1.0 / c.H;
     const double h133 = 1.33 * c.particle_step;
     const double r_lim = sqrt(2.0) * h133;
@@ -643,11 +684,13 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
         const T eps2 = T(0.0001 * c.H_sq), alpha = T(c.visc_alpha);
         Rows cu;
         cu.init(nl, p, fluid ? nl.count[i] : 0);
-        for (int base = 0; base < U; base += TILE) {
-            const int nt = min(TILE, U - base);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
 #pragma unroll
                 for (int a = 0; a < D; ++a) tp[a][t] = s.x[(int64_t)sl * D + a];
                 tvol[t] = ax.vol[sl];
@@ -820,6 +863,7 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
     const double inv_H = 1.0 / c.H;
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
@@ -849,11 +893,13 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
         int zone = (fluid && s.surf[i] == 1) ? 1 : 0;
         Rows cu;
         cu.init(nl, p, fluid ? nl.count[i] : 0);
-        for (int base = 0; base < U; base += TILE) {
-            const int nt = min(TILE, U - base);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     tp[a][t] = s.x[(int64_t)sl * D + a];
@@ -938,6 +984,7 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
     const double inv_H = 1.0 / c.H;
     const double inv_Wdx = 1.0 / c.W_dx;
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
@@ -959,11 +1006,13 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
         double maxU2 = 0.0, dmin = 2.0;
         Rows cu;
         cu.init(nl, p, work ? nl.count[i] : 0);
-        for (int base = 0; base < U; base += TILE) {
-            const int nt = min(TILE, U - base);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     tp[a][t] = s.x[(int64_t)sl * D + a];
@@ -1256,6 +1305,7 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
     constexpr bool F32 = sizeof(T) == 4;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
     /* reference point of the block (its first target) for the FP32
      * relative coordinates */
     double xref[D], vref[D], xbref[D];
@@ -1306,11 +1356,13 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
         double Rrho_ = 0.0, Rrhoc_ = 0.0;
         Rows cu;
         cu.init(nl, p, fluid ? nl.count[i] : 0);
-        for (int base = 0; base < U; base += TILE) {
-            const int nt = min(TILE, U - base);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, base + t);
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
                 if constexpr (F32) {
                     float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
