@@ -448,7 +448,8 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
                               hipMemcpyDeviceToHost, ctx->st));
         sync(ctx);
         if (ctx->sc_host->overflow)
-            throw std::string("neighbour candidate union of a tile exceeds 127 LDS tiles (65024 particles; union of ") +
+            throw std::string("neighbour candidate union of a tile exceeds ") + std::to_string(NO_ENTRY / TILE) +
+                " LDS tiles (union of " +
                 std::to_string(ctx->sc_host->overflow) + "); particles are too clustered for the tile grid";
         const int32_t mc = ctx->sc_host->max_count;
         ctx->max_count = mc;

@@ -40,9 +40,10 @@ struct DevState {
 };
 
 /* Neighbour lists in rows of 4 entries: row r of the target at position p
- * of the target order (TileGrid::vorder) is lu4[r*npad + p], four 16-bit indices into the
- * candidate union of the target's tile block, ascending, all in one TILE
- * of the union; unused entries hold NO_ENTRY.  The targets of one wave
+ * of the target order (TileGrid::vorder) is lu4[r*npad + p], four 16-bit
+ * entries k*TILE + t (slot t of LDS tile k of the candidate union of the
+ * target's tile block), ascending, all in one tile; unused entries hold
+ * NO_ENTRY.  The targets of one wave
  * hold the same number of rows per tile (padding rows are all NO_ENTRY).
  * count[i] is the row count of slot i, nnb[i] its neighbours (self
  * excluded).  The frozen distance of src/Resid.cpp:288-289 is not stored:
@@ -56,8 +57,14 @@ struct NbrList {
     int32_t f32;  /* mixed precision: FP32 pair terms */
 };
 
-/* Candidate-union tile shared by the list build and every pair pass. */
-constexpr int TILE = 512;
+/* Candidate-union tile shared by the list build and every pair pass
+ * (a multiple of 4, the group size in which union slots are dealt to
+ * the tiles). */
+#ifndef FJ_TILE
+#define FJ_TILE 512
+#endif
+constexpr int TILE = FJ_TILE;
+static_assert(TILE % 4 == 0, "tile slots come in groups of 4");
 constexpr uint32_t NO_ENTRY = 0xFFFF;
 
 /* Tile decomposition of the last neighbour build: block b covers cells
@@ -132,7 +139,7 @@ struct DevScalars {
     double red[8];    /* find_timestep reduction results */
     double bbox[6];   /* lo[3], hi[3] */
     int32_t max_count;
-    int32_t overflow; /* largest candidate union beyond 127 tiles (0 if none) */
+    int32_t overflow; /* largest candidate union beyond 0xFFFF / TILE tiles (0 if none) */
     int32_t mesh_lost; /* aero particles in no mesh cell at the last lookup */
 };
 

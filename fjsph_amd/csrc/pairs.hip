@@ -625,6 +625,111 @@ void launch_aero_velocity(int dim, const Consts& c, DevState s, const int32_t* p
 
 /* -------------------------------------- Detect_Surface + dissipation_terms */
 
+/* Marrone et al. (2010) point-T test of one pair, src/Geometry.cpp:43-98:
+ * does neighbour j (at xj, frozen distance r) fall in the test region of
+ * point T = xi + h133 nhat? */
+template <int D>
+__device__ __forceinline__ bool marrone_pair(const double* xi, const double* xj, double r, const double* nhat,
+                                             const double* tau_hat, const double* pT, double h133, double r_lim)
+{
+    double xjT[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) xjT[a] = xj[a] - pT[a];
+    if (r >= r_lim) return sqrt(dotd<D>(xjT, xjT)) < h133;
+    if (D == 2) return fabs(dotd<D>(nhat, xjT)) + fabs(dotd<D>(tau_hat, xjT)) < h133;
+    double mr[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) mr[a] = -(xi[a] - xj[a]) / r;
+    return acos(dotd<D>(nhat, mr)) < kPi / 4.0;
+}
+
+/* The FP64 point-T test of pair (i, j) from the global state: normal,
+ * test point and frozen distance rebuilt as k_surface1_t forms them. */
+template <int D, bool FR>
+__device__ __noinline__ bool marrone_exact(const Consts& c, const DevState& s, const Aux& ax, int64_t i, int64_t j)
+{
+    const double h133 = 1.33 * c.particle_step;
+    const double r_lim = sqrt(2.0) * h133;
+    double xi[D], xj[D], ni[D], nhat[D], tau_hat[D], pT[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        xi[a] = s.x[i * D + a];
+        xj[a] = s.x[j * D + a];
+        ni[a] = s.norm[i * D + a];
+        tau_hat[a] = 0.0;
+    }
+    normalize<D>(ni, nhat);
+    if (D == 2) {
+        double tau[D];
+        tau[0] = ni[1 % D];
+        tau[1 % D] = -ni[0];
+        normalize<D>(tau, tau_hat);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) pT[a] = xi[a] + h133 * nhat[a];
+    double d2;
+    if constexpr (FR) {
+        double xbi[D], xbj[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xbi[a] = ax.xb[i * D + a];
+            xbj[a] = ax.xb[j * D + a];
+        }
+        d2 = exact_dist2<D>(xbi, xbj);
+    } else {
+        d2 = exact_dist2<D>(xi, xj);
+    }
+    return marrone_pair<D>(xi, xj, sqrt(d2), nhat, tau_hat, pT, h133, r_lim);
+}
+
+/* Per-target results of Detect_Surface loop 1 and dissipation_terms
+ * (src/Geometry.cpp:100-143, src/Shifting.cpp:175-185). */
+template <int D, int MODE>
+__device__ __forceinline__ void surface1_out(const Consts& c, DevState& s, int64_t i, bool fluid, int32_t bi,
+                                             double lam_nb, double lami, bool found, const double* nrm,
+                                             const double* av, double rd, double* norms_tmp)
+{
+    constexpr bool with_surf = MODE != 2;
+    constexpr bool with_diss = MODE != 0;
+    if (!fluid) {
+        if constexpr (with_surf) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = 0.0;
+        }
+        return;
+    }
+    if constexpr (with_diss) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) s.aVisc[i * D + a] = av[a];
+        s.deltaD[i] = c.dsph_cont * rd;
+    }
+    if constexpr (with_surf) {
+        int surf;
+        if (bi < FJSPH_PIPE) surf = 0;
+        else if (lam_nb < 0.2) surf = 1;
+        else if (lam_nb < 0.75) surf = found ? 0 : 1;
+        else surf = 0;
+        s.surf[i] = surf;
+        if (bi < FJSPH_PIPE) s.woccl[i] = 1.0;
+        double Ln[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            double t = 0.0;
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) t += s.L[i * D * D + a * D + b2] * nrm[b2];
+            Ln[a] = t;
+        }
+        double out[D];
+        if (sqrt(dotd<D>(Ln, Ln)) > 0.1 * lami / c.H) normalize<D>(Ln, out);
+        else {
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[a] = 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = out[a];
+    }
+}
+
 /* MODE 0: Detect_Surface loop 1 only; 1: with dissipation_terms;
  * 2: dissipation_terms only.  T is the arithmetic of the dissipation terms
  * (FP32 on the mixed path). */
@@ -714,23 +819,9 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
 #pragma unroll
                 for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
                 frozen_r<D, FR>(xi, xj, xbi, ax.xb, FR ? tsl[FR ? t : 0] : 0, d2, r);
-                if (testing && !found) {
-                    /* Marrone et al. (2010) point-T test, src/Geometry.cpp:43-98;
-                     * the outcome does not depend on the order of the pairs */
-                    double xjT[D];
-#pragma unroll
-                    for (int a = 0; a < D; ++a) xjT[a] = xj[a] - pT[a];
-                    if (r >= r_lim) {
-                        if (sqrt(dotd<D>(xjT, xjT)) < h133) found = true;
-                    } else if (D == 2) {
-                        if (fabs(dotd<D>(nhat, xjT)) + fabs(dotd<D>(tau_hat, xjT)) < h133) found = true;
-                    } else {
-                        double mr[D];
-#pragma unroll
-                        for (int a = 0; a < D; ++a) mr[a] = -(xi[a] - xj[a]) / r;
-                        if (acos(dotd<D>(nhat, mr)) < kPi / 4.0) found = true;
-                    }
-                }
+                /* the outcome of the point-T test does not depend on the
+                 * order of the pairs */
+                if (testing && !found && marrone_pair<D>(xi, xj, r, nhat, tau_hat, pT, h133, r_lim)) found = true;
                 double W, fg;
                 wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
                 if constexpr (with_surf) {
@@ -778,43 +869,233 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
                 }
             FJ_TILE_ENTRIES_END(cu, nl)
         }
-        if (!active) continue;
-        if (!fluid) {
-            if constexpr (with_surf) {
-#pragma unroll
-                for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = 0.0;
-            }
-            continue;
-        }
-        if constexpr (with_diss) {
-#pragma unroll
-            for (int a = 0; a < D; ++a) s.aVisc[i * D + a] = av[a];
-            s.deltaD[i] = c.dsph_cont * rd;
-        }
-        if (!with_surf) continue;
-        int surf;
-        if (bi < FJSPH_PIPE) surf = 0;
-        else if (lam_nb < 0.2) surf = 1;
-        else if (lam_nb < 0.75) surf = found ? 0 : 1;
-        else surf = 0;
-        s.surf[i] = surf;
-        if (bi < FJSPH_PIPE) s.woccl[i] = 1.0;
-        double Ln[D];
+        if (active) surface1_out<D, MODE>(c, s, i, fluid, bi, lam_nb, lami, found, nrm, av, rd, norms_tmp);
+    }
+}
+
+/* Mixed-precision form of k_surface1_t: the pair terms in FP32 on
+ * coordinates relative to the block's first target (positions, velocities,
+ * build positions) and densities relative to its density, every sum in
+ * FP64; 64 B of LDS per candidate (80 B with build positions).
+ *
+ * The point-T test is decided in FP32 unless one of its comparisons lies
+ * within a margin of its threshold (1e-3 h133 for lengths, 1e-3 for the
+ * cosine; the FP32 coordinates are off by at most |x - xref| 2^-24, well
+ * under 1e-5 h133 for any block); such pairs repeat the test in FP64 from
+ * the global positions, so the surface flags are those of the FP64 test.
+ * acos(nhat . m) < pi/4 is evaluated as cos(pi/4) < nhat . m <= 1. */
+#ifndef FJ_SURF1_OCC
+#define FJ_SURF1_OCC __attribute__((amdgpu_waves_per_eu(4, 8)))
+#endif
+template <int D, int MODE, bool FR>
+__global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                    TileGrid tg, NbrList nl, double* norms_tmp)
+{
+    constexpr bool with_surf = MODE != 2;
+    constexpr bool with_diss = MODE != 0;
+    constexpr int TD = with_diss ? TILE : 1;
+    __shared__ float4 tA[TILE];            /* x_j - xref, V_j            */
+    __shared__ float4 tB[TILE];            /* v_j - vref, lambda_j       */
+    __shared__ float4 tG[TD];              /* gradRho_j, rho_j - rref    */
+    __shared__ float2 tE[TD];              /* 1/rho_j, m_j (negative: j is PISTON or BOUND) */
+    __shared__ float4 tX[FR ? TILE : 1];   /* xb_j - xbref               */
+    __shared__ int32_t tsl[with_surf ? TILE : 1];
+    __shared__ BlockHead h;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
+    double xref[D], vref[D], xbref[D], rref;
+    {
+        const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            double t = 0.0;
-#pragma unroll
-            for (int b2 = 0; b2 < D; ++b2) t += s.L[i * D * D + a * D + b2] * nrm[b2];
-            Ln[a] = t;
+            xref[a] = s.x[i0 * D + a];
+            vref[a] = s.v[i0 * D + a];
+            xbref[a] = FR ? ax.xb[i0 * D + a] : 0.0;
         }
-        double out[D];
-        if (sqrt(dotd<D>(Ln, Ln)) > 0.1 * lami / c.H) normalize<D>(Ln, out);
-        else {
+        rref = s.rho[i0];
+    }
+    const double h133 = 1.33 * c.particle_step;
+    const double r_lim = sqrt(2.0) * h133;
+    const float h133f = float(h133), r_limf = float(r_lim);
+    const float tol = float(1e-3 * h133), tolc = 1e-3f;
+    const float cos45 = float(cos(kPi / 4.0));
+    const float Bg = float(c.B * c.gam), Ht = float(c.H), inv_H = float(1.0 / c.H), Wc = float(c.Wc);
+    const float eps2 = float(0.0001 * c.H_sq), alpha = float(c.visc_alpha);
+    const float rreff = float(rref);
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.vorder[p] : 0;
+        const bool fluid = active && part_id[i] >= c.n_bound;
+        double nrm[D], av[D], rd = 0.0;
+        float xit[D], xbit[D], vit[D], gri[D], nhf[D], tauf[D], pTt[D];
+        {
+            double xi[D], ni[D], nhat[D], tau_hat[D];
 #pragma unroll
-            for (int a = 0; a < D; ++a) out[a] = 0.0;
+            for (int a = 0; a < D; ++a) {
+                xi[a] = fluid ? s.x[i * D + a] : xref[a];
+                ni[a] = fluid ? s.norm[i * D + a] : 0.0;
+                xit[a] = float(xi[a] - xref[a]);
+                xbit[a] = (FR && fluid) ? float(ax.xb[i * D + a] - xbref[a]) : 0.f;
+                vit[a] = fluid ? float(s.v[i * D + a] - vref[a]) : 0.f;
+                gri[a] = fluid ? float(s.gradRho[i * D + a]) : 0.f;
+                nrm[a] = 0.0;
+                av[a] = 0.0;
+                tau_hat[a] = 0.0;
+            }
+            normalize<D>(ni, nhat);
+            if (D == 2) {
+                double tau[D];
+                tau[0] = ni[1 % D];
+                tau[1 % D] = -ni[0];
+                normalize<D>(tau, tau_hat);
+            }
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                pTt[a] = float(xi[a] + h133 * nhat[a] - xref[a]);
+                nhf[a] = float(nhat[a]);
+                tauf[a] = float(tau_hat[a]);
+            }
         }
+        const int32_t bi = fluid ? s.b[i] : 0;
+        const double lam_nb = fluid ? s.lam_nb[i] : 1.0;
+        const double lami = fluid ? s.lam[i] : 1.0;
+        const double rhoi = fluid ? s.rho[i] : rref;
+        const bool testing = with_surf && fluid && bi >= FJSPH_PIPE && lam_nb >= 0.2 && lam_nb < 0.75;
+        bool found = false;
+        const bool high = lami > 0.7;
+        const float lamf = float(lami);
+        const float drhoi = float(rhoi - rref), rhoif = float(rhoi);
+        const float cfac = float(sqrt(c.B * c.gam / rhoi));
+        Rows cu;
+        cu.init(nl, p, fluid ? nl.count[i] : 0);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-        for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = out[a];
+                for (int a = 0; a < D; ++a) {
+                    pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
+                    if constexpr (with_diss) va[a] = float(s.v[(int64_t)sl * D + a] - vref[a]);
+                    if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
+                }
+                tA[t] = make_float4(pa[0], pa[1], pa[2], float(ax.vol[sl]));
+                tB[t] = make_float4(va[0], va[1], va[2], with_surf ? float(s.lam[sl]) : 0.f);
+                if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
+                if constexpr (with_surf) tsl[t] = sl;
+                if constexpr (with_diss) {
+                    float ga[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int a = 0; a < D; ++a) ga[a] = float(s.gradRho[(int64_t)sl * D + a]);
+                    tG[t] = make_float4(ga[0], ga[1], ga[2], float(s.rho[sl] - rref));
+                    const float mj = float(s.m[sl]);
+                    tE[t] = make_float2(float(ax.invrho[sl]), s.b[sl] > FJSPH_PISTON ? mj : -mj);
+                }
+            }
+            __syncthreads();
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
+                const float4 A = tA[t];
+                const float pj[3] = {A.x, A.y, A.z};
+                float Rji[D], rr;
+#pragma unroll
+                for (int a = 0; a < D; ++a) Rji[a] = pj[a] - xit[a];
+                if constexpr (FR) {
+                    const float4 X = tX[FR ? t : 0];
+                    const float xbj[3] = {X.x, X.y, X.z};
+                    float Rb[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) Rb[a] = xbj[a] - xbit[a];
+                    rr = Rb[0] * Rb[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
+                } else {
+                    rr = Rji[0] * Rji[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rji[a] * Rji[a];
+                }
+                const float r = fast_sqrt<float>(rr);
+                float W, fg;
+                wendland<float>(r, Ht, inv_H, Wc, W, fg);
+                const float4 B = tB[t];
+                if constexpr (with_surf) {
+                    if (testing && !found) {
+                        /* point-T test (src/Geometry.cpp:43-98) in FP32 unless
+                         * within the margin of a threshold */
+                        float xjT[D];
+#pragma unroll
+                        for (int a = 0; a < D; ++a) xjT[a] = pj[a] - pTt[a];
+                        const float dr = r - r_limf;
+                        float q;
+                        bool sure = fabsf(dr) >= tol;
+                        if (dr >= 0.f) {
+                            float z = xjT[0] * xjT[0];
+#pragma unroll
+                            for (int a = 1; a < D; ++a) z += xjT[a] * xjT[a];
+                            q = h133f - fast_sqrt<float>(z);
+                            sure = sure && fabsf(q) >= tol;
+                        } else if (D == 2) {
+                            float z1 = nhf[0] * xjT[0], z2 = tauf[0] * xjT[0];
+#pragma unroll
+                            for (int a = 1; a < D; ++a) { z1 += nhf[a] * xjT[a]; z2 += tauf[a] * xjT[a]; }
+                            q = h133f - (fabsf(z1) + fabsf(z2));
+                            sure = sure && fabsf(q) >= tol;
+                        } else {
+                            float z = nhf[0] * Rji[0];
+#pragma unroll
+                            for (int a = 1; a < D; ++a) z += nhf[a] * Rji[a];
+                            /* acos of a cosine past 1 (moved positions,
+                             * frozen r) is NaN: no hit */
+                            const float cq = z / r;
+                            q = (cq - cos45) * (1.f - cq);
+                            sure = sure && fabsf(cq - cos45) >= tolc && fabsf(1.f - cq) >= tolc;
+                        }
+                        found = sure ? q > 0.f : marrone_exact<D, FR>(c, s, ax, i, tsl[with_surf ? t : 0]);
+                    }
+                    const float f = A.w * (high ? (B.w - lamf) : B.w) * fg;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) nrm[a] -= double(f * Rji[a]);
+                }
+                if constexpr (with_diss) {
+                    /* dissipation_terms pair term (src/Shifting.cpp:141-173) */
+                    const float4 G = tG[t];
+                    const float2 E = tE[t];
+                    const float vj[3] = {B.x, B.y, B.z}, grj[3] = {G.x, G.y, G.z};
+                    float g[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) g[a] = fg * Rji[a];
+                    const float idist2 = fast_rcp<float>(rr + eps2);
+                    float rdotg = Rji[0] * g[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rdotg += Rji[a] * g[a];
+                    const float drho = G.w - drhoi;
+                    if (E.y > 0.f) {
+                        float vdotr = (vj[0] - vit[0]) * Rji[0];
+#pragma unroll
+                        for (int a = 1; a < D; ++a) vdotr += (vj[a] - vit[a]) * Rji[a];
+                        if (!(vdotr > 0.f)) {
+                            const float muij = Ht * vdotr * idist2;
+                            const float cbar = 0.5f * (cfac + fast_sqrt<float>(Bg * E.x));
+                            const float ff =
+                                E.y * alpha * cbar * muij * fast_rcp<float>(0.5f * (rhoif + (rreff + G.w)));
+#pragma unroll
+                            for (int a = 0; a < D; ++a) av[a] += double(g[a] * ff);
+                        }
+                        float gsum = (gri[0] + grj[0]) * Rji[0];
+#pragma unroll
+                        for (int a = 1; a < D; ++a) gsum += (gri[a] + grj[a]) * Rji[a];
+                        rd += double(A.w * (drho + 0.5f * gsum) * rdotg * idist2);
+                    } else {
+                        rd += double(A.w * drho * rdotg * idist2);
+                    }
+                }
+            FJ_TILE_ENTRIES_END(cu, nl)
+        }
+        if (active) surface1_out<D, MODE>(c, s, i, fluid, bi, lam_nb, lami, found, nrm, av, rd, norms_tmp);
     }
 }
 
@@ -823,10 +1104,17 @@ static void launch_surface1_m(const Consts& c, DevState s, Aux a, const int32_t*
                               int r_from_x, double* norms_tmp, hipStream_t st)
 {
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (r_from_x)
-        hipLaunchKernelGGL((k_surface1_t<D, T, MODE, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
-    else
-        hipLaunchKernelGGL((k_surface1_t<D, T, MODE, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+    if constexpr (sizeof(T) == 4) {
+        if (r_from_x)
+            hipLaunchKernelGGL((k_surface1_m<D, MODE, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+        else
+            hipLaunchKernelGGL((k_surface1_m<D, MODE, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+    } else {
+        if (r_from_x)
+            hipLaunchKernelGGL((k_surface1_t<D, T, MODE, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+        else
+            hipLaunchKernelGGL((k_surface1_t<D, T, MODE, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+    }
 }
 
 template <int D, typename T>
