@@ -61,7 +61,7 @@ struct NbrList {
  * (a multiple of 4, the group size in which union slots are dealt to
  * the tiles). */
 #ifndef FJ_TILE
-#define FJ_TILE 512
+#define FJ_TILE 496
 #endif
 constexpr int TILE = FJ_TILE;
 static_assert(TILE % 4 == 0, "tile slots come in groups of 4");

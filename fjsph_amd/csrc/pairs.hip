@@ -135,6 +135,18 @@ __device__ __forceinline__ void block_head(const TileGrid& tg, BlockHead& h)
     __syncthreads();
 }
 
+/* x - ref as an FP32 value hi and its FP32 rounding residual lo (hi + lo
+ * = x - ref to about 1e-14 relative).  A pair difference is formed as
+ * (xj_hi - hi) - lo: the target's own rounding then cancels, and only the
+ * neighbour's (independent from pair to pair) remains, instead of an
+ * error common to every pair of the target that sums coherently. */
+__device__ __forceinline__ void split_rel(double x, double ref, float& hi, float& lo)
+{
+    const double d = x - ref;
+    hi = float(d);
+    lo = float(d - double(hi));
+}
+
 /* ------------------------------------------------------------ list rows */
 
 /* Lock-step walk of one target's rows: `row` advances in every lane of the
@@ -416,6 +428,69 @@ __device__ __forceinline__ void aero_velocity_one(const Consts& c, DevState& s, 
     s.cellID[i] = cond ? 1 : -3;
 }
 
+/* One pair of dSPH_PreStep in FP64 (src/Shifting.cpp:30-66): adds to the
+ * correction matrices (upper triangles), the density-gradient and normal
+ * sums, the kernel sum and colour; returns the pair's share of npd. */
+template <int D>
+__device__ __forceinline__ double dsph_pair(const Consts& c, double inv_H, const double* xi, const double* xj, double r,
+                                            double volj, double rhoj, bool nb, double rhoi,
+                                            double (&Lm)[D * (D + 1) / 2], double (&Lnb)[D * (D + 1) / 2], double* gR,
+                                            double* nr, double& kernsum, double& colour)
+{
+    double Rji[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) Rji[a] = xj[a] - xi[a];
+    double W, fg;
+    wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
+    double G[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) G[a] = -fg * Rji[a];
+    const double drho = volj * (rhoj - rhoi);
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const double vr = volj * Rji[a];
+#pragma unroll
+        for (int b2 = a; b2 < D; ++b2, ++k) {
+            const double tt = vr * G[b2];
+            Lm[k] -= tt;
+            if (nb) Lnb[k] -= tt;
+        }
+        gR[a] += drho * G[a];
+    }
+    if (!nb) return 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) nr[a] += volj * G[a];
+    kernsum += W;
+    colour += volj * W;
+    return W;
+}
+
+/* Per-slot outputs of the dSPH_PreStep pair sums. */
+template <int D>
+__device__ __forceinline__ void dsph_out(DevState& s, Aux& ax, int64_t i, const double (&Lm)[D * (D + 1) / 2],
+                                         const double (&Lnb)[D * (D + 1) / 2], const double* gR, const double* nr,
+                                         double kernsum, double colour)
+{
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b2 = a; b2 < D; ++b2, ++k) {
+            s.L[i * D * D + a * D + b2] = Lm[k];
+            s.L[i * D * D + b2 * D + a] = Lm[k];
+            ax.lnb[i * D * D + a * D + b2] = Lnb[k];
+            ax.lnb[i * D * D + b2 * D + a] = Lnb[k];
+        }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        s.gradRho[i * D + a] = gR[a];
+        s.norm[i * D + a] = nr[a];
+    }
+    s.kernsum[i] = kernsum;
+    s.colour[i] = colour;
+}
+
 /* Pair sums of dSPH_PreStep (src/Shifting.cpp:12-123): the correction
  * matrix L over all neighbours and over non-boundary neighbours (both
  * symmetric, sum_j V_j W'(r)/r R_ji R_ji^T, accumulated on the upper
@@ -470,58 +545,183 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
             }
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
-                double xj[D], Rji[D], d2, r;
+                double xj[D], d2, r;
 #pragma unroll
-                for (int a = 0; a < D; ++a) { xj[a] = tp[a][t]; Rji[a] = xj[a] - xi[a]; }
+                for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
                 frozen_r<D, FR>(xi, xj, xbi, ax.xb, FR ? tsl[FR ? t : 0] : 0, d2, r);
-                const double volj = tvol[t];
-                double W, fg;
-                wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
-                double G[D];
+                npd_acc += dsph_pair<D>(c, inv_H, xi, xj, r, tvol[t], trho[t], tnb[t] != 0, rhoi, Lm, Lnb, gR, nr,
+                                        kernsum, colour);
+            FJ_TILE_ENTRIES_END(cu, nl)
+        }
+        if (active) dsph_out<D>(s, ax, i, Lm, Lnb, gR, nr, kernsum, colour);
+    }
+    const double bs = block_reduce(npd_acc, OpSum(), lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+
+/* Mixed-precision form of k_dsph_t: FP32 pair terms on coordinates relative
+ * to the block's first target, summed in FP32 over one LDS tile and added
+ * to FP64 totals after each tile (about 40 pairs per tile and target);
+ * 20 B of LDS per candidate (36 B with build positions).  A target with
+ * fewer than 2 D neighbours takes its pairs in FP64 from the global state
+ * (dsph_pair), so the rank test of its correction matrix, which decides
+ * between L^-1 and the identity, sees the matrix of the FP64 path. */
+template <int D, bool FR>
+__global__ __launch_bounds__(TPB) void k_dsph_m(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl, double* partial)
+{
+    constexpr int NS = D * (D + 1) / 2;
+    __shared__ float4 tA[TILE];            /* x_j - xref, +V_j (-V_j: j is PISTON or BOUND) */
+    __shared__ float tR[TILE];             /* rho_j - rref */
+    __shared__ float4 tX[FR ? TILE : 1];   /* xb_j - xbref */
+    __shared__ BlockHead h;
+    __shared__ double lds[NWAVE];
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
+    double xref[D], xbref[D], rref;
+    {
+        const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xref[a] = s.x[i0 * D + a];
+            xbref[a] = FR ? ax.xb[i0 * D + a] : 0.0;
+        }
+        rref = s.rho[i0];
+    }
+    const float Ht = float(c.H), inv_Hf = float(1.0 / c.H), Wcf = float(c.Wc);
+    const double inv_H = 1.0 / c.H;
+    double npd_acc = 0.0;
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.vorder[p] : 0;
+        const bool slow = active && nl.nnb[i] < 2 * D;
+        float xit[D], xlo[D], xbit[D], xblo[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            split_rel(active ? s.x[i * D + a] : xref[a], xref[a], xit[a], xlo[a]);
+            split_rel((FR && active) ? ax.xb[i * D + a] : xbref[a], xbref[a], xbit[a], xblo[a]);
+        }
+        const double rhoi = active ? s.rho[i] : rref;
+        float drhoi, rlo;
+        split_rel(rhoi, rref, drhoi, rlo);
+        double Lm[NS], Lnb[NS], gR[D], nr[D], kernsum = c.Wc, colour = 0.0;
+        float fLm[NS], fLnb[NS], fgR[D], fnr[D], fks = 0.f, fcol = 0.f;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) { Lm[k] = 0.0; Lnb[k] = 0.0; fLm[k] = 0.f; fLnb[k] = 0.f; }
+#pragma unroll
+        for (int a = 0; a < D; ++a) { gR[a] = 0.0; nr[a] = 0.0; fgR[a] = 0.f; fnr[a] = 0.f; }
+        Rows cu;
+        cu.init(nl, p, (active && !slow) ? nl.count[i] : 0);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                float pa[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
+                    if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
+                }
+                const float v = float(ax.vol[sl]);
+                tA[t] = make_float4(pa[0], pa[1], pa[2], s.b[sl] > FJSPH_PISTON ? v : -v);
+                tR[t] = float(s.rho[sl] - rref);
+                if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
+            }
+            __syncthreads();
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
+                const float4 A = tA[t];
+                const float pj[3] = {A.x, A.y, A.z};
+                float Rji[D], rr;
+#pragma unroll
+                for (int a = 0; a < D; ++a) Rji[a] = (pj[a] - xit[a]) - xlo[a];
+                if constexpr (FR) {
+                    const float4 X = tX[FR ? t : 0];
+                    const float xbj[3] = {X.x, X.y, X.z};
+                    float Rb[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) Rb[a] = (xbj[a] - xbit[a]) - xblo[a];
+                    rr = Rb[0] * Rb[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
+                } else {
+                    rr = Rji[0] * Rji[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rji[a] * Rji[a];
+                }
+                float W, fg;
+                wendland<float>(fast_sqrt<float>(rr), Ht, inv_Hf, Wcf, W, fg);
+                const bool nb = A.w > 0.f;
+                const float volj = fabsf(A.w);
+                const float nbf = nb ? 1.f : 0.f;
+                float G[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) G[a] = -fg * Rji[a];
-                const double drho = volj * (trho[t] - rhoi);
-                const bool nb = tnb[t] != 0;
+                const float drho = volj * ((tR[t] - drhoi) - rlo);
                 int k = 0;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    const double vr = volj * Rji[a];
+                    const float vr = volj * Rji[a];
 #pragma unroll
                     for (int b2 = a; b2 < D; ++b2, ++k) {
-                        const double tt = vr * G[b2];
-                        Lm[k] -= tt;
-                        if (nb) Lnb[k] -= tt;
+                        const float tt = vr * G[b2];
+                        fLm[k] -= tt;
+                        fLnb[k] -= nbf * tt;
                     }
-                    gR[a] += drho * G[a];
+                    fgR[a] += drho * G[a];
+                    fnr[a] += nbf * volj * G[a];
                 }
-                if (nb) {
-#pragma unroll
-                    for (int a = 0; a < D; ++a) nr[a] += volj * G[a];
-                    kernsum += W;
-                    colour += volj * W;
-                    npd_acc += W;
-                }
+                fks += nbf * W;
+                fcol += nbf * volj * W;
             FJ_TILE_ENTRIES_END(cu, nl)
-        }
-        if (active) {
-            int k = 0;
+            /* the tile's FP32 partial sums into the FP64 totals */
 #pragma unroll
-            for (int a = 0; a < D; ++a)
-#pragma unroll
-                for (int b2 = a; b2 < D; ++b2, ++k) {
-                    s.L[i * D * D + a * D + b2] = Lm[k];
-                    s.L[i * D * D + b2 * D + a] = Lm[k];
-                    ax.lnb[i * D * D + a * D + b2] = Lnb[k];
-                    ax.lnb[i * D * D + b2 * D + a] = Lnb[k];
-                }
+            for (int k = 0; k < NS; ++k) {
+                Lm[k] += double(fLm[k]);
+                Lnb[k] += double(fLnb[k]);
+                fLm[k] = 0.f;
+                fLnb[k] = 0.f;
+            }
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                s.gradRho[i * D + a] = gR[a];
-                s.norm[i * D + a] = nr[a];
+                gR[a] += double(fgR[a]);
+                nr[a] += double(fnr[a]);
+                fgR[a] = 0.f;
+                fnr[a] = 0.f;
             }
-            s.kernsum[i] = kernsum;
-            s.colour[i] = colour;
+            kernsum += double(fks);
+            colour += double(fcol);
+            npd_acc += double(fks);
+            fks = 0.f;
+            fcol = 0.f;
         }
+        if (slow) {
+            double xi[D], xbi[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                xi[a] = s.x[i * D + a];
+                xbi[a] = FR ? ax.xb[i * D + a] : 0.0;
+            }
+            const int nrows = min(nl.count[i], nl.cap >> 2);
+            for (int r = 0; r < nrows; ++r) {
+                const uint64_t e4 = nl.lu4[(int64_t)r * nl.npad + p];
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t e = uint32_t(e4 >> (16 * q)) & 0xFFFFu;
+                    if (e == NO_ENTRY) continue;
+                    const int64_t j = union_slot(tg, h.seg_a, h.pre, h.nseg, entry_union_index(e, NT));
+                    double xj[D], d2, rj;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) xj[a] = s.x[j * D + a];
+                    frozen_r<D, FR>(xi, xj, xbi, ax.xb, (int32_t)j, d2, rj);
+                    npd_acc += dsph_pair<D>(c, inv_H, xi, xj, rj, ax.vol[j], s.rho[j], s.b[j] > FJSPH_PISTON, rhoi,
+                                            Lm, Lnb, gR, nr, kernsum, colour);
+                }
+            }
+        }
+        if (active) dsph_out<D>(s, ax, i, Lm, Lnb, gR, nr, kernsum, colour);
     }
     const double bs = block_reduce(npd_acc, OpSum(), lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = bs;
@@ -592,7 +792,15 @@ void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* par
                  int64_t n, int r_from_x, int with_aero_vel, double* partial, DevScalars* sc, hipStream_t st)
 {
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-    if (dim == 2) {
+    if (nl.f32) {
+        if (dim == 2) {
+            if (r_from_x) hipLaunchKernelGGL((k_dsph_m<2, false>), g, b, 0, st, c, s, a, tg, nl, partial);
+            else hipLaunchKernelGGL((k_dsph_m<2, true>), g, b, 0, st, c, s, a, tg, nl, partial);
+        } else {
+            if (r_from_x) hipLaunchKernelGGL((k_dsph_m<3, false>), g, b, 0, st, c, s, a, tg, nl, partial);
+            else hipLaunchKernelGGL((k_dsph_m<3, true>), g, b, 0, st, c, s, a, tg, nl, partial);
+        }
+    } else if (dim == 2) {
         if (r_from_x) hipLaunchKernelGGL((k_dsph_t<2, false>), g, b, 0, st, c, s, a, tg, nl, partial);
         else hipLaunchKernelGGL((k_dsph_t<2, true>), g, b, 0, st, c, s, a, tg, nl, partial);
     } else {
@@ -929,16 +1137,16 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
         const int64_t i = active ? tg.vorder[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
         double nrm[D], av[D], rd = 0.0;
-        float xit[D], xbit[D], vit[D], gri[D], nhf[D], tauf[D], pTt[D];
+        float xit[D], xlo[D], xbit[D], xblo[D], vit[D], vlo[D], gri[D], nhf[D], tauf[D], pTt[D];
         {
             double xi[D], ni[D], nhat[D], tau_hat[D];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 xi[a] = fluid ? s.x[i * D + a] : xref[a];
                 ni[a] = fluid ? s.norm[i * D + a] : 0.0;
-                xit[a] = float(xi[a] - xref[a]);
-                xbit[a] = (FR && fluid) ? float(ax.xb[i * D + a] - xbref[a]) : 0.f;
-                vit[a] = fluid ? float(s.v[i * D + a] - vref[a]) : 0.f;
+                split_rel(xi[a], xref[a], xit[a], xlo[a]);
+                split_rel((FR && fluid) ? ax.xb[i * D + a] : xbref[a], xbref[a], xbit[a], xblo[a]);
+                split_rel(fluid ? s.v[i * D + a] : vref[a], vref[a], vit[a], vlo[a]);
                 gri[a] = fluid ? float(s.gradRho[i * D + a]) : 0.f;
                 nrm[a] = 0.0;
                 av[a] = 0.0;
@@ -966,7 +1174,9 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
         bool found = false;
         const bool high = lami > 0.7;
         const float lamf = float(lami);
-        const float drhoi = float(rhoi - rref), rhoif = float(rhoi);
+        float drhoi, rlo;
+        split_rel(rhoi, rref, drhoi, rlo);
+        const float rhoif = float(rhoi);
         const float cfac = float(sqrt(c.B * c.gam / rhoi));
         Rows cu;
         cu.init(nl, p, fluid ? nl.count[i] : 0);
@@ -1003,13 +1213,13 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
                 const float pj[3] = {A.x, A.y, A.z};
                 float Rji[D], rr;
 #pragma unroll
-                for (int a = 0; a < D; ++a) Rji[a] = pj[a] - xit[a];
+                for (int a = 0; a < D; ++a) Rji[a] = (pj[a] - xit[a]) - xlo[a];
                 if constexpr (FR) {
                     const float4 X = tX[FR ? t : 0];
                     const float xbj[3] = {X.x, X.y, X.z};
                     float Rb[D];
 #pragma unroll
-                    for (int a = 0; a < D; ++a) Rb[a] = xbj[a] - xbit[a];
+                    for (int a = 0; a < D; ++a) Rb[a] = (xbj[a] - xbit[a]) - xblo[a];
                     rr = Rb[0] * Rb[0];
 #pragma unroll
                     for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
@@ -1072,11 +1282,11 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
                     float rdotg = Rji[0] * g[0];
 #pragma unroll
                     for (int a = 1; a < D; ++a) rdotg += Rji[a] * g[a];
-                    const float drho = G.w - drhoi;
+                    const float drho = (G.w - drhoi) - rlo;
                     if (E.y > 0.f) {
-                        float vdotr = (vj[0] - vit[0]) * Rji[0];
+                        float vdotr = ((vj[0] - vit[0]) - vlo[0]) * Rji[0];
 #pragma unroll
-                        for (int a = 1; a < D; ++a) vdotr += (vj[a] - vit[a]) * Rji[a];
+                        for (int a = 1; a < D; ++a) vdotr += ((vj[a] - vit[a]) - vlo[a]) * Rji[a];
                         if (!(vdotr > 0.f)) {
                             const float muij = Ht * vdotr * idist2;
                             const float cbar = 0.5f * (cfac + fast_sqrt<float>(Bg * E.x));
@@ -1242,11 +1452,169 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
     }
 }
 
+/* Mixed-precision form of k_surface2_t: FP32 pair terms on coordinates
+ * relative to the block's first target; the curvature summed in FP32 per
+ * LDS tile and in FP64 over the tiles; 32 B of LDS per candidate (48 B
+ * with build positions).  The surface-zone flag is exact; a zero normal
+ * stays exactly zero in FP32. */
+template <int D, bool FR>
+__global__ __launch_bounds__(TPB) void k_surface2_m(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                    TileGrid tg, NbrList nl, const double* __restrict__ norms_tmp)
+{
+    __shared__ float4 tA[TILE];            /* x_j - xref, V_j */
+    __shared__ float4 tN[TILE];            /* surface normal of j (Detect_Surface loop 1), surf_j */
+    __shared__ float4 tX[FR ? TILE : 1];   /* xb_j - xbref */
+    __shared__ BlockHead h;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
+    double xref[D], xbref[D];
+    {
+        const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xref[a] = s.x[i0 * D + a];
+            xbref[a] = FR ? ax.xb[i0 * D + a] : 0.0;
+        }
+    }
+    const float Ht = float(c.H), inv_Hf = float(1.0 / c.H), Wcf = float(c.Wc);
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.vorder[p] : 0;
+        const bool fluid = active && part_id[i] >= c.n_bound;
+        float xit[D], xlo[D], xbit[D], xblo[D], ni[D], Vd[D], Li[D][D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            split_rel(fluid ? s.x[i * D + a] : xref[a], xref[a], xit[a], xlo[a]);
+            split_rel((FR && fluid) ? ax.xb[i * D + a] : xbref[a], xbref[a], xbit[a], xblo[a]);
+            ni[a] = fluid ? float(norms_tmp[i * D + a]) : 0.f;
+            Vd[a] = 0.f;
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) Li[a][b2] = fluid ? float(s.L[i * D * D + a * D + b2]) : 0.f;
+        }
+        const int32_t bi = fluid ? s.b[i] : 0;
+        const bool occl = fluid && (c.aero_case == FJSPH_GISSLER) && bi == FJSPH_FREE;
+        double Vd64[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) Vd64[a] = 0.0;
+        if (occl && s.cellID[i] != -1) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                Vd64[a] = (c.aero_source == FJSPH_MESH_INFL ? s.cellV[i * D + a] : c.v_inf[a]) - s.v[i * D + a];
+                Vd[a] = float(Vd64[a]);
+            }
+        }
+        bool ni_nz = false;
+#pragma unroll
+        for (int a = 0; a < D; ++a) ni_nz = ni_nz || ni[a] != 0.f;
+        double curve = 0.0;
+        float fcurve = 0.f, qmax = 0.f;
+        bool zone = fluid && s.surf[i] == 1;
+        Rows cu;
+        cu.init(nl, p, fluid ? nl.count[i] : 0);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                float pa[3] = {0.f, 0.f, 0.f}, na[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
+                    na[a] = float(norms_tmp[(int64_t)sl * D + a]);
+                    if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
+                }
+                tA[t] = make_float4(pa[0], pa[1], pa[2], float(ax.vol[sl]));
+                tN[t] = make_float4(na[0], na[1], na[2], s.surf[sl] == 1 ? 1.f : 0.f);
+                if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
+            }
+            __syncthreads();
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
+                const float4 A = tA[t], N = tN[t];
+                const float pj[3] = {A.x, A.y, A.z}, nj[3] = {N.x, N.y, N.z};
+                float Rij[D], rr;
+#pragma unroll
+                for (int a = 0; a < D; ++a) Rij[a] = (pj[a] - xit[a]) - xlo[a];
+                if constexpr (FR) {
+                    const float4 X = tX[FR ? t : 0];
+                    const float xbj[3] = {X.x, X.y, X.z};
+                    float Rb[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) Rb[a] = (xbj[a] - xbit[a]) - xblo[a];
+                    rr = Rb[0] * Rb[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
+                } else {
+                    rr = Rij[0] * Rij[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rij[a] * Rij[a];
+                }
+                const float r = fast_sqrt<float>(rr);
+                bool nj_nz = false;
+#pragma unroll
+                for (int a = 0; a < D; ++a) nj_nz = nj_nz || nj[a] != 0.f;
+                if (ni_nz && nj_nz) {
+                    float W, fg;
+                    wendland<float>(r, Ht, inv_Hf, Wcf, W, fg);
+                    float tt = 0.f;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        float la = 0.f;
+#pragma unroll
+                        for (int b2 = 0; b2 < D; ++b2) la += Li[a][b2] * (nj[b2] - ni[b2]);
+                        tt += la * (fg * Rij[a]);
+                    }
+                    fcurve += A.w * tt;
+                }
+                if (occl) {
+                    float q = Rij[0] * Vd[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) q += Rij[a] * Vd[a];
+                    q = -q / r;
+                    if (q > qmax) qmax = q;
+                }
+                zone = zone || N.w != 0.f;
+            FJ_TILE_ENTRIES_END(cu, nl)
+            curve += double(fcurve);
+            fcurve = 0.f;
+        }
+        if (!fluid) continue;
+        const double Vdn = sqrt(dotd<D>(Vd64, Vd64));
+        const double wo = (occl && Vdn > 0.0) ? double(qmax) / Vdn : 0.0;
+        s.woccl[i] = s.lam_nb[i] < c.lam_cutoff ? fmax(0.0, fmin(wo, 1.0)) : 1.0;
+        double n64[D], nh[D];
+        load_vec<D>(norms_tmp, i, n64);
+        normalize<D>(n64, nh);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            s.norm[i * D + a] = n64[a];
+            ax.nhat[i * D + a] = nh[a];
+        }
+        s.curve[i] = curve;
+        s.norm_curve[i] = c.dx * curve;
+        s.pDist[i] = s.lam[i];
+        if (c.ale) s.surfzone[i] = zone ? 1 : 0;
+    }
+}
+
 void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
                      int64_t n, int r_from_x, const double* norms_tmp, hipStream_t st)
 {
     (void)n;
     const dim3 g((unsigned)tg.nblocks), b(TPB);
+    if (nl.f32) {
+        if (dim == 2) {
+            if (r_from_x) hipLaunchKernelGGL((k_surface2_m<2, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+            else hipLaunchKernelGGL((k_surface2_m<2, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+        } else {
+            if (r_from_x) hipLaunchKernelGGL((k_surface2_m<3, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+            else hipLaunchKernelGGL((k_surface2_m<3, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+        }
+        return;
+    }
     if (dim == 2) {
         if (r_from_x) hipLaunchKernelGGL((k_surface2_t<2, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
         else hipLaunchKernelGGL((k_surface2_t<2, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
@@ -1257,6 +1625,47 @@ void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t*
 }
 
 /* ------------------------------------------------------------ particle_shift */
+
+/* Per-slot result of particle_shift (src/Shifting.cpp:254-289) from the
+ * pair sums: du = sum_j (1 + 0.2 (W/W_dx)^4) V_j W'(r)/r R_ji, maxU2 =
+ * max_j |v_j - v_i|^2, dmin = min_j n_i . n_j over the non-boundary j. */
+template <int D>
+__device__ __forceinline__ void shift_out(const Consts& c, DevState& s, int64_t i, bool work, double lam_nb,
+                                          const double* vi, const double* nhat, double* du, double maxU2, double dmin)
+{
+    double out[D];
+    if (!work) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) out[a] = 0.0;
+    } else {
+        const double wo = dmin <= 1.0 ? acos(dmin) : 0.0;
+        const double fac = -2.0 * c.H * sqrt(dotd<D>(vi, vi));
+#pragma unroll
+        for (int a = 0; a < D; ++a) du[a] *= fac;
+        double dun[D];
+        normalize<D>(du, dun);
+        const double mag = fmin(sqrt(dotd<D>(du, du)), fmin(sqrt(maxU2) / 2.0, c.max_shift_vel));
+#pragma unroll
+        for (int a = 0; a < D; ++a) du[a] = mag * dun[a];
+        if (s.surfzone[i] == 0 && lam_nb > 0.55) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[a] = du[a];
+        } else if (wo < kPi / 12.0) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                double t = 0.0;
+#pragma unroll
+                for (int b2 = 0; b2 < D; ++b2) t += (((a == b2) ? 1.0 : 0.0) - nhat[a] * nhat[b2]) * du[b2];
+                out[a] = t;
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[a] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) s.vPert[i * D + a] = out[a];
+}
 
 /* src/Shifting.cpp:189-290.  max_j acos(n_i.n_j) is taken as acos(min_j
  * n_i.n_j) over the dot products acos accepts ([-1, 1]): NaN angles never
@@ -1338,39 +1747,132 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
                 if (dv2 > maxU2) maxU2 = dv2;
             FJ_TILE_ENTRIES_END(cu, nl)
         }
-        if (!fluid) continue;
-        double out[D];
-        if (!work) {
+        if (fluid) shift_out<D>(c, s, i, work, lam_nb, vi, nhat, du, maxU2, dmin);
+    }
+}
+
+/* Mixed-precision form of k_shift_t: FP32 pair terms on positions and
+ * velocities relative to the block's first target, the shift sum in FP32
+ * per LDS tile and in FP64 over the tiles; the normal products n_i . n_j
+ * stay in FP64 (they decide between the projected and the zero shift);
+ * 56 B of LDS per candidate (72 B with build positions). */
+template <int D, bool FR>
+__global__ __launch_bounds__(TPB) void k_shift_m(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+                                                 TileGrid tg, NbrList nl)
+{
+    __shared__ float4 tA[TILE];            /* x_j - xref, +V_j (-V_j: j is PISTON or BOUND) */
+    __shared__ float4 tV[TILE];            /* v_j - vref */
+    __shared__ double tn[D][TILE];         /* normalised normal of j */
+    __shared__ float4 tX[FR ? TILE : 1];   /* xb_j - xbref */
+    __shared__ BlockHead h;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
+    double xref[D], vref[D], xbref[D];
+    {
+        const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
 #pragma unroll
-            for (int a = 0; a < D; ++a) out[a] = 0.0;
-        } else {
-            const double wo = dmin <= 1.0 ? acos(dmin) : 0.0;
-            const double fac = -2.0 * c.H * sqrt(dotd<D>(vi, vi));
+        for (int a = 0; a < D; ++a) {
+            xref[a] = s.x[i0 * D + a];
+            vref[a] = s.v[i0 * D + a];
+            xbref[a] = FR ? ax.xb[i0 * D + a] : 0.0;
+        }
+    }
+    const float Ht = float(c.H), inv_Hf = float(1.0 / c.H), Wcf = float(c.Wc), inv_Wdx = float(1.0 / c.W_dx);
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.vorder[p] : 0;
+        const bool fluid = active && part_id[i] >= c.n_bound;
+        const double lam_nb = fluid ? s.lam_nb[i] : 0.0;
+        const bool work = fluid && !(lam_nb < 0.55 || s.b[i] == FJSPH_BUFFER);
+        double nhat[D], du[D];
+        float xit[D], xlo[D], xbit[D], xblo[D], vit[D], vlo[D], fdu[D];
 #pragma unroll
-            for (int a = 0; a < D; ++a) du[a] *= fac;
-            double dun[D];
-            normalize<D>(du, dun);
-            const double mag = fmin(sqrt(dotd<D>(du, du)), fmin(sqrt(maxU2) / 2.0, c.max_shift_vel));
-#pragma unroll
-            for (int a = 0; a < D; ++a) du[a] = mag * dun[a];
-            if (s.surfzone[i] == 0 && lam_nb > 0.55) {
-#pragma unroll
-                for (int a = 0; a < D; ++a) out[a] = du[a];
-            } else if (wo < kPi / 12.0) {
+        for (int a = 0; a < D; ++a) {
+            split_rel(work ? s.x[i * D + a] : xref[a], xref[a], xit[a], xlo[a]);
+            split_rel((FR && work) ? ax.xb[i * D + a] : xbref[a], xbref[a], xbit[a], xblo[a]);
+            split_rel(work ? s.v[i * D + a] : vref[a], vref[a], vit[a], vlo[a]);
+            nhat[a] = work ? ax.nhat[i * D + a] : 0.0;
+            du[a] = 0.0;
+            fdu[a] = 0.f;
+        }
+        float maxU2 = 0.f;
+        double dmin = 2.0;
+        Rows cu;
+        cu.init(nl, p, work ? nl.count[i] : 0);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    double t = 0.0;
-#pragma unroll
-                    for (int b2 = 0; b2 < D; ++b2) t += (((a == b2) ? 1.0 : 0.0) - nhat[a] * nhat[b2]) * du[b2];
-                    out[a] = t;
+                    pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
+                    va[a] = float(s.v[(int64_t)sl * D + a] - vref[a]);
+                    tn[a][t] = ax.nhat[(int64_t)sl * D + a];
+                    if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
                 }
-            } else {
+                const float v = float(ax.vol[sl]);
+                tA[t] = make_float4(pa[0], pa[1], pa[2], s.b[sl] > FJSPH_PISTON ? v : -v);
+                tV[t] = make_float4(va[0], va[1], va[2], 0.f);
+                if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
+            }
+            __syncthreads();
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
+                const float4 A = tA[t], V = tV[t];
+                const float pj[3] = {A.x, A.y, A.z}, vj[3] = {V.x, V.y, V.z};
+                float Rji[D], rr;
 #pragma unroll
-                for (int a = 0; a < D; ++a) out[a] = 0.0;
+                for (int a = 0; a < D; ++a) Rji[a] = (pj[a] - xit[a]) - xlo[a];
+                if constexpr (FR) {
+                    const float4 X = tX[FR ? t : 0];
+                    const float xbj[3] = {X.x, X.y, X.z};
+                    float Rb[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) Rb[a] = (xbj[a] - xbit[a]) - xblo[a];
+                    rr = Rb[0] * Rb[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
+                } else {
+                    rr = Rji[0] * Rji[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rji[a] * Rji[a];
+                }
+                float W, fg;
+                wendland<float>(fast_sqrt<float>(rr), Ht, inv_Hf, Wcf, W, fg);
+                const float w = W * inv_Wdx;
+                const float w2 = w * w;
+                const float f = (1.f + 0.2f * (w2 * w2)) * fg * fabsf(A.w);
+                float dv2 = 0.f;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    fdu[a] += f * Rji[a];
+                    const float dv = (vj[a] - vit[a]) - vlo[a];
+                    dv2 += dv * dv;
+                }
+                if (A.w > 0.f) {
+                    double d = 0.0;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) d += nhat[a] * tn[a][t];
+                    if (d >= -1.0 && d <= 1.0 && d < dmin) dmin = d;
+                }
+                maxU2 = fmaxf(maxU2, dv2);
+            FJ_TILE_ENTRIES_END(cu, nl)
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                du[a] += double(fdu[a]);
+                fdu[a] = 0.f;
             }
         }
+        if (!fluid) continue;
+        double vi[D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) s.vPert[i * D + a] = out[a];
+        for (int a = 0; a < D; ++a) vi[a] = work ? s.v[i * D + a] : 0.0;
+        shift_out<D>(c, s, i, work, lam_nb, vi, nhat, du, double(maxU2), dmin);
     }
 }
 
@@ -1379,6 +1881,16 @@ void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
 {
     (void)n;
     const dim3 g((unsigned)tg.nblocks), b(TPB);
+    if (nl.f32) {
+        if (dim == 2) {
+            if (r_from_x) hipLaunchKernelGGL((k_shift_m<2, false>), g, b, 0, st, c, s, a, part_id, tg, nl);
+            else hipLaunchKernelGGL((k_shift_m<2, true>), g, b, 0, st, c, s, a, part_id, tg, nl);
+        } else {
+            if (r_from_x) hipLaunchKernelGGL((k_shift_m<3, false>), g, b, 0, st, c, s, a, part_id, tg, nl);
+            else hipLaunchKernelGGL((k_shift_m<3, true>), g, b, 0, st, c, s, a, part_id, tg, nl);
+        }
+        return;
+    }
     if (dim == 2) {
         if (r_from_x) hipLaunchKernelGGL((k_shift_t<2, false>), g, b, 0, st, c, s, a, part_id, tg, nl);
         else hipLaunchKernelGGL((k_shift_t<2, true>), g, b, 0, st, c, s, a, part_id, tg, nl);
@@ -1620,14 +2132,22 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
         const bool fluid = active && part_id[i] >= c.n_bound;
         double xi[D], vi[D], xbi[D];
         T xit[D], vit[D], xbit[D], vpi[D];
+        float xlo[D], vlo[D], xblo[D]; /* FP32 rounding residuals (split_rel) */
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            xi[a] = active ? s.x[i * D + a] : 0.0;
-            vi[a] = active ? s.v[i * D + a] : 0.0;
-            xbi[a] = (FR && active) ? ax.xb[i * D + a] : 0.0;
-            xit[a] = T(xi[a] - xref[a]);
-            vit[a] = T(vi[a] - vref[a]);
-            xbit[a] = T(xbi[a] - xbref[a]);
+            xi[a] = active ? s.x[i * D + a] : xref[a];
+            vi[a] = active ? s.v[i * D + a] : vref[a];
+            xbi[a] = (FR && active) ? ax.xb[i * D + a] : xbref[a];
+            if constexpr (F32) {
+                split_rel(xi[a], xref[a], xit[a], xlo[a]);
+                split_rel(vi[a], vref[a], vit[a], vlo[a]);
+                split_rel(xbi[a], xbref[a], xbit[a], xblo[a]);
+            } else {
+                xit[a] = xi[a];
+                vit[a] = vi[a];
+                xbit[a] = xbi[a];
+                xlo[a] = vlo[a] = xblo[a] = 0.f;
+            }
             vpi[a] = (fluid && c.ale) ? T(s.vPert[i * D + a]) : T(0);
         }
         const double rhoi = fluid ? s.rho[i] : 1.0, mi = fluid ? s.m[i] : 1.0;
@@ -1688,14 +2208,14 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
                     irj = X.w;
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
-                        Rji[a] = pj[a] - xit[a];
-                        Vji[a] = vj[a] - vit[a];
+                        Rji[a] = (pj[a] - xit[a]) - xlo[a];
+                        Vji[a] = (vj[a] - vit[a]) - vlo[a];
                         vjr[a] = vj[a];
                     }
                     if constexpr (FR) {
                         T Rb[D];
 #pragma unroll
-                        for (int a = 0; a < D; ++a) Rb[a] = xbj[a] - xbit[a];
+                        for (int a = 0; a < D; ++a) Rb[a] = (xbj[a] - xbit[a]) - xblo[a];
                         rr = Rb[0] * Rb[0];
 #pragma unroll
                         for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
