@@ -432,8 +432,6 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         tg.order = ctx->order;
         tg.vorder = ctx->vorder;
         tg.pos = ctx->pos;
-        for (int k = 0; k < 3; ++k) tg.lo[k] = lo[k];
-        tg.inv_cs = inv_cs;
         tg.cell = ctx->cell;
         if (tg.nblocks > 0x7fffffff) throw std::string("tile grid too large");
         if (tg.nblocks > ctx->partial_cap) {

@@ -77,10 +77,8 @@ struct TileGrid {
     const int32_t* start; /* cell_start, ncell + 1 */
     const int32_t* order; /* slots in cell order (the candidate unions) */
     const int32_t* cell;  /* cell of every slot */
-    int32_t* vorder;      /* target order: order[] regrouped per block chunk */
+    int32_t* vorder;      /* target order of the lists (order[] at the build) */
     int32_t* pos;         /* position of every slot in vorder[] */
-    double lo[3];         /* grid origin */
-    double inv_cs;        /* 1 / cell edge */
 };
 
 /* Per-particle quantities derived from the current rho, p, m, b (and vPert),

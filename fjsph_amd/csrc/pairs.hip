@@ -3,18 +3,17 @@
  *
  * Work decomposition: a block is a chunk of C consecutive cells of one cell
  * row; its targets are the particles of those cells (positions [p0, p1) of
- * the cell-sorted order[] array, taken in the target order vorder[], which
- * is order[] itself unless FJ_GROUP_TARGETS regroups them) and its candidate
- * union is the 3^(D-1)
- * neighbour rows over cells [cx0 - 1, cx1].  The union is staged in LDS in
- * tiles of TILE records with coalesced loads; each lane walks its own
- * neighbour list tile by tile.  Pair terms never scatter: every pass is
- * owner-computes.
+ * the cell-sorted order[] array, recorded as the target order vorder[]) and
+ * its candidate union is the 3^(D-1) neighbour rows over cells
+ * [cx0 - 1, cx1].  The union is staged in LDS in tiles of TILE records,
+ * dealt to the tiles round robin in groups of 4 (tile_union_index); each
+ * lane walks its own neighbour list tile by tile.  Pair terms never
+ * scatter: every pass is owner-computes.
  *
  * Neighbour lists (built by k_build_t, read by every pass): rows of four
- * 16-bit union indices, row r of the target at position p of vorder[]
- * stored at lu4[r*npad + p].  Within a row the indices ascend and lie in one union
- * tile; unused entries hold NO_ENTRY.  The 64 targets of a wave hold the
+ * 16-bit entries k*TILE + t (slot t of tile k), row r of the target at
+ * position p of vorder[] stored at lu4[r*npad + p].  Within a row the
+ * entries ascend and lie in one tile; unused entries hold NO_ENTRY.  The 64 targets of a wave hold the
  * same number of rows for every tile (rows padded to the wave's maximum), so
  * a wave walks its lists in lock-step: every row load is one coalesced
  * 512-byte read.  Distances are not stored: the reference's frozen distance
@@ -200,14 +199,6 @@ struct Rows {
         (cu).advance(nl);           \
     }
 
-/* Regroup each chunk's targets by the (y, z) half of their cell (1) or keep
- * the cell order (0).  Measured on the 4M jet column: regrouping made
- * dSPH 8% faster but Detect_Surface, shifting and the build slower (their
- * per-target loads coalesce less), 1% slower overall, so it is off. */
-#ifndef FJ_GROUP_TARGETS
-#define FJ_GROUP_TARGETS 0
-#endif
-
 __device__ __forceinline__ int wave_max(int v)
 {
 #pragma unroll
@@ -215,15 +206,31 @@ __device__ __forceinline__ int wave_max(int v)
     return v;
 }
 
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 /* ------------------------------------------------------------ list build */
 
+/* A candidate is a neighbour when its FP64 squared distance from the
+ * target is below sr (src/Neighbours.cpp:44, nanoflann's strict test).
+ * The test runs first in FP32 on coordinates relative to the block's first
+ * target (split_rel): with every coordinate within L of that point, the
+ * FP32 squared distance near sr is off by at most about 2^-23 (L / H) sr
+ * plus a few ulp, and L < (C + 2) cs sqrt(3) < 1000 H for any tile grid
+ * (C <= 256 cells of size 2H), so below (1 - 1e-3) sr it is surely in,
+ * above (1 + 1e-3) sr surely out, and only the thin shell between is
+ * decided by the exact FP64 test.  The lists are those of the FP64 test. */
 template <int D>
 __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, TileGrid tg, double sr, NbrList nl,
                                                  DevScalars* sc)
 {
     __shared__ double tp[D][TILE];
+    __shared__ float4 tf[(TILE + 31) / 32 * 32]; /* read in chunks of 32 */
     __shared__ int32_t tsl[TILE];
-    __shared__ uint64_t scan[TPB];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -233,58 +240,36 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
         if (threadIdx.x == 0) atomicMax(&sc->overflow, U);
         return;
     }
+    double xref[D];
+    {
+        const int64_t i0 = tg.order[h.p0];
+#pragma unroll
+        for (int a = 0; a < D; ++a) xref[a] = x[i0 * D + a];
+    }
+    const float sr_lo = float(sr * (1.0 - 1e-3)), sr_hi = float(sr * (1.0 + 1e-3));
     const int rcap = nl.cap >> 2;
-    const int64_t brow = blockIdx.x / tg.nchunk;
-    const int cy = (int)(brow % tg.dims[1]);
-    const int cz = (D == 3) ? (int)(brow / tg.dims[1]) : 0;
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
-        /* target order of this chunk: stable regrouping by the (y, z) half of
-         * the cell the target sits in, so a wave's targets reach the same
-         * neighbour rows and their per-tile list lengths are alike */
-        if (!FJ_GROUP_TARGETS) {
-            if (active) {
-                const int32_t sl = tg.order[p];
-                tg.vorder[p] = sl;
-                tg.pos[sl] = p;
-            }
-            __syncthreads();
-        } else {
-            const int64_t sl = active ? tg.order[p] : 0;
-            int key = 0;
-            if (active) {
-                const double fy = (x[sl * D + 1] - tg.lo[1]) * tg.inv_cs - cy;
-                key = fy >= 0.5 ? 1 : 0;
-                if (D == 3) {
-                    const double fz = (x[sl * D + 2] - tg.lo[2]) * tg.inv_cs - cz;
-                    key += fz >= 0.5 ? 2 : 0;
-                }
-            }
-            const uint64_t mine = active ? (1ull << (16 * key)) : 0ull;
-            uint64_t incl = mine;
-            scan[threadIdx.x] = incl;
-            __syncthreads();
-            for (int off = 1; off < TPB; off <<= 1) {
-                const uint64_t o = threadIdx.x >= off ? scan[threadIdx.x - off] : 0ull;
-                __syncthreads();
-                incl += o;
-                scan[threadIdx.x] = incl;
-                __syncthreads();
-            }
-            const uint64_t tot = scan[TPB - 1], excl = incl - mine;
-            int rank = int((excl >> (16 * key)) & 0xFFFFull);
-            for (int k = 0; k < key; ++k) rank += int((tot >> (16 * k)) & 0xFFFFull);
-            if (active) {
-                tg.vorder[pb + rank] = (int32_t)sl;
-                tg.pos[sl] = pb + rank;
-            }
-            __syncthreads();
+        if (active) {
+            const int32_t sl = tg.order[p];
+            tg.vorder[p] = sl;
+            tg.pos[sl] = p;
         }
-        const int64_t i = active ? tg.vorder[p] : 0;
+        const int64_t i = active ? tg.order[p] : 0;
         double xi[D];
+        float xit[3] = {0.f, 0.f, 0.f}, xlo[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-        for (int a = 0; a < D; ++a) xi[a] = active ? x[i * D + a] : 0.0;
+        for (int a = 0; a < D; ++a) {
+            xi[a] = active ? x[i * D + a] : xref[a];
+            split_rel(xi[a], xref[a], xit[a], xlo[a]);
+        }
+        /* x-cells within reach of the wave's targets (the block's targets
+         * share one cell row; a candidate two cells away in x is farther
+         * than the cell size 2H) */
+        const int cxi = active ? (int)(tg.cell[i] % tg.dims[0]) : 0;
+        const int wlo = wave_min(active ? cxi : 0x7fffffff) - 1;
+        const int whi = wave_max(active ? cxi : -0x7fffffff) + 1;
         int nrow = 0, nreal = 0;
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
@@ -292,25 +277,81 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
             for (int t = threadIdx.x; t < nt; t += TPB) {
                 const int u = tile_union_index(tk, t, NT);
                 if (u >= U) {
+                    /* padding: far away, never a neighbour */
                     tsl[t] = -1;
+                    tf[t] = make_float4(1e30f, 1e30f, 1e30f, -2.f);
                     continue;
                 }
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
                 tsl[t] = sl;
+                float pa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-                for (int a = 0; a < D; ++a) tp[a][t] = x[(int64_t)sl * D + a];
+                for (int a = 0; a < D; ++a) {
+                    tp[a][t] = x[(int64_t)sl * D + a];
+                    pa[a] = float(tp[a][t] - xref[a]);
+                }
+                tf[t] = make_float4(pa[0], pa[1], pa[2], float(tg.cell[sl] % tg.dims[0]));
             }
             __syncthreads();
+            /* groups of 4 slots with a candidate in reach: one bit per
+             * group, wave-uniform (tiles hold at most 128 groups) */
+            uint64_t gmask[2];
+#pragma unroll
+            for (int hgrp = 0; hgrp < 2; ++hgrp) {
+                const int g = hgrp * 64 + (threadIdx.x & 63);
+                bool reach = false;
+                if (4 * g < nt) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float cx = tf[4 * g + e].w;
+                        reach = reach || (cx >= float(wlo) && cx <= float(whi));
+                    }
+                }
+                gmask[hgrp] = __ballot(reach);
+            }
             int rows_t = 0;
             if (active) {
                 int q = 0;
                 uint64_t cu = ~0ull;
-                for (int t = 0; t < nt; ++t) {
-                    double xj[D];
+                /* 32 candidates at a time: branch-free FP32 tests into an
+                 * "in" and an "uncertain" bit mask, the uncertain ones then
+                 * tested in FP64, the entries emitted in ascending order */
+                for (int c0 = 0; c0 < nt; c0 += 32) {
+                    const uint32_t gm = uint32_t(gmask[c0 >> 8] >> ((c0 >> 2) & 63)) & 0xFFu;
+                    if (gm == 0u) continue;
+                    uint32_t win = 0u, wsh = 0u;
 #pragma unroll
-                    for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
-                    const double d2 = exact_dist2<D>(xi, xj);
-                    if (d2 < sr && tsl[t] != (int32_t)i && tsl[t] >= 0) {
+                    for (int gj = 0; gj < 8; ++gj) {
+                        if (!((gm >> gj) & 1u)) continue;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int k = 4 * gj + e;
+                            const float4 P = tf[c0 + k];
+                            const float d0 = (P.x - xit[0]) - xlo[0];
+                            const float d1 = (P.y - xit[1]) - xlo[1];
+                            const float d2f = (P.z - xit[2]) - xlo[2];
+                            const float rr = d0 * d0 + d1 * d1 + d2f * d2f;
+                            win |= (rr < sr_lo ? 1u : 0u) << k;
+                            wsh |= (rr <= sr_hi ? 1u : 0u) << k;
+                        }
+                    }
+                    const int rem = nt - c0;
+                    const uint32_t valid = rem >= 32 ? ~0u : ((1u << rem) - 1u);
+                    win &= valid;
+                    wsh &= valid & ~win;
+                    while (wsh) {
+                        const int k = __builtin_ctz(wsh);
+                        wsh &= wsh - 1u;
+                        double xj[D];
+#pragma unroll
+                        for (int a = 0; a < D; ++a) xj[a] = tp[a][c0 + k];
+                        if (exact_dist2<D>(xi, xj) < sr) win |= 1u << k;
+                    }
+                    while (win) {
+                        const int k = __builtin_ctz(win);
+                        win &= win - 1u;
+                        const int t = c0 + k;
+                        if (tsl[t] == (int32_t)i) continue;
                         cu = (cu >> 16) | (uint64_t(base + t) << 48);
                         ++nreal;
                         if (++q == 4) {
