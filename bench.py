@@ -168,7 +168,6 @@ def main():
     ap.add_argument("--mesh-points", type=int, default=300_000)
     ap.add_argument("--train-samples", type=int, default=1_000_000)
     ap.add_argument("--train-steps", type=int, default=3000)
-    ap.add_argument("--train-device", default=None, help="torch device of the surrogate training (default: GPU)")
     ap.add_argument("--cpu-particles", type=int, default=60_000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -204,8 +203,7 @@ def main():
             from fjsph_amd.engine import Engine
             lab = Engine(case.settings)
             lab.set_mesh(case.mesh)
-            sg = surrogate.train(case.mesh, n_samples=args.train_samples, steps=args.train_steps, engine=lab,
-                                 device=args.train_device)
+            sg = surrogate.train(case.mesh, lab, n_samples=args.train_samples, steps=args.train_steps)
             lab.close()
             aero_info["train_seconds"] = time.perf_counter() - t0
             case.settings.aero_source = abi.SURROGATE

@@ -1437,6 +1437,177 @@ fjsph_status fjsph_set_surrogate(fjsph_ctx* ctx, int32_t n_hidden, const float* 
     });
 }
 
+extern "C++" {
+namespace {
+
+/* Device buffers of surrogate training, freed on scope exit. */
+struct TrainBufs {
+    std::vector<void*> ptrs;
+    template <typename T>
+    T* alloc(size_t count)
+    {
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+    ~TrainBufs()
+    {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+int64_t mlp_nw(int D, int L) { return int64_t(64) * D + int64_t(L - 1) * 64 * 64 + int64_t(D + 2) * 64; }
+int64_t mlp_nb(int D, int L) { return int64_t(L) * 64 + D + 2; }
+
+/* One forward/backward pass over the batch idx (on the device) with the
+ * parameters p (weights then biases): gradients into g, the batch's sum of
+ * squared errors into *sse (device). */
+void mlp_pass(fjsph_ctx* ctx, MlpTrain& m, float* p, float* g, float* sse, void* wf, void* wt)
+{
+    const int D = ctx->s.dim, L = m.n_hidden;
+    const int64_t nwaves = m.batch / 64;
+    launch_mlp_pack(D, p, L, wf, wt, ctx->st);
+    m.wf = wf;
+    m.wt = wt;
+    m.b = p + m.nw;
+    m.bo = p + m.nw + int64_t(L) * 64;
+    launch_mlp_train(D, m, ctx->st);
+    launch_mlp_reduce(m.dWp, nwaves, m.nw, m.nw, g, ctx->st);
+    launch_mlp_reduce(m.dbp, nwaves, m.nb, m.nb, g + m.nw, ctx->st);
+    launch_mlp_reduce(m.lossp, nwaves, 1, 1, sse, ctx->st);
+}
+
+/* Scratch of a pass over a batch of B samples. */
+void mlp_scratch(TrainBufs& tb, MlpTrain& m, int D, int L, int32_t B)
+{
+    m.n_hidden = L;
+    m.batch = B;
+    m.nw = mlp_nw(D, L);
+    m.nb = mlp_nb(D, L);
+    const int64_t nwaves = B / 64;
+    m.Z = tb.alloc<float>(size_t(L) * B * 64);
+    m.H = tb.alloc<__bf16>(size_t(L + 1) * B * 64);
+    m.dWp = tb.alloc<float>(size_t(nwaves) * m.nw);
+    m.dbp = tb.alloc<float>(size_t(nwaves) * m.nb);
+    m.lossp = tb.alloc<float>(size_t(nwaves));
+}
+
+}  // namespace
+}  // extern "C++"
+
+fjsph_status fjsph_surrogate_grad(fjsph_ctx* ctx, int32_t n_hidden, int32_t batch, const float* xn, const float* tn,
+                                  const float* weights, const float* biases, float* dweights, float* dbiases,
+                                  double* loss)
+{
+    if (!ctx || n_hidden < 1 || n_hidden > 8 || batch <= 0 || batch % 64 != 0 || !xn || !tn || !weights ||
+        !biases || !dweights || !dbiases)
+        return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        const int D = ctx->s.dim, NO = D + 2, L = n_hidden;
+        TrainBufs tb;
+        MlpTrain m{};
+        mlp_scratch(tb, m, D, L, batch);
+        std::vector<float> X(size_t(batch) * 4, 0.f), T(size_t(batch) * 8, 0.f);
+        std::vector<int32_t> idx(batch);
+        for (int32_t j = 0; j < batch; ++j) {
+            for (int a = 0; a < D; ++a) X[size_t(j) * 4 + a] = xn[size_t(j) * D + a];
+            for (int a = 0; a < NO; ++a) T[size_t(j) * 8 + a] = tn[size_t(j) * NO + a];
+            idx[j] = j;
+        }
+        float* dX = tb.alloc<float>(X.size());
+        float* dT = tb.alloc<float>(T.size());
+        int32_t* didx = tb.alloc<int32_t>(idx.size());
+        float* p = tb.alloc<float>(size_t(m.nw + m.nb));
+        float* g = tb.alloc<float>(size_t(m.nw + m.nb));
+        float* sse = tb.alloc<float>(1);
+        void* wf = tb.alloc<uint16_t>(size_t(mlp_tiles_fwd(L)) * 64 * 8);
+        void* wt = tb.alloc<uint16_t>(size_t(mlp_tiles_bwd(L)) * 64 * 8);
+        sync(ctx);
+        HIPCHK(hipMemcpy(dX, X.data(), X.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dT, T.data(), T.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(didx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(p, weights, size_t(m.nw) * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(p + m.nw, biases, size_t(m.nb) * 4, hipMemcpyHostToDevice));
+        m.X = dX;
+        m.T = dT;
+        m.idx = didx;
+        mlp_pass(ctx, m, p, g, sse, wf, wt);
+        sync(ctx);
+        HIPCHK(hipMemcpy(dweights, g, size_t(m.nw) * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(dbiases, g + m.nw, size_t(m.nb) * 4, hipMemcpyDeviceToHost));
+        float s = 0.f;
+        HIPCHK(hipMemcpy(&s, sse, 4, hipMemcpyDeviceToHost));
+        if (loss) *loss = double(s) / (double(batch) * NO);
+    });
+}
+
+fjsph_status fjsph_surrogate_train(fjsph_ctx* ctx, int64_t n, const double* points, const double* values,
+                                   int32_t n_hidden, int32_t steps, int32_t batch, double lr, uint64_t seed,
+                                   const double* in_shift, const double* in_scale, const double* out_shift,
+                                   const double* out_scale, float* weights, float* biases, double* loss)
+{
+    if (!ctx || n <= 0 || n > 0x7fffffff || !points || !values || n_hidden < 1 || n_hidden > 8 || steps < 0 ||
+        batch <= 0 || batch % 64 != 0 || !(lr > 0.0) || !in_shift || !in_scale || !out_shift || !out_scale ||
+        !weights || !biases)
+        return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        const int D = ctx->s.dim, NO = D + 2, L = n_hidden;
+        for (int a = 0; a < D; ++a)
+            if (!(in_scale[a] != 0.0)) throw std::string("surrogate_train: zero input scale");
+        for (int a = 0; a < NO; ++a)
+            if (!(out_scale[a] != 0.0)) throw std::string("surrogate_train: zero output scale");
+        TrainBufs tb;
+        MlpTrain m{};
+        mlp_scratch(tb, m, D, L, batch);
+        std::vector<float> X(size_t(n) * 4, 0.f), T(size_t(n) * 8, 0.f);
+        for (int64_t j = 0; j < n; ++j) {
+            for (int a = 0; a < D; ++a) X[size_t(j) * 4 + a] = float((points[j * D + a] - in_shift[a]) / in_scale[a]);
+            for (int a = 0; a < NO; ++a)
+                T[size_t(j) * 8 + a] = float((values[j * NO + a] - out_shift[a]) / out_scale[a]);
+        }
+        const int64_t np = m.nw + m.nb;
+        float* dX = tb.alloc<float>(X.size());
+        float* dT = tb.alloc<float>(T.size());
+        int32_t* didx = tb.alloc<int32_t>(size_t(batch));
+        float* p = tb.alloc<float>(size_t(np));
+        float* g = tb.alloc<float>(size_t(np));
+        float* m1 = tb.alloc<float>(size_t(np));
+        float* m2 = tb.alloc<float>(size_t(np));
+        float* sse = tb.alloc<float>(size_t(std::max(steps, 1)));
+        void* wf = tb.alloc<uint16_t>(size_t(mlp_tiles_fwd(L)) * 64 * 8);
+        void* wt = tb.alloc<uint16_t>(size_t(mlp_tiles_bwd(L)) * 64 * 8);
+        sync(ctx);
+        HIPCHK(hipMemcpy(dX, X.data(), X.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dT, T.data(), T.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(p, weights, size_t(m.nw) * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(p + m.nw, biases, size_t(m.nb) * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemsetAsync(m1, 0, size_t(np) * 4, ctx->st));
+        HIPCHK(hipMemsetAsync(m2, 0, size_t(np) * 4, ctx->st));
+        m.X = dX;
+        m.T = dT;
+        m.idx = didx;
+        /* Adam (beta 0.9, 0.999, eps 1e-8) with a cosine learning-rate
+         * schedule from lr to 0 over the steps */
+        const double b1 = 0.9, b2 = 0.999;
+        for (int32_t t = 1; t <= steps; ++t) {
+            launch_mlp_batch(seed, t, n, batch, didx, ctx->st);
+            mlp_pass(ctx, m, p, g, sse + (t - 1), wf, wt);
+            const double lr_t = lr * 0.5 * (1.0 + std::cos(3.14159265358979323846 * double(t - 1) / double(steps)));
+            launch_mlp_adam(p, g, m1, m2, np, float(lr_t), float(b1), float(b2), 1e-8f,
+                            float(1.0 - std::pow(b1, t)), float(1.0 - std::pow(b2, t)), ctx->st);
+        }
+        sync(ctx);
+        HIPCHK(hipMemcpy(weights, p, size_t(m.nw) * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(biases, p + m.nw, size_t(m.nb) * 4, hipMemcpyDeviceToHost));
+        if (loss && steps > 0) {
+            std::vector<float> l(steps);
+            HIPCHK(hipMemcpy(l.data(), sse, size_t(steps) * 4, hipMemcpyDeviceToHost));
+            for (int32_t t = 0; t < steps; ++t) loss[t] = double(l[t]) / (double(batch) * NO);
+        }
+    });
+}
+
 fjsph_status fjsph_surrogate_eval(fjsph_ctx* ctx, int64_t n, const double* points, double* out)
 {
     if (!ctx || n < 0 || (n > 0 && (!points || !out))) return FJSPH_ERR_ARG;

@@ -130,6 +130,24 @@ struct Surrogate {
     double out_shift[5], out_scale[5];
 };
 
+/* Device buffers of one surrogate training step (surrogate_train.hip). */
+struct MlpTrain {
+    const float* X;      /* samples: normalised inputs, 4 per sample (D used) */
+    const float* T;      /* normalised targets, 8 per sample (D + 2 used)     */
+    const int32_t* idx;  /* batch: sample of each position                    */
+    const void* wf;      /* forward weight tiles (BF16, A layout)             */
+    const void* wt;      /* backward tiles of the transposes                  */
+    const float* b;      /* hidden biases, n_hidden x 64                      */
+    const float* bo;     /* output biases (D + 2)                             */
+    float* Z;            /* pre-activations, n_hidden x batch x 64            */
+    __bf16* H;           /* activations (H_0 = input), (n_hidden + 1) x batch x 64 */
+    float* dWp;          /* per-wave weight-gradient partials, nw each        */
+    float* dbp;          /* per-wave bias-gradient partials, nb each          */
+    float* lossp;        /* per-wave sums of squared errors                   */
+    int64_t nw, nb;
+    int32_t n_hidden, batch;
+};
+
 /* Scalars kept on the device between kernels. */
 struct DevScalars {
     double npd;       /* numerical particle density of the last dSPH pass */
@@ -185,6 +203,14 @@ void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double*
 /* Surrogate on the aero particles of s (mode 0) or on points (mode 1). */
 void launch_surrogate(int dim, const Consts& c, DevState s, const int32_t* part_id, Surrogate sg, int64_t n, int mode,
                       const double* pts, double* out, hipStream_t st);
+void launch_mlp_batch(uint64_t seed, int64_t step, int64_t n, int32_t batch, int32_t* idx, hipStream_t st);
+void launch_mlp_train(int dim, const MlpTrain& m, hipStream_t st);
+void launch_mlp_reduce(const float* part, int64_t nwaves, int64_t stride, int64_t n, float* grad, hipStream_t st);
+void launch_mlp_adam(float* p, const float* g, float* m1, float* m2, int64_t n, float lr, float b1, float b2,
+                     float eps, float c1, float c2, hipStream_t st);
+int64_t mlp_tiles_fwd(int n_hidden);
+int64_t mlp_tiles_bwd(int n_hidden);
+void launch_mlp_pack(int dim, const float* w, int32_t n_hidden, void* wf, void* wt, hipStream_t st);
 /* The mesh field at points: containing cell (-1 if none) and (vel, p, rho). */
 void launch_mesh_eval(int dim, DevMesh m, int64_t n, const double* pts, int32_t* cell, double* vals, hipStream_t st);
 /* FindCell on the aero particles of s (mesh.hip). */

@@ -6,54 +6,13 @@
  * width 64, GELU in its erf form, xn = (x - in_shift) / in_scale, and the
  * outputs (gas velocity, pressure, density) = y * out_scale + out_shift.
  *
- * Each wave evaluates 64 points.  A layer is a 64 x 64 (features x points)
- * product formed from 16 x 16 output tiles with v_mfma_f32_16x16x32_bf16:
- * BF16 operands, FP32 accumulation.  Operand layout of that instruction
- * (lane l = 0..63):
- *     A[i][k]  (16 x 32): lane i + 16*(k/8), element k%8
- *     B[k][j]  (32 x 16): lane j + 16*(k/8), element k%8
- *     D[i][j]  (16 x 16): lane j + 16*(i/4), element i%4
- * The weights are stored on the host already in the A layout (8 BF16 per
- * lane per tile, one 16-byte load); activations sit in LDS as BF16 rows per
- * point, so a lane reads its 8 B-operand values with one 16-byte LDS read.
+ * Each wave evaluates 64 points, one layer at a time with the MFMA layer
+ * product of mlp_common.hpp; the weights are stored on the host already in
+ * the A layout, the activations sit in LDS as BF16 rows per point.
  */
-#include "device_common.hpp"
+#include "mlp_common.hpp"
 
 namespace fj {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-constexpr int SW = 64;       /* hidden width */
-constexpr int SROW = SW + 8; /* LDS row of one point, padded against bank conflicts */
-
-__device__ __forceinline__ float gelu_erf(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-
-/* layer input: KC chunks of 32 features from the LDS rows; output: MT tiles
- * of 16 features (registers acc[MT][4 column tiles]) */
-template <int MT, int KC>
-__device__ __forceinline__ void mlp_layer(const bf16x8* __restrict__ wA, const __bf16* act, f32x4 (&acc)[4][4],
-                                          int lane)
-{
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kc = 0; kc < KC; ++kc) {
-        bf16x8 b[4];
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-            b[nt] = *(const bf16x8*)(act + (nt * 16 + (lane & 15)) * SROW + kc * 32 + 8 * (lane >> 4));
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            const bf16x8 a = wA[(mt * KC + kc) * 64 + lane];
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
-        }
-    }
-}
 
 /* bias + GELU, back to the LDS rows (after every lane has read them) */
 __device__ __forceinline__ void store_hidden(const f32x4 (&acc)[4][4], const float* __restrict__ bias, __bf16* act,

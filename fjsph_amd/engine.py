@@ -66,6 +66,9 @@ SIGNATURES = {
     "fjsph_mesh_eval": (C.c_int, [_VP, C.c_int64, _D, _I32, _D]),
     "fjsph_set_surrogate": (C.c_int, [_VP, C.c_int32, _F32, _F32, _D, _D, _D, _D]),
     "fjsph_surrogate_eval": (C.c_int, [_VP, C.c_int64, _D, _D]),
+    "fjsph_surrogate_train": (C.c_int, [_VP, C.c_int64, _D, _D, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                        C.c_uint64, _D, _D, _D, _D, _F32, _F32, _D]),
+    "fjsph_surrogate_grad": (C.c_int, [_VP, C.c_int32, C.c_int32, _F32, _F32, _F32, _F32, _F32, _F32, _D]),
 }
 
 
@@ -261,6 +264,35 @@ class Engine:
         out = np.zeros((pts.shape[0], self.dim + 2))
         self._chk(self.lib.fjsph_surrogate_eval(self.h, pts.shape[0], _ptr(pts), _ptr(out)), "surrogate_eval")
         return out
+
+    def surrogate_train(self, points, values, n_hidden, weights, biases, in_shift, in_scale, out_shift, out_scale,
+                        steps, batch, lr, seed):
+        """fjsph_surrogate_train: returns (weights, biases, per-step loss)."""
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        vals = np.ascontiguousarray(values, dtype=np.float64)
+        w = np.array(weights, dtype=np.float32)
+        b = np.array(biases, dtype=np.float32)
+        sh = [np.ascontiguousarray(a, dtype=np.float64) for a in (in_shift, in_scale, out_shift, out_scale)]
+        loss = np.zeros(max(int(steps), 1))
+        self._chk(self.lib.fjsph_surrogate_train(self.h, pts.shape[0], _ptr(pts), _ptr(vals), int(n_hidden),
+                                                 int(steps), int(batch), float(lr), int(seed) & (2 ** 64 - 1),
+                                                 _ptr(sh[0]), _ptr(sh[1]), _ptr(sh[2]), _ptr(sh[3]), _ptr(w, _F32),
+                                                 _ptr(b, _F32), _ptr(loss)), "surrogate_train")
+        return w, b, loss[:int(steps)]
+
+    def surrogate_grad(self, n_hidden, weights, biases, xn, tn):
+        """fjsph_surrogate_grad: (dweights, dbiases, loss) of one batch of
+        normalised samples (batch a multiple of 64)."""
+        x = np.ascontiguousarray(xn, dtype=np.float32)
+        t = np.ascontiguousarray(tn, dtype=np.float32)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        b = np.ascontiguousarray(biases, dtype=np.float32)
+        dw, db = np.zeros_like(w), np.zeros_like(b)
+        loss = np.zeros(1)
+        self._chk(self.lib.fjsph_surrogate_grad(self.h, int(n_hidden), x.shape[0], _ptr(x, _F32), _ptr(t, _F32),
+                                                _ptr(w, _F32), _ptr(b, _F32), _ptr(dw, _F32), _ptr(db, _F32),
+                                                _ptr(loss)), "surrogate_grad")
+        return dw, db, float(loss[0])
 
     def neighbour_stats(self):
         pairs = np.zeros(1, np.int64)

@@ -1,12 +1,14 @@
 """Training of the MLP surrogate of the aerodynamic mesh field.
 
-The engine evaluates the network in its HIP kernel (BF16 MFMA, FP32
-accumulation) once fjsph_set_surrogate has the weights; this module only
-prepares them.  Samples are points drawn uniformly in the mesh's bounding
-box, labelled with the values of the cell that contains them (the same
-piecewise-constant field the mesh lookup returns).  The network maps the
-normalised position to the normalised (velocity, pressure, density):
-n_hidden layers of width 64 with GELU (erf form), trained in FP32 with Adam.
+The engine trains the network with its own HIP kernels (BF16 MFMA forward
+and backward passes with FP32 accumulation, Adam on FP32 master weights)
+and evaluates it in its inference kernel once fjsph_set_surrogate has the
+weights; this module draws the samples and sets up the call.  Samples are
+points drawn uniformly in the mesh's bounding box, labelled with the
+values of the cell that contains them (the same piecewise-constant field
+the mesh lookup returns).  The network maps the normalised position to the
+normalised (velocity, pressure, density): n_hidden layers of width 64 with
+GELU (erf form).
 """
 from dataclasses import dataclass
 
@@ -25,6 +27,7 @@ class Surrogate:
     out_shift: np.ndarray
     out_scale: np.ndarray
     train_rmse: np.ndarray = None  # per output, in physical units, on held-out samples
+    loss: np.ndarray = None        # per training step: mean squared error of the normalised batch
 
 
 def mesh_cell_values(mesh):
@@ -86,12 +89,23 @@ def torch_forward(sg, x):
     return y.double().numpy() * sg.out_scale + sg.out_shift
 
 
-def train(mesh, n_samples=1_000_000, n_hidden=4, steps=3000, batch=16384, lr=3e-3, seed=0, device=None,
-          engine=None):
-    """Fit the surrogate to the mesh field (samples labelled by `engine`'s
-    mesh lookup when given); returns a Surrogate."""
-    import torch
+def init_params(dim, n_hidden, seed=0):
+    """Initial weights and biases: U(-1/sqrt(fan_in), 1/sqrt(fan_in)) for
+    every weight and bias of a layer (the usual default for dense layers)."""
+    rng = np.random.default_rng(seed)
+    sizes = [(WIDTH, dim)] + [(WIDTH, WIDTH)] * (n_hidden - 1) + [(dim + 2, WIDTH)]
+    w = np.concatenate([rng.uniform(-1, 1, size=o * i) / np.sqrt(i) for o, i in sizes]).astype(np.float32)
+    b = np.concatenate([rng.uniform(-1, 1, size=o) / np.sqrt(i) for o, i in sizes]).astype(np.float32)
+    return w, b
 
+
+def train(mesh, engine, n_samples=1_000_000, n_hidden=4, steps=3000, batch=16384, lr=3e-3, seed=0):
+    """Fit the surrogate to the mesh field with the engine's own training
+    kernels (fjsph_surrogate_train: BF16 MFMA forward and backward passes,
+    FP32 Adam).  `engine` holds the mesh: it labels the samples with its
+    mesh lookup and trains the network.  Returns a Surrogate whose
+    train_rmse is the error on held-out samples (1 in 20), evaluated by the
+    engine's inference kernel."""
     x, y = sample_mesh(mesh, n_samples, seed=seed, engine=engine)
     dim = x.shape[1]
     lo, hi = mesh.verts.min(axis=0), mesh.verts.max(axis=0)
@@ -99,36 +113,15 @@ def train(mesh, n_samples=1_000_000, n_hidden=4, steps=3000, batch=16384, lr=3e-
     out_shift, out_scale = y.mean(axis=0), y.std(axis=0)
     out_scale = np.where(out_scale > 0, out_scale, 1.0)
     n_test = max(1, len(x) // 20)
-    if device is None:
-        device = "cuda" if torch.cuda.is_available() else "cpu"
-    torch.manual_seed(seed)
-    X = torch.tensor((x - in_shift) / in_scale, dtype=torch.float32, device=device)
-    Y = torch.tensor((y - out_shift) / out_scale, dtype=torch.float32, device=device)
-    Xtr, Ytr, Xte, Yte = X[n_test:], Y[n_test:], X[:n_test], Y[:n_test]
-    layers = []
-    width_in = dim
-    for _ in range(n_hidden):
-        layers += [torch.nn.Linear(width_in, WIDTH), torch.nn.GELU()]
-        width_in = WIDTH
-    layers.append(torch.nn.Linear(WIDTH, dim + 2))
-    net = torch.nn.Sequential(*layers).to(device)
-    opt = torch.optim.Adam(net.parameters(), lr=lr)
-    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps)
-    g = torch.Generator(device=device)
-    g.manual_seed(seed)
-    for _ in range(steps):
-        idx = torch.randint(0, Xtr.shape[0], (batch,), device=device, generator=g)
-        loss = torch.nn.functional.mse_loss(net(Xtr[idx]), Ytr[idx])
-        opt.zero_grad(set_to_none=True)
-        loss.backward()
-        opt.step()
-        sched.step()
-    with torch.no_grad():
-        err = (net(Xte) - Yte).pow(2).mean(dim=0).sqrt().cpu().numpy() * out_scale
-    lin = [m for m in net if isinstance(m, torch.nn.Linear)]
-    weights = np.concatenate([m.weight.detach().cpu().numpy().ravel() for m in lin]).astype(np.float32)
-    biases = np.concatenate([m.bias.detach().cpu().numpy().ravel() for m in lin]).astype(np.float32)
-    return Surrogate(n_hidden, weights, biases, in_shift, in_scale, out_shift, out_scale, train_rmse=err)
+    w0, b0 = init_params(dim, n_hidden, seed=seed)
+    w, b, loss = engine.surrogate_train(x[n_test:], y[n_test:], n_hidden, w0, b0, in_shift, in_scale, out_shift,
+                                        out_scale, steps=steps, batch=batch, lr=lr, seed=seed)
+    sg = Surrogate(n_hidden, w, b, in_shift, in_scale, out_shift, out_scale)
+    engine.set_surrogate(sg)
+    err = engine.surrogate_eval(x[:n_test]) - y[:n_test]
+    sg.train_rmse = np.sqrt(np.mean(err ** 2, axis=0))
+    sg.loss = loss
+    return sg
 
 
 def random_surrogate(dim, n_hidden=4, seed=0, in_shift=None, in_scale=None):

@@ -316,6 +316,29 @@ fjsph_status fjsph_set_surrogate(fjsph_ctx* ctx, int32_t n_hidden, const float* 
                                  const double* out_scale);
 /* Evaluate the surrogate at n host points (n*dim) into out (n*(dim+2)). */
 fjsph_status fjsph_surrogate_eval(fjsph_ctx* ctx, int64_t n, const double* points, double* out);
+/* Train the surrogate on the GPU with the engine's own kernels (BF16 MFMA
+ * forward and backward passes with FP32 accumulation, Adam on FP32 master
+ * parameters, cosine learning-rate schedule from lr to 0) on n samples:
+ * points (n*dim) and values (n*(dim+2), the (vel, p, rho) to fit),
+ * normalised with the given shifts and scales.  Each of `steps` steps
+ * draws `batch` samples (a multiple of 64) by a hash of seed, step and
+ * position, and minimises their mean squared error.  weights/biases (the
+ * layout of fjsph_set_surrogate): initial values in, trained values out;
+ * loss (steps values, may be NULL): the mean squared error of each step's
+ * batch before its update.  No counterpart in the reference. */
+fjsph_status fjsph_surrogate_train(fjsph_ctx* ctx, int64_t n, const double* points, const double* values,
+                                   int32_t n_hidden, int32_t steps, int32_t batch, double lr, uint64_t seed,
+                                   const double* in_shift, const double* in_scale, const double* out_shift,
+                                   const double* out_scale, float* weights, float* biases, double* loss);
+/* Gradients of the mean squared error over one batch of `batch` (a
+ * multiple of 64) normalised samples xn (batch*dim) with targets tn
+ * (batch*(dim+2)) for the given parameters, by the training kernels: into
+ * dweights/dbiases (the layout of fjsph_set_surrogate), the loss into
+ * *loss.  For checks of the training kernels.  No counterpart in the
+ * reference. */
+fjsph_status fjsph_surrogate_grad(fjsph_ctx* ctx, int32_t n_hidden, int32_t batch, const float* xn, const float* tn,
+                                  const float* weights, const float* biases, float* dweights, float* dbiases,
+                                  double* loss);
 
 /* ---- measurement ---------------------------------------------------------- */
 

@@ -9,6 +9,12 @@ Tolerances (outputs normalised by their scale):
   surrogate aero field vs the mesh lookup it was trained on: 5% of the
   stream speed (RMS), the network being a smooth fit of a piecewise-constant
   field.
+
+The training kernels (fjsph_surrogate_grad) against the NumPy oracle of
+oracle/mlp_oracle.py (itself pinned by finite differences): with BF16
+rounding emulated at the kernels' rounding points, normwise 2e-2 (FP32 sums
+in another order can move a BF16 rounding of an activation or of dZ by one
+unit); against the exact FP64 gradient, normwise 5e-2.
 """
 import numpy as np
 import pytest
@@ -66,8 +72,12 @@ def test_surrogate_aero_step():
     aero particles' gas velocity tracks the mesh lookup, and steps run."""
     from fjsph_amd.engine import Engine
     base = cases.with_mesh(cases.droplet_3d(diameter=1.0e-3, dx=5e-5), n_points=20000)
-    sg = surrogate.train(base.mesh, n_samples=200_000, steps=1500, batch=8192)
-    print("surrogate held-out rmse:", sg.train_rmse, flush=True)
+    lab = Engine(base.settings)
+    lab.set_mesh(base.mesh)
+    sg = surrogate.train(base.mesh, lab, n_samples=200_000, steps=1500, batch=8192)
+    lab.close()
+    print("surrogate held-out rmse:", sg.train_rmse, "loss", sg.loss[0], "->", sg.loss[-1], flush=True)
+    assert sg.loss[-100:].mean() < 0.1 * sg.loss[:10].mean()
     eng_m = Engine(base)
     eng_m.build_neighbours()
     eng_m.dsph_prestep()
@@ -116,3 +126,53 @@ def test_mesh_eval_matches_scipy():
     assert np.array_equal(vals[ok, 3], cp[cell[ok]])
     del Delaunay
     eng.close()
+
+
+def _normwise(a, b):
+    return float(np.max(np.abs(np.asarray(a, dtype=np.float64) - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("dim,n_hidden", [(3, 4), (2, 2)])
+def test_training_gradients_match_oracle(dim, n_hidden):
+    """One forward/backward pass of the training kernels against the oracle."""
+    from fjsph_amd.engine import Engine
+    from oracle import mlp_oracle as mo
+    s = abi.default_settings(dim)
+    s.particle_step = 1e-4
+    eng = Engine(s)
+    rng = np.random.default_rng(11 * dim + n_hidden)
+    w, b = surrogate.init_params(dim, n_hidden, seed=dim)
+    b = (b + rng.normal(0, 0.2, size=b.size)).astype(np.float32)
+    batch = 256
+    xn = rng.uniform(-1, 1, size=(batch, dim)).astype(np.float32)
+    tn = rng.normal(size=(batch, dim + 2)).astype(np.float32)
+    print(f"[train grad] dim={dim} n_hidden={n_hidden} batch={batch}", flush=True)
+    dw, db, loss = eng.surrogate_grad(n_hidden, w, b, xn, tn)
+    eng.close()
+    rw, rb, rloss = mo.forward_backward(w, b, n_hidden, dim, xn.astype(np.float64), tn.astype(np.float64))
+    fw, fb, floss = mo.forward_backward(w, b, n_hidden, dim, xn.astype(np.float64), tn.astype(np.float64),
+                                        bf16=False)
+    errs = {"dW_bf16": _normwise(dw, rw), "db_bf16": _normwise(db, rb),
+            "dW_fp64": _normwise(dw, fw), "db_fp64": _normwise(db, fb)}
+    print("gradient errors:", {k: f"{v:.1e}" for k, v in errs.items()}, "loss", loss, rloss, floss, flush=True)
+    assert abs(loss - rloss) <= 1e-3 * rloss
+    assert errs["dW_bf16"] <= 2e-2 and errs["db_bf16"] <= 2e-2
+    assert errs["dW_fp64"] <= 5e-2 and errs["db_fp64"] <= 5e-2
+
+
+def test_training_reduces_loss_and_fits_mesh():
+    """fjsph_surrogate_train on a mesh field: the loss falls, and the trained
+    network's gas velocity on held-out samples is within 1% of the stream
+    speed (RMS) of the mesh lookup's."""
+    from fjsph_amd.engine import Engine
+    base = cases.with_mesh(cases.droplet_3d(diameter=1.0e-3, dx=5e-5), n_points=20000)
+    lab = Engine(base.settings)
+    lab.set_mesh(base.mesh)
+    sg = surrogate.train(base.mesh, lab, n_samples=400_000, steps=8000, batch=16384)
+    lab.close()
+    speed = float(base.settings.v_inf[0])
+    vel_rms = float(np.sqrt(np.sum(sg.train_rmse[:3] ** 2)))
+    print(f"loss {sg.loss[0]:.3e} -> {sg.loss[-1]:.3e}; held-out gas velocity rms {vel_rms:.4f} m/s "
+          f"({100 * vel_rms / speed:.2f}% of {speed} m/s)", flush=True)
+    assert sg.loss[-100:].mean() < 0.05 * sg.loss[:10].mean()
+    assert vel_rms < 0.01 * speed
