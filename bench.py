@@ -165,6 +165,10 @@ def main():
     ap.add_argument("--aero", default="surrogate", choices=["surrogate", "mesh", "const"],
                     help="gas velocity source: MLP surrogate of the mesh (timed, with the mesh lookup timed "
                          "beside it), the mesh lookup alone, or the constant coflow velocity")
+    ap.add_argument("--mesh-cells", type=int, default=20_000_000,
+                    help="cells of the structured tetrahedral gas mesh (the coflow configuration's 20M)")
+    ap.add_argument("--mesh-kind", default="kuhn", choices=["kuhn", "delaunay"],
+                    help="structured tetrahedral mesh, or a seeded Delaunay mesh of --mesh-points points")
     ap.add_argument("--mesh-points", type=int, default=300_000)
     ap.add_argument("--train-samples", type=int, default=1_000_000)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -190,11 +194,17 @@ def main():
     else:
         from fjsph_amd import surrogate
         t0 = time.perf_counter()
-        cases.with_mesh(case, n_points=args.mesh_points, speed=float(case.settings.v_inf[0]))
+        cases.with_mesh(case, n_points=args.mesh_points, speed=float(case.settings.v_inf[0]), kind=args.mesh_kind,
+                        n_cells=args.mesh_cells)
         t_mesh = time.perf_counter() - t0
         print(f"[bench] mesh: {case.mesh.ncells} cells ({t_mesh:.0f} s)", file=sys.stderr, flush=True)
         mesh_run = run_engine(case, args, "mesh aero")
+        def lookup_ms(r):
+            ms, calls = r["timers"]["aero"]
+            return ms / calls if calls else None
+
         aero_info = {"mesh_cells": int(case.mesh.ncells), "mesh_points": int(case.mesh.nverts),
+                     "mesh_lookup_ms_per_call": lookup_ms(mesh_run),
                      "mesh_aero_value": mesh_run["value"], "mesh_aero_ms_per_step": mesh_run["elapsed"] / args.steps * 1e3,
                      "mesh_lost": mesh_run.get("mesh_lost"), "aero_particles": mesh_run.get("aero_particles")}
         run = mesh_run
@@ -217,6 +227,7 @@ def main():
             dim = case.settings.dim
             aero_info.update({
                 "surrogate_value": run["value"],
+                "surrogate_lookup_ms_per_call": lookup_ms(run),
                 "surrogate_vs_mesh_speedup": run["value"] / mesh_run["value"],
                 "surrogate_rms_error": {"vel_m_s": float(np.sqrt(np.mean(np.sum(err[:, :dim] ** 2, axis=1)))),
                                         "p_pa": float(np.sqrt(np.mean(err[:, dim] ** 2))),
@@ -225,7 +236,8 @@ def main():
                              f"{args.train_samples} mesh-lookup samples",
             })
         desc += (f", gas velocity from {'the MLP surrogate of ' if args.aero == 'surrogate' else ''}a "
-                 f"{case.mesh.ncells}-cell Delaunay mesh (potential flow past a cylinder, "
+                 f"{case.mesh.ncells}-cell {'structured tetrahedral' if args.mesh_kind == 'kuhn' else 'Delaunay'} "
+                 f"mesh (potential flow past a cylinder, "
                  f"{case.settings.v_inf[0]:.0f} m/s)")
 
     timers = run["timers"]

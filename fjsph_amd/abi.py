@@ -6,7 +6,7 @@ fjsph_settings_default().
 """
 import ctypes as C
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # fjsph_status
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
@@ -41,7 +41,7 @@ def field_width(name, dim):
 
 
 TIMER_NAMES = ["neighbour", "dsph", "surface", "dissipation", "shift",
-               "boundary", "force", "reduce", "update"]
+               "boundary", "force", "reduce", "update", "aero"]
 
 
 class Settings(C.Structure):

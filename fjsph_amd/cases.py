@@ -378,9 +378,117 @@ def delaunay_mesh(lo, hi, n_points, seed=7, speed=20.0, rho_g=1.2, p_inf=101325.
                 cell_faces, np.ascontiguousarray(vel), p, rho, simplices=simp, tri=tri)
 
 
-def with_mesh(case, n_points=20000, margin=4.0, seed=7, speed=None):
-    """Switch a case to the mesh aero source with a Delaunay mesh around its
-    particles (`margin` particle spacings beyond their bounding box)."""
+_KUHN_PERMS = np.array([(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)], dtype=np.int64)
+
+
+def kuhn_mesh(lo, hi, n_cells, speed=20.0, rho_g=1.2, p_inf=101325.0, cyl_radius=None, cyl_centre=None):
+    """Structured 3D tetrahedral mesh of about n_cells cells over [lo, hi]:
+    a grid of cubes, each split into the 6 tetrahedra of its main diagonal
+    (p0, p0 + e_s0, p0 + e_s0 + e_s1, p0 + (1,1,1) for the permutations s
+    of the axes; the split is conforming across cubes).  Faces are
+    enumerated directly: per cube the 6 faces through its diagonal, and
+    per grid vertex p0 the 2 triangles of the cube face x_d = p0_d that
+    starts at p0, for each axis d (the lower face of the cube at p0 and the
+    upper face of the cube below it).  The gas solution at the vertices is
+    the potential flow of `cylinder_flow`, as for delaunay_mesh."""
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    ext = hi - lo
+    h = (float(np.prod(ext)) * 6.0 / float(n_cells)) ** (1.0 / 3.0)
+    n = np.maximum(1, np.round(ext / h)).astype(np.int64)
+    nx, ny, nz = (int(v) for v in n)
+    vx, vy = nx + 1, ny + 1
+    g = [np.linspace(lo[k], hi[k], int(n[k]) + 1) for k in range(3)]
+    pts = np.stack(np.meshgrid(g[0], g[1], g[2], indexing="ij"), -1).transpose(2, 1, 0, 3).reshape(-1, 3)
+    pts = np.ascontiguousarray(pts)
+    nv = pts.shape[0]
+    ncube = nx * ny * nz
+    ci = np.arange(ncube, dtype=np.int64)
+    cx, cy, cz = ci % nx, (ci // nx) % ny, ci // (nx * ny)
+    v0 = (cz * vy + cy) * vx + cx                       # cube origin vertex
+    step = np.array([1, vx, vx * vy], dtype=np.int64)   # vertex index step along each axis
+    # ---- faces: interior faces of cube c (id 6c + m), lower faces of vertex v (id 6 ncube + 6 v + 2 d + t)
+    nint = 6 * ncube
+    nfaces_all = nint + 6 * nv
+    # ---- cells: id 6c + s
+    perms = _KUHN_PERMS
+    s0, s1, s2 = perms[:, 0], perms[:, 1], perms[:, 2]
+    cell_faces = np.empty((ncube, 6, 4), dtype=np.int64)
+    t_omit0 = (s1 > s2).astype(np.int64)   # triangle of the upper face in direction s0
+    t_omit3 = (s0 > s1).astype(np.int64)   # triangle of the lower face in direction s2
+    cell_faces[:, :, 0] = nint + 6 * (v0[:, None] + step[s0][None, :]) + 2 * s0[None, :] + t_omit0[None, :]
+    cell_faces[:, :, 1] = 6 * ci[:, None] + 3 + s2[None, :]
+    cell_faces[:, :, 2] = 6 * ci[:, None] + s0[None, :]
+    cell_faces[:, :, 3] = nint + 6 * v0[:, None] + 2 * s2[None, :] + t_omit3[None, :]
+    cell_faces = cell_faces.reshape(-1, 4)
+    simplices = np.empty((ncube, 6, 4), dtype=np.int64)
+    simplices[:, :, 0] = v0[:, None]
+    simplices[:, :, 1] = v0[:, None] + step[s0][None, :]
+    simplices[:, :, 2] = simplices[:, :, 1] + step[s1][None, :]
+    simplices[:, :, 3] = v0[:, None] + step.sum()
+    simplices = simplices.reshape(-1, 4)
+    # ---- face vertices and left/right cells
+    fv = np.empty((nfaces_all, 3), dtype=np.int64)
+    lr = np.full((nfaces_all, 2), -2, dtype=np.int64)
+    m = np.arange(6)
+    mid = np.where(m < 3, step[m % 3], step.sum() - step[m % 3])
+    fv[:nint, 0] = np.repeat(v0, 6)
+    fv[:nint, 1] = (v0[:, None] + mid[None, :]).reshape(-1)
+    fv[:nint, 2] = np.repeat(v0 + step.sum(), 6)
+    # interior face m < 3: omit-p2 face of the 2 tets with s0 = m; m >= 3: omit-p1 face of the tets with s2 = m - 3
+    pair = np.array([np.nonzero(s0 == k)[0] for k in range(3)] + [np.nonzero(s2 == k)[0] for k in range(3)])
+    lr[:nint] = (6 * ci[:, None, None] + pair[None, :, :]).reshape(-1, 2)
+    vi = np.arange(nv, dtype=np.int64)
+    vxi, vyi, vzi = vi % vx, (vi // vx) % vy, vi // (vx * vy)
+    vc = np.stack([vxi, vyi, vzi], 1)
+    lim = np.array([nx, ny, nz])
+    for d in range(3):
+        a, b = [k for k in range(3) if k != d]
+        for t in range(2):
+            ids = nint + 6 * vi + 2 * d + t
+            first = a if t == 0 else b
+            fv[ids, 0] = vi
+            fv[ids, 1] = vi + step[first]
+            fv[ids, 2] = vi + step[a] + step[b]
+            # lower face of the cube at v (tet with s2 = d, first step `first`)
+            s_lo = np.nonzero((s2 == d) & (s0 == first))[0][0]
+            in_lo = np.all(vc < lim, axis=1)
+            # upper face of the cube at v - e_d (tet with s0 = d, s1 = first)
+            s_up = np.nonzero((s0 == d) & (s1 == first))[0][0]
+            below = vc.copy()
+            below[:, d] -= 1
+            in_up = np.all((below >= 0) & (below < lim), axis=1)
+            cube_lo = (vzi * ny + vyi) * nx + vxi
+            cube_up = (below[:, 2] * ny + below[:, 1]) * nx + below[:, 0]
+            c_lo = np.where(in_lo, 6 * cube_lo + s_lo, -2)
+            c_up = np.where(in_up, 6 * cube_up + s_up, -2)
+            both = in_lo & in_up
+            lr[ids, 0] = np.where(both, np.minimum(c_lo, c_up), np.where(in_lo, c_lo, c_up))
+            lr[ids, 1] = np.where(both, np.maximum(c_lo, c_up), -2)
+    # ---- keep the faces some cell has
+    used = np.zeros(nfaces_all, dtype=bool)
+    used[cell_faces.reshape(-1)] = True
+    remap = np.cumsum(used) - 1
+    fv = fv[used]
+    lr = lr[used]
+    cell_faces = remap[cell_faces]
+    nf = fv.shape[0]
+    face_off = np.arange(nf + 1, dtype=np.int64) * 3
+    cell_off = np.arange(cell_faces.shape[0] + 1, dtype=np.int64) * 4
+    if cyl_radius is None:
+        cyl_radius = 0.25 * float(hi[1] - lo[1])
+    if cyl_centre is None:
+        cyl_centre = (float(lo[0]) - 1.5 * cyl_radius, 0.5 * float(lo[1] + hi[1]))
+    vel, p, rho = cylinder_flow(pts, speed, cyl_radius, cyl_centre, rho_g, p_inf)
+    return Mesh(pts, face_off, np.ascontiguousarray(fv.reshape(-1)), np.ascontiguousarray(lr.reshape(-1)), cell_off,
+                np.ascontiguousarray(cell_faces.reshape(-1)), np.ascontiguousarray(vel), p, rho, simplices=simplices)
+
+
+def with_mesh(case, n_points=20000, margin=4.0, seed=7, speed=None, kind="delaunay", n_cells=None):
+    """Switch a case to the mesh aero source with a mesh around its
+    particles (`margin` particle spacings beyond their bounding box): a
+    seeded Delaunay mesh of n_points random points (kind "delaunay") or a
+    structured tetrahedral mesh of about n_cells cells (kind "kuhn", 3D)."""
     s = case.settings
     dx = s.particle_step
     lo = case.x.min(axis=0) - margin * dx
@@ -388,7 +496,10 @@ def with_mesh(case, n_points=20000, margin=4.0, seed=7, speed=None):
     sp = float(speed if speed is not None else s.v_inf[0])
     rho_g = float(s.rho_g)
     p_inf = float(s.p_ref)
-    case.mesh = delaunay_mesh(lo, hi, n_points, seed=seed, speed=sp, rho_g=rho_g, p_inf=p_inf)
+    if kind == "kuhn":
+        case.mesh = kuhn_mesh(lo, hi, n_cells, speed=sp, rho_g=rho_g, p_inf=p_inf)
+    else:
+        case.mesh = delaunay_mesh(lo, hi, n_points, seed=seed, speed=sp, rho_g=rho_g, p_inf=p_inf)
     s.aero_source = abi.MESH_INFL
     case.name = case.name + "_mesh"
     return case

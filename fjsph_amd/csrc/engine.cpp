@@ -123,6 +123,9 @@ struct fjsph_ctx {
     DevMesh dm{};
     std::vector<void*> mesh_bufs;
     /* surrogate */
+    int32_t* aero_list = nullptr;   /* segmented list of the aero particles of the last lookup */
+    int32_t* aero_counts = nullptr; /* its per-segment lengths */
+    int64_t aero_list_cap = 0;      /* segments allocated */
     bool have_surrogate = false;
     Surrogate sg{};
     std::vector<void*> sg_bufs;
@@ -254,7 +257,9 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->order); dfree(ctx->vorder); dfree(ctx->pos); dfree(ctx->tile_sums);
     dfree(ctx->nl.lu4); dfree(ctx->nl.count); dfree(ctx->nl.nnb); dfree(ctx->export_buf);
     dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
-    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb);
+    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb); dfree(ctx->aero_list);
+    dfree(ctx->aero_counts);
+    ctx->aero_list_cap = 0;
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
     ctx->ncell_cap = 0;
     ctx->nl.cap = 0;
@@ -481,23 +486,36 @@ int r_from_x(const fjsph_ctx* ctx) { return ctx->x_at_build ? 1 : 0; }
 void mesh_lookup(fjsph_ctx* ctx)
 {
     if (!ctx->have_mesh) throw std::string("aero_source is the mesh but no mesh was set (fjsph_set_mesh)");
+    PassTimer t(ctx, FJSPH_T_AERO);
     HIPCHK(hipMemsetAsync(&ctx->sc->mesh_lost, 0, sizeof(int32_t), ctx->st));
     launch_find_cell(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->dm, ctx->n, &ctx->sc->mesh_lost, ctx->st);
 }
 
-/* get_aero_velocity for the surrogate source: the MLP in place of FindCell. */
+/* get_aero_velocity for the surrogate source: the MLP in place of FindCell,
+ * over the compacted list of the aero particles. */
 void surrogate_lookup(fjsph_ctx* ctx)
 {
     if (!ctx->have_surrogate) throw std::string("aero_source is the surrogate but none was set (fjsph_set_surrogate)");
-    launch_surrogate(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->sg, ctx->n, 0, nullptr, nullptr, ctx->st);
+    PassTimer t(ctx, FJSPH_T_AERO);
+    const int64_t nseg = (ctx->n + AERO_SEG - 1) / AERO_SEG;
+    if (ctx->aero_list_cap < nseg) {
+        ctx->aero_list_cap = nseg + nseg / 8 + 1;
+        dalloc(ctx->aero_list, size_t(ctx->aero_list_cap * AERO_SEG));
+        dalloc(ctx->aero_counts, size_t(ctx->aero_list_cap * AERO_CSTRIDE));
+    }
+    HIPCHK(hipMemsetAsync(ctx->aero_counts, 0, sizeof(int32_t) * nseg * AERO_CSTRIDE, ctx->st));
+    launch_aero_list(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->n, ctx->aero_list, ctx->aero_counts, ctx->st);
+    launch_surrogate_list(ctx->s.dim, ctx->pnp1, ctx->sg, ctx->n, ctx->aero_list, ctx->aero_counts, ctx->st);
 }
 
 void dsph(fjsph_ctx* ctx, bool with_aero)
 {
-    PassTimer t(ctx, FJSPH_T_DSPH);
-    refresh_aux(ctx);
-    launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
-                with_aero ? 1 : 0, ctx->partial, ctx->sc, ctx->st);
+    {
+        PassTimer t(ctx, FJSPH_T_DSPH);
+        refresh_aux(ctx);
+        launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
+                    with_aero ? 1 : 0, ctx->partial, ctx->sc, ctx->st);
+    }
     if (with_aero && ctx->s.aero_source == FJSPH_MESH_INFL) mesh_lookup(ctx);
     if (with_aero && ctx->s.aero_source == FJSPH_SURROGATE) surrogate_lookup(ctx);
 }

@@ -130,6 +130,11 @@ struct Surrogate {
     double out_shift[5], out_scale[5];
 };
 
+/* Segmented list of the aero particles (surrogate.hip): segment length and
+ * the stride of its counters (one 128-byte line each). */
+constexpr int64_t AERO_SEG = 65536;
+constexpr int64_t AERO_CSTRIDE = 32;
+
 /* Device buffers of one surrogate training step (surrogate_train.hip). */
 struct MlpTrain {
     const float* X;      /* samples: normalised inputs, 4 per sample (D used) */
@@ -203,6 +208,10 @@ void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double*
 /* Surrogate on the aero particles of s (mode 0) or on points (mode 1). */
 void launch_surrogate(int dim, const Consts& c, DevState s, const int32_t* part_id, Surrogate sg, int64_t n, int mode,
                       const double* pts, double* out, hipStream_t st);
+void launch_aero_list(int dim, const Consts& c, DevState s, const int32_t* part_id, int64_t n, int32_t* list,
+                      int32_t* count, hipStream_t st);
+void launch_surrogate_list(int dim, DevState s, Surrogate sg, int64_t n, const int32_t* list, const int32_t* count,
+                           hipStream_t st);
 void launch_mlp_batch(uint64_t seed, int64_t step, int64_t n, int32_t batch, int32_t* idx, hipStream_t st);
 void launch_mlp_train(int dim, const MlpTrain& m, hipStream_t st);
 void launch_mlp_reduce(const float* part, int64_t nwaves, int64_t stride, int64_t n, float* grad, hipStream_t st);

@@ -22,13 +22,29 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int SW = 64;       /* hidden width */
 constexpr int SROW = SW + 8; /* LDS row, padded against bank conflicts */
 
-__device__ __forceinline__ float gelu_erf(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+/* erf by Abramowitz and Stegun 7.1.26 (|error| <= 1.5e-7): for x >= 0,
+ * erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2),
+ * t = 1 / (1 + p x); odd in x.  Far below the BF16 rounding of the
+ * activations it feeds. */
+__device__ __forceinline__ float erf_as(float x)
+{
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
+    const float poly =
+        t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float y = 1.0f - poly * __expf(-ax * ax);
+    return copysignf(y, x);
+}
+
+__device__ __forceinline__ float gelu_erf(float x) { return 0.5f * x * (1.0f + erf_as(x * 0.70710678118654752f)); }
 
 /* d gelu / dx = Phi(x) + x phi(x) */
 __device__ __forceinline__ float gelu_erf_grad(float x)
 {
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+    return 0.5f * (1.0f + erf_as(x * 0.70710678118654752f)) + x * 0.39894228040143268f * __expf(-0.5f * x * x);
 }
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 /* out[MT tiles of 16 features][4 tiles of 16 points] = W (A tiles, KC
  * chunks of 32 inputs) x rows (point-major LDS rows, B operand) */

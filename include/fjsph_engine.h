@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FJSPH_ABI_VERSION 2
+#define FJSPH_ABI_VERSION 3
 
 typedef enum fjsph_status {
     FJSPH_OK = 0,
@@ -344,12 +344,14 @@ fjsph_status fjsph_surrogate_grad(fjsph_ctx* ctx, int32_t n_hidden, int32_t batc
 
 /* Pass ids of the engine's own timers (HIP events on the context stream).
  * FJSPH_T_FORCE is the pair kernel of get_acc_and_Rrho alone;
+ * FJSPH_T_AERO the gas-field lookup of get_aero_velocity (mesh FindCell or
+ * the surrogate; the constant source is part of FJSPH_T_DSPH);
  * FJSPH_T_UPDATE the per-particle tail (aerodynamic force, artificial
  * viscosity, gravity) and the Newmark-Beta update. */
 enum {
     FJSPH_T_NEIGHBOUR = 0, FJSPH_T_DSPH, FJSPH_T_SURFACE, FJSPH_T_DISSIPATION,
     FJSPH_T_SHIFT, FJSPH_T_BOUNDARY, FJSPH_T_FORCE, FJSPH_T_REDUCE,
-    FJSPH_T_UPDATE, FJSPH_T_COUNT
+    FJSPH_T_UPDATE, FJSPH_T_AERO, FJSPH_T_COUNT
 };
 /* Enable (1) / disable (0) per-pass timing; enabling clears the totals. */
 fjsph_status fjsph_enable_timers(fjsph_ctx* ctx, int32_t on);

@@ -29,6 +29,7 @@ Parity unpinned (cannot be pinned without running the reference):
   and the multi-threaded npd race of src/Shifting.cpp:16-68 (the oracle uses
   the single-thread sum).
 """
+import itertools
 import math
 
 import numpy as np
@@ -611,3 +612,62 @@ def test_check_cell_matches_barycentric(po):
             assert got == 0
             agree += 1
     assert agree > 350
+
+
+def _kuhn_cell(mesh_lo, h, n, x):
+    """Closed-form containing cell of the structured mesh: the cube of x,
+    then the tetrahedron of the axis order of the fractional coordinates
+    (u_s0 >= u_s1 >= u_s2 for the permutation s)."""
+    q = (x - mesh_lo) / h
+    c = np.floor(q).astype(np.int64)
+    u = q - c
+    order = np.argsort(-u, axis=1, kind="stable")
+    perms = [tuple(p) for p in cases._KUHN_PERMS.tolist()]
+    s = np.array([perms.index(tuple(o)) for o in order.tolist()])
+    cube = (c[:, 2] * n[1] + c[:, 1]) * n[0] + c[:, 0]
+    return 6 * cube + s
+
+
+def test_kuhn_mesh_is_conforming():
+    """Each cell's faces are the 4 facets of its tetrahedron, every face has
+    the cells that list it as left/right, and the cells tile the box."""
+    m = cases.kuhn_mesh([0, 0, 0], [1, 2, 1.5], 3000)
+    cf = m.cell_faces.reshape(-1, 4)
+    fv = m.face_verts.reshape(-1, 3)
+    for k in range(m.ncells):
+        facets = {frozenset(fv[f].tolist()) for f in cf[k]}
+        assert facets == {frozenset(t) for t in itertools.combinations(m.simplices[k].tolist(), 3)}
+    owners = {}
+    for k in range(m.ncells):
+        for f in cf[k]:
+            owners.setdefault(int(f), []).append(k)
+    lr = m.face_lr.reshape(-1, 2)
+    assert len(owners) == m.nfaces
+    for f, cs in owners.items():
+        assert sorted(cs) == sorted(int(v) for v in lr[f] if v >= 0)
+    P = m.verts[m.simplices]
+    vol = np.abs(np.einsum("ij,ij->i", np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), P[:, 3] - P[:, 0])) / 6
+    assert abs(vol.sum() - 3.0) < 1e-12 and vol.min() > 0
+
+
+def test_check_cell_kuhn_matches_closed_form(po):
+    """CheckCell's ray parity on the structured mesh finds the cell the
+    closed form (cube, then axis order of the fractional coordinates)
+    gives, for random points."""
+    s = abi.default_settings(3)
+    s.particle_step = 1e-4
+    cases._gas(s, 20.0, 1.2)
+    c = cases._finish("kuhn", s, np.zeros((0, 3)), np.full((1, 3), 5e-3), 0.0, "one particle")
+    lo, hi = np.zeros(3), np.array([1e-2, 8e-3, 6e-3])
+    c.mesh = cases.kuhn_mesh(lo, hi, 4000, speed=20.0, rho_g=1.2, p_inf=s.p_ref)
+    s.aero_source = abi.MESH_INFL
+    o = po.Oracle(c, nthreads=1)
+    m = c.mesh
+    n = np.array([len(np.unique(m.verts[:, k])) - 1 for k in range(3)])
+    h = (hi - lo) / n
+    x = np.random.default_rng(5).uniform(lo, hi, size=(300, 3))
+    ref = _kuhn_cell(lo, h, n, x)
+    for k in range(x.shape[0]):
+        assert o.check_cell(int(ref[k]), x[k]) == 1
+        other = int((ref[k] + 1 + k % 5) % m.ncells)
+        assert o.check_cell(other, x[k]) == 0
