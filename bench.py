@@ -31,6 +31,19 @@ sys.path.insert(0, ROOT)
 from fjsph_amd import abi, cases  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+
+
+def force_flops_per_pair(dim, ale):
+    """FP32 operations of one pair in k_force_t (FMA = 2), frozen-distance
+    form: R_ji and v_ji with the target's rounding residual (4D), build-
+    position difference (2D), r^2 (2D-1), sqrt (1), V_j (1), 1/(r^2+eps)
+    (2), Wendland W and W'/r (11), gradient (D), pressure factor (2), r.g
+    (2D-1), viscous factor (5), surface tension (7), momentum sum (5D);
+    with ALE the five ALE dot products (16D), the ALE momentum terms (7D)
+    and the two density sums (4); without it the continuity term (2D+1)."""
+    base = 4 * dim + 2 * dim + (2 * dim - 1) + 1 + 1 + 2 + 11 + dim + 2 + (2 * dim - 1) + 5 + 7 + 5 * dim
+    return base + (16 * dim + 7 * dim + 4 if ale else 2 * dim + 1)
 METRIC = "particle-steps/s and %HBM roofline at 4M 3D particles; surrogate vs mesh aero"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_t_jet4m_traffic.json")
 
@@ -275,7 +288,13 @@ def main():
         "launches": launches,
         "pairs_per_launch": pairs + case.n,
         "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tmeta else None,
+        "traffic_rate_GBs": traffic / avg_s / 1e9 if traffic else None,
     }
+    # the pair loop is FP32-vector bound: its side of the roofline too
+    fpp = force_flops_per_pair(case.settings.dim, bool(case.settings.ale))
+    tflops = fpp * pairs / avg_s / 1e12
+    roofline["valu"] = {"achieved": tflops, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": tflops / FP32_VECTOR_PEAK_TFLOPS, "flop_per_pair": fpp, "pairs_per_launch": pairs}
 
     cpu = None
     if not args.no_cpu:
