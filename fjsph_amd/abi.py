@@ -6,7 +6,7 @@ fjsph_settings_default().
 """
 import ctypes as C
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # fjsph_status
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
@@ -94,6 +94,33 @@ class Block(C.Structure):
         ("bound_solver", C.c_int32), ("no_slip", C.c_int32), ("n_times", C.c_int32),
         ("vel", C.c_double * 3),
     ]
+
+
+NO_PLANE = 9999999.0  # default_val: an unset plane
+
+
+class BlockPlanes(C.Structure):
+    _fields_ = [
+        ("inlet", C.c_int32), ("fixed_vel_or_dynamic", C.c_int32),
+        ("delete_norm", C.c_double * 3), ("delconst", C.c_double),
+        ("insert_norm", C.c_double * 3), ("insconst", C.c_double),
+        ("aero_norm", C.c_double * 3), ("aeroconst", C.c_double),
+    ]
+
+
+def make_planes(inlet=0, fixed_vel_or_dynamic=0, delete=None, insert=None, aero=None):
+    """BlockPlanes with the reference's defaults (unset planes at
+    default_val) for the planes not given; a plane is (normal, constant)."""
+    q = BlockPlanes()
+    q.inlet = int(inlet)
+    q.fixed_vel_or_dynamic = int(fixed_vel_or_dynamic)
+    for name, plane in (("delete", delete), ("insert", insert), ("aero", aero)):
+        nrm, c = plane if plane is not None else ((NO_PLANE,) * 3, NO_PLANE)
+        vec = getattr(q, name + "_norm")
+        for a in range(3):
+            vec[a] = float(nrm[a]) if a < len(nrm) else 0.0
+        setattr(q, {"delete": "delconst", "insert": "insconst", "aero": "aeroconst"}[name], float(c))
+    return q
 
 
 class StepInfo(C.Structure):

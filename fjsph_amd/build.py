@@ -13,7 +13,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfjsph_engine.so")
-SOURCES = ["kernels.hip", "pairs.hip", "mesh.hip", "surrogate.hip", "surrogate_train.hip", "engine.cpp", "settings.cpp"]
+SOURCES = ["kernels.hip", "pairs.hip", "mesh.hip", "surrogate.hip", "surrogate_train.hip", "inlet.hip", "engine.cpp",
+           "settings.cpp"]
 ARCH = os.environ.get("FJSPH_OFFLOAD_ARCH", "gfx950")
 
 

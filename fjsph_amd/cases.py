@@ -30,6 +30,9 @@ class Case:
     blocks: list = field(default_factory=list)
     notes: str = ""
     mesh: object = None
+    planes: dict = None       # block -> keyword arguments of abi.make_planes
+    inlets: list = None       # (block, back host indices, buffer host indices [n_cols][n_buf])
+    max_points: int = None
 
     @property
     def n(self):
@@ -198,6 +201,59 @@ def jet_column_3d(n_target=4_000_000, diameter=2e-3, dx=4e-5, seed=7, precision=
     return _finish("jet_column_3d", s, np.zeros((0, 3)), pts, vel,
                    f"3D coflow jet column D={diameter} dx={dx}: {pts.shape[0]} particles "
                    f"({nlayers} layers of {per_layer}), seed {seed}")
+
+
+def pipe_inlet_3d(diameter=6e-4, dx=5e-5, speed=5.0, n_pipe=6, n_buf=4, mode=0, precision=abi.PREC_FP64,
+                  H_fac=1.3, gas=20.0, rho_g=1.2, back_to_insert=0.1, exit_to_delete=0.05, max_points=None):
+    """Pipe inlet along +z (the inlet zone of the jet-in-crossflow
+    configuration, src/shapes/inlet.cpp): a disk lattice of spacing dx
+    (columns), n_pipe layers of PIPE particles at z = dx .. n_pipe dx, the
+    BACK layer at z = 0 and n_buf BUFFER layers behind it, all moving at
+    `speed` along +z.  The insertion plane lies back_to_insert dx above the
+    back layer (the first column advance comes after a few steps); the aero
+    plane just below the top pipe layer (it turns FREE at the first
+    Check_Pipe_Outlet); the deletion plane exit_to_delete dx above it (that
+    layer is deleted after a few steps).  mode: 0 fixed-velocity buffer, 1
+    dynamic.  Host order: pipe layers, back layer, buffer layers by depth."""
+    s = abi.default_settings(3)
+    s.particle_step = dx
+    s.H_fac = H_fac
+    s.rho_rest = 1000.0
+    s.sig = 0.0728
+    s.precision = precision
+    s.aero_case = abi.GISSLER
+    _gas(s, gas, rho_g)
+    r = diameter / 2.0
+    nr = int(math.ceil(2.0 * r / dx)) + 1
+    g = (np.arange(nr) - (nr - 1) / 2.0) * dx
+    xx, yy = np.meshgrid(g, g, indexing="ij")
+    keep = (xx ** 2 + yy ** 2) <= r * r + 1e-12
+    disk = np.stack([xx[keep], yy[keep]], axis=1)
+    ncol = disk.shape[0]
+    def layer(z):
+        return np.concatenate([disk, np.full((ncol, 1), z)], axis=1)
+    pipe = np.concatenate([layer(k * dx) for k in range(1, n_pipe + 1)])
+    back = layer(0.0)
+    buf = np.concatenate([layer(-k * dx) for k in range(1, n_buf + 1)])
+    pts = np.concatenate([pipe, back, buf])
+    vel = np.zeros_like(pts)
+    vel[:, 2] = speed
+    c = _finish("pipe_inlet_3d", s, np.zeros((0, 3)), pts, vel,
+                f"pipe inlet D={diameter} dx={dx}: {ncol} columns, {n_pipe} pipe layers, {n_buf} buffer layers, "
+                f"{'dynamic' if mode else 'fixed-velocity'} buffer", fluid_type=abi.PIPE)
+    npipe = pipe.shape[0]
+    c.b[npipe:npipe + ncol] = abi.BACK
+    c.b[npipe + ncol:] = abi.BUFFER
+    back_ids = np.arange(npipe, npipe + ncol)
+    buffer_ids = np.stack([npipe + ncol + k * ncol + np.arange(ncol) for k in range(n_buf)], axis=1)
+    top = n_pipe * dx
+    c.planes = {0: dict(inlet=1, fixed_vel_or_dynamic=mode,
+                        insert=((0.0, 0.0, 1.0), back_to_insert * dx),
+                        aero=((0.0, 0.0, 1.0), top - 0.1 * dx),
+                        delete=((0.0, 0.0, 1.0), top + exit_to_delete * dx))}
+    c.inlets = [(0, back_ids, buffer_ids)]
+    c.max_points = max_points
+    return c
 
 
 def fluid_block(dim, n_side, dx=1e-3, jitter=0.0, seed=0, precision=abi.PREC_FP64, H_fac=1.3,

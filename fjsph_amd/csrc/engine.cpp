@@ -171,6 +171,26 @@ struct fjsph_ctx {
 
     int64_t pairs = 0;
     int32_t max_count = 0;
+
+    /* inlets, deletion and the pipe exit; per-slot arrays hold `cap` slots */
+    int64_t cap = 0;
+    int64_t max_points = 9999999; /* svar.max_points (src/Var.h:369) */
+    std::vector<fjsph_block_planes> planes; /* per block */
+    struct InletCols {
+        int32_t block, col_base, ncols, nbuf;
+    };
+    std::vector<InletCols> inlets;
+    std::vector<int32_t> starved; /* per column: advances since its last new particle */
+    int32_t ncols = 0;
+    bool planes_on = false, planes_dirty = true;
+    int32_t *col = nullptr, *depth = nullptr;                       /* per slot */
+    int32_t *c_adv = nullptr, *c_back = nullptr, *c_last = nullptr; /* per column */
+    int32_t *c_rank = nullptr, *c_block = nullptr;
+    double *pl_del = nullptr, *pl_ins = nullptr, *pl_aero = nullptr;
+    int32_t *pl_mode = nullptr, *pl_nbuf = nullptr;
+    DevPlanes dpl{};
+    int32_t *keep = nullptr, *hflag = nullptr, *kstart = nullptr, *hstart = nullptr; /* deletion scratch */
+    int32_t *perm = nullptr, *del_tiles = nullptr, *blkcount = nullptr;
 };
 
 namespace {
@@ -220,7 +240,7 @@ void alloc_state(fjsph_ctx* ctx, DevState& s)
 {
     for (int f = 0; f < kNumFields; ++f) {
         void** slot = field_slot(s, f);
-        const size_t cnt = size_t(ctx->n) * width_of(ctx, f);
+        const size_t cnt = size_t(std::max(ctx->cap, ctx->n)) * width_of(ctx, f);
         if (field_desc(f).is_int) dalloc(*(int32_t**)slot, cnt);
         else dalloc(*(double**)slot, cnt);
         HIPCHK(hipMemsetAsync(*slot, 0, cnt * (field_desc(f).is_int ? 4 : 8), ctx->st));
@@ -261,8 +281,104 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->aero_counts);
     ctx->aero_list_cap = 0;
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
+    dfree(ctx->col); dfree(ctx->depth); dfree(ctx->keep); dfree(ctx->hflag); dfree(ctx->kstart);
+    dfree(ctx->hstart); dfree(ctx->perm); dfree(ctx->del_tiles); dfree(ctx->blkcount);
+    dfree(ctx->c_adv); dfree(ctx->c_back); dfree(ctx->c_last); dfree(ctx->c_rank); dfree(ctx->c_block);
+    dfree(ctx->pl_del); dfree(ctx->pl_ins); dfree(ctx->pl_aero); dfree(ctx->pl_mode); dfree(ctx->pl_nbuf);
+    ctx->inlets.clear();
+    ctx->starved.clear();
+    ctx->ncols = 0;
+    ctx->planes_dirty = true;
     ctx->ncell_cap = 0;
     ctx->nl.cap = 0;
+    ctx->cap = 0;
+}
+
+/* New allocation of `count` elements keeping the first `keep` of the old. */
+template <typename T>
+void dgrow(T*& p, size_t count, size_t keep)
+{
+    T* q = nullptr;
+    hipError_t e = hipMalloc((void**)&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) throw HipError{e, "hipMalloc"};
+    if (p && keep) {
+        e = hipMemcpy(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice);
+        if (e != hipSuccess) throw HipError{e, "hipMemcpy"};
+    }
+    if (p) (void)hipFree(p);
+    p = q;
+}
+
+/* (Re)allocate every per-slot array for `cap` slots, keeping the first
+ * `keep` entries (the particles so far); new slots of the states are zero,
+ * of col/depth -1.  The neighbour lists are dropped (rebuilt at the next
+ * build). */
+void alloc_slots(fjsph_ctx* ctx, int64_t cap, int64_t keep)
+{
+    const size_t D = size_t(ctx->s.dim), c = size_t(cap), k = size_t(keep);
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (DevState* st : {&ctx->pn, &ctx->pnp1}) {
+        for (int f = 0; f < kNumFields; ++f) {
+            void** slot = field_slot(*st, f);
+            const size_t w = size_t(width_of(ctx, f)), es = field_desc(f).is_int ? 4 : 8;
+            char* q = (char*)*slot;
+            dgrow(q, c * w * es, k * w * es);
+            HIPCHK(hipMemset(q + k * w * es, 0, (c - k) * w * es));
+            *slot = q;
+        }
+    }
+    if (ctx->rk_alloc) {
+        free_state(ctx->rk1);
+        free_state(ctx->rk2);
+        free_state(ctx->rk3);
+        ctx->rk_alloc = false;
+    }
+    dgrow(ctx->x_nxt, c * D, k * D);
+    dgrow(ctx->v_nxt, c * D, k * D);
+    dgrow(ctx->rho_nxt, c, k);
+    dgrow(ctx->p_nxt, c, k);
+    dgrow(ctx->part_id, c, k);
+    dgrow(ctx->blk, c, k);
+    dgrow(ctx->near_inlet, c, k);
+    HIPCHK(hipMemset(ctx->near_inlet + k, 0, (c - k) * 4));
+    dgrow(ctx->col, c, k);
+    dgrow(ctx->depth, c, k);
+    HIPCHK(hipMemset(ctx->col + k, 0xFF, (c - k) * 4));
+    HIPCHK(hipMemset(ctx->depth + k, 0xFF, (c - k) * 4));
+    dgrow(ctx->norms_tmp, c * D, 0);
+    dgrow(ctx->gather_tmp, c * D * D, 0);
+    dgrow(ctx->gather_tmp_i, c, 0);
+    dgrow(ctx->cell, c, k);
+    dgrow(ctx->order, c, k);
+    dgrow(ctx->pos, c, k);
+    dgrow(ctx->vorder, c, k);
+    dgrow(ctx->aux.vol, c, 0);
+    dgrow(ctx->aux.P, c, 0);
+    dgrow(ctx->aux.invrho, c, 0);
+    dgrow(ctx->aux.fq, c, 0);
+    dgrow(ctx->aux.fale, c, 0);
+    dgrow(ctx->aux.nhat, c * D, k * D);
+    HIPCHK(hipMemset(ctx->aux.nhat + k * D, 0, (c - k) * D * 8));
+    dgrow(ctx->aux.lnb, c * D * D, 0);
+    dgrow(ctx->aux.xb, c * D, 0);
+    dgrow(ctx->keep, c, 0);
+    dgrow(ctx->hflag, c, 0);
+    dgrow(ctx->kstart, c + 1, 0);
+    dgrow(ctx->hstart, c + 1, 0);
+    dgrow(ctx->perm, c, 0);
+    dgrow(ctx->del_tiles, c / 1024 + 2, 0);
+    ctx->nl.npad = (cap + 63) / 64 * 64;
+    dgrow(ctx->nl.count, size_t(ctx->nl.npad), 0);
+    dgrow(ctx->nl.nnb, size_t(ctx->nl.npad), 0);
+    dfree(ctx->nl.lu4);
+    ctx->nl.cap = 0;
+    const int64_t parts = int64_t(reduction_parts(cap)) * 8;
+    if (parts > ctx->partial_cap) {
+        dalloc(ctx->partial, size_t(parts));
+        ctx->partial_cap = parts;
+    }
+    ctx->cap = cap;
+    ctx->have_nbrs = false;
 }
 
 void fill_consts(fjsph_ctx* ctx)
@@ -347,10 +463,11 @@ void copy_state(fjsph_ctx* ctx, DevState& dst, DevState& src)
     }
 }
 
-/* Permute every per-slot array by perm (new slot s <- old slot perm[s]). */
-void permute_all(fjsph_ctx* ctx, const int32_t* perm)
+/* Permute every per-slot array by perm (new slot s <- old slot perm[s]),
+ * for the first `count` new slots (ctx->n when negative). */
+void permute_all(fjsph_ctx* ctx, const int32_t* perm, int64_t count = -1)
 {
-    const int64_t n = ctx->n;
+    const int64_t n = count < 0 ? ctx->n : count;
     for (DevState* st : {&ctx->pn, &ctx->pnp1}) {
         for (int f = 0; f < kNumFields; ++f) {
             void** slot = field_slot(*st, f);
@@ -366,7 +483,7 @@ void permute_all(fjsph_ctx* ctx, const int32_t* perm)
             }
         }
     }
-    for (int32_t** arr : {&ctx->part_id, &ctx->blk, &ctx->cell}) {
+    for (int32_t** arr : {&ctx->part_id, &ctx->blk, &ctx->cell, &ctx->col, &ctx->depth}) {
         launch_gather_i(*arr, ctx->gather_tmp_i, perm, n, ctx->st);
         std::swap(*arr, ctx->gather_tmp_i);
     }
@@ -421,6 +538,7 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
     if (reorder) {
         permute_all(ctx, ctx->order);
         launch_iota(ctx->order, n, ctx->st);
+        if (ctx->ncols > 0) launch_inlet_backslot(ctx->col, ctx->depth, n, ctx->c_back, ctx->st);
     }
     HIPCHK(hipMemcpyAsync(ctx->aux.xb, ctx->pnp1.x, sizeof(double) * n * D, hipMemcpyDeviceToDevice, ctx->st));
     /* tile grid: chunks of C cells sized for about one workgroup of targets */
@@ -549,13 +667,77 @@ void shift(fjsph_ctx* ctx)
                  ctx->st);
 }
 
+/* The reference's unset planes (bound_block constructor, src/Var.h:827-831):
+ * no deletion, insertion and aero planes at default_val. */
+fjsph_block_planes default_planes()
+{
+    fjsph_block_planes q{};
+    for (int a = 0; a < 3; ++a) {
+        q.delete_norm[a] = FJSPH_NO_PLANE;
+        q.insert_norm[a] = FJSPH_NO_PLANE;
+        q.aero_norm[a] = FJSPH_NO_PLANE;
+    }
+    q.delconst = q.insconst = q.aeroconst = FJSPH_NO_PLANE;
+    return q;
+}
+
+/* Per-block planes to the device (4 doubles each: normal, constant). */
+void upload_planes(fjsph_ctx* ctx)
+{
+    if (!ctx->planes_dirty) return;
+    const size_t nbk = std::max<size_t>(ctx->blocks.size(), 1);
+    if (ctx->planes.size() < ctx->blocks.size()) ctx->planes.resize(ctx->blocks.size(), default_planes());
+    std::vector<double> del(nbk * 4, 0.0), ins(nbk * 4, 0.0), aero(nbk * 4, 0.0);
+    std::vector<int32_t> mode(nbk, 0), nbuf(nbk, 0);
+    for (size_t k = 0; k < ctx->blocks.size(); ++k) {
+        const fjsph_block_planes& q = ctx->planes[k];
+        for (int a = 0; a < 3; ++a) {
+            del[k * 4 + a] = q.delete_norm[a];
+            ins[k * 4 + a] = q.insert_norm[a];
+            aero[k * 4 + a] = q.aero_norm[a];
+        }
+        del[k * 4 + 3] = q.delconst;
+        ins[k * 4 + 3] = q.insconst;
+        aero[k * 4 + 3] = q.aeroconst;
+        mode[k] = q.fixed_vel_or_dynamic;
+    }
+    for (const auto& in : ctx->inlets) nbuf[size_t(in.block)] = in.nbuf;
+    dalloc(ctx->pl_del, del.size());
+    dalloc(ctx->pl_ins, ins.size());
+    dalloc(ctx->pl_aero, aero.size());
+    dalloc(ctx->pl_mode, mode.size());
+    dalloc(ctx->pl_nbuf, nbuf.size());
+    HIPCHK(hipMemcpy(ctx->pl_del, del.data(), del.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->pl_ins, ins.data(), ins.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->pl_aero, aero.data(), aero.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->pl_mode, mode.data(), mode.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->pl_nbuf, nbuf.data(), nbuf.size() * 4, hipMemcpyHostToDevice));
+    ctx->dpl = DevPlanes{ctx->pl_del, ctx->pl_ins, ctx->pl_aero, ctx->pl_mode, ctx->pl_nbuf};
+    ctx->planes_dirty = false;
+}
+
+/* Check_Pipe_Outlet (src/Containment.cpp:822-890): PIPE particles past
+ * their block's aero plane become FREE; with the mesh source the new FREE
+ * particles get their cell (FirstCell; the lookup of the others finds
+ * their cached cell again).  The reference's deletion of a particle that
+ * FirstCell places in no cell is not made: it is counted by the lookup. */
+void pipe_outlet(fjsph_ctx* ctx)
+{
+    if (!ctx->planes_on) return;
+    upload_planes(ctx);
+    launch_pipe_outlet(ctx->s.dim, ctx->pnp1, ctx->part_id, ctx->blk, ctx->dpl, ctx->n, ctx->nb, ctx->st);
+    if (ctx->s.aero_source == FJSPH_MESH_INFL && ctx->have_mesh) mesh_lookup(ctx);
+}
+
 /* dSPH_PreStep, get_aero_velocity, Detect_Surface, dissipation_terms,
- * particle_shift (src/Integration.cpp:45-61 and 80-96). */
+ * particle_shift, Check_Pipe_Outlet (src/Integration.cpp:45-61 and
+ * 80-98). */
 void prepasses(fjsph_ctx* ctx)
 {
     dsph(ctx, true);
     surface_and_dissipation(ctx, true);
     if (ctx->s.ale) shift(ctx);
+    pipe_outlet(ctx);
 }
 
 void upload_block_velocities(fjsph_ctx* ctx, bool rk)
@@ -604,6 +786,13 @@ void force(fjsph_ctx* ctx, bool update)
     launch_nb_update(ctx->s.dim, ctx->c, ctx->pnp1, ctx->pn, update ? ctx->x_nxt : nullptr, ctx->v_nxt,
                      ctx->rho_nxt, ctx->p_nxt, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->nl, ctx->n,
                      ctx->delta_t, ctx->partial, ctx->st);
+    if (update && ctx->ncols > 0) {
+        /* the inlet buffer of Do_NB_Iter, after the fluid update */
+        upload_planes(ctx);
+        launch_inlet_buffer(ctx->s.dim, ctx->c, ctx->pnp1, ctx->pn,
+                            NextState{ctx->x_nxt, ctx->v_nxt, ctx->rho_nxt, ctx->p_nxt}, ctx->col, ctx->depth,
+                            ctx->blk, ctx->c_back, ctx->dpl, ctx->n, ctx->delta_t, ctx->d.dx, ctx->partial, ctx->st);
+    }
 }
 
 /* Do_NB_Iter + the residual of Check_Error; returns log10 rms. */
@@ -746,6 +935,7 @@ double rk_error(fjsph_ctx* ctx, const DevState& a, const DevState& b, double log
  * state of the first build, then Runge_Kutta4 after the second. */
 double integrate_no_update_rk(fjsph_ctx* ctx)
 {
+    if (ctx->ncols > 0) throw std::string("inlet blocks need the Newmark-Beta integrator");
     if (!ctx->rk_alloc) {
         alloc_state(ctx, ctx->rk1);
         alloc_state(ctx, ctx->rk2);
@@ -799,10 +989,122 @@ double integrate_no_update(fjsph_ctx* ctx)
     return rms;
 }
 
-/* update_data (src/Integration.cpp:109-226) without inlets or deletion. */
+void set_count(fjsph_ctx* ctx, int64_t n)
+{
+    ctx->n = n;
+    ctx->nparts = reduction_parts(n);
+    ctx->have_nbrs = false;
+}
+
+/* update_buffer_region (src/shapes/inlet.cpp:578-640): back particles past
+ * the insertion plane become PIPE, their columns move up, and each such
+ * column gets a new BUFFER particle behind its last one while the count is
+ * under max_points.  New particles go to the end of their block in host
+ * order, in column order (the reference's insert at index.second). */
+void inlet_update(fjsph_ctx* ctx)
+{
+    const int D = ctx->s.dim;
+    const int32_t nc = ctx->ncols;
+    upload_planes(ctx);
+    HIPCHK(hipMemsetAsync(ctx->c_last, 0xFF, sizeof(int32_t) * nc, ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->c_adv, 0, sizeof(int32_t) * nc, ctx->st));
+    launch_inlet_back(D, ctx->pnp1, ctx->col, ctx->depth, ctx->blk, ctx->dpl, ctx->n, ctx->c_adv, ctx->c_last, ctx->st);
+    launch_inlet_shift(ctx->pnp1, ctx->col, ctx->depth, ctx->blk, ctx->dpl, ctx->n, ctx->c_adv, ctx->st);
+    std::vector<int32_t> adv(nc), last(nc), rank(nc, -1);
+    HIPCHK(hipMemcpyAsync(adv.data(), ctx->c_adv, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(last.data(), ctx->c_last, sizeof(int32_t) * nc, hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    const int64_t n_old = ctx->n;
+    int32_t nadd = 0;
+    std::vector<int32_t> per_block(ctx->blocks.size(), 0);
+    for (const auto& in : ctx->inlets) /* inlets are kept in block order */
+        for (int32_t c = in.col_base; c < in.col_base + in.ncols; ++c) {
+            if (!adv[c]) continue;
+            if (last[c] >= 0 && n_old + nadd < ctx->max_points) {
+                rank[c] = nadd++;
+                per_block[size_t(in.block)]++;
+                ctx->starved[size_t(c)] = 0;
+            } else if (++ctx->starved[size_t(c)] >= in.nbuf) {
+                /* the reference's buffer list would now hold the back
+                 * particle itself: a state this engine does not mirror */
+                throw std::string("inlet column ") + std::to_string(c - in.col_base) + " of block " +
+                    std::to_string(in.block) + " advanced " + std::to_string(in.nbuf) +
+                    " times without a new particle (max_points reached): its buffer is used up";
+            }
+        }
+    if (nadd > 0) {
+        if (n_old + nadd > ctx->cap) alloc_slots(ctx, n_old + nadd + std::max<int64_t>(n_old / 8, 1024), n_old);
+        /* new slots start from zero (SPHPart(x, pj, BUFFER, id)) */
+        for (int f = 0; f < kNumFields; ++f) {
+            const size_t w = size_t(width_of(ctx, f)), es = field_desc(f).is_int ? 4 : 8;
+            char* base = (char*)*field_slot(ctx->pnp1, f);
+            HIPCHK(hipMemsetAsync(base + size_t(n_old) * w * es, 0, size_t(nadd) * w * es, ctx->st));
+        }
+        HIPCHK(hipMemsetAsync(ctx->near_inlet + n_old, 0, sizeof(int32_t) * nadd, ctx->st));
+        HIPCHK(hipMemcpyAsync(ctx->c_rank, rank.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice, ctx->st));
+        launch_inlet_insert(D, ctx->pnp1, ctx->blk, ctx->col, ctx->depth, ctx->c_block, ctx->c_rank, ctx->c_last,
+                            ctx->dpl, nc, n_old, ctx->d.dx, ctx->st);
+        /* host ids: block by block in host order */
+        int32_t rank_base = 0;
+        for (size_t k = 0; k < ctx->blocks.size(); ++k) {
+            const int32_t a = per_block[k];
+            if (a == 0) continue;
+            const int32_t at = int32_t(ctx->blocks[k].b.last);
+            launch_shift_ids(ctx->part_id, n_old + rank_base, at, a, ctx->st);
+            launch_new_ids(ctx->part_id, ctx->c_block, ctx->c_rank, nc, int32_t(k), n_old, rank_base, at, ctx->st);
+            ctx->blocks[k].b.last += a;
+            for (size_t j = k + 1; j < ctx->blocks.size(); ++j) {
+                ctx->blocks[j].b.first += a;
+                ctx->blocks[j].b.last += a;
+            }
+            rank_base += a;
+        }
+        set_count(ctx, n_old + nadd);
+    }
+    launch_inlet_backslot(ctx->col, ctx->depth, ctx->n, ctx->c_back, ctx->st);
+}
+
+/* Deletion plane of update_data (src/Integration.cpp:126-205): fluid
+ * particles past their block's active plane are removed, the rest keep
+ * their order (slots and host ids are compacted). */
+void delete_update(fjsph_ctx* ctx)
+{
+    const int D = ctx->s.dim;
+    const int64_t n = ctx->n;
+    const size_t nbk = ctx->blocks.size();
+    upload_planes(ctx);
+    if (!ctx->blkcount) dalloc(ctx->blkcount, std::max<size_t>(nbk, 1));
+    HIPCHK(hipMemsetAsync(ctx->blkcount, 0, sizeof(int32_t) * std::max<size_t>(nbk, 1), ctx->st));
+    launch_delete_flag(D, ctx->pnp1, ctx->part_id, ctx->blk, ctx->dpl, n, ctx->nb, ctx->keep, ctx->hflag,
+                       ctx->blkcount, ctx->st);
+    std::vector<int32_t> cnt(std::max<size_t>(nbk, 1));
+    HIPCHK(hipMemcpyAsync(cnt.data(), ctx->blkcount, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    int64_t ndel = 0;
+    for (size_t k = 0; k < nbk; ++k) ndel += cnt[k];
+    if (ndel == 0) return;
+    launch_scan(ctx->keep, n, ctx->kstart, ctx->del_tiles, ctx->st);
+    launch_scan(ctx->hflag, n, ctx->hstart, ctx->del_tiles, ctx->st);
+    launch_compact_perm(ctx->keep, ctx->kstart, n, ctx->perm, ctx->st);
+    launch_compact_ids(ctx->part_id, ctx->hstart, n, ctx->st);
+    permute_all(ctx, ctx->perm, n - ndel);
+    int64_t shift = 0;
+    for (size_t k = 0; k < nbk; ++k) {
+        ctx->blocks[k].b.first -= shift;
+        shift += cnt[k];
+        ctx->blocks[k].b.last -= shift;
+    }
+    set_count(ctx, n - ndel);
+    if (ctx->ncols > 0) launch_inlet_backslot(ctx->col, ctx->depth, ctx->n, ctx->c_back, ctx->st);
+}
+
+/* update_data (src/Integration.cpp:109-226): density floor, inlets,
+ * deletion, pn = pnp1. */
 void update_data(fjsph_ctx* ctx)
 {
     launch_fix_rho(ctx->pnp1, ctx->part_id, ctx->nb, ctx->n, ctx->s.rho_rest, ctx->st);
+    if (ctx->ncols > 0) inlet_update(ctx);
+    if (ctx->planes_on) delete_update(ctx);
     copy_state(ctx, ctx->pn, ctx->pnp1);
 }
 
@@ -949,43 +1251,13 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
     }
     return guarded(ctx, [&] {
         free_all(ctx);
+        ctx->partial_cap = 0;
         ctx->n = n_total;
         ctx->nb = n_bound;
         const int D = ctx->s.dim;
-        alloc_state(ctx, ctx->pn);
-        alloc_state(ctx, ctx->pnp1);
-        dalloc(ctx->x_nxt, size_t(n_total) * D);
-        dalloc(ctx->v_nxt, size_t(n_total) * D);
-        dalloc(ctx->rho_nxt, size_t(n_total));
-        dalloc(ctx->p_nxt, size_t(n_total));
-        dalloc(ctx->part_id, size_t(n_total));
-        dalloc(ctx->blk, size_t(n_total));
-        dalloc(ctx->near_inlet, size_t(n_total));
-        dalloc(ctx->norms_tmp, size_t(n_total) * D);
+        alloc_slots(ctx, n_total, 0);
         ctx->nparts = reduction_parts(n_total);
-        dalloc(ctx->partial, size_t(ctx->nparts) * 8);
-        ctx->partial_cap = int64_t(ctx->nparts) * 8;
-        dalloc(ctx->gather_tmp, size_t(n_total) * D * D);
-        dalloc(ctx->gather_tmp_i, size_t(n_total));
-        dalloc(ctx->cell, size_t(n_total));
-        dalloc(ctx->order, size_t(n_total));
-        dalloc(ctx->pos, size_t(n_total));
-        dalloc(ctx->vorder, size_t(n_total));
-        dalloc(ctx->aux.vol, size_t(n_total));
-        dalloc(ctx->aux.P, size_t(n_total));
-        dalloc(ctx->aux.invrho, size_t(n_total));
-        dalloc(ctx->aux.fq, size_t(n_total));
-        dalloc(ctx->aux.fale, size_t(n_total));
-        dalloc(ctx->aux.nhat, size_t(n_total) * D);
-        dalloc(ctx->aux.lnb, size_t(n_total) * D * D);
-        dalloc(ctx->aux.xb, size_t(n_total) * D);
-        HIPCHK(hipMemsetAsync(ctx->aux.nhat, 0, sizeof(double) * n_total * D, ctx->st));
         ctx->nl.f32 = ctx->s.precision == FJSPH_PREC_MIXED ? 1 : 0;
-        ctx->nl.npad = (n_total + 63) / 64 * 64;
-        dalloc(ctx->nl.count, size_t(ctx->nl.npad));
-        dalloc(ctx->nl.nnb, size_t(ctx->nl.npad));
-        ctx->nl.cap = 0;
-        HIPCHK(hipMemsetAsync(ctx->near_inlet, 0, sizeof(int32_t) * n_total, ctx->st));
         sync(ctx); /* the blocking copies below run on the null stream */
         /* host-side staging of the initial state (src/Init.cpp:288-493,
          * src/FJSPH.cpp:115-126) */
@@ -1056,6 +1328,9 @@ fjsph_status fjsph_set_blocks(fjsph_ctx* ctx, int32_t n_blocks, const fjsph_bloc
         }
     }
     ctx->blocks.clear();
+    ctx->planes.assign(size_t(n_blocks), default_planes());
+    ctx->planes_on = false;
+    ctx->planes_dirty = true;
     for (int k = 0; k < n_blocks; ++k) {
         HostBlock hb;
         hb.b = blocks[k];
@@ -1089,6 +1364,96 @@ fjsph_status fjsph_set_blocks(fjsph_ctx* ctx, int32_t n_blocks, const fjsph_bloc
         HIPCHK(hipMemcpy(ctx->bl.vel, vel.data(), sizeof(double) * vel.size(), hipMemcpyHostToDevice));
         ctx->bl.n = nbk;
     });
+}
+
+fjsph_status fjsph_set_block_planes(fjsph_ctx* ctx, int32_t block, const fjsph_block_planes* planes)
+{
+    if (!ctx || !planes || block < 0 || block >= int32_t(ctx->blocks.size())) return FJSPH_ERR_ARG;
+    if (planes->fixed_vel_or_dynamic < 0 || planes->fixed_vel_or_dynamic > 1) return FJSPH_ERR_ARG;
+    if (ctx->planes.size() < ctx->blocks.size()) ctx->planes.resize(ctx->blocks.size(), default_planes());
+    ctx->planes[size_t(block)] = *planes;
+    ctx->planes_on = true;
+    ctx->planes_dirty = true;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_set_inlet(fjsph_ctx* ctx, int32_t block, int32_t n_cols, int32_t n_buf, const int64_t* back,
+                             const int64_t* buffer)
+{
+    if (!ctx || block < 0 || block >= int32_t(ctx->blocks.size()) || n_cols <= 0 || n_buf <= 0 || !back || !buffer)
+        return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        require(ctx, false);
+        if (ctx->s.integrator != FJSPH_NEWMARK_BETA) throw std::string("inlet blocks need the Newmark-Beta integrator");
+        const fjsph_block& bk = ctx->blocks[size_t(block)].b;
+        if (!bk.is_fluid) throw std::string("set_inlet: not a fluid block");
+        for (const auto& in : ctx->inlets)
+            if (in.block == block) throw std::string("set_inlet: block already has an inlet");
+        for (int64_t k = 0; k < int64_t(n_cols) * (1 + n_buf); ++k) {
+            const int64_t h = k < n_cols ? back[k] : buffer[k - n_cols];
+            if (h < bk.first || h >= bk.last) throw std::string("set_inlet: particle outside the block");
+        }
+        /* host id -> slot */
+        std::vector<int32_t>& sh = slot_to_host(ctx);
+        std::vector<int32_t> slot_of(size_t(ctx->n));
+        for (int64_t sl = 0; sl < ctx->n; ++sl) slot_of[size_t(sh[size_t(sl)])] = int32_t(sl);
+        std::vector<int32_t> col(size_t(ctx->n)), depth(size_t(ctx->n));
+        HIPCHK(hipMemcpy(col.data(), ctx->col, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(depth.data(), ctx->depth, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost));
+        const int32_t base = ctx->ncols;
+        for (int32_t c = 0; c < n_cols; ++c) {
+            const int32_t sb = slot_of[size_t(back[c])];
+            col[size_t(sb)] = base + c;
+            depth[size_t(sb)] = 0;
+            for (int32_t j = 0; j < n_buf; ++j) {
+                const int32_t sl = slot_of[size_t(buffer[size_t(c) * n_buf + j])];
+                col[size_t(sl)] = base + c;
+                depth[size_t(sl)] = j + 1;
+            }
+        }
+        HIPCHK(hipMemcpy(ctx->col, col.data(), sizeof(int32_t) * ctx->n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->depth, depth.data(), sizeof(int32_t) * ctx->n, hipMemcpyHostToDevice));
+        ctx->inlets.push_back({block, base, n_cols, n_buf});
+        std::sort(ctx->inlets.begin(), ctx->inlets.end(),
+                  [](const fjsph_ctx::InletCols& a, const fjsph_ctx::InletCols& b) { return a.block < b.block; });
+        ctx->ncols = base + n_cols;
+        ctx->starved.resize(size_t(ctx->ncols), 0);
+        std::vector<int32_t> cblock(size_t(ctx->ncols));
+        for (const auto& in : ctx->inlets)
+            for (int32_t c = in.col_base; c < in.col_base + in.ncols; ++c) cblock[size_t(c)] = in.block;
+        for (int32_t** a : {&ctx->c_adv, &ctx->c_back, &ctx->c_last, &ctx->c_rank, &ctx->c_block})
+            dalloc(*a, size_t(ctx->ncols));
+        HIPCHK(hipMemcpy(ctx->c_block, cblock.data(), sizeof(int32_t) * ctx->ncols, hipMemcpyHostToDevice));
+        if (ctx->planes.size() < ctx->blocks.size()) ctx->planes.resize(ctx->blocks.size(), default_planes());
+        ctx->planes[size_t(block)].inlet = 1;
+        ctx->planes_on = true;
+        ctx->planes_dirty = true;
+        launch_inlet_backslot(ctx->col, ctx->depth, ctx->n, ctx->c_back, ctx->st);
+        sync(ctx);
+    });
+}
+
+fjsph_status fjsph_set_max_points(fjsph_ctx* ctx, int64_t max_points)
+{
+    if (!ctx || max_points <= 0) return FJSPH_ERR_ARG;
+    ctx->max_points = max_points;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_get_counts(fjsph_ctx* ctx, int64_t* n_total, int64_t* n_bound)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    if (n_total) *n_total = ctx->n;
+    if (n_bound) *n_bound = ctx->nb;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_get_block_range(fjsph_ctx* ctx, int32_t block, int64_t* first, int64_t* last)
+{
+    if (!ctx || block < 0 || block >= int32_t(ctx->blocks.size())) return FJSPH_ERR_ARG;
+    if (first) *first = ctx->blocks[size_t(block)].b.first;
+    if (last) *last = ctx->blocks[size_t(block)].b.last;
+    return FJSPH_OK;
 }
 
 fjsph_status fjsph_set_block_motion(fjsph_ctx* ctx, int32_t block, int32_t n_times, const double* times,

@@ -130,6 +130,18 @@ struct Surrogate {
     double out_shift[5], out_scale[5];
 };
 
+/* Next Newmark iterate (k_nb_update writes, the engine swaps it in). */
+struct NextState {
+    double *x, *v, *rho, *p;
+};
+
+/* Per-block planes on the device (inlet.hip): 4 doubles each (normal, c)
+ * for deletion, insertion and the aero plane; inlet mode and buffer depth. */
+struct DevPlanes {
+    const double *del, *ins, *aero;
+    const int32_t *mode, *nbuf;
+};
+
 /* Segmented list of the aero particles (surrogate.hip): segment length and
  * the stride of its counters (one 128-byte line each). */
 constexpr int64_t AERO_SEG = 65536;
@@ -212,6 +224,26 @@ void launch_aero_list(int dim, const Consts& c, DevState s, const int32_t* part_
                       int32_t* count, hipStream_t st);
 void launch_surrogate_list(int dim, DevState s, Surrogate sg, int64_t n, const int32_t* list, const int32_t* count,
                            hipStream_t st);
+void launch_pipe_outlet(int dim, DevState s, const int32_t* part_id, const int32_t* blk, DevPlanes pl, int64_t n,
+                        int64_t n_bound, hipStream_t st);
+void launch_inlet_backslot(const int32_t* col, const int32_t* depth, int64_t n, int32_t* back_slot, hipStream_t st);
+void launch_inlet_buffer(int dim, const Consts& c, DevState s, DevState pn, NextState nx, const int32_t* col,
+                         const int32_t* depth, const int32_t* blk, const int32_t* back_slot, DevPlanes pl, int64_t n,
+                         double dt, double dx, double* partial, hipStream_t st);
+void launch_inlet_back(int dim, DevState s, int32_t* col, int32_t* depth, const int32_t* blk, DevPlanes pl, int64_t n,
+                       int32_t* adv, int32_t* last_slot, hipStream_t st);
+void launch_inlet_shift(DevState s, int32_t* col, int32_t* depth, const int32_t* blk, DevPlanes pl, int64_t n,
+                        const int32_t* adv, hipStream_t st);
+void launch_inlet_insert(int dim, DevState s, int32_t* blk, int32_t* col, int32_t* depth, const int32_t* col_block,
+                         const int32_t* rank, const int32_t* last_slot, DevPlanes pl, int32_t ncols, int64_t n_old,
+                         double dx, hipStream_t st);
+void launch_shift_ids(int32_t* part_id, int64_t n, int32_t from, int32_t delta, hipStream_t st);
+void launch_new_ids(int32_t* part_id, const int32_t* col_block, const int32_t* rank, int32_t ncols, int32_t k,
+                    int64_t n_old, int32_t rank_base, int32_t first_new, hipStream_t st);
+void launch_delete_flag(int dim, DevState s, const int32_t* part_id, const int32_t* blk, DevPlanes pl, int64_t n,
+                        int64_t n_bound, int32_t* keep, int32_t* hflag, int32_t* blkcount, hipStream_t st);
+void launch_compact_perm(const int32_t* keep, const int32_t* start, int64_t n, int32_t* perm, hipStream_t st);
+void launch_compact_ids(int32_t* part_id, const int32_t* hstart, int64_t n, hipStream_t st);
 void launch_mlp_batch(uint64_t seed, int64_t step, int64_t n, int32_t batch, int32_t* idx, hipStream_t st);
 void launch_mlp_train(int dim, const MlpTrain& m, hipStream_t st);
 void launch_mlp_reduce(const float* part, int64_t nwaves, int64_t stride, int64_t n, float* grad, hipStream_t st);

@@ -85,7 +85,11 @@ __device__ __forceinline__ int32_t union_slot(const TileGrid& tg, const int32_t*
  * from it).  Tile-local slot t of tile k is union index
  * ((t / TGRP) NT + k) TGRP + t % TGRP; a list entry stores k TILE + t, so
  * entries ascend with the tile. */
-constexpr int TGRP = 4;
+#ifndef FJ_TGRP
+#define FJ_TGRP 8
+#endif
+constexpr int TGRP = FJ_TGRP;
+static_assert(TILE % TGRP == 0, "tile slots come in whole groups");
 
 __device__ __forceinline__ int n_tiles(int U) { return (U + TILE - 1) / TILE; }
 
@@ -2100,9 +2104,6 @@ __device__ void gissler(const Consts& c, const double* Vd, double rho, double pr
     for (int a = 0; a < D; ++a) out[a] = f * Vd[a];
 }
 
-struct NextState {
-    double *x, *v, *rho, *p;
-};
 
 /* Occupancy request for the force pass: 5 waves per SIMD (at most 102
  * VGPRs) measured faster than the compiler's 4 at 97 VGPRs. */

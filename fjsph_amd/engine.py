@@ -69,6 +69,11 @@ SIGNATURES = {
     "fjsph_surrogate_train": (C.c_int, [_VP, C.c_int64, _D, _D, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                         C.c_uint64, _D, _D, _D, _D, _F32, _F32, _D]),
     "fjsph_surrogate_grad": (C.c_int, [_VP, C.c_int32, C.c_int32, _F32, _F32, _F32, _F32, _F32, _F32, _D]),
+    "fjsph_set_block_planes": (C.c_int, [_VP, C.c_int32, C.POINTER(abi.BlockPlanes)]),
+    "fjsph_set_inlet": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _I64, _I64]),
+    "fjsph_set_max_points": (C.c_int, [_VP, C.c_int64]),
+    "fjsph_get_counts": (C.c_int, [_VP, _I64, _I64]),
+    "fjsph_get_block_range": (C.c_int, [_VP, C.c_int32, _I64, _I64]),
 }
 
 
@@ -133,9 +138,39 @@ class Engine:
         if rc != abi.OK:
             raise EngineError(f"fjsph_create failed with status {rc}")
         self.h = h
-        self.n = 0
         if hasattr(case_or_settings, "x"):
             self.upload_case(case_or_settings)
+
+    @property
+    def n(self):
+        """Current particle count (inlets add particles, deletion removes them)."""
+        return self.counts()[0]
+
+    def counts(self):
+        nt, nb = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        self._chk(self.lib.fjsph_get_counts(self.h, _ptr(nt, _I64), _ptr(nb, _I64)), "get_counts")
+        return int(nt[0]), int(nb[0])
+
+    def block_range(self, block):
+        f, l = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        self._chk(self.lib.fjsph_get_block_range(self.h, int(block), _ptr(f, _I64), _ptr(l, _I64)), "block_range")
+        return int(f[0]), int(l[0])
+
+    def set_block_planes(self, block, planes):
+        """planes: an abi.BlockPlanes (abi.make_planes)."""
+        self._planes = planes
+        self._chk(self.lib.fjsph_set_block_planes(self.h, int(block), C.byref(planes)), "set_block_planes")
+
+    def set_inlet(self, block, back, buffer):
+        """back: host indices of the columns' BACK particles (n_cols);
+        buffer: (n_cols, n_buf) host indices of their BUFFER particles."""
+        bk = np.ascontiguousarray(back, np.int64)
+        bf = np.ascontiguousarray(buffer, np.int64).reshape(bk.size, -1)
+        self._chk(self.lib.fjsph_set_inlet(self.h, int(block), bk.size, bf.shape[1], _ptr(bk, _I64), _ptr(bf, _I64)),
+                  "set_inlet")
+
+    def set_max_points(self, max_points):
+        self._chk(self.lib.fjsph_set_max_points(self.h, int(max_points)), "set_max_points")
 
     def close(self):
         if getattr(self, "h", None):
@@ -165,6 +200,12 @@ class Engine:
         self.upload_state(case.x, case.v, case.rho, case.p, case.m, case.b, case.n_bound)
         if getattr(case, "mesh", None) is not None:
             self.set_mesh(case.mesh)
+        for blk, kw in (getattr(case, "planes", None) or {}).items():
+            self.set_block_planes(blk, abi.make_planes(**kw))
+        for blk, back, buffer in getattr(case, "inlets", None) or []:
+            self.set_inlet(blk, back, buffer)
+        if getattr(case, "max_points", None):
+            self.set_max_points(case.max_points)
 
     def upload_state(self, x, v, rho, p, m, b, n_bound):
         x = np.ascontiguousarray(x, np.float64)
@@ -173,8 +214,7 @@ class Engine:
         p = np.ascontiguousarray(p, np.float64)
         m = np.ascontiguousarray(m, np.float64)
         b = np.ascontiguousarray(b, np.int32)
-        self.n = x.shape[0]
-        self._chk(self.lib.fjsph_upload_state(self.h, self.n, int(n_bound), _ptr(x), _ptr(v), _ptr(rho),
+        self._chk(self.lib.fjsph_upload_state(self.h, x.shape[0], int(n_bound), _ptr(x), _ptr(v), _ptr(rho),
                                               _ptr(p), _ptr(m), _ptr(b, _I32)), "upload_state")
 
     def set_blocks(self, specs):
@@ -188,9 +228,10 @@ class Engine:
 
     def field(self, name, which=abi.STATE_NP1):
         w = abi.field_width(name, self.dim)
-        out = np.zeros(self.n * w)
+        n = self.n
+        out = np.zeros(n * w)
         self._chk(self.lib.fjsph_download_field(self.h, which, abi.F[name], _ptr(out)), f"download {name}")
-        return out.reshape(self.n, w) if w > 1 else out
+        return out.reshape(n, w) if w > 1 else out
 
     def set_field(self, name, values, which=abi.STATE_NP1):
         a = np.ascontiguousarray(values, np.float64).ravel()

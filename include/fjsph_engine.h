@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FJSPH_ABI_VERSION 3
+#define FJSPH_ABI_VERSION 4
 
 typedef enum fjsph_status {
     FJSPH_OK = 0,
@@ -148,6 +148,20 @@ typedef struct fjsph_block {
     double vel[3];        /* vels[0] */
 } fjsph_block;
 
+/* Planes of a fluid block (bound_block, src/Var.h:858-866): a particle
+ * crosses plane (norm, c) when x . norm > c.  delconst = FJSPH_NO_PLANE
+ * (default_val, src/VarDefs.h:195) switches deletion off; the aero plane
+ * is applied as given (the reference's unset value is norm = (default_val,
+ * default_val, default_val), c = default_val). */
+#define FJSPH_NO_PLANE 9999999.0
+typedef struct fjsph_block_planes {
+    int32_t inlet;                /* block_type == inletZone */
+    int32_t fixed_vel_or_dynamic; /* inlet buffer: 0 fixed velocity, 1 dynamic */
+    double delete_norm[3], delconst;
+    double insert_norm[3], insconst;
+    double aero_norm[3], aeroconst;
+} fjsph_block_planes;
+
 /* Per-step report of Integrator::integrate (src/Integration.cpp:233-303). */
 typedef struct fjsph_step_info {
     double rms_error;   /* log-residual of the last sub-iteration        */
@@ -216,6 +230,29 @@ fjsph_status fjsph_set_blocks(fjsph_ctx* ctx, int32_t n_blocks, const fjsph_bloc
 /* Time-velocity table of a moving boundary block (bound_block::times/vels). */
 fjsph_status fjsph_set_block_motion(fjsph_ctx* ctx, int32_t block, int32_t n_times,
                                     const double* times, const double* vels);
+
+/* Planes of block `block` (bound_block::delete_norm/delconst,
+ * insert_norm/insconst, aero_norm/aeroconst, block_type,
+ * fixed_vel_or_dynamic).  Check_Pipe_Outlet (src/Containment.cpp:822-890)
+ * turns PIPE particles past the aero plane FREE after every pre-pass;
+ * update_data (src/Integration.cpp:109-226) deletes fluid particles past
+ * an active deletion plane. */
+fjsph_status fjsph_set_block_planes(fjsph_ctx* ctx, int32_t block, const fjsph_block_planes* planes);
+/* Inlet columns of an inlet block (bound_block::back and ::buffer, built
+ * by the inlet shape, src/shapes/inlet.cpp): back[c] is the host index of
+ * column c's BACK particle, buffer[c*n_buf + j] its j-th BUFFER particle
+ * behind it.  update_buffer_region (src/shapes/inlet.cpp:578-640) then
+ * runs in every step and the buffer follows the inlet mode in Do_NB_Iter
+ * (src/Newmark_Beta.cpp:243-292).  Needs the Newmark-Beta integrator. */
+fjsph_status fjsph_set_inlet(fjsph_ctx* ctx, int32_t block, int32_t n_cols, int32_t n_buf, const int64_t* back,
+                             const int64_t* buffer);
+/* SPH maximum particle count (svar.max_points, src/Var.h:369): no new inlet
+ * particles past it. */
+fjsph_status fjsph_set_max_points(fjsph_ctx* ctx, int64_t max_points);
+/* Current particle counts (inlets add particles, deletion removes them). */
+fjsph_status fjsph_get_counts(fjsph_ctx* ctx, int64_t* n_total, int64_t* n_bound);
+/* Current host range of a block (index.first, index.second). */
+fjsph_status fjsph_get_block_range(fjsph_ctx* ctx, int32_t block, int64_t* first, int64_t* last);
 
 /* Download one field of pn or pnp1 in host order (see FJSPH_F_*). */
 fjsph_status fjsph_download_field(fjsph_ctx* ctx, int32_t which, int32_t field, double* out);

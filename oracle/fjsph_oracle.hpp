@@ -692,7 +692,27 @@ struct Block {
     fjsph_block b;
     std::vector<double> times;
     std::vector<double> vels; /* n_times * 3 */
+    /* planes (bound_block, src/Var.h:858-866), unset at default_val */
+    fjsph_block_planes pl = unset_planes();
+    std::vector<size_t> back;                 /* inlet: back particle of each column  */
+    std::vector<std::vector<size_t>> buffer;  /* inlet: buffer particles of each column */
+
+    static fjsph_block_planes unset_planes()
+    {
+        fjsph_block_planes q{};
+        for (int a = 0; a < 3; ++a) q.delete_norm[a] = q.insert_norm[a] = q.aero_norm[a] = FJSPH_NO_PLANE;
+        q.delconst = q.insconst = q.aeroconst = FJSPH_NO_PLANE;
+        return q;
+    }
 };
+
+template <int D>
+inline double dot_raw(const Vec<D>& x, const double* n)
+{
+    double s = 0.0;
+    for (int a = 0; a < D; ++a) s += x[a] * n[a];
+    return s;
+}
 
 /* --------------------------------------------------------------- the sim */
 
@@ -854,6 +874,7 @@ struct Sim {
     OUTL outlist;
     std::vector<Block> blocks;
     size_t bound_points = 0, total_points = 0;
+    size_t max_points = 9999999; /* svar.max_points (src/Var.h:369) */
     int nthreads = 1;
     int serial_dissipation = 1; /* src/Shifting.cpp:131 has no pragma */
 
@@ -1424,6 +1445,35 @@ struct Sim {
                     pnp1[ii].xi = pn[ii].xi + dt * pnp1[ii].v;
                 }
             }
+            /* the inlet buffer (src/Newmark_Beta.cpp:243-292), after the
+             * fluid update of the block */
+            if (!bl.pl.inlet) continue;
+            if (bl.pl.fixed_vel_or_dynamic == 1) {
+                Vec<D> unorm;
+                double len = 0.0;
+                for (int a = 0; a < D; ++a) len += bl.pl.insert_norm[a] * bl.pl.insert_norm[a];
+                len = std::sqrt(len);
+                for (int a = 0; a < D; ++a) unorm[a] = bl.pl.insert_norm[a] / len;
+                for (size_t c = 0; c < bl.back.size(); ++c) {
+                    const Part<D>& bk = pnp1[bl.back[c]];
+                    for (size_t jj = 0; jj < bl.buffer[c].size(); ++jj) {
+                        Part<D>& q = pnp1[bl.buffer[c][jj]];
+                        q.xi = bk.xi - (d.dx * (double(jj) + 1.0)) * unorm;
+                        q.v = bk.v;
+                        q.rho = bk.rho;
+                        q.p = bk.p;
+                    }
+                }
+            } else {
+                for (size_t c = 0; c < bl.back.size(); ++c)
+                    for (size_t jj = 0; jj < bl.buffer[c].size(); ++jj) {
+                        const size_t id = bl.buffer[c][jj];
+                        const double rho = std::max(d.rho_min, std::min(d.rho_max, pn[id].rho + dt * (gamma_t1 * pnp1[id].Rrho + gamma_t2 * pn[id].Rrho)));
+                        pnp1[id].rho = rho;
+                        pnp1[id].p = get_pressure(s, d, rho);
+                        pnp1[id].xi = pn[id].xi + dt * pn[id].v;
+                    }
+            }
         }
     }
 
@@ -1620,6 +1670,26 @@ struct Sim {
         Detect_Surface(start_index, end_index);
         dissipation_terms(start_index, end_index);
         if (s.ale) particle_shift(start_index, end_index);
+        Check_Pipe_Outlet();
+    }
+
+    /* Check_Pipe_Outlet (src/Containment.cpp:822-890): PIPE particles past
+     * the aero plane of their block become FREE; with the mesh source they
+     * get their cell (FirstCell; the lookup of every aero particle finds
+     * the cached cell of the others again).  The deletion of a particle
+     * FirstCell places in no cell is not made (counted by the lookup). */
+    void Check_Pipe_Outlet()
+    {
+        bool any = false;
+        for (const Block& bl : blocks) {
+            if (!bl.b.is_fluid) continue;
+            for (size_t ii = size_t(bl.b.first); ii < size_t(bl.b.last); ++ii)
+                if (pnp1[ii].b == FJSPH_PIPE && dot_raw<D>(pnp1[ii].xi, bl.pl.aero_norm) > bl.pl.aeroconst) {
+                    pnp1[ii].b = FJSPH_FREE;
+                    any = true;
+                }
+        }
+        if (any && s.aero_source == FJSPH_MESH_INFL && have_mesh) find_cell(start_index, end_index);
     }
 
     /* integrate_no_update (src/Integration.cpp:27-107) */
@@ -1659,11 +1729,90 @@ struct Sim {
         return rms_error;
     }
 
-    /* update_data (src/Integration.cpp:109-226) without inlets/deletion. */
+    /* update_buffer_region (src/shapes/inlet.cpp:578-640): a back particle
+     * past the insertion plane becomes PIPE, the first buffer particle
+     * becomes the back one, the buffer moves up, and while under max_points
+     * a new BUFFER particle is inserted at the block's end, made from the
+     * last buffer particle as SPHPart(x, pj, BUFFER, id) does
+     * (src/Var.h:550-592), dx behind it along the insertion normal. */
+    size_t update_buffer_region()
+    {
+        size_t nAdd = 0;
+        for (size_t bi = 0; bi < blocks.size(); ++bi) {
+            Block& block = blocks[bi];
+            if (!block.b.is_fluid || !block.pl.inlet) continue;
+            for (size_t ii = 0; ii < block.back.size(); ++ii) {
+                const size_t pID = block.back[ii];
+                if (!(dot_raw<D>(pnp1[pID].xi, block.pl.insert_norm) > block.pl.insconst)) continue;
+                pnp1[pID].b = FJSPH_PIPE;
+                pnp1[block.buffer[ii][0]].b = FJSPH_BACK;
+                block.back[ii] = block.buffer[ii][0];
+                for (size_t jj = 0; jj + 1 < block.buffer[ii].size(); ++jj) block.buffer[ii][jj] = block.buffer[ii][jj + 1];
+                if (total_points < max_points) {
+                    const Part<D>& pj = pnp1[block.buffer[ii].back()];
+                    Part<D> q;
+                    for (int a = 0; a < D; ++a) q.xi[a] = pj.xi[a] - d.dx * block.pl.insert_norm[a];
+                    q.v = pj.v;
+                    q.rho = pj.rho;
+                    q.p = pj.p;
+                    q.m = pj.m;
+                    q.cellP = pj.cellP;
+                    q.cellRho = pj.cellRho;
+                    q.cellID = kNoCell;
+                    q.b = FJSPH_BUFFER;
+                    const size_t at = size_t(block.b.last);
+                    pnp1.insert(pnp1.begin() + long(at), q);
+                    block.buffer[ii].back() = at;
+                    block.b.last++;
+                    for (size_t jj = bi + 1; jj < blocks.size(); ++jj) {
+                        blocks[jj].b.first++;
+                        blocks[jj].b.last++;
+                    }
+                    total_points++;
+                    end_index++;
+                    nAdd++;
+                }
+            }
+        }
+        return nAdd;
+    }
+
+    /* update_data (src/Integration.cpp:109-226): inlets, density floor,
+     * deletion plane (IPT conversion, out of scope, not made), pn = pnp1. */
     void update_data()
     {
+        update_buffer_region();
         for (size_t ii = start_index; ii < end_index; ++ii)
             if (pnp1[ii].rho < 0.0001) pnp1[ii].rho = s.rho_rest;
+        std::vector<size_t> to_del;
+        std::vector<size_t> n_del(blocks.size(), 0);
+        for (size_t bi = 0; bi < blocks.size(); ++bi) {
+            const Block& bl = blocks[bi];
+            if (!bl.b.is_fluid || bl.pl.delconst == FJSPH_NO_PLANE) continue;
+            for (size_t ii = size_t(bl.b.first); ii < size_t(bl.b.last); ++ii)
+                if (dot_raw<D>(pnp1[ii].xi, bl.pl.delete_norm) > bl.pl.delconst) {
+                    to_del.push_back(ii);
+                    n_del[bi]++;
+                }
+        }
+        if (!to_del.empty()) {
+            std::sort(to_del.begin(), to_del.end());
+            for (auto it = to_del.rbegin(); it != to_del.rend(); ++it) {
+                pnp1.erase(pnp1.begin() + long(*it));
+                total_points--;
+            }
+            size_t delshift = 0;
+            for (size_t bi = 0; bi < blocks.size(); ++bi) {
+                Block& bl = blocks[bi];
+                if (!bl.b.is_fluid) continue;
+                bl.b.first -= int64_t(delshift);
+                delshift += n_del[bi];
+                bl.b.last -= int64_t(delshift);
+                for (size_t& x : bl.back) x -= delshift;
+                for (auto& col : bl.buffer)
+                    for (size_t& x : col) x -= delshift;
+            }
+        }
         pn = pnp1;
     }
 

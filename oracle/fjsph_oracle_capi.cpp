@@ -271,6 +271,56 @@ int orc_set_block_motion(void* hp, int block, int n_times, const double* times, 
     return FJSPH_OK;
 }
 
+int orc_set_block_planes(void* hp, int block, const fjsph_block_planes* pl)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    std::vector<Block>& bl = h->dim == 2 ? h->s2->blocks : h->s3->blocks;
+    if (!pl || block < 0 || block >= int(bl.size())) return FJSPH_ERR_ARG;
+    const int inlet = bl[block].pl.inlet;
+    bl[block].pl = *pl;
+    bl[block].pl.inlet = pl->inlet || inlet;
+    return FJSPH_OK;
+}
+
+/* back[c], buffer[c*n_buf + j]: host indices (bound_block::back/buffer) */
+int orc_set_inlet(void* hp, int block, int n_cols, int n_buf, const int64_t* back, const int64_t* buffer)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    std::vector<Block>& bl = h->dim == 2 ? h->s2->blocks : h->s3->blocks;
+    if (block < 0 || block >= int(bl.size()) || n_cols <= 0 || n_buf <= 0) return FJSPH_ERR_ARG;
+    Block& b = bl[block];
+    b.back.assign(back, back + n_cols);
+    b.buffer.assign(size_t(n_cols), std::vector<size_t>());
+    for (int c = 0; c < n_cols; ++c) b.buffer[c].assign(buffer + size_t(c) * n_buf, buffer + size_t(c + 1) * n_buf);
+    b.pl.inlet = 1;
+    return FJSPH_OK;
+}
+
+int orc_set_max_points(void* hp, int64_t max_points)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    if (h->dim == 2) h->s2->max_points = size_t(max_points); else h->s3->max_points = size_t(max_points);
+    return FJSPH_OK;
+}
+
+int orc_get_counts(void* hp, int64_t* n_total, int64_t* n_bound)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    *n_total = int64_t(h->dim == 2 ? h->s2->pnp1.size() : h->s3->pnp1.size());
+    *n_bound = int64_t(h->dim == 2 ? h->s2->bound_points : h->s3->bound_points);
+    return FJSPH_OK;
+}
+
+int orc_get_block_range(void* hp, int block, int64_t* first, int64_t* last)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    const std::vector<Block>& bl = h->dim == 2 ? h->s2->blocks : h->s3->blocks;
+    if (block < 0 || block >= int(bl.size())) return FJSPH_ERR_ARG;
+    *first = bl[block].b.first;
+    *last = bl[block].b.last;
+    return FJSPH_OK;
+}
+
 int orc_download_field(void* hp, int which, int field, double* out)
 {
     Handle* h = static_cast<Handle*>(hp);

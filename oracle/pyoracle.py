@@ -40,6 +40,11 @@ def load(kind="oracle"):
         "orc_upload_state": (C.c_int, [vp, C.c_int64, C.c_int64, _D, _D, _D, _D, _D, _I32]),
         "orc_set_blocks": (C.c_int, [vp, C.c_int, C.POINTER(abi.Block)]),
         "orc_set_block_motion": (C.c_int, [vp, C.c_int, C.c_int, _D, _D]),
+        "orc_set_block_planes": (C.c_int, [vp, C.c_int, C.POINTER(abi.BlockPlanes)]),
+        "orc_set_inlet": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _I64, _I64]),
+        "orc_set_max_points": (C.c_int, [vp, C.c_int64]),
+        "orc_get_counts": (C.c_int, [vp, _I64, _I64]),
+        "orc_get_block_range": (C.c_int, [vp, C.c_int, _I64, _I64]),
         "orc_download_field": (C.c_int, [vp, C.c_int, C.c_int, _D]),
         "orc_upload_field": (C.c_int, [vp, C.c_int, C.c_int, _D]),
         "orc_build_neighbours": (C.c_int, [vp, C.c_int]),
@@ -110,9 +115,32 @@ class Oracle:
         self.h = self.lib.orc_create(C.byref(self.settings), int(nthreads))
         if not self.h:
             raise RuntimeError("orc_create failed")
-        self.n = 0
         if hasattr(case_or_settings, "x"):
             self.upload_case(case_or_settings)
+
+    @property
+    def n(self):
+        nt, nb = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        self._chk(self.lib.orc_get_counts(self.h, _ptr(nt, _I64), _ptr(nb, _I64)), "get_counts")
+        return int(nt[0])
+
+    def block_range(self, block):
+        f, l = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        self._chk(self.lib.orc_get_block_range(self.h, int(block), _ptr(f, _I64), _ptr(l, _I64)), "block_range")
+        return int(f[0]), int(l[0])
+
+    def set_block_planes(self, block, planes):
+        self._planes = planes
+        self._chk(self.lib.orc_set_block_planes(self.h, int(block), C.byref(planes)), "set_block_planes")
+
+    def set_inlet(self, block, back, buffer):
+        bk = np.ascontiguousarray(back, np.int64)
+        bf = np.ascontiguousarray(buffer, np.int64).reshape(bk.size, -1)
+        self._chk(self.lib.orc_set_inlet(self.h, int(block), bk.size, bf.shape[1], _ptr(bk, _I64), _ptr(bf, _I64)),
+                  "set_inlet")
+
+    def set_max_points(self, max_points):
+        self._chk(self.lib.orc_set_max_points(self.h, int(max_points)), "set_max_points")
 
     def close(self):
         if self.h:
@@ -134,6 +162,12 @@ class Oracle:
         self.set_blocks(case.blocks)
         if getattr(case, "mesh", None) is not None:
             self.set_mesh(case.mesh)
+        for blk, kw in (getattr(case, "planes", None) or {}).items():
+            self.set_block_planes(blk, abi.make_planes(**kw))
+        for blk, back, buffer in getattr(case, "inlets", None) or []:
+            self.set_inlet(blk, back, buffer)
+        if getattr(case, "max_points", None):
+            self.set_max_points(case.max_points)
 
     def set_mesh(self, mesh):
         self._mesh = mesh
@@ -169,8 +203,7 @@ class Oracle:
         p = np.ascontiguousarray(p, np.float64)
         m = np.ascontiguousarray(m, np.float64)
         b = np.ascontiguousarray(b, np.int32)
-        self.n = x.shape[0]
-        self._chk(self.lib.orc_upload_state(self.h, self.n, int(n_bound), _ptr(x), _ptr(v), _ptr(rho),
+        self._chk(self.lib.orc_upload_state(self.h, x.shape[0], int(n_bound), _ptr(x), _ptr(v), _ptr(rho),
                                             _ptr(p), _ptr(m), _ptr(b, _I32)), "upload")
 
     def set_blocks(self, specs):
@@ -182,9 +215,10 @@ class Oracle:
 
     def field(self, name, which=abi.STATE_NP1):
         w = abi.field_width(name, self.dim)
-        out = np.zeros(self.n * w)
+        n = self.n
+        out = np.zeros(n * w)
         self._chk(self.lib.orc_download_field(self.h, which, abi.F[name], _ptr(out)), f"download {name}")
-        return out.reshape(self.n, w) if w > 1 else out
+        return out.reshape(n, w) if w > 1 else out
 
     def set_field(self, name, values, which=abi.STATE_NP1):
         a = np.ascontiguousarray(values, np.float64).ravel()

@@ -324,3 +324,62 @@ def test_dam_break_drift_fp64():
     for f in ("X", "V", "RHO", "P"):
         e = assert_close(f, eng.field(f), orc.field(f), 1e-7)
         print(f"drift {f}: {e:.2e}", flush=True)
+
+
+def _inlet_cases():
+    out = [("fixed", cases.pipe_inlet_3d(mode=0)), ("dynamic", cases.pipe_inlet_3d(mode=1))]
+    for mode, name in ((0, "fixed_capped"), (1, "dynamic_capped")):
+        capped = cases.pipe_inlet_3d(mode=mode)
+        ncol = capped.inlets[0][1].shape[0]
+        # the insertion comes after one layer is deleted; max_points stops
+        # it part way through the columns
+        capped.max_points = capped.n - ncol // 2
+        out.append((name, capped))
+    return out
+
+
+@pytest.mark.parametrize("name,case", _inlet_cases(), ids=lambda v: v if isinstance(v, str) else "")
+def test_inlet_parity_fp64(name, case):
+    """Pipe inlet (src/shapes/inlet.cpp: buffer, column advance and
+    insertion), the pipe exit (Check_Pipe_Outlet) and the deletion plane
+    (src/Integration.cpp update_data) over 6 steps that see one deletion
+    and one insertion: counts, block ranges and particle types exactly,
+    the fields to the FP64 step tolerance, in host order."""
+    print(f"[inlet parity] {name}: n={case.n} max_points={case.max_points}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    eng.integrate_no_update()
+    orc.integrate_no_update()
+    ns = []
+    for step in range(6):
+        ie, io = eng.integrate_step(), orc.integrate_step()
+        assert ie["iterations"] == io["iterations"], f"step {step}: {ie} vs {io}"
+        assert abs(ie["dt"] - io["dt"]) <= 1e-10 * io["dt"]
+        assert eng.n == orc.n, f"step {step}: {eng.n} vs {orc.n} particles"
+        assert tuple(eng.block_range(0)) == tuple(orc.block_range(0))
+        assert np.array_equal(eng.field("B"), orc.field("B")), f"step {step}: particle types"
+        ns.append(eng.n)
+        for f in ("X", "V", "RHO", "P"):
+            assert_close(f, eng.field(f), orc.field(f), FP64_STEP)
+    print("particle counts:", ns, flush=True)
+    assert min(ns) < case.n, "no deletion seen"
+    assert max(ns) > min(ns), "no insertion seen"
+
+
+def test_inlet_parity_mixed():
+    """The pipe inlet under the mixed-precision pair passes: the integer
+    decisions (insertion, deletion, types) exactly, fields to the mixed
+    tolerance."""
+    case = cases.pipe_inlet_3d(mode=0, precision=abi.PREC_MIXED)
+    print(f"[inlet parity mixed] n={case.n}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    eng.integrate_no_update()
+    orc.integrate_no_update()
+    for step in range(6):
+        eng.integrate_step()
+        orc.integrate_step()
+        assert eng.n == orc.n
+        assert np.array_equal(eng.field("B"), orc.field("B")), f"step {step}: particle types"
+    rho0 = case.settings.rho_rest
+    for f, g, r in (("V", eng.field("V"), orc.field("V")),
+                    ("RHO-rho0", eng.field("RHO") - rho0, orc.field("RHO") - rho0)):
+        print(f"{f}: {assert_close(f, g, r, 1e-3):.2e}", flush=True)

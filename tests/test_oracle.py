@@ -671,3 +671,109 @@ def test_check_cell_kuhn_matches_closed_form(po):
         assert o.check_cell(int(ref[k]), x[k]) == 1
         other = int((ref[k] + 1 + k % 5) % m.ncells)
         assert o.check_cell(other, x[k]) == 0
+
+
+def _columns(c):
+    """xy of the inlet columns (the disk lattice) of a pipe_inlet_3d case."""
+    return c.x[c.inlets[0][1], :2]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_inlet_buffer_insertion_deletion(po, mode):
+    """The inlet (update_buffer_region, the Do_NB_Iter buffer update) and the
+    deletion plane of the oracle against closed forms over 6 steps that see
+    one deletion and one column advance in every column:
+    - fixed velocity (mode 0): every BUFFER particle sits at
+      z = -k dx + speed * t (k = 1..n_buf+1) on its column;
+    - dynamic (mode 1): every BUFFER particle sits k dx behind the BACK
+      particle of its column, with its velocity;
+    - there are always n_buf BUFFER particles per column;
+    - a column advance turns the BACK layer PIPE and appends the column's
+      new BUFFER particles at the end of the block, (n_buf+1) dx behind
+      the former back layer;
+    - no particle is left past the deletion plane, and the count changes
+      by the insertions minus the deletions."""
+    c = cases.pipe_inlet_3d(mode=mode, n_pipe=4, n_buf=3)
+    cols = _columns(c)
+    ncol, nbuf = cols.shape[0], 3
+    dx = c.settings.particle_step
+    speed = float(c.v[0, 2])
+    zdel = c.planes[0]["delete"][1]
+    o = po.Oracle(c, nthreads=4)
+    o.integrate_no_update()
+    t = 0.0
+    inserted = deleted = 0
+    for _ in range(6):
+        n0 = o.n
+        npipe0 = int((o.field("B") == abi.PIPE).sum())
+        t += o.integrate_step()["dt"]
+        x, v, b = o.field("X"), o.field("V"), o.field("B").astype(int)
+        buf = b == abi.BUFFER
+        assert buf.sum() == ncol * nbuf
+        # every buffer particle is on a column
+        dxy = np.abs(x[buf, None, :2] - cols[None, :, :2]).max(axis=2)
+        col_of = dxy.argmin(axis=1)
+        # dynamic BACK particles drift off their column by rounding only
+        assert dxy.min(axis=1).max() < 1e-6 * dx
+        if mode == 0:
+            k = (speed * t - x[buf, 2]) / dx
+            assert np.abs(k - np.round(k)).max() < 1e-7 and k.min() > 0.5 and k.max() < nbuf + 1.5
+        else:
+            backs = np.nonzero(b == abi.BACK)[0]
+            bxy = np.abs(x[backs, None, :2] - cols[None, :, :2]).max(axis=2).argmin(axis=1)
+            back_of_col = dict(zip(bxy.tolist(), backs.tolist()))
+            bk = np.array([back_of_col[cc] for cc in col_of])
+            k = (x[bk, 2] - x[buf, 2]) / dx
+            assert np.abs(k - np.round(k)).max() < 1e-9 and set(np.round(k).astype(int)) <= set(range(1, nbuf + 1))
+            assert np.array_equal(v[buf], v[bk])
+        npipe1 = int((b == abi.PIPE).sum())
+        ins = 0
+        if npipe1 == npipe0 + ncol:
+            ins = ncol
+            first, last = o.block_range(0)
+            new = np.arange(last - ncol, last)
+            assert np.all(b[new] == abi.BUFFER)
+            if mode == 0:
+                assert np.allclose(x[new, 2], speed * t - (nbuf + 1) * dx, rtol=0, atol=1e-12)
+        assert not np.any(x[b > abi.BUFFER, 2] > zdel)
+        inserted += ins
+        deleted += n0 + ins - o.n
+    assert inserted == ncol
+    assert deleted == ncol
+    assert o.n == c.n + inserted - deleted
+
+
+def test_inlet_capped_columns(po):
+    """max_points reached part way through the columns (src/shapes/inlet.cpp
+    update_buffer_region): the columns are served in order until the count
+    reaches max_points; a column left without a new particle keeps n_buf - 1
+    BUFFER particles, its last one listed twice, so in dynamic mode the
+    buffer pass puts it n_buf dx behind the back particle (depths
+    1 .. n_buf - 2 and n_buf)."""
+    c = cases.pipe_inlet_3d(mode=1, n_pipe=4, n_buf=3)
+    cols = _columns(c)
+    ncol, nbuf = cols.shape[0], 3
+    dx = c.settings.particle_step
+    c.max_points = c.n - ncol // 2
+    o = po.Oracle(c, nthreads=4)
+    o.integrate_no_update()
+    for _ in range(6):
+        o.integrate_step()
+    assert o.n == c.max_points
+    x, b = o.field("X"), o.field("B").astype(int)
+    backs = np.nonzero(b == abi.BACK)[0]
+    assert backs.size == ncol
+    bcol = np.abs(x[backs, None, :2] - cols[None, :, :2]).max(axis=2).argmin(axis=1)
+    back_of_col = dict(zip(bcol.tolist(), backs.tolist()))
+    buf = np.nonzero(b == abi.BUFFER)[0]
+    fcol = np.abs(x[buf, None, :2] - cols[None, :, :2]).max(axis=2).argmin(axis=1)
+    depths = {}
+    for i, cc in zip(buf.tolist(), fcol.tolist()):
+        k = (x[back_of_col[cc], 2] - x[i, 2]) / dx
+        assert abs(k - round(k)) < 1e-9
+        depths.setdefault(cc, []).append(int(round(k)))
+    full = sorted(cc for cc, d in depths.items() if sorted(d) == list(range(1, nbuf + 1)))
+    short = sorted(cc for cc, d in depths.items() if sorted(d) == list(range(1, nbuf - 1)) + [nbuf])
+    assert len(full) + len(short) == ncol
+    # the first columns in order are the served ones
+    assert full == list(range(len(full))) and len(full) == c.max_points - (c.n - ncol)
