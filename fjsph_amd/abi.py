@@ -6,7 +6,7 @@ fjsph_settings_default().
 """
 import ctypes as C
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 # fjsph_status
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
@@ -26,7 +26,7 @@ FIELDS = [
     "X", "V", "ACC", "AF", "AVISC", "RHO", "P", "RRHO", "M", "DELTAD", "L",
     "GRADRHO", "NORM", "LAM", "LAM_NB", "COLOURG", "COLOUR", "KERNSUM", "SURF",
     "CURVE", "NORM_CURVE", "WOCCL", "PDIST", "CELLV", "CELLP", "CELLRHO",
-    "CELLID", "VPERT", "SURFZONE", "B",
+    "CELLID", "VPERT", "SURFZONE", "B", "PID",
 ]
 F = {name: i for i, name in enumerate(FIELDS)}
 VECTOR_FIELDS = {"X", "V", "ACC", "AF", "AVISC", "GRADRHO", "NORM", "CELLV", "VPERT"}
@@ -121,6 +121,16 @@ def make_planes(inlet=0, fixed_vel_or_dynamic=0, delete=None, insert=None, aero=
             vec[a] = float(nrm[a]) if a < len(nrm) else 0.0
         setattr(q, {"delete": "delconst", "insert": "insconst", "aero": "aeroconst"}[name], float(c))
     return q
+
+
+class Clock(C.Structure):
+    """fjsph_clock: the integrator state carried across steps and restarts."""
+    _fields_ = [("current_time", C.c_double), ("last_frame_time", C.c_double), ("delta_t", C.c_double),
+                ("cfl", C.c_double), ("current_frame", C.c_int64), ("n_stable", C.c_int64),
+                ("n_unstable", C.c_int64), ("next_part_id", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class StepInfo(C.Structure):

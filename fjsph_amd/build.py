@@ -1,9 +1,11 @@
-"""Build the engine library fjsph_amd/libfjsph_engine.so for gfx950.
+"""Build the engine library fjsph_amd/libfjsph_engine.so for gfx950, and the
+restart-file library fjsph_amd/libfjsph_restart.so.
 
 hipcc compiles the kernels and the host C-ABI in one shared object.  Packed
 FP32 instructions are disabled (-packed-fp32-ops) on purpose; the FP32 paths
-are scalar.  The CPU restatement used as the test oracle is built separately
-by oracle/Makefile.
+are scalar.  The restart library is host code (g++) over the engine's C-ABI
+and the HDF5 C library of the image (/opt/conda).  The CPU restatement used
+as the test oracle is built separately by oracle/Makefile.
 """
 import os
 import shutil
@@ -13,6 +15,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfjsph_engine.so")
+RESTART_LIB = os.path.join(HERE, "libfjsph_restart.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+HDF5_ROOT = os.environ.get("FJSPH_HDF5_ROOT", "/opt/conda")
+SYSTEM_LIBDIR = "/usr/lib/x86_64-linux-gnu"
 SOURCES = ["kernels.hip", "pairs.hip", "mesh.hip", "surrogate.hip", "surrogate_train.hip", "inlet.hip", "engine.cpp",
            "settings.cpp"]
 ARCH = os.environ.get("FJSPH_OFFLOAD_ARCH", "gfx950")
@@ -59,5 +65,25 @@ def build(verbose=False, lib=LIB, build_dir=None, defines=()):
     return lib
 
 
+def build_restart(verbose=False):
+    """Link libfjsph_restart.so against libfjsph_engine.so and libhdf5."""
+    inc = os.path.join(HDF5_ROOT, "include")
+    libdir = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        raise RuntimeError(f"hdf5.h not found under {inc}")
+    tmp = RESTART_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", f"-I{inc}",
+           os.path.join(CSRC, "restart.cpp"), "-o", tmp, f"-L{HERE}", "-lfjsph_engine", f"-L{libdir}", "-lhdf5",
+           # the system library directory comes before the HDF5 one, so that
+           # only libhdf5 (and not an older libstdc++) is taken from there
+           f"-Wl,-rpath,$ORIGIN:{SYSTEM_LIBDIR}:{libdir}", "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, RESTART_LIB)
+    return RESTART_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_restart(verbose="-v" in sys.argv))

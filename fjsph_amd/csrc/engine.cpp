@@ -40,7 +40,7 @@ FieldDesc field_desc(int f)
         return {1, false};
     case FJSPH_F_L:
         return {2, false};
-    case FJSPH_F_SURF: case FJSPH_F_CELLID: case FJSPH_F_SURFZONE: case FJSPH_F_B:
+    case FJSPH_F_SURF: case FJSPH_F_CELLID: case FJSPH_F_SURFZONE: case FJSPH_F_B: case FJSPH_F_PID:
         return {0, true};
     default:
         return {0, false};
@@ -80,6 +80,7 @@ void** field_slot(DevState& s, int f)
     case FJSPH_F_VPERT: return (void**)&s.vPert;
     case FJSPH_F_SURFZONE: return (void**)&s.surfzone;
     case FJSPH_F_B: return (void**)&s.b;
+    case FJSPH_F_PID: return (void**)&s.pid;
     default: return nullptr;
     }
 }
@@ -158,6 +159,7 @@ struct fjsph_ctx {
     /* integrator state (INTEG_SETT + Integrator members) */
     double current_time = 0.0, last_frame_time = 0.0, delta_t = 2e-10, cfl = 1.0, safe_dt = 0.0;
     unsigned n_stable = 0, n_unstable = 0, iteration = 0;
+    int64_t current_frame = 0, next_part_id = 0;
     int rollbacks = 0;
     double maxf = 0, maxAf = 0, maxRho_pc = 0, maxRhoi = 0, maxdrho = 0, minST = 0, maxU = 0, maxShift = 0;
     double last_npd = 1.0;
@@ -1042,8 +1044,11 @@ void inlet_update(fjsph_ctx* ctx)
         }
         HIPCHK(hipMemsetAsync(ctx->near_inlet + n_old, 0, sizeof(int32_t) * nadd, ctx->st));
         HIPCHK(hipMemcpyAsync(ctx->c_rank, rank.data(), sizeof(int32_t) * nc, hipMemcpyHostToDevice, ctx->st));
+        /* particle ids svar.part_id++ in insertion order (src/shapes/inlet.cpp:614-632) */
+        if (ctx->next_part_id + nadd >= (int64_t(1) << 31)) throw std::string("particle ids past 2^31");
         launch_inlet_insert(D, ctx->pnp1, ctx->blk, ctx->col, ctx->depth, ctx->c_block, ctx->c_rank, ctx->c_last,
-                            ctx->dpl, nc, n_old, ctx->d.dx, ctx->st);
+                            ctx->dpl, nc, n_old, ctx->d.dx, ctx->next_part_id, ctx->st);
+        ctx->next_part_id += nadd;
         /* host ids: block by block in host order */
         int32_t rank_base = 0;
         for (size_t k = 0; k < ctx->blocks.size(); ++k) {
@@ -1286,6 +1291,8 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
             }
         }
         HIPCHK(hipMemcpy(ctx->part_id, ids.data(), sizeof(int32_t) * n_total, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->pn.pid, ids.data(), sizeof(int32_t) * n_total, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->pnp1.pid, ids.data(), sizeof(int32_t) * n_total, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->blk, blk.data(), sizeof(int32_t) * n_total, hipMemcpyHostToDevice));
         ctx->c.n_bound = n_bound;
         ctx->have_state = true;
@@ -1295,6 +1302,8 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
         ctx->cfl = ctx->s.cfl;
         ctx->delta_t = ctx->s.delta_t_min > 0 ? ctx->s.delta_t_min : 2e-10;
         ctx->n_stable = ctx->n_unstable = 0;
+        ctx->current_frame = 0;
+        ctx->next_part_id = n_total; /* svar.part_id after Init_Particles (src/Init.cpp:478) */
         /* blocks may have been set before the state: push them */
         if (!ctx->blocks.empty()) {
             const int nbk = int(ctx->blocks.size());
@@ -1440,6 +1449,13 @@ fjsph_status fjsph_set_max_points(fjsph_ctx* ctx, int64_t max_points)
     return FJSPH_OK;
 }
 
+fjsph_status fjsph_get_max_points(fjsph_ctx* ctx, int64_t* max_points)
+{
+    if (!ctx || !max_points) return FJSPH_ERR_ARG;
+    *max_points = ctx->max_points;
+    return FJSPH_OK;
+}
+
 fjsph_status fjsph_get_counts(fjsph_ctx* ctx, int64_t* n_total, int64_t* n_bound)
 {
     if (!ctx) return FJSPH_ERR_ARG;
@@ -1453,6 +1469,111 @@ fjsph_status fjsph_get_block_range(fjsph_ctx* ctx, int32_t block, int64_t* first
     if (!ctx || block < 0 || block >= int32_t(ctx->blocks.size())) return FJSPH_ERR_ARG;
     if (first) *first = ctx->blocks[size_t(block)].b.first;
     if (last) *last = ctx->blocks[size_t(block)].b.last;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_get_blocks(fjsph_ctx* ctx, int32_t* n_blocks, fjsph_block* blocks, int32_t capacity)
+{
+    if (!ctx || !n_blocks) return FJSPH_ERR_ARG;
+    *n_blocks = int32_t(ctx->blocks.size());
+    if (!blocks) return FJSPH_OK;
+    if (capacity < *n_blocks) return FJSPH_ERR_ARG;
+    for (size_t k = 0; k < ctx->blocks.size(); ++k) blocks[k] = ctx->blocks[k].b;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_get_block_motion(fjsph_ctx* ctx, int32_t block, int32_t* n_times, double* times, double* vels)
+{
+    if (!ctx || !n_times || block < 0 || block >= int32_t(ctx->blocks.size())) return FJSPH_ERR_ARG;
+    const HostBlock& hb = ctx->blocks[size_t(block)];
+    *n_times = int32_t(hb.times.size());
+    if (times) std::copy(hb.times.begin(), hb.times.end(), times);
+    if (vels) std::copy(hb.vels.begin(), hb.vels.end(), vels);
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_get_block_planes(fjsph_ctx* ctx, int32_t block, fjsph_block_planes* out)
+{
+    if (!ctx || !out || block < 0 || block >= int32_t(ctx->blocks.size())) return FJSPH_ERR_ARG;
+    *out = size_t(block) < ctx->planes.size() ? ctx->planes[size_t(block)] : default_planes();
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_get_inlet(fjsph_ctx* ctx, int32_t block, int32_t* n_cols, int32_t* n_buf, int64_t* back,
+                             int64_t* buffer)
+{
+    if (!ctx || !n_cols || !n_buf || block < 0 || block >= int32_t(ctx->blocks.size())) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        *n_cols = 0;
+        *n_buf = 0;
+        const fjsph_ctx::InletCols* in = nullptr;
+        for (const auto& c : ctx->inlets)
+            if (c.block == block) in = &c;
+        if (!in) return;
+        *n_cols = in->ncols;
+        *n_buf = in->nbuf;
+        if (!back || !buffer) return;
+        const int64_t n = ctx->n;
+        const std::vector<int32_t>& sh = slot_to_host(ctx);
+        std::vector<int32_t> col(static_cast<size_t>(n)), depth(static_cast<size_t>(n));
+        HIPCHK(hipMemcpy(col.data(), ctx->col, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(depth.data(), ctx->depth, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        const int32_t nb = in->nbuf;
+        std::vector<int64_t> bk(size_t(in->ncols), -1), bf(size_t(in->ncols) * nb, -1);
+        for (int64_t sl = 0; sl < n; ++sl) {
+            const int32_t c = col[size_t(sl)] - in->col_base;
+            if (c < 0 || c >= in->ncols) continue;
+            const int32_t d = depth[size_t(sl)];
+            if (d == 0) bk[size_t(c)] = sh[size_t(sl)];
+            else if (d > 0 && d <= nb) bf[size_t(c) * nb + (d - 1)] = sh[size_t(sl)];
+        }
+        for (int32_t c = 0; c < in->ncols; ++c) {
+            const int64_t deepest = bf[size_t(c) * nb + (nb - 1)];
+            if (bk[size_t(c)] < 0 || deepest < 0) throw std::string("get_inlet: a column lost its back or last particle");
+            /* places vacated by advances without a new particle repeat the
+             * deepest particle, as the reference's shifted list does */
+            for (int32_t j = 0; j < nb; ++j)
+                if (bf[size_t(c) * nb + j] < 0) bf[size_t(c) * nb + j] = deepest;
+            back[c] = bk[size_t(c)];
+            for (int32_t j = 0; j < nb; ++j) buffer[size_t(c) * nb + j] = bf[size_t(c) * nb + j];
+        }
+    });
+}
+
+fjsph_status fjsph_get_clock(fjsph_ctx* ctx, fjsph_clock* out)
+{
+    if (!ctx || !out) return FJSPH_ERR_ARG;
+    out->current_time = ctx->current_time;
+    out->last_frame_time = ctx->last_frame_time;
+    out->delta_t = ctx->delta_t;
+    out->cfl = ctx->cfl;
+    out->current_frame = ctx->current_frame;
+    out->n_stable = ctx->n_stable;
+    out->n_unstable = ctx->n_unstable;
+    out->next_part_id = ctx->next_part_id;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_set_clock(fjsph_ctx* ctx, const fjsph_clock* in)
+{
+    if (!ctx || !in || in->n_stable < 0 || in->n_unstable < 0 || in->current_frame < 0 || in->next_part_id < 0 ||
+        in->next_part_id >= (int64_t(1) << 31))
+        return FJSPH_ERR_ARG;
+    ctx->current_time = in->current_time;
+    ctx->last_frame_time = in->last_frame_time;
+    ctx->delta_t = in->delta_t;
+    ctx->cfl = in->cfl;
+    ctx->current_frame = in->current_frame;
+    ctx->n_stable = unsigned(in->n_stable);
+    ctx->n_unstable = unsigned(in->n_unstable);
+    ctx->next_part_id = in->next_part_id;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_get_settings(fjsph_ctx* ctx, fjsph_settings* out)
+{
+    if (!ctx || !out) return FJSPH_ERR_ARG;
+    *out = ctx->s;
     return FJSPH_OK;
 }
 
@@ -2158,6 +2279,7 @@ fjsph_status fjsph_advance_frame(fjsph_ctx* ctx)
 {
     if (!ctx) return FJSPH_ERR_ARG;
     ctx->last_frame_time += ctx->s.frame_time_interval;
+    ctx->current_frame++;
     return FJSPH_OK;
 }
 

@@ -36,7 +36,7 @@ struct DevState {
     double *x, *v, *acc, *Af, *aVisc, *Rrho, *rho, *p, *m, *deltaD, *L, *gradRho, *norm;
     double *lam, *lam_nb, *colourG, *colour, *kernsum, *curve, *norm_curve, *woccl, *pDist;
     double *cellV, *cellP, *cellRho, *vPert;
-    int32_t *surf, *cellID, *surfzone, *b;
+    int32_t *surf, *cellID, *surfzone, *b, *pid;
 };
 
 /* Neighbour lists in rows of 4 entries: row r of the target at position p
@@ -236,7 +236,7 @@ void launch_inlet_shift(DevState s, int32_t* col, int32_t* depth, const int32_t*
                         const int32_t* adv, hipStream_t st);
 void launch_inlet_insert(int dim, DevState s, int32_t* blk, int32_t* col, int32_t* depth, const int32_t* col_block,
                          const int32_t* rank, const int32_t* last_slot, DevPlanes pl, int32_t ncols, int64_t n_old,
-                         double dx, hipStream_t st);
+                         double dx, int64_t id_base, hipStream_t st);
 void launch_shift_ids(int32_t* part_id, int64_t n, int32_t from, int32_t delta, hipStream_t st);
 void launch_new_ids(int32_t* part_id, const int32_t* col_block, const int32_t* rank, int32_t ncols, int32_t k,
                     int64_t n_old, int32_t rank_base, int32_t first_new, hipStream_t st);

@@ -161,7 +161,7 @@ __global__ __launch_bounds__(TPB) void k_inlet_insert(DevState s, int32_t* __res
                                                       const int32_t* __restrict__ col_block,
                                                       const int32_t* __restrict__ rank,
                                                       const int32_t* __restrict__ last_slot, DevPlanes pl,
-                                                      int32_t ncols, int64_t n_old, double dx)
+                                                      int32_t ncols, int64_t n_old, double dx, int64_t id_base)
 {
     const int cc = blockIdx.x * TPB + threadIdx.x;
     if (cc >= ncols || rank[cc] < 0) return;
@@ -181,6 +181,7 @@ __global__ __launch_bounds__(TPB) void k_inlet_insert(DevState s, int32_t* __res
     s.cellRho[dst] = s.cellRho[src];
     s.cellID[dst] = -3;
     s.b[dst] = FJSPH_BUFFER;
+    s.pid[dst] = int32_t(id_base + rank[cc]);
     blk[dst] = k;
     col[dst] = cc;
     depth[dst] = pl.nbuf[k];
@@ -291,15 +292,15 @@ void launch_inlet_shift(DevState s, int32_t* col, int32_t* depth, const int32_t*
 
 void launch_inlet_insert(int dim, DevState s, int32_t* blk, int32_t* col, int32_t* depth, const int32_t* col_block,
                          const int32_t* rank, const int32_t* last_slot, DevPlanes pl, int32_t ncols, int64_t n_old,
-                         double dx, hipStream_t st)
+                         double dx, int64_t id_base, hipStream_t st)
 {
     const dim3 g(grid_of(ncols)), b(TPB);
     if (dim == 2)
         hipLaunchKernelGGL(k_inlet_insert<2>, g, b, 0, st, s, blk, col, depth, col_block, rank, last_slot, pl, ncols,
-                           n_old, dx);
+                           n_old, dx, id_base);
     else
         hipLaunchKernelGGL(k_inlet_insert<3>, g, b, 0, st, s, blk, col, depth, col_block, rank, last_slot, pl, ncols,
-                           n_old, dx);
+                           n_old, dx, id_base);
 }
 
 void launch_shift_ids(int32_t* part_id, int64_t n, int32_t from, int32_t delta, hipStream_t st)

@@ -74,6 +74,14 @@ SIGNATURES = {
     "fjsph_set_max_points": (C.c_int, [_VP, C.c_int64]),
     "fjsph_get_counts": (C.c_int, [_VP, _I64, _I64]),
     "fjsph_get_block_range": (C.c_int, [_VP, C.c_int32, _I64, _I64]),
+    "fjsph_get_blocks": (C.c_int, [_VP, _I32, C.POINTER(abi.Block), C.c_int32]),
+    "fjsph_get_block_motion": (C.c_int, [_VP, C.c_int32, _I32, _D, _D]),
+    "fjsph_get_block_planes": (C.c_int, [_VP, C.c_int32, C.POINTER(abi.BlockPlanes)]),
+    "fjsph_get_inlet": (C.c_int, [_VP, C.c_int32, _I32, _I32, _I64, _I64]),
+    "fjsph_get_clock": (C.c_int, [_VP, C.POINTER(abi.Clock)]),
+    "fjsph_set_clock": (C.c_int, [_VP, C.POINTER(abi.Clock)]),
+    "fjsph_get_settings": (C.c_int, [_VP, C.POINTER(abi.Settings)]),
+    "fjsph_get_max_points": (C.c_int, [_VP, _I64]),
 }
 
 
@@ -168,6 +176,44 @@ class Engine:
         bf = np.ascontiguousarray(buffer, np.int64).reshape(bk.size, -1)
         self._chk(self.lib.fjsph_set_inlet(self.h, int(block), bk.size, bf.shape[1], _ptr(bk, _I64), _ptr(bf, _I64)),
                   "set_inlet")
+
+    def inlet(self, block):
+        """(back, buffer) of the block's inlet columns in host indices, or
+        None when the block has no inlet."""
+        nc, nb = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        self._chk(self.lib.fjsph_get_inlet(self.h, int(block), _ptr(nc, _I32), _ptr(nb, _I32), None, None), "get_inlet")
+        if nc[0] == 0:
+            return None
+        bk = np.zeros(int(nc[0]), np.int64)
+        bf = np.zeros((int(nc[0]), int(nb[0])), np.int64)
+        self._chk(self.lib.fjsph_get_inlet(self.h, int(block), _ptr(nc, _I32), _ptr(nb, _I32), _ptr(bk, _I64),
+                                           _ptr(bf, _I64)), "get_inlet")
+        return bk, bf
+
+    def blocks(self):
+        nb = np.zeros(1, np.int32)
+        self._chk(self.lib.fjsph_get_blocks(self.h, _ptr(nb, _I32), None, 0), "get_blocks")
+        arr = (abi.Block * int(nb[0]))()
+        self._chk(self.lib.fjsph_get_blocks(self.h, _ptr(nb, _I32), arr, int(nb[0])), "get_blocks")
+        return list(arr)
+
+    def block_planes(self, block):
+        pl = abi.BlockPlanes()
+        self._chk(self.lib.fjsph_get_block_planes(self.h, int(block), C.byref(pl)), "get_block_planes")
+        return pl
+
+    def clock(self):
+        c = abi.Clock()
+        self._chk(self.lib.fjsph_get_clock(self.h, C.byref(c)), "get_clock")
+        return c
+
+    def set_clock(self, clock):
+        self._chk(self.lib.fjsph_set_clock(self.h, C.byref(clock)), "set_clock")
+
+    def settings(self):
+        st = abi.Settings()
+        self._chk(self.lib.fjsph_get_settings(self.h, C.byref(st)), "get_settings")
+        return st
 
     def set_max_points(self, max_points):
         self._chk(self.lib.fjsph_set_max_points(self.h, int(max_points)), "set_max_points")

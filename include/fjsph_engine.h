@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FJSPH_ABI_VERSION 4
+#define FJSPH_ABI_VERSION 5
 
 typedef enum fjsph_status {
     FJSPH_OK = 0,
@@ -187,6 +187,7 @@ enum {
     FJSPH_F_CURVE, FJSPH_F_NORM_CURVE, FJSPH_F_WOCCL, FJSPH_F_PDIST,
     FJSPH_F_CELLV, FJSPH_F_CELLP, FJSPH_F_CELLRHO, FJSPH_F_CELLID,
     FJSPH_F_VPERT, FJSPH_F_SURFZONE, FJSPH_F_B,
+    FJSPH_F_PID, /* SPHPart::part_id: host index at upload, then svar.part_id++ per inserted particle */
     FJSPH_F_COUNT
 };
 enum { FJSPH_STATE_N = 0, FJSPH_STATE_NP1 = 1 }; /* pn / pnp1 */
@@ -249,6 +250,7 @@ fjsph_status fjsph_set_inlet(fjsph_ctx* ctx, int32_t block, int32_t n_cols, int3
 /* SPH maximum particle count (svar.max_points, src/Var.h:369): no new inlet
  * particles past it. */
 fjsph_status fjsph_set_max_points(fjsph_ctx* ctx, int64_t max_points);
+fjsph_status fjsph_get_max_points(fjsph_ctx* ctx, int64_t* max_points);
 /* Current particle counts (inlets add particles, deletion removes them). */
 fjsph_status fjsph_get_counts(fjsph_ctx* ctx, int64_t* n_total, int64_t* n_bound);
 /* Current host range of a block (index.first, index.second). */
@@ -298,12 +300,42 @@ fjsph_status fjsph_integrate_no_update(fjsph_ctx* ctx, fjsph_step_info* info);
 /* Integrator::integrate (src/Integration.cpp:233-303): one full step,
  * including update_data's pn <- pnp1 and the adaptive CFL. */
 fjsph_status fjsph_integrate_step(fjsph_ctx* ctx, fjsph_step_info* info);
-/* Advance the frame clock: last_frame_time += frame_time_interval
- * (src/FJSPH.cpp:322-323). */
+/* Advance the frame clock: last_frame_time += frame_time_interval,
+ * current_frame + 1 (src/FJSPH.cpp:322-323 and the frame loop). */
 fjsph_status fjsph_advance_frame(fjsph_ctx* ctx);
 /* Simulation clock and timestep (INTEG_SETT current_time / delta_t). */
 fjsph_status fjsph_get_time(fjsph_ctx* ctx, double* time, double* delta_t, double* cfl);
 fjsph_status fjsph_set_time(fjsph_ctx* ctx, double time, double delta_t, double cfl);
+
+/* The integrator state a run carries from step to step and from frame to
+ * frame (INTEG_SETT current_time, last_frame_time, delta_t, cfl,
+ * current_frame, n_stable, n_unstable, src/Var.h:63-92; svar.part_id, the
+ * next particle id, src/Var.h:365): what Write_HDF5 / Read_HDF5 carry
+ * across a restart (src/H5IO.cpp:915-1318). */
+typedef struct fjsph_clock {
+    double current_time, last_frame_time, delta_t, cfl;
+    int64_t current_frame, n_stable, n_unstable, next_part_id;
+} fjsph_clock;
+fjsph_status fjsph_get_clock(fjsph_ctx* ctx, fjsph_clock* out);
+fjsph_status fjsph_set_clock(fjsph_ctx* ctx, const fjsph_clock* in);
+/* The settings the context was created with. */
+fjsph_status fjsph_get_settings(fjsph_ctx* ctx, fjsph_settings* out);
+/* Current blocks (LIMITS: ranges follow insertions and deletions) and
+ * their planes. */
+fjsph_status fjsph_get_blocks(fjsph_ctx* ctx, int32_t* n_blocks, fjsph_block* blocks, int32_t capacity);
+fjsph_status fjsph_get_block_planes(fjsph_ctx* ctx, int32_t block, fjsph_block_planes* out);
+/* Time-velocity table of a block (bound_block::times, ::vels; 3 values
+ * per time, as fjsph_set_block_motion takes them); times/vels may be null
+ * to query n_times. */
+fjsph_status fjsph_get_block_motion(fjsph_ctx* ctx, int32_t block, int32_t* n_times, double* times, double* vels);
+/* Current inlet columns of block `block` in host indices, the layout of
+ * fjsph_set_inlet (bound_block::back, ::buffer).  A column that advanced
+ * without a new particle (max_points reached) lists its deepest buffer
+ * particle in every place past its last one, as the reference's list does
+ * (src/shapes/inlet.cpp:604-615).  n_cols = 0 when the block has no
+ * inlet; back and buffer may be null to query the sizes. */
+fjsph_status fjsph_get_inlet(fjsph_ctx* ctx, int32_t block, int32_t* n_cols, int32_t* n_buf, int64_t* back,
+                             int64_t* buffer);
 
 /* ---- aerodynamic mesh (aero_source = FJSPH_MESH_INFL) --------------------- */
 

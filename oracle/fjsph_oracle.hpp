@@ -881,6 +881,8 @@ struct Sim {
     /* INTEG_SETT state */
     double current_time = 0.0, last_frame_time = 0.0, delta_t = 2e-10, cfl = 1.0;
     unsigned n_stable = 0, n_unstable = 0;
+    unsigned current_frame = 0; /* INTEG_SETT::current_frame (src/Var.h:71) */
+    size_t part_id_next = 0;    /* svar.part_id (src/Var.h:365), the next particle id */
     /* Integrator members (src/Integration.h:52-67) */
     double safe_dt = 0.0;
     double maxf = kEps, maxAf = kEps, maxRho_pc = kEps, maxRhoi = kEps, maxdrho = kEps, minST = 9999999.0, maxU = kEps, maxShift = kEps;
@@ -1760,6 +1762,7 @@ struct Sim {
                     q.cellRho = pj.cellRho;
                     q.cellID = kNoCell;
                     q.b = FJSPH_BUFFER;
+                    q.part_id = int64_t(part_id_next++);
                     const size_t at = size_t(block.b.last);
                     pnp1.insert(pnp1.begin() + long(at), q);
                     block.buffer[ii].back() = at;

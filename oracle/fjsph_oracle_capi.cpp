@@ -88,6 +88,7 @@ int download(Handle* h, int which, int field, double* out)
         case FJSPH_F_CELLID: out[i] = double(q.cellID); break;
         case FJSPH_F_SURFZONE: out[i] = q.surfzone; break;
         case FJSPH_F_B: out[i] = q.b; break;
+        case FJSPH_F_PID: out[i] = double(q.part_id); break;
         default: {
             double* p = vec_ptr<D>(q, field);
             if (!p) return FJSPH_ERR_ARG;
@@ -111,6 +112,7 @@ int upload(Handle* h, int which, int field, const double* in)
         case FJSPH_F_CELLID: q.cellID = long(in[i]); break;
         case FJSPH_F_SURFZONE: q.surfzone = uint32_t(in[i]); break;
         case FJSPH_F_B: q.b = uint32_t(in[i]); break;
+        case FJSPH_F_PID: q.part_id = int64_t(in[i]); break;
         default: {
             double* p = vec_ptr<D>(q, field);
             if (!p) return FJSPH_ERR_ARG;
@@ -152,6 +154,8 @@ void upload_state(Handle* h, int64_t n, int64_t nb, const double* x, const doubl
     S.cfl = S.s.cfl;
     S.delta_t = S.s.delta_t_min > 0 ? S.s.delta_t_min : 2e-10;
     S.n_stable = S.n_unstable = 0;
+    S.current_frame = 0;
+    S.part_id_next = size_t(n); /* src/Init.cpp:478 */
     S.outlist.clear();
 }
 
@@ -293,6 +297,22 @@ int orc_set_inlet(void* hp, int block, int n_cols, int n_buf, const int64_t* bac
     b.buffer.assign(size_t(n_cols), std::vector<size_t>());
     for (int c = 0; c < n_cols; ++c) b.buffer[c].assign(buffer + size_t(c) * n_buf, buffer + size_t(c + 1) * n_buf);
     b.pl.inlet = 1;
+    return FJSPH_OK;
+}
+
+int orc_get_inlet(void* hp, int block, int* n_cols, int* n_buf, int64_t* back, int64_t* buffer)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    std::vector<Block>& bl = h->dim == 2 ? h->s2->blocks : h->s3->blocks;
+    if (block < 0 || block >= int(bl.size())) return FJSPH_ERR_ARG;
+    const Block& b = bl[block];
+    *n_cols = int(b.back.size());
+    *n_buf = b.buffer.empty() ? 0 : int(b.buffer[0].size());
+    if (!back || !buffer) return FJSPH_OK;
+    for (size_t c = 0; c < b.back.size(); ++c) {
+        back[c] = int64_t(b.back[c]);
+        for (size_t j = 0; j < b.buffer[c].size(); ++j) buffer[c * size_t(*n_buf) + j] = int64_t(b.buffer[c][j]);
+    }
     return FJSPH_OK;
 }
 
@@ -499,7 +519,50 @@ int orc_integrate_step(void* hp, fjsph_step_info* info)
 int orc_advance_frame(void* hp)
 {
     Handle* h = static_cast<Handle*>(hp);
-    SIMCALL(h, S.last_frame_time += S.s.frame_time_interval, S.last_frame_time += S.s.frame_time_interval);
+    SIMCALL(h, { S.last_frame_time += S.s.frame_time_interval; S.current_frame++; },
+            { S.last_frame_time += S.s.frame_time_interval; S.current_frame++; });
+    return FJSPH_OK;
+}
+
+extern "C++" {
+template <class SimT>
+void get_clock(const SimT& S, fjsph_clock* c)
+{
+    c->current_time = S.current_time;
+    c->last_frame_time = S.last_frame_time;
+    c->delta_t = S.delta_t;
+    c->cfl = S.cfl;
+    c->current_frame = S.current_frame;
+    c->n_stable = S.n_stable;
+    c->n_unstable = S.n_unstable;
+    c->next_part_id = int64_t(S.part_id_next);
+}
+
+template <class SimT>
+void set_clock(SimT& S, const fjsph_clock* c)
+{
+    S.current_time = c->current_time;
+    S.last_frame_time = c->last_frame_time;
+    S.delta_t = c->delta_t;
+    S.cfl = c->cfl;
+    S.current_frame = unsigned(c->current_frame);
+    S.n_stable = unsigned(c->n_stable);
+    S.n_unstable = unsigned(c->n_unstable);
+    S.part_id_next = size_t(c->next_part_id);
+}
+}  // extern "C++"
+
+int orc_get_clock(void* hp, fjsph_clock* c)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    SIMCALL(h, get_clock(S, c), get_clock(S, c));
+    return FJSPH_OK;
+}
+
+int orc_set_clock(void* hp, const fjsph_clock* c)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    SIMCALL(h, set_clock(S, c), set_clock(S, c));
     return FJSPH_OK;
 }
 

@@ -64,6 +64,9 @@ def load(kind="oracle"):
         "orc_advance_frame": (C.c_int, [vp]),
         "orc_get_time": (C.c_int, [vp, _D, _D, _D]),
         "orc_set_time": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+        "orc_get_clock": (C.c_int, [vp, C.POINTER(abi.Clock)]),
+        "orc_get_inlet": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _I64, _I64]),
+        "orc_set_clock": (C.c_int, [vp, C.POINTER(abi.Clock)]),
         "orc_qr_inverse": (C.c_int, [C.c_int, _D, _D]),
         "orc_min_eig": (C.c_double, [C.c_int, _D]),
         "orc_kernel": (C.c_double, [C.c_double, C.c_double, C.c_double]),
@@ -138,6 +141,18 @@ class Oracle:
         bf = np.ascontiguousarray(buffer, np.int64).reshape(bk.size, -1)
         self._chk(self.lib.orc_set_inlet(self.h, int(block), bk.size, bf.shape[1], _ptr(bk, _I64), _ptr(bf, _I64)),
                   "set_inlet")
+
+    def inlet(self, block):
+        """(back, buffer) of the block's inlet columns, or None."""
+        nc, nb = C.c_int(), C.c_int()
+        self._chk(self.lib.orc_get_inlet(self.h, int(block), C.byref(nc), C.byref(nb), None, None), "get_inlet")
+        if nc.value == 0:
+            return None
+        bk = np.zeros(nc.value, np.int64)
+        bf = np.zeros((nc.value, nb.value), np.int64)
+        self._chk(self.lib.orc_get_inlet(self.h, int(block), C.byref(nc), C.byref(nb), _ptr(bk, _I64), _ptr(bf, _I64)),
+                  "get_inlet")
+        return bk, bf
 
     def set_max_points(self, max_points):
         self._chk(self.lib.orc_set_max_points(self.h, int(max_points)), "set_max_points")
@@ -288,3 +303,11 @@ class Oracle:
 
     def set_time(self, t, dt, cfl):
         self.lib.orc_set_time(self.h, float(t), float(dt), float(cfl))
+
+    def clock(self):
+        c = abi.Clock()
+        self.lib.orc_get_clock(self.h, C.byref(c))
+        return c
+
+    def set_clock(self, clock):
+        self.lib.orc_set_clock(self.h, C.byref(clock))

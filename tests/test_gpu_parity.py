@@ -357,6 +357,9 @@ def test_inlet_parity_fp64(name, case):
         assert eng.n == orc.n, f"step {step}: {eng.n} vs {orc.n} particles"
         assert tuple(eng.block_range(0)) == tuple(orc.block_range(0))
         assert np.array_equal(eng.field("B"), orc.field("B")), f"step {step}: particle types"
+        assert np.array_equal(eng.field("PID"), orc.field("PID")), f"step {step}: particle ids"
+        (eb, ef), (ob, of) = eng.inlet(0), orc.inlet(0)
+        assert np.array_equal(eb, ob) and np.array_equal(ef, of), f"step {step}: inlet columns"
         ns.append(eng.n)
         for f in ("X", "V", "RHO", "P"):
             assert_close(f, eng.field(f), orc.field(f), FP64_STEP)
