@@ -20,7 +20,8 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HDF5_ROOT = os.environ.get("FJSPH_HDF5_ROOT", "/opt/conda")
 SYSTEM_LIBDIR = "/usr/lib/x86_64-linux-gnu"
 SOURCES = ["kernels.hip", "pairs.hip", "mesh.hip", "surrogate.hip", "surrogate_train.hip", "inlet.hip", "engine.cpp",
-           "settings.cpp"]
+           "settings.cpp", "tau.cpp"]
+HOST_ONLY = {"settings.cpp", "tau.cpp"}  # plain C++, no device code
 ARCH = os.environ.get("FJSPH_OFFLOAD_ARCH", "gfx950")
 
 
@@ -46,9 +47,10 @@ def build(verbose=False, lib=LIB, build_dir=None, defines=()):
     cmds, objs = [], []
     for src in SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
-        lang = ["-x", "hip"] if src.endswith((".hip", ".cpp")) and src != "settings.cpp" else []
+        lang = ["-x", "hip"] if src.endswith((".hip", ".cpp")) and src not in HOST_ONLY else []
         extra = ["-ffp-contract=off"] if src == "settings.cpp" else []
-        cmds.append([hipcc()] + common + extra + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        inc = [f"-I{INCLUDE}"]
+        cmds.append([hipcc()] + common + extra + lang + inc + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
 
     def run(cmd):

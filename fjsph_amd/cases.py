@@ -160,6 +160,32 @@ def droplet_3d(diameter=4e-3, dx=5e-5, seed=1234, precision=abi.PREC_MIXED, spee
                    f"3D droplet D={diameter} dx={dx}: {xf.shape[0]} particles, seed {seed}")
 
 
+def droplet_2d(diameter=4e-3, dx=1e-4, centre=(0.0, 0.0), seed=1234, precision=abi.PREC_FP64, speed=20.0,
+               rho_g=1.2, H_fac=1.3):
+    """2D droplet: a disk of the given diameter on a square lattice of
+    spacing dx (jittered by U(-0.01dx, +0.01dx) per component), FREE
+    particles, Gissler aero in a constant stream along +x (or a mesh, with
+    aero_source set to MESH_INFL).  rho0 = 1000, sigma = 0.0728."""
+    s = abi.default_settings(2)
+    s.particle_step = dx
+    s.H_fac = H_fac
+    s.rho_rest = 1000.0
+    s.sig = 0.0728
+    s.precision = precision
+    s.aero_case = abi.GISSLER
+    _gas(s, speed, rho_g)
+    rng = np.random.default_rng(seed)
+    r = diameter / 2.0
+    n = max(1, int(math.ceil(2.0 * r / dx)))
+    g = np.arange(n, dtype=np.float64) * dx
+    j, i = np.meshgrid(g, g, indexing="ij")
+    pts = np.stack([i.ravel(), j.ravel()], axis=1) + np.asarray(centre, np.float64) - r
+    pts += rng.uniform(-0.01 * dx, 0.01 * dx, size=pts.shape)
+    xf = pts[((pts - np.asarray(centre)) ** 2).sum(axis=1) <= r * r]
+    return _finish("droplet_2d", s, np.zeros((0, 2)), xf, 0.0,
+                   f"2D droplet D={diameter} dx={dx}: {xf.shape[0]} particles, seed {seed}")
+
+
 def jet_column_3d(n_target=4_000_000, diameter=2e-3, dx=4e-5, seed=7, precision=abi.PREC_MIXED,
                   jet_speed=10.0, coflow=30.0, rho_g=1.2, ale=1, H_fac=2.0):
     """Large 3D coflow jet column: the 4M-particle single-GPU workload.

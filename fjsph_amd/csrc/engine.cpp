@@ -1780,15 +1780,34 @@ fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts,
             }
         }
 
-        /* voxel candidate grid, about 4 cells per voxel */
-        double glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
-        for (int a = 0; a < D; ++a) {
-            glo[a] = verts[a];
-            ghi[a] = verts[a];
-            for (int64_t v = 0; v < nverts; ++v) {
-                glo[a] = std::min(glo[a], verts[v * D + a]);
-                ghi[a] = std::max(ghi[a], verts[v * D + a]);
+        /* voxel candidate grid, about 4 cells per voxel, over the cells of
+         * ordinary size: a cell more than 64 times the median extent (a
+         * degenerate cell, say one with a vertex at a file's fill value) goes
+         * to an overflow list that every lookup also scans, so that it
+         * neither stretches the grid nor fills millions of voxels */
+        std::vector<double> ext(size_t(ncells), 0.0);
+        for (int64_t c = 0; c < ncells; ++c)
+            for (int a = 0; a < D; ++a) ext[size_t(c)] = std::max(ext[size_t(c)], cbox[c * 6 + 3 + a] - cbox[c * 6 + a]);
+        std::vector<double> sorted_ext(ext);
+        std::nth_element(sorted_ext.begin(), sorted_ext.begin() + ncells / 2, sorted_ext.end());
+        const double ext_cap = 64.0 * std::max(sorted_ext[size_t(ncells / 2)], 1e-300);
+        std::vector<char> big(size_t(ncells), 0);
+        std::vector<int32_t> big_cells;
+        for (int64_t c = 0; c < ncells; ++c)
+            if (!(ext[size_t(c)] <= ext_cap)) {
+                big[size_t(c)] = 1;
+                big_cells.push_back(int32_t(c));
             }
+        if (int64_t(big_cells.size()) == ncells) throw std::string("set_mesh: no cell of ordinary size");
+        double glo[3] = {0, 0, 0}, ghi[3] = {0, 0, 0};
+        bool first_cell = true;
+        for (int64_t c = 0; c < ncells; ++c) {
+            if (big[size_t(c)]) continue;
+            for (int a = 0; a < D; ++a) {
+                glo[a] = first_cell ? cbox[c * 6 + a] : std::min(glo[a], cbox[c * 6 + a]);
+                ghi[a] = first_cell ? cbox[c * 6 + 3 + a] : std::max(ghi[a], cbox[c * 6 + 3 + a]);
+            }
+            first_cell = false;
         }
         double vol = 1.0;
         for (int a = 0; a < D; ++a) vol *= std::max(ghi[a] - glo[a], 1e-300);
@@ -1821,6 +1840,7 @@ fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts,
                 gcells.assign(size_t(gcount[nvox]), 0);
             }
             for (int64_t c = 0; c < ncells; ++c) {
+                if (big[size_t(c)]) continue;
                 int32_t r[3][2];
                 for (int a = 0; a < 3; ++a) vrange(c, a, r[a][0], r[a][1]);
                 for (int32_t z = r[2][0]; z <= r[2][1]; ++z)
@@ -1867,6 +1887,8 @@ fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts,
             m.cRho = (const double*)up(ctx->mrho.data(), ctx->mrho.size() * 8);
             m.gstart = (const int32_t*)up(gs.data(), gs.size() * 4);
             m.gcells = (const int32_t*)up(gcells.data(), gcells.size() * 4);
+            m.big = (const int32_t*)up(big_cells.data(), big_cells.size() * 4);
+            m.nbig = int32_t(big_cells.size());
             for (int a = 0; a < 3; ++a) { m.glo[a] = glo[a]; m.gdims[a] = gd[a]; }
             m.ginv = ginv;
         }

@@ -117,6 +117,8 @@ struct DevMesh {
     double ginv;
     int32_t gdims[3];
     const int32_t *gstart, *gcells;
+    const int32_t* big; /* cells too large for the grid, scanned by every lookup */
+    int32_t nbig;
 };
 
 /* MLP surrogate of the mesh field (surrogate.hip): BF16 weights already in

@@ -133,6 +133,8 @@ __global__ __launch_bounds__(TPB) void k_find_cell(Consts c, DevState s, const i
                 }
             }
         }
+        for (int32_t k = 0; found < 0 && k < M.nbig; ++k)
+            if (check_cell<D>(M, M.big[k], testp)) found = M.big[k];
     }
     if (found < 0) {
         atomicAdd(lost, 1);
@@ -175,6 +177,8 @@ __global__ __launch_bounds__(TPB) void k_mesh_eval(DevMesh M, int64_t n, const d
             }
         }
     }
+    for (int32_t k = 0; found < 0 && k < M.nbig; ++k)
+        if (check_cell<D>(M, M.big[k], testp)) found = M.big[k];
     cell_out[i] = found;
 #pragma unroll
     for (int a = 0; a < D; ++a) vals[i * (D + 2) + a] = found >= 0 ? M.cVel[(int64_t)found * D + a] : 0.0;
