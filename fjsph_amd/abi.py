@@ -6,6 +6,8 @@ fjsph_settings_default().
 """
 import ctypes as C
 
+import numpy as np
+
 ABI_VERSION = 5
 
 # fjsph_status
@@ -131,6 +133,34 @@ class Clock(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class IptSettings(C.Structure):
+    """fjsph_ipt_settings (IPT_SETT, src/Var.h:313-331)."""
+    _fields_ = [("using_ipt", C.c_int32), ("eq_order", C.c_int32), ("max_x", C.c_double), ("relax", C.c_double),
+                ("n_relax", C.c_int32), ("max_steps", C.c_int32)]
+
+
+def default_ipt(**kw):
+    s = IptSettings(using_ipt=0, eq_order=2, max_x=9999999.0, relax=0.6, n_relax=5, max_steps=1000000)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+IPT_SUCCESS, IPT_FAILED, IPT_STOPPED = 1, 2, 3
+
+
+class IptRecord(C.Structure):
+    """fjsph_ipt_record: the end of one IPT track."""
+    _fields_ = [("part_id", C.c_int64), ("status", C.c_int32), ("steps", C.c_int32), ("cell", C.c_int32),
+                ("face", C.c_int32), ("t0", C.c_double), ("t", C.c_double), ("x0", C.c_double * 3),
+                ("x", C.c_double * 3), ("v", C.c_double * 3)]
+
+
+IPT_DTYPE = np.dtype([("part_id", np.int64), ("status", np.int32), ("steps", np.int32), ("cell", np.int32),
+                      ("face", np.int32), ("t0", np.float64), ("t", np.float64), ("x0", np.float64, 3),
+                      ("x", np.float64, 3), ("v", np.float64, 3)])
 
 
 class StepInfo(C.Structure):

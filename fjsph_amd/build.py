@@ -19,8 +19,9 @@ RESTART_LIB = os.path.join(HERE, "libfjsph_restart.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HDF5_ROOT = os.environ.get("FJSPH_HDF5_ROOT", "/opt/conda")
 SYSTEM_LIBDIR = "/usr/lib/x86_64-linux-gnu"
-SOURCES = ["kernels.hip", "pairs.hip", "mesh.hip", "surrogate.hip", "surrogate_train.hip", "inlet.hip", "engine.cpp",
-           "settings.cpp", "tau.cpp"]
+SOURCES = ["kernels.hip", "pairs.hip", "mesh.hip", "surrogate.hip", "surrogate_train.hip", "inlet.hip", "ipt.hip",
+           "engine.cpp", "settings.cpp", "tau.cpp"]
+NO_CONTRACT = {"settings.cpp", "ipt.hip"}  # operation order of the CPU restatement, no fused multiply-adds
 HOST_ONLY = {"settings.cpp", "tau.cpp"}  # plain C++, no device code
 ARCH = os.environ.get("FJSPH_OFFLOAD_ARCH", "gfx950")
 
@@ -48,7 +49,7 @@ def build(verbose=False, lib=LIB, build_dir=None, defines=()):
     for src in SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith((".hip", ".cpp")) and src not in HOST_ONLY else []
-        extra = ["-ffp-contract=off"] if src == "settings.cpp" else []
+        extra = ["-ffp-contract=off"] if src in NO_CONTRACT else []
         inc = [f"-I{INCLUDE}"]
         cmds.append([hipcc()] + common + extra + lang + inc + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

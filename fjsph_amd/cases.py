@@ -566,6 +566,49 @@ def kuhn_mesh(lo, hi, n_cells, speed=20.0, rho_g=1.2, p_inf=101325.0, cyl_radius
                 np.ascontiguousarray(cell_faces.reshape(-1)), np.ascontiguousarray(vel), p, rho, simplices=simplices)
 
 
+def quad_mesh(nx, ny, h, vel, p=1.0e5, rho=1.2, lo=(0.0, 0.0)):
+    """Structured 2D mesh of nx x ny square cells of side h from `lo`, with a
+    uniform gas (velocity `vel`, pressure p, density rho) at the vertices:
+    vertical edges first (row by row), then horizontal ones; boundary edges
+    have the far-field marker -2 as right cell."""
+    xs = lo[0] + np.arange(nx + 1) * h
+    ys = lo[1] + np.arange(ny + 1) * h
+    X, Y = np.meshgrid(xs, ys, indexing="xy")
+    verts = np.stack([X.ravel(), Y.ravel()], axis=1)
+
+    def vid(i, j):
+        return j * (nx + 1) + i
+
+    def cid(i, j):
+        return j * nx + i
+
+    faces, lr = [], []
+    for j in range(ny):
+        for i in range(nx + 1):
+            faces.append((vid(i, j), vid(i, j + 1)))
+            cells = [c for c in ((cid(i - 1, j) if i > 0 else -2), (cid(i, j) if i < nx else -2))]
+            lr.append(tuple(sorted(cells, key=lambda c: c < 0)))
+    for j in range(ny + 1):
+        for i in range(nx):
+            faces.append((vid(i, j), vid(i + 1, j)))
+            cells = [c for c in ((cid(i, j - 1) if j > 0 else -2), (cid(i, j) if j < ny else -2))]
+            lr.append(tuple(sorted(cells, key=lambda c: c < 0)))
+    faces = np.array(faces, np.int64)
+    lr = np.array(lr, np.int64)
+    cf = [[] for _ in range(nx * ny)]
+    for f, (l, r) in enumerate(lr):
+        cf[l].append(f)
+        if r >= 0:
+            cf[r].append(f)
+    co = np.zeros(nx * ny + 1, np.int64)
+    co[1:] = np.cumsum([len(c) for c in cf])
+    nv = verts.shape[0]
+    return Mesh(verts=verts, face_off=np.arange(0, 2 * len(faces) + 1, 2, dtype=np.int64), face_verts=faces.ravel(),
+                face_lr=lr, cell_off=co, cell_faces=np.concatenate(cf).astype(np.int64),
+                vert_vel=np.tile(np.asarray(vel, np.float64), (nv, 1)), vert_p=np.full(nv, float(p)),
+                vert_rho=np.full(nv, float(rho)))
+
+
 def with_mesh(case, n_points=20000, margin=4.0, seed=7, speed=None, kind="delaunay", n_cells=None):
     """Switch a case to the mesh aero source with a mesh around its
     particles (`margin` particle spacings beyond their bounding box): a

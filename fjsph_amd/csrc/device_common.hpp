@@ -325,6 +325,34 @@ __device__ __forceinline__ T fast_cos(T x) { return cos(x); }
 template <>
 __device__ __forceinline__ float fast_cos<float>(float x) { return __cosf(x); }
 
+/* Determinant of the 4x4 matrix with rows (p_k, 1): Laplace expansion in
+ * 2x2 minors of columns 0-1 (the signed volume of the mesh crossing tests;
+ * contraction off, so its sign agrees with the CPU restatement). */
+__device__ __forceinline__ double det4_rows(const double* p0, const double* p1, const double* p2, const double* p3)
+{
+#pragma clang fp contract(off)
+    const double* r[4] = {p0, p1, p2, p3};
+    double m[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        m[i][0] = r[i][0];
+        m[i][1] = r[i][1];
+        m[i][2] = r[i][2];
+        m[i][3] = 1.0;
+    }
+    auto d2 = [&](int i0, int i1) { return m[i0][0] * m[i1][1] - m[i1][0] * m[i0][1]; };
+    const double d2_01 = d2(0, 1), d2_02 = d2(0, 2), d2_03 = d2(0, 3);
+    const double d2_12 = d2(1, 2), d2_13 = d2(1, 3), d2_23 = d2(2, 3);
+    auto d3 = [&](int i0, double a0, int i1, double a1, int i2, double a2) {
+        return m[i0][2] * a0 + (-m[i1][2] * a1 + m[i2][2] * a2);
+    };
+    const double d3_0 = d3(1, d2_23, 2, d2_13, 3, d2_12);
+    const double d3_1 = d3(0, d2_23, 2, d2_03, 3, d2_02);
+    const double d3_2 = d3(0, d2_13, 1, d2_03, 3, d2_01);
+    const double d3_3 = d3(0, d2_12, 1, d2_02, 2, d2_01);
+    return (-m[0][3] * d3_0 + m[1][3] * d3_1) + (-m[2][3] * d3_2 + m[3][3] * d3_3);
+}
+
 }  // namespace fj
 
 #endif

@@ -121,6 +121,15 @@ struct fjsph_ctx {
     bool have_mesh = false;
     int64_t mesh_ncells = 0;
     std::vector<double> mcentre, mvel, mp, mrho;
+    double mesh_max_len = 0.0; /* MESH::maxlength */
+    /* implicit particle tracking */
+    fjsph_ipt_settings ipt{};
+    std::vector<fjsph_ipt_record> ipt_records;
+    int64_t ipt_success = 0, ipt_failed = 0;
+    IptStart* ipt_in = nullptr;
+    fjsph_ipt_record* ipt_out = nullptr;
+    int32_t* ipt_slots = nullptr;
+    int64_t ipt_cap = 0;
     DevMesh dm{};
     std::vector<void*> mesh_bufs;
     /* surrogate */
@@ -269,6 +278,8 @@ void free_all(fjsph_ctx* ctx)
     for (void* p : ctx->mesh_bufs) (void)hipFree(p);
     ctx->mesh_bufs.clear();
     ctx->have_mesh = false;
+    dfree(ctx->ipt_in); dfree(ctx->ipt_out); dfree(ctx->ipt_slots);
+    ctx->ipt_cap = 0;
     for (void* p : ctx->sg_bufs) (void)hipFree(p);
     ctx->sg_bufs.clear();
     ctx->have_surrogate = false;
@@ -1072,6 +1083,101 @@ void inlet_update(fjsph_ctx* ctx)
 /* Deletion plane of update_data (src/Integration.cpp:126-205): fluid
  * particles past their block's active plane are removed, the rest keep
  * their order (slots and host ids are compacted). */
+/* MESH::maxlength as the TAU readers set it (Place_Edges, Place_Faces,
+ * src/CDFIO.cpp:880-929, 1130-1220): the longest edge of the 2D edges, or
+ * of the triangles and the longer diagonal of the other faces in 3D, times
+ * 4 (2D) or 5 (3D).  Computed for every mesh here; the reference sets it
+ * only when it reads a TAU mesh. */
+double mesh_max_length(int D, int64_t nfaces, const int64_t* face_off, const int64_t* face_verts, const double* verts)
+{
+    auto dist = [&](int64_t a, int64_t b) {
+        double s = 0.0;
+        for (int k = 0; k < D; ++k) {
+            const double d = verts[a * D + k] - verts[b * D + k];
+            s += d * d;
+        }
+        return std::sqrt(s);
+    };
+    double mx = 0.0;
+    for (int64_t f = 0; f < nfaces; ++f) {
+        const int64_t* fv = face_verts + face_off[f];
+        const int64_t nv = face_off[f + 1] - face_off[f];
+        double e = 0.0;
+        if (D == 2) e = dist(fv[0], fv[1]);
+        else if (nv == 3) e = std::max(dist(fv[0], fv[1]), std::max(dist(fv[0], fv[2]), dist(fv[1], fv[2])));
+        else if (nv >= 4) e = std::max(dist(fv[0], fv[2]), dist(fv[1], fv[3]));
+        mx = std::max(mx, e);
+    }
+    return mx * (D == 2 ? 4.0 : 5.0);
+}
+
+IptParams ipt_params(const fjsph_ctx* ctx)
+{
+    const fjsph_settings& s = ctx->s;
+    IptParams p{};
+    for (int a = 0; a < 3; ++a) p.grav[a] = s.grav[a];
+    p.mu_g = s.mu_g;
+    p.rho_l = s.rho_rest;
+    p.relax = ctx->ipt.relax;
+    p.max_x = ctx->ipt.max_x;
+    p.max_len = ctx->mesh_max_len;
+    /* Set_Values (src/IO.cpp:126-127) */
+    p.diam = std::pow((6.0 * ctx->d.sim_mass) / (3.14159265358979323846 * s.rho_rest), 1.0 / 3.0);
+    p.area = 3.14159265358979323846 * p.diam * p.diam / 4.0;
+    p.eq_order = ctx->ipt.eq_order;
+    p.n_relax = ctx->ipt.n_relax;
+    p.max_subits = s.max_subits;
+    p.max_steps = ctx->ipt.max_steps;
+    return p;
+}
+
+void ipt_reserve(fjsph_ctx* ctx, int64_t n)
+{
+    if (n <= ctx->ipt_cap) return;
+    const int64_t c = std::max<int64_t>(n, 2 * ctx->ipt_cap);
+    dfree(ctx->ipt_in);
+    dfree(ctx->ipt_out);
+    dfree(ctx->ipt_slots);
+    dalloc(ctx->ipt_in, size_t(c));
+    dalloc(ctx->ipt_out, size_t(c));
+    dalloc(ctx->ipt_slots, size_t(c));
+    ctx->ipt_cap = c;
+}
+
+/* update_data's hand-off (src/Integration.cpp:150-168): the particles the
+ * deletion planes remove, in ascending host index (the reference's
+ * std::set), become IPT particles at the current time and are tracked. */
+void ipt_handoff(fjsph_ctx* ctx, int64_t ndel)
+{
+    const int64_t n = ctx->n;
+    std::vector<int32_t> keep(static_cast<size_t>(n)), host(static_cast<size_t>(n));
+    HIPCHK(hipMemcpyAsync(keep.data(), ctx->keep, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipMemcpyAsync(host.data(), ctx->part_id, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    std::vector<std::pair<int32_t, int32_t>> del;
+    del.reserve(size_t(ndel));
+    for (int64_t sl = 0; sl < n; ++sl)
+        if (!keep[size_t(sl)]) del.emplace_back(host[size_t(sl)], int32_t(sl));
+    std::sort(del.begin(), del.end());
+    std::vector<int32_t> slots(del.size());
+    for (size_t k = 0; k < del.size(); ++k) slots[k] = del[k].second;
+    const int64_t m = int64_t(slots.size());
+    if (m == 0) return;
+    ipt_reserve(ctx, m);
+    HIPCHK(hipMemcpyAsync(ctx->ipt_slots, slots.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, ctx->st));
+    launch_ipt_gather(ctx->s.dim, ctx->pnp1, ctx->ipt_slots, m, ctx->current_time, ctx->ipt_in, ctx->st);
+    launch_ipt_track(ctx->s.dim, ctx->dm, ipt_params(ctx), m, ctx->ipt_in, ctx->ipt_out, ctx->st);
+    const size_t at = ctx->ipt_records.size();
+    ctx->ipt_records.resize(at + size_t(m));
+    HIPCHK(hipMemcpyAsync(ctx->ipt_records.data() + at, ctx->ipt_out, sizeof(fjsph_ipt_record) * m,
+                          hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    for (size_t k = at; k < ctx->ipt_records.size(); ++k) {
+        if (ctx->ipt_records[k].status == FJSPH_IPT_SUCCESS) ctx->ipt_success++;
+        else if (ctx->ipt_records[k].status == FJSPH_IPT_FAILED) ctx->ipt_failed++;
+    }
+}
+
 void delete_update(fjsph_ctx* ctx)
 {
     const int D = ctx->s.dim;
@@ -1088,6 +1194,7 @@ void delete_update(fjsph_ctx* ctx)
     int64_t ndel = 0;
     for (size_t k = 0; k < nbk; ++k) ndel += cnt[k];
     if (ndel == 0) return;
+    if (ctx->ipt.using_ipt && ctx->s.aero_source == FJSPH_MESH_INFL && ctx->have_mesh) ipt_handoff(ctx, ndel);
     launch_scan(ctx->keep, n, ctx->kstart, ctx->del_tiles, ctx->st);
     launch_scan(ctx->hflag, n, ctx->hstart, ctx->del_tiles, ctx->st);
     launch_compact_perm(ctx->keep, ctx->kstart, n, ctx->perm, ctx->st);
@@ -1210,6 +1317,7 @@ fjsph_status fjsph_create(const fjsph_settings* s, int32_t device, fjsph_ctx** o
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FJSPH_ERR_NO_DEVICE;
     if (device < 0 || device >= ndev) return FJSPH_ERR_ARG;
     fjsph_ctx* ctx = new fjsph_ctx();
+    fjsph_ipt_settings_default(&ctx->ipt);
     ctx->s = *s;
     ctx->d = d;
     ctx->device = device;
@@ -1538,6 +1646,82 @@ fjsph_status fjsph_get_inlet(fjsph_ctx* ctx, int32_t block, int32_t* n_cols, int
             for (int32_t j = 0; j < nb; ++j) buffer[size_t(c) * nb + j] = bf[size_t(c) * nb + j];
         }
     });
+}
+
+void fjsph_ipt_settings_default(fjsph_ipt_settings* s)
+{
+    if (!s) return;
+    s->using_ipt = 0;
+    s->eq_order = 2;        /* IPT_SETT::ipt_eq_order (src/Var.h:316) */
+    s->max_x = 9999999.0;   /* IPT_SETT::max_x (src/Var.h:318) */
+    s->relax = 0.6;         /* IPT_SETT::relax (src/Var.h:323) */
+    s->n_relax = 5;         /* IPT_SETT::n_relax (src/Var.h:324) */
+    s->max_steps = 1000000;
+}
+
+fjsph_status fjsph_set_ipt(fjsph_ctx* ctx, const fjsph_ipt_settings* s)
+{
+    if (!ctx || !s || (s->eq_order != 1 && s->eq_order != 2) || s->max_steps <= 0 || s->n_relax < 0)
+        return FJSPH_ERR_ARG;
+    ctx->ipt = *s;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_ipt_track(fjsph_ctx* ctx, int64_t n, const double* x, const double* v, const int32_t* cell,
+                             const int64_t* part_id, double t0, fjsph_ipt_record* out)
+{
+    if (!ctx || n < 0 || (n > 0 && (!x || !v || !cell || !out))) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->have_mesh) throw std::string("ipt_track: no mesh set");
+        if (n == 0) return;
+        const int D = ctx->s.dim;
+        std::vector<IptStart> st(static_cast<size_t>(n));
+        for (int64_t i = 0; i < n; ++i) {
+            IptStart& q = st[size_t(i)];
+            std::memset(&q, 0, sizeof(q));
+            for (int a = 0; a < D; ++a) {
+                q.x[a] = x[i * D + a];
+                q.v[a] = v[i * D + a];
+            }
+            q.cell = cell[i];
+            if (q.cell >= 0 && q.cell < ctx->mesh_ncells) {
+                for (int a = 0; a < D; ++a) q.cellV[a] = ctx->mvel[size_t(q.cell) * D + a];
+                q.cellRho = ctx->mrho[size_t(q.cell)];
+            }
+            q.t = t0;
+            q.part_id = part_id ? part_id[i] : i;
+        }
+        ipt_reserve(ctx, n);
+        HIPCHK(hipMemcpyAsync(ctx->ipt_in, st.data(), sizeof(IptStart) * n, hipMemcpyHostToDevice, ctx->st));
+        launch_ipt_track(D, ctx->dm, ipt_params(ctx), n, ctx->ipt_in, ctx->ipt_out, ctx->st);
+        HIPCHK(hipMemcpyAsync(out, ctx->ipt_out, sizeof(fjsph_ipt_record) * n, hipMemcpyDeviceToHost, ctx->st));
+        sync(ctx);
+    });
+}
+
+fjsph_status fjsph_ipt_count(fjsph_ctx* ctx, int64_t* n_records, int64_t* n_success, int64_t* n_failed)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    if (n_records) *n_records = int64_t(ctx->ipt_records.size());
+    if (n_success) *n_success = ctx->ipt_success;
+    if (n_failed) *n_failed = ctx->ipt_failed;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_ipt_records(fjsph_ctx* ctx, int64_t first, int64_t count, fjsph_ipt_record* out)
+{
+    if (!ctx || first < 0 || count < 0 || first + count > int64_t(ctx->ipt_records.size()) || (count > 0 && !out))
+        return FJSPH_ERR_ARG;
+    std::copy(ctx->ipt_records.begin() + first, ctx->ipt_records.begin() + first + count, out);
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_ipt_clear(fjsph_ctx* ctx)
+{
+    if (!ctx) return FJSPH_ERR_ARG;
+    ctx->ipt_records.clear();
+    ctx->ipt_success = ctx->ipt_failed = 0;
+    return FJSPH_OK;
 }
 
 fjsph_status fjsph_get_clock(fjsph_ctx* ctx, fjsph_clock* out)
@@ -1889,10 +2073,18 @@ fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts,
             m.gcells = (const int32_t*)up(gcells.data(), gcells.size() * 4);
             m.big = (const int32_t*)up(big_cells.data(), big_cells.size() * 4);
             m.nbig = int32_t(big_cells.size());
+            std::vector<int32_t> lr32(size_t(nfaces) * 2);
+            for (int64_t k = 0; k < nfaces * 2; ++k) lr32[size_t(k)] = int32_t(face_lr[k]);
+            m.face_lr = (const int32_t*)up(lr32.data(), lr32.size() * 4);
+            std::vector<double> c3(size_t(ncells) * 3, 0.0);
+            for (int64_t c = 0; c < ncells; ++c)
+                for (int a = 0; a < D; ++a) c3[size_t(c) * 3 + a] = ctx->mcentre[size_t(c) * D + a];
+            m.cCentre = (const double*)up(c3.data(), c3.size() * 8);
             for (int a = 0; a < 3; ++a) { m.glo[a] = glo[a]; m.gdims[a] = gd[a]; }
             m.ginv = ginv;
         }
         ctx->mesh_ncells = ncells;
+        ctx->mesh_max_len = mesh_max_length(D, nfaces, face_off, face_verts, verts);
         ctx->have_mesh = true;
     });
 }

@@ -119,6 +119,24 @@ struct DevMesh {
     const int32_t *gstart, *gcells;
     const int32_t* big; /* cells too large for the grid, scanned by every lookup */
     int32_t nbig;
+    const int32_t* face_lr;  /* left, right cell per face (right < 0 on the boundary) */
+    const double* cCentre;   /* 3 per cell (z = 0 in 2D) */
+};
+
+/* IPT constants (IPT_SETT, src/Var.h:313-331; Set_Values ipt_diam,
+ * src/IO.cpp:126-127; MESH::maxlength, src/CDFIO.cpp:867-929, 1117-1220). */
+struct IptParams {
+    double grav[3];
+    double mu_g, rho_l, relax, max_x, max_len, diam, area;
+    int32_t eq_order, n_relax, max_subits, max_steps;
+};
+
+/* Start of an IPT track (IPTPart from an SPH particle, src/Var.h:734-762). */
+struct IptStart {
+    double x[3], v[3], cellV[3];
+    double cellRho, t;
+    int64_t part_id;
+    int32_t cell, pad;
 };
 
 /* MLP surrogate of the mesh field (surrogate.hip): BF16 weights already in
@@ -256,6 +274,9 @@ int64_t mlp_tiles_bwd(int n_hidden);
 void launch_mlp_pack(int dim, const float* w, int32_t n_hidden, void* wf, void* wt, hipStream_t st);
 /* The mesh field at points: containing cell (-1 if none) and (vel, p, rho). */
 void launch_mesh_eval(int dim, DevMesh m, int64_t n, const double* pts, int32_t* cell, double* vals, hipStream_t st);
+void launch_ipt_track(int dim, const DevMesh& m, const IptParams& p, int64_t n, const IptStart* in,
+                      fjsph_ipt_record* out, hipStream_t st);
+void launch_ipt_gather(int dim, DevState s, const int32_t* slots, int64_t n, double t, IptStart* out, hipStream_t st);
 /* FindCell on the aero particles of s (mesh.hip). */
 void launch_find_cell(int dim, const Consts& c, DevState s, const int32_t* part_id, DevMesh m, int64_t n,
                       int32_t* lost, hipStream_t st);

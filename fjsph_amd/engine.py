@@ -82,6 +82,12 @@ SIGNATURES = {
     "fjsph_set_clock": (C.c_int, [_VP, C.POINTER(abi.Clock)]),
     "fjsph_get_settings": (C.c_int, [_VP, C.POINTER(abi.Settings)]),
     "fjsph_get_max_points": (C.c_int, [_VP, _I64]),
+    "fjsph_ipt_settings_default": (None, [C.POINTER(abi.IptSettings)]),
+    "fjsph_set_ipt": (C.c_int, [_VP, C.POINTER(abi.IptSettings)]),
+    "fjsph_ipt_track": (C.c_int, [_VP, C.c_int64, _D, _D, _I32, _I64, C.c_double, C.c_void_p]),
+    "fjsph_ipt_count": (C.c_int, [_VP, _I64, _I64, _I64]),
+    "fjsph_ipt_records": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_void_p]),
+    "fjsph_ipt_clear": (C.c_int, [_VP]),
 }
 
 
@@ -214,6 +220,38 @@ class Engine:
         st = abi.Settings()
         self._chk(self.lib.fjsph_get_settings(self.h, C.byref(st)), "get_settings")
         return st
+
+    def set_ipt(self, settings):
+        """settings: abi.IptSettings (abi.default_ipt)."""
+        self._chk(self.lib.fjsph_set_ipt(self.h, C.byref(settings)), "set_ipt")
+
+    def ipt_track(self, x, v, cell, part_id=None, t0=0.0):
+        """Track particles through the mesh; a structured array of records
+        (abi.IPT_DTYPE)."""
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        c = np.ascontiguousarray(cell, np.int32)
+        n = c.size
+        pid = None if part_id is None else np.ascontiguousarray(part_id, np.int64)
+        out = np.zeros(n, abi.IPT_DTYPE)
+        self._chk(self.lib.fjsph_ipt_track(self.h, n, _ptr(x), _ptr(v), _ptr(c, _I32),
+                                           None if pid is None else _ptr(pid, _I64), float(t0),
+                                           out.ctypes.data_as(C.c_void_p)), "ipt_track")
+        return out
+
+    def ipt_count(self):
+        a, b, c = np.zeros(1, np.int64), np.zeros(1, np.int64), np.zeros(1, np.int64)
+        self._chk(self.lib.fjsph_ipt_count(self.h, _ptr(a, _I64), _ptr(b, _I64), _ptr(c, _I64)), "ipt_count")
+        return int(a[0]), int(b[0]), int(c[0])
+
+    def ipt_records(self):
+        n = self.ipt_count()[0]
+        out = np.zeros(n, abi.IPT_DTYPE)
+        self._chk(self.lib.fjsph_ipt_records(self.h, 0, n, out.ctypes.data_as(C.c_void_p)), "ipt_records")
+        return out
+
+    def ipt_clear(self):
+        self._chk(self.lib.fjsph_ipt_clear(self.h), "ipt_clear")
 
     def set_max_points(self, max_points):
         self._chk(self.lib.fjsph_set_max_points(self.h, int(max_points)), "set_max_points")

@@ -337,6 +337,50 @@ fjsph_status fjsph_get_block_motion(fjsph_ctx* ctx, int32_t block, int32_t* n_ti
 fjsph_status fjsph_get_inlet(fjsph_ctx* ctx, int32_t block, int32_t* n_cols, int32_t* n_buf, int64_t* back,
                              int64_t* buffer);
 
+/* ---- implicit particle tracking (IPT) ------------------------------------ */
+
+/* IPT_SETT (src/Var.h:313-331): with using_ipt on and the mesh aero source,
+ * fluid particles removed by a deletion plane are tracked through the gas
+ * mesh as point droplets (IPT::Integrate, src/IPT.cpp:870-1106) in
+ * update_data (src/Integration.cpp:150-168), on the GPU, one thread per
+ * particle; the tracks' ends are kept as records. */
+typedef struct fjsph_ipt_settings {
+    int32_t using_ipt;  /* "Transition to IPT" (0)                         */
+    int32_t eq_order;   /* "Velocity equation order": 1 or 2 (2)           */
+    double max_x;       /* "Maximum x trajectory coordinate" (9999999)     */
+    double relax;       /* IPT_SETT::relax (0.6)                           */
+    int32_t n_relax;    /* IPT_SETT::n_relax (5)                           */
+    int32_t max_steps;  /* cell-to-cell steps per track before it stops; no
+                           counterpart in the reference (1000000)          */
+} fjsph_ipt_settings;
+
+enum { FJSPH_IPT_SUCCESS = 1, FJSPH_IPT_FAILED = 2, FJSPH_IPT_STOPPED = 3 };
+
+/* The end of a track (the last entry Terminate_Particle keeps,
+ * src/IPT.cpp:735-818): status success (left the mesh, or past max_x),
+ * failed (no face crossed, |v| > 1e3, or a step longer than the mesh's
+ * maximum length) or stopped (step cap); the cell (negative: the boundary
+ * marker of the face it left through) and the last face crossed. */
+typedef struct fjsph_ipt_record {
+    int64_t part_id;
+    int32_t status, steps, cell, face;
+    double t0, t;
+    double x0[3], x[3], v[3];
+} fjsph_ipt_record;
+
+void fjsph_ipt_settings_default(fjsph_ipt_settings* s);
+fjsph_status fjsph_set_ipt(fjsph_ctx* ctx, const fjsph_ipt_settings* s);
+/* Track n particles from the given positions, velocities and cells of the
+ * current mesh (the cells' gas values start the tracks), at time t0; the
+ * records go to out (n of them).  Diameter and area are Set_Values' ipt_diam
+ * and ipt_area (src/IO.cpp:126-127). */
+fjsph_status fjsph_ipt_track(fjsph_ctx* ctx, int64_t n, const double* x, const double* v, const int32_t* cell,
+                             const int64_t* part_id, double t0, fjsph_ipt_record* out);
+/* Records of the particles tracked by update_data since the last clear. */
+fjsph_status fjsph_ipt_count(fjsph_ctx* ctx, int64_t* n_records, int64_t* n_success, int64_t* n_failed);
+fjsph_status fjsph_ipt_records(fjsph_ctx* ctx, int64_t first, int64_t count, fjsph_ipt_record* out);
+fjsph_status fjsph_ipt_clear(fjsph_ctx* ctx);
+
 /* ---- aerodynamic mesh (aero_source = FJSPH_MESH_INFL) --------------------- */
 
 /* The MESH of src/Var.h:396-451 as flat arrays (host order, 0-based):

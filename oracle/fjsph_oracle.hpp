@@ -866,10 +866,63 @@ inline std::vector<int64_t> knn_cells(const MeshData& M, const double* p, size_t
     return out;
 }
 
+#include "ipt_oracle.hpp"
+
+/* MESH::maxlength as the TAU readers set it (Place_Edges, Place_Faces,
+ * src/CDFIO.cpp:880-929, 1130-1220): the longest 2D edge, or the longest
+ * triangle edge / longer diagonal of the other faces in 3D, times 4 (2D) or
+ * 5 (3D); computed here for every mesh. */
+inline double mesh_max_length(const MeshData& M)
+{
+    const int D = M.dim;
+    auto dist = [&](int64_t a, int64_t b) {
+        double s = 0.0;
+        for (int k = 0; k < D; ++k) {
+            const double d = M.vert(a)[k] - M.vert(b)[k];
+            s += d * d;
+        }
+        return std::sqrt(s);
+    };
+    double mx = 0.0;
+    for (size_t f = 0; f + 1 < M.face_off.size(); ++f) {
+        const int64_t* fv = &M.face_v[size_t(M.face_off[f])];
+        const int64_t nv = M.face_off[f + 1] - M.face_off[f];
+        double e = 0.0;
+        if (D == 2) e = dist(fv[0], fv[1]);
+        else if (nv == 3) e = std::max(dist(fv[0], fv[1]), std::max(dist(fv[0], fv[2]), dist(fv[1], fv[2])));
+        else if (nv >= 4) e = std::max(dist(fv[0], fv[2]), dist(fv[1], fv[3]));
+        mx = std::max(mx, e);
+    }
+    return mx * (D == 2 ? 4.0 : 5.0);
+}
+
 template <int D>
 struct Sim {
     fjsph_settings s;
     fjsph_derived d;
+    /* implicit particle tracking (IPT_SETT) and the ends of the tracks */
+    fjsph_ipt_settings ipt{0, 2, 9999999.0, 0.6, 5, 1000000};
+    std::vector<IptRecord> ipt_records;
+    int64_t ipt_success = 0, ipt_failed = 0;
+
+    IptParams ipt_params() const
+    {
+        IptParams p{};
+        for (int a = 0; a < 3; ++a) p.grav[a] = s.grav[a];
+        p.mu_g = s.mu_g;
+        p.rho_l = s.rho_rest;
+        p.relax = ipt.relax;
+        p.max_x = ipt.max_x;
+        p.max_len = mesh_max_length(mesh);
+        /* Set_Values (src/IO.cpp:126-127) */
+        p.diam = std::pow((6.0 * d.sim_mass) / (3.14159265358979323846 * s.rho_rest), 1.0 / 3.0);
+        p.area = 3.14159265358979323846 * p.diam * p.diam / 4.0;
+        p.eq_order = ipt.eq_order;
+        p.n_relax = ipt.n_relax;
+        p.max_subits = s.max_subits;
+        p.max_steps = ipt.max_steps;
+        return p;
+    }
     std::vector<Part<D>> pn, pnp1;
     OUTL outlist;
     std::vector<Block> blocks;
@@ -1800,6 +1853,27 @@ struct Sim {
         }
         if (!to_del.empty()) {
             std::sort(to_del.begin(), to_del.end());
+            /* IPT hand-off (src/Integration.cpp:150-168): IPTPart(pnp1[ii],
+             * current_time, ipt_diam, ipt_area), tracked to its end */
+            if (ipt.using_ipt && s.aero_source == FJSPH_MESH_INFL && have_mesh) {
+                const IptParams P = ipt_params();
+                for (size_t ii : to_del) {
+                    IptP<D> q;
+                    q.pid = pnp1[ii].part_id;
+                    q.t = current_time;
+                    q.cellID = pnp1[ii].cellID;
+                    q.cellV = pnp1[ii].cellV;
+                    q.cellRho = pnp1[ii].cellRho;
+                    q.v = pnp1[ii].v;
+                    q.xi = pnp1[ii].xi;
+                    q.d = P.diam;
+                    q.A = P.area;
+                    const IptRecord r = ipt_integrate<D>(mesh, P, q);
+                    ipt_records.push_back(r);
+                    if (r.status == IPT_SUCCESS) ipt_success++;
+                    else if (r.status == IPT_FAILED) ipt_failed++;
+                }
+            }
             for (auto it = to_del.rbegin(); it != to_del.rend(); ++it) {
                 pnp1.erase(pnp1.begin() + long(*it));
                 total_points--;

@@ -205,6 +205,53 @@ int set_mesh_t(Handle* h, int64_t nverts, const double* verts, int64_t nfaces, c
     return FJSPH_OK;
 }
 
+void to_public(const orc::IptRecord& a, fjsph_ipt_record& b)
+{
+    b.part_id = a.pid;
+    b.status = a.status;
+    b.steps = a.steps;
+    b.cell = a.cell;
+    b.face = a.face;
+    b.t0 = a.t0;
+    b.t = a.t;
+    for (int k = 0; k < 3; ++k) {
+        b.x0[k] = a.x0[k];
+        b.x[k] = a.x[k];
+        b.v[k] = a.v[k];
+    }
+}
+
+template <int D>
+int ipt_track_t(Handle* h, int64_t n, const double* x, const double* v, const int32_t* cell, const int64_t* pid,
+                double t0, fjsph_ipt_record* out)
+{
+    Sim<D>& S = sim<D>(h);
+    if (!S.have_mesh) return FJSPH_ERR_STATE;
+    const orc::IptParams P = S.ipt_params();
+    const orc::MeshData& M = S.mesh;
+    std::vector<orc::IptRecord> rec(static_cast<size_t>(n));
+#pragma omp parallel for num_threads(S.nthreads) schedule(dynamic, 4)
+    for (int64_t i = 0; i < n; ++i) {
+        orc::IptP<D> q;
+        q.pid = pid ? pid[i] : i;
+        q.t = t0;
+        q.cellID = cell[i];
+        for (int a = 0; a < D; ++a) {
+            q.xi.c[a] = x[i * D + a];
+            q.v.c[a] = v[i * D + a];
+        }
+        if (q.cellID >= 0 && q.cellID < int64_t(M.ncells())) {
+            for (int a = 0; a < D; ++a) q.cellV.c[a] = M.cVel[size_t(q.cellID) * D + a];
+            q.cellRho = M.cRho[size_t(q.cellID)];
+        }
+        q.d = P.diam;
+        q.A = P.area;
+        rec[size_t(i)] = orc::ipt_integrate<D>(M, P, q);
+    }
+    for (int64_t i = 0; i < n; ++i) to_public(rec[size_t(i)], out[i]);
+    return FJSPH_OK;
+}
+
 }  // namespace
 
 #define DISPATCH(h, call) ((h)->dim == 2 ? call<2> : call<3>)
@@ -551,6 +598,45 @@ void set_clock(SimT& S, const fjsph_clock* c)
     S.part_id_next = size_t(c->next_part_id);
 }
 }  // extern "C++"
+
+int orc_set_ipt(void* hp, const fjsph_ipt_settings* s)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    if (!s) return FJSPH_ERR_ARG;
+    if (h->dim == 2) h->s2->ipt = *s; else h->s3->ipt = *s;
+    return FJSPH_OK;
+}
+
+int orc_ipt_count(void* hp, int64_t* n, int64_t* ok, int64_t* failed)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    SIMCALL(h, { *n = int64_t(S.ipt_records.size()); *ok = S.ipt_success; *failed = S.ipt_failed; },
+            { *n = int64_t(S.ipt_records.size()); *ok = S.ipt_success; *failed = S.ipt_failed; });
+    return FJSPH_OK;
+}
+
+int orc_ipt_records(void* hp, fjsph_ipt_record* out)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    const std::vector<orc::IptRecord>& r = h->dim == 2 ? h->s2->ipt_records : h->s3->ipt_records;
+    for (size_t i = 0; i < r.size(); ++i) to_public(r[i], out[i]);
+    return FJSPH_OK;
+}
+
+int orc_ipt_track(void* hp, int64_t n, const double* x, const double* v, const int32_t* cell, const int64_t* pid,
+                  double t0, fjsph_ipt_record* out)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    return h->dim == 2 ? ipt_track_t<2>(h, n, x, v, cell, pid, t0, out) : ipt_track_t<3>(h, n, x, v, cell, pid, t0, out);
+}
+
+/* The reference's ray-triangle test (MollerTrumbore, src/Geometry.cpp:668-716)
+ * on one triangle, for unit checks: returns 1 when it reports a crossing. */
+int orc_moller(const double* v0, const double* v1, const double* v2, const double* origin, const double* dir,
+               double* dt, double* denom)
+{
+    return orc::moller(orc::v3(v0), orc::v3(v1), orc::v3(v2), orc::v3(origin), orc::v3(dir), *dt, *denom) ? 1 : 0;
+}
 
 int orc_get_clock(void* hp, fjsph_clock* c)
 {

@@ -65,6 +65,11 @@ def load(kind="oracle"):
         "orc_get_time": (C.c_int, [vp, _D, _D, _D]),
         "orc_set_time": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "orc_get_clock": (C.c_int, [vp, C.POINTER(abi.Clock)]),
+        "orc_set_ipt": (C.c_int, [vp, C.POINTER(abi.IptSettings)]),
+        "orc_moller": (C.c_int, [_D, _D, _D, _D, _D, _D, _D]),
+        "orc_ipt_count": (C.c_int, [vp, _I64, _I64, _I64]),
+        "orc_ipt_records": (C.c_int, [vp, C.c_void_p]),
+        "orc_ipt_track": (C.c_int, [vp, C.c_int64, _D, _D, C.POINTER(C.c_int32), _I64, C.c_double, C.c_void_p]),
         "orc_get_inlet": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _I64, _I64]),
         "orc_set_clock": (C.c_int, [vp, C.POINTER(abi.Clock)]),
         "orc_qr_inverse": (C.c_int, [C.c_int, _D, _D]),
@@ -153,6 +158,39 @@ class Oracle:
         self._chk(self.lib.orc_get_inlet(self.h, int(block), C.byref(nc), C.byref(nb), _ptr(bk, _I64), _ptr(bf, _I64)),
                   "get_inlet")
         return bk, bf
+
+    @staticmethod
+    def moller(v0, v1, v2, origin, direction, kind="oracle"):
+        """(hit, dt, denom) of the reference's ray-triangle test."""
+        lib = load(kind)
+        a = [np.ascontiguousarray(q, np.float64) for q in (v0, v1, v2, origin, direction)]
+        dt, dn = np.zeros(1), np.zeros(1)
+        hit = lib.orc_moller(*[_ptr(q) for q in a], _ptr(dt), _ptr(dn))
+        return bool(hit), float(dt[0]), float(dn[0])
+
+    def set_ipt(self, settings):
+        self._chk(self.lib.orc_set_ipt(self.h, C.byref(settings)), "set_ipt")
+
+    def ipt_track(self, x, v, cell, part_id=None, t0=0.0):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        c = np.ascontiguousarray(cell, np.int32)
+        pid = None if part_id is None else np.ascontiguousarray(part_id, np.int64)
+        out = np.zeros(c.size, abi.IPT_DTYPE)
+        self._chk(self.lib.orc_ipt_track(self.h, c.size, _ptr(x), _ptr(v), c.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         None if pid is None else _ptr(pid, _I64), float(t0),
+                                         out.ctypes.data_as(C.c_void_p)), "ipt_track")
+        return out
+
+    def ipt_count(self):
+        a, b, c = np.zeros(1, np.int64), np.zeros(1, np.int64), np.zeros(1, np.int64)
+        self._chk(self.lib.orc_ipt_count(self.h, _ptr(a, _I64), _ptr(b, _I64), _ptr(c, _I64)), "ipt_count")
+        return int(a[0]), int(b[0]), int(c[0])
+
+    def ipt_records(self):
+        out = np.zeros(self.ipt_count()[0], abi.IPT_DTYPE)
+        self._chk(self.lib.orc_ipt_records(self.h, out.ctypes.data_as(C.c_void_p)), "ipt_records")
+        return out
 
     def set_max_points(self, max_points):
         self._chk(self.lib.orc_set_max_points(self.h, int(max_points)), "set_max_points")
