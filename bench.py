@@ -48,6 +48,9 @@ METRIC = "particle-steps/s and %HBM roofline at 4M 3D particles; surrogate vs me
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_t_jet4m_traffic.json")
 
 
+AUTOTUNE_CANDIDATES = (128, 192, 256, 384, 512, 768, 1024)
+
+
 def workload(name, n_target):
     if name == "jet4m":
         c = cases.jet_column_3d(n_target=n_target, precision=abi.PREC_MIXED)
@@ -58,6 +61,11 @@ def workload(name, n_target):
         return cases.droplet_3d(precision=abi.PREC_MIXED), "3D droplet D=4 mm, dx=0.05 mm, 20 m/s crossflow"
     if name == "dam2d":
         return cases.dam_break_2d(), "2D dam break 1x2 m column, dx=0.02 m, FP64"
+    if name == "batched":
+        c = cases.batched_droplets(precision=abi.PREC_MIXED)
+        desc = ("256 independent 3D droplets, diameters log-uniform in [0.5, 2] mm, dx=0.05 mm, crossflow "
+                "5-40 m/s, Gissler drag, one launch per pass over all droplets")
+        return c, desc
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -82,6 +90,8 @@ def cpu_baseline(args):
     nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if args.workload == "jet4m":
         c = cases.jet_column_3d(n_target=args.cpu_particles, precision=abi.PREC_FP64)
+    elif args.workload == "batched":
+        c = cases.batched_droplets(n_drops=8, precision=abi.PREC_FP64)  # drops of the same size law
     else:
         c, _ = workload(args.workload, args.particles)
     o = Oracle(c, nthreads=nthreads, kind="baseline")
@@ -144,13 +154,20 @@ def run_engine(case, args, label):
     eng.integrate_no_update()
     for _ in range(args.warmup):
         eng.integrate_step()
+    tune = None
+    if not args.no_autotune:
+        # tile blocks chosen by the engine's autotuner on this state (untimed)
+        best, ms = eng.autotune_blocks(AUTOTUNE_CANDIDATES, reps=2)
+        tune = {"block_targets": best, "ms_per_candidate": {str(k): v for k, v in ms.items()}}
+        print(f"[bench] {label}: autotuned tile blocks: {best} targets per block", file=sys.stderr, flush=True)
+        eng.integrate_step()
     eng.enable_timers(True)
     its = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         its.append(eng.integrate_step()["iterations"])
     elapsed = time.perf_counter() - t0
-    r = {"elapsed": elapsed, "value": case.n * args.steps / elapsed, "its": float(np.mean(its)),
+    r = {"elapsed": elapsed, "value": case.n * args.steps / elapsed, "its": float(np.mean(its)), "autotune": tune,
          "timers": eng.timers(), "stats": eng.neighbour_stats(),
          "force_bpp": eng.pass_bytes(abi.TIMER_NAMES.index("force")),
          "pass_bytes": {k: eng.pass_bytes(abi.TIMER_NAMES.index(k)) for k in abi.TIMER_NAMES}}
@@ -188,6 +205,7 @@ def main():
     ap.add_argument("--cpu-particles", type=int, default=60_000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile blocks (256 targets)")
     ap.add_argument("--copy-probe", type=int, default=0)
     args = ap.parse_args()
 
@@ -201,6 +219,8 @@ def main():
 
     case, desc = workload(args.workload, args.particles)
     aero_info = None
+    if args.workload == "batched":
+        args.aero = "const"  # each drop's crossflow is its own initial velocity in still gas
     if args.aero == "const":
         desc += ", gas velocity: the constant coflow velocity"
         run = run_engine(case, args, "constant-velocity aero")
@@ -322,6 +342,7 @@ def main():
             "max_neighbours": max_count + 1,
             "subiterations_per_step": run["its"],
             "aero": aero_info,
+            "tile_blocks": run.get("autotune"),
             "passes": passes,
         },
         "roofline": roofline,

@@ -122,6 +122,7 @@ struct fjsph_ctx {
     int64_t mesh_ncells = 0;
     std::vector<double> mcentre, mvel, mp, mrho;
     double mesh_max_len = 0.0; /* MESH::maxlength */
+    int32_t block_targets = 0; /* targets per tile block (0: 256) */
     /* implicit particle tracking */
     fjsph_ipt_settings ipt{};
     std::vector<fjsph_ipt_record> ipt_records;
@@ -558,7 +559,8 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
     {
         TileGrid& tg = ctx->tg;
         const double occ = std::pow(cs / ctx->s.particle_step, D);
-        int32_t C = int32_t(std::lround(256.0 / std::max(occ, 1.0)));
+        const double targets = ctx->block_targets > 0 ? double(ctx->block_targets) : 256.0;
+        int32_t C = int32_t(std::lround(targets / std::max(occ, 1.0)));
         C = std::max(1, std::min(C, ctx->dims[0]));
         for (int k = 0; k < 3; ++k) tg.dims[k] = ctx->dims[k];
         tg.C = C;
@@ -1292,6 +1294,10 @@ std::vector<int32_t>& slot_to_host(fjsph_ctx* ctx)
     HIPCHK(hipMemcpyAsync(ctx->slot_host.data(), ctx->part_id, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost,
                           ctx->st));
     sync(ctx);
+    /* the host arrays are indexed by it: an inconsistent map must not write
+     * out of their bounds */
+    for (int32_t h : ctx->slot_host)
+        if (h < 0 || h >= ctx->n) throw std::string("internal: slot to host map out of range");
     return ctx->slot_host;
 }
 
@@ -1722,6 +1728,125 @@ fjsph_status fjsph_ipt_clear(fjsph_ctx* ctx)
     ctx->ipt_records.clear();
     ctx->ipt_success = ctx->ipt_failed = 0;
     return FJSPH_OK;
+}
+
+fjsph_status fjsph_set_block_targets(fjsph_ctx* ctx, int32_t targets)
+{
+    if (!ctx || targets < 0 || targets > 65536) return FJSPH_ERR_ARG;
+    ctx->block_targets = targets;
+    ctx->have_nbrs = false;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_get_block_targets(fjsph_ctx* ctx, int32_t* targets)
+{
+    if (!ctx || !targets) return FJSPH_ERR_ARG;
+    *targets = ctx->block_targets > 0 ? ctx->block_targets : 256;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_autotune_blocks(fjsph_ctx* ctx, int32_t n_cand, const int32_t* targets, int32_t reps, double* ms,
+                                   int32_t* best)
+{
+    if (!ctx || n_cand <= 0 || !targets || reps <= 0 || !ms || !best) return FJSPH_ERR_ARG;
+    for (int32_t c = 0; c < n_cand; ++c)
+        if (targets[c] <= 0 || targets[c] > 65536) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        require(ctx, false);
+        const int64_t n = ctx->n;
+        /* every per-slot array, so that the state comes back bit for bit.
+         * The list builds permute the arrays by swapping buffers, so the
+         * pointers are collected again, in the same order, to restore. */
+        auto collect = [&] {
+            std::vector<std::pair<void*, size_t>> v;
+            for (DevState* st : {&ctx->pn, &ctx->pnp1})
+                for (int f = 0; f < kNumFields; ++f)
+                    v.emplace_back(*field_slot(*st, f),
+                                   size_t(n) * size_t(width_of(ctx, f)) * (field_desc(f).is_int ? 4 : 8));
+            for (int32_t* a : {ctx->part_id, ctx->blk, ctx->cell, ctx->col, ctx->depth})
+                v.emplace_back(a, size_t(n) * 4);
+            return v;
+        };
+        std::vector<std::pair<void*, size_t>> arrays = collect();
+        size_t total = 0;
+        for (auto& a : arrays) total += (a.second + 255) / 256 * 256;
+        char* snap = nullptr;
+        HIPCHK(hipMalloc((void**)&snap, total));
+        struct Free {
+            char* p;
+            ~Free() { if (p) (void)hipFree(p); }
+        } guard{snap};
+        size_t off = 0;
+        for (auto& a : arrays) {
+            HIPCHK(hipMemcpyAsync(snap + off, a.first, a.second, hipMemcpyDeviceToDevice, ctx->st));
+            off += (a.second + 255) / 256 * 256;
+        }
+        const bool timing = ctx->timing;
+        drain_timers(ctx);
+        ctx->timing = false;
+        const int32_t saved = ctx->block_targets;
+        /* one step's pair work: two list builds with their pre-passes, and
+         * the force pass once per Newmark iteration of the last step */
+        const int iters = ctx->iteration > 0 ? int(ctx->iteration) : 10;
+        auto step_work = [&] {
+            build_neighbours(ctx, true);
+            prepasses(ctx);
+            build_neighbours(ctx, false);
+            prepasses(ctx);
+            for (int i = 0; i < iters; ++i) force(ctx, false);
+        };
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        double best_ms = 0.0;
+        int32_t best_k = -1;
+        try {
+            for (int32_t c = 0; c < n_cand; ++c) {
+                ctx->block_targets = targets[c];
+                ctx->have_nbrs = false;
+                step_work(); /* warm-up: list capacity for this tiling */
+                HIPCHK(hipEventRecord(e0, ctx->st));
+                for (int32_t r = 0; r < reps; ++r) step_work();
+                HIPCHK(hipEventRecord(e1, ctx->st));
+                HIPCHK(hipEventSynchronize(e1));
+                float el = 0.0f;
+                HIPCHK(hipEventElapsedTime(&el, e0, e1));
+                ms[c] = double(el) / reps;
+                if (best_k < 0 || ms[c] < best_ms) {
+                    best_ms = ms[c];
+                    best_k = c;
+                }
+            }
+        } catch (...) {
+            ctx->block_targets = saved;
+            ctx->timing = timing;
+            ctx->have_nbrs = false;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            arrays = collect();
+            off = 0;
+            for (auto& a : arrays) {
+                (void)hipMemcpyAsync(a.first, snap + off, a.second, hipMemcpyDeviceToDevice, ctx->st);
+                off += (a.second + 255) / 256 * 256;
+            }
+            (void)hipStreamSynchronize(ctx->st);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        arrays = collect();
+        off = 0;
+        for (auto& a : arrays) {
+            HIPCHK(hipMemcpyAsync(a.first, snap + off, a.second, hipMemcpyDeviceToDevice, ctx->st));
+            off += (a.second + 255) / 256 * 256;
+        }
+        sync(ctx);
+        ctx->timing = timing;
+        ctx->block_targets = targets[best_k];
+        ctx->have_nbrs = false;
+        ctx->x_at_build = false;
+        *best = targets[best_k];
+    });
 }
 
 fjsph_status fjsph_get_clock(fjsph_ctx* ctx, fjsph_clock* out)

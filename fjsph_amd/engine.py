@@ -88,6 +88,9 @@ SIGNATURES = {
     "fjsph_ipt_count": (C.c_int, [_VP, _I64, _I64, _I64]),
     "fjsph_ipt_records": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_void_p]),
     "fjsph_ipt_clear": (C.c_int, [_VP]),
+    "fjsph_set_block_targets": (C.c_int, [_VP, C.c_int32]),
+    "fjsph_get_block_targets": (C.c_int, [_VP, _I32]),
+    "fjsph_autotune_blocks": (C.c_int, [_VP, C.c_int32, _I32, C.c_int32, _D, _I32]),
 }
 
 
@@ -220,6 +223,23 @@ class Engine:
         st = abi.Settings()
         self._chk(self.lib.fjsph_get_settings(self.h, C.byref(st)), "get_settings")
         return st
+
+    def set_block_targets(self, targets):
+        self._chk(self.lib.fjsph_set_block_targets(self.h, int(targets)), "set_block_targets")
+
+    def block_targets(self):
+        t = np.zeros(1, np.int32)
+        self._chk(self.lib.fjsph_get_block_targets(self.h, _ptr(t, _I32)), "get_block_targets")
+        return int(t[0])
+
+    def autotune_blocks(self, candidates=(128, 192, 256, 384, 512), reps=3):
+        """fjsph_autotune_blocks: (best targets per block, {candidate: ms})."""
+        c = np.ascontiguousarray(candidates, np.int32)
+        ms = np.zeros(c.size)
+        best = np.zeros(1, np.int32)
+        self._chk(self.lib.fjsph_autotune_blocks(self.h, c.size, _ptr(c, _I32), int(reps), _ptr(ms), _ptr(best, _I32)),
+                  "autotune_blocks")
+        return int(best[0]), dict(zip(c.tolist(), ms.tolist()))
 
     def set_ipt(self, settings):
         """settings: abi.IptSettings (abi.default_ipt)."""

@@ -337,6 +337,23 @@ fjsph_status fjsph_get_block_motion(fjsph_ctx* ctx, int32_t block, int32_t* n_ti
 fjsph_status fjsph_get_inlet(fjsph_ctx* ctx, int32_t block, int32_t* n_cols, int32_t* n_buf, int64_t* back,
                              int64_t* buffer);
 
+/* ---- tile blocks and their autotuner --------------------------------------- */
+
+/* Targets per tile block of the pair passes (the cells of one block row
+ * whose particles share one candidate union in LDS); 0 restores the
+ * default, 256.  No counterpart in the reference.  Results change only in
+ * the order of summation. */
+fjsph_status fjsph_set_block_targets(fjsph_ctx* ctx, int32_t targets);
+fjsph_status fjsph_get_block_targets(fjsph_ctx* ctx, int32_t* targets);
+/* Autotuner of the tile blocks: for each candidate, one step's pair work
+ * (two list builds with their pre-passes, the force pass once per Newmark
+ * iteration of the last step) timed over `reps` repetitions on the current
+ * state; ms[c] receives the time per repetition, *best the fastest
+ * candidate, which is then set.  The particle state is restored bit for bit
+ * afterwards.  Allocates a device snapshot of the per-particle arrays. */
+fjsph_status fjsph_autotune_blocks(fjsph_ctx* ctx, int32_t n_cand, const int32_t* targets, int32_t reps, double* ms,
+                                   int32_t* best);
+
 /* ---- implicit particle tracking (IPT) ------------------------------------ */
 
 /* IPT_SETT (src/Var.h:313-331): with using_ipt on and the mesh aero source,
