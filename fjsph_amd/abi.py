@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 # fjsph_status
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
@@ -28,7 +28,7 @@ FIELDS = [
     "X", "V", "ACC", "AF", "AVISC", "RHO", "P", "RRHO", "M", "DELTAD", "L",
     "GRADRHO", "NORM", "LAM", "LAM_NB", "COLOURG", "COLOUR", "KERNSUM", "SURF",
     "CURVE", "NORM_CURVE", "WOCCL", "PDIST", "CELLV", "CELLP", "CELLRHO",
-    "CELLID", "VPERT", "SURFZONE", "B", "PID",
+    "CELLID", "VPERT", "SURFZONE", "B", "PID", "NFAILED", "INTERNAL",
 ]
 F = {name: i for i, name in enumerate(FIELDS)}
 VECTOR_FIELDS = {"X", "V", "ACC", "AF", "AVISC", "GRADRHO", "NORM", "CELLV", "VPERT"}
@@ -130,6 +130,14 @@ class Clock(C.Structure):
     _fields_ = [("current_time", C.c_double), ("last_frame_time", C.c_double), ("delta_t", C.c_double),
                 ("cfl", C.c_double), ("current_frame", C.c_int64), ("n_stable", C.c_int64),
                 ("n_unstable", C.c_int64), ("next_part_id", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LostInfo(C.Structure):
+    """fjsph_lost_info: FindCell's handling of particles in no cell."""
+    _fields_ = [("lost", C.c_int64), ("deleted", C.c_int64), ("internal", C.c_int64), ("failed", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

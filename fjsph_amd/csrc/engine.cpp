@@ -41,6 +41,7 @@ FieldDesc field_desc(int f)
     case FJSPH_F_L:
         return {2, false};
     case FJSPH_F_SURF: case FJSPH_F_CELLID: case FJSPH_F_SURFZONE: case FJSPH_F_B: case FJSPH_F_PID:
+    case FJSPH_F_NFAILED: case FJSPH_F_INTERNAL:
         return {0, true};
     default:
         return {0, false};
@@ -81,6 +82,8 @@ void** field_slot(DevState& s, int f)
     case FJSPH_F_SURFZONE: return (void**)&s.surfzone;
     case FJSPH_F_B: return (void**)&s.b;
     case FJSPH_F_PID: return (void**)&s.pid;
+    case FJSPH_F_NFAILED: return (void**)&s.nfail;
+    case FJSPH_F_INTERNAL: return (void**)&s.internal;
     default: return nullptr;
     }
 }
@@ -133,6 +136,11 @@ struct fjsph_ctx {
     int64_t ipt_cap = 0;
     DevMesh dm{};
     std::vector<void*> mesh_bufs;
+    /* particles a mesh lookup placed in no cell */
+    int32_t *lost_list = nullptr, *lost_del = nullptr;
+    int64_t lost_cap = 0;
+    int64_t lost_deleted = 0, lost_internal = 0, lost_failed = 0;
+    bool tuning = false; /* the tile-block autotuner is timing passes: no deletions */
     /* surrogate */
     int32_t* aero_list = nullptr;   /* segmented list of the aero particles of the last lookup */
     int32_t* aero_counts = nullptr; /* its per-segment lengths */
@@ -203,6 +211,7 @@ struct fjsph_ctx {
     DevPlanes dpl{};
     int32_t *keep = nullptr, *hflag = nullptr, *kstart = nullptr, *hstart = nullptr; /* deletion scratch */
     int32_t *perm = nullptr, *del_tiles = nullptr, *blkcount = nullptr;
+    size_t blkcount_cap = 0;
 };
 
 namespace {
@@ -281,6 +290,9 @@ void free_all(fjsph_ctx* ctx)
     ctx->have_mesh = false;
     dfree(ctx->ipt_in); dfree(ctx->ipt_out); dfree(ctx->ipt_slots);
     ctx->ipt_cap = 0;
+    dfree(ctx->lost_list); dfree(ctx->lost_del);
+    ctx->lost_cap = 0;
+    ctx->lost_deleted = ctx->lost_internal = ctx->lost_failed = 0;
     for (void* p : ctx->sg_bufs) (void)hipFree(p);
     ctx->sg_bufs.clear();
     ctx->have_surrogate = false;
@@ -297,6 +309,7 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
     dfree(ctx->col); dfree(ctx->depth); dfree(ctx->keep); dfree(ctx->hflag); dfree(ctx->kstart);
     dfree(ctx->hstart); dfree(ctx->perm); dfree(ctx->del_tiles); dfree(ctx->blkcount);
+    ctx->blkcount_cap = 0;
     dfree(ctx->c_adv); dfree(ctx->c_back); dfree(ctx->c_last); dfree(ctx->c_rank); dfree(ctx->c_block);
     dfree(ctx->pl_del); dfree(ctx->pl_ins); dfree(ctx->pl_aero); dfree(ctx->pl_mode); dfree(ctx->pl_nbuf);
     ctx->inlets.clear();
@@ -615,13 +628,71 @@ void refresh_aux(fjsph_ctx* ctx)
 
 int r_from_x(const fjsph_ctx* ctx) { return ctx->x_at_build ? 1 : 0; }
 
-/* get_aero_velocity for the mesh source (src/Resid.cpp:479-523): FindCell. */
-void mesh_lookup(fjsph_ctx* ctx)
+void compact_deleted(fjsph_ctx* ctx, const std::vector<int32_t>& cnt, int64_t ndel);
+
+/* Per-block deletion counters, zeroed. */
+void zero_blkcount(fjsph_ctx* ctx)
+{
+    const size_t need = std::max<size_t>(ctx->blocks.size(), 1);
+    if (ctx->blkcount_cap < need) {
+        dalloc(ctx->blkcount, need);
+        ctx->blkcount_cap = need;
+    }
+    HIPCHK(hipMemsetAsync(ctx->blkcount, 0, sizeof(int32_t) * need, ctx->st));
+}
+
+/* get_aero_velocity for the mesh source (src/Resid.cpp:479-523): FindCell.
+ * With `classify`, the particles in no cell get FindCell's boundary test
+ * and those it condemns are deleted from both states (returns true; the
+ * caller builds the neighbour lists again).  Without it (the lookup after
+ * Check_Pipe_Outlet, whose FirstCell leaves those counters alone) they are
+ * only counted. */
+bool mesh_lookup(fjsph_ctx* ctx, bool classify)
 {
     if (!ctx->have_mesh) throw std::string("aero_source is the mesh but no mesh was set (fjsph_set_mesh)");
     PassTimer t(ctx, FJSPH_T_AERO);
-    HIPCHK(hipMemsetAsync(&ctx->sc->mesh_lost, 0, sizeof(int32_t), ctx->st));
-    launch_find_cell(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->dm, ctx->n, &ctx->sc->mesh_lost, ctx->st);
+    const int64_t n = ctx->n;
+    if (classify && ctx->lost_cap < n) {
+        const int64_t cap = n + n / 8 + 64;
+        dalloc(ctx->lost_list, size_t(cap));
+        dalloc(ctx->lost_del, size_t(cap));
+        ctx->lost_cap = cap;
+    }
+    HIPCHK(hipMemsetAsync(&ctx->sc->mesh_lost, 0, 5 * sizeof(int32_t), ctx->st));
+    launch_find_cell(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->dm, n, ctx->sc,
+                     classify ? ctx->lost_list : nullptr, ctx->st);
+    if (!classify) return false;
+    HIPCHK(hipMemcpyAsync(&ctx->sc_host->mesh_lost, &ctx->sc->mesh_lost, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->st));
+    sync(ctx);
+    const int32_t nlost = ctx->sc_host->mesh_lost;
+    if (nlost == 0) return false;
+    launch_lost_classify(ctx->s.dim, ctx->pnp1, ctx->dm, ctx->lost_list, nlost, ctx->lost_del, ctx->sc, ctx->st);
+    HIPCHK(hipMemcpyAsync(&ctx->sc_host->mesh_lost, &ctx->sc->mesh_lost, 5 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->st));
+    sync(ctx);
+    const DevScalars& h = *ctx->sc_host;
+    if (h.lost_err)
+        throw std::string("mesh lookup: the nearest-centre search of a particle in no cell did not converge "
+                          "(thousands of cell centres at one distance)");
+    ctx->lost_internal += h.lost_int;
+    ctx->lost_failed += h.lost_fail;
+    if (h.lost_del == 0) return false;
+    const int64_t ndel = h.lost_del;
+    const size_t nbk = ctx->blocks.size();
+    zero_blkcount(ctx);
+    launch_lost_marks(ctx->lost_list, ctx->lost_del, nlost, ctx->part_id, ctx->blk, n, ctx->keep, ctx->hflag,
+                      ctx->blkcount, ctx->st);
+    std::vector<int32_t> cnt(std::max<size_t>(nbk, 1));
+    HIPCHK(hipMemcpyAsync(cnt.data(), ctx->blkcount, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    int64_t inblocks = 0;
+    for (size_t k = 0; k < nbk; ++k) inblocks += cnt[k];
+    if (nbk > 0 && inblocks != ndel) throw std::string("mesh lookup: a deleted particle belongs to no block");
+    compact_deleted(ctx, cnt, ndel);
+    ctx->lost_deleted += ndel;
+    ctx->have_nbrs = false;
+    return true;
 }
 
 /* get_aero_velocity for the surrogate source: the MLP in place of FindCell,
@@ -641,6 +712,17 @@ void surrogate_lookup(fjsph_ctx* ctx)
     launch_surrogate_list(ctx->s.dim, ctx->pnp1, ctx->sg, ctx->n, ctx->aero_list, ctx->aero_counts, ctx->st);
 }
 
+void dsph(fjsph_ctx* ctx, bool with_aero);
+
+/* get_aero_velocity with the mesh: FindCell, and after deletions the
+ * neighbour lists and dSPH_PreStep again (src/Resid.cpp:483-519). */
+void aero_mesh(fjsph_ctx* ctx)
+{
+    if (!mesh_lookup(ctx, !ctx->tuning)) return;
+    build_neighbours(ctx, false);
+    dsph(ctx, false);
+}
+
 void dsph(fjsph_ctx* ctx, bool with_aero)
 {
     {
@@ -649,7 +731,7 @@ void dsph(fjsph_ctx* ctx, bool with_aero)
         launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
                     with_aero ? 1 : 0, ctx->partial, ctx->sc, ctx->st);
     }
-    if (with_aero && ctx->s.aero_source == FJSPH_MESH_INFL) mesh_lookup(ctx);
+    if (with_aero && ctx->s.aero_source == FJSPH_MESH_INFL) aero_mesh(ctx);
     if (with_aero && ctx->s.aero_source == FJSPH_SURROGATE) surrogate_lookup(ctx);
 }
 
@@ -741,7 +823,7 @@ void pipe_outlet(fjsph_ctx* ctx)
     if (!ctx->planes_on) return;
     upload_planes(ctx);
     launch_pipe_outlet(ctx->s.dim, ctx->pnp1, ctx->part_id, ctx->blk, ctx->dpl, ctx->n, ctx->nb, ctx->st);
-    if (ctx->s.aero_source == FJSPH_MESH_INFL && ctx->have_mesh) mesh_lookup(ctx);
+    if (ctx->s.aero_source == FJSPH_MESH_INFL && ctx->have_mesh) mesh_lookup(ctx, false);
 }
 
 /* dSPH_PreStep, get_aero_velocity, Detect_Surface, dissipation_terms,
@@ -1186,8 +1268,7 @@ void delete_update(fjsph_ctx* ctx)
     const int64_t n = ctx->n;
     const size_t nbk = ctx->blocks.size();
     upload_planes(ctx);
-    if (!ctx->blkcount) dalloc(ctx->blkcount, std::max<size_t>(nbk, 1));
-    HIPCHK(hipMemsetAsync(ctx->blkcount, 0, sizeof(int32_t) * std::max<size_t>(nbk, 1), ctx->st));
+    zero_blkcount(ctx);
     launch_delete_flag(D, ctx->pnp1, ctx->part_id, ctx->blk, ctx->dpl, n, ctx->nb, ctx->keep, ctx->hflag,
                        ctx->blkcount, ctx->st);
     std::vector<int32_t> cnt(std::max<size_t>(nbk, 1));
@@ -1197,13 +1278,22 @@ void delete_update(fjsph_ctx* ctx)
     for (size_t k = 0; k < nbk; ++k) ndel += cnt[k];
     if (ndel == 0) return;
     if (ctx->ipt.using_ipt && ctx->s.aero_source == FJSPH_MESH_INFL && ctx->have_mesh) ipt_handoff(ctx, ndel);
+    compact_deleted(ctx, cnt, ndel);
+}
+
+/* Removes the slots with keep[slot] == 0 (hflag[host id] == 1), cnt[k] of
+ * them in block k: both states are compacted in order, host ids close up,
+ * block ranges shift. */
+void compact_deleted(fjsph_ctx* ctx, const std::vector<int32_t>& cnt, int64_t ndel)
+{
+    const int64_t n = ctx->n;
     launch_scan(ctx->keep, n, ctx->kstart, ctx->del_tiles, ctx->st);
     launch_scan(ctx->hflag, n, ctx->hstart, ctx->del_tiles, ctx->st);
     launch_compact_perm(ctx->keep, ctx->kstart, n, ctx->perm, ctx->st);
     launch_compact_ids(ctx->part_id, ctx->hstart, n, ctx->st);
     permute_all(ctx, ctx->perm, n - ndel);
     int64_t shift = 0;
-    for (size_t k = 0; k < nbk; ++k) {
+    for (size_t k = 0; k < ctx->blocks.size(); ++k) {
         ctx->blocks[k].b.first -= shift;
         shift += cnt[k];
         ctx->blocks[k].b.last -= shift;
@@ -1784,6 +1874,7 @@ fjsph_status fjsph_autotune_blocks(fjsph_ctx* ctx, int32_t n_cand, const int32_t
         const bool timing = ctx->timing;
         drain_timers(ctx);
         ctx->timing = false;
+        ctx->tuning = true;
         const int32_t saved = ctx->block_targets;
         /* one step's pair work: two list builds with their pre-passes, and
          * the force pass once per Newmark iteration of the last step */
@@ -1820,6 +1911,7 @@ fjsph_status fjsph_autotune_blocks(fjsph_ctx* ctx, int32_t n_cand, const int32_t
         } catch (...) {
             ctx->block_targets = saved;
             ctx->timing = timing;
+            ctx->tuning = false;
             ctx->have_nbrs = false;
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -1842,6 +1934,7 @@ fjsph_status fjsph_autotune_blocks(fjsph_ctx* ctx, int32_t n_cand, const int32_t
         }
         sync(ctx);
         ctx->timing = timing;
+        ctx->tuning = false;
         ctx->block_targets = targets[best_k];
         ctx->have_nbrs = false;
         ctx->x_at_build = false;
@@ -2014,7 +2107,7 @@ fjsph_status fjsph_aero_velocity(fjsph_ctx* ctx)
 {
     return guarded(ctx, [&] {
         require(ctx, true);
-        if (ctx->s.aero_source == FJSPH_MESH_INFL) mesh_lookup(ctx);
+        if (ctx->s.aero_source == FJSPH_MESH_INFL) aero_mesh(ctx);
         else if (ctx->s.aero_source == FJSPH_SURROGATE) surrogate_lookup(ctx);
         else launch_aero_velocity(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->nl, ctx->n, ctx->st);
         sync(ctx);
@@ -2043,7 +2136,6 @@ fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts,
             if (face_verts[k] < 0 || face_verts[k] >= nverts) throw std::string("set_mesh: face vertex out of range");
         for (int64_t k = 0; k < ncf; ++k)
             if (cell_faces[k] < 0 || cell_faces[k] >= nfaces) throw std::string("set_mesh: cell face out of range");
-        (void)face_lr; /* boundary markers: used by deletion, which is not built */
 
         /* Average_Point_Data_to_Cell (src/CDFIO.cpp:112-155): compensated sums
          * over the cell's unique vertices in ascending order */
@@ -2207,6 +2299,51 @@ fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts,
             m.cCentre = (const double*)up(c3.data(), c3.size() * 8);
             for (int a = 0; a < 3; ++a) { m.glo[a] = glo[a]; m.gdims[a] = gd[a]; }
             m.ginv = ginv;
+            /* cell centres binned on a grid of about 4 centres per voxel
+             * (nearest-centre queries of the particles in no cell) */
+            double clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+            for (int a = 0; a < D; ++a) {
+                clo[a] = chi[a] = c3[size_t(a)];
+                for (int64_t c = 0; c < ncells; ++c) {
+                    clo[a] = std::min(clo[a], c3[size_t(c) * 3 + a]);
+                    chi[a] = std::max(chi[a], c3[size_t(c) * 3 + a]);
+                }
+            }
+            double span = 0.0, vol = 1.0;
+            for (int a = 0; a < D; ++a) span = std::max(span, chi[a] - clo[a]);
+            if (!(span > 0.0)) span = 1.0;
+            for (int a = 0; a < D; ++a) vol *= std::max(chi[a] - clo[a], 1e-6 * span);
+            double ch = std::pow(vol * 4.0 / double(ncells), 1.0 / D);
+            int64_t cd[3] = {1, 1, 1}, ncv = 1;
+            for (int it = 0; it < 64; ++it) {
+                ncv = 1;
+                for (int a = 0; a < D; ++a) {
+                    cd[a] = int64_t(std::floor((chi[a] - clo[a]) / ch)) + 1;
+                    ncv *= cd[a];
+                }
+                if (ncv <= 2 * ncells + 64) break;
+                ch *= 1.25;
+            }
+            const double cinv = 1.0 / ch;
+            std::vector<int32_t> cst(size_t(ncv) + 1, 0), ccl(static_cast<size_t>(ncells)), cvox(static_cast<size_t>(ncells));
+            for (int64_t c = 0; c < ncells; ++c) {
+                int64_t vx = 0;
+                for (int a = D - 1; a >= 0; --a) {
+                    const int64_t g = std::min<int64_t>(cd[a] - 1, int64_t((c3[size_t(c) * 3 + a] - clo[a]) * cinv));
+                    vx = vx * cd[a] + std::max<int64_t>(g, 0);
+                }
+                cvox[size_t(c)] = int32_t(vx);
+                cst[size_t(vx) + 1]++;
+            }
+            for (int64_t v = 0; v < ncv; ++v) cst[size_t(v) + 1] += cst[size_t(v)];
+            {
+                std::vector<int32_t> fill(cst.begin(), cst.end() - 1);
+                for (int64_t c = 0; c < ncells; ++c) ccl[size_t(fill[size_t(cvox[size_t(c)])]++)] = int32_t(c);
+            }
+            m.cstart = (const int32_t*)up(cst.data(), cst.size() * 4);
+            m.ccells = (const int32_t*)up(ccl.data(), ccl.size() * 4);
+            for (int a = 0; a < 3; ++a) { m.clo[a] = clo[a]; m.cdims[a] = int32_t(cd[a]); }
+            m.cinv = cinv;
         }
         ctx->mesh_ncells = ncells;
         ctx->mesh_max_len = mesh_max_length(D, nfaces, face_off, face_verts, verts);
@@ -2515,6 +2652,20 @@ fjsph_status fjsph_mesh_lost(fjsph_ctx* ctx, int64_t* lost)
                               ctx->st));
         sync(ctx);
         *lost = ctx->sc_host->mesh_lost;
+    });
+}
+
+fjsph_status fjsph_mesh_lost_info(fjsph_ctx* ctx, fjsph_lost_info* info)
+{
+    if (!ctx || !info) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        HIPCHK(hipMemcpyAsync(&ctx->sc_host->mesh_lost, &ctx->sc->mesh_lost, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->st));
+        sync(ctx);
+        info->lost = ctx->sc_host->mesh_lost;
+        info->deleted = ctx->lost_deleted;
+        info->internal = ctx->lost_internal;
+        info->failed = ctx->lost_failed;
     });
 }
 

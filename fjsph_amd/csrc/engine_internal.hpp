@@ -36,7 +36,7 @@ struct DevState {
     double *x, *v, *acc, *Af, *aVisc, *Rrho, *rho, *p, *m, *deltaD, *L, *gradRho, *norm;
     double *lam, *lam_nb, *colourG, *colour, *kernsum, *curve, *norm_curve, *woccl, *pDist;
     double *cellV, *cellP, *cellRho, *vPert;
-    int32_t *surf, *cellID, *surfzone, *b, *pid;
+    int32_t *surf, *cellID, *surfzone, *b, *pid, *nfail, *internal;
 };
 
 /* Neighbour lists in rows of 4 entries: row r of the target at position p
@@ -121,6 +121,11 @@ struct DevMesh {
     int32_t nbig;
     const int32_t* face_lr;  /* left, right cell per face (right < 0 on the boundary) */
     const double* cCentre;   /* 3 per cell (z = 0 in 2D) */
+    /* cell centres binned on a uniform grid (nearest-centre queries) */
+    double clo[3];
+    double cinv;
+    int32_t cdims[3];
+    const int32_t *cstart, *ccells;
 };
 
 /* IPT constants (IPT_SETT, src/Var.h:313-331; Set_Values ipt_diam,
@@ -194,6 +199,10 @@ struct DevScalars {
     int32_t max_count;
     int32_t overflow; /* largest candidate union beyond 0xFFFF / TILE tiles (0 if none) */
     int32_t mesh_lost; /* aero particles in no mesh cell at the last lookup */
+    int32_t lost_del;  /* of those, deleted by the boundary test */
+    int32_t lost_int;  /* of those, found across a wall face */
+    int32_t lost_fail; /* of those, with no net boundary crossing */
+    int32_t lost_err;  /* nearest-centre search failed (too many equal distances) */
 };
 
 /* ---- launch wrappers (kernels.hip); every one enqueues on `st` ---------- */
@@ -278,8 +287,20 @@ void launch_ipt_track(int dim, const DevMesh& m, const IptParams& p, int64_t n, 
                       fjsph_ipt_record* out, hipStream_t st);
 void launch_ipt_gather(int dim, DevState s, const int32_t* slots, int64_t n, double t, IptStart* out, hipStream_t st);
 /* FindCell on the aero particles of s (mesh.hip). */
+/* FindCell over the aero particles: the previous cell, then the voxel
+ * candidates.  Particles in no cell are counted in sc->mesh_lost and, when
+ * lost_list is given (room for n), listed there. */
 void launch_find_cell(int dim, const Consts& c, DevState s, const int32_t* part_id, DevMesh m, int64_t n,
-                      int32_t* lost, hipStream_t st);
+                      DevScalars* sc, int32_t* lost_list, hipStream_t st);
+/* FindCell's boundary test for the nlost listed particles, one block each:
+ * lost_del[k] = 1 when particle k is to be deleted; sc->lost_* counted. */
+void launch_lost_classify(int dim, DevState s, DevMesh m, const int32_t* lost_list, int32_t nlost, int32_t* lost_del,
+                          DevScalars* sc, hipStream_t st);
+/* keep (per slot) and hflag (per host id) for a deletion of the listed
+ * particles with lost_del set; blkcount[k] counts them per block. */
+void launch_lost_marks(const int32_t* lost_list, const int32_t* lost_del, int32_t nlost, const int32_t* part_id,
+                       const int32_t* blk, int64_t n, int32_t* keep, int32_t* hflag, int32_t* blkcount,
+                       hipStream_t st);
 /* Runge-Kutta 4 (pairs.hip): stage update of s from pn (final_stage: the
  * weighted combination with stages s1..s3), boundary densities, and the
  * squared position change partials of Check_RK_Error. */

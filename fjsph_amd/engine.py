@@ -63,6 +63,7 @@ SIGNATURES = {
                                  _D, _D, _D]),
     "fjsph_get_mesh_cells": (C.c_int, [_VP, _D, _D, _D, _D]),
     "fjsph_mesh_lost": (C.c_int, [_VP, _I64]),
+    "fjsph_mesh_lost_info": (C.c_int, [_VP, C.POINTER(abi.LostInfo)]),
     "fjsph_mesh_eval": (C.c_int, [_VP, C.c_int64, _D, _I32, _D]),
     "fjsph_set_surrogate": (C.c_int, [_VP, C.c_int32, _F32, _F32, _D, _D, _D, _D]),
     "fjsph_surrogate_eval": (C.c_int, [_VP, C.c_int64, _D, _D]),
@@ -390,6 +391,12 @@ class Engine:
         v = np.zeros(1, np.int64)
         self._chk(self.lib.fjsph_mesh_lost(self.h, _ptr(v, _I64)), "mesh_lost")
         return int(v[0])
+
+    def lost_info(self):
+        """fjsph_mesh_lost_info as a dict (lost, deleted, internal, failed)."""
+        r = abi.LostInfo()
+        self._chk(self.lib.fjsph_mesh_lost_info(self.h, C.byref(r)), "mesh_lost_info")
+        return r.as_dict()
 
     def set_surrogate(self, sg):
         """fjsph_set_surrogate with a surrogate.Surrogate (or dict of the same fields)."""

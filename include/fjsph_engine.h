@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FJSPH_ABI_VERSION 5
+#define FJSPH_ABI_VERSION 6
 
 typedef enum fjsph_status {
     FJSPH_OK = 0,
@@ -188,6 +188,8 @@ enum {
     FJSPH_F_CELLV, FJSPH_F_CELLP, FJSPH_F_CELLRHO, FJSPH_F_CELLID,
     FJSPH_F_VPERT, FJSPH_F_SURFZONE, FJSPH_F_B,
     FJSPH_F_PID, /* SPHPart::part_id: host index at upload, then svar.part_id++ per inserted particle */
+    FJSPH_F_NFAILED,  /* SPHPart::ipt_n_failed: consecutive mesh lookups that placed the particle in no cell */
+    FJSPH_F_INTERNAL, /* SPHPart::internal: a lookup found it across a wall (marker -1) boundary face */
     FJSPH_F_COUNT
 };
 enum { FJSPH_STATE_N = 0, FJSPH_STATE_NP1 = 1 }; /* pn / pnp1 */
@@ -425,6 +427,23 @@ fjsph_status fjsph_set_mesh(fjsph_ctx* ctx, int64_t nverts, const double* verts,
 fjsph_status fjsph_get_mesh_cells(fjsph_ctx* ctx, double* centre, double* vel, double* p, double* rho);
 /* Aero particles found in no cell by the last lookup. */
 fjsph_status fjsph_mesh_lost(fjsph_ctx* ctx, int64_t* lost);
+
+/* What FindCell did with the particles it placed in no cell
+ * (src/Containment.cpp:734-805, src/Resid.cpp:481-519), counted since the
+ * state was uploaded: a ray from the particle to each of the 500 nearest
+ * cell centres is tested against those cells' boundary faces; a crossing of
+ * a far-field face (right cell -2) deletes the particle, a wall face (-1)
+ * sets FJSPH_F_INTERNAL; with no net crossing FJSPH_F_NFAILED counts up and
+ * the particle is deleted once it is past 10.  Deleted particles leave both
+ * states, the neighbour lists are built again and dSPH_PreStep runs again,
+ * inside the same lookup (Integration.cpp:47-48, 82-83). */
+typedef struct fjsph_lost_info {
+    int64_t lost;     /* aero particles in no cell at the last lookup */
+    int64_t deleted;  /* particles the lookups deleted */
+    int64_t internal; /* lookups that found a wall crossing */
+    int64_t failed;   /* lookups that found no net crossing */
+} fjsph_lost_info;
+fjsph_status fjsph_mesh_lost_info(fjsph_ctx* ctx, fjsph_lost_info* info);
 /* The mesh field at n host points (n*dim): the containing cell (-1 if none)
  * and its (vel[dim], p, rho) into vals (n*(dim+2), zeros if none).  No
  * counterpart in the reference; used to label the surrogate's training

@@ -362,7 +362,7 @@ template <int D>
 struct Part {
     int64_t part_id = 0;
     long cellID = kNoCell;
-    uint32_t b = 0, surf = 0, surfzone = 0, internal = 0;
+    uint32_t b = 0, surf = 0, surfzone = 0, internal = 0, n_failed = 0;
     Vec<D> xi = Vec<D>::zero(), v = Vec<D>::zero(), acc = Vec<D>::zero(), Af = Vec<D>::zero(), aVisc = Vec<D>::zero();
     double Rrho = 0, rho = 0, p = 0, m = 0, curve = 0, norm_curve = 0, woccl = 0, pDist = 0, deltaD = 0;
     Vec<D> cellV = Vec<D>::zero();
@@ -866,6 +866,66 @@ inline std::vector<int64_t> knn_cells(const MeshData& M, const double* p, size_t
     return out;
 }
 
+/* FindCell's 500-nearest list as the reference iterates it: the nearest
+ * centres first, then, with fewer cells than k, the zero-initialised tail
+ * of its result vector (cell 0). */
+inline std::vector<int64_t> knn_cells_padded(const MeshData& M, const double* p, size_t k)
+{
+    std::vector<int64_t> out = knn_cells(M, p, k);
+    out.resize(k, 0);
+    return out;
+}
+
+/* get_line_intersection (src/Geometry.cpp:312-341): does the segment
+ * p1-cellC cross the edge?  One-sided: denominators below machine epsilon
+ * (negative ones included) report no crossing. */
+inline int line_intersection(const MeshData& M, int64_t f, const double* p1, const double* cellC)
+{
+    const int64_t* fv = &M.face_v[size_t(M.face_off[size_t(f)])];
+    const double* e1 = M.vert(fv[0]);
+    const double* e2 = M.vert(fv[1]);
+    const double s0 = cellC[0] - p1[0], s1 = cellC[1] - p1[1];
+    const double r0 = e2[0] - e1[0], r1 = e2[1] - e1[1];
+    const double denom = (-r0 * s1 + s0 * r1);
+    if (denom < std::numeric_limits<double>::epsilon()) return 0;
+    const double u = (-s1 * (p1[0] - e1[0]) + s0 * (p1[1] - e1[1])) / denom;
+    const double t = (r0 * (p1[1] - e1[1]) - r1 * (p1[0] - e1[0])) / denom;
+    return (u > 0 && u < 1 && t > 0 && t < 1) ? 1 : 0;
+}
+
+/* FindCell's boundary test for a point in no cell (src/Containment.cpp:
+ * 734-778): for each listed cell in order, the segment from the point to
+ * the cell's centre against the cell's boundary faces (right < 0) in face
+ * order; a hit flips `cross`; a wall face (-1) sets `wall`, a far-field
+ * face (-2) sets `far`, and either ends that cell's face loop. */
+struct BoundaryTest {
+    int cross = 0;
+    bool wall = false, far = false;
+};
+inline BoundaryTest boundary_test(const MeshData& M, const double* testp, const std::vector<int64_t>& cand)
+{
+    BoundaryTest r;
+    for (int64_t idx : cand) {
+        const double* rayp = &M.cCentre[size_t(idx) * size_t(M.dim)];
+        for (int64_t k = M.cell_off[size_t(idx)]; k < M.cell_off[size_t(idx) + 1]; ++k) {
+            const int64_t f = M.cell_f[size_t(k)];
+            const int64_t right = M.face_r[size_t(f)];
+            if (right >= 0) continue;
+            const int ints = M.dim == 3 ? crossings3d(M, f, testp, rayp) : line_intersection(M, f, testp, rayp);
+            if (!ints) continue;
+            r.cross = !r.cross;
+            if (right == -1) {
+                r.wall = true;
+                break;
+            } else if (right == -2) {
+                r.far = true;
+                break;
+            }
+        }
+    }
+    return r;
+}
+
 #include "ipt_oracle.hpp"
 
 /* MESH::maxlength as the TAU readers set it (Place_Edges, Place_Faces,
@@ -947,17 +1007,23 @@ struct Sim {
     MeshData mesh;
     bool have_mesh = false;
     int64_t mesh_lost = 0; /* aero particles in no cell at the last FindCell */
+    int64_t mesh_deleted = 0, mesh_internal = 0, mesh_failed = 0; /* boundary-test outcomes since upload */
 
-    /* FindCell (src/Containment.cpp:706-820) for the aero particles of pnp1
+    /* FindCell (src/Containment.cpp:579-820) for the aero particles of pnp1
      * (FREE with lam_nb <= lam_cutoff; the others get no cell and keep their
-     * cell values): the previous cell, then the 5 and the 500 nearest cell
-     * centres.  A particle whose previous cell is unset (negative) skips the
-     * first test.  Particles in no cell keep their values and are counted
-     * (deletion at outer boundaries is not built). */
-    void find_cell(size_t start, size_t end)
+     * cell values): the previous cell (ipt_n_failed reset), then the 5 and
+     * the 500 nearest cell centres (ipt_n_failed and internal reset).  A
+     * particle whose previous cell is unset (negative) skips the first test.
+     * With `classify`, a particle in no cell gets the boundary test: a wall
+     * crossing sets internal, a far-field crossing condemns it, and with no
+     * net crossing ipt_n_failed counts up or, past 10, condemns it.  Returns
+     * the condemned indices, ascending, each once (the reference's list can
+     * hold an index twice). */
+    std::vector<size_t> find_cell(size_t start, size_t end, bool classify)
     {
-        int64_t lost = 0;
-#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : lost)
+        int64_t lost = 0, nint = 0, nfail = 0;
+        std::vector<char> del(end, 0);
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : lost, nint, nfail)
         for (long ii = long(start); ii < long(end); ++ii) {
             Part<D>& q = pnp1[ii];
             if (q.b != FJSPH_FREE || q.lam_nb > s.lam_cutoff) {
@@ -973,6 +1039,7 @@ struct Sim {
             };
             if (q.cellID >= 0 && size_t(q.cellID) < mesh.ncells() && check_cell(mesh, q.cellID, testp)) {
                 take(q.cellID);
+                q.n_failed = 0;
                 continue;
             }
             bool found = false;
@@ -980,14 +1047,64 @@ struct Sim {
                 for (int64_t cell : knn_cells(mesh, testp, k))
                     if (check_cell(mesh, cell, testp)) {
                         take(cell);
+                        q.n_failed = 0;
+                        q.internal = 0;
                         found = true;
                         break;
                     }
                 if (found) break;
             }
-            if (!found) ++lost;
+            if (found) continue;
+            ++lost;
+            if (!classify) continue;
+            const BoundaryTest r = boundary_test(mesh, testp, knn_cells_padded(mesh, testp, 500));
+            if (r.wall) {
+                q.internal = 1;
+                ++nint;
+            }
+            bool condemned = r.far;
+            if (r.cross == 0) {
+                ++nfail;
+                if (q.n_failed > 10) condemned = true;
+                else q.n_failed++;
+            }
+            if (condemned) del[size_t(ii)] = 1;
         }
         mesh_lost = lost;
+        mesh_internal += nint;
+        mesh_failed += nfail;
+        std::vector<size_t> out;
+        for (size_t ii = start; ii < end; ++ii)
+            if (del[ii]) out.push_back(ii);
+        return out;
+    }
+
+    /* get_aero_velocity's deletion (src/Resid.cpp:483-519): the condemned
+     * particles leave pn and pnp1, then the neighbour lists and
+     * dSPH_PreStep are made again.  Block ranges and inlet index lists
+     * shift by the deletions below them (the reference leaves the ranges
+     * and shifts the lists by all of them). */
+    void delete_lost(const std::vector<size_t>& del, double& npd)
+    {
+        if (del.empty()) return;
+        for (auto it = del.rbegin(); it != del.rend(); ++it) {
+            pnp1.erase(pnp1.begin() + long(*it));
+            pn.erase(pn.begin() + long(*it));
+        }
+        auto below = [&](size_t x) { return size_t(std::lower_bound(del.begin(), del.end(), x) - del.begin()); };
+        for (Block& bl : blocks) {
+            const int64_t f = bl.b.first, l = bl.b.last;
+            bl.b.first -= int64_t(below(size_t(f)));
+            bl.b.last -= int64_t(below(size_t(l)));
+            for (size_t& x : bl.back) x -= below(x);
+            for (auto& col : bl.buffer)
+                for (size_t& x : col) x -= below(x);
+        }
+        total_points -= del.size();
+        end_index -= del.size();
+        mesh_deleted += int64_t(del.size());
+        update_neighbours();
+        dSPH_PreStep(total_points, npd);
     }
 
     /* -------------------------------------------------------- neighbours */
@@ -1052,10 +1169,10 @@ struct Sim {
 
     /* ------------------------------------- get_aero_velocity (constVel)
      * src/Resid.cpp:569-611 */
-    void get_aero_velocity(size_t start, size_t end)
+    void get_aero_velocity(size_t start, size_t end, double& npd)
     {
         if (s.aero_source == FJSPH_MESH_INFL) {
-            if (have_mesh) find_cell(start, end);
+            if (have_mesh) delete_lost(find_cell(start, end, true), npd);
             return;
         }
         if (s.aero_source != FJSPH_CONST_VEL) return;
@@ -1721,7 +1838,7 @@ struct Sim {
     void prepasses(double& npd)
     {
         dSPH_PreStep(total_points, npd);
-        get_aero_velocity(start_index, end_index);
+        get_aero_velocity(start_index, end_index, npd);
         Detect_Surface(start_index, end_index);
         dissipation_terms(start_index, end_index);
         if (s.ale) particle_shift(start_index, end_index);
@@ -1744,7 +1861,7 @@ struct Sim {
                     any = true;
                 }
         }
-        if (any && s.aero_source == FJSPH_MESH_INFL && have_mesh) find_cell(start_index, end_index);
+        if (any && s.aero_source == FJSPH_MESH_INFL && have_mesh) (void)find_cell(start_index, end_index, false);
     }
 
     /* integrate_no_update (src/Integration.cpp:27-107) */

@@ -89,6 +89,8 @@ int download(Handle* h, int which, int field, double* out)
         case FJSPH_F_SURFZONE: out[i] = q.surfzone; break;
         case FJSPH_F_B: out[i] = q.b; break;
         case FJSPH_F_PID: out[i] = double(q.part_id); break;
+        case FJSPH_F_NFAILED: out[i] = q.n_failed; break;
+        case FJSPH_F_INTERNAL: out[i] = q.internal; break;
         default: {
             double* p = vec_ptr<D>(q, field);
             if (!p) return FJSPH_ERR_ARG;
@@ -113,6 +115,8 @@ int upload(Handle* h, int which, int field, const double* in)
         case FJSPH_F_SURFZONE: q.surfzone = uint32_t(in[i]); break;
         case FJSPH_F_B: q.b = uint32_t(in[i]); break;
         case FJSPH_F_PID: q.part_id = int64_t(in[i]); break;
+        case FJSPH_F_NFAILED: q.n_failed = uint32_t(in[i]); break;
+        case FJSPH_F_INTERNAL: q.internal = uint32_t(in[i]); break;
         default: {
             double* p = vec_ptr<D>(q, field);
             if (!p) return FJSPH_ERR_ARG;
@@ -454,7 +458,9 @@ int orc_dsph_prestep(void* hp, double* npd)
 int orc_aero_velocity(void* hp)
 {
     Handle* h = static_cast<Handle*>(hp);
-    SIMCALL(h, S.get_aero_velocity(S.bound_points, S.total_points), S.get_aero_velocity(S.bound_points, S.total_points));
+    double npd = 1.0;
+    SIMCALL(h, S.get_aero_velocity(S.bound_points, S.total_points, npd),
+            S.get_aero_velocity(S.bound_points, S.total_points, npd));
     return FJSPH_OK;
 }
 
@@ -485,6 +491,36 @@ int orc_mesh_lost(void* hp, int64_t* lost)
     Handle* h = static_cast<Handle*>(hp);
     *lost = h->dim == 2 ? h->s2->mesh_lost : h->s3->mesh_lost;
     return FJSPH_OK;
+}
+
+/* lost, deleted, internal, failed (fjsph_lost_info). */
+int orc_mesh_lost_info(void* hp, int64_t* out4)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    SIMCALL(h, (out4[0] = S.mesh_lost, out4[1] = S.mesh_deleted, out4[2] = S.mesh_internal, out4[3] = S.mesh_failed),
+            (out4[0] = S.mesh_lost, out4[1] = S.mesh_deleted, out4[2] = S.mesh_internal, out4[3] = S.mesh_failed));
+    return FJSPH_OK;
+}
+
+/* FindCell's boundary test of one point against its 500-nearest list:
+ * out3 = (cross, wall, far) (for pinning tests). */
+int orc_boundary_test(void* hp, const double* point, int32_t* out3)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    const MeshData& M = h->dim == 2 ? h->s2->mesh : h->s3->mesh;
+    const BoundaryTest r = boundary_test(M, point, knn_cells_padded(M, point, 500));
+    out3[0] = r.cross;
+    out3[1] = r.wall ? 1 : 0;
+    out3[2] = r.far ? 1 : 0;
+    return FJSPH_OK;
+}
+
+/* get_line_intersection of one segment against one 2D edge. */
+int orc_line_intersection(void* hp, int64_t face, const double* p1, const double* p2)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    const MeshData& M = h->dim == 2 ? h->s2->mesh : h->s3->mesh;
+    return line_intersection(M, face, p1, p2);
 }
 
 /* CheckCell of one point against one cell (for pinning tests). */

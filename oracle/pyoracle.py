@@ -81,6 +81,9 @@ def load(kind="oracle"):
         "orc_get_mesh_cells": (C.c_int, [vp, _D, _D, _D, _D]),
         "orc_mesh_lost": (C.c_int, [vp, _I64]),
         "orc_check_cell": (C.c_int, [vp, C.c_int64, _D]),
+        "orc_mesh_lost_info": (C.c_int, [vp, _I64]),
+        "orc_boundary_test": (C.c_int, [vp, _D, _I32]),
+        "orc_line_intersection": (C.c_int, [vp, C.c_int64, _D, _D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -249,6 +252,24 @@ class Oracle:
         p = np.zeros(3)
         p[:len(point)] = point
         return int(self.lib.orc_check_cell(self.h, int(cell), _ptr(p)))
+
+    def lost_info(self):
+        v = np.zeros(4, np.int64)
+        self._chk(self.lib.orc_mesh_lost_info(self.h, _ptr(v, _I64)), "mesh_lost_info")
+        return dict(zip(("lost", "deleted", "internal", "failed"), (int(a) for a in v)))
+
+    def boundary_test(self, point):
+        """FindCell's boundary test of a point: (cross, wall, far)."""
+        p = np.zeros(3)
+        p[:len(point)] = point
+        out = np.zeros(3, np.int32)
+        self._chk(self.lib.orc_boundary_test(self.h, _ptr(p), _ptr(out, _I32)), "boundary_test")
+        return int(out[0]), bool(out[1]), bool(out[2])
+
+    def line_intersection(self, face, p1, p2):
+        a, b = np.zeros(3), np.zeros(3)
+        a[:len(p1)], b[:len(p2)] = p1, p2
+        return int(self.lib.orc_line_intersection(self.h, int(face), _ptr(a), _ptr(b)))
     def upload_state(self, x, v, rho, p, m, b, n_bound):
         x = np.ascontiguousarray(x, np.float64)
         v = np.ascontiguousarray(v, np.float64)
