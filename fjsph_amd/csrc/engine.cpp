@@ -1010,6 +1010,11 @@ void rk_stage(fjsph_ctx* ctx, DevState& dst, DevState& prev, double dt, bool fin
                      ctx->near_inlet, ctx->nl, ctx->n, dt, ctx->partial, ctx->st);
     launch_rk_update(D, ctx->c, dst, ctx->pn, ctx->rk1, ctx->rk2, ctx->rk3, ctx->part_id, ctx->n, dt,
                      final_stage ? 1 : 0, ctx->st);
+    if (ctx->ncols > 0) {
+        upload_planes(ctx);
+        launch_rk_buffer(D, ctx->c, dst, ctx->pn, ctx->rk1, ctx->rk2, ctx->rk3, ctx->col, ctx->depth, ctx->blk,
+                         ctx->c_back, ctx->dpl, ctx->n, dt, ctx->d.dx, final_stage ? 1 : 0, ctx->st);
+    }
 }
 
 /* Check_RK_Error (src/Runge_Kutta.cpp:10-26): log10 rms of b.x - a.x over
@@ -1032,7 +1037,6 @@ double rk_error(fjsph_ctx* ctx, const DevState& a, const DevState& b, double log
  * state of the first build, then Runge_Kutta4 after the second. */
 double integrate_no_update_rk(fjsph_ctx* ctx)
 {
-    if (ctx->ncols > 0) throw std::string("inlet blocks need the Newmark-Beta integrator");
     if (!ctx->rk_alloc) {
         alloc_state(ctx, ctx->rk1);
         alloc_state(ctx, ctx->rk2);
@@ -1597,7 +1601,6 @@ fjsph_status fjsph_set_inlet(fjsph_ctx* ctx, int32_t block, int32_t n_cols, int3
         return FJSPH_ERR_ARG;
     return guarded(ctx, [&] {
         require(ctx, false);
-        if (ctx->s.integrator != FJSPH_NEWMARK_BETA) throw std::string("inlet blocks need the Newmark-Beta integrator");
         const fjsph_block& bk = ctx->blocks[size_t(block)].b;
         if (!bk.is_fluid) throw std::string("set_inlet: not a fluid block");
         for (const auto& in : ctx->inlets)

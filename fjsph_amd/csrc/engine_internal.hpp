@@ -256,6 +256,9 @@ void launch_surrogate_list(int dim, DevState s, Surrogate sg, int64_t n, const i
 void launch_pipe_outlet(int dim, DevState s, const int32_t* part_id, const int32_t* blk, DevPlanes pl, int64_t n,
                         int64_t n_bound, hipStream_t st);
 void launch_inlet_backslot(const int32_t* col, const int32_t* depth, int64_t n, int32_t* back_slot, hipStream_t st);
+void launch_rk_buffer(int dim, const Consts& c, DevState s, DevState pn, DevState s1, DevState s2, DevState s3,
+                      const int32_t* col, const int32_t* depth, const int32_t* blk, const int32_t* back_slot,
+                      DevPlanes pl, int64_t n, double dt, double dx, int final_stage, hipStream_t st);
 void launch_inlet_buffer(int dim, const Consts& c, DevState s, DevState pn, NextState nx, const int32_t* col,
                          const int32_t* depth, const int32_t* blk, const int32_t* back_slot, DevPlanes pl, int64_t n,
                          double dt, double dx, double* partial, hipStream_t st);

@@ -106,6 +106,50 @@ __global__ __launch_bounds__(TPB) void k_inlet_buffer(Consts c, DevState s, DevS
     if (threadIdx.x == 0 && bs != 0.0) partial[blockIdx.x] += bs;
 }
 
+/* The inlet buffer of a Runge-Kutta stage (src/Runge_Kutta.cpp:171-229,
+ * 389-447), after k_rk_update: with fixed_vel_or_dynamic = 1 off the
+ * column's back particle of this stage; otherwise the density from pn and
+ * the particle's own stage rate (dt Rrho, or dt/6 of the four rates in the
+ * last stage) and the position x_n + dt v_n.  The reference's branch for
+ * the latter reads the buffer list of block `jj` (the buffer depth) and, in
+ * the last stage, the rates of particle `ii` (the column number); the
+ * block's own buffer and the particle's own rates are used here. */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_rk_buffer(Consts c, DevState s, DevState pn, DevState s1, DevState s2,
+                                                   DevState s3, const int32_t* __restrict__ col,
+                                                   const int32_t* __restrict__ depth,
+                                                   const int32_t* __restrict__ blk,
+                                                   const int32_t* __restrict__ back_slot, DevPlanes pl, int64_t n,
+                                                   double dt, double dx, int final_stage)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n || col[i] < 0 || depth[i] <= 0 || s.b[i] != FJSPH_BUFFER) return;
+    const int k = blk[i];
+    if (pl.mode[k] == 1) {
+        const int64_t bk = back_slot[col[i]];
+        const double* nrm = pl.ins + 4 * k;
+        double len = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) len += nrm[a] * nrm[a];
+        len = sqrt(len);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            s.x[i * D + a] = s.x[bk * D + a] - dx * double(depth[i]) * (nrm[a] / len);
+            s.v[i * D + a] = s.v[bk * D + a];
+        }
+        s.rho[i] = s.rho[bk];
+        s.p[i] = s.p[bk];
+        return;
+    }
+    const double dr = final_stage ? (dt / 6.0) * (((pn.Rrho[i] + 2.0 * s1.Rrho[i]) + 2.0 * s2.Rrho[i]) + s3.Rrho[i])
+                                  : dt * s.Rrho[i];
+    const double rho = fmax(c.rho_min, fmin(c.rho_max, pn.rho[i] + dr));
+    s.rho[i] = rho;
+    s.p[i] = eos_pressure(c, rho);
+#pragma unroll
+    for (int a = 0; a < D; ++a) s.x[i * D + a] = pn.x[i * D + a] + dt * pn.v[i * D + a];
+}
+
 /* Back particles: crossed the insertion plane -> PIPE, column advances.
  * Also records the slot of each column's last buffer particle (depth
  * nbuf), the source of a new particle. */
@@ -261,6 +305,19 @@ void launch_pipe_outlet(int dim, DevState s, const int32_t* part_id, const int32
 void launch_inlet_backslot(const int32_t* col, const int32_t* depth, int64_t n, int32_t* back_slot, hipStream_t st)
 {
     hipLaunchKernelGGL(k_inlet_backslot, dim3(grid_of(n)), dim3(TPB), 0, st, col, depth, n, back_slot);
+}
+
+void launch_rk_buffer(int dim, const Consts& c, DevState s, DevState pn, DevState s1, DevState s2, DevState s3,
+                      const int32_t* col, const int32_t* depth, const int32_t* blk, const int32_t* back_slot,
+                      DevPlanes pl, int64_t n, double dt, double dx, int final_stage, hipStream_t st)
+{
+    const dim3 g(grid_of(n)), b(TPB);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_rk_buffer<2>, g, b, 0, st, c, s, pn, s1, s2, s3, col, depth, blk, back_slot, pl, n, dt,
+                           dx, final_stage);
+    else
+        hipLaunchKernelGGL(k_rk_buffer<3>, g, b, 0, st, c, s, pn, s1, s2, s3, col, depth, blk, back_slot, pl, n, dt,
+                           dx, final_stage);
 }
 
 void launch_inlet_buffer(int dim, const Consts& c, DevState s, DevState pn, NextState nx, const int32_t* col,

@@ -1750,6 +1750,49 @@ struct Sim {
         }
     }
 
+    /* The inlet buffer of a Runge-Kutta stage (src/Runge_Kutta.cpp:171-229,
+     * 389-447), after the block's fluid update.  fixed_vel_or_dynamic = 1:
+     * off the column's back particle of this stage.  Otherwise the density
+     * from part_n and the stage's rate of the particle itself (dt Rrho, or
+     * dt/6 of the four rates in the last stage, s1..s3 given) and the
+     * position x_n + dt v_n.  The reference's branch for the latter reads the
+     * buffer list of block `jj` (the buffer depth) and, in the last stage,
+     * the rates of particle `ii` (the column number); the block's own list
+     * and the particle's own rates are used here. */
+    void rk_buffer(const Block& bl, std::vector<Part<D>>& st, const std::vector<Part<D>>& part_n, double dt,
+                   const std::vector<Part<D>>* s1, const std::vector<Part<D>>* s2, const std::vector<Part<D>>* s3)
+    {
+        if (!bl.pl.inlet) return;
+        if (bl.pl.fixed_vel_or_dynamic == 1) {
+            Vec<D> unorm;
+            double len = 0.0;
+            for (int a = 0; a < D; ++a) len += bl.pl.insert_norm[a] * bl.pl.insert_norm[a];
+            len = std::sqrt(len);
+            for (int a = 0; a < D; ++a) unorm[a] = bl.pl.insert_norm[a] / len;
+            for (size_t c = 0; c < bl.back.size(); ++c) {
+                const Part<D>& bk = st[bl.back[c]];
+                for (size_t jj = 0; jj < bl.buffer[c].size(); ++jj) {
+                    Part<D>& q = st[bl.buffer[c][jj]];
+                    q.xi = bk.xi - (d.dx * (double(jj) + 1.0)) * unorm;
+                    q.v = bk.v;
+                    q.rho = bk.rho;
+                    q.p = bk.p;
+                }
+            }
+            return;
+        }
+        for (size_t c = 0; c < bl.back.size(); ++c)
+            for (size_t jj = 0; jj < bl.buffer[c].size(); ++jj) {
+                const size_t id = bl.buffer[c][jj];
+                const double dr = s1 ? (dt / 6.0) * (part_n[id].Rrho + 2.0 * (*s1)[id].Rrho + 2.0 * (*s2)[id].Rrho + (*s3)[id].Rrho)
+                                     : dt * st[id].Rrho;
+                const double rho = std::max(d.rho_min, std::min(d.rho_max, part_n[id].rho + dr));
+                st[id].rho = rho;
+                st[id].p = get_pressure(s, d, rho);
+                st[id].xi = part_n[id].xi + dt * part_n[id].v;
+            }
+    }
+
     std::vector<Part<D>> rk_intermediate(double npd, const std::vector<Part<D>>& part_n, const std::vector<Part<D>>& part_prev, double dt_inter)
     {
         std::vector<Part<D>> part_inter = part_prev;
@@ -1769,6 +1812,7 @@ struct Sim {
                     q.p = get_pressure(s, d, rho);
                 }
             }
+            rk_buffer(bl, part_inter, part_n, dt_inter, nullptr, nullptr, nullptr);
         }
         return part_inter;
     }
@@ -1798,6 +1842,7 @@ struct Sim {
                     q.xi = part_n[ii].xi + dt * q.v;
                 }
             }
+            rk_buffer(bl, part_np1, part_n, dt, &st_1, &st_2, &st_3);
         }
         return part_np1;
     }

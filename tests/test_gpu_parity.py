@@ -335,6 +335,10 @@ def _inlet_cases():
         # it part way through the columns
         capped.max_points = capped.n - ncol // 2
         out.append((name, capped))
+    for mode, name in ((0, "fixed_rk4"), (1, "dynamic_rk4")):
+        rk = cases.pipe_inlet_3d(mode=mode)
+        rk.settings.integrator = abi.RUNGE_KUTTA
+        out.append((name, rk))
     return out
 
 

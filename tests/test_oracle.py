@@ -678,11 +678,13 @@ def _columns(c):
     return c.x[c.inlets[0][1], :2]
 
 
+@pytest.mark.parametrize("integrator", ["newmark", "rk4"])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_inlet_buffer_insertion_deletion(po, mode):
-    """The inlet (update_buffer_region, the Do_NB_Iter buffer update) and the
-    deletion plane of the oracle against closed forms over 6 steps that see
-    one deletion and one column advance in every column:
+def test_inlet_buffer_insertion_deletion(po, mode, integrator):
+    """The inlet (update_buffer_region, the buffer update of Do_NB_Iter or of
+    the Runge-Kutta stages) and the deletion plane of the oracle against
+    closed forms over 6 steps that see one deletion and one column advance
+    in every column:
     - fixed velocity (mode 0): every BUFFER particle sits at
       z = -k dx + speed * t (k = 1..n_buf+1) on its column;
     - dynamic (mode 1): every BUFFER particle sits k dx behind the BACK
@@ -694,6 +696,8 @@ def test_inlet_buffer_insertion_deletion(po, mode):
     - no particle is left past the deletion plane, and the count changes
       by the insertions minus the deletions."""
     c = cases.pipe_inlet_3d(mode=mode, n_pipe=4, n_buf=3)
+    if integrator == "rk4":
+        c.settings.integrator = abi.RUNGE_KUTTA
     cols = _columns(c)
     ncol, nbuf = cols.shape[0], 3
     dx = c.settings.particle_step
@@ -713,8 +717,12 @@ def test_inlet_buffer_insertion_deletion(po, mode):
         # every buffer particle is on a column
         dxy = np.abs(x[buf, None, :2] - cols[None, :, :2]).max(axis=2)
         col_of = dxy.argmin(axis=1)
-        # dynamic BACK particles drift off their column by rounding only
-        assert dxy.min(axis=1).max() < 1e-6 * dx
+        # dynamic BACK particles drift off their column by rounding only under
+        # Newmark-Beta; under Runge-Kutta the first step's stages start from
+        # pn, whose cell velocity is still the upload's free stream, so the
+        # back layer takes a small uniform drag across the pipe (about 1.5e-5
+        # dx in 6 steps)
+        assert dxy.min(axis=1).max() < (1e-6 if integrator == "newmark" else 1e-4) * dx
         if mode == 0:
             k = (speed * t - x[buf, 2]) / dx
             assert np.abs(k - np.round(k)).max() < 1e-7 and k.min() > 0.5 and k.max() < nbuf + 1.5
