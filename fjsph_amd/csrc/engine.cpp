@@ -153,7 +153,7 @@ struct fjsph_ctx {
     double* norms_tmp = nullptr;
     double* partial = nullptr;
     int nparts = 0;
-    double* gather_tmp = nullptr; /* n * 9 doubles */
+    double* gather_tmp[3] = {nullptr, nullptr, nullptr}; /* spare field buffers: scalar, vector, matrix */
     int32_t* gather_tmp_i = nullptr;
 
     /* cell grid */
@@ -298,7 +298,8 @@ void free_all(fjsph_ctx* ctx)
     ctx->have_surrogate = false;
     dfree(ctx->x_nxt); dfree(ctx->v_nxt); dfree(ctx->rho_nxt); dfree(ctx->p_nxt);
     dfree(ctx->part_id); dfree(ctx->blk); dfree(ctx->near_inlet); dfree(ctx->norms_tmp);
-    dfree(ctx->partial); dfree(ctx->gather_tmp); dfree(ctx->gather_tmp_i);
+    dfree(ctx->partial); dfree(ctx->gather_tmp[0]); dfree(ctx->gather_tmp[1]); dfree(ctx->gather_tmp[2]);
+    dfree(ctx->gather_tmp_i);
     dfree(ctx->cell); dfree(ctx->cell_count); dfree(ctx->cell_start); dfree(ctx->cell_fill);
     dfree(ctx->order); dfree(ctx->vorder); dfree(ctx->pos); dfree(ctx->tile_sums);
     dfree(ctx->nl.lu4); dfree(ctx->nl.count); dfree(ctx->nl.nnb); dfree(ctx->export_buf);
@@ -373,7 +374,9 @@ void alloc_slots(fjsph_ctx* ctx, int64_t cap, int64_t keep)
     HIPCHK(hipMemset(ctx->col + k, 0xFF, (c - k) * 4));
     HIPCHK(hipMemset(ctx->depth + k, 0xFF, (c - k) * 4));
     dgrow(ctx->norms_tmp, c * D, 0);
-    dgrow(ctx->gather_tmp, c * D * D, 0);
+    dgrow(ctx->gather_tmp[0], c, 0);
+    dgrow(ctx->gather_tmp[1], c * D, 0);
+    dgrow(ctx->gather_tmp[2], c * D * D, 0);
     dgrow(ctx->gather_tmp_i, c, 0);
     dgrow(ctx->cell, c, k);
     dgrow(ctx->order, c, k);
@@ -503,10 +506,11 @@ void permute_all(fjsph_ctx* ctx, const int32_t* perm, int64_t count = -1)
                 std::swap(*(int32_t**)slot, ctx->gather_tmp_i);
             } else {
                 const int w = width_of(ctx, f);
-                launch_gather_d((double*)*slot, ctx->gather_tmp, perm, n, w, ctx->st);
-                /* gather_tmp is sized for the widest field; swap buffers only
-                 * when the widths match, otherwise copy back. */
-                HIPCHK(hipMemcpyAsync(*slot, ctx->gather_tmp, size_t(n) * w * 8, hipMemcpyDeviceToDevice, ctx->st));
+                /* the spare buffer of the field's width (same capacity)
+                 * takes the gathered values and becomes the field */
+                double*& spare = ctx->gather_tmp[field_desc(f).width_kind];
+                launch_gather_d((double*)*slot, spare, perm, n, w, ctx->st);
+                std::swap(*(double**)slot, spare);
             }
         }
     }
