@@ -149,6 +149,7 @@ def run_engine(case, args, label):
 
     print(f"[bench] {label}: {case.n} particles; building engine", file=sys.stderr, flush=True)
     eng = Engine(case)
+    eng.set_reorder_interval(args.reorder_every)
     if getattr(case, "surrogate", None) is not None:
         eng.set_surrogate(case.surrogate)
     eng.integrate_no_update()
@@ -206,6 +207,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile blocks (256 targets)")
+    ap.add_argument("--reorder-every", type=int, default=4,
+                    help="steps between the cell-order permutations of the particle arrays")
     ap.add_argument("--copy-probe", type=int, default=0)
     args = ap.parse_args()
 
@@ -343,6 +346,7 @@ def main():
             "subiterations_per_step": run["its"],
             "aero": aero_info,
             "tile_blocks": run.get("autotune"),
+            "reorder_every": args.reorder_every,
             "passes": passes,
         },
         "roofline": roofline,

@@ -126,6 +126,7 @@ struct fjsph_ctx {
     std::vector<double> mcentre, mvel, mp, mrho;
     double mesh_max_len = 0.0; /* MESH::maxlength */
     int32_t block_targets = 0; /* targets per tile block (0: 256) */
+    int32_t reorder_every = 4, steps_to_reorder = 0; /* cell-order permutation interval, steps left */
     /* implicit particle tracking */
     fjsph_ipt_settings ipt{};
     std::vector<fjsph_ipt_record> ipt_records;
@@ -1037,6 +1038,18 @@ double rk_error(fjsph_ctx* ctx, const DevState& a, const DevState& b, double log
     return std::log10(std::sqrt(ctx->sc_host->residual / nf)) - logbase;
 }
 
+/* Whether this step's first list build moves the particles into cell
+ * order. */
+bool reorder_due(fjsph_ctx* ctx)
+{
+    if (ctx->steps_to_reorder > 0) {
+        --ctx->steps_to_reorder;
+        return false;
+    }
+    ctx->steps_to_reorder = ctx->reorder_every - 1;
+    return true;
+}
+
 /* integrate_no_update with the Runge-Kutta solver: Get_First_RK on the
  * state of the first build, then Runge_Kutta4 after the second. */
 double integrate_no_update_rk(fjsph_ctx* ctx)
@@ -1050,7 +1063,7 @@ double integrate_no_update_rk(fjsph_ctx* ctx)
     ctx->iteration = 0;
     ctx->rollbacks = 0;
     ctx->delta_t = find_timestep(ctx);
-    build_neighbours(ctx, true);
+    build_neighbours(ctx, reorder_due(ctx));
     prepasses(ctx);
     rk_stage(ctx, ctx->pnp1, ctx->pn, 0.5 * ctx->delta_t, false);
     ctx->x_at_build = false;
@@ -1079,7 +1092,7 @@ double integrate_no_update(fjsph_ctx* ctx)
     ctx->rollbacks = 0;
     double logbase = 0.0;
     ctx->delta_t = find_timestep(ctx);
-    build_neighbours(ctx, true);
+    build_neighbours(ctx, reorder_due(ctx));
     prepasses(ctx);
     const double le0 = nb_iter(ctx);
     logbase = le0; /* Check_Error at iteration 0 */
@@ -1515,6 +1528,7 @@ fjsph_status fjsph_upload_state(fjsph_ctx* ctx, int64_t n_total, int64_t n_bound
         ctx->delta_t = ctx->s.delta_t_min > 0 ? ctx->s.delta_t_min : 2e-10;
         ctx->n_stable = ctx->n_unstable = 0;
         ctx->current_frame = 0;
+        ctx->steps_to_reorder = 0;
         ctx->next_part_id = n_total; /* svar.part_id after Init_Particles (src/Init.cpp:478) */
         /* blocks may have been set before the state: push them */
         if (!ctx->blocks.empty()) {
@@ -1832,6 +1846,14 @@ fjsph_status fjsph_set_block_targets(fjsph_ctx* ctx, int32_t targets)
     if (!ctx || targets < 0 || targets > 65536) return FJSPH_ERR_ARG;
     ctx->block_targets = targets;
     ctx->have_nbrs = false;
+    return FJSPH_OK;
+}
+
+fjsph_status fjsph_set_reorder_interval(fjsph_ctx* ctx, int32_t steps)
+{
+    if (!ctx || steps < 1) return FJSPH_ERR_ARG;
+    ctx->reorder_every = steps;
+    ctx->steps_to_reorder = 0;
     return FJSPH_OK;
 }
 

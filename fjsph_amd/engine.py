@@ -90,6 +90,7 @@ SIGNATURES = {
     "fjsph_ipt_records": (C.c_int, [_VP, C.c_int64, C.c_int64, C.c_void_p]),
     "fjsph_ipt_clear": (C.c_int, [_VP]),
     "fjsph_set_block_targets": (C.c_int, [_VP, C.c_int32]),
+    "fjsph_set_reorder_interval": (C.c_int, [_VP, C.c_int32]),
     "fjsph_get_block_targets": (C.c_int, [_VP, _I32]),
     "fjsph_autotune_blocks": (C.c_int, [_VP, C.c_int32, _I32, C.c_int32, _D, _I32]),
 }
@@ -227,6 +228,10 @@ class Engine:
 
     def set_block_targets(self, targets):
         self._chk(self.lib.fjsph_set_block_targets(self.h, int(targets)), "set_block_targets")
+
+    def set_reorder_interval(self, steps):
+        """fjsph_set_reorder_interval: cell-order permutation every `steps` steps."""
+        self._chk(self.lib.fjsph_set_reorder_interval(self.h, int(steps)), "set_reorder_interval")
 
     def block_targets(self):
         t = np.zeros(1, np.int32)

@@ -347,6 +347,13 @@ fjsph_status fjsph_get_inlet(fjsph_ctx* ctx, int32_t block, int32_t* n_cols, int
  * the order of summation. */
 fjsph_status fjsph_set_block_targets(fjsph_ctx* ctx, int32_t targets);
 fjsph_status fjsph_get_block_targets(fjsph_ctx* ctx, int32_t* targets);
+/* The first list build of a step moves the particles into cell order
+ * (a permutation of every per-particle array) once every `steps` steps;
+ * the builds in between index the particles through the cell order.
+ * Default 4 (the first step after an upload or this call reorders).  No
+ * counterpart in the reference; results change only in the order of
+ * summation. */
+fjsph_status fjsph_set_reorder_interval(fjsph_ctx* ctx, int32_t steps);
 /* Autotuner of the tile blocks: for each candidate, one step's pair work
  * (two list builds with their pre-passes, the force pass once per Newmark
  * iteration of the last step) timed over `reps` repetitions on the current
