@@ -150,6 +150,8 @@ def run_engine(case, args, label):
     print(f"[bench] {label}: {case.n} particles; building engine", file=sys.stderr, flush=True)
     eng = Engine(case)
     eng.set_reorder_interval(args.reorder_every)
+    if args.block_targets:
+        eng.set_block_targets(args.block_targets)
     if getattr(case, "surrogate", None) is not None:
         eng.set_surrogate(case.surrogate)
     eng.integrate_no_update()
@@ -206,7 +208,9 @@ def main():
     ap.add_argument("--cpu-particles", type=int, default=60_000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile blocks (256 targets)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the tile blocks as set (default 256 targets)")
+    ap.add_argument("--block-targets", type=int, default=0,
+                    help="targets per tile block before any autotuning (0: the engine's default, 256)")
     ap.add_argument("--reorder-every", type=int, default=4,
                     help="steps between the cell-order permutations of the particle arrays")
     ap.add_argument("--copy-probe", type=int, default=0)
@@ -345,7 +349,7 @@ def main():
             "max_neighbours": max_count + 1,
             "subiterations_per_step": run["its"],
             "aero": aero_info,
-            "tile_blocks": run.get("autotune"),
+            "tile_blocks": run.get("autotune") or {"block_targets": args.block_targets or 256, "autotuned": False},
             "reorder_every": args.reorder_every,
             "passes": passes,
         },
