@@ -72,22 +72,43 @@ struct Grid3 {
     int32_t dims[3];
 };
 
+/* Runs of equal consecutive values in a wave (every lane takes part):
+ * the lane's run head and, for a head, the run length.  The particles are
+ * in cell order after a permutation and nearly so between permutations, so
+ * one atomic per run replaces one per particle. */
+__device__ __forceinline__ void wave_runs(int32_t v, int lane, int& head, int& len)
+{
+    const int32_t prev = __shfl_up(v, 1, 64);
+    const bool is_head = lane == 0 || prev != v;
+    const uint64_t heads = __ballot(is_head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    head = 63 - __clzll(heads & upto);
+    const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1));
+    len = after ? __ffsll((unsigned long long)after) : 64 - lane;
+}
+
 template <int D>
 __global__ __launch_bounds__(TPB) void k_cells(const double* __restrict__ x, int64_t n, Grid3 g, int32_t* cell,
                                                int32_t* cell_count)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
-    if (i >= n) return;
-    int32_t ci[3] = {0, 0, 0};
+    const int lane = threadIdx.x & 63;
+    const bool valid = i < n;
+    int32_t lin = -1 - lane; /* lanes past n: distinct values, runs of their own */
+    if (valid) {
+        int32_t ci[3] = {0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-        double f = floor((x[i * D + k] - g.lo[k]) * g.inv_cs);
-        int32_t c = (int32_t)fmin(fmax(f, 0.0), double(g.dims[k] - 1));
-        ci[k] = c;
+        for (int k = 0; k < D; ++k) {
+            double f = floor((x[i * D + k] - g.lo[k]) * g.inv_cs);
+            int32_t c = (int32_t)fmin(fmax(f, 0.0), double(g.dims[k] - 1));
+            ci[k] = c;
+        }
+        lin = (D == 3) ? ((ci[2] * g.dims[1] + ci[1]) * g.dims[0] + ci[0]) : (ci[1] * g.dims[0] + ci[0]);
+        cell[i] = lin;
     }
-    const int32_t lin = (D == 3) ? ((ci[2] * g.dims[1] + ci[1]) * g.dims[0] + ci[0]) : (ci[1] * g.dims[0] + ci[0]);
-    cell[i] = lin;
-    atomicAdd(&cell_count[lin], 1);
+    int head, len;
+    wave_runs(lin, lane, head, len);
+    if (valid && head == lane) atomicAdd(&cell_count[lin], len);
 }
 
 void launch_cells(int dim, const double* x, int64_t n, const double* lo, double inv_cs, const int32_t* dims,
@@ -193,10 +214,15 @@ __global__ __launch_bounds__(TPB) void k_scatter(const int32_t* __restrict__ cel
                                                  const int32_t* __restrict__ start, int32_t* fill, int32_t* order)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
-    if (i >= n) return;
-    const int32_t c = cell[i];
-    const int32_t pos = start[c] + atomicAdd(&fill[c], 1);
-    order[pos] = (int32_t)i;
+    const int lane = threadIdx.x & 63;
+    const bool valid = i < n;
+    const int32_t c = valid ? cell[i] : -1 - lane;
+    int head, len;
+    wave_runs(c, lane, head, len);
+    int32_t base = 0;
+    if (valid && head == lane) base = atomicAdd(&fill[c], len);
+    base = __shfl(base, head, 64);
+    if (valid) order[start[c] + base + (lane - head)] = (int32_t)i;
 }
 
 void launch_scatter(const int32_t* cell, int64_t n, const int32_t* start, int32_t* fill, int32_t* order,
