@@ -206,7 +206,7 @@ def main():
     ap.add_argument("--train-samples", type=int, default=1_000_000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--cpu-particles", type=int, default=60_000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the tile blocks as set (default 256 targets)")
     ap.add_argument("--block-targets", type=int, default=0,
