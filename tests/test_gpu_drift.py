@@ -5,8 +5,11 @@ relative after 1000 steps):
   dx = 0.02 m, 6.5k particles), 1000 Newmark-Beta steps;
 - a 3D droplet in crossflow (0.8 mm, dx = 0.05 mm, 2.1k particles) with the
   mixed-precision pair passes (FP32 pair terms, FP64 sums, positions and
-  time) against the FP64 oracle, 500 steps (the droplet configuration's
-  length).
+  time) against the FP64 oracle: 500 steps with the adaptive time step (the
+  droplet configuration's length), and 1000 steps at a fixed time step.
+  Beyond about 700 adaptive steps the two runs no longer take the same time
+  steps (a sub-iteration count that differs moves the CFL number), so the
+  1000-step comparison holds the step fixed.
 Errors are field-normwise (max |engine - oracle| / max |oracle|) and are
 printed for every field."""
 import numpy as np
@@ -63,3 +66,20 @@ def test_droplet_drift_500_steps_mixed():
                     ("P", eng.field("P"), orc.field("P"))):
         errs[f] = assert_close(f, g, r, DRIFT)
     print("  drift after 500 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+
+
+def test_droplet_drift_1000_steps_mixed_fixed_step():
+    case = cases.droplet_3d(diameter=0.8e-3, dx=5e-5, precision=abi.PREC_MIXED)
+    case.settings.delta_t_min = case.settings.delta_t_max = 3.2e-7  # the adaptive scheme's first steps
+    print(f"[drift] droplet, mixed precision, fixed step 3.2e-7 s: n={case.n}, 1000 steps", flush=True)
+    eng, orc, dts, its = _run(case, 1000)
+    assert np.all(dts[:, 0] == dts[:, 1])
+    print(f"  sub-iteration counts differ on {(its[:, 0] != its[:, 1]).sum()} of 1000 steps", flush=True)
+    rho0 = case.settings.rho_rest
+    errs = {}
+    for f, g, r in (("X-x0", eng.field("X") - case.x, orc.field("X") - case.x),
+                    ("V", eng.field("V"), orc.field("V")),
+                    ("RHO-rho0", eng.field("RHO") - rho0, orc.field("RHO") - rho0),
+                    ("P", eng.field("P"), orc.field("P"))):
+        errs[f] = assert_close(f, g, r, DRIFT)
+    print("  drift after 1000 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
