@@ -231,44 +231,48 @@ void launch_scatter(const int32_t* cell, int64_t n, const int32_t* start, int32_
     hipLaunchKernelGGL(k_scatter, dim3(grid_of(n)), dim3(TPB), 0, st, cell, n, start, fill, order);
 }
 
-/* Sort each cell's slot list ascending so list order is deterministic: one
- * workgroup per cell, segment staged in LDS, each lane writes its elements
- * at their rank.  Segments longer than the LDS tile fall back to an
+/* Sort each cell's slot list ascending so list order is deterministic: a
+ * workgroup per cell at a time, segment staged in LDS, each lane writes its
+ * elements at their rank.  Segments longer than the LDS tile fall back to an
  * insertion sort in one lane. */
 constexpr int SORT_TILE = 1024;
 
 __global__ __launch_bounds__(64) void k_sort_cells(const int32_t* __restrict__ start, int64_t ncell, int32_t* order)
 {
     __shared__ int32_t seg[SORT_TILE];
-    const int64_t c = blockIdx.x;
-    if (c >= ncell) return;
-    const int32_t a = start[c], b = start[c + 1];
-    const int32_t k = b - a;
-    if (k <= 1) return;
-    if (k > SORT_TILE) {
-        if (threadIdx.x == 0) {
-            for (int32_t i = a + 1; i < b; ++i) {
-                const int32_t v = order[i];
-                int32_t j = i - 1;
-                while (j >= a && order[j] > v) { order[j + 1] = order[j]; --j; }
-                order[j + 1] = v;
+    /* cells dealt to the workgroups in turn (sparse grids are mostly empty
+     * cells, which a workgroup each would only launch and retire) */
+    for (int64_t c = blockIdx.x; c < ncell; c += gridDim.x) {
+        const int32_t a = start[c], b = start[c + 1];
+        const int32_t k = b - a;
+        if (k <= 1) continue;
+        if (k > SORT_TILE) {
+            if (threadIdx.x == 0) {
+                for (int32_t i = a + 1; i < b; ++i) {
+                    const int32_t v = order[i];
+                    int32_t j = i - 1;
+                    while (j >= a && order[j] > v) { order[j + 1] = order[j]; --j; }
+                    order[j + 1] = v;
+                }
             }
+            continue;
         }
-        return;
-    }
-    for (int t = threadIdx.x; t < k; t += 64) seg[t] = order[a + t];
-    __syncthreads();
-    for (int t = threadIdx.x; t < k; t += 64) {
-        const int32_t v = seg[t];
-        int rank = 0;
-        for (int u = 0; u < k; ++u) rank += seg[u] < v;
-        order[a + rank] = v;
+        __syncthreads(); /* seg is read by the previous cell's ranking */
+        for (int t = threadIdx.x; t < k; t += 64) seg[t] = order[a + t];
+        __syncthreads();
+        for (int t = threadIdx.x; t < k; t += 64) {
+            const int32_t v = seg[t];
+            int rank = 0;
+            for (int u = 0; u < k; ++u) rank += seg[u] < v;
+            order[a + rank] = v;
+        }
     }
 }
 
 void launch_sort_cells(const int32_t* start, int64_t ncell, int32_t* order, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_sort_cells, dim3((unsigned)ncell), dim3(64), 0, st, start, ncell, order);
+    const int64_t g = std::min<int64_t>(std::max<int64_t>(ncell, 1), 65536);
+    hipLaunchKernelGGL(k_sort_cells, dim3((unsigned)g), dim3(64), 0, st, start, ncell, order);
 }
 
 /* Per-particle derived quantities (see Aux). */
