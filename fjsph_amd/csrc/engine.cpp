@@ -574,12 +574,8 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
     }
     HIPCHK(hipMemcpyAsync(ctx->aux.xb, ctx->pnp1.x, sizeof(double) * n * D, hipMemcpyDeviceToDevice, ctx->st));
     /* tile grid: chunks of C cells sized for about one workgroup of targets */
-    {
+    auto set_tiles = [&](int32_t C) {
         TileGrid& tg = ctx->tg;
-        const double occ = std::pow(cs / ctx->s.particle_step, D);
-        const double targets = ctx->block_targets > 0 ? double(ctx->block_targets) : 256.0;
-        int32_t C = int32_t(std::lround(targets / std::max(occ, 1.0)));
-        C = std::max(1, std::min(C, ctx->dims[0]));
         for (int k = 0; k < 3; ++k) tg.dims[k] = ctx->dims[k];
         tg.C = C;
         tg.nchunk = (ctx->dims[0] + C - 1) / C;
@@ -594,17 +590,33 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
             ctx->partial_cap = tg.nblocks + tg.nblocks / 4 + 64;
             dalloc(ctx->partial, size_t(std::max<int64_t>(ctx->partial_cap, int64_t(ctx->nparts) * 8)));
         }
+    };
+    int32_t C;
+    {
+        const double occ = std::pow(cs / ctx->s.particle_step, D);
+        const double targets = ctx->block_targets > 0 ? double(ctx->block_targets) : 256.0;
+        C = int32_t(std::lround(targets / std::max(occ, 1.0)));
+        /* at most kMaxBlockCells cells per block: the list build's FP32
+         * prefilter band (k_build_t) holds for blocks up to that length */
+        C = std::max(1, std::min({C, ctx->dims[0], kMaxBlockCells}));
     }
-    for (int attempt = 0; attempt < 2; ++attempt) {
+    set_tiles(C);
+    for (int attempt = 0; attempt < 40; ++attempt) {
         HIPCHK(hipMemsetAsync(&ctx->sc->max_count, 0, 2 * sizeof(int32_t), ctx->st));
         launch_build_lists(D, ctx->pnp1.x, n, ctx->tg, ctx->d.sr, ctx->nl, ctx->sc, ctx->st);
         HIPCHK(hipMemcpyAsync(&ctx->sc_host->max_count, &ctx->sc->max_count, 2 * sizeof(int32_t),
                               hipMemcpyDeviceToHost, ctx->st));
         sync(ctx);
-        if (ctx->sc_host->overflow)
-            throw std::string("neighbour candidate union of a tile exceeds ") + std::to_string(NO_ENTRY / TILE) +
-                " LDS tiles (union of " +
-                std::to_string(ctx->sc_host->overflow) + "); particles are too clustered for the tile grid";
+        if (ctx->sc_host->overflow) {
+            /* a clustered region: shorter blocks have smaller unions */
+            if (ctx->tg.C > 1) {
+                set_tiles(std::max(1, ctx->tg.C / 2));
+                continue;
+            }
+            throw std::string("neighbour candidate union of a one-cell tile block exceeds ") +
+                std::to_string(NO_ENTRY / TILE) + " LDS tiles (union of " + std::to_string(ctx->sc_host->overflow) +
+                "); particles are too clustered for the tile grid";
+        }
         const int32_t mc = ctx->sc_host->max_count;
         ctx->max_count = mc;
         if (mc <= ctx->nl.cap) break;
@@ -1950,6 +1962,8 @@ fjsph_status fjsph_autotune_blocks(fjsph_ctx* ctx, int32_t n_cand, const int32_t
                 (void)hipMemcpyAsync(a.first, snap + off, a.second, hipMemcpyDeviceToDevice, ctx->st);
                 off += (a.second + 255) / 256 * 256;
             }
+            /* the back slot of every inlet column follows the restored order */
+            if (ctx->ncols > 0) launch_inlet_backslot(ctx->col, ctx->depth, n, ctx->c_back, ctx->st);
             (void)hipStreamSynchronize(ctx->st);
             throw;
         }
@@ -1961,6 +1975,9 @@ fjsph_status fjsph_autotune_blocks(fjsph_ctx* ctx, int32_t n_cand, const int32_t
             HIPCHK(hipMemcpyAsync(a.first, snap + off, a.second, hipMemcpyDeviceToDevice, ctx->st));
             off += (a.second + 255) / 256 * 256;
         }
+        /* the tuning builds moved the inlet columns' back slots with their
+         * permutations: find them again in the restored order */
+        if (ctx->ncols > 0) launch_inlet_backslot(ctx->col, ctx->depth, n, ctx->c_back, ctx->st);
         sync(ctx);
         ctx->timing = timing;
         ctx->tuning = false;

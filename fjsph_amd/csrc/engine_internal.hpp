@@ -66,6 +66,10 @@ struct NbrList {
 constexpr int TILE = FJ_TILE;
 static_assert(TILE % 4 == 0, "tile slots come in groups of 4");
 constexpr uint32_t NO_ENTRY = 0xFFFF;
+/* Longest tile block in cells: every coordinate of a block's candidate
+ * union then lies within 1000 H of its first target, the bound the FP32
+ * prefilter of the list build relies on. */
+constexpr int32_t kMaxBlockCells = 256;
 
 /* Tile decomposition of the last neighbour build: block b covers cells
  * [q*C, q*C + C) of cell row b / nchunk, q = b % nchunk. */

@@ -344,7 +344,10 @@ fjsph_status fjsph_get_inlet(fjsph_ctx* ctx, int32_t block, int32_t* n_cols, int
 /* Targets per tile block of the pair passes (the cells of one block row
  * whose particles share one candidate union in LDS); 0 restores the
  * default, 256.  No counterpart in the reference.  Results change only in
- * the order of summation. */
+ * the order of summation.  A block spans at most 256 cells whatever the
+ * setting, and a block whose candidate union overflows the 16-bit list
+ * entries (a clustered region) makes the build halve the block length
+ * and try again, down to one cell. */
 fjsph_status fjsph_set_block_targets(fjsph_ctx* ctx, int32_t targets);
 fjsph_status fjsph_get_block_targets(fjsph_ctx* ctx, int32_t* targets);
 /* The first list build of a step moves the particles into cell order
