@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 # fjsph_status
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_STATE, ERR_UNSUPPORTED, ERR_NO_DEVICE = range(7)
@@ -137,7 +137,8 @@ class Clock(C.Structure):
 
 class LostInfo(C.Structure):
     """fjsph_lost_info: FindCell's handling of particles in no cell."""
-    _fields_ = [("lost", C.c_int64), ("deleted", C.c_int64), ("internal", C.c_int64), ("failed", C.c_int64)]
+    _fields_ = [("lost", C.c_int64), ("deleted", C.c_int64), ("internal", C.c_int64), ("failed", C.c_int64),
+                ("first_cell", C.c_int64), ("first_cell_deleted", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -282,6 +282,251 @@ def pipe_inlet_3d(diameter=6e-4, dx=5e-5, speed=5.0, n_pipe=6, n_buf=4, mode=0, 
     return c
 
 
+def _wall_rows(x0, x1, y, dx, layers, outward):
+    """`layers` rows of wall particles at y + outward (k + 1/2) dx, k <
+    layers, spaced dx from x0 + dx/2 while below x1."""
+    xs = x0 + (np.arange(int(round((x1 - x0) / dx))) + 0.5) * dx
+    return np.concatenate([np.stack([xs, np.full_like(xs, y + outward * (k + 0.5) * dx)], axis=1)
+                           for k in range(layers)])
+
+
+def _validation_case(name, s, walls, xf, notes):
+    """Walls (list of (points, spec)) first, one block each, then the fluid."""
+    xb = np.concatenate([w for w, _ in walls]) if walls else np.zeros((0, 2))
+    c = _finish(name, s, xb, xf, 0.0, notes)
+    blocks, first = [], 0
+    for pts, spec in walls:
+        blocks.append(dict(first=first, last=first + pts.shape[0], is_fluid=0, **spec))
+        first += pts.shape[0]
+    blocks.append(dict(first=first, last=c.n, is_fluid=1))
+    c.blocks = blocks
+    return c
+
+
+def couette_2d(dx=0.02, length=3.0, height=1.0, v_wall=1e-3, layers=4, precision=abi.PREC_FP64):
+    """Couette start-up (Examples/Couette: para, fluid.bmap, boundary.bmap):
+    rho0 = 1, mu = 1e-2, sigma = 0, c0 = 1, alpha = 0.01, H = 2 dx, CFL 1,
+    maximum step 0.5 s, no gravity.  Fluid in 0 < x < length between a wall
+    at rest below y = 0 and a wall above y = height whose particles move at
+    v_wall along +x (the block velocity, which enters the fluid's viscous
+    terms; wall positions stay, as in the reference); 4 rows of wall
+    particles each (Adami pressure, free slip), reaching 4 dx past the
+    fluid's ends, which are free.  The example's lattice puts its first wall
+    row on y = 0 and its fluid rows from dx; here the fluid sits at cell
+    centres ((i + 1/2) dx, (j + 1/2) dx) and the wall rows at -(k + 1/2) dx
+    and height + (k + 1/2) dx, so both walls lie where the analytic solution
+    puts them.  The example's analytic profiles are in
+    Examples/Couette/SteadyState.dat."""
+    s = abi.default_settings(2)
+    s.particle_step = dx
+    s.H_fac = 2.0
+    s.rho_rest = 1.0
+    s.mu = 1e-2
+    s.sig = 0.0
+    s.speed_sound = 1.0
+    s.visc_alpha = 0.01
+    s.cfl = 1.0
+    s.delta_t_max = 0.5
+    s.grav[0] = s.grav[1] = 0.0
+    s.precision = precision
+    s.aero_case = abi.NO_AERO
+    nx, ny = int(round(length / dx)), int(round(height / dx))
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    xf = np.stack([(i.ravel() + 0.5) * dx, (j.ravel() + 0.5) * dx], axis=1)
+    pad = 4 * dx
+    bottom = _wall_rows(-pad, length + pad, 0.0, dx, layers, -1.0)
+    top = _wall_rows(-pad, length + pad, height, dx, layers, +1.0)
+    return _validation_case("couette_2d", s, [(bottom, dict(bound_solver=abi.PRESSURE_G)),
+                                              (top, dict(bound_solver=abi.PRESSURE_G, vel=(v_wall, 0.0, 0.0)))],
+                            xf, f"Couette {nx}x{ny} fluid, dx={dx}, top wall at {v_wall} m/s")
+
+
+def poiseuille_2d(dx=4e-5, length=5e-3, half_height=5e-4, g=1e-4, layers=4, precision=abi.PREC_FP64):
+    """Body-force-driven channel start-up (Examples/Poiseuille: para,
+    boundary.bmap): rho0 = 1000, mu = 1e-3, sigma = 0, c0 = 1, alpha =
+    0.001, H = 2 dx, CFL 1.111112, gravity g along +x, walls at y = +-0.5 mm
+    of 4 particle rows (Adami pressure, free slip).  The example feeds the
+    channel from a dynamic inlet and drains it past x = 5 mm; here the 5 mm
+    channel's ends are free, so its core accelerates under the body force
+    alone, as the fully developed flow the analytic profiles describe does
+    (the slug moves about 6 um in 0.5 s, against a spacing of 40 um).
+    Lattice: fluid at cell centres, wall rows at -+(h + (k + 1/2) dx).  The
+    analytic profiles are in Examples/Poiseuille/SteadyState.dat."""
+    s = abi.default_settings(2)
+    s.particle_step = dx
+    s.H_fac = 2.0
+    s.rho_rest = 1000.0
+    s.mu = 1e-3
+    s.sig = 0.0
+    s.speed_sound = 1.0
+    s.visc_alpha = 0.001
+    s.cfl = 1.111112
+    s.grav[0], s.grav[1] = g, 0.0
+    s.precision = precision
+    s.aero_case = abi.NO_AERO
+    nx, ny = int(round(length / dx)), int(round(2 * half_height / dx))
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    xf = np.stack([(i.ravel() + 0.5) * dx, (j.ravel() + 0.5) * dx - half_height], axis=1)
+    pad = 4 * dx
+    bottom = _wall_rows(-pad, length + pad, -half_height, dx, layers, -1.0)
+    top = _wall_rows(-pad, length + pad, half_height, dx, layers, +1.0)
+    return _validation_case("poiseuille_2d", s, [(bottom, dict(bound_solver=abi.PRESSURE_G)),
+                                                 (top, dict(bound_solver=abi.PRESSURE_G))],
+                            xf, f"Poiseuille channel {nx}x{ny} fluid, dx={dx}")
+
+
+def standing_column_2d(dx=0.01, width=1.0, height=2.0, layers=4, precision=abi.PREC_FP64):
+    """Hydrostatic column (Examples/Standing_Column: para, fluid.bmap,
+    boundary.bmap): a 1 x 2 m water column in a tank of its width, rho0 =
+    1000, mu = 8.94e-4, sigma = 0, c0 = 100, alpha = 0.1, H = 2 dx, CFL 1,
+    g = -9.81 along y, walls of 4 particle rows (Adami pressure) left, right
+    and below, and the example's hydrostatic initial pressure p = rho0 g (H
+    - y) (density from the Cole equation of state).  Fluid at cell centres,
+    wall rows outside the tank at -(k + 1/2) dx.  The ideal line P / (rho g
+    H) = (H - Y) / H is Examples/Standing_Column/Ideal.dat."""
+    s = abi.default_settings(2)
+    s.particle_step = dx
+    s.H_fac = 2.0
+    s.rho_rest = 1000.0
+    s.mu = 8.94e-4
+    s.sig = 0.0
+    s.speed_sound = 100.0
+    s.visc_alpha = 0.1
+    s.cfl = 1.0
+    s.grav[0], s.grav[1] = 0.0, -9.81
+    s.precision = precision
+    s.aero_case = abi.NO_AERO
+    nx, ny = int(round(width / dx)), int(round(height / dx))
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    xf = np.stack([(i.ravel() + 0.5) * dx, (j.ravel() + 0.5) * dx], axis=1)
+    walls = []
+    for k in range(layers):
+        off = (k + 0.5) * dx
+        xs = (np.arange(-layers, nx + layers) + 0.5) * dx
+        walls.append(np.stack([xs, np.full_like(xs, -off)], axis=1))
+        ys = (np.arange(0, ny + 4) + 0.5) * dx
+        walls.append(np.stack([np.full_like(ys, -off), ys], axis=1))
+        walls.append(np.stack([np.full_like(ys, width + off), ys], axis=1))
+    xb = np.concatenate(walls)
+    c = _validation_case("standing_column_2d", s, [(xb, dict(bound_solver=abi.PRESSURE_G))], xf,
+                         f"standing column {nx}x{ny}, dx={dx}")
+    nb = c.n_bound
+    B = s.rho_rest * s.speed_sound ** 2 / s.gam
+    p = s.rho_rest * 9.81 * (height - c.x[nb:, 1])
+    c.p = c.p.copy()
+    c.rho = c.rho.copy()
+    c.p[nb:] = p
+    c.rho[nb:] = s.rho_rest * (1.0 + p / B) ** (1.0 / s.gam)
+    c.m = np.full(c.n, s.rho_rest * dx * dx)
+    return c
+
+
+def hollow_cylinder_3d(radius, length, dx, layers):
+    """Wall of a pipe along +y ending at y = 0 (src/shapes/cylinder.cpp
+    create_hollow_cylinder, square ordering, rotated so that its axis is
+    +y): rings of ni = ceil(2 pi radius / dx) particles, dtheta = 2 pi / ni,
+    at radii radius + k dx for k < layers, every dx from y = 0 down to
+    y = -(nj - 1) dx, nj = ceil(length / dx) + 1."""
+    ni = max(1, int(math.ceil(2.0 * math.pi / (dx / radius))))
+    nj = max(1, int(math.ceil(length / dx)) + 1)
+    dtheta = 2.0 * math.pi / ni
+    pts = []
+    for jj in range(nj):
+        for kk in range(layers):
+            r = radius + kk * dx
+            th = np.arange(ni) * dtheta
+            pts.append(np.stack([np.cos(th) * r, np.full(ni, -jj * dx), np.sin(th) * r], axis=1))
+    return np.concatenate(pts)
+
+
+def jet_in_crossflow_3d(n_target=1_000_000, diameter=1e-3, dx=3e-5, speed=5.0, n_pipe=8, n_buf=4, mode=0,
+                        wall_layers=2, precision=abi.PREC_MIXED, H_fac=2.0, gas=50.0, rho_g=1.2262,
+                        jet_layers=None, back_to_insert=0.1, delete_above=None):
+    """3D jet in crossflow: a liquid jet of diameter D issuing at `speed`
+    along +y from a pipe into a crossflow along +x (Examples/Crossflow:
+    para3D, fluid_3D.bmap, pipe_3D.bmap).
+
+    - pipe wall: a hollow cylinder of ghost particles (pipe_3D.bmap: Shape
+      Cylinder, Sub-shape Hollow, Boundary solver Ghost, no slip 0, two
+      radial particle layers) of inner radius D/2 + dx, from the pipe exit
+      at y = 0 down past the inlet buffer;
+    - inlet (fluid_3D.bmap: Shape Inlet, Sub-shape Circle): columns on the
+      square lattice of spacing dx inside the circle of radius D/2, n_pipe
+      PIPE layers up to the exit (y = 0), the BACK layer below them and
+      n_buf BUFFER layers below that, all at `speed` along +y; the
+      insertion plane back_to_insert dx above the back layer, the
+      aerodynamic entry plane half a spacing above the exit (PIPE particles
+      beyond it turn FREE), mode 0 fixed-velocity / 1 dynamic buffer;
+    - the jet already issued: jet_layers FREE layers of the columns above
+      the exit at `speed` (by default as many as make n_target particles in
+      all: the configuration grows to 1M particles, and the state starts
+      grown);
+    - a deletion plane `delete_above` (m, along +y) if given.
+    Physics of para3D: rho0 = 997, c0 = 300, alpha = 0.05, mu = 8.94e-4,
+    sigma = 0.0708, Gissler drag, gas rho_g = 1.2262 in a stream of `gas`
+    m/s along +x (constant; cases.with_mesh gives it a mesh).  The
+    reference example's fluid radius is 4e-4 in a wall of radius 4.9e-4 at
+    dx = 3e-5; here the wall sits one spacing outside the outermost column.
+    Host order: wall, pipe layers, back layer, buffer layers by depth, jet
+    layers."""
+    s = abi.default_settings(3)
+    s.particle_step = dx
+    s.H_fac = H_fac
+    s.rho_rest = 997.0
+    s.speed_sound = 300.0
+    s.visc_alpha = 0.05
+    s.mu = 8.94e-4
+    s.sig = 0.0708
+    s.precision = precision
+    s.aero_case = abi.GISSLER
+    s.mu_g = 1.85e-5
+    _gas(s, gas, rho_g)
+    r = diameter / 2.0
+    nr = int(math.ceil(2.0 * r / dx)) + 1
+    g = (np.arange(nr) - (nr - 1) / 2.0) * dx
+    xx, zz = np.meshgrid(g, g, indexing="ij")
+    keep = (xx ** 2 + zz ** 2) <= r * r + 1e-12
+    disk = np.stack([xx[keep], zz[keep]], axis=1)
+    ncol = disk.shape[0]
+
+    def layer(y):
+        return np.stack([disk[:, 0], np.full(ncol, y), disk[:, 1]], axis=1)
+
+    wall = hollow_cylinder_3d(r + dx, (n_pipe + n_buf + 2) * dx, dx, wall_layers)
+    if jet_layers is None:
+        jet_layers = max(0, int(round((n_target - wall.shape[0]) / ncol)) - (n_pipe + 1 + n_buf))
+    pipe = np.concatenate([layer(-k * dx) for k in range(n_pipe - 1, -1, -1)])
+    y_back = -n_pipe * dx
+    back = layer(y_back)
+    buf = np.concatenate([layer(y_back - k * dx) for k in range(1, n_buf + 1)])
+    jet = (np.concatenate([layer(k * dx) for k in range(1, jet_layers + 1)]) if jet_layers
+           else np.zeros((0, 3)))
+    pts = np.concatenate([pipe, back, buf, jet])
+    vel = np.zeros_like(pts)
+    vel[:, 1] = speed
+    c = _finish("jet_in_crossflow_3d", s, wall, pts, vel,
+                f"jet in crossflow D={diameter} dx={dx}: {ncol} columns, {wall.shape[0]} ghost wall particles, "
+                f"{n_pipe} pipe + {n_buf} buffer + {jet_layers} jet layers", bound_solver=abi.GHOST,
+                fluid_type=abi.PIPE)
+    nb = wall.shape[0]
+    npipe = pipe.shape[0]
+    c.b[nb + npipe:nb + npipe + ncol] = abi.BACK
+    c.b[nb + npipe + ncol:nb + npipe + ncol + buf.shape[0]] = abi.BUFFER
+    c.b[nb + npipe + ncol + buf.shape[0]:] = abi.FREE
+    back_ids = np.arange(nb + npipe, nb + npipe + ncol)
+    buffer_ids = np.stack([nb + npipe + ncol + k * ncol + np.arange(ncol) for k in range(n_buf)], axis=1)
+    planes = dict(inlet=1, fixed_vel_or_dynamic=mode,
+                  insert=((0.0, 1.0, 0.0), y_back + back_to_insert * dx),
+                  aero=((0.0, 1.0, 0.0), 0.5 * dx))
+    if delete_above is not None:
+        planes["delete"] = ((0.0, 1.0, 0.0), float(delete_above))
+    c.planes = {1: planes}
+    c.inlets = [(1, back_ids, buffer_ids)]
+    c.jet_top = jet_layers * dx
+    c.n_columns = ncol
+    return c
+
+
 def fluid_block(dim, n_side, dx=1e-3, jitter=0.0, seed=0, precision=abi.PREC_FP64, H_fac=1.3,
                 with_walls=False, layers=3, speed=0.0, aero=abi.NO_AERO, sig=0.0, mu=8.94e-4):
     """Small square/cube of fluid (i fastest, src/shapes/square.cpp:112-138),
@@ -360,6 +605,17 @@ def batched_droplets(n_drops=256, dx=5e-5, seed=99, precision=abi.PREC_MIXED, d_
                    f"{n_drops} droplets, diameters log-U{d_range}, {xf.shape[0]} particles")
     case.owner = np.concatenate(owner)
     return case
+
+
+def drop_of_batch(case, g):
+    """Drop g of a batched_droplets case as a Case of its own (its particles
+    in batch host order, the batch's settings): the standalone simulation
+    the batch's group g must follow."""
+    sel = np.flatnonzero(case.owner == g)
+    sub = _finish(f"drop_{g}", case.settings.copy(), np.zeros((0, case.dim)), case.x[sel], case.v[sel],
+                  f"drop {g} of {case.name}: {sel.size} particles")
+    sub.rho, sub.p, sub.m, sub.b = case.rho[sel].copy(), case.p[sel].copy(), case.m[sel].copy(), case.b[sel].copy()
+    return sub
 
 
 @dataclass
@@ -609,15 +865,22 @@ def quad_mesh(nx, ny, h, vel, p=1.0e5, rho=1.2, lo=(0.0, 0.0)):
                 vert_rho=np.full(nv, float(rho)))
 
 
-def with_mesh(case, n_points=20000, margin=4.0, seed=7, speed=None, kind="delaunay", n_cells=None):
+def with_mesh(case, n_points=20000, margin=4.0, seed=7, speed=None, kind="delaunay", n_cells=None, box=None):
     """Switch a case to the mesh aero source with a mesh around its
-    particles (`margin` particle spacings beyond their bounding box): a
-    seeded Delaunay mesh of n_points random points (kind "delaunay") or a
-    structured tetrahedral mesh of about n_cells cells (kind "kuhn", 3D)."""
+    particles (`margin` particle spacings beyond their bounding box, or the
+    given box (lo, hi)): a seeded Delaunay mesh of n_points random points
+    (kind "delaunay") or a structured tetrahedral mesh of about n_cells
+    cells (kind "kuhn", 3D)."""
     s = case.settings
     dx = s.particle_step
-    lo = case.x.min(axis=0) - margin * dx
-    hi = case.x.max(axis=0) + margin * dx
+    if box is not None:
+        lo = np.asarray(box[0], dtype=np.float64)
+        hi = np.asarray(box[1], dtype=np.float64)
+        if np.any(case.x.min(axis=0) <= lo) or np.any(case.x.max(axis=0) >= hi):
+            raise ValueError("with_mesh: the particles must lie inside the box")
+    else:
+        lo = case.x.min(axis=0) - margin * dx
+        hi = case.x.max(axis=0) + margin * dx
     sp = float(speed if speed is not None else s.v_inf[0])
     rho_g = float(s.rho_g)
     p_inf = float(s.p_ref)

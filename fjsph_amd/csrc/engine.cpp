@@ -141,6 +141,9 @@ struct fjsph_ctx {
     int32_t *lost_list = nullptr, *lost_del = nullptr;
     int64_t lost_cap = 0;
     int64_t lost_deleted = 0, lost_internal = 0, lost_failed = 0;
+    /* particles leaving the pipe: FirstCell's list, deletion marks, counters */
+    int32_t *fresh = nullptr, *fc_del = nullptr;
+    int64_t fresh_cap = 0, first_cell = 0, first_cell_deleted = 0;
     bool tuning = false; /* the tile-block autotuner is timing passes: no deletions */
     /* surrogate */
     int32_t* aero_list = nullptr;   /* segmented list of the aero particles of the last lookup */
@@ -182,6 +185,30 @@ struct fjsph_ctx {
     int rollbacks = 0;
     double maxf = 0, maxAf = 0, maxRho_pc = 0, maxRhoi = 0, maxdrho = 0, minST = 0, maxU = 0, maxShift = 0;
     double last_npd = 1.0;
+
+    /* independent simulations in one context (fjsph_set_groups; 0: one).
+     * Slots are group-major: group g holds slots [gstart_h[g], gstart_h[g+1]).
+     * Each group keeps the integrator state the reference keeps per
+     * simulation (src/Integration.cpp:233-303, src/Newmark_Beta.cpp:10-52). */
+    struct Group {
+        double time = 0.0, last_frame_time = 0.0, dt = 2e-10, cfl = 1.0, safe_dt = 0.0;
+        unsigned n_stable = 0, n_unstable = 0, iteration = 0;
+        int rollbacks = 0;
+        bool active = true;
+        double logbase = 0.0, rms = 0.0, npd = 1.0;
+        double maxf = 0, maxAf = 0, maxRho_pc = 0, maxRhoi = 0, maxdrho = 0, minST = 0, maxU = 0, maxShift = 0;
+    };
+    int32_t ngroups = 0;
+    std::vector<Group> grs;
+    std::vector<int64_t> gstart_h; /* slot ranges, ngroups + 1 */
+    std::vector<int64_t> gblk_h;   /* tile-block ranges of the last build, ngroups + 1 */
+    std::vector<int32_t> gl0_h;    /* first cell layer and layer count, 2 per group */
+    int32_t* grp = nullptr;        /* group of every slot */
+    int64_t *gstart = nullptr, *gblk = nullptr;
+    double *gbox = nullptr, *glo = nullptr, *gdt = nullptr, *gnpd = nullptr, *gscale = nullptr;
+    double *gred = nullptr, *gres = nullptr, *gd2 = nullptr;
+    int32_t *gl0 = nullptr, *gactive = nullptr, *gbuild = nullptr, *lgrp = nullptr;
+    int64_t lgrp_cap = 0, gd2_cap = 0;
 
     /* timers */
     bool timing = false;
@@ -293,6 +320,9 @@ void free_all(fjsph_ctx* ctx)
     ctx->ipt_cap = 0;
     dfree(ctx->lost_list); dfree(ctx->lost_del);
     ctx->lost_cap = 0;
+    dfree(ctx->fresh); dfree(ctx->fc_del);
+    ctx->fresh_cap = 0;
+    ctx->first_cell = ctx->first_cell_deleted = 0;
     ctx->lost_deleted = ctx->lost_internal = ctx->lost_failed = 0;
     for (void* p : ctx->sg_bufs) (void)hipFree(p);
     ctx->sg_bufs.clear();
@@ -314,6 +344,12 @@ void free_all(fjsph_ctx* ctx)
     ctx->blkcount_cap = 0;
     dfree(ctx->c_adv); dfree(ctx->c_back); dfree(ctx->c_last); dfree(ctx->c_rank); dfree(ctx->c_block);
     dfree(ctx->pl_del); dfree(ctx->pl_ins); dfree(ctx->pl_aero); dfree(ctx->pl_mode); dfree(ctx->pl_nbuf);
+    dfree(ctx->grp); dfree(ctx->gstart); dfree(ctx->gblk); dfree(ctx->gbox); dfree(ctx->glo); dfree(ctx->gdt);
+    dfree(ctx->gnpd); dfree(ctx->gscale); dfree(ctx->gred); dfree(ctx->gres); dfree(ctx->gd2); dfree(ctx->gl0);
+    dfree(ctx->gactive); dfree(ctx->gbuild); dfree(ctx->lgrp);
+    ctx->lgrp_cap = ctx->gd2_cap = 0;
+    ctx->ngroups = 0;
+    ctx->grs.clear();
     ctx->inlets.clear();
     ctx->starved.clear();
     ctx->ncols = 0;
@@ -379,6 +415,11 @@ void alloc_slots(fjsph_ctx* ctx, int64_t cap, int64_t keep)
     dgrow(ctx->gather_tmp[1], c * D, 0);
     dgrow(ctx->gather_tmp[2], c * D * D, 0);
     dgrow(ctx->gather_tmp_i, c, 0);
+    if (ctx->grp) dgrow(ctx->grp, c, k);
+    if (ctx->gd2) {
+        dgrow(ctx->gd2, c, 0);
+        ctx->gd2_cap = int64_t(c);
+    }
     dgrow(ctx->cell, c, k);
     dgrow(ctx->order, c, k);
     dgrow(ctx->pos, c, k);
@@ -515,37 +556,123 @@ void permute_all(fjsph_ctx* ctx, const int32_t* perm, int64_t count = -1)
             }
         }
     }
-    for (int32_t** arr : {&ctx->part_id, &ctx->blk, &ctx->cell, &ctx->col, &ctx->depth}) {
+    for (int32_t** arr : {&ctx->part_id, &ctx->blk, &ctx->cell, &ctx->col, &ctx->depth, &ctx->grp}) {
+        if (!*arr) continue;
         launch_gather_i(*arr, ctx->gather_tmp_i, perm, n, ctx->st);
         std::swap(*arr, ctx->gather_tmp_i);
     }
 }
 
+/* copy_state for the slots [first, first + count) only */
+void copy_state_range(fjsph_ctx* ctx, DevState& dst, DevState& src, int64_t first, int64_t count)
+{
+    for (int f = 0; f < kNumFields; ++f) {
+        const int w = width_of(ctx, f);
+        const size_t es = field_desc(f).is_int ? 4 : 8;
+        char* d = (char*)*field_slot(dst, f) + size_t(first) * w * es;
+        const char* s = (const char*)*field_slot(src, f) + size_t(first) * w * es;
+        HIPCHK(hipMemcpyAsync(d, s, size_t(count) * w * es, hipMemcpyDeviceToDevice, ctx->st));
+    }
+}
+
+bool grouped(const fjsph_ctx* ctx) { return ctx->ngroups > 0; }
+
 /* update_neighbours (src/Neighbours.cpp:7-31) on pnp1.  When `reorder` is
  * set the particles are first moved into cell order (a permutation only;
  * the neighbour sets and every result are unchanged). */
+/* Cell layout of independent simulations: each group is binned from its
+ * own lower corner, and the groups are stacked along the last axis in
+ * group order with one empty cell layer between consecutive groups, so
+ * the cell order is group-major, no candidate union crosses a group and
+ * every tile block lies in one group.  Sets dims; returns the cell count. */
+double group_layout(fjsph_ctx* ctx, double cs)
+{
+    const int D = ctx->s.dim, G = ctx->ngroups;
+    launch_group_bbox(D, ctx->pnp1.x, ctx->gstart, G, ctx->gbox, ctx->st);
+    std::vector<double> box(size_t(G) * 6);
+    HIPCHK(hipMemcpyAsync(box.data(), ctx->gbox, sizeof(double) * box.size(), hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    std::vector<double> glo(size_t(G) * 3, 0.0);
+    ctx->gl0_h.assign(size_t(G) * 2, 0);
+    for (int k = 0; k < 3; ++k) ctx->dims[k] = 1;
+    int64_t layer = 0;
+    for (int g = 0; g < G; ++g) {
+        for (int k = 0; k < D; ++k) {
+            const double lo = box[g * 6 + k], hi = box[g * 6 + 3 + k];
+            if (!std::isfinite(lo) || !std::isfinite(hi)) throw std::string("non-finite particle position");
+            glo[g * 3 + k] = lo;
+            const double span = (hi - lo) / cs;
+            if (span > 1e9) throw std::string("a group's particle cloud is too large for the cell grid");
+            const int32_t nk = int32_t(std::floor(span)) + 1;
+            if (k < D - 1) ctx->dims[k] = std::max(ctx->dims[k], nk);
+            else {
+                ctx->gl0_h[g * 2] = int32_t(layer);
+                ctx->gl0_h[g * 2 + 1] = nk;
+                layer += nk + (g + 1 < G ? 1 : 0);
+            }
+        }
+    }
+    if (layer > 0x7fffffff) throw std::string("too many cell layers");
+    ctx->dims[D - 1] = int32_t(layer);
+    if (layer > ctx->lgrp_cap) {
+        ctx->lgrp_cap = layer + layer / 4 + 64;
+        dalloc(ctx->lgrp, size_t(ctx->lgrp_cap));
+    }
+    std::vector<int32_t> lg(size_t(layer), -1);
+    for (int g = 0; g < G; ++g)
+        for (int32_t l = 0; l < ctx->gl0_h[g * 2 + 1]; ++l) lg[size_t(ctx->gl0_h[g * 2] + l)] = g;
+    HIPCHK(hipMemcpyAsync(ctx->glo, glo.data(), sizeof(double) * glo.size(), hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->gl0, ctx->gl0_h.data(), sizeof(int32_t) * ctx->gl0_h.size(), hipMemcpyHostToDevice,
+                          ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->lgrp, lg.data(), sizeof(int32_t) * lg.size(), hipMemcpyHostToDevice, ctx->st));
+    sync(ctx);
+    double ncell_d = 1.0;
+    for (int k = 0; k < 3; ++k) ncell_d *= ctx->dims[k];
+    return ncell_d;
+}
+
+/* Tile-block ranges of the groups for the tile grid of the last build. */
+void group_blocks(fjsph_ctx* ctx)
+{
+    const int D = ctx->s.dim, G = ctx->ngroups;
+    const int64_t per_layer = (D == 3 ? int64_t(ctx->tg.dims[1]) : 1) * ctx->tg.nchunk;
+    ctx->gblk_h.assign(size_t(G) + 1, 0);
+    for (int g = 0; g < G; ++g) ctx->gblk_h[g] = int64_t(ctx->gl0_h[g * 2]) * per_layer;
+    ctx->gblk_h[G] = int64_t(ctx->gl0_h[(G - 1) * 2] + ctx->gl0_h[(G - 1) * 2 + 1]) * per_layer;
+    /* the gap layer after group g belongs to no group: its blocks are empty
+     * and write zero partials, so the ranges may include them */
+    for (int g = 0; g + 1 < G; ++g) ctx->gblk_h[g + 1] = std::max(ctx->gblk_h[g + 1], ctx->gblk_h[g]);
+    HIPCHK(hipMemcpyAsync(ctx->gblk, ctx->gblk_h.data(), sizeof(int64_t) * ctx->gblk_h.size(), hipMemcpyHostToDevice,
+                          ctx->st));
+}
+
 void build_neighbours(fjsph_ctx* ctx, bool reorder)
 {
     PassTimer t(ctx, FJSPH_T_NEIGHBOUR);
     const int D = ctx->s.dim;
     const int64_t n = ctx->n;
-    launch_bbox(D, ctx->pnp1.x, n, ctx->partial, ctx->nparts, ctx->sc->bbox, ctx->st);
-    HIPCHK(hipMemcpyAsync(ctx->sc_host->bbox, ctx->sc->bbox, sizeof(double) * 6, hipMemcpyDeviceToHost, ctx->st));
-    sync(ctx);
-    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    for (int k = 0; k < D; ++k) { lo[k] = ctx->sc_host->bbox[k]; hi[k] = ctx->sc_host->bbox[D + k]; }
-    for (int k = 0; k < D; ++k)
-        if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]))
-            throw std::string("non-finite particle position");
     const double cs = std::sqrt(ctx->d.sr) * (1.0 + 1e-7);
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     double ncell_d = 1.0;
-    for (int k = 0; k < 3; ++k) {
-        ctx->dims[k] = 1;
-        if (k < D) {
-            const double span = (hi[k] - lo[k]) / cs;
-            ctx->dims[k] = int32_t(std::floor(span)) + 1;
+    if (grouped(ctx)) {
+        ncell_d = group_layout(ctx, cs);
+    } else {
+        launch_bbox(D, ctx->pnp1.x, n, ctx->partial, ctx->nparts, ctx->sc->bbox, ctx->st);
+        HIPCHK(hipMemcpyAsync(ctx->sc_host->bbox, ctx->sc->bbox, sizeof(double) * 6, hipMemcpyDeviceToHost,
+                              ctx->st));
+        sync(ctx);
+        for (int k = 0; k < D; ++k) { lo[k] = ctx->sc_host->bbox[k]; hi[k] = ctx->sc_host->bbox[D + k]; }
+        for (int k = 0; k < D; ++k)
+            if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]))
+                throw std::string("non-finite particle position");
+        for (int k = 0; k < 3; ++k) {
+            ctx->dims[k] = 1;
+            if (k < D) {
+                const double span = (hi[k] - lo[k]) / cs;
+                ctx->dims[k] = int32_t(std::floor(span)) + 1;
+            }
+            ncell_d *= ctx->dims[k];
         }
-        ncell_d *= ctx->dims[k];
     }
     const double cell_limit = std::max(4.0 * double(n), 1.0e6) + 64.0;
     if (ncell_d > cell_limit || ncell_d > 2.0e9)
@@ -563,7 +690,8 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
     const double inv_cs = 1.0 / cs;
     HIPCHK(hipMemsetAsync(ctx->cell_count, 0, sizeof(int32_t) * ncell, ctx->st));
     HIPCHK(hipMemsetAsync(ctx->cell_fill, 0, sizeof(int32_t) * ncell, ctx->st));
-    launch_cells(D, ctx->pnp1.x, n, lo, inv_cs, ctx->dims, ctx->cell, ctx->cell_count, ctx->st);
+    launch_cells(D, ctx->pnp1.x, n, lo, inv_cs, ctx->dims, ctx->cell, ctx->cell_count, ctx->st, ctx->grp, ctx->glo,
+                 ctx->gl0);
     launch_scan(ctx->cell_count, ncell, ctx->cell_start, ctx->tile_sums, ctx->st);
     launch_scatter(ctx->cell, n, ctx->cell_start, ctx->cell_fill, ctx->order, ctx->st);
     launch_sort_cells(ctx->cell_start, ncell, ctx->order, ctx->st);
@@ -585,6 +713,9 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         tg.vorder = ctx->vorder;
         tg.pos = ctx->pos;
         tg.cell = ctx->cell;
+        tg.lgrp = grouped(ctx) ? ctx->lgrp : nullptr;
+        tg.gactive = grouped(ctx) ? ctx->gactive : nullptr;
+        tg.gnpd = grouped(ctx) ? ctx->gnpd : nullptr;
         if (tg.nblocks > 0x7fffffff) throw std::string("tile grid too large");
         if (tg.nblocks > ctx->partial_cap) {
             ctx->partial_cap = tg.nblocks + tg.nblocks / 4 + 64;
@@ -626,6 +757,7 @@ void build_neighbours(fjsph_ctx* ctx, bool reorder)
         ctx->nl.cap = cap;
     }
     if (ctx->max_count > ctx->nl.cap) throw std::string("neighbour list capacity could not be established");
+    if (grouped(ctx)) group_blocks(ctx);
     ctx->have_nbrs = true;
     ctx->x_at_build = true;
 }
@@ -747,6 +879,8 @@ void dsph(fjsph_ctx* ctx, bool with_aero)
         refresh_aux(ctx);
         launch_dsph(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
                     with_aero ? 1 : 0, ctx->partial, ctx->sc, ctx->st);
+        /* each group's npd: its blocks' partials over its particle count */
+        if (grouped(ctx)) launch_seg_sum(ctx->partial, ctx->gblk, ctx->ngroups, ctx->gscale, ctx->gnpd, ctx->st);
     }
     if (with_aero && ctx->s.aero_source == FJSPH_MESH_INFL) aero_mesh(ctx);
     if (with_aero && ctx->s.aero_source == FJSPH_SURROGATE) surrogate_lookup(ctx);
@@ -839,8 +973,58 @@ void pipe_outlet(fjsph_ctx* ctx)
 {
     if (!ctx->planes_on) return;
     upload_planes(ctx);
-    launch_pipe_outlet(ctx->s.dim, ctx->pnp1, ctx->part_id, ctx->blk, ctx->dpl, ctx->n, ctx->nb, ctx->st);
-    if (ctx->s.aero_source == FJSPH_MESH_INFL && ctx->have_mesh) mesh_lookup(ctx, false);
+    const bool mesh = ctx->s.aero_source == FJSPH_MESH_INFL && ctx->have_mesh;
+    const bool first = mesh && !ctx->tuning;
+    if (first) {
+        if (ctx->fresh_cap < ctx->n) {
+            const int64_t cap = ctx->n + ctx->n / 8 + 64;
+            dalloc(ctx->fresh, size_t(cap));
+            dalloc(ctx->fc_del, size_t(cap));
+            ctx->fresh_cap = cap;
+        }
+        HIPCHK(hipMemsetAsync(&ctx->sc->fresh_n, 0, 3 * sizeof(int32_t), ctx->st));
+    }
+    launch_pipe_outlet(ctx->s.dim, ctx->pnp1, ctx->part_id, ctx->blk, ctx->dpl, ctx->n, ctx->nb, ctx->st,
+                       first ? ctx->fresh : nullptr, ctx->s.lam_cutoff, ctx->sc);
+    if (!mesh) return;
+    mesh_lookup(ctx, false);
+    if (!first) return;
+    /* FirstCell for the particles leaving the pipe that the lookup placed in
+     * no cell (src/Containment.cpp:466-573) */
+    HIPCHK(hipMemcpyAsync(&ctx->sc_host->fresh_n, &ctx->sc->fresh_n, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->st));
+    sync(ctx);
+    const int32_t nfresh = ctx->sc_host->fresh_n;
+    if (nfresh == 0) return;
+    HIPCHK(hipMemsetAsync(&ctx->sc->lost_err, 0, sizeof(int32_t), ctx->st));
+    launch_first_cell(ctx->s.dim, ctx->pnp1, ctx->dm, ctx->fresh, nfresh, ctx->fc_del, ctx->sc, ctx->st);
+    HIPCHK(hipMemcpyAsync(&ctx->sc_host->lost_err, &ctx->sc->lost_err, 4 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          ctx->st));
+    sync(ctx);
+    const DevScalars& h = *ctx->sc_host;
+    if (h.lost_err & 1)
+        throw std::string("FirstCell: the nearest-centre search of a particle leaving the pipe did not converge");
+    if (h.lost_err & 2)
+        throw std::string("FirstCell: first containing cell for a particle leaving the pipe not found (no boundary "
+                          "crossing; the reference stops here)");
+    ctx->first_cell += h.fc_count;
+    if (h.fc_del == 0) return;
+    /* the reference erases these particles from pnp1 alone and keeps its
+     * neighbour lists; here they leave both states and the lists and
+     * dSPH_PreStep are made again, as after FindCell's deletions */
+    const size_t nbk = ctx->blocks.size();
+    zero_blkcount(ctx);
+    launch_lost_marks(ctx->fresh, ctx->fc_del, nfresh, ctx->part_id, ctx->blk, ctx->n, ctx->keep, ctx->hflag,
+                      ctx->blkcount, ctx->st);
+    std::vector<int32_t> cnt(std::max<size_t>(nbk, 1));
+    HIPCHK(hipMemcpyAsync(cnt.data(), ctx->blkcount, sizeof(int32_t) * cnt.size(), hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    compact_deleted(ctx, cnt, h.fc_del);
+    ctx->first_cell_deleted += h.fc_del;
+    ctx->lost_deleted += h.fc_del;
+    ctx->have_nbrs = false;
+    build_neighbours(ctx, false);
+    dsph(ctx, false);
 }
 
 /* dSPH_PreStep, get_aero_velocity, Detect_Surface, dissipation_terms,
@@ -897,9 +1081,10 @@ void force(fjsph_ctx* ctx, bool update)
                      ctx->st);
     }
     PassTimer t(ctx, FJSPH_T_UPDATE);
+    const DevGroups gr = grouped(ctx) ? DevGroups{ctx->grp, ctx->gactive, ctx->gdt, ctx->gd2} : DevGroups{};
     launch_nb_update(ctx->s.dim, ctx->c, ctx->pnp1, ctx->pn, update ? ctx->x_nxt : nullptr, ctx->v_nxt,
                      ctx->rho_nxt, ctx->p_nxt, ctx->part_id, ctx->blk, ctx->bl, ctx->near_inlet, ctx->nl, ctx->n,
-                     ctx->delta_t, ctx->partial, ctx->st);
+                     ctx->delta_t, ctx->partial, ctx->st, gr);
     if (update && ctx->ncols > 0) {
         /* the inlet buffer of Do_NB_Iter, after the fluid update */
         upload_planes(ctx);
@@ -931,6 +1116,37 @@ double nb_iter(fjsph_ctx* ctx)
     return std::log10(std::sqrt(ctx->sc_host->residual / nf));
 }
 
+/* The time step of Integrator::find_timestep (src/Integration.cpp:370-443)
+ * from the seven reduced maxima/minima r, for one simulation's state st
+ * (the context or a group: maxima, CFL number, safe step). */
+template <typename S>
+double timestep_of(const fjsph_ctx* ctx, S& st, const double* r, double time, double last_frame_time)
+{
+    const fjsph_settings& s = ctx->s;
+    const fjsph_derived& d = ctx->d;
+    st.maxf = r[0];
+    st.maxAf = r[1];
+    st.maxdrho = r[2];
+    st.maxRhoi = r[3];
+    st.maxU = r[4];
+    st.maxShift = r[5];
+    st.minST = r[6];
+    st.maxRho_pc = 100 * st.maxRhoi / s.rho_rest;
+    double mn = 0.25 * std::sqrt(d.H / st.maxf);
+    mn = std::min(mn, 2 * d.H / st.maxU);
+    if (s.mu != 0.0) mn = std::min(mn, 0.125 * d.H_sq * s.rho_rest / s.mu);
+    mn = std::min(mn, 0.067 * st.minST);
+    mn = std::min(mn, 0.5 * std::sqrt(d.H / st.maxdrho));
+    mn = std::min(mn, 1.5 * d.H / s.speed_sound);
+    st.safe_dt = 0.75 * mn;
+    double dt = st.cfl * st.safe_dt;
+    if (dt < s.delta_t_min) dt = s.delta_t_min;
+    else if (dt > s.delta_t_max) dt = s.delta_t_max;
+    if (dt > last_frame_time + s.frame_time_interval - time)
+        dt = last_frame_time + s.frame_time_interval - time + s.delta_t_min;
+    return dt;
+}
+
 /* Integrator::find_timestep (src/Integration.cpp:370-443). */
 double find_timestep(fjsph_ctx* ctx)
 {
@@ -941,30 +1157,30 @@ double find_timestep(fjsph_ctx* ctx)
         HIPCHK(hipMemcpyAsync(ctx->sc_host->red, ctx->sc->red, sizeof(double) * 7, hipMemcpyDeviceToHost, ctx->st));
     }
     sync(ctx);
-    const double* r = ctx->sc_host->red;
-    const fjsph_settings& s = ctx->s;
-    const fjsph_derived& d = ctx->d;
-    ctx->maxf = r[0];
-    ctx->maxAf = r[1];
-    ctx->maxdrho = r[2];
-    ctx->maxRhoi = r[3];
-    ctx->maxU = r[4];
-    ctx->maxShift = r[5];
-    ctx->minST = r[6];
-    ctx->maxRho_pc = 100 * ctx->maxRhoi / s.rho_rest;
-    double mn = 0.25 * std::sqrt(d.H / ctx->maxf);
-    mn = std::min(mn, 2 * d.H / ctx->maxU);
-    if (s.mu != 0.0) mn = std::min(mn, 0.125 * d.H_sq * s.rho_rest / s.mu);
-    mn = std::min(mn, 0.067 * ctx->minST);
-    mn = std::min(mn, 0.5 * std::sqrt(d.H / ctx->maxdrho));
-    mn = std::min(mn, 1.5 * d.H / s.speed_sound);
-    ctx->safe_dt = 0.75 * mn;
-    double dt = ctx->cfl * ctx->safe_dt;
-    if (dt < s.delta_t_min) dt = s.delta_t_min;
-    else if (dt > s.delta_t_max) dt = s.delta_t_max;
-    if (dt > ctx->last_frame_time + s.frame_time_interval - ctx->current_time)
-        dt = ctx->last_frame_time + s.frame_time_interval - ctx->current_time + s.delta_t_min;
-    return dt;
+    return timestep_of(ctx, *ctx, ctx->sc_host->red, ctx->current_time, ctx->last_frame_time);
+}
+
+/* find_timestep of every group: the reduction over each group's particles,
+ * then each group's own formula, CFL number and frame end; the time steps
+ * go to the device for the Newmark update. */
+void find_timestep_groups(fjsph_ctx* ctx)
+{
+    const int G = ctx->ngroups;
+    {
+        PassTimer t(ctx, FJSPH_T_REDUCE);
+        launch_group_dt(ctx->s.dim, ctx->c, ctx->pnp1, ctx->part_id, ctx->gstart, G, ctx->gred, ctx->st);
+    }
+    std::vector<double> r(size_t(G) * 8);
+    HIPCHK(hipMemcpyAsync(r.data(), ctx->gred, sizeof(double) * r.size(), hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    std::vector<double> dts(static_cast<size_t>(G));
+    for (int g = 0; g < G; ++g) {
+        fjsph_ctx::Group& gr = ctx->grs[g];
+        gr.dt = timestep_of(ctx, gr, &r[size_t(g) * 8], gr.time, gr.last_frame_time);
+        dts[g] = gr.dt;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->gdt, dts.data(), sizeof(double) * G, hipMemcpyHostToDevice, ctx->st));
+    sync(ctx);
 }
 
 void rollback(fjsph_ctx* ctx)
@@ -995,6 +1211,193 @@ double newmark_loop(fjsph_ctx* ctx, double& logbase)
         ctx->iteration++;
     }
     return rms;
+}
+
+/* ---- independent simulations (fjsph_set_groups) */
+
+bool reorder_due(fjsph_ctx* ctx);
+
+void upload_active(fjsph_ctx* ctx)
+{
+    std::vector<int32_t> a(ctx->grs.size());
+    for (size_t g = 0; g < a.size(); ++g) a[g] = ctx->grs[g].active ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(ctx->gactive, a.data(), sizeof(int32_t) * a.size(), hipMemcpyHostToDevice, ctx->st));
+    sync(ctx);
+}
+
+void upload_group_dt(fjsph_ctx* ctx)
+{
+    std::vector<double> d(ctx->grs.size());
+    for (size_t g = 0; g < d.size(); ++g) d[g] = ctx->grs[g].dt;
+    HIPCHK(hipMemcpyAsync(ctx->gdt, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice, ctx->st));
+    sync(ctx);
+}
+
+/* Do_NB_Iter + Check_Error's residual for every group still iterating
+ * (the others keep their state): le[g] = log10 rms of group g. */
+void nb_iter_groups(fjsph_ctx* ctx, std::vector<double>& le)
+{
+    boundary_update(ctx);
+    force(ctx, true);
+    std::swap(ctx->pnp1.x, ctx->x_nxt);
+    std::swap(ctx->pnp1.v, ctx->v_nxt);
+    std::swap(ctx->pnp1.rho, ctx->rho_nxt);
+    std::swap(ctx->pnp1.p, ctx->p_nxt);
+    ctx->x_at_build = false;
+    const int G = ctx->ngroups;
+    {
+        PassTimer t(ctx, FJSPH_T_REDUCE);
+        launch_seg_sum(ctx->gd2, ctx->gstart, G, nullptr, ctx->gres, ctx->st);
+    }
+    std::vector<double> res(static_cast<size_t>(G));
+    HIPCHK(hipMemcpyAsync(res.data(), ctx->gres, sizeof(double) * G, hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    le.resize(size_t(G));
+    for (int g = 0; g < G; ++g) {
+        const double nf = double(ctx->gstart_h[g + 1] - ctx->gstart_h[g]);
+        le[g] = std::log10(std::sqrt(res[g] / nf));
+    }
+}
+
+/* The lists of the groups in `mask` built again at their current positions
+ * (after a rollback); the other groups keep their cells, lists and build
+ * positions, as separate simulations would.  Every particle is binned from
+ * its build position, which gives the other groups the cells of their last
+ * build, so their candidate unions are unchanged.  Falls back to a full
+ * build if the rebuilt lists need more room or shorter tile blocks. */
+void rebuild_groups(fjsph_ctx* ctx, const std::vector<int32_t>& mask)
+{
+    const int D = ctx->s.dim;
+    const int64_t n = ctx->n;
+    {
+        PassTimer t(ctx, FJSPH_T_NEIGHBOUR);
+        for (int g = 0; g < ctx->ngroups; ++g)
+            if (mask[g]) {
+                const int64_t a = ctx->gstart_h[g], e = ctx->gstart_h[g + 1];
+                HIPCHK(hipMemcpyAsync(ctx->aux.xb + a * D, ctx->pnp1.x + a * D, sizeof(double) * (e - a) * D,
+                                      hipMemcpyDeviceToDevice, ctx->st));
+            }
+        const double cs = std::sqrt(ctx->d.sr) * (1.0 + 1e-7);
+        const double lo[3] = {0, 0, 0};
+        HIPCHK(hipMemsetAsync(ctx->cell_count, 0, sizeof(int32_t) * ctx->ncell, ctx->st));
+        HIPCHK(hipMemsetAsync(ctx->cell_fill, 0, sizeof(int32_t) * ctx->ncell, ctx->st));
+        launch_cells(D, ctx->aux.xb, n, lo, 1.0 / cs, ctx->dims, ctx->cell, ctx->cell_count, ctx->st, ctx->grp,
+                     ctx->glo, ctx->gl0);
+        launch_scan(ctx->cell_count, ctx->ncell, ctx->cell_start, ctx->tile_sums, ctx->st);
+        launch_scatter(ctx->cell, n, ctx->cell_start, ctx->cell_fill, ctx->order, ctx->st);
+        launch_sort_cells(ctx->cell_start, ctx->ncell, ctx->order, ctx->st);
+        HIPCHK(hipMemcpyAsync(ctx->gbuild, mask.data(), sizeof(int32_t) * mask.size(), hipMemcpyHostToDevice,
+                              ctx->st));
+        HIPCHK(hipMemsetAsync(&ctx->sc->max_count, 0, 2 * sizeof(int32_t), ctx->st));
+        TileGrid tg = ctx->tg;
+        tg.gbuild = ctx->gbuild;
+        launch_build_lists(D, ctx->aux.xb, n, tg, ctx->d.sr, ctx->nl, ctx->sc, ctx->st);
+        HIPCHK(hipMemcpyAsync(&ctx->sc_host->max_count, &ctx->sc->max_count, 2 * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, ctx->st));
+        sync(ctx);
+    }
+    if (ctx->sc_host->overflow || ctx->sc_host->max_count > ctx->nl.cap) {
+        /* every group's lists again, at the current positions */
+        build_neighbours(ctx, false);
+    } else {
+        ctx->max_count = std::max(ctx->max_count, ctx->sc_host->max_count);
+    }
+    ctx->x_at_build = false;
+}
+
+/* Newmark_Beta::Newmark_Beta + Check_Error (src/Newmark_Beta.cpp:10-52,
+ * 303-331) for every group at once: each group iterates until its own
+ * residual converges (then it keeps its state and its blocks are skipped),
+ * rolls back on its own with its own time step halved. */
+void newmark_loop_groups(fjsph_ctx* ctx)
+{
+    const fjsph_settings& s = ctx->s;
+    const int G = ctx->ngroups;
+    for (fjsph_ctx::Group& g : ctx->grs) {
+        g.rms = 0.0;
+        g.active = true;
+    }
+    upload_active(ctx);
+    std::vector<double> le;
+    std::vector<int32_t> roll(size_t(G), 0);
+    for (;;) {
+        bool any = false;
+        for (const fjsph_ctx::Group& g : ctx->grs) any |= g.active;
+        if (!any) break;
+        nb_iter_groups(ctx, le);
+        bool rolled = false;
+        std::fill(roll.begin(), roll.end(), 0);
+        for (int k = 0; k < G; ++k) {
+            fjsph_ctx::Group& g = ctx->grs[k];
+            if (!g.active) continue;
+            if (g.iteration == 0) g.logbase = le[k];
+            g.rms = le[k] - g.logbase;
+            if (g.iteration > unsigned(s.max_subits)) {
+                if (g.rms > 0.0) {
+                    roll[k] = 1;
+                    rolled = true;
+                    g.dt *= 0.5;
+                    g.rollbacks++;
+                    g.iteration = 0;
+                    g.rms = 0.0;
+                    continue;
+                }
+                g.active = false;
+                continue;
+            }
+            g.iteration++;
+            if (!(g.rms > s.min_residual)) g.active = false;
+        }
+        if (rolled) {
+            for (int k = 0; k < G; ++k)
+                if (roll[k])
+                    copy_state_range(ctx, ctx->pnp1, ctx->pn, ctx->gstart_h[k], ctx->gstart_h[k + 1] - ctx->gstart_h[k]);
+            upload_group_dt(ctx);
+            rebuild_groups(ctx, roll);
+        }
+        upload_active(ctx);
+    }
+}
+
+/* integrate_no_update (src/Integration.cpp:27-107) for every group. */
+double integrate_no_update_groups(fjsph_ctx* ctx)
+{
+    const int G = ctx->ngroups;
+    for (fjsph_ctx::Group& g : ctx->grs) {
+        g.iteration = 0;
+        g.rollbacks = 0;
+        g.active = true;
+    }
+    upload_active(ctx);
+    find_timestep_groups(ctx);
+    build_neighbours(ctx, reorder_due(ctx));
+    prepasses(ctx);
+    std::vector<double> le;
+    nb_iter_groups(ctx, le);
+    for (int k = 0; k < G; ++k) {
+        ctx->grs[k].logbase = le[k];
+        ctx->grs[k].iteration++;
+    }
+    build_neighbours(ctx, false);
+    prepasses(ctx);
+    newmark_loop_groups(ctx);
+    std::vector<double> npd(static_cast<size_t>(G));
+    HIPCHK(hipMemcpyAsync(npd.data(), ctx->gnpd, sizeof(double) * G, hipMemcpyDeviceToHost, ctx->st));
+    sync(ctx);
+    double worst = -1e300;
+    unsigned it = 0;
+    int rb = 0;
+    for (int k = 0; k < G; ++k) {
+        fjsph_ctx::Group& g = ctx->grs[k];
+        g.npd = npd[k];
+        worst = std::max(worst, g.rms);
+        it = std::max(it, g.iteration);
+        rb += g.rollbacks;
+    }
+    ctx->iteration = it;
+    ctx->rollbacks = rb;
+    drain_timers(ctx);
+    return worst;
 }
 
 /* ---- Runge-Kutta 4 (src/Runge_Kutta.cpp:10-515, src/Integration.cpp:305-365).
@@ -1098,6 +1501,7 @@ double integrate_no_update_rk(fjsph_ctx* ctx)
 /* integrate_no_update (src/Integration.cpp:27-107). */
 double integrate_no_update(fjsph_ctx* ctx)
 {
+    if (grouped(ctx)) return integrate_no_update_groups(ctx);
     if (ctx->s.integrator == FJSPH_RUNGE_KUTTA) return integrate_no_update_rk(ctx);
     if (ctx->s.integrator != FJSPH_NEWMARK_BETA) throw std::string("unknown integrator");
     ctx->iteration = 0;
@@ -1319,6 +1723,9 @@ void delete_update(fjsph_ctx* ctx)
  * block ranges shift. */
 void compact_deleted(fjsph_ctx* ctx, const std::vector<int32_t>& cnt, int64_t ndel)
 {
+    if (grouped(ctx))
+        throw std::string("independent simulations (groups): a particle deletion would change the groups' slot "
+                          "ranges, which is not supported");
     const int64_t n = ctx->n;
     launch_scan(ctx->keep, n, ctx->kstart, ctx->del_tiles, ctx->st);
     launch_scan(ctx->hflag, n, ctx->hstart, ctx->del_tiles, ctx->st);
@@ -1345,39 +1752,86 @@ void update_data(fjsph_ctx* ctx)
     copy_state(ctx, ctx->pn, ctx->pnp1);
 }
 
+/* The CFL control of integrate (src/Integration.cpp:270-300) for one
+ * simulation's state st (the context or a group). */
+template <typename S>
+void cfl_control(const fjsph_settings& s, S& st, double step_error)
+{
+    if (step_error > s.min_residual || st.maxRho_pc > s.rho_max_iter) {
+        if (step_error > 0.6 * s.min_residual) {
+            st.cfl = std::max(s.cfl_min, st.cfl - s.cfl_step);
+            st.n_unstable = 0;
+        } else if (st.n_unstable > unsigned(s.n_unstable_limit)) {
+            st.cfl = std::max(s.cfl_min, st.cfl - s.cfl_step);
+            st.n_unstable = 0;
+        } else
+            st.n_unstable++;
+    } else
+        st.n_unstable = 0;
+    if (st.iteration < s.subits_factor * s.max_subits && st.n_unstable == 0) {
+        if (st.n_stable > unsigned(s.n_stable_limit)) {
+            st.cfl = std::min(s.cfl_max, st.cfl + s.cfl_step);
+            st.n_stable = 0;
+        } else
+            st.n_stable++;
+    } else
+        st.n_stable = 0;
+}
+
+double integrate_groups(fjsph_ctx* ctx);
+
 /* integrate (src/Integration.cpp:233-303). */
 double integrate(fjsph_ctx* ctx)
 {
+    if (grouped(ctx)) return integrate_groups(ctx);
     const double step_error = integrate_no_update(ctx);
     update_data(ctx);
     ctx->current_time += ctx->delta_t;
-    const fjsph_settings& s = ctx->s;
-    if (step_error > s.min_residual || ctx->maxRho_pc > s.rho_max_iter) {
-        if (step_error > 0.6 * s.min_residual) {
-            ctx->cfl = std::max(s.cfl_min, ctx->cfl - s.cfl_step);
-            ctx->n_unstable = 0;
-        } else if (ctx->n_unstable > unsigned(s.n_unstable_limit)) {
-            ctx->cfl = std::max(s.cfl_min, ctx->cfl - s.cfl_step);
-            ctx->n_unstable = 0;
-        } else
-            ctx->n_unstable++;
-    } else
-        ctx->n_unstable = 0;
-    if (ctx->iteration < s.subits_factor * s.max_subits && ctx->n_unstable == 0) {
-        if (ctx->n_stable > unsigned(s.n_stable_limit)) {
-            ctx->cfl = std::min(s.cfl_max, ctx->cfl + s.cfl_step);
-            ctx->n_stable = 0;
-        } else
-            ctx->n_stable++;
-    } else
-        ctx->n_stable = 0;
+    cfl_control(ctx->s, *ctx, step_error);
     sync(ctx);
     return step_error;
+}
+
+/* integrate for independent simulations: every group takes one step with
+ * its own time step, CFL control and clock. */
+double integrate_groups(fjsph_ctx* ctx)
+{
+    const double worst = integrate_no_update(ctx);
+    update_data(ctx);
+    for (fjsph_ctx::Group& g : ctx->grs) {
+        g.time += g.dt;
+        cfl_control(ctx->s, g, g.rms);
+    }
+    sync(ctx);
+    return worst;
 }
 
 void fill_info(fjsph_ctx* ctx, double rms, fjsph_step_info* info)
 {
     if (!info) return;
+    if (grouped(ctx)) {
+        /* the batch: the largest residual, iteration count, density
+         * variation and accelerations, the smallest time step, CFL number
+         * and clock, the sum of the rollbacks, the mean npd (each group's
+         * own values: fjsph_get_group_info) */
+        *info = fjsph_step_info{};
+        info->rms_error = rms;
+        info->dt = info->cfl = info->time = 1e300;
+        double npd = 0.0;
+        for (const fjsph_ctx::Group& g : ctx->grs) {
+            info->dt = std::min(info->dt, g.dt);
+            info->cfl = std::min(info->cfl, g.cfl);
+            info->time = std::min(info->time, g.time);
+            info->max_rho_pc = std::max(info->max_rho_pc, g.maxRho_pc);
+            info->maxf = std::max(info->maxf, g.maxf);
+            info->maxAf = std::max(info->maxAf, g.maxAf);
+            info->iterations = std::max(info->iterations, int32_t(g.iteration));
+            info->rollbacks += g.rollbacks;
+            npd += g.npd;
+        }
+        info->npd = npd / double(ctx->grs.size());
+        return;
+    }
     info->rms_error = rms;
     info->dt = ctx->delta_t;
     info->cfl = ctx->cfl;
@@ -1439,7 +1893,9 @@ fjsph_status fjsph_create(const fjsph_settings* s, int32_t device, fjsph_ctx** o
     fjsph_derived d;
     if (fjsph_derive(s, &d) != FJSPH_OK) return FJSPH_ERR_ARG;
     if (s->precision != FJSPH_PREC_FP64 && s->precision != FJSPH_PREC_MIXED) return FJSPH_ERR_ARG;
-    if (s->aero_case != FJSPH_NO_AERO && s->aero_case != FJSPH_GISSLER) return FJSPH_ERR_UNSUPPORTED;
+    if (s->aero_case != FJSPH_NO_AERO && s->aero_case != FJSPH_GISSLER && s->aero_case != FJSPH_INDUCED_PRESSURE &&
+        s->aero_case != FJSPH_SKIN_FRIC)
+        return FJSPH_ERR_ARG;
     if (s->aero_source != FJSPH_CONST_VEL && s->aero_source != FJSPH_MESH_INFL && s->aero_source != FJSPH_SURROGATE)
         return FJSPH_ERR_UNSUPPORTED;
     int ndev = 0;
@@ -1617,6 +2073,10 @@ fjsph_status fjsph_set_block_planes(fjsph_ctx* ctx, int32_t block, const fjsph_b
 {
     if (!ctx || !planes || block < 0 || block >= int32_t(ctx->blocks.size())) return FJSPH_ERR_ARG;
     if (planes->fixed_vel_or_dynamic < 0 || planes->fixed_vel_or_dynamic > 1) return FJSPH_ERR_ARG;
+    if (grouped(ctx)) {
+        ctx->err = "set_block_planes: not supported with independent simulations (groups)";
+        return FJSPH_ERR_STATE;
+    }
     if (ctx->planes.size() < ctx->blocks.size()) ctx->planes.resize(ctx->blocks.size(), default_planes());
     ctx->planes[size_t(block)] = *planes;
     ctx->planes_on = true;
@@ -1631,6 +2091,7 @@ fjsph_status fjsph_set_inlet(fjsph_ctx* ctx, int32_t block, int32_t n_cols, int3
         return FJSPH_ERR_ARG;
     return guarded(ctx, [&] {
         require(ctx, false);
+        if (grouped(ctx)) throw std::string("set_inlet: not supported with independent simulations (groups)");
         const fjsph_block& bk = ctx->blocks[size_t(block)].b;
         if (!bk.is_fluid) throw std::string("set_inlet: not a fluid block");
         for (const auto& in : ctx->inlets)
@@ -1853,6 +2314,116 @@ fjsph_status fjsph_ipt_clear(fjsph_ctx* ctx)
     return FJSPH_OK;
 }
 
+fjsph_status fjsph_set_groups(fjsph_ctx* ctx, int32_t n_groups, const int32_t* group)
+{
+    if (!ctx || n_groups < 0 || (n_groups > 0 && !group)) return FJSPH_ERR_ARG;
+    return guarded(ctx, [&] {
+        require(ctx, false);
+        if (n_groups == 0) {
+            ctx->ngroups = 0;
+            ctx->grs.clear();
+            dfree(ctx->grp);
+            dfree(ctx->gd2);
+            ctx->have_nbrs = false;
+            return;
+        }
+        if (ctx->nb != 0)
+            throw std::string("set_groups: boundary particles are not supported (each group is a body of fluid)");
+        if (ctx->ncols > 0 || ctx->planes_on)
+            throw std::string("set_groups: inlets and deletion planes are not supported with groups");
+        if (ctx->s.integrator != FJSPH_NEWMARK_BETA)
+            throw std::string("set_groups: independent simulations use the Newmark-Beta integrator");
+        const int64_t n = ctx->n;
+        const std::vector<int32_t> hostof = slot_to_host(ctx);
+        std::vector<int32_t> gs(static_cast<size_t>(n));
+        std::vector<int64_t> cnt(size_t(n_groups), 0);
+        for (int64_t i = 0; i < n; ++i) {
+            const int32_t g = group[hostof[i]];
+            if (g < 0 || g >= n_groups) throw std::string("set_groups: group index out of range");
+            gs[i] = g;
+            cnt[g]++;
+        }
+        for (int32_t g = 0; g < n_groups; ++g)
+            if (cnt[g] == 0) throw std::string("set_groups: every group needs at least one particle");
+        /* group-major slots: a stable order by group */
+        std::vector<int32_t> perm(static_cast<size_t>(n));
+        for (int64_t i = 0; i < n; ++i) perm[i] = int32_t(i);
+        std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return gs[a] < gs[b]; });
+        bool identity = true;
+        for (int64_t i = 0; i < n && identity; ++i) identity = perm[i] == i;
+        if (!identity) {
+            HIPCHK(hipMemcpy(ctx->perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+            permute_all(ctx, ctx->perm);
+            std::vector<int32_t> sorted(static_cast<size_t>(n));
+            for (int64_t i = 0; i < n; ++i) sorted[i] = gs[perm[i]];
+            gs.swap(sorted);
+        }
+        const size_t G = size_t(n_groups);
+        dalloc(ctx->grp, size_t(ctx->cap));
+        dalloc(ctx->gd2, size_t(ctx->cap));
+        ctx->gd2_cap = ctx->cap;
+        dalloc(ctx->gstart, G + 1);
+        dalloc(ctx->gblk, G + 1);
+        dalloc(ctx->gbox, G * 6);
+        dalloc(ctx->glo, G * 3);
+        dalloc(ctx->gl0, G * 2);
+        dalloc(ctx->gdt, G);
+        dalloc(ctx->gnpd, G);
+        dalloc(ctx->gscale, G);
+        dalloc(ctx->gred, G * 8);
+        dalloc(ctx->gres, G);
+        dalloc(ctx->gactive, G);
+        dalloc(ctx->gbuild, G);
+        ctx->gstart_h.assign(G + 1, 0);
+        std::vector<double> scale(G);
+        for (size_t g = 0; g < G; ++g) {
+            ctx->gstart_h[g + 1] = ctx->gstart_h[g] + cnt[g];
+            scale[g] = 1.0 / double(cnt[g]);
+        }
+        sync(ctx);
+        HIPCHK(hipMemcpy(ctx->grp, gs.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->gstart, ctx->gstart_h.data(), sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->gscale, scale.data(), sizeof(double) * G, hipMemcpyHostToDevice));
+        fjsph_ctx::Group g0;
+        g0.time = ctx->current_time;
+        g0.last_frame_time = ctx->last_frame_time;
+        g0.dt = ctx->delta_t;
+        g0.cfl = ctx->cfl;
+        g0.n_stable = ctx->n_stable;
+        g0.n_unstable = ctx->n_unstable;
+        ctx->grs.assign(G, g0);
+        ctx->ngroups = n_groups;
+        upload_active(ctx);
+        upload_group_dt(ctx);
+        ctx->have_nbrs = false;
+    });
+}
+
+fjsph_status fjsph_get_group_info(fjsph_ctx* ctx, int32_t first, int32_t count, fjsph_step_info* out)
+{
+    if (!ctx || first < 0 || count < 0 || (count > 0 && !out)) return FJSPH_ERR_ARG;
+    if (int64_t(first) + count > ctx->ngroups) {
+        ctx->err = "get_group_info: group range out of bounds";
+        return FJSPH_ERR_ARG;
+    }
+    for (int32_t k = 0; k < count; ++k) {
+        const fjsph_ctx::Group& g = ctx->grs[size_t(first + k)];
+        fjsph_step_info& o = out[k];
+        o = fjsph_step_info{};
+        o.rms_error = g.rms;
+        o.dt = g.dt;
+        o.cfl = g.cfl;
+        o.time = g.time;
+        o.max_rho_pc = g.maxRho_pc;
+        o.maxf = g.maxf;
+        o.maxAf = g.maxAf;
+        o.npd = g.npd;
+        o.iterations = int32_t(g.iteration);
+        o.rollbacks = g.rollbacks;
+    }
+    return FJSPH_OK;
+}
+
 fjsph_status fjsph_set_block_targets(fjsph_ctx* ctx, int32_t targets)
 {
     if (!ctx || targets < 0 || targets > 65536) return FJSPH_ERR_ARG;
@@ -1894,8 +2465,8 @@ fjsph_status fjsph_autotune_blocks(fjsph_ctx* ctx, int32_t n_cand, const int32_t
                 for (int f = 0; f < kNumFields; ++f)
                     v.emplace_back(*field_slot(*st, f),
                                    size_t(n) * size_t(width_of(ctx, f)) * (field_desc(f).is_int ? 4 : 8));
-            for (int32_t* a : {ctx->part_id, ctx->blk, ctx->cell, ctx->col, ctx->depth})
-                v.emplace_back(a, size_t(n) * 4);
+            for (int32_t* a : {ctx->part_id, ctx->blk, ctx->cell, ctx->col, ctx->depth, ctx->grp})
+                if (a) v.emplace_back(a, size_t(n) * 4);
             return v;
         };
         std::vector<std::pair<void*, size_t>> arrays = collect();
@@ -1920,6 +2491,10 @@ fjsph_status fjsph_autotune_blocks(fjsph_ctx* ctx, int32_t n_cand, const int32_t
         /* one step's pair work: two list builds with their pre-passes, and
          * the force pass once per Newmark iteration of the last step */
         const int iters = ctx->iteration > 0 ? int(ctx->iteration) : 10;
+        if (grouped(ctx)) {
+            for (fjsph_ctx::Group& g : ctx->grs) g.active = true;
+            upload_active(ctx);
+        }
         auto step_work = [&] {
             build_neighbours(ctx, true);
             prepasses(ctx);
@@ -2712,6 +3287,8 @@ fjsph_status fjsph_mesh_lost_info(fjsph_ctx* ctx, fjsph_lost_info* info)
         info->deleted = ctx->lost_deleted;
         info->internal = ctx->lost_internal;
         info->failed = ctx->lost_failed;
+        info->first_cell = ctx->first_cell;
+        info->first_cell_deleted = ctx->first_cell_deleted;
     });
 }
 
@@ -2763,6 +3340,9 @@ fjsph_status fjsph_boundary_update(fjsph_ctx* ctx)
 fjsph_status fjsph_acc_rrho(fjsph_ctx* ctx, double npd)
 {
     return guarded(ctx, [&] {
+        if (grouped(ctx))
+            throw std::string("this per-pass entry point acts on one simulation; with groups, step them with "
+                              "fjsph_integrate_step");
         require(ctx, true);
         HIPCHK(hipMemcpyAsync(&ctx->sc->npd, &npd, sizeof(double), hipMemcpyHostToDevice, ctx->st));
         force(ctx, false);
@@ -2774,6 +3354,9 @@ fjsph_status fjsph_acc_rrho(fjsph_ctx* ctx, double npd)
 fjsph_status fjsph_nb_iter(fjsph_ctx* ctx, double npd, double* log_error)
 {
     return guarded(ctx, [&] {
+        if (grouped(ctx))
+            throw std::string("this per-pass entry point acts on one simulation; with groups, step them with "
+                              "fjsph_integrate_step");
         require(ctx, true);
         HIPCHK(hipMemcpyAsync(&ctx->sc->npd, &npd, sizeof(double), hipMemcpyHostToDevice, ctx->st));
         const double le = nb_iter(ctx);
@@ -2785,6 +3368,9 @@ fjsph_status fjsph_nb_iter(fjsph_ctx* ctx, double npd, double* log_error)
 fjsph_status fjsph_find_timestep(fjsph_ctx* ctx, double* dt)
 {
     return guarded(ctx, [&] {
+        if (grouped(ctx))
+            throw std::string("this per-pass entry point acts on one simulation; with groups, step them with "
+                              "fjsph_integrate_step");
         require(ctx, false);
         const double v = find_timestep(ctx);
         if (dt) *dt = v;
@@ -2816,6 +3402,7 @@ fjsph_status fjsph_advance_frame(fjsph_ctx* ctx)
     if (!ctx) return FJSPH_ERR_ARG;
     ctx->last_frame_time += ctx->s.frame_time_interval;
     ctx->current_frame++;
+    for (fjsph_ctx::Group& g : ctx->grs) g.last_frame_time += ctx->s.frame_time_interval;
     return FJSPH_OK;
 }
 
@@ -2834,6 +3421,11 @@ fjsph_status fjsph_set_time(fjsph_ctx* ctx, double time, double delta_t, double 
     ctx->current_time = time;
     ctx->delta_t = delta_t;
     ctx->cfl = cfl;
+    for (fjsph_ctx::Group& g : ctx->grs) {
+        g.time = time;
+        g.dt = delta_t;
+        g.cfl = cfl;
+    }
     return FJSPH_OK;
 }
 

@@ -83,6 +83,25 @@ struct TileGrid {
     const int32_t* cell;  /* cell of every slot */
     int32_t* vorder;      /* target order of the lists (order[] at the build) */
     int32_t* pos;         /* position of every slot in vorder[] */
+    /* independent simulations (null otherwise): the group of every cell
+     * layer along the last axis (-1 between groups), whether each group is
+     * still iterating, and each group's numerical particle density */
+    const int32_t* lgrp;
+    const int32_t* gactive;
+    const double* gnpd;
+    const int32_t* gbuild; /* list build: only the blocks of these groups (null: all) */
+};
+
+/* Independent simulations in one context, for the per-slot Newmark update:
+ * the group of every slot, whether the group is still iterating (a group
+ * that is not keeps its state), each group's time step, and a per-slot
+ * array that takes the squared position change for the per-group
+ * residuals.  grp == nullptr: one simulation. */
+struct DevGroups {
+    const int32_t* grp;
+    const int32_t* active;
+    const double* dt;
+    double* d2;
 };
 
 /* Per-particle quantities derived from the current rho, p, m, b (and vPert),
@@ -206,14 +225,26 @@ struct DevScalars {
     int32_t lost_del;  /* of those, deleted by the boundary test */
     int32_t lost_int;  /* of those, found across a wall face */
     int32_t lost_fail; /* of those, with no net boundary crossing */
-    int32_t lost_err;  /* nearest-centre search failed (too many equal distances) */
+    int32_t lost_err;  /* 1: a nearest-centre search failed (too many equal distances);
+                          2: FirstCell found no boundary crossing */
+    int32_t fresh_n;   /* particles leaving the pipe listed for FirstCell */
+    int32_t fc_count;  /* of those, in no cell (FirstCell's boundary branch) */
+    int32_t fc_del;    /* of those, across a far-field face */
 };
 
 /* ---- launch wrappers (kernels.hip); every one enqueues on `st` ---------- */
 
 void launch_bbox(int dim, const double* x, int64_t n, double* partial, int nparts, double* out, hipStream_t st);
 void launch_cells(int dim, const double* x, int64_t n, const double* lo, double inv_cs, const int32_t* dims,
-                  int32_t* cell, int32_t* cell_count, hipStream_t st);
+                  int32_t* cell, int32_t* cell_count, hipStream_t st, const int32_t* grp = nullptr,
+                  const double* glo = nullptr, const int32_t* gl0 = nullptr);
+/* Independent simulations: per-group bounding boxes (6 per group), sums
+ * over segments, and find_timestep's reduction per group (8 per group). */
+void launch_group_bbox(int dim, const double* x, const int64_t* gstart, int32_t ngroups, double* out, hipStream_t st);
+void launch_seg_sum(const double* vals, const int64_t* seg, int32_t nseg, const double* scale, double* out,
+                    hipStream_t st);
+void launch_group_dt(int dim, const Consts& c, DevState s, const int32_t* part_id, const int64_t* gstart,
+                     int32_t ngroups, double* out, hipStream_t st);
 void launch_scan(const int32_t* count, int64_t ncell, int32_t* start, int32_t* block_sums, hipStream_t st);
 void launch_scatter(const int32_t* cell, int64_t n, const int32_t* start, int32_t* fill, int32_t* order,
                     hipStream_t st);
@@ -249,7 +280,8 @@ void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
 /* next_x == nullptr: only the acceleration tail, no Newmark update. */
 void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double* next_x, double* next_v,
                       double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk, DevBlocks bl,
-                      const int32_t* near_inlet, NbrList nl, int64_t n, double dt, double* partial, hipStream_t st);
+                      const int32_t* near_inlet, NbrList nl, int64_t n, double dt, double* partial, hipStream_t st,
+                      DevGroups gr = DevGroups{});
 /* Surrogate on the aero particles of s (mode 0) or on points (mode 1). */
 void launch_surrogate(int dim, const Consts& c, DevState s, const int32_t* part_id, Surrogate sg, int64_t n, int mode,
                       const double* pts, double* out, hipStream_t st);
@@ -258,7 +290,13 @@ void launch_aero_list(int dim, const Consts& c, DevState s, const int32_t* part_
 void launch_surrogate_list(int dim, DevState s, Surrogate sg, int64_t n, const int32_t* list, const int32_t* count,
                            hipStream_t st);
 void launch_pipe_outlet(int dim, DevState s, const int32_t* part_id, const int32_t* blk, DevPlanes pl, int64_t n,
-                        int64_t n_bound, hipStream_t st);
+                        int64_t n_bound, hipStream_t st, int32_t* fresh = nullptr, double lam_cutoff = 0.0,
+                        DevScalars* sc = nullptr);
+/* FirstCell (src/Containment.cpp:423-573) for the particles leaving the pipe
+ * that the lookup placed in no cell: the nearest cells again, then the
+ * boundary branch (reflection at walls, deletion mark at the far field). */
+void launch_first_cell(int dim, DevState s, DevMesh m, const int32_t* fresh, int32_t nfresh, int32_t* fc_del,
+                       DevScalars* sc, hipStream_t st);
 void launch_inlet_backslot(const int32_t* col, const int32_t* depth, int64_t n, int32_t* back_slot, hipStream_t st);
 void launch_rk_buffer(int dim, const Consts& c, DevState s, DevState pn, DevState s1, DevState s2, DevState s3,
                       const int32_t* col, const int32_t* depth, const int32_t* blk, const int32_t* back_slot,

@@ -36,13 +36,18 @@ __device__ __forceinline__ double dot_plane(const double* __restrict__ x, int64_
 template <int D>
 __global__ __launch_bounds__(TPB) void k_pipe_outlet(DevState s, const int32_t* __restrict__ part_id,
                                                      const int32_t* __restrict__ blk, DevPlanes pl, int64_t n,
-                                                     int64_t n_bound)
+                                                     int64_t n_bound, int32_t* __restrict__ fresh, double lam_cutoff,
+                                                     DevScalars* sc)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n || part_id[i] < n_bound || s.b[i] != FJSPH_PIPE) return;
     const int k = blk[i];
     if (k < 0) return;
-    if (dot_plane<D>(s.x, i, pl.aero + 4 * k) > pl.aero[4 * k + 3]) s.b[i] = FJSPH_FREE;
+    if (dot_plane<D>(s.x, i, pl.aero + 4 * k) > pl.aero[4 * k + 3]) {
+        s.b[i] = FJSPH_FREE;
+        /* with the mesh source, FirstCell for the new aero particles */
+        if (fresh && s.lam_nb[i] < lam_cutoff) fresh[atomicAdd(&sc->fresh_n, 1)] = int32_t(i);
+    }
 }
 
 /* Slot of every column's back particle (depth 0). */
@@ -295,11 +300,13 @@ __global__ __launch_bounds__(TPB) void k_compact_ids(int32_t* __restrict__ part_
 /* ------------------------------------------------------------ launchers */
 
 void launch_pipe_outlet(int dim, DevState s, const int32_t* part_id, const int32_t* blk, DevPlanes pl, int64_t n,
-                        int64_t n_bound, hipStream_t st)
+                        int64_t n_bound, hipStream_t st, int32_t* fresh, double lam_cutoff, DevScalars* sc)
 {
     const dim3 g(grid_of(n)), b(TPB);
-    if (dim == 2) hipLaunchKernelGGL(k_pipe_outlet<2>, g, b, 0, st, s, part_id, blk, pl, n, n_bound);
-    else hipLaunchKernelGGL(k_pipe_outlet<3>, g, b, 0, st, s, part_id, blk, pl, n, n_bound);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_pipe_outlet<2>, g, b, 0, st, s, part_id, blk, pl, n, n_bound, fresh, lam_cutoff, sc);
+    else
+        hipLaunchKernelGGL(k_pipe_outlet<3>, g, b, 0, st, s, part_id, blk, pl, n, n_bound, fresh, lam_cutoff, sc);
 }
 
 void launch_inlet_backslot(const int32_t* col, const int32_t* depth, int64_t n, int32_t* back_slot, hipStream_t st)

@@ -70,6 +70,13 @@ struct Grid3 {
     double lo[3];
     double inv_cs;
     int32_t dims[3];
+    /* independent simulations (grp != nullptr): each group is binned from
+     * its own lower corner glo[3 g..] into its own cell layers along the
+     * last axis, gl0[2 g] (first) and gl0[2 g + 1] (count), so the groups
+     * sit in disjoint layers in group order */
+    const int32_t* grp;
+    const double* glo;
+    const int32_t* gl0;
 };
 
 /* Runs of equal consecutive values in a wave (every lane takes part):
@@ -97,11 +104,24 @@ __global__ __launch_bounds__(TPB) void k_cells(const double* __restrict__ x, int
     int32_t lin = -1 - lane; /* lanes past n: distinct values, runs of their own */
     if (valid) {
         int32_t ci[3] = {0, 0, 0};
+        if (g.grp) {
+            /* a particle outside its group's box (after a rollback) is
+             * clamped into the group's own cells: clamping keeps
+             * neighbouring particles in neighbouring cells */
+            const int32_t gi = g.grp[i];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            double f = floor((x[i * D + k] - g.lo[k]) * g.inv_cs);
-            int32_t c = (int32_t)fmin(fmax(f, 0.0), double(g.dims[k] - 1));
-            ci[k] = c;
+            for (int k = 0; k < D; ++k) {
+                const double f = floor((x[i * D + k] - g.glo[gi * 3 + k]) * g.inv_cs);
+                const double top = (k == D - 1) ? double(g.gl0[2 * gi + 1] - 1) : double(g.dims[k] - 1);
+                ci[k] = (int32_t)fmin(fmax(f, 0.0), top) + (k == D - 1 ? g.gl0[2 * gi] : 0);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                double f = floor((x[i * D + k] - g.lo[k]) * g.inv_cs);
+                int32_t c = (int32_t)fmin(fmax(f, 0.0), double(g.dims[k] - 1));
+                ci[k] = c;
+            }
         }
         lin = (D == 3) ? ((ci[2] * g.dims[1] + ci[1]) * g.dims[0] + ci[0]) : (ci[1] * g.dims[0] + ci[0]);
         cell[i] = lin;
@@ -112,11 +132,15 @@ __global__ __launch_bounds__(TPB) void k_cells(const double* __restrict__ x, int
 }
 
 void launch_cells(int dim, const double* x, int64_t n, const double* lo, double inv_cs, const int32_t* dims,
-                  int32_t* cell, int32_t* cell_count, hipStream_t st)
+                  int32_t* cell, int32_t* cell_count, hipStream_t st, const int32_t* grp, const double* glo,
+                  const int32_t* gl0)
 {
     Grid3 g;
     for (int k = 0; k < 3; ++k) { g.lo[k] = k < dim ? lo[k] : 0.0; g.dims[k] = dims[k]; }
     g.inv_cs = inv_cs;
+    g.grp = grp;
+    g.glo = glo;
+    g.gl0 = gl0;
     if (dim == 2) hipLaunchKernelGGL(k_cells<2>, dim3(grid_of(n)), dim3(TPB), 0, st, x, n, g, cell, cell_count);
     else hipLaunchKernelGGL(k_cells<3>, dim3(grid_of(n)), dim3(TPB), 0, st, x, n, g, cell, cell_count);
 }
@@ -373,6 +397,24 @@ void launch_sum_partials_scaled(const double* partial, int nparts, int width, do
  * and the surface-tension limit sqrt(rho dx^2 / (2 pi sig |curve|)) (min,
  * skipped where the denominator is zero). */
 template <int D>
+__device__ __forceinline__ void dt_terms(const Consts& c, const DevState& s, int64_t i, double* v)
+{
+    double t[D];
+    load_vec<D>(s.acc, i, t);
+    v[0] = sqrt(dotd<D>(t, t));
+    load_vec<D>(s.Af, i, t);
+    v[1] = sqrt(dotd<D>(t, t));
+    v[2] = fabs(s.Rrho[i]);
+    v[3] = fabs(s.rho[i] - c.rho0);
+    load_vec<D>(s.v, i, t);
+    v[4] = sqrt(dotd<D>(t, t));
+    load_vec<D>(s.vPert, i, t);
+    v[5] = sqrt(dotd<D>(t, t));
+    const double den = 2.0 * kPi * c.sig * fabs(s.curve[i]);
+    if (den != 0.0) v[6] = sqrt(s.rho[i] * c.dx * c.dx / den);
+}
+
+template <int D>
 __global__ __launch_bounds__(TPB) void k_dt_reduce(Consts c, DevState s, const int32_t* __restrict__ part_id,
                                                    int64_t n, double* partial)
 {
@@ -380,21 +422,7 @@ __global__ __launch_bounds__(TPB) void k_dt_reduce(Consts c, DevState s, const i
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     const double eps = 2.220446049250313e-16;
     double v[7] = {eps, eps, eps, eps, eps, eps, 9999999.0};
-    if (i < n && part_id[i] >= c.n_bound) {
-        double t[D];
-        load_vec<D>(s.acc, i, t);
-        v[0] = sqrt(dotd<D>(t, t));
-        load_vec<D>(s.Af, i, t);
-        v[1] = sqrt(dotd<D>(t, t));
-        v[2] = fabs(s.Rrho[i]);
-        v[3] = fabs(s.rho[i] - c.rho0);
-        load_vec<D>(s.v, i, t);
-        v[4] = sqrt(dotd<D>(t, t));
-        load_vec<D>(s.vPert, i, t);
-        v[5] = sqrt(dotd<D>(t, t));
-        const double den = 2.0 * kPi * c.sig * fabs(s.curve[i]);
-        if (den != 0.0) v[6] = sqrt(s.rho[i] * c.dx * c.dx / den);
-    }
+    if (i < n && part_id[i] >= c.n_bound) dt_terms<D>(c, s, i, v);
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         const double r = block_reduce(v[k], OpMax(), lds);
@@ -437,6 +465,92 @@ __global__ __launch_bounds__(TPB) void k_fix_rho(DevState s, const int32_t* __re
 void launch_fix_rho(DevState s, const int32_t* part_id, int64_t nb, int64_t n, double rho0, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fix_rho, dim3(grid_of(n)), dim3(TPB), 0, st, s, part_id, nb, n, rho0);
+}
+
+/* ------------------------------------------ independent simulations */
+
+/* Bounding box of each group's slots [gstart[g], gstart[g+1]) (slots are
+ * group-major): one block per group, lo then hi per axis in out[6 g..]. */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_group_bbox(const double* __restrict__ x, const int64_t* __restrict__ gstart,
+                                                    double* out)
+{
+    __shared__ double lds[NWAVE];
+    const int64_t g = blockIdx.x, a = gstart[g], e = gstart[g + 1];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double lo = 1e300, hi = -1e300;
+        for (int64_t i = a + threadIdx.x; i < e; i += TPB) {
+            const double v = x[i * D + k];
+            lo = fmin(lo, v);
+            hi = fmax(hi, v);
+        }
+        lo = block_reduce(lo, OpMin(), lds);
+        hi = block_reduce(hi, OpMax(), lds);
+        if (threadIdx.x == 0) {
+            out[g * 6 + k] = lo;
+            out[g * 6 + 3 + k] = hi;
+        }
+    }
+}
+
+/* find_timestep's reduction per group (slots [gstart[g], gstart[g+1]),
+ * fluid only): out[8 g + k] as k_dt_final's out[k]. */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_group_dt(Consts c, DevState s, const int32_t* __restrict__ part_id,
+                                                  const int64_t* __restrict__ gstart, double* out)
+{
+    __shared__ double lds[NWAVE];
+    const int64_t g = blockIdx.x, a = gstart[g], e = gstart[g + 1];
+    const double eps = 2.220446049250313e-16;
+    double r[7] = {eps, eps, eps, eps, eps, eps, 9999999.0};
+    for (int64_t i = a + threadIdx.x; i < e; i += TPB) {
+        if (part_id[i] < c.n_bound) continue;
+        double v[7] = {eps, eps, eps, eps, eps, eps, 9999999.0};
+        dt_terms<D>(c, s, i, v);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) r[k] = fmax(r[k], v[k]);
+        r[6] = fmin(r[6], v[6]);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double m = block_reduce(r[k], OpMax(), lds);
+        if (threadIdx.x == 0) out[g * 8 + k] = m;
+    }
+    const double m = block_reduce(r[6], OpMin(), lds);
+    if (threadIdx.x == 0) out[g * 8 + 6] = m;
+}
+
+void launch_group_dt(int dim, const Consts& c, DevState s, const int32_t* part_id, const int64_t* gstart,
+                     int32_t ngroups, double* out, hipStream_t st)
+{
+    if (dim == 2) hipLaunchKernelGGL(k_group_dt<2>, dim3(ngroups), dim3(TPB), 0, st, c, s, part_id, gstart, out);
+    else hipLaunchKernelGGL(k_group_dt<3>, dim3(ngroups), dim3(TPB), 0, st, c, s, part_id, gstart, out);
+}
+
+void launch_group_bbox(int dim, const double* x, const int64_t* gstart, int32_t ngroups, double* out, hipStream_t st)
+{
+    if (dim == 2) hipLaunchKernelGGL(k_group_bbox<2>, dim3(ngroups), dim3(TPB), 0, st, x, gstart, out);
+    else hipLaunchKernelGGL(k_group_bbox<3>, dim3(ngroups), dim3(TPB), 0, st, x, gstart, out);
+}
+
+/* out[g] = scale[g] * sum of vals[seg[g] .. seg[g+1]) in a fixed order
+ * (scale may be null: 1), one block per segment. */
+__global__ __launch_bounds__(TPB) void k_seg_sum(const double* __restrict__ vals, const int64_t* __restrict__ seg,
+                                                 const double* __restrict__ scale, double* __restrict__ out)
+{
+    __shared__ double lds[NWAVE];
+    const int64_t g = blockIdx.x, a = seg[g], e = seg[g + 1];
+    double acc = 0.0;
+    for (int64_t i = a + threadIdx.x; i < e; i += TPB) acc += vals[i];
+    acc = block_reduce(acc, OpSum(), lds);
+    if (threadIdx.x == 0) out[g] = scale ? acc * scale[g] : acc;
+}
+
+void launch_seg_sum(const double* vals, const int64_t* seg, int32_t nseg, const double* scale, double* out,
+                    hipStream_t st)
+{
+    hipLaunchKernelGGL(k_seg_sum, dim3(nseg), dim3(TPB), 0, st, vals, seg, scale, out);
 }
 
 }  // namespace fj

@@ -252,21 +252,18 @@ __device__ void for_centres(const DevMesh& M, const int32_t* lo, const int32_t* 
     }
 }
 
+/* The K nearest cell centres of p, nearest first (equal distances by cell
+ * index), into list[0..K) (cell 0 past the mesh's cell count): a radius
+ * holding between min(K, ncells) and KNN_CAP centres found by bisection
+ * over the centre grid, those centres sorted in LDS.  Every thread of the
+ * block calls it; false if no such radius was found. */
 template <int D>
-__global__ __launch_bounds__(TPB) void k_lost_classify(DevState s, DevMesh M, const int32_t* __restrict__ lost_list,
-                                                       int32_t* __restrict__ lost_del, DevScalars* sc)
+__device__ bool nearest_list(const DevMesh& M, const double* p, int32_t K, double* key, int32_t* idx, int32_t* list,
+                             int32_t* s_cnt)
 {
-    __shared__ double key[KNN_CAP];
-    __shared__ int32_t idx[KNN_CAP];
-    __shared__ int32_t list[KNN_K];
-    __shared__ int32_t s_cnt, s_first, s_par, s_wall, s_far;
-    const int32_t slot = lost_list[blockIdx.x];
-    double p[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int a = 0; a < D; ++a) p[a] = s.x[(int64_t)slot * D + a];
-    const int32_t kt = M.ncells < KNN_K ? M.ncells : KNN_K;
+    const int32_t kt = M.ncells < K ? M.ncells : K;
 
-    /* 1. a radius with kt..KNN_CAP centres (all of them if they fit) */
+    /* a radius with kt..KNN_CAP centres (all of them if they fit) */
     double r = INFINITY, rlo = 0.0, rhi = INFINITY;
     if (M.ncells > KNN_CAP) {
         double out2 = 0.0;
@@ -280,15 +277,15 @@ __global__ __launch_bounds__(TPB) void k_lost_classify(DevState s, DevMesh M, co
     }
     bool ok = false;
     for (int it = 0; it < 200 && !ok; ++it) {
-        if (threadIdx.x == 0) s_cnt = 0;
+        if (threadIdx.x == 0) *s_cnt = 0;
         __syncthreads();
         int32_t lo[3], hi[3], mine = 0;
         const double r2 = r * r;
         if (centre_range<D>(M, p, r, lo, hi))
             for_centres(M, lo, hi, [&](int32_t cell) { mine += centre_d2<D>(M, cell, p) <= r2 ? 1 : 0; });
-        if (mine) atomicAdd(&s_cnt, mine);
+        if (mine) atomicAdd(s_cnt, mine);
         __syncthreads();
-        const int32_t cnt = s_cnt;
+        const int32_t cnt = *s_cnt;
         __syncthreads();
         if (cnt < kt) {
             rlo = r;
@@ -299,16 +296,10 @@ __global__ __launch_bounds__(TPB) void k_lost_classify(DevState s, DevMesh M, co
         } else
             ok = true;
     }
-    if (!ok) {
-        if (threadIdx.x == 0) {
-            lost_del[blockIdx.x] = 0;
-            atomicOr(&sc->lost_err, 1);
-        }
-        return;
-    }
+    if (!ok) return false;
 
     /* the centres within r, sorted by (distance, index) */
-    if (threadIdx.x == 0) s_cnt = 0;
+    if (threadIdx.x == 0) *s_cnt = 0;
     __syncthreads();
     {
         int32_t lo[3], hi[3];
@@ -317,7 +308,7 @@ __global__ __launch_bounds__(TPB) void k_lost_classify(DevState s, DevMesh M, co
             for_centres(M, lo, hi, [&](int32_t cell) {
                 const double d2 = centre_d2<D>(M, cell, p);
                 if (d2 <= r2) {
-                    const int32_t k = atomicAdd(&s_cnt, 1);
+                    const int32_t k = atomicAdd(s_cnt, 1);
                     if (k < KNN_CAP) {
                         key[k] = d2;
                         idx[k] = cell;
@@ -326,14 +317,9 @@ __global__ __launch_bounds__(TPB) void k_lost_classify(DevState s, DevMesh M, co
             });
     }
     __syncthreads();
-    const int32_t cnt = s_cnt;
-    if (cnt < kt || cnt > KNN_CAP) { /* the count above was made with the same test */
-        if (threadIdx.x == 0) {
-            lost_del[blockIdx.x] = 0;
-            atomicOr(&sc->lost_err, 1);
-        }
-        return;
-    }
+    const int32_t cnt = *s_cnt;
+    __syncthreads();
+    if (cnt < kt || cnt > KNN_CAP) return false; /* the count above was made with the same test */
     int32_t P = 1;
     while (P < cnt) P <<= 1;
     for (int32_t k = cnt + threadIdx.x; k < P; k += TPB) {
@@ -359,7 +345,32 @@ __global__ __launch_bounds__(TPB) void k_lost_classify(DevState s, DevMesh M, co
             }
             __syncthreads();
         }
-    for (int32_t j = threadIdx.x; j < KNN_K; j += TPB) list[j] = j < kt ? idx[j] : 0;
+    for (int32_t j = threadIdx.x; j < K; j += TPB) list[j] = j < kt ? idx[j] : 0;
+    __syncthreads();
+    return true;
+}
+
+template <int D>
+__global__ __launch_bounds__(TPB) void k_lost_classify(DevState s, DevMesh M, const int32_t* __restrict__ lost_list,
+                                                       int32_t* __restrict__ lost_del, DevScalars* sc)
+{
+    __shared__ double key[KNN_CAP];
+    __shared__ int32_t idx[KNN_CAP];
+    __shared__ int32_t list[KNN_K];
+    __shared__ int32_t s_cnt, s_first, s_par, s_wall, s_far;
+    const int32_t slot = lost_list[blockIdx.x];
+    double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < D; ++a) p[a] = s.x[(int64_t)slot * D + a];
+
+    /* 1. */
+    if (!nearest_list<D>(M, p, KNN_K, key, idx, list, &s_cnt)) {
+        if (threadIdx.x == 0) {
+            lost_del[blockIdx.x] = 0;
+            atomicOr(&sc->lost_err, 1);
+        }
+        return;
+    }
     if (threadIdx.x == 0) {
         s_first = KNN_K;
         s_par = s_wall = s_far = 0;
@@ -428,6 +439,141 @@ __global__ __launch_bounds__(TPB) void k_lost_classify(DevState s, DevMesh M, co
         lost_del[blockIdx.x] = del ? 1 : 0;
         if (del) atomicAdd(&sc->lost_del, 1);
     }
+}
+
+/* FirstCell for a particle leaving the pipe (one block each): if the
+ * lookup before gave it no cell that holds it, the KNN_FIRST nearest cells
+ * (the reference's 150 in 3D, 20 in 2D) and CheckCell over them; if none
+ * holds it, the boundary branch in the reference's order, in one thread:
+ * for every listed cell and each of its boundary faces in face order, the
+ * segment from the particle's position before this test to the cell's
+ * centre; every hit flips the parity, a wall face reflects the velocity
+ * off the face's normal (cross product of its first two edges, normalised;
+ * in 2D the edge turned left) and moves the particle onto the face's plane
+ * (through its second vertex), a far-field face marks it for deletion.  No
+ * crossing at all is the reference's fatal error (lost_err 2). */
+template <int D>
+__global__ __launch_bounds__(TPB) void k_first_cell(DevState s, DevMesh M, const int32_t* __restrict__ fresh,
+                                                    int32_t* __restrict__ fc_del, DevScalars* sc)
+{
+#pragma clang fp contract(off)
+    constexpr int32_t K = D == 3 ? 150 : 20;
+    __shared__ double key[KNN_CAP];
+    __shared__ int32_t idx[KNN_CAP];
+    __shared__ int32_t list[K];
+    __shared__ int32_t s_cnt, s_first;
+    const int32_t slot = fresh[blockIdx.x];
+    if (threadIdx.x == 0) fc_del[blockIdx.x] = 0;
+    double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < D; ++a) p[a] = s.x[(int64_t)slot * D + a];
+    const int32_t cur = s.cellID[slot];
+    if (cur >= 0 && cur < M.ncells && check_cell<D>(M, cur, p)) return; /* found by the lookup */
+    if (!nearest_list<D>(M, p, K, key, idx, list, &s_cnt)) {
+        if (threadIdx.x == 0) atomicOr(&sc->lost_err, 1);
+        return;
+    }
+    if (threadIdx.x == 0) s_first = K;
+    __syncthreads();
+    for (int32_t j = threadIdx.x; j < K; j += TPB)
+        if (check_cell<D>(M, list[j], p)) atomicMin(&s_first, j);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (s_first < K) {
+        const int32_t cell = list[s_first];
+        s.cellID[slot] = cell;
+#pragma unroll
+        for (int a = 0; a < D; ++a) s.cellV[(int64_t)slot * D + a] = M.cVel[(int64_t)cell * D + a];
+        s.cellP[slot] = M.cP[cell];
+        s.cellRho[slot] = M.cRho[cell];
+        return;
+    }
+    double xi[D], v[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        xi[a] = p[a];
+        v[a] = s.v[(int64_t)slot * D + a];
+    }
+    int cross = 0;
+    bool del = false;
+    for (int32_t j = 0; j < K; ++j) {
+        const int32_t cell = list[j];
+        const double* rayp = M.cCentre + (int64_t)cell * 3;
+        for (int32_t k = M.cell_off[cell]; k < M.cell_off[cell + 1]; ++k) {
+            const int32_t f = M.cell_f[k];
+            const int32_t right = M.face_lr[2 * (int64_t)f + 1];
+            if (right >= 0) continue;
+            const int hit = D == 3 ? crossings3d(M, f, p, rayp) : line_intersection(M, f, p, rayp);
+            if (!hit) continue;
+            cross ^= 1;
+            if (right == -1) {
+                const int32_t* fv = M.face_v + M.face_off[f];
+                double nrm[D];
+                if (D == 3) {
+                    const double* v0 = mvert(M, fv[0]);
+                    const double* v1 = mvert(M, fv[1]);
+                    const double* v2 = mvert(M, fv[2]);
+                    const double r1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+                    const double r2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+                    nrm[0] = r1[1] * r2[2] - r1[2] * r2[1];
+                    nrm[1] = r1[2] * r2[0] - r1[0] * r2[2];
+                    nrm[D - 1] = r1[0] * r2[1] - r1[1] * r2[0];
+                } else {
+                    const double* v0 = mvert(M, fv[0]);
+                    const double* v1 = mvert(M, fv[1]);
+                    nrm[0] = -(v1[1] - v0[1]);
+                    nrm[1] = v1[0] - v0[0];
+                }
+                double z = 0.0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) z += nrm[a] * nrm[a];
+                if (z > 0.0) {
+                    const double rn = sqrt(z);
+#pragma unroll
+                    for (int a = 0; a < D; ++a) nrm[a] = nrm[a] / rn;
+                }
+                double vn = 0.0, xn = 0.0, nn = 0.0, plane = 0.0;
+                const double* q = mvert(M, fv[1]);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    vn += v[a] * nrm[a];
+                    plane += nrm[a] * q[a];
+                }
+#pragma unroll
+                for (int a = 0; a < D; ++a) v[a] = v[a] - (2 * vn) * nrm[a];
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    xn += xi[a] * nrm[a];
+                    nn += nrm[a] * nrm[a];
+                }
+                const double dist = (plane - xn) / nn;
+#pragma unroll
+                for (int a = 0; a < D; ++a) xi[a] = xi[a] + dist * nrm[a];
+            } else if (right == -2) {
+                del = true;
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        s.x[(int64_t)slot * D + a] = xi[a];
+        s.v[(int64_t)slot * D + a] = v[a];
+    }
+    atomicAdd(&sc->fc_count, 1);
+    if (!cross) atomicOr(&sc->lost_err, 2);
+    if (del) {
+        fc_del[blockIdx.x] = 1;
+        atomicAdd(&sc->fc_del, 1);
+    }
+}
+
+void launch_first_cell(int dim, DevState s, DevMesh m, const int32_t* fresh, int32_t nfresh, int32_t* fc_del,
+                       DevScalars* sc, hipStream_t st)
+{
+    if (nfresh <= 0) return;
+    const dim3 g(static_cast<unsigned>(nfresh)), b(TPB);
+    if (dim == 2) hipLaunchKernelGGL(k_first_cell<2>, g, b, 0, st, s, m, fresh, fc_del, sc);
+    else hipLaunchKernelGGL(k_first_cell<3>, g, b, 0, st, s, m, fresh, fc_del, sc);
 }
 
 __global__ __launch_bounds__(TPB) void k_keep_all(int32_t* __restrict__ keep, int32_t* __restrict__ hflag, int64_t n)

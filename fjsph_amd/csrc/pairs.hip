@@ -118,6 +118,16 @@ __device__ __forceinline__ int64_t block_of_slot(const TileGrid& tg, int64_t i)
     return row * tg.nchunk + cx / tg.C;
 }
 
+/* Group of block b (independent simulations): the groups occupy disjoint
+ * cell layers along the last axis, and a block lies in one cell row. */
+template <int D>
+__device__ __forceinline__ int32_t block_group(const TileGrid& tg, int64_t b)
+{
+    const int64_t row = b / tg.nchunk;
+    const int64_t layer = (D == 3) ? row / tg.dims[1] : row;
+    return tg.lgrp[layer];
+}
+
 /* Shared block prologue: union table in LDS, uniform p0/p1/U. */
 struct BlockHead {
     int32_t seg_a[MAXSEG];
@@ -240,6 +250,12 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
     const int U = h.pre[h.nseg];
     const int NT = n_tiles(U);
     if (h.p0 == h.p1) return;
+    if (tg.gbuild) {
+        /* a partial build (independent simulations): other groups keep
+         * their lists */
+        const int32_t g = block_group<D>(tg, blockIdx.x);
+        if (g < 0 || !tg.gbuild[g]) return;
+    }
     if (NT * TILE > int(NO_ENTRY)) {
         if (threadIdx.x == 0) atomicMax(&sc->overflow, U);
         return;
@@ -2105,6 +2121,126 @@ __device__ void gissler(const Consts& c, const double* Vd, double rho, double pr
 }
 
 
+/* Eigen's normalized(): a zero vector is returned unchanged. */
+template <int D>
+__device__ void normalized_d(const double* a, double* out)
+{
+    const double z = dotd<D>(a, a);
+    const double r = z > 0.0 ? sqrt(z) : 1.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) out[k] = z > 0.0 ? a[k] / r : a[k];
+}
+
+/* GetCd (src/Aero.h:10-18). */
+__device__ inline double sphere_cd(double Re)
+{
+    return (1.0 + 0.197 * pow(Re, 0.63) + 2.6e-04 * pow(Re, 1.38)) * (24.0 / (Re + 0.00001));
+}
+
+/* induced_pressure (src/Aero.h:106-202) in FP64 for one particle: a
+ * pressure coefficient blended from sphere, plate and bowl curves by the
+ * angle between the surface normal and the gas velocity and by the
+ * particle's scaled curvature, the induced pressure on the plate area along
+ * the normal, a seventh-power-law skin friction on the tangential velocity,
+ * blended with the droplet drag by the neighbourhood fraction. */
+template <int D>
+__device__ void induced_pressure(const Consts& c, const double* Vd, const double* nrm, double lam, double nneigh,
+                                 double norm_curve, double cellP, double cellRho, double mass, double* out)
+{
+    double nn[D], vh[D];
+    normalized_d<D>(nrm, nn);
+    normalized_d<D>(Vd, vh);
+    double cth = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) cth += -nn[k] * vh[k];
+    const double theta = fabs(acos(cth));
+    double Cp_s, Cp_p, Cp_b, Cp_tot;
+    if (theta < 2.4455) {
+        const double sn = sin(theta);
+        Cp_s = 1.0 - (2.25) * (sn * sn);
+    } else {
+        Cp_s = 0.075;
+    }
+    if (theta < 1.570797) Cp_p = cos(theta);
+    else if (theta < 1.9918) Cp_p = -pow(cos(6.0 * theta + 0.5 * kPi), 1.5);
+    else if (theta < 2.0838) Cp_p = 5.5836 * theta - 11.5601;
+    else Cp_p = 0.075;
+    if (theta < 0.7854) Cp_b = 1.0;
+    else if (theta < 1.570797) Cp_b = 0.5 * (cos(4.0 * theta - kPi) + 1.0);
+    else Cp_b = 0.0;
+    const double fac1 = 0.25, ifac1 = 1 / fac1;
+    if (norm_curve < -fac1) {
+        Cp_tot = Cp_b;
+    } else if (norm_curve < 0.0) {
+        const double frac = (norm_curve + fac1) * ifac1;
+        Cp_tot = frac * Cp_b + (1.0 - frac) * Cp_p;
+    } else if (norm_curve < fac1) {
+        const double frac = norm_curve * ifac1;
+        Cp_tot = frac * Cp_p + (1.0 - frac) * Cp_s;
+    } else {
+        Cp_tot = Cp_s;
+    }
+    const double sos2 = c.sos * c.sos;
+    const double V2 = dotd<D>(Vd, Vd);
+    const double Pi = 0.5 * V2 / sos2 * c.gamma_g * cellP * Cp_tot;
+    const double Vn = sqrt(V2);
+    const double Cdi = sphere_cd(cellRho * Vn * c.L / c.mu_g);
+    double Vnorm = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) Vnorm += Vd[k] * nn[k];
+    double Vpar[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) Vpar[k] = Vd[k] - Vnorm * nn[k];
+    const double Vpn = sqrt(dotd<D>(Vpar, Vpar));
+    const double Cf = 0.027 / pow(cellRho * Vpn * c.L / c.mu_g + 1e-6, 1.0 / 7.0);
+    const double frac1 = c.use_lam ? fmin(c.interp_fac * lam, 1.0) : fmin(c.interp_fac * nneigh * c.i_n_full, 1.0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double acc_drop = 0.5 * Vd[k] * Vn / sos2 * c.gamma_g * cellP * (kPi * c.L * c.L * 0.25) * Cdi / mass;
+        const double acc_kern = -Pi * c.A_plate * nn[k] / mass;
+        const double acc_skin = 0.5 * Vpn * Vpar[k] / sos2 * c.gamma_g * cellP * Cf * c.A_plate / mass;
+        out[k] = frac1 * (acc_kern + acc_skin) + (1.0 - frac1) * acc_drop;
+    }
+}
+
+/* The skin-friction case of CalcAeroAcc (src/Aero.h:223-256): stagnation
+ * pressure of the normal velocity and seventh-power-law friction of the
+ * tangential one on the plate area, blended with the droplet drag by
+ * min(1.5 lambda, 1); nothing unless the gas moves into the surface. */
+template <int D>
+__device__ void skin_friction(const Consts& c, const double* Vd, const double* nrm, double lam, double mass,
+                              double* out)
+{
+    double nn[D];
+    normalized_d<D>(nrm, nn);
+    double Vnorm = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) Vnorm += Vd[k] * nn[k];
+    if (!(Vnorm > 0.001)) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) out[k] = 0.0;
+        return;
+    }
+    const double Vn = sqrt(dotd<D>(Vd, Vd));
+    const double Re = c.rho_g * Vn * c.L / c.mu_g;
+    double Vpar[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) Vpar[k] = Vd[k] - fabs(Vnorm) * nn[k];
+    const double Cf = 0.027 / pow(Re, 1.0 / 7.0);
+    const double frac2 = fmin(1.5 * lam, 1.0);
+    const double frac1 = 1.0 - frac2;
+    const double Cdi = sphere_cd(Re);
+    const double Vpn = sqrt(dotd<D>(Vpar, Vpar));
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double acc_press = 0.5 * c.rho_g * Vnorm * Vnorm * c.A_plate * nn[k] / mass;
+        const double acc_skin = 0.5 * c.rho_g * Vpn * Cf * c.A_plate * Vpar[k] / mass;
+        const double acc_drop = 0.5 * c.rho_g * Vn * Vd[k] * (kPi * c.L * c.L / 4) * Cdi / mass;
+        out[k] = frac2 * (acc_press + acc_skin) + frac1 * acc_drop;
+    }
+}
+
+
 /* Occupancy request for the force pass: 5 waves per SIMD (at most 102
  * VGPRs) measured faster than the compiler's 4 at 97 VGPRs. */
 #ifndef FJ_FORCE_OCC
@@ -2160,7 +2296,14 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
             xbref[a] = (F32 && FR) ? ax.xb[i0 * D + a] : 0.0;
         }
     }
-    const double npd = sc->npd;
+    double npd = sc->npd;
+    if (tg.lgrp) {
+        /* independent simulations: a block holds one group's targets; a
+         * group that has converged keeps its rates */
+        const int32_t g = block_group<D>(tg, blockIdx.x);
+        if (g < 0 || !tg.gactive[g]) return;
+        npd = tg.gnpd[g];
+    }
     const T ste = T(-(0.5 * c.sig / c.lam_st) / (npd * npd));
     const T pi3o4H = T(3.0 * kPi / 4.0 / c.H);
     const T inv_H = T(1.0 / c.H), Ht = T(c.H), Wc = T(c.Wc);
@@ -2366,12 +2509,26 @@ __global__ __launch_bounds__(TPB) void k_nb_update(Consts c, DevState s, DevStat
                                                    const int32_t* __restrict__ blk, DevBlocks bl,
                                                    const int32_t* __restrict__ near_inlet,
                                                    const int32_t* __restrict__ nnb, int64_t n, double dt,
-                                                   double* partial)
+                                                   double* partial, DevGroups gr)
 {
     __shared__ double lds[NWAVE];
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     double resid = 0.0;
-    if (i < n) {
+    bool held = false; /* a group that has converged keeps its state */
+    if (gr.grp && i < n) {
+        const int32_t g = gr.grp[i];
+        held = !gr.active[g];
+        dt = gr.dt[g];
+    }
+    if (held) {
+        if (do_update) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) { nx.x[i * D + a] = s.x[i * D + a]; nx.v[i * D + a] = s.v[i * D + a]; }
+            nx.rho[i] = s.rho[i];
+            nx.p[i] = s.p[i];
+            gr.d2[i] = 0.0;
+        }
+    } else if (i < n) {
         const bool fluid = part_id[i] >= c.n_bound;
         double xi[D], vi[D], acc_new[D];
         load_vec<D>(s.x, i, xi);
@@ -2386,6 +2543,15 @@ __global__ __launch_bounds__(TPB) void k_nb_update(Consts c, DevState s, DevStat
                 if (c.aero_case == FJSPH_GISSLER) {
                     gissler<D>(c, Vd, s.cellRho[i], s.cellP[i], s.m[i], s.lam_nb[i], double(nnb[i] + 1), s.woccl[i],
                                aero);
+                } else if (c.aero_case == FJSPH_INDUCED_PRESSURE) {
+                    double nrm[D];
+                    load_vec<D>(s.norm, i, nrm);
+                    induced_pressure<D>(c, Vd, nrm, s.lam_nb[i], double(nnb[i] + 1), s.norm_curve[i], s.cellP[i],
+                                        s.cellRho[i], s.m[i], aero);
+                } else if (c.aero_case == FJSPH_SKIN_FRIC) {
+                    double nrm[D];
+                    load_vec<D>(s.norm, i, nrm);
+                    skin_friction<D>(c, Vd, nrm, s.lam_nb[i], s.m[i], aero);
                 } else {
 #pragma unroll
                     for (int a = 0; a < D; ++a) aero[a] = 0.0;
@@ -2453,9 +2619,10 @@ __global__ __launch_bounds__(TPB) void k_nb_update(Consts c, DevState s, DevStat
             for (int a = 0; a < D; ++a) { nx.x[i * D + a] = xo[a]; nx.v[i * D + a] = vo[a]; }
             nx.rho[i] = rho_o;
             nx.p[i] = p_o;
+            if (gr.grp) gr.d2[i] = resid;
         }
     }
-    if (do_update) {
+    if (do_update && !gr.grp) {
         const double bs = block_reduce(resid, OpSum(), lds);
         if (threadIdx.x == 0) partial[blockIdx.x] = bs;
     }
@@ -2609,17 +2776,18 @@ void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
 
 void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double* next_x, double* next_v,
                       double* next_rho, double* next_p, const int32_t* part_id, const int32_t* blk, DevBlocks bl,
-                      const int32_t* near_inlet, NbrList nl, int64_t n, double dt, double* partial, hipStream_t st)
+                      const int32_t* near_inlet, NbrList nl, int64_t n, double dt, double* partial, hipStream_t st,
+                      DevGroups gr)
 {
     NextState nx{next_x, next_v, next_rho, next_p};
     const int upd = next_x != nullptr;
     const dim3 g(grid_of(n)), b(TPB);
     if (dim == 2)
         hipLaunchKernelGGL(k_nb_update<2>, g, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl.nnb, n, dt,
-                           partial);
+                           partial, gr);
     else
         hipLaunchKernelGGL(k_nb_update<3>, g, b, 0, st, c, s, pn, nx, upd, part_id, blk, bl, near_inlet, nl.nnb, n, dt,
-                           partial);
+                           partial, gr);
 }
 
 }  // namespace fj

@@ -70,6 +70,8 @@ SIGNATURES = {
     "fjsph_surrogate_train": (C.c_int, [_VP, C.c_int64, _D, _D, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                         C.c_uint64, _D, _D, _D, _D, _F32, _F32, _D]),
     "fjsph_surrogate_grad": (C.c_int, [_VP, C.c_int32, C.c_int32, _F32, _F32, _F32, _F32, _F32, _F32, _D]),
+    "fjsph_set_groups": (C.c_int, [_VP, C.c_int32, _I32]),
+    "fjsph_get_group_info": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_void_p]),
     "fjsph_set_block_planes": (C.c_int, [_VP, C.c_int32, C.POINTER(abi.BlockPlanes)]),
     "fjsph_set_inlet": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _I64, _I64]),
     "fjsph_set_max_points": (C.c_int, [_VP, C.c_int64]),
@@ -450,6 +452,21 @@ class Engine:
                                                 _ptr(w, _F32), _ptr(b, _F32), _ptr(dw, _F32), _ptr(db, _F32),
                                                 _ptr(loss)), "surrogate_grad")
         return dw, db, float(loss[0])
+
+    def set_groups(self, group):
+        """fjsph_set_groups: independent simulations, group[i] of host
+        particle i (0 .. n_groups - 1)."""
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        ng = int(g.max()) + 1 if g.size else 0
+        self._chk(self.lib.fjsph_set_groups(self.h, ng, _ptr(g, _I32)), "set_groups")
+        self.n_groups = ng
+
+    def group_info(self):
+        """Per-group step information of the last step (list of dicts)."""
+        ng = getattr(self, "n_groups", 0)
+        arr = (abi.StepInfo * max(ng, 1))()
+        self._chk(self.lib.fjsph_get_group_info(self.h, 0, ng, C.cast(arr, C.c_void_p)), "get_group_info")
+        return [arr[k].as_dict() for k in range(ng)]
 
     def neighbour_stats(self):
         pairs = np.zeros(1, np.int64)

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FJSPH_ABI_VERSION 6
+#define FJSPH_ABI_VERSION 7
 
 typedef enum fjsph_status {
     FJSPH_OK = 0,
@@ -341,6 +341,32 @@ fjsph_status fjsph_get_inlet(fjsph_ctx* ctx, int32_t block, int32_t* n_cols, int
 
 /* ---- tile blocks and their autotuner --------------------------------------- */
 
+/* Independent simulations in one context: group[i] (0..n_groups-1, every
+ * group non-empty) assigns host particle i to a simulation.  Replaces the
+ * reference's one process per simulation (src/FJSPH.cpp): each group keeps
+ * its own integrator state -- time step (find_timestep over its own
+ * particles), CFL control and counters, clock and frame end
+ * (src/Integration.cpp:233-303, 370-443), Newmark-Beta residual and
+ * iteration count (a group whose residual has converged keeps its state
+ * while the others iterate; a diverged group rolls back alone with its own
+ * time step halved, src/Newmark_Beta.cpp:10-52) and npd
+ * (src/Shifting.cpp:16-68, 121) -- and one fjsph_integrate_step advances
+ * every group by one step.  All pair passes run as one launch over every
+ * group.  Groups must not interact: particles of different groups farther
+ * apart than 2H (the engine bins each group apart, so nearer particles of
+ * two groups are simply not neighbours).  Requires the Newmark-Beta
+ * integrator, no boundary particles, no inlets or deletion planes; the
+ * per-pass entry points fjsph_acc_rrho, fjsph_nb_iter and
+ * fjsph_find_timestep are refused; fjsph_set_time sets every group.
+ * n_groups = 0 returns to one simulation.  fjsph_integrate_step's info
+ * then describes the batch: largest residual, iteration count, density
+ * variation and accelerations, smallest time step, CFL number and clock,
+ * summed rollbacks, mean npd. */
+fjsph_status fjsph_set_groups(fjsph_ctx* ctx, int32_t n_groups, const int32_t* group);
+/* The step information of groups [first, first + count) after the last
+ * step (out: count records). */
+fjsph_status fjsph_get_group_info(fjsph_ctx* ctx, int32_t first, int32_t count, fjsph_step_info* out);
+
 /* Targets per tile block of the pair passes (the cells of one block row
  * whose particles share one candidate union in LDS); 0 restores the
  * default, 256.  No counterpart in the reference.  Results change only in
@@ -452,6 +478,8 @@ typedef struct fjsph_lost_info {
     int64_t deleted;  /* particles the lookups deleted */
     int64_t internal; /* lookups that found a wall crossing */
     int64_t failed;   /* lookups that found no net crossing */
+    int64_t first_cell;         /* particles leaving the pipe in no cell (FirstCell's boundary branch) */
+    int64_t first_cell_deleted; /* of those, deleted across a far-field face */
 } fjsph_lost_info;
 fjsph_status fjsph_mesh_lost_info(fjsph_ctx* ctx, fjsph_lost_info* info);
 /* The mesh field at n host points (n*dim): the containing cell (-1 if none)
@@ -498,6 +526,7 @@ fjsph_status fjsph_surrogate_train(fjsph_ctx* ctx, int64_t n, const double* poin
 fjsph_status fjsph_surrogate_grad(fjsph_ctx* ctx, int32_t n_hidden, int32_t batch, const float* xn, const float* tn,
                                   const float* weights, const float* biases, float* dweights, float* dbiases,
                                   double* loss);
+
 
 /* ---- measurement ---------------------------------------------------------- */
 

@@ -45,6 +45,7 @@
 #include <limits>
 #include <string>
 #include <utility>
+#include <stdexcept>
 #include <vector>
 #include <set>
 
@@ -926,6 +927,67 @@ inline BoundaryTest boundary_test(const MeshData& M, const double* testp, const 
     return r;
 }
 
+/* FirstCell's branch for a particle leaving the pipe in none of its
+ * nearest cells (src/Containment.cpp:466-573): for each listed cell in
+ * order, the segment from the particle's position before this test to the
+ * cell's centre against each of the cell's boundary faces in face order.
+ * Every hit flips `cross` (no face loop ends early); a wall face (-1)
+ * reflects the velocity off the face and moves the particle onto the
+ * face's plane (Eigen's cross product and normalized(), the plane through
+ * the face's second vertex); a far-field face (-2) marks the particle for
+ * deletion. */
+struct FirstCellResult {
+    int cross = 0;
+    bool del = false;
+};
+template <int D>
+inline FirstCellResult first_cell_boundary(const MeshData& M, const std::vector<int64_t>& cand, Vec<D>& xi,
+                                           Vec<D>& v)
+{
+    FirstCellResult r;
+    Vec<D> testp = xi;
+    for (int64_t idx : cand) {
+        const double* rayp = &M.cCentre[size_t(idx) * size_t(M.dim)];
+        for (int64_t k = M.cell_off[size_t(idx)]; k < M.cell_off[size_t(idx) + 1]; ++k) {
+            const int64_t f = M.cell_f[size_t(k)];
+            const int64_t right = M.face_r[size_t(f)];
+            if (right >= 0) continue;
+            const int ints = D == 3 ? crossings3d(M, f, testp.c, rayp) : line_intersection(M, f, testp.c, rayp);
+            if (!ints) continue;
+            r.cross = !r.cross;
+            if (right == -1) {
+                const int64_t* fv = &M.face_v[size_t(M.face_off[size_t(f)])];
+                Vec<D> nrm;
+                if (D == 3) {
+                    double r1[3], r2[3];
+                    for (int a = 0; a < 3; ++a) {
+                        r1[a] = M.vert(fv[1])[a] - M.vert(fv[0])[a];
+                        r2[a] = M.vert(fv[2])[a] - M.vert(fv[0])[a];
+                    }
+                    nrm.c[0] = r1[1] * r2[2] - r1[2] * r2[1];
+                    nrm.c[1] = r1[2] * r2[0] - r1[0] * r2[2];
+                    nrm.c[D - 1] = r1[0] * r2[1] - r1[1] * r2[0];
+                } else {
+                    const double r0 = M.vert(fv[1])[0] - M.vert(fv[0])[0];
+                    const double r1 = M.vert(fv[1])[1] - M.vert(fv[0])[1];
+                    nrm.c[0] = -r1;
+                    nrm.c[1] = r0;
+                }
+                nrm = normalized(nrm);
+                v = v - (2 * dot(v, nrm)) * nrm;
+                Vec<D> p1;
+                for (int a = 0; a < D; ++a) p1.c[a] = M.vert(fv[1])[a];
+                const double plane = dot(nrm, p1);
+                const double dist = (plane - dot(xi, nrm)) / dot(nrm, nrm);
+                xi = xi + dist * nrm;
+            } else if (right == -2) {
+                r.del = true;
+            }
+        }
+    }
+    return r;
+}
+
 #include "ipt_oracle.hpp"
 
 /* MESH::maxlength as the TAU readers set it (Place_Edges, Place_Faces,
@@ -1008,6 +1070,7 @@ struct Sim {
     bool have_mesh = false;
     int64_t mesh_lost = 0; /* aero particles in no cell at the last FindCell */
     int64_t mesh_deleted = 0, mesh_internal = 0, mesh_failed = 0; /* boundary-test outcomes since upload */
+    int64_t first_cell_reflected = 0, first_cell_deleted = 0; /* FirstCell's boundary branch since upload */
 
     /* FindCell (src/Containment.cpp:579-820) for the aero particles of pnp1
      * (FREE with lam_nb <= lam_cutoff; the others get no cell and keep their
@@ -1887,7 +1950,7 @@ struct Sim {
         Detect_Surface(start_index, end_index);
         dissipation_terms(start_index, end_index);
         if (s.ale) particle_shift(start_index, end_index);
-        Check_Pipe_Outlet();
+        Check_Pipe_Outlet(npd);
     }
 
     /* Check_Pipe_Outlet (src/Containment.cpp:822-890): PIPE particles past
@@ -1895,18 +1958,52 @@ struct Sim {
      * get their cell (FirstCell; the lookup of every aero particle finds
      * the cached cell of the others again).  The deletion of a particle
      * FirstCell places in no cell is not made (counted by the lookup). */
-    void Check_Pipe_Outlet()
+    void Check_Pipe_Outlet(double& npd)
     {
         bool any = false;
+        std::vector<size_t> fresh;
         for (const Block& bl : blocks) {
             if (!bl.b.is_fluid) continue;
             for (size_t ii = size_t(bl.b.first); ii < size_t(bl.b.last); ++ii)
                 if (pnp1[ii].b == FJSPH_PIPE && dot_raw<D>(pnp1[ii].xi, bl.pl.aero_norm) > bl.pl.aeroconst) {
                     pnp1[ii].b = FJSPH_FREE;
                     any = true;
+                    if (pnp1[ii].lam_nb < s.lam_cutoff) fresh.push_back(ii);
                 }
         }
-        if (any && s.aero_source == FJSPH_MESH_INFL && have_mesh) (void)find_cell(start_index, end_index, false);
+        if (!(any && s.aero_source == FJSPH_MESH_INFL && have_mesh)) return;
+        (void)find_cell(start_index, end_index, false);
+        /* FirstCell for the new aero particles in no cell: the K nearest
+         * cells (150 in 3D, 20 in 2D, padded with cell 0 as the reference's
+         * result vector is), then the boundary branch */
+        const size_t K = D == 3 ? 150 : 20;
+        std::vector<size_t> del;
+        for (size_t ii : fresh) {
+            Part<D>& q = pnp1[ii];
+            if (q.cellID >= 0 && size_t(q.cellID) < mesh.ncells() && check_cell(mesh, q.cellID, q.xi.c)) continue;
+            const std::vector<int64_t> cand = knn_cells_padded(mesh, q.xi.c, K);
+            bool found = false;
+            for (int64_t cell : cand)
+                if (check_cell(mesh, cell, q.xi.c)) {
+                    q.cellID = long(cell);
+                    for (int a = 0; a < D; ++a) q.cellV.c[a] = mesh.cVel[size_t(cell) * D + a];
+                    q.cellP = mesh.cP[size_t(cell)];
+                    q.cellRho = mesh.cRho[size_t(cell)];
+                    found = true;
+                    break;
+                }
+            if (found) continue;
+            const FirstCellResult r = first_cell_boundary<D>(mesh, cand, q.xi, q.v);
+            if (r.cross == 0)
+                throw std::runtime_error("FirstCell: first containing cell for a particle leaving the pipe not found");
+            first_cell_reflected += 1;
+            if (r.del) del.push_back(ii);
+        }
+        /* the reference erases the condemned particles from pnp1 alone and
+         * keeps its neighbour lists; here they leave both states and the
+         * lists and dSPH_PreStep are made again, as after FindCell */
+        first_cell_deleted += int64_t(del.size());
+        delete_lost(del, npd);
     }
 
     /* integrate_no_update (src/Integration.cpp:27-107) */

@@ -183,7 +183,13 @@ template <int D>
 int step(Handle* h, int kind, fjsph_step_info* info)
 {
     Sim<D>& S = sim<D>(h);
-    double rms = kind == 0 ? S.integrate_no_update() : S.integrate();
+    double rms = 0.0;
+    try {
+        rms = kind == 0 ? S.integrate_no_update() : S.integrate();
+    } catch (const std::exception&) {
+        /* a fatal condition of the reference (it prints and exits) */
+        return FJSPH_ERR_STATE;
+    }
     fill_info(info, rms, S.delta_t, S.cfl, S.current_time, S.maxRho_pc, S.maxf, S.maxAf, S.last_npd, S.iteration, S.rollbacks);
     return FJSPH_OK;
 }
@@ -493,12 +499,15 @@ int orc_mesh_lost(void* hp, int64_t* lost)
     return FJSPH_OK;
 }
 
-/* lost, deleted, internal, failed (fjsph_lost_info). */
-int orc_mesh_lost_info(void* hp, int64_t* out4)
+/* lost, deleted, internal, failed, first_cell, first_cell_deleted
+ * (fjsph_lost_info). */
+int orc_mesh_lost_info(void* hp, int64_t* out6)
 {
     Handle* h = static_cast<Handle*>(hp);
-    SIMCALL(h, (out4[0] = S.mesh_lost, out4[1] = S.mesh_deleted, out4[2] = S.mesh_internal, out4[3] = S.mesh_failed),
-            (out4[0] = S.mesh_lost, out4[1] = S.mesh_deleted, out4[2] = S.mesh_internal, out4[3] = S.mesh_failed));
+    SIMCALL(h, (out6[0] = S.mesh_lost, out6[1] = S.mesh_deleted, out6[2] = S.mesh_internal, out6[3] = S.mesh_failed,
+                out6[4] = S.first_cell_reflected, out6[5] = S.first_cell_deleted),
+            (out6[0] = S.mesh_lost, out6[1] = S.mesh_deleted, out6[2] = S.mesh_internal, out6[3] = S.mesh_failed,
+             out6[4] = S.first_cell_reflected, out6[5] = S.first_cell_deleted));
     return FJSPH_OK;
 }
 

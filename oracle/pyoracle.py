@@ -254,9 +254,10 @@ class Oracle:
         return int(self.lib.orc_check_cell(self.h, int(cell), _ptr(p)))
 
     def lost_info(self):
-        v = np.zeros(4, np.int64)
+        v = np.zeros(6, np.int64)
         self._chk(self.lib.orc_mesh_lost_info(self.h, _ptr(v, _I64)), "mesh_lost_info")
-        return dict(zip(("lost", "deleted", "internal", "failed"), (int(a) for a in v)))
+        return dict(zip(("lost", "deleted", "internal", "failed", "first_cell", "first_cell_deleted"),
+                        (int(a) for a in v)))
 
     def boundary_test(self, point):
         """FindCell's boundary test of a point: (cross, wall, far)."""

@@ -132,7 +132,7 @@ def test_create_rejects_unbuilt_options():
     lib = engine.load()
     s = abi.default_settings(3)
     s.particle_step = 1e-3
-    s.aero_source = abi.MESH_INFL
+    s.aero_source = abi.VLM_INFL
     h = C.c_void_p()
     assert lib.fjsph_create(C.byref(s), 0, C.byref(h)) in (abi.ERR_UNSUPPORTED, abi.ERR_NO_DEVICE)
     s.aero_source = abi.CONST_VEL

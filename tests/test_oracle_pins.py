@@ -27,6 +27,19 @@ sign or a wrong factor fails one of them.
                                                       at 45 degrees the lattice angles (0, 1/sqrt 2, 1)
   Gissler plate branch (src/Aero.h:37-98)             frac2 = 1: a flat plate of area dx^2 with Cd = 1,
                                                       times the exposed fraction 1 - woccl
+  induced pressure (src/Aero.h:106-202)               a two-layer sheet (scaled curvature > 1/4) at frac1 = 1:
+                                                      the stagnation pressure 1/2 rho_g U^2 on the exposed
+                                                      face (Cp = 1) pushing along the stream, the base
+                                                      Cp = 0.075 on the rear face pushing against its
+                                                      normal; with a tangential gas velocity W the
+                                                      sphere curve Cp = 1 - 9/4 sin^2 theta on the
+                                                      normal and the seventh-power skin friction
+                                                      1/2 rho_g W^2 0.027 Re_W^(-1/7) on the tangent
+  skin friction (src/Aero.h:223-256)                  no force on a face the gas leaves (V . n <= 0);
+                                                      on the others the stagnation pressure of the
+                                                      normal velocity plus the skin friction of the
+                                                      tangential one, blended with the droplet's sphere
+                                                      drag by min(1.5 lambda, 1)
   ghost boundary (src/Resid.cpp:122-160)              drho/dt = -rho div v of the surrounding fluid
                                                       (0.2%); near_inlet clamps (src/Newmark_Beta.cpp:
                                                       163-193)
@@ -35,6 +48,8 @@ sign or a wrong factor fails one of them.
   DBC (src/Resid.cpp:78-119, src/Newmark_Beta.cpp:    zero boundary acceleration, density held
     145-160)
   block motion table (src/Newmark_Beta.cpp:69-95)     the velocity of the last time strictly before now
+  whole runs: Couette and Poiseuille start-up        the analytic profiles the reference ships
+    (Examples/Couette, Examples/Poiseuille)           (Examples/*/SteadyState.dat) at t = 5 s and 0.02 s
   Newmark rollback (src/Newmark_Beta.cpp:32-48)       a step started with 8x a stable dt diverges, is
                                                       rolled back once to pn with dt halved, converges,
                                                       and its final positions satisfy the Newmark update
@@ -483,3 +498,140 @@ def test_newmark_rollback_halves_dt(po):
     expect = xn[nb:] + dt * vn[nb:] + dt * dt * (0.25 * an[nb:] + 0.25 * a[nb:])
     assert np.allclose(x[nb:], expect, rtol=0, atol=1e-13)
     assert np.array_equal(x[:nb], xn[:nb])
+
+
+# --------------------------------------- induced pressure and skin friction
+
+def _plate_centre(c, dim):
+    """Particles of the sheet at least 3 dx from its edges, where the
+    normals are along x."""
+    dx = c.settings.particle_step
+    lo, hi = c.x.min(axis=0), c.x.max(axis=0)
+    sel = np.ones(c.n, bool)
+    for k in range(1, dim):
+        sel &= (c.x[:, k] > lo[k] + 3 * dx) & (c.x[:, k] < hi[k] - 3 * dx)
+    return sel
+
+
+def _run_aero(po, c):
+    o = po.Oracle(c, nthreads=4)
+    o.build_neighbours()
+    npd = o.dsph_prestep()
+    o.aero_velocity()
+    o.detect_surface()
+    o.dissipation_terms()
+    o.acc_rrho(npd)
+    return o
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("W", [0.0, 12.0])
+def test_induced_pressure_flat_sheet(po, dim, W):
+    c = _sheet(dim, False, i_interp_fac=0.1)
+    s = c.settings
+    s.aero_case = abi.INDUCED_PRESSURE
+    s.v_inf[1] = W
+    o = _run_aero(po, c)
+    dx = s.particle_step
+    mid = _plate_centre(c, dim)
+    nrm = o.field("NORM")
+    front = c.x[:, 0] < dx
+    assert np.all(o.field("LAM_NB")[mid] * 10.0 >= 1.0)
+    assert np.allclose(nrm[mid & front, 0], -1.0, atol=1e-12) and np.allclose(nrm[mid & ~front, 0], 1.0, atol=1e-12)
+    # the two faces' opposite normals give a scaled curvature dx kappa near
+    # 2, above 0.25: the pressure coefficient is the sphere curve alone
+    assert np.all(o.field("NORM_CURVE")[mid] > 0.25)
+    af = o.field("AF")
+    rho_g, U = 1.2, 20.0
+    A, m = dx ** (dim - 1), c.m[0]
+    V2 = U * U + W * W
+    sin2 = W * W / V2
+    # exposed face: Cp = 1 - 9/4 sin^2(theta) along +x, friction along +y
+    cp = 1.0 - 2.25 * sin2
+    assert np.allclose(af[mid & front, 0], 0.5 * rho_g * V2 * cp * A / m, rtol=1e-9, atol=0)
+    mu_g = s.mu_g
+    L = dx * (3.0 / (4.0 * math.pi)) ** (1.0 / 3.0) if dim == 3 else dx / math.sqrt(math.pi)
+    cf = 0.027 / (rho_g * W * L / mu_g + 1e-6) ** (1.0 / 7.0)
+    scale = 0.5 * rho_g * V2 * A / m
+    assert np.allclose(af[mid & front, 1], 0.5 * rho_g * W * W * cf * A / m, rtol=1e-9, atol=1e-12 * scale)
+    # rear face: theta >= 2.4455 for W = 0 (base Cp 0.075 against the normal)
+    if W == 0.0:
+        assert np.allclose(af[mid & ~front, 0], -0.5 * rho_g * U * U * 0.075 * A / m, rtol=1e-9, atol=0)
+        assert np.all(np.abs(af[mid][:, 1:]) <= 1e-9 * np.abs(af[mid, 0]).max())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_skin_friction_flat_sheet(po, dim):
+    c = _sheet(dim, False)
+    s = c.settings
+    s.aero_case = abi.SKIN_FRIC
+    W = 8.0
+    s.v_inf[1] = W
+    o = _run_aero(po, c)
+    dx = s.particle_step
+    mid = _plate_centre(c, dim)
+    front = c.x[:, 0] < dx
+    af = o.field("AF")
+    # the gas leaves the upstream face's outward normal (-x): no force
+    assert np.all(af[mid & front] == 0.0)
+    rear = mid & ~front
+    lam = o.field("LAM_NB")[rear]
+    frac = np.minimum(1.5 * lam, 1.0)
+    rho_g, U = 1.2, 20.0
+    A, m = dx ** (dim - 1), c.m[0]
+    L = dx * (3.0 / (4.0 * math.pi)) ** (1.0 / 3.0) if dim == 3 else dx / math.sqrt(math.pi)
+    Vn = math.hypot(U, W)
+    Re = rho_g * Vn * L / s.mu_g
+    cf = 0.027 / Re ** (1.0 / 7.0)
+    # plate: stagnation pressure of the normal velocity U, friction of W
+    plate = np.array([0.5 * rho_g * U * U * A / m, 0.5 * rho_g * W * W * cf * A / m])
+    # sphere drag of a droplet of diameter L, cross-section pi L^2 / 4
+    cd = (1.0 + 0.197 * Re ** 0.63 + 2.6e-4 * Re ** 1.38) * 24.0 / (Re + 1e-5)
+    drop = 0.5 * rho_g * Vn * np.array([U, W]) * math.pi * L * L / 4 * cd / m
+    expect = frac[:, None] * plate[None, :] + (1 - frac[:, None]) * drop[None, :]
+    assert np.allclose(af[rear][:, :2], expect, rtol=1e-9, atol=0)
+
+
+# ------------------------------------ the reference's analytic validations
+
+def _validation_profiles():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "validation_profiles.json")) as f:
+        return json.load(f)
+
+
+def test_couette_startup_matches_the_reference_profile(po):
+    """Whole oracle run of the Couette example to t = 5 s against the
+    analytic start-up profile of Examples/Couette/SteadyState.dat (walls
+    no-slip; RMS 0.1% of the wall speed measured): viscosity, the moving
+    wall's velocity, no-slip walls and the frame clock together."""
+    from tests.test_gpu_validation import _profile_error, run_frames
+    prof = _validation_profiles()["couette"]["t=5s"]
+    case = cases.couette_2d()
+    for b in case.blocks[:2]:
+        b["no_slip"] = 1
+    case.settings.frame_time_interval = 1.0
+    o = po.Oracle(case, nthreads=8)
+    o.integrate_no_update()
+    x, v, _ = run_frames(o, 5.0, 1.0)[5.0]
+    rms, mx, _ = _profile_error(x, v, case.n_bound, np.array(prof["z"]), np.array(prof["v"]), 1e-3)
+    assert rms <= 5e-3 and mx <= 1e-2
+
+
+def test_poiseuille_startup_matches_the_reference_profile(po):
+    """Whole oracle run of the body-force-driven channel to t = 0.02 s
+    against Examples/Poiseuille/SteadyState.dat (RMS 1.6% of the peak
+    measured; the profile is young, its values near the wall small)."""
+    from tests.test_gpu_validation import _profile_error, run_frames
+    prof = _validation_profiles()["poiseuille"]["t=0.02s"]
+    case = cases.poiseuille_2d()
+    for b in case.blocks[:2]:
+        b["no_slip"] = 1
+    case.settings.frame_time_interval = 0.02
+    o = po.Oracle(case, nthreads=8)
+    o.integrate_no_update()
+    x, v, _ = run_frames(o, 0.02, 0.02)[0.02]
+    u = np.array(prof["v"])
+    rms, mx, _ = _profile_error(x, v, case.n_bound, np.array(prof["z"]), u, np.abs(u).max())
+    assert rms <= 3e-2 and mx <= 6e-2
