@@ -61,6 +61,12 @@ def workload(name, n_target):
         return cases.droplet_3d(precision=abi.PREC_MIXED), "3D droplet D=4 mm, dx=0.05 mm, 20 m/s crossflow"
     if name == "dam2d":
         return cases.dam_break_2d(), "2D dam break 1x2 m column, dx=0.02 m, FP64"
+    if name == "jicf":
+        c = cases.jet_in_crossflow_3d(n_target=1_000_000, precision=abi.PREC_MIXED)
+        desc = (f"3D jet in crossflow: D=1 mm pipe with ghost walls (2 layers) injecting at 5 m/s along +y, "
+                f"dx=0.03 mm, H=2dx, {c.n_columns} inlet columns, the jet issued to about 1M particles and "
+                "growing, Gissler drag in a 50 m/s crossflow")
+        return c, desc
     if name == "batched":
         c = cases.batched_droplets(precision=abi.PREC_MIXED)
         desc = ("256 independent 3D droplets, diameters log-uniform in [0.5, 2] mm, dx=0.05 mm, crossflow "
@@ -97,7 +103,9 @@ def cpu_baseline(args):
         c, _ = workload(args.workload, args.particles)
     o = Oracle(c, nthreads=nthreads, kind="baseline")
     o.integrate_no_update()
-    for _ in range(2):
+    # the GPU runs time steps warmup + 2 onwards (warm-up steps, then one
+    # step after the tile autotuner): the same window here
+    for _ in range(args.warmup + 1):
         o.integrate_step()
     res = {}
     for serial in (True, False):
@@ -113,23 +121,46 @@ def cpu_baseline(args):
         "unit": "particle-steps/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": (f"{c.name} with {c.n} particles (same dx, H and physics), FP64, {args.cpu_steps} timed steps "
-                   f"after start-up + 2 steps, mean {res[True][1]:.1f} sub-iterations/step; cell-list neighbour "
-                   "search, serial dissipation_terms as in the reference"),
+        "sample": (f"{c.name} with {c.n} particles (same dx, H and physics, constant gas velocity), FP64, "
+                   f"{args.cpu_steps} timed steps after start-up + {args.warmup + 1} steps, mean "
+                   f"{res[True][1]:.1f} sub-iterations/step; cell-list neighbour search, serial "
+                   "dissipation_terms as in the reference"),
+        "subiterations_per_step": res[True][1],
         "value_parallel_dissipation": res[False][0],
-        "cpu_model": _cpu_model(),
+        "subiterations_per_step_parallel_dissipation": res[False][1],
+        "host": _host(),
     }
 
 
-def _cpu_model():
+def _host():
+    """The CPU the baseline ran on: model string, logical CPUs listed and
+    usable (affinity mask), sockets/cores as /proc/cpuinfo gives them."""
+    info = {"nproc": os.cpu_count(), "affinity": None, "cpu_model": None, "logical_cpus_listed": 0,
+            "physical_ids": 0, "cores_per_socket": None}
     try:
+        aff = sorted(os.sched_getaffinity(0))
+        info["affinity"] = len(aff)
+        info["affinity_cpus"] = f"{aff[0]}-{aff[-1]}" if aff and aff[-1] - aff[0] + 1 == len(aff) else str(aff)
+    except (AttributeError, OSError):
+        pass
+    try:
+        phys = set()
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "processor":
+                    info["logical_cpus_listed"] += 1
+                elif key == "model name" and info["cpu_model"] is None:
+                    info["cpu_model"] = val
+                elif key == "physical id":
+                    phys.add(val)
+                elif key == "cpu cores" and info["cores_per_socket"] is None:
+                    info["cores_per_socket"] = int(val)
+        info["physical_ids"] = len(phys)
     except OSError:
         pass
-    return None
+    return info
 
 
 def copy_probe(nbytes):
@@ -144,10 +175,11 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
 
 
-def run_engine(case, args, label):
+def run_engine(case, args, label, autotune=True, block_targets=None, steps=None):
     """Warm up, then time `steps` steps; returns the measurements."""
     from fjsph_amd.engine import Engine
 
+    steps = steps or args.steps
     print(f"[bench] {label}: {case.n} particles; building engine", file=sys.stderr, flush=True)
     eng = Engine(case)
     grouped = getattr(case, "owner", None) is not None
@@ -156,15 +188,15 @@ def run_engine(case, args, label):
         # control, Newmark-Beta convergence and npd
         eng.set_groups(case.owner)
     eng.set_reorder_interval(args.reorder_every)
-    if args.block_targets:
-        eng.set_block_targets(args.block_targets)
+    if block_targets or args.block_targets:
+        eng.set_block_targets(block_targets or args.block_targets)
     if getattr(case, "surrogate", None) is not None:
         eng.set_surrogate(case.surrogate)
     eng.integrate_no_update()
     for _ in range(args.warmup):
         eng.integrate_step()
     tune = None
-    if not args.no_autotune:
+    if autotune and not args.no_autotune:
         # tile blocks chosen by the engine's autotuner on this state (untimed)
         best, ms = eng.autotune_blocks(AUTOTUNE_CANDIDATES, reps=2)
         tune = {"block_targets": best, "ms_per_candidate": {str(k): v for k, v in ms.items()}}
@@ -172,13 +204,19 @@ def run_engine(case, args, label):
         eng.integrate_step()
     eng.enable_timers(True)
     its, group_its = [], []
+    growing = bool(getattr(case, "inlets", None))
+    particle_steps = 0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
+        # a growing particle set (inlets) counts the particles each step starts with
+        particle_steps += eng.n if growing else case.n
         its.append(eng.integrate_step()["iterations"])
         if grouped:
             group_its.append([g["iterations"] for g in eng.group_info()])
     elapsed = time.perf_counter() - t0
-    r = {"elapsed": elapsed, "value": case.n * args.steps / elapsed, "its": float(np.mean(its)), "autotune": tune,
+    r = {"elapsed": elapsed, "value": particle_steps / elapsed, "its": float(np.mean(its)), "autotune": tune,
+         "particles_end": eng.n,
+         "steps": steps, "block_targets": eng.get_block_targets(),
          "timers": eng.timers(), "stats": eng.neighbour_stats(),
          "force_bpp": eng.pass_bytes(abi.TIMER_NAMES.index("force")),
          "pass_bytes": {k: eng.pass_bytes(abi.TIMER_NAMES.index(k)) for k in abi.TIMER_NAMES}}
@@ -223,6 +261,8 @@ def main():
     ap.add_argument("--cpu-particles", type=int, default=60_000)
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the constant-gas and FP64 runs and the error check (profiling runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the tile blocks as set (default 256 targets)")
     ap.add_argument("--block-targets", type=int, default=0,
                     help="targets per tile block before any autotuning (0: the engine's default, 256)")
@@ -239,21 +279,40 @@ def main():
         copy_probe(args.copy_probe)
         return
 
+    phases = {}
+    t0 = time.perf_counter()
     case, desc = workload(args.workload, args.particles)
+    phases["case_build_s"] = time.perf_counter() - t0
+    if args.workload == "jicf" and args.aero != "const":
+        # the configuration's mesh: a seeded Delaunay triangulation of 2M points in a 0.2 m box
+        args.mesh_kind = "delaunay"
+        args.mesh_points = args.mesh_points if args.mesh_points != 300_000 else 2_000_000
     aero_info = None
     if args.workload == "batched":
         args.aero = "const"  # each drop's crossflow is its own initial velocity in still gas
+    const_case = None
     if args.aero == "const":
         desc += ", gas velocity: the constant coflow velocity"
+        t0 = time.perf_counter()
         run = run_engine(case, args, "constant-velocity aero")
+        phases["engine_const_s"] = time.perf_counter() - t0
+        const_run = run
     else:
         from fjsph_amd import surrogate
+        const_case = _copy_case(case)
         t0 = time.perf_counter()
-        cases.with_mesh(case, n_points=args.mesh_points, speed=float(case.settings.v_inf[0]), kind=args.mesh_kind,
-                        n_cells=args.mesh_cells)
+        mesh_kw = dict(n_points=args.mesh_points, speed=float(case.settings.v_inf[0]), kind=args.mesh_kind,
+                       n_cells=args.mesh_cells)
+        if args.workload == "jicf":
+            mesh_kw["box"] = ((-0.1, -0.1, -0.1), (0.1, 0.1, 0.1))
+        cases.with_mesh(case, **mesh_kw)
         t_mesh = time.perf_counter() - t0
+        phases["mesh_build_s"] = t_mesh
         print(f"[bench] mesh: {case.mesh.ncells} cells ({t_mesh:.0f} s)", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
         mesh_run = run_engine(case, args, "mesh aero")
+        phases["engine_mesh_s"] = time.perf_counter() - t0
+
         def lookup_ms(r):
             ms, calls = r["timers"]["aero"]
             return ms / calls if calls else None
@@ -261,6 +320,7 @@ def main():
         aero_info = {"mesh_cells": int(case.mesh.ncells), "mesh_points": int(case.mesh.nverts),
                      "mesh_lookup_ms_per_call": lookup_ms(mesh_run),
                      "mesh_aero_value": mesh_run["value"], "mesh_aero_ms_per_step": mesh_run["elapsed"] / args.steps * 1e3,
+                     "mesh_subiterations_per_step": mesh_run["its"],
                      "mesh_lost": mesh_run.get("mesh_lost"), "aero_particles": mesh_run.get("aero_particles")}
         run = mesh_run
         if args.aero == "surrogate":
@@ -270,10 +330,13 @@ def main():
             lab.set_mesh(case.mesh)
             sg = surrogate.train(case.mesh, lab, n_samples=args.train_samples, steps=args.train_steps)
             lab.close()
-            aero_info["train_seconds"] = time.perf_counter() - t0
+            phases["surrogate_train_s"] = time.perf_counter() - t0
+            aero_info["train_seconds"] = phases["surrogate_train_s"]
             case.settings.aero_source = abi.SURROGATE
             case.surrogate = sg
+            t0 = time.perf_counter()
             run = run_engine(case, args, "surrogate aero")
+            phases["engine_surrogate_s"] = time.perf_counter() - t0
             chk = Engine(case.settings)
             chk.set_surrogate(sg)
             pred = chk.surrogate_eval(mesh_run["aero_x"])
@@ -294,6 +357,13 @@ def main():
                  f"{case.mesh.ncells}-cell {'structured tetrahedral' if args.mesh_kind == 'kuhn' else 'Delaunay'} "
                  f"mesh (potential flow past a cylinder, "
                  f"{case.settings.v_inf[0]:.0f} m/s)")
+        const_run = None
+        if not args.no_extra:
+            # the same state with the constant gas velocity the CPU sample uses
+            t0 = time.perf_counter()
+            const_run = run_engine(const_case, args, "constant-velocity aero (CPU-comparable)", autotune=False,
+                                   block_targets=run["block_targets"])
+            phases["engine_const_s"] = time.perf_counter() - t0
 
     timers = run["timers"]
     pairs, max_count = run["stats"]
@@ -311,41 +381,67 @@ def main():
             "alg_GBs": round(bpp * case.n / avg_s / 1e9, 1),
         }
 
-    # roofline of the dominant kernel: k_force_t (HIP events on the engine stream)
+    # roofline of the dominant kernel: k_force_t (HIP events on the engine stream).  The pair loop is
+    # bound by the FP32 vector pipe, not by HBM: 176 B per particle against ~240 pairs of ~150
+    # operations.  Both sides are reported; `bound` names the larger fraction.
     ms, launches = timers["force"]
     avg_s = (ms / 1e3) / max(launches, 1)
     bpp = run["force_bpp"]
-    achieved = bpp * case.n / avg_s / 1e9
+    achieved_gbs = bpp * case.n / avg_s / 1e9
     traffic, tmeta = load_traffic(case.n)
+    fpp = force_flops_per_pair(case.settings.dim, bool(case.settings.ale))
+    tflops = fpp * pairs / avg_s / 1e12
+    hbm = {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+           "alg_bytes_per_particle": bpp, "traffic": traffic,
+           "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tmeta else None,
+           "traffic_rate_GBs": traffic / avg_s / 1e9 if traffic else None}
+    valu = {"achieved": tflops, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tflops / FP32_VECTOR_PEAK_TFLOPS,
+            # the published FP32 peak counts packed (2-wide) FP32; the engine is built without packed
+            # FP32 instructions, whose single-issue peak is half of it
+            "peak_unpacked": FP32_VECTOR_PEAK_TFLOPS / 2, "frac_unpacked": tflops / (FP32_VECTOR_PEAK_TFLOPS / 2),
+            "flop_per_pair": fpp, "pairs_per_launch": pairs}
+    bound = "valu" if valu["frac"] >= hbm["frac"] else "hbm"
+    side = valu if bound == "valu" else hbm
     roofline = {
-        "bound": "hbm",
-        "achieved": achieved,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS,
+        "bound": bound,
+        "achieved": side["achieved"],
+        "peak": side["peak"],
+        "unit": side["unit"],
+        "frac": side["frac"],
         "traffic": traffic,
         "kernel": "k_force_t<3,float> (pair sums of get_acc_and_Rrho_on_i)",
-        "alg_bytes_per_particle": bpp,
         "avg_launch_ms": avg_s * 1e3,
         "launches": launches,
         "pairs_per_launch": pairs + case.n,
-        "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tmeta else None,
-        "traffic_rate_GBs": traffic / avg_s / 1e9 if traffic else None,
+        "hbm": hbm,
+        "valu": valu,
     }
-    # the pair loop is FP32-vector bound: its side of the roofline too
-    fpp = force_flops_per_pair(case.settings.dim, bool(case.settings.ale))
-    tflops = fpp * pairs / avg_s / 1e12
-    # the published FP32 peak counts packed (2-wide) FP32; the engine is built without packed FP32
-    # instructions, whose single-issue peak is half of it
-    roofline["valu"] = {"achieved": tflops, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": tflops / FP32_VECTOR_PEAK_TFLOPS, "peak_unpacked": FP32_VECTOR_PEAK_TFLOPS / 2,
-                        "frac_unpacked": tflops / (FP32_VECTOR_PEAK_TFLOPS / 2), "flop_per_pair": fpp,
-                        "pairs_per_launch": pairs}
+
+    # FP64 GPU run and the error of the mixed step against it (the FP64 engine is the oracle-pinned path)
+    fp64 = None
+    if not args.no_extra and args.workload == "jet4m":
+        t0 = time.perf_counter()
+        fp64 = fp64_check(const_case or case, args, run["block_targets"])
+        phases["fp64_and_error_s"] = time.perf_counter() - t0
 
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
         cpu = cpu_baseline(args)
+        phases["cpu_s"] = time.perf_counter() - t0
+        ref = const_run or run
+        cpu["like_for_like"] = {
+            "gpu_value_constant_gas": ref["value"],
+            "gpu_subiterations_per_step": ref["its"],
+            "cpu_subiterations_per_step": cpu["subiterations_per_step"],
+            "ratio": ref["value"] / cpu["value"],
+            "ratio_per_subiteration": (ref["value"] * ref["its"]) / (cpu["value"] * cpu["subiterations_per_step"]),
+            "ratio_fp64": (fp64["value"] / cpu["value"]) if fp64 else None,
+            "note": "both with the constant gas velocity; the GPU state is the 4M column, the CPU sample a "
+                    "short column of the same physics; ratio_per_subiteration compares force passes per second",
+        }
 
     value = run["value"]
     out = {
@@ -369,16 +465,63 @@ def main():
             "subiterations_per_step": run["its"],
             "groups": run.get("groups"),
             "aero": aero_info,
-            "tile_blocks": run.get("autotune") or {"block_targets": args.block_targets or 256, "autotuned": False},
+            "tile_blocks": run.get("autotune") or {"block_targets": run["block_targets"], "autotuned": False},
             "reorder_every": args.reorder_every,
             "passes": passes,
+            "constant_gas_value": const_run["value"] if const_run else None,
+            "phases_s": {k: round(v, 2) for k, v in phases.items()},
         },
+        "fp64_value": fp64["value"] if fp64 else None,
+        "oracle_error": fp64["oracle_error"] if fp64 else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     if cpu:
         out["config"]["speedup_vs_cpu"] = value / cpu["value"]
     print(json.dumps(out), flush=True)
+
+
+def _copy_case(case):
+    import copy
+    c = copy.copy(case)
+    c.settings = case.settings.copy()
+    return c
+
+
+def fp64_check(case, args, block_targets):
+    """The 4M state in FP64 on the GPU: its throughput (the like-for-like
+    precision of the FP64 CPU restatement), and the error of the mixed-
+    precision step against the FP64 one from the same state, normwise
+    (max |mixed - fp64| / max |fp64|) per field after the start-up pass and
+    one step.  The FP64 engine is the path pinned against the oracle."""
+    from fjsph_amd.engine import Engine
+    c64 = _copy_case(case)
+    c64.settings.precision = abi.PREC_FP64
+    cm = _copy_case(case)
+    cm.settings.precision = abi.PREC_MIXED
+    res = {}
+    infos = {}
+    for label, c in (("fp64", c64), ("mixed", cm)):
+        eng = Engine(c)
+        eng.set_block_targets(block_targets)
+        eng.integrate_no_update()
+        infos[label] = eng.integrate_step()
+        res[label] = {f: eng.field(f) for f in ("X", "V", "RHO", "ACC")}
+        eng.close()
+    err = {}
+    for f in ("X", "V", "RHO", "ACC"):
+        a, b = res["mixed"][f], res["fp64"][f]
+        if f == "X":
+            # positions: the error against the displacement of the step
+            x0 = c64.x
+            err[f] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b - x0)), 1e-300))
+        else:
+            err[f] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    run = run_engine(c64, args, "FP64", autotune=False, block_targets=block_targets, steps=max(2, args.steps // 2))
+    return {"value": run["value"], "its": run["its"],
+            "oracle_error": {"fields": err, "per": "one step from the same 4M state, normwise; X against the "
+                             "step's displacement", "subiterations": {"fp64": infos["fp64"]["iterations"],
+                                                                      "mixed": infos["mixed"]["iterations"]}}}
 
 
 if __name__ == "__main__":

@@ -467,8 +467,11 @@ def jet_in_crossflow_3d(n_target=1_000_000, diameter=1e-3, dx=3e-5, speed=5.0, n
     m/s along +x (constant; cases.with_mesh gives it a mesh).  The
     reference example's fluid radius is 4e-4 in a wall of radius 4.9e-4 at
     dx = 3e-5; here the wall sits one spacing outside the outermost column.
-    Host order: wall, pipe layers, back layer, buffer layers by depth, jet
-    layers."""
+    Host order: wall, jet layers (top first), pipe layers (top first), back
+    layer, buffer layers by depth: the particles a deletion plane removes
+    then come before the inlet's, as in the reference's own runs (it inserts
+    at the block's end and shifts the inlet's index lists by every deletion
+    in the block, src/Integration.cpp:186-205)."""
     s = abi.default_settings(3)
     s.particle_step = dx
     s.H_fac = H_fac
@@ -495,13 +498,13 @@ def jet_in_crossflow_3d(n_target=1_000_000, diameter=1e-3, dx=3e-5, speed=5.0, n
     wall = hollow_cylinder_3d(r + dx, (n_pipe + n_buf + 2) * dx, dx, wall_layers)
     if jet_layers is None:
         jet_layers = max(0, int(round((n_target - wall.shape[0]) / ncol)) - (n_pipe + 1 + n_buf))
-    pipe = np.concatenate([layer(-k * dx) for k in range(n_pipe - 1, -1, -1)])
+    jet = (np.concatenate([layer(k * dx) for k in range(jet_layers, 0, -1)]) if jet_layers
+           else np.zeros((0, 3)))
+    pipe = np.concatenate([layer(-k * dx) for k in range(n_pipe)])
     y_back = -n_pipe * dx
     back = layer(y_back)
     buf = np.concatenate([layer(y_back - k * dx) for k in range(1, n_buf + 1)])
-    jet = (np.concatenate([layer(k * dx) for k in range(1, jet_layers + 1)]) if jet_layers
-           else np.zeros((0, 3)))
-    pts = np.concatenate([pipe, back, buf, jet])
+    pts = np.concatenate([jet, pipe, back, buf])
     vel = np.zeros_like(pts)
     vel[:, 1] = speed
     c = _finish("jet_in_crossflow_3d", s, wall, pts, vel,
@@ -509,12 +512,12 @@ def jet_in_crossflow_3d(n_target=1_000_000, diameter=1e-3, dx=3e-5, speed=5.0, n
                 f"{n_pipe} pipe + {n_buf} buffer + {jet_layers} jet layers", bound_solver=abi.GHOST,
                 fluid_type=abi.PIPE)
     nb = wall.shape[0]
-    npipe = pipe.shape[0]
-    c.b[nb + npipe:nb + npipe + ncol] = abi.BACK
-    c.b[nb + npipe + ncol:nb + npipe + ncol + buf.shape[0]] = abi.BUFFER
-    c.b[nb + npipe + ncol + buf.shape[0]:] = abi.FREE
-    back_ids = np.arange(nb + npipe, nb + npipe + ncol)
-    buffer_ids = np.stack([nb + npipe + ncol + k * ncol + np.arange(ncol) for k in range(n_buf)], axis=1)
+    nj, npipe = jet.shape[0], pipe.shape[0]
+    c.b[nb:nb + nj] = abi.FREE
+    c.b[nb + nj + npipe:nb + nj + npipe + ncol] = abi.BACK
+    c.b[nb + nj + npipe + ncol:] = abi.BUFFER
+    back_ids = np.arange(nb + nj + npipe, nb + nj + npipe + ncol)
+    buffer_ids = np.stack([nb + nj + npipe + ncol + k * ncol + np.arange(ncol) for k in range(n_buf)], axis=1)
     planes = dict(inlet=1, fixed_vel_or_dynamic=mode,
                   insert=((0.0, 1.0, 0.0), y_back + back_to_insert * dx),
                   aero=((0.0, 1.0, 0.0), 0.5 * dx))

@@ -164,15 +164,21 @@ __device__ __forceinline__ void split_rel(double x, double ref, float& hi, float
 
 /* Lock-step walk of one target's rows: `row` advances in every lane of the
  * wave together; the next row is in flight while the current one is used.
- * Rows past nrows read as all-NO_ENTRY. */
+ * Rows past nrows read as all-NO_ENTRY.  Every lane with a list walks it,
+ * also a lane whose target the pass does not compute (`on` false): the
+ * build pads each tile's rows to the wave's largest count, and the wave
+ * leaves a tile only when no lane has an entry of it left, so a lane that
+ * dropped out could leave the others on padding rows of the tile. */
 struct Rows {
     int64_t p;
     int row, nrows;
+    bool on;
     uint64_t u, un;
-    __device__ __forceinline__ void init(const NbrList& nl, int64_t p_, int nrows_)
+    __device__ __forceinline__ void init(const NbrList& nl, int64_t p_, int nrows_, bool on_)
     {
         p = p_;
         row = 0;
+        on = on_;
         nrows = nrows_;
         u = nrows > 0 ? nl.lu4[p] : ~0ull;
         un = nrows > 1 ? nl.lu4[nl.npad + p] : ~0ull;
@@ -189,7 +195,7 @@ struct Rows {
  * runs body for every list entry of the lane's target in the union tile
  * [base, base + TILE), with `t` the tile-local index.  A row belongs to the
  * tile when its first index does; padding rows read NO_ENTRY first, so a
- * lane is idle in them.  The wave leaves the loop at the first row where
+ * lane is idle in them, as is a lane with `on` false.  The wave leaves the loop at the first row where
  * no lane has an entry of the tile, which is the first row of the next
  * tile in every lane.  The 4 entries of a row go through one copy of the
  * body (a rolled loop); the body is written inline, not as a closure, so
@@ -198,7 +204,7 @@ struct Rows {
     for (;;) {                                                                            \
         const bool fj_act = (uint32_t((cu).u) & 0xFFFFu) < uint32_t((base) + TILE);        \
         if (!__any(fj_act)) break;                                                        \
-        if (fj_act) {                                                                     \
+        if (fj_act && (cu).on) {                                                          \
             uint64_t fj_u = (cu).u;                                                       \
             _Pragma("unroll 1") for (int fj_q = 0; fj_q < 4; ++fj_q)                     \
             {                                                                             \
@@ -589,7 +595,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
         const double rhoi = active ? s.rho[i] : 0.0;
         double kernsum = c.Wc, colour = 0.0;
         Rows cu;
-        cu.init(nl, p, active ? nl.count[i] : 0);
+        cu.init(nl, p, active ? nl.count[i] : 0, active);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
@@ -673,7 +679,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_m(Consts c, DevState s, Aux ax, Ti
 #pragma unroll
         for (int a = 0; a < D; ++a) { gR[a] = 0.0; nr[a] = 0.0; fgR[a] = 0.f; fnr[a] = 0.f; }
         Rows cu;
-        cu.init(nl, p, (active && !slow) ? nl.count[i] : 0);
+        cu.init(nl, p, active ? nl.count[i] : 0, active && !slow);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
@@ -1057,7 +1063,7 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
         const T Bg = T(c.B * c.gam), cfac = T(sqrt(c.B * c.gam / rhoi)), Ht = T(c.H);
         const T eps2 = T(0.0001 * c.H_sq), alpha = T(c.visc_alpha);
         Rows cu;
-        cu.init(nl, p, fluid ? nl.count[i] : 0);
+        cu.init(nl, p, active ? nl.count[i] : 0, fluid);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
@@ -1144,8 +1150,9 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
 
 /* Mixed-precision form of k_surface1_t: the pair terms in FP32 on
  * coordinates relative to the block's first target (positions, velocities,
- * build positions) and densities relative to its density, every sum in
- * FP64; 64 B of LDS per candidate (80 B with build positions).
+ * build positions) and densities relative to its density, summed in FP32
+ * over each LDS tile and the tile sums in FP64; 64 B of LDS per candidate
+ * (80 B with build positions).
  *
  * The point-T test is decided in FP32 unless one of its comparisons lies
  * within a margin of its threshold (1e-3 h133 for lengths, 1e-3 for the
@@ -1198,6 +1205,11 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
         const int64_t i = active ? tg.vorder[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
         double nrm[D], av[D], rd = 0.0;
+        /* per-tile FP32 partial sums, added into the FP64 totals after
+         * each tile (about 40 pairs per tile and target) */
+        float fnrm[D], fav[D], frd = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) { fnrm[a] = 0.f; fav[a] = 0.f; }
         float xit[D], xlo[D], xbit[D], xblo[D], vit[D], vlo[D], gri[D], nhf[D], tauf[D], pTt[D];
         {
             double xi[D], ni[D], nhat[D], tau_hat[D];
@@ -1240,7 +1252,7 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
         const float rhoif = float(rhoi);
         const float cfac = float(sqrt(c.B * c.gam / rhoi));
         Rows cu;
-        cu.init(nl, p, fluid ? nl.count[i] : 0);
+        cu.init(nl, p, active ? nl.count[i] : 0, fluid);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
@@ -1329,7 +1341,7 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
                     }
                     const float f = A.w * (high ? (B.w - lamf) : B.w) * fg;
 #pragma unroll
-                    for (int a = 0; a < D; ++a) nrm[a] -= double(f * Rji[a]);
+                    for (int a = 0; a < D; ++a) fnrm[a] -= f * Rji[a];
                 }
                 if constexpr (with_diss) {
                     /* dissipation_terms pair term (src/Shifting.cpp:141-173) */
@@ -1354,17 +1366,26 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
                             const float ff =
                                 E.y * alpha * cbar * muij * fast_rcp<float>(0.5f * (rhoif + (rreff + G.w)));
 #pragma unroll
-                            for (int a = 0; a < D; ++a) av[a] += double(g[a] * ff);
+                            for (int a = 0; a < D; ++a) fav[a] += g[a] * ff;
                         }
                         float gsum = (gri[0] + grj[0]) * Rji[0];
 #pragma unroll
                         for (int a = 1; a < D; ++a) gsum += (gri[a] + grj[a]) * Rji[a];
-                        rd += double(A.w * (drho + 0.5f * gsum) * rdotg * idist2);
+                        frd += A.w * (drho + 0.5f * gsum) * rdotg * idist2;
                     } else {
-                        rd += double(A.w * drho * rdotg * idist2);
+                        frd += A.w * drho * rdotg * idist2;
                     }
                 }
             FJ_TILE_ENTRIES_END(cu, nl)
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                nrm[a] += double(fnrm[a]);
+                av[a] += double(fav[a]);
+                fnrm[a] = 0.f;
+                fav[a] = 0.f;
+            }
+            rd += double(frd);
+            frd = 0.f;
         }
         if (active) surface1_out<D, MODE>(c, s, i, fluid, bi, lam_nb, lami, found, nrm, av, rd, norms_tmp);
     }
@@ -1451,7 +1472,7 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
         double curve = 0.0, qmax = 0.0;
         int zone = (fluid && s.surf[i] == 1) ? 1 : 0;
         Rows cu;
-        cu.init(nl, p, fluid ? nl.count[i] : 0);
+        cu.init(nl, p, active ? nl.count[i] : 0, fluid);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
@@ -1573,7 +1594,7 @@ __global__ __launch_bounds__(TPB) void k_surface2_m(Consts c, DevState s, Aux ax
         float fcurve = 0.f, qmax = 0.f;
         bool zone = fluid && s.surf[i] == 1;
         Rows cu;
-        cu.init(nl, p, fluid ? nl.count[i] : 0);
+        cu.init(nl, p, active ? nl.count[i] : 0, fluid);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
@@ -1763,7 +1784,7 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
         }
         double maxU2 = 0.0, dmin = 2.0;
         Rows cu;
-        cu.init(nl, p, work ? nl.count[i] : 0);
+        cu.init(nl, p, active ? nl.count[i] : 0, work);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
@@ -1861,7 +1882,7 @@ __global__ __launch_bounds__(TPB) void k_shift_m(Consts c, DevState s, Aux ax, c
         float maxU2 = 0.f;
         double dmin = 2.0;
         Rows cu;
-        cu.init(nl, p, work ? nl.count[i] : 0);
+        cu.init(nl, p, active ? nl.count[i] : 0, work);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
@@ -2348,7 +2369,7 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
         for (int a = 0; a < D; ++a) acc[a] = 0.0;
         double Rrho_ = 0.0, Rrhoc_ = 0.0;
         Rows cu;
-        cu.init(nl, p, fluid ? nl.count[i] : 0);
+        cu.init(nl, p, active ? nl.count[i] : 0, fluid);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();

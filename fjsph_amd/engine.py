@@ -231,6 +231,11 @@ class Engine:
     def set_block_targets(self, targets):
         self._chk(self.lib.fjsph_set_block_targets(self.h, int(targets)), "set_block_targets")
 
+    def get_block_targets(self):
+        v = np.zeros(1, np.int32)
+        self._chk(self.lib.fjsph_get_block_targets(self.h, _ptr(v, _I32)), "get_block_targets")
+        return int(v[0])
+
     def set_reorder_interval(self, steps):
         """fjsph_set_reorder_interval: cell-order permutation every `steps` steps."""
         self._chk(self.lib.fjsph_set_reorder_interval(self.h, int(steps)), "set_reorder_interval")

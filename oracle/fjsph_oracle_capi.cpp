@@ -524,6 +524,36 @@ int orc_boundary_test(void* hp, const double* point, int32_t* out3)
     return FJSPH_OK;
 }
 
+/* FirstCell's boundary branch for one point and velocity (pinning tests):
+ * xv = (x[3], v[3]) in and out; out2 = (cross, del). */
+}  // extern "C"
+template <int D>
+int first_cell_t(Handle* h, double* xv, int32_t* out2)
+{
+    const MeshData& M = sim<D>(h).mesh;
+    Vec<D> x, v;
+    for (int a = 0; a < D; ++a) {
+        x.c[a] = xv[a];
+        v.c[a] = xv[3 + a];
+    }
+    const std::vector<int64_t> cand = knn_cells_padded(M, x.c, D == 3 ? 150 : 20);
+    const FirstCellResult r = first_cell_boundary<D>(M, cand, x, v);
+    for (int a = 0; a < D; ++a) {
+        xv[a] = x.c[a];
+        xv[3 + a] = v.c[a];
+    }
+    out2[0] = r.cross;
+    out2[1] = r.del ? 1 : 0;
+    return FJSPH_OK;
+}
+extern "C" {
+
+int orc_first_cell_boundary(void* hp, double* xv, int32_t* out2)
+{
+    Handle* h = static_cast<Handle*>(hp);
+    return h->dim == 2 ? first_cell_t<2>(h, xv, out2) : first_cell_t<3>(h, xv, out2);
+}
+
 /* get_line_intersection of one segment against one 2D edge. */
 int orc_line_intersection(void* hp, int64_t face, const double* p1, const double* p2)
 {

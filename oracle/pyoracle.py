@@ -83,6 +83,7 @@ def load(kind="oracle"):
         "orc_check_cell": (C.c_int, [vp, C.c_int64, _D]),
         "orc_mesh_lost_info": (C.c_int, [vp, _I64]),
         "orc_boundary_test": (C.c_int, [vp, _D, _I32]),
+        "orc_first_cell_boundary": (C.c_int, [vp, _D, _I32]),
         "orc_line_intersection": (C.c_int, [vp, C.c_int64, _D, _D]),
     }
     for name, (res, args) in sig.items():
@@ -266,6 +267,17 @@ class Oracle:
         out = np.zeros(3, np.int32)
         self._chk(self.lib.orc_boundary_test(self.h, _ptr(p), _ptr(out, _I32)), "boundary_test")
         return int(out[0]), bool(out[1]), bool(out[2])
+
+    def first_cell_boundary(self, x, v):
+        """FirstCell's boundary branch for one point: (x, v after it, cross,
+        delete)."""
+        xv = np.zeros(6)
+        xv[:len(x)] = x
+        xv[3:3 + len(v)] = v
+        out = np.zeros(2, np.int32)
+        self._chk(self.lib.orc_first_cell_boundary(self.h, _ptr(xv), _ptr(out, _I32)), "first_cell_boundary")
+        d = len(x)
+        return xv[:d].copy(), xv[3:3 + d].copy(), int(out[0]), bool(out[1])
 
     def line_intersection(self, face, p1, p2):
         a, b = np.zeros(3), np.zeros(3)

@@ -193,7 +193,7 @@ def test_ghost_near_inlet_gpu():
 def test_newmark_rollback_gpu():
     """Steps forced to 8x the stable time step (delta_t_min) diverge, roll
     back to pn and halve dt; the engine does the same as the oracle."""
-    base = _perturb(cases.dam_break_2d(dx=0.05), seed=14)
+    base = cases.dam_break_2d(dx=0.05)
     from fjsph_amd.engine import Engine
     from oracle.pyoracle import Oracle
     probe = Oracle(base, nthreads=8)

@@ -57,7 +57,11 @@ def _small_cases():
     dam = _perturb(cases.dam_break_2d(dx=0.05), seed=1)
     drop = _perturb(cases.droplet_3d(diameter=1.0e-3, dx=5e-5, precision=abi.PREC_FP64), seed=2)
     jet = _perturb(cases.jet_column_3d(n_target=6000, diameter=0.6e-3, dx=5e-5, precision=abi.PREC_FP64), seed=3)
-    return [("dam2d", dam), ("drop3d", drop), ("jet3d_ale", jet)]
+    # Hydrostatic column, H = 2 dx: 16-cell tile blocks whose waves mix wall
+    # lanes (no surface, shift or force terms of their own) with fluid lanes
+    # that have fewer rows than the wall lanes in some tiles
+    column = _perturb(cases.standing_column_2d(dx=0.02), seed=4)
+    return [("dam2d", dam), ("drop3d", drop), ("jet3d_ale", jet), ("column2d", column)]
 
 
 @pytest.mark.parametrize("name,case", _small_cases(), ids=lambda v: v if isinstance(v, str) else "")

@@ -635,3 +635,34 @@ def test_poiseuille_startup_matches_the_reference_profile(po):
     u = np.array(prof["v"])
     rms, mx, _ = _profile_error(x, v, case.n_bound, np.array(prof["z"]), u, np.abs(u).max())
     assert rms <= 3e-2 and mx <= 6e-2
+
+
+# ------------------------------------------------------------- FirstCell
+
+@pytest.mark.parametrize("wall", [False, True], ids=["far_field", "wall"])
+def test_first_cell_reflects_or_deletes(po, wall):
+    """FirstCell's branch (src/Containment.cpp:466-573) for points just
+    under a structured mesh, each crossing the face above it once: across a
+    far-field face the point is marked for deletion and left as it was;
+    across a wall face it is mirrored, v -> v - 2 (v.n) n with the face
+    normal along y, and moved onto the face's plane y = 0.54 dx."""
+    from tests.test_gpu_jicf import _first_cell_case
+    case = _first_cell_case(wall)
+    o = po.Oracle(case, nthreads=2)
+    dx = case.settings.particle_step
+    rng = np.random.default_rng(4)
+    # column positions of the jet, just under the mesh
+    cols = case.x[case.b == abi.BACK][:, [0, 2]]
+    for k in range(20):
+        xz = cols[rng.integers(cols.shape[0])]
+        x = np.array([xz[0], (0.54 - rng.uniform(0.005, 0.04)) * dx, xz[1]])
+        v = np.array(rng.uniform(-1, 1, 3)) + np.array([0.0, 5.0, 0.0])
+        x2, v2, cross, dele = o.first_cell_boundary(x, v)
+        assert cross == 1
+        if wall:
+            assert not dele
+            assert abs(x2[1] - 0.54 * dx) <= 1e-15 and x2[0] == x[0] and x2[2] == x[2]
+            assert v2[1] == -v[1] and v2[0] == v[0] and v2[2] == v[2]
+        else:
+            assert dele
+            assert np.array_equal(x2, x) and np.array_equal(v2, v)
