@@ -20,7 +20,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HDF5_ROOT = os.environ.get("FJSPH_HDF5_ROOT", "/opt/conda")
 SYSTEM_LIBDIR = "/usr/lib/x86_64-linux-gnu"
 SOURCES = ["kernels.hip", "pairs.hip", "mesh.hip", "surrogate.hip", "surrogate_train.hip", "inlet.hip", "ipt.hip",
-           "engine.cpp", "settings.cpp", "tau.cpp"]
+           "noalloc.hip", "engine.cpp", "settings.cpp", "tau.cpp"]
 NO_CONTRACT = {"settings.cpp", "ipt.hip"}  # operation order of the CPU restatement, no fused multiply-adds
 HOST_ONLY = {"settings.cpp", "tau.cpp"}  # plain C++, no device code
 ARCH = os.environ.get("FJSPH_OFFLOAD_ARCH", "gfx950")

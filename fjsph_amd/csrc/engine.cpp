@@ -1912,6 +1912,9 @@ fjsph_status fjsph_create(const fjsph_settings* s, int32_t device, fjsph_ctx** o
         HIPCHK(hipMalloc((void**)&ctx->sc, sizeof(DevScalars)));
         HIPCHK(hipMemsetAsync(ctx->sc, 0, sizeof(DevScalars), ctx->st));
         HIPCHK(hipHostMalloc((void**)&ctx->sc_host, sizeof(DevScalars), hipHostMallocDefault));
+        /* this device's address of the scratch array of the entry points
+         * that allocate nothing (once per device, the device current) */
+        HIPCHK(noalloc_init(device));
         fill_consts(ctx);
     });
     if (rc != FJSPH_OK) {
@@ -3140,6 +3143,68 @@ fjsph_status fjsph_surrogate_grad(fjsph_ctx* ctx, int32_t n_hidden, int32_t batc
         float s = 0.f;
         HIPCHK(hipMemcpy(&s, sse, 4, hipMemcpyDeviceToHost));
         if (loss) *loss = double(s) / (double(batch) * NO);
+    });
+}
+
+fjsph_status fjsph_surrogate_grad_noalloc(fjsph_ctx* ctx, int32_t n_hidden, int32_t batch, const float* xn,
+                                          const float* tn, const float* weights, const float* biases,
+                                          float* dweights, float* dbiases, double* loss)
+{
+    if (!ctx || n_hidden < 1 || n_hidden > 8 || batch <= 0 || batch % 64 != 0 || !xn || !tn || !weights ||
+        !biases || !dweights || !dbiases)
+        return FJSPH_ERR_ARG;
+    if (batch > FJSPH_NOALLOC_MAX_BATCH) {
+        set_err(ctx, "surrogate_grad_noalloc: batch %d exceeds the maximum %d of the scratch array", int(batch),
+                int(FJSPH_NOALLOC_MAX_BATCH));
+        return FJSPH_ERR_ARG;
+    }
+    return guarded(ctx, [&] {
+        unsigned char* base = (unsigned char*)noalloc_scratch(ctx->device);
+        if (!base) throw std::string("surrogate_grad_noalloc: the scratch array of this device was not set up");
+        const int D = ctx->s.dim, NO = D + 2, L = n_hidden;
+        const int64_t nwaves = batch / 64;
+        MlpTrain m{};
+        m.n_hidden = L;
+        m.batch = batch;
+        m.nw = mlp_nw(D, L);
+        m.nb = mlp_nb(D, L);
+        /* carve the scratch array (256-byte aligned pieces) */
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            unsigned char* p = base + off;
+            off += (bytes + 255) / 256 * 256;
+            return (void*)p;
+        };
+        float* X = (float*)carve(size_t(batch) * 4 * sizeof(float));
+        float* T = (float*)carve(size_t(batch) * 8 * sizeof(float));
+        int32_t* idx = (int32_t*)carve(size_t(batch) * sizeof(int32_t));
+        m.Z = (float*)carve(size_t(L) * batch * 64 * sizeof(float));
+        m.H = (__bf16*)carve(size_t(L + 1) * batch * 64 * sizeof(__bf16));
+        m.dWp = (float*)carve(size_t(nwaves) * m.nw * sizeof(float));
+        m.dbp = (float*)carve(size_t(nwaves) * m.nb * sizeof(float));
+        m.lossp = (float*)carve(size_t(nwaves) * sizeof(float));
+        float* sse = (float*)carve(sizeof(float));
+        void* wf = carve(size_t(mlp_tiles_fwd(L)) * 64 * 8 * sizeof(uint16_t));
+        void* wt = carve(size_t(mlp_tiles_bwd(L)) * 64 * 8 * sizeof(uint16_t));
+        if (off > kNoallocScratchBytes) throw std::string("surrogate_grad_noalloc: scratch array too small");
+        /* every scratch word the kernels read is written first in this call */
+        launch_mlp_stage(xn, tn, batch, D, X, T, idx, ctx->st);
+        launch_mlp_pack(D, weights, L, wf, wt, ctx->st);
+        m.X = X;
+        m.T = T;
+        m.idx = idx;
+        m.wf = wf;
+        m.wt = wt;
+        m.b = biases;
+        m.bo = biases + int64_t(L) * 64;
+        launch_mlp_train(D, m, ctx->st);
+        launch_mlp_reduce(m.dWp, nwaves, m.nw, m.nw, dweights, ctx->st);
+        launch_mlp_reduce(m.dbp, nwaves, m.nb, m.nb, dbiases, ctx->st);
+        if (loss) {
+            launch_mlp_reduce(m.lossp, nwaves, 1, 1, sse, ctx->st);
+            launch_mlp_loss(sse, double(batch) * NO, loss, ctx->st);
+        }
+        sync(ctx);
     });
 }
 

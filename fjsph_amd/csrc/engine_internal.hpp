@@ -326,6 +326,15 @@ void launch_mlp_adam(float* p, const float* g, float* m1, float* m2, int64_t n, 
 int64_t mlp_tiles_fwd(int n_hidden);
 int64_t mlp_tiles_bwd(int n_hidden);
 void launch_mlp_pack(int dim, const float* w, int32_t n_hidden, void* wf, void* wt, hipStream_t st);
+/* Scratch of the entry points that allocate nothing (noalloc.hip): a
+ * file-scope device array per GPU, its address taken once per device by
+ * noalloc_init (with that device current) and returned by noalloc_scratch. */
+constexpr size_t kNoallocScratchBytes = FJSPH_NOALLOC_SCRATCH_BYTES;
+hipError_t noalloc_init(int device);
+void* noalloc_scratch(int device);
+void launch_mlp_stage(const float* xn, const float* tn, int32_t batch, int32_t dim, float* X, float* T, int32_t* idx,
+                      hipStream_t st);
+void launch_mlp_loss(const float* sse, double denom, double* loss, hipStream_t st);
 /* The mesh field at points: containing cell (-1 if none) and (vel, p, rho). */
 void launch_mesh_eval(int dim, DevMesh m, int64_t n, const double* pts, int32_t* cell, double* vals, hipStream_t st);
 void launch_ipt_track(int dim, const DevMesh& m, const IptParams& p, int64_t n, const IptStart* in,

@@ -72,6 +72,8 @@ SIGNATURES = {
     "fjsph_surrogate_grad": (C.c_int, [_VP, C.c_int32, C.c_int32, _F32, _F32, _F32, _F32, _F32, _F32, _D]),
     "fjsph_set_groups": (C.c_int, [_VP, C.c_int32, _I32]),
     "fjsph_get_group_info": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_void_p]),
+    # device pointers (no allocation inside)
+    "fjsph_surrogate_grad_noalloc": (C.c_int, [_VP, C.c_int32, C.c_int32, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "fjsph_set_block_planes": (C.c_int, [_VP, C.c_int32, C.POINTER(abi.BlockPlanes)]),
     "fjsph_set_inlet": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, _I64, _I64]),
     "fjsph_set_max_points": (C.c_int, [_VP, C.c_int64]),

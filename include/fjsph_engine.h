@@ -527,6 +527,27 @@ fjsph_status fjsph_surrogate_grad(fjsph_ctx* ctx, int32_t n_hidden, int32_t batc
                                   const float* weights, const float* biases, float* dweights, float* dbiases,
                                   double* loss);
 
+/* Largest batch of fjsph_surrogate_grad_noalloc, and the size of the
+ * device scratch array it works in (one per GPU, held until the process
+ * ends). */
+#define FJSPH_NOALLOC_MAX_BATCH 4096
+#define FJSPH_NOALLOC_SCRATCH_BYTES (24u << 20)
+
+/* fjsph_surrogate_grad without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving or
+ * training loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same results as fjsph_surrogate_grad, but every pointer is
+ * DEVICE memory: xn (batch*dim), tn (batch*(dim+2)), weights, biases in;
+ * dweights, dbiases and *loss (may be NULL) out.  The activations, the
+ * per-wave gradient partials and the packed weight tiles live in a
+ * file-scope device array of FJSPH_NOALLOC_SCRATCH_BYTES; batch must be a
+ * multiple of 64 and at most FJSPH_NOALLOC_MAX_BATCH (FJSPH_ERR_ARG
+ * otherwise), n_hidden 1..8.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (several contexts or threads) race.
+ * Synchronous on the context stream, like the other calls. */
+fjsph_status fjsph_surrogate_grad_noalloc(fjsph_ctx* ctx, int32_t n_hidden, int32_t batch, const float* xn,
+                                          const float* tn, const float* weights, const float* biases,
+                                          float* dweights, float* dbiases, double* loss);
 
 /* ---- measurement ---------------------------------------------------------- */
 
