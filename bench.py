@@ -9,7 +9,7 @@ per second = particles x steps / wall time, with the particle state resident
 in HBM when the timed region starts.
 
 Prints ONE JSON line with the metric, the roofline of the dominant kernel
-(k_force_t, the pair sums of get_acc_and_Rrho), per-pass timings and the CPU
+(k_force_m, the pair sums of get_acc_and_Rrho), per-pass timings and the CPU
 baseline (the FP64 restatement built with the reference's compiler flags,
 timed on the host cores on a bounded sample of the same workload).
 
@@ -35,17 +35,20 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 
 
 def force_flops_per_pair(dim, ale):
-    """FP32 operations of one pair in k_force_t (FMA = 2), frozen-distance
-    form: R_ji and v_ji with the target's rounding residual (4D), build-
-    position difference (2D), r^2 (2D-1), sqrt (1), V_j (1), 1/(r^2+eps)
-    (2), Wendland W and W'/r (11), gradient (D), pressure factor (2), r.g
-    (2D-1), viscous factor (5), surface tension (7), momentum sum (5D);
-    with ALE the five ALE dot products (16D), the ALE momentum terms (7D)
-    and the two density sums (4); without it the continuity term (2D+1)."""
-    base = 4 * dim + 2 * dim + (2 * dim - 1) + 1 + 1 + 2 + 11 + dim + 2 + (2 * dim - 1) + 5 + 7 + 5 * dim
-    return base + (16 * dim + 7 * dim + 4 if ale else 2 * dim + 1)
+    """FP32 operations of one pair in k_force_m (FMA = 2) as it evaluates
+    them, frozen-distance form: R_ji and v_ji with the target's rounding
+    residual (4D), build-position difference (2D), r^2 (2D-1), sqrt (1), V_j
+    (1), 1/(r^2+eps) (2), W'/r (6), gradient (D), pressure factor (2), r.g and
+    v.g (2(2D-1)), viscous factor (5), surface tension (6), momentum sums
+    (6D); with ALE vPert_j.g (2D-1), its share of the viscous factor (2), the
+    two density sums (6) and sum_j V_j g (2D); without it the continuity sum
+    (2)."""
+    base = 4 * dim + 2 * dim + (2 * dim - 1) + 1 + 1 + 2 + 6 + dim + 2 + 2 * (2 * dim - 1) + 5 + 6 + 6 * dim
+    return base + ((2 * dim - 1) + 2 + 6 + 2 * dim if ale else 2)
+
+
 METRIC = "particle-steps/s and %HBM roofline at 4M 3D particles; surrogate vs mesh aero"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_t_jet4m_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_force_m_jet4m_traffic.json")
 
 
 AUTOTUNE_CANDIDATES = (128, 192, 256, 384, 512, 768, 1024)
@@ -77,7 +80,7 @@ def workload(name, n_target):
 
 
 def load_traffic(n):
-    """HBM bytes per k_force_t launch from the committed rocprofv3 --pmc
+    """HBM bytes per k_force_m launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
     rescaled to n particles."""
     try:
@@ -381,8 +384,8 @@ def main():
             "alg_GBs": round(bpp * case.n / avg_s / 1e9, 1),
         }
 
-    # roofline of the dominant kernel: k_force_t (HIP events on the engine stream).  The pair loop is
-    # bound by the FP32 vector pipe, not by HBM: 176 B per particle against ~240 pairs of ~150
+    # roofline of the dominant kernel: k_force_m (HIP events on the engine stream).  The pair loop is
+    # bound by the FP32 vector pipe, not by HBM: 176 B per particle against ~240 pairs of ~100
     # operations.  Both sides are reported; `bound` names the larger fraction.
     ms, launches = timers["force"]
     avg_s = (ms / 1e3) / max(launches, 1)
@@ -410,7 +413,7 @@ def main():
         "unit": side["unit"],
         "frac": side["frac"],
         "traffic": traffic,
-        "kernel": "k_force_t<3,float> (pair sums of get_acc_and_Rrho_on_i)",
+        "kernel": "k_force_m<3,FR,ALE> (pair sums of get_acc_and_Rrho_on_i, mixed precision)",
         "avg_launch_ms": avg_s * 1e3,
         "launches": launches,
         "pairs_per_launch": pairs + case.n,

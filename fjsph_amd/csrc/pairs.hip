@@ -128,11 +128,13 @@ __device__ __forceinline__ int32_t block_group(const TileGrid& tg, int64_t b)
     return tg.lgrp[layer];
 }
 
-/* Shared block prologue: union table in LDS, uniform p0/p1/U. */
+/* Shared block prologue: union table in LDS, uniform p0/p1/U, and the
+ * block b the workgroup runs. */
 struct BlockHead {
     int32_t seg_a[MAXSEG];
     int32_t pre[MAXSEG + 1];
     int32_t nseg, p0, p1;
+    int64_t b;
 };
 
 template <int D>
@@ -140,7 +142,11 @@ __device__ __forceinline__ void block_head(const TileGrid& tg, BlockHead& h)
 {
     if (threadIdx.x == 0) {
         int nseg, p0, p1;
-        union_of_block<D>(tg, blockIdx.x, h.seg_a, h.pre, nseg, p0, p1);
+        /* workgroup w runs block w: dispatch order is cell order, so the
+         * blocks in flight cover a band of neighbouring cell rows */
+        const int64_t b = blockIdx.x;
+        h.b = b;
+        union_of_block<D>(tg, b, h.seg_a, h.pre, nseg, p0, p1);
         h.nseg = nseg;
         h.p0 = p0;
         h.p1 = p1;
@@ -259,7 +265,7 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
     if (tg.gbuild) {
         /* a partial build (independent simulations): other groups keep
          * their lists */
-        const int32_t g = block_group<D>(tg, blockIdx.x);
+        const int32_t g = block_group<D>(tg, h.b);
         if (g < 0 || !tg.gbuild[g]) return;
     }
     if (NT * TILE > int(NO_ENTRY)) {
@@ -623,7 +629,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
         if (active) dsph_out<D>(s, ax, i, Lm, Lnb, gR, nr, kernsum, colour);
     }
     const double bs = block_reduce(npd_acc, OpSum(), lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+    if (threadIdx.x == 0) partial[h.b] = bs;
 }
 
 /* Mixed-precision form of k_dsph_t: FP32 pair terms on coordinates relative
@@ -791,7 +797,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_m(Consts c, DevState s, Aux ax, Ti
         if (active) dsph_out<D>(s, ax, i, Lm, Lnb, gR, nr, kernsum, colour);
     }
     const double bs = block_reduce(npd_acc, OpSum(), lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+    if (threadIdx.x == 0) partial[h.b] = bs;
 }
 
 /* Per-slot tail of dSPH_PreStep: L^-1 (column-pivoting QR, identity when
@@ -2262,106 +2268,72 @@ __device__ void skin_friction(const Consts& c, const double* Vd, const double* n
 }
 
 
-/* Occupancy request for the force pass: 5 waves per SIMD (at most 102
+/* Occupancy request for the force passes: 5 waves per SIMD (at most 102
  * VGPRs) measured faster than the compiler's 4 at 97 VGPRs. */
 #ifndef FJ_FORCE_OCC
 #define FJ_FORCE_OCC __attribute__((amdgpu_waves_per_eu(5, 8)))
 #endif
 
-/* LDS tile of the force pass, one record per union slot.
- * FP32 (mixed path), four float4 arrays, four LDS reads per pair:
- *   A {x - xref, m (negative for BOUND)}, B {v - vref, p/rho^2},
- *   X {xb - xbref (build positions, FR only), 1/rho}, E {vPert, rho};
- *   V_j = |m_j| / rho_j is formed as |m_j| * (1/rho_j).
- * FP64 (parity path), one array per value: x, m, v, p/rho^2, xb (FR), vPert,
- * rho; 1/rho and m/rho are recomputed per pair with the same operations as
- * launch_aux, so they are bit-identical to the stored ones. */
-template <typename T, bool FR>
+/* LDS tile of the FP64 force pass (parity path), one array per value: x, m,
+ * v, p/rho^2, xb (FR), vPert, rho; 1/rho and m/rho are recomputed per pair
+ * with the same operations as launch_aux, so they are bit-identical to the
+ * stored ones. */
+template <bool FR>
 struct ForceTile {
     double P[8][TILE];
     double XB[3][FR ? TILE : 1];
     double EA[4][TILE];
 };
-template <bool FR>
-struct ForceTile<float, FR> {
-    float4 A[TILE], B[TILE], X[TILE], E[TILE];
-};
 
 /* Pair sums of get_acc_and_Rrho_on_i (src/Resid.cpp:243-418) for fluid
- * slots: writes the pair sum of the acceleration to acc (the aerodynamic,
- * artificial-viscosity and gravity terms are added by k_nb_update) and the
- * full Rrho.  Pair terms are formed in T; every sum is accumulated in FP64.
- * The pressure, viscous and surface-tension terms of a pair are combined
- * before accumulation: acc_i = sum_j (visc + ST / m_i - m_j (P_i + P_j) g).
+ * slots, in FP64: writes the pair sum of the acceleration to acc (the
+ * aerodynamic, artificial-viscosity and gravity terms are added by
+ * k_nb_update) and the full Rrho.  The pressure, viscous and
+ * surface-tension terms of a pair are combined before accumulation:
+ * acc_i = sum_j (visc + ST / m_i - m_j (P_i + P_j) g).
  * FR: the positions have moved since the build, r comes from Aux::xb. */
-template <int D, typename T, bool FR>
+template <int D, bool FR>
 __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState s, Aux ax,
                                                               const int32_t* __restrict__ part_id, TileGrid tg,
                                                               NbrList nl, const DevScalars* sc)
 {
-    __shared__ ForceTile<T, FR> ft;
+    __shared__ ForceTile<FR> ft;
     __shared__ BlockHead h;
-    constexpr bool F32 = sizeof(T) == 4;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
     const int NT = n_tiles(U);
-    /* reference point of the block (its first target) for the FP32
-     * relative coordinates */
-    double xref[D], vref[D], xbref[D];
-    {
-        const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            xref[a] = F32 ? s.x[i0 * D + a] : 0.0;
-            vref[a] = F32 ? s.v[i0 * D + a] : 0.0;
-            xbref[a] = (F32 && FR) ? ax.xb[i0 * D + a] : 0.0;
-        }
-    }
     double npd = sc->npd;
     if (tg.lgrp) {
         /* independent simulations: a block holds one group's targets; a
          * group that has converged keeps its rates */
-        const int32_t g = block_group<D>(tg, blockIdx.x);
+        const int32_t g = block_group<D>(tg, h.b);
         if (g < 0 || !tg.gactive[g]) return;
         npd = tg.gnpd[g];
     }
-    const T ste = T(-(0.5 * c.sig / c.lam_st) / (npd * npd));
-    const T pi3o4H = T(3.0 * kPi / 4.0 / c.H);
-    const T inv_H = T(1.0 / c.H), Ht = T(c.H), Wc = T(c.Wc);
-    const T eps2 = T(0.001 * c.H_sq);
-    const T nu = T(c.nu);
-    const T contang_fac = T(1.0 + 0.5 * cos(0.5 * kPi * 7.0 / 9.0));
+    const double ste = -(0.5 * c.sig / c.lam_st) / (npd * npd);
+    const double pi3o4H = 3.0 * kPi / 4.0 / c.H;
+    const double inv_H = 1.0 / c.H, Ht = c.H, Wc = c.Wc;
+    const double eps2 = 0.001 * c.H_sq;
+    const double nu = c.nu;
+    const double contang_fac = 1.0 + 0.5 * cos(0.5 * kPi * 7.0 / 9.0);
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
         const int64_t i = active ? tg.vorder[p] : 0;
         const bool fluid = active && part_id[i] >= c.n_bound;
-        double xi[D], vi[D], xbi[D];
-        T xit[D], vit[D], xbit[D], vpi[D];
-        float xlo[D], vlo[D], xblo[D]; /* FP32 rounding residuals (split_rel) */
+        double xi[D], vi[D], xbi[D], vpi[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            xi[a] = active ? s.x[i * D + a] : xref[a];
-            vi[a] = active ? s.v[i * D + a] : vref[a];
-            xbi[a] = (FR && active) ? ax.xb[i * D + a] : xbref[a];
-            if constexpr (F32) {
-                split_rel(xi[a], xref[a], xit[a], xlo[a]);
-                split_rel(vi[a], vref[a], vit[a], vlo[a]);
-                split_rel(xbi[a], xbref[a], xbit[a], xblo[a]);
-            } else {
-                xit[a] = xi[a];
-                vit[a] = vi[a];
-                xbit[a] = xbi[a];
-                xlo[a] = vlo[a] = xblo[a] = 0.f;
-            }
-            vpi[a] = (fluid && c.ale) ? T(s.vPert[i * D + a]) : T(0);
+            xi[a] = active ? s.x[i * D + a] : 0.0;
+            vi[a] = active ? s.v[i * D + a] : 0.0;
+            xbi[a] = (FR && active) ? ax.xb[i * D + a] : 0.0;
+            vpi[a] = (fluid && c.ale) ? s.vPert[i * D + a] : 0.0;
         }
         const double rhoi = fluid ? s.rho[i] : 1.0, mi = fluid ? s.m[i] : 1.0;
         const int32_t bi = fluid ? s.b[i] : 0;
-        const T Pi = fluid ? T(ax.P[i]) : T(0);
-        const T iri = fluid ? T(ax.invrho[i]) : T(0);
-        const T inv_mi = T(1.0 / mi);
-        const T rhoit = T(rhoi);
+        const double Pi = fluid ? ax.P[i] : 0.0;
+        const double iri = fluid ? ax.invrho[i] : 0.0;
+        const double inv_mi = 1.0 / mi;
         const bool st_on = (!c.ale) || (fluid && s.surfzone[i] == 1);
         const bool bound_i = bi == FJSPH_BOUND;
         double acc[D];
@@ -2377,139 +2349,81 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
                 const int u = tile_union_index(tk, t, NT);
                 if (u >= U) continue;
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
-                if constexpr (F32) {
-                    float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
-                        va[a] = float(s.v[(int64_t)sl * D + a] - vref[a]);
-                        if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
-                    }
-                    const float4 q = ax.fq[sl];
-                    ft.A[t] = make_float4(pa[0], pa[1], pa[2], q.x);
-                    ft.B[t] = make_float4(va[0], va[1], va[2], q.y);
-                    ft.X[t] = make_float4(xa[0], xa[1], xa[2], q.z);
-                    if (c.ale) ft.E[t] = ax.fale[sl];
-                } else {
-#pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        ft.P[a][t] = s.x[(int64_t)sl * D + a];
-                        ft.P[4 + a][t] = s.v[(int64_t)sl * D + a];
-                        if constexpr (FR) ft.XB[FR ? a : 0][FR ? t : 0] = ax.xb[(int64_t)sl * D + a];
-                        ft.EA[a][t] = c.ale ? s.vPert[(int64_t)sl * D + a] : 0.0;
-                    }
-                    ft.P[3][t] = s.b[sl] == FJSPH_BOUND ? -s.m[sl] : s.m[sl];
-                    ft.P[7][t] = ax.P[sl];
-                    ft.EA[3][t] = s.rho[sl];
+                for (int a = 0; a < D; ++a) {
+                    ft.P[a][t] = s.x[(int64_t)sl * D + a];
+                    ft.P[4 + a][t] = s.v[(int64_t)sl * D + a];
+                    if constexpr (FR) ft.XB[FR ? a : 0][FR ? t : 0] = ax.xb[(int64_t)sl * D + a];
+                    ft.EA[a][t] = c.ale ? s.vPert[(int64_t)sl * D + a] : 0.0;
                 }
+                ft.P[3][t] = s.b[sl] == FJSPH_BOUND ? -s.m[sl] : s.m[sl];
+                ft.P[7][t] = ax.P[sl];
+                ft.EA[3][t] = s.rho[sl];
             }
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
-                T Rji[D], Vji[D], vjr[3], msg, Pj, irj, volj, r, rr, rhoj, vpj[3];
-                if constexpr (F32) {
-                    const float4 A = ft.A[t], B = ft.B[t], X = ft.X[t];
-                    const float pj[3] = {A.x, A.y, A.z}, vj[3] = {B.x, B.y, B.z}, xbj[3] = {X.x, X.y, X.z};
-                    msg = A.w;
-                    Pj = B.w;
-                    irj = X.w;
+                double Rji[D], Vji[D], vjr[D], vpj[D], xj[D], d2;
 #pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        Rji[a] = (pj[a] - xit[a]) - xlo[a];
-                        Vji[a] = (vj[a] - vit[a]) - vlo[a];
-                        vjr[a] = vj[a];
-                    }
-                    if constexpr (FR) {
-                        T Rb[D];
-#pragma unroll
-                        for (int a = 0; a < D; ++a) Rb[a] = (xbj[a] - xbit[a]) - xblo[a];
-                        rr = Rb[0] * Rb[0];
-#pragma unroll
-                        for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
-                    } else {
-                        rr = Rji[0] * Rji[0];
-#pragma unroll
-                        for (int a = 1; a < D; ++a) rr += Rji[a] * Rji[a];
-                    }
-                    r = fast_sqrt<T>(rr);
-                    volj = fabsf(msg) * irj;
-                    if (c.ale) {
-                        const float4 E = ft.E[t];
-                        vpj[0] = E.x; vpj[1] = E.y; vpj[2] = E.z;
-                        rhoj = E.w;
-                    } else {
-                        vpj[0] = vpj[1] = vpj[2] = T(0);
-                        rhoj = T(0);
-                    }
-                } else {
-                    double xj[D], d2;
-#pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        xj[a] = ft.P[a][t];
-                        Rji[a] = T(xj[a] - xi[a]);
-                        Vji[a] = T(ft.P[4 + a][t] - vi[a]);
-                        vjr[a] = T(ft.P[4 + a][t]);
-                        vpj[a] = T(ft.EA[a][t]);
-                    }
-                    if constexpr (FR) {
-                        double xbj[D];
-#pragma unroll
-                        for (int a = 0; a < D; ++a) xbj[a] = ft.XB[FR ? a : 0][FR ? t : 0];
-                        d2 = exact_dist2<D>(xbi, xbj);
-                    } else {
-                        d2 = exact_dist2<D>(xi, xj);
-                    }
-                    rr = T(d2);
-                    r = T(sqrt(d2));
-                    msg = T(ft.P[3][t]);
-                    Pj = T(ft.P[7][t]);
-                    const double rj = ft.EA[3][t];
-                    rhoj = T(rj);
-                    irj = T(1.0 / rj);
-                    volj = T(fabs(double(msg)) / rj);
+                for (int a = 0; a < D; ++a) {
+                    xj[a] = ft.P[a][t];
+                    Rji[a] = xj[a] - xi[a];
+                    Vji[a] = ft.P[4 + a][t] - vi[a];
+                    vjr[a] = ft.P[4 + a][t];
+                    vpj[a] = ft.EA[a][t];
                 }
-                const bool bound_j = msg < T(0);
-                const T mj = bound_j ? -msg : msg;
-                const T idist2 = fast_rcp<T>(rr + eps2);
-                T W, fg;
-                wendland<T>(r, Ht, inv_H, Wc, W, fg);
-                T g[D];
+                if constexpr (FR) {
+                    double xbj[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) xbj[a] = ft.XB[FR ? a : 0][FR ? t : 0];
+                    d2 = exact_dist2<D>(xbi, xbj);
+                } else {
+                    d2 = exact_dist2<D>(xi, xj);
+                }
+                const double rr = d2, r = sqrt(d2);
+                const double msg = ft.P[3][t];
+                const double Pj = ft.P[7][t];
+                const double rhoj = ft.EA[3][t];
+                const double irj = 1.0 / rhoj;
+                const double volj = fabs(msg) / rhoj;
+                const bool bound_j = msg < 0.0;
+                const double mj = bound_j ? -msg : msg;
+                const double idist2 = 1.0 / (rr + eps2);
+                double W, fg;
+                wendland<double>(r, Ht, inv_H, Wc, W, fg);
+                double g[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) g[a] = fg * Rji[a];
-                const T pc = mj * (Pi + Pj);
-                T rdotg = Rji[0] * g[0];
+                const double pc = mj * (Pi + Pj);
+                double rdotg = Rji[0] * g[0];
 #pragma unroll
                 for (int a = 1; a < D; ++a) rdotg += Rji[a] * g[a];
-                const T vf = mj * nu * (iri + irj) * rdotg * idist2;
-                T sf = T(0);
+                const double vf = mj * nu * (iri + irj) * rdotg * idist2;
+                double sf = 0.0;
                 if (st_on) {
-                    const T fac = (bound_i || bound_j) ? contang_fac : T(1);
-                    sf = ste * fac * fast_cos<T>(pi3o4H * r) * fast_rcp<T>(r) * inv_mi;
+                    const double fac = (bound_i || bound_j) ? contang_fac : 1.0;
+                    sf = ste * fac * cos(pi3o4H * r) * (1.0 / r) * inv_mi;
                 }
 #pragma unroll
-                for (int a = 0; a < D; ++a) acc[a] += double(vf * Vji[a] + sf * Rji[a] - pc * g[a]);
+                for (int a = 0; a < D; ++a) acc[a] += vf * Vji[a] + sf * Rji[a] - pc * g[a];
                 if (c.ale) {
-                    T aj = T(0), ai = T(0), cd = T(0), cont = T(0), c2 = T(0);
+                    double aj = 0.0, ai = 0.0, cd = 0.0, cont = 0.0, c2 = 0.0;
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
                         aj += vpj[a] * g[a];
                         ai += vpi[a] * g[a];
                         cd += (vpj[a] - vpi[a]) * g[a];
                         cont += (Vji[a] + (vpj[a] - vpi[a])) * g[a];
-                        c2 += (rhoj * vpj[a] + rhoit * vpi[a]) * g[a];
+                        c2 += (rhoj * vpj[a] + rhoi * vpi[a]) * g[a];
                     }
 #pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        const T via = T(vi[a]);
-                        const T vja = F32 ? (T(vref[a]) + vjr[a]) : vjr[a];
-                        acc[a] += double((vja * aj + via * ai - via * cd) * volj);
-                    }
-                    Rrho_ -= double(cont * volj);
-                    Rrhoc_ += double(c2 * volj);
+                    for (int a = 0; a < D; ++a) acc[a] += (vjr[a] * aj + vi[a] * ai - vi[a] * cd) * volj;
+                    Rrho_ -= cont * volj;
+                    Rrhoc_ += c2 * volj;
                 } else {
-                    T vg = Vji[0] * g[0];
+                    double vg = Vji[0] * g[0];
 #pragma unroll
                     for (int a = 1; a < D; ++a) vg += Vji[a] * g[a];
-                    Rrho_ -= double(volj * vg);
+                    Rrho_ -= volj * vg;
                 }
             FJ_TILE_ENTRIES_END(cu, nl)
         }
@@ -2517,6 +2431,202 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
 #pragma unroll
         for (int a = 0; a < D; ++a) s.acc[i * D + a] = acc[a];
         s.Rrho[i] = c.ale ? (Rrho_ * rhoi + Rrhoc_ + s.deltaD[i]) : (Rrho_ * rhoi + s.deltaD[i]);
+    }
+}
+
+/* LDS tile of the mixed-precision force pass, one record per union slot,
+ * four float4 reads per pair (three without ALE):
+ *   A {x - xref, m (negative for BOUND)}, B {v - vref, p/rho^2},
+ *   X {xb - xbref (build positions, FR only), 1/rho}, E {vPert, rho};
+ *   V_j = |m_j| / rho_j is formed as |m_j| * (1/rho_j). */
+template <bool ALE>
+struct ForceTileM {
+    float4 A[TILE], B[TILE], X[TILE], E[ALE ? TILE : 1];
+};
+
+/* Mixed-precision form of k_force_t: pair terms in FP32 on coordinates and
+ * velocities relative to the block's first target (split_rel), each sum
+ * carried in FP32 over one LDS tile (a few dozen pairs) and the tile sums
+ * added into FP64 totals.
+ *
+ * The ALE terms are rearranged (exact in real arithmetic): with a_j = vPert_j
+ * . g, a_i = vPert_i . g and V_ji = v_j - v_i, the reference's
+ * (v_j a_j + v_i a_i - v_i (a_j - a_i)) V_j equals a_j V_j V_ji + 2 a_i V_j
+ * v_i, and its continuity terms -(V_ji . g + a_j - a_i) V_j and (rho_j a_j +
+ * rho_i a_i) V_j hold a_i only through sum_j V_j g.  So the pair loop carries
+ * a_j and S = sum_j V_j g, and vPert_i . S enters the three sums once per
+ * target. */
+template <int D, bool FR, bool ALE>
+__global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_m(Consts c, DevState s, Aux ax,
+                                                              const int32_t* __restrict__ part_id, TileGrid tg,
+                                                              NbrList nl, const DevScalars* sc)
+{
+    __shared__ ForceTileM<ALE> ft;
+    __shared__ BlockHead h;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
+    /* reference point of the block (its first target) for the FP32
+     * relative coordinates */
+    double xref[D], vref[D], xbref[D];
+    {
+        const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xref[a] = s.x[i0 * D + a];
+            vref[a] = s.v[i0 * D + a];
+            xbref[a] = FR ? ax.xb[i0 * D + a] : 0.0;
+        }
+    }
+    double npd = sc->npd;
+    if (tg.lgrp) {
+        const int32_t g = block_group<D>(tg, h.b);
+        if (g < 0 || !tg.gactive[g]) return;
+        npd = tg.gnpd[g];
+    }
+    const float ste = float(-(0.5 * c.sig / c.lam_st) / (npd * npd));
+    const float pi3o4H = float(3.0 * kPi / 4.0 / c.H);
+    const float inv_H = float(1.0 / c.H);
+    const float fgc = float(5.0 * c.Wc / c.H_sq);
+    const float eps2 = float(0.001 * c.H_sq);
+    const float nu = float(c.nu);
+    const float contang_fac = float(1.0 + 0.5 * cos(0.5 * kPi * 7.0 / 9.0));
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.vorder[p] : 0;
+        const bool fluid = active && part_id[i] >= c.n_bound;
+        float xit[D], xlo[D], vit[D], vlo[D], xbit[D], xblo[D], vpi[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            split_rel(active ? s.x[i * D + a] : xref[a], xref[a], xit[a], xlo[a]);
+            split_rel(active ? s.v[i * D + a] : vref[a], vref[a], vit[a], vlo[a]);
+            if constexpr (FR) split_rel(active ? ax.xb[i * D + a] : xbref[a], xbref[a], xbit[a], xblo[a]);
+            else xbit[a] = xblo[a] = 0.f;
+            vpi[a] = (ALE && fluid) ? float(s.vPert[i * D + a]) : 0.f;
+        }
+        const double rhoi = fluid ? s.rho[i] : 1.0, mi = fluid ? s.m[i] : 1.0;
+        const int32_t bi = fluid ? s.b[i] : 0;
+        const float Pi = fluid ? float(ax.P[i]) : 0.f;
+        const float iri = fluid ? float(ax.invrho[i]) : 0.f;
+        const float stm = ste * float(1.0 / mi);
+        /* surface tension: everywhere without ALE, in the surface zone with
+         * it; no terms at all when sigma = 0 */
+        const bool st_on = c.sig != 0.0 && (!ALE || (fluid && s.surfzone[i] == 1));
+        const bool bound_i = bi == FJSPH_BOUND;
+        double acc[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) acc[a] = 0.0;
+        double Rrho_ = 0.0, Rrhoc_ = 0.0, vS = 0.0;
+        Rows cu;
+        cu.init(nl, p, active ? nl.count[i] : 0, fluid);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int u = tile_union_index(tk, t, NT);
+                if (u >= U) continue;
+                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
+                    va[a] = float(s.v[(int64_t)sl * D + a] - vref[a]);
+                    if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
+                }
+                const float4 q = ax.fq[sl];
+                ft.A[t] = make_float4(pa[0], pa[1], pa[2], q.x);
+                ft.B[t] = make_float4(va[0], va[1], va[2], q.y);
+                ft.X[t] = make_float4(xa[0], xa[1], xa[2], q.z);
+                if constexpr (ALE) ft.E[t] = ax.fale[sl];
+            }
+            __syncthreads();
+            float fa[3] = {0.f, 0.f, 0.f}, fS[3] = {0.f, 0.f, 0.f}, fr = 0.f, frc = 0.f;
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
+                const float4 A = ft.A[t], B = ft.B[t], X = ft.X[t];
+                const float pj[3] = {A.x, A.y, A.z}, vj[3] = {B.x, B.y, B.z}, xbj[3] = {X.x, X.y, X.z};
+                float Rji[D], Vji[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    Rji[a] = (pj[a] - xit[a]) - xlo[a];
+                    Vji[a] = (vj[a] - vit[a]) - vlo[a];
+                }
+                float rr;
+                if constexpr (FR) {
+                    float Rb[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) Rb[a] = (xbj[a] - xbit[a]) - xblo[a];
+                    rr = Rb[0] * Rb[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
+                } else {
+                    rr = Rji[0] * Rji[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rji[a] * Rji[a];
+                }
+                const float r = fast_sqrt<float>(rr);
+                const float msg = A.w, Pj = B.w, irj = X.w;
+                const bool bound_j = msg < 0.f;
+                const float mj = fabsf(msg);
+                const float volj = mj * irj;
+                const float idist2 = fast_rcp<float>(rr + eps2);
+                const float qh = r * inv_H;
+                const float tq = 1.f - 0.5f * qh;
+                const float fg = (qh < 1e-12f) ? 0.f : fgc * (tq * tq * tq);
+                float g[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) g[a] = fg * Rji[a];
+                const float pc = mj * (Pi + Pj);
+                float rdotg = Rji[0] * g[0], vg = Vji[0] * g[0];
+#pragma unroll
+                for (int a = 1; a < D; ++a) {
+                    rdotg += Rji[a] * g[a];
+                    vg += Vji[a] * g[a];
+                }
+                float vf = mj * nu * (iri + irj) * rdotg * idist2;
+                float sf = 0.f;
+                if (st_on) {
+                    const float fac = (bound_i || bound_j) ? contang_fac : 1.f;
+                    sf = stm * fac * fast_cos<float>(pi3o4H * r) * fast_rcp<float>(r);
+                }
+                if constexpr (ALE) {
+                    const float4 E = ft.E[t];
+                    const float vpj[3] = {E.x, E.y, E.z};
+                    float aj = vpj[0] * g[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) aj += vpj[a] * g[a];
+                    vf += volj * aj;
+                    fr += volj * (vg + aj);
+                    frc += (volj * E.w) * aj;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) fS[a] += volj * g[a];
+                } else {
+                    fr += volj * vg;
+                }
+#pragma unroll
+                for (int a = 0; a < D; ++a) fa[a] += vf * Vji[a] + sf * Rji[a] - pc * g[a];
+            FJ_TILE_ENTRIES_END(cu, nl)
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc[a] += double(fa[a]);
+            Rrho_ -= double(fr);
+            if constexpr (ALE) {
+                float vs = vpi[0] * fS[0];
+#pragma unroll
+                for (int a = 1; a < D; ++a) vs += vpi[a] * fS[a];
+                Rrhoc_ += double(frc);
+                vS += double(vs);
+            }
+        }
+        if (!fluid) continue;
+        if constexpr (ALE) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc[a] += 2.0 * s.v[i * D + a] * vS;
+            Rrho_ += vS;
+            Rrhoc_ += rhoi * vS;
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) s.acc[i * D + a] = acc[a];
+        s.Rrho[i] = ALE ? (Rrho_ * rhoi + Rrhoc_ + s.deltaD[i]) : (Rrho_ * rhoi + s.deltaD[i]);
     }
 }
 
@@ -2783,16 +2893,20 @@ void launch_force(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
                   int r_from_x, const DevScalars* sc, hipStream_t st)
 {
     const dim3 g((unsigned)tg.nblocks), b(TPB);
-#define FJ_FORCE_LAUNCH(DD, TT, FF) \
-    hipLaunchKernelGGL((k_force_t<DD, TT, FF>), g, b, 0, st, c, s, a, part_id, tg, nl, sc)
+#define FJ_FORCE_T(DD, FF) hipLaunchKernelGGL((k_force_t<DD, FF>), g, b, 0, st, c, s, a, part_id, tg, nl, sc)
+#define FJ_FORCE_M(DD, FF, AA) hipLaunchKernelGGL((k_force_m<DD, FF, AA>), g, b, 0, st, c, s, a, part_id, tg, nl, sc)
+#define FJ_FORCE_MA(DD, FF) \
+    do { if (c.ale) FJ_FORCE_M(DD, FF, true); else FJ_FORCE_M(DD, FF, false); } while (0)
     if (dim == 2) {
-        if (nl.f32) { if (r_from_x) FJ_FORCE_LAUNCH(2, float, false); else FJ_FORCE_LAUNCH(2, float, true); }
-        else { if (r_from_x) FJ_FORCE_LAUNCH(2, double, false); else FJ_FORCE_LAUNCH(2, double, true); }
+        if (nl.f32) { if (r_from_x) FJ_FORCE_MA(2, false); else FJ_FORCE_MA(2, true); }
+        else { if (r_from_x) FJ_FORCE_T(2, false); else FJ_FORCE_T(2, true); }
     } else {
-        if (nl.f32) { if (r_from_x) FJ_FORCE_LAUNCH(3, float, false); else FJ_FORCE_LAUNCH(3, float, true); }
-        else { if (r_from_x) FJ_FORCE_LAUNCH(3, double, false); else FJ_FORCE_LAUNCH(3, double, true); }
+        if (nl.f32) { if (r_from_x) FJ_FORCE_MA(3, false); else FJ_FORCE_MA(3, true); }
+        else { if (r_from_x) FJ_FORCE_T(3, false); else FJ_FORCE_T(3, true); }
     }
-#undef FJ_FORCE_LAUNCH
+#undef FJ_FORCE_MA
+#undef FJ_FORCE_M
+#undef FJ_FORCE_T
 }
 
 void launch_nb_update(int dim, const Consts& c, DevState s, DevState pn, double* next_x, double* next_v,
