@@ -172,6 +172,7 @@ struct fjsph_ctx {
     int64_t partial_cap = 0;
     int32_t* export_buf = nullptr;
     bool x_at_build = false;   /* pnp1.x unchanged since the last build     */
+    bool f64_tail = false;     /* this step's remaining force passes in FP64 */
     DevScalars* sc = nullptr;
     DevScalars* sc_host = nullptr; /* pinned */
     DevBlocks bl{};
@@ -195,6 +196,8 @@ struct fjsph_ctx {
         unsigned n_stable = 0, n_unstable = 0, iteration = 0;
         int rollbacks = 0;
         bool active = true;
+        bool f64_tail = false; /* this step's remaining force passes in FP64 */
+        double xmax = 0.0;     /* max |coordinate| at the last build */
         double logbase = 0.0, rms = 0.0, npd = 1.0;
         double maxf = 0, maxAf = 0, maxRho_pc = 0, maxRhoi = 0, maxdrho = 0, minST = 0, maxU = 0, maxShift = 0;
     };
@@ -208,6 +211,9 @@ struct fjsph_ctx {
     double *gbox = nullptr, *glo = nullptr, *gdt = nullptr, *gnpd = nullptr, *gscale = nullptr;
     double *gred = nullptr, *gres = nullptr, *gd2 = nullptr;
     int32_t *gl0 = nullptr, *gactive = nullptr, *gbuild = nullptr, *lgrp = nullptr;
+    /* force-pass masks of the groups iterating in FP32 pair terms and of
+     * those whose step has gone to FP64 ones */
+    int32_t *gact_m = nullptr, *gact_d = nullptr;
     int64_t lgrp_cap = 0, gd2_cap = 0;
 
     /* timers */
@@ -335,7 +341,7 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->order); dfree(ctx->vorder); dfree(ctx->pos); dfree(ctx->tile_sums);
     dfree(ctx->nl.lu4); dfree(ctx->nl.count); dfree(ctx->nl.nnb); dfree(ctx->export_buf);
     dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
-    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb); dfree(ctx->aero_list);
+    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb); dfree(ctx->aux.spend); dfree(ctx->aero_list);
     dfree(ctx->aero_counts);
     ctx->aero_list_cap = 0;
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
@@ -347,6 +353,7 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->grp); dfree(ctx->gstart); dfree(ctx->gblk); dfree(ctx->gbox); dfree(ctx->glo); dfree(ctx->gdt);
     dfree(ctx->gnpd); dfree(ctx->gscale); dfree(ctx->gred); dfree(ctx->gres); dfree(ctx->gd2); dfree(ctx->gl0);
     dfree(ctx->gactive); dfree(ctx->gbuild); dfree(ctx->lgrp);
+    dfree(ctx->gact_m); dfree(ctx->gact_d);
     ctx->lgrp_cap = ctx->gd2_cap = 0;
     ctx->ngroups = 0;
     ctx->grs.clear();
@@ -433,6 +440,7 @@ void alloc_slots(fjsph_ctx* ctx, int64_t cap, int64_t keep)
     HIPCHK(hipMemset(ctx->aux.nhat + k * D, 0, (c - k) * D * 8));
     dgrow(ctx->aux.lnb, c * D * D, 0);
     dgrow(ctx->aux.xb, c * D, 0);
+    dgrow(ctx->aux.spend, c, 0);
     dgrow(ctx->keep, c, 0);
     dgrow(ctx->hflag, c, 0);
     dgrow(ctx->kstart, c + 1, 0);
@@ -597,9 +605,11 @@ double group_layout(fjsph_ctx* ctx, double cs)
     for (int k = 0; k < 3; ++k) ctx->dims[k] = 1;
     int64_t layer = 0;
     for (int g = 0; g < G; ++g) {
+        ctx->grs[size_t(g)].xmax = 0.0;
         for (int k = 0; k < D; ++k) {
             const double lo = box[g * 6 + k], hi = box[g * 6 + 3 + k];
             if (!std::isfinite(lo) || !std::isfinite(hi)) throw std::string("non-finite particle position");
+            ctx->grs[size_t(g)].xmax = std::max(ctx->grs[size_t(g)].xmax, std::max(std::fabs(lo), std::fabs(hi)));
             glo[g * 3 + k] = lo;
             const double span = (hi - lo) / cs;
             if (span > 1e9) throw std::string("a group's particle cloud is too large for the cell grid");
@@ -1077,8 +1087,31 @@ void force(fjsph_ctx* ctx, bool update)
     refresh_aux(ctx);
     {
         PassTimer t(ctx, FJSPH_T_FORCE);
-        launch_force(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, r_from_x(ctx), ctx->sc,
-                     ctx->st);
+        NbrList nl64 = ctx->nl;
+        nl64.f32 = 0;
+        bool tail = false;
+        if (grouped(ctx))
+            for (const fjsph_ctx::Group& g : ctx->grs) tail |= g.active && g.f64_tail;
+        if (!ctx->nl.f32 || (!grouped(ctx) && !ctx->f64_tail)) {
+            launch_force(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, r_from_x(ctx),
+                         ctx->sc, ctx->st);
+        } else if (!grouped(ctx)) {
+            launch_force(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, nl64, r_from_x(ctx), ctx->sc,
+                         ctx->st);
+        } else if (!tail) {
+            launch_force(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, r_from_x(ctx),
+                         ctx->sc, ctx->st);
+        } else {
+            /* groups whose step went to FP64 pair terms and the others, each
+             * launch skipping the other's blocks */
+            TileGrid tm = ctx->tg, td = ctx->tg;
+            tm.gactive = ctx->gact_m;
+            td.gactive = ctx->gact_d;
+            launch_force(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, tm, ctx->nl, r_from_x(ctx), ctx->sc,
+                         ctx->st);
+            launch_force(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, td, nl64, r_from_x(ctx), ctx->sc,
+                         ctx->st);
+        }
     }
     PassTimer t(ctx, FJSPH_T_UPDATE);
     const DevGroups gr = grouped(ctx) ? DevGroups{ctx->grp, ctx->gactive, ctx->gdt, ctx->gd2} : DevGroups{};
@@ -1185,10 +1218,29 @@ void find_timestep_groups(fjsph_ctx* ctx)
 
 void rollback(fjsph_ctx* ctx)
 {
+    ctx->f64_tail = false;
     copy_state(ctx, ctx->pnp1, ctx->pn);
     build_neighbours(ctx, false);
     ctx->delta_t = 0.5 * ctx->delta_t;
     ctx->rollbacks++;
+}
+
+/* log10 of 100 FP64 ulps of coordinates up to xmax.  Once the rms position
+ * change of a sub-iteration is below it, the iterates differ in the last
+ * bits of the positions, where FP32 pair terms change from one iterate to
+ * the next by more than the FP64 ones they stand for: from there on the
+ * mixed path runs the step's force passes in FP64, which settle on a fixed
+ * point as the reference's do (a column at rest: 13 sub-iterations to an
+ * exact fixed point in FP64, the iteration limit in FP32 pair terms, whose
+ * rms change stays at ~1e-18 m).  Steps that converge above this level,
+ * the usual case, never reach it. */
+double rounding_level(double xmax) { return xmax > 0.0 ? std::log10(std::ldexp(xmax, -52)) + 2.0 : -HUGE_VAL; }
+
+double build_xmax(const fjsph_ctx* ctx)
+{
+    double m = 0.0;
+    for (int k = 0; k < 2 * ctx->s.dim; ++k) m = std::max(m, std::fabs(ctx->sc_host->bbox[k]));
+    return m;
 }
 
 /* Newmark_Beta::Newmark_Beta + Check_Error (src/Newmark_Beta.cpp:10-52, 303-331). */
@@ -1199,6 +1251,7 @@ double newmark_loop(fjsph_ctx* ctx, double& logbase)
         const double le = nb_iter(ctx);
         if (ctx->iteration == 0) logbase = le;
         rms = le - logbase;
+        if (ctx->nl.f32 && rms > ctx->s.min_residual && le < rounding_level(build_xmax(ctx))) ctx->f64_tail = true;
         if (ctx->iteration > unsigned(ctx->s.max_subits)) {
             if (rms > 0.0) {
                 rollback(ctx);
@@ -1210,6 +1263,7 @@ double newmark_loop(fjsph_ctx* ctx, double& logbase)
         }
         ctx->iteration++;
     }
+    ctx->f64_tail = false;
     return rms;
 }
 
@@ -1219,9 +1273,17 @@ bool reorder_due(fjsph_ctx* ctx);
 
 void upload_active(fjsph_ctx* ctx)
 {
-    std::vector<int32_t> a(ctx->grs.size());
-    for (size_t g = 0; g < a.size(); ++g) a[g] = ctx->grs[g].active ? 1 : 0;
-    HIPCHK(hipMemcpyAsync(ctx->gactive, a.data(), sizeof(int32_t) * a.size(), hipMemcpyHostToDevice, ctx->st));
+    const size_t G = ctx->grs.size();
+    std::vector<int32_t> a(3 * G);
+    for (size_t g = 0; g < G; ++g) {
+        const fjsph_ctx::Group& gr = ctx->grs[g];
+        a[g] = gr.active ? 1 : 0;
+        a[G + g] = (gr.active && !gr.f64_tail) ? 1 : 0;
+        a[2 * G + g] = (gr.active && gr.f64_tail) ? 1 : 0;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->gactive, a.data(), sizeof(int32_t) * G, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->gact_m, a.data() + G, sizeof(int32_t) * G, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->gact_d, a.data() + 2 * G, sizeof(int32_t) * G, hipMemcpyHostToDevice, ctx->st));
     sync(ctx);
 }
 
@@ -1316,6 +1378,7 @@ void newmark_loop_groups(fjsph_ctx* ctx)
     for (fjsph_ctx::Group& g : ctx->grs) {
         g.rms = 0.0;
         g.active = true;
+        g.f64_tail = false;
     }
     upload_active(ctx);
     std::vector<double> le;
@@ -1332,10 +1395,12 @@ void newmark_loop_groups(fjsph_ctx* ctx)
             if (!g.active) continue;
             if (g.iteration == 0) g.logbase = le[k];
             g.rms = le[k] - g.logbase;
+            if (ctx->nl.f32 && g.rms > s.min_residual && le[k] < rounding_level(g.xmax)) g.f64_tail = true;
             if (g.iteration > unsigned(s.max_subits)) {
                 if (g.rms > 0.0) {
                     roll[k] = 1;
                     rolled = true;
+                    g.f64_tail = false;
                     g.dt *= 0.5;
                     g.rollbacks++;
                     g.iteration = 0;
@@ -1357,6 +1422,7 @@ void newmark_loop_groups(fjsph_ctx* ctx)
         }
         upload_active(ctx);
     }
+    for (fjsph_ctx::Group& g : ctx->grs) g.f64_tail = false;
 }
 
 /* integrate_no_update (src/Integration.cpp:27-107) for every group. */
@@ -1506,6 +1572,7 @@ double integrate_no_update(fjsph_ctx* ctx)
     if (ctx->s.integrator != FJSPH_NEWMARK_BETA) throw std::string("unknown integrator");
     ctx->iteration = 0;
     ctx->rollbacks = 0;
+    ctx->f64_tail = false;
     double logbase = 0.0;
     ctx->delta_t = find_timestep(ctx);
     build_neighbours(ctx, reorder_due(ctx));
@@ -2376,6 +2443,8 @@ fjsph_status fjsph_set_groups(fjsph_ctx* ctx, int32_t n_groups, const int32_t* g
         dalloc(ctx->gred, G * 8);
         dalloc(ctx->gres, G);
         dalloc(ctx->gactive, G);
+        dalloc(ctx->gact_m, G);
+        dalloc(ctx->gact_d, G);
         dalloc(ctx->gbuild, G);
         ctx->gstart_h.assign(G + 1, 0);
         std::vector<double> scale(G);

@@ -116,6 +116,8 @@ struct Aux {
     double* lnb;     /* dSPH_PreStep: correction matrix over non-boundary
                         neighbours (D x D per slot), for its eigenvalue     */
     double* xb;      /* positions at the last neighbour build (D per slot)   */
+    int4* spend;     /* Detect_Surface: the first four neighbours whose FP32
+                        point-T test was undecided                          */
 };
 
 /* Per-block boundary settings visible to kernels. */

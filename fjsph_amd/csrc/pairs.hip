@@ -154,6 +154,29 @@ __device__ __forceinline__ void block_head(const TileGrid& tg, BlockHead& h)
     __syncthreads();
 }
 
+/* The slots a thread stages into LDS tile tk (t = threadIdx.x and
+ * threadIdx.x + TPB; -1 where there is none).  Loaded one tile ahead, so
+ * the order[] lookup of the next tile is in flight while the current
+ * tile's pairs run and staging after the barrier waits only for the data. */
+static_assert(TILE <= 2 * TPB, "a thread stages at most two slots per tile");
+struct TileSlots {
+    int32_t sl[2];
+    __device__ __forceinline__ void load(const TileGrid& tg, const BlockHead& h, int tk, int NT, int U)
+    {
+        const int nt = tk < NT ? tile_len(tk, U, NT) : 0;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int t = threadIdx.x + r * TPB;
+            sl[r] = -1;
+            if (t < nt) {
+                const int u = tile_union_index(tk, t, NT);
+                if (u < U) sl[r] = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+            }
+        }
+    }
+    __device__ __forceinline__ int32_t at(int t) const { return t < TPB ? sl[0] : sl[1]; }
+};
+
 /* x - ref as an FP32 value hi and its FP32 rounding residual lo (hi + lo
  * = x - ref to about 1e-14 relative).  A pair difference is formed as
  * (xj_hi - hi) - lo: the target's own rounding then cancels, and only the
@@ -244,12 +267,15 @@ __device__ __forceinline__ int wave_min(int v)
 /* A candidate is a neighbour when its FP64 squared distance from the
  * target is below sr (src/Neighbours.cpp:44, nanoflann's strict test).
  * The test runs first in FP32 on coordinates relative to the block's first
- * target (split_rel): with every coordinate within L of that point, the
- * FP32 squared distance near sr is off by at most about 2^-23 (L / H) sr
- * plus a few ulp, and L < (C + 2) cs sqrt(3) < 1000 H for any tile grid
- * (C <= 256 cells of size 2H), so below (1 - 1e-3) sr it is surely in,
- * above (1 + 1e-3) sr surely out, and only the thin shell between is
- * decided by the exact FP64 test.  The lists are those of the FP64 test. */
+ * target, each rounded once to FP32: with every coordinate within L of that
+ * point, the FP32 squared distance near sr is off by at most about
+ * 2^-23 (L / H) sr plus a few ulp, and L < (C + 2) cs sqrt(3) < 1000 H for
+ * any tile grid (C <= 256 cells of size 2H), so below (1 - 1e-3) sr it is
+ * surely in, above (1 + 1e-3) sr surely out, and only the thin shell
+ * between is decided by the exact FP64 test.  The lists are those of the
+ * FP64 test.  The candidate loop only marks what is not surely out; the
+ * in/shell split and the FP64 test run on the marked ones (the ~240
+ * neighbours of ~2000 candidates), in ascending order. */
 template <int D>
 __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, TileGrid tg, double sr, NbrList nl,
                                                  DevScalars* sc)
@@ -290,11 +316,11 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
         }
         const int64_t i = active ? tg.order[p] : 0;
         double xi[D];
-        float xit[3] = {0.f, 0.f, 0.f}, xlo[3] = {0.f, 0.f, 0.f};
+        float xit[3] = {0.f, 0.f, 0.f};
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             xi[a] = active ? x[i * D + a] : xref[a];
-            split_rel(xi[a], xref[a], xit[a], xlo[a]);
+            xit[a] = float(xi[a] - xref[a]);
         }
         /* x-cells within reach of the wave's targets (the block's targets
          * share one cell row; a candidate two cells away in x is farther
@@ -345,13 +371,13 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
             if (active) {
                 int q = 0;
                 uint64_t cu = ~0ull;
-                /* 32 candidates at a time: branch-free FP32 tests into an
-                 * "in" and an "uncertain" bit mask, the uncertain ones then
-                 * tested in FP64, the entries emitted in ascending order */
+                /* 32 candidates at a time: a branch-free FP32 test into a
+                 * bit mask of those not surely out, then per marked
+                 * candidate (ascending) surely in, or the FP64 test */
                 for (int c0 = 0; c0 < nt; c0 += 32) {
                     const uint32_t gm = uint32_t(gmask[c0 >> 8] >> ((c0 >> 2) & 63)) & 0xFFu;
                     if (gm == 0u) continue;
-                    uint32_t win = 0u, wsh = 0u;
+                    uint32_t near = 0u;
 #pragma unroll
                     for (int gj = 0; gj < 8; ++gj) {
                         if (!((gm >> gj) & 1u)) continue;
@@ -359,30 +385,28 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
                         for (int e = 0; e < 4; ++e) {
                             const int k = 4 * gj + e;
                             const float4 P = tf[c0 + k];
-                            const float d0 = (P.x - xit[0]) - xlo[0];
-                            const float d1 = (P.y - xit[1]) - xlo[1];
-                            const float d2f = (P.z - xit[2]) - xlo[2];
+                            const float d0 = P.x - xit[0], d1 = P.y - xit[1], d2f = P.z - xit[2];
                             const float rr = d0 * d0 + d1 * d1 + d2f * d2f;
-                            win |= (rr < sr_lo ? 1u : 0u) << k;
-                            wsh |= (rr <= sr_hi ? 1u : 0u) << k;
+                            near |= (rr <= sr_hi ? 1u : 0u) << k;
                         }
                     }
                     const int rem = nt - c0;
-                    const uint32_t valid = rem >= 32 ? ~0u : ((1u << rem) - 1u);
-                    win &= valid;
-                    wsh &= valid & ~win;
-                    while (wsh) {
-                        const int k = __builtin_ctz(wsh);
-                        wsh &= wsh - 1u;
-                        double xj[D];
-#pragma unroll
-                        for (int a = 0; a < D; ++a) xj[a] = tp[a][c0 + k];
-                        if (exact_dist2<D>(xi, xj) < sr) win |= 1u << k;
-                    }
-                    while (win) {
-                        const int k = __builtin_ctz(win);
-                        win &= win - 1u;
+                    near &= rem >= 32 ? ~0u : ((1u << rem) - 1u);
+                    while (near) {
+                        const int k = __builtin_ctz(near);
+                        near &= near - 1u;
                         const int t = c0 + k;
+                        {
+                            const float4 P = tf[t];
+                            const float d0 = P.x - xit[0], d1 = P.y - xit[1], d2f = P.z - xit[2];
+                            if (!(d0 * d0 + d1 * d1 + d2f * d2f < sr_lo)) {
+                                /* the shell: the FP64 test decides */
+                                double xj[D];
+#pragma unroll
+                                for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
+                                if (!(exact_dist2<D>(xi, xj) < sr)) continue;
+                            }
+                        }
                         if (tsl[t] == (int32_t)i) continue;
                         cu = (cu >> 16) | (uint64_t(base + t) << 48);
                         ++nreal;
@@ -602,13 +626,14 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
         double kernsum = c.Wc, colour = 0.0;
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, active);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
 #pragma unroll
                 for (int a = 0; a < D; ++a) tp[a][t] = s.x[(int64_t)sl * D + a];
                 tvol[t] = ax.vol[sl];
@@ -616,6 +641,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
                 tnb[t] = s.b[sl] > FJSPH_PISTON;
                 if constexpr (FR) tsl[t] = sl;
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 double xj[D], d2, r;
@@ -686,13 +712,14 @@ __global__ __launch_bounds__(TPB) void k_dsph_m(Consts c, DevState s, Aux ax, Ti
         for (int a = 0; a < D; ++a) { gR[a] = 0.0; nr[a] = 0.0; fgR[a] = 0.f; fnr[a] = 0.f; }
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, active && !slow);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
                 float pa[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
@@ -704,6 +731,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_m(Consts c, DevState s, Aux ax, Ti
                 tR[t] = float(s.rho[sl] - rref);
                 if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 const float4 A = tA[t];
@@ -926,6 +954,12 @@ __device__ __forceinline__ bool marrone_pair(const double* xi, const double* xj,
 
 /* The FP64 point-T test of pair (i, j) from the global state: normal,
  * test point and frozen distance rebuilt as k_surface1_t forms them. */
+/* surface flag of a target whose point-T test awaits the FP64 decision:
+ * kSurfUndecided + the number of undecided pairs (kSurfUndecided + 5: more
+ * than the four whose slots Aux::spend keeps) */
+constexpr int32_t kSurfUndecided = 2;
+constexpr int kSurfPendMax = 4;
+
 template <int D, bool FR>
 __device__ __noinline__ bool marrone_exact(const Consts& c, const DevState& s, const Aux& ax, int64_t i, int64_t j)
 {
@@ -1070,13 +1104,14 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
         const T eps2 = T(0.0001 * c.H_sq), alpha = T(c.visc_alpha);
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, fluid);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
 #pragma unroll
                 for (int a = 0; a < D; ++a) tp[a][t] = s.x[(int64_t)sl * D + a];
                 tvol[t] = ax.vol[sl];
@@ -1094,6 +1129,7 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
                     tnb[t] = s.b[sl] > FJSPH_PISTON;
                 }
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 double xj[D], d2, r;
@@ -1179,7 +1215,7 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
     __shared__ float4 tA[TILE];            /* x_j - xref, V_j            */
     __shared__ float4 tB[TILE];            /* v_j - vref, lambda_j       */
     __shared__ float4 tG[TD];              /* gradRho_j, rho_j - rref    */
-    __shared__ float2 tE[TD];              /* 1/rho_j, m_j (negative: j is PISTON or BOUND) */
+    __shared__ float2 tE[TD];              /* sqrt(B gam / rho_j), m_j (negative: j is PISTON or BOUND) */
     __shared__ float4 tX[FR ? TILE : 1];   /* xb_j - xbref               */
     __shared__ int32_t tsl[with_surf ? TILE : 1];
     __shared__ BlockHead h;
@@ -1251,6 +1287,9 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
         const double rhoi = fluid ? s.rho[i] : rref;
         const bool testing = with_surf && fluid && bi >= FJSPH_PIPE && lam_nb >= 0.2 && lam_nb < 0.75;
         bool found = false;
+        /* pairs whose FP32 point-T test is undecided (decided in FP64 by
+         * k_surface1_resolve, so no FP64 call sits in the pair loop) */
+        int npend = 0;
         const bool high = lami > 0.7;
         const float lamf = float(lami);
         float drhoi, rlo;
@@ -1259,13 +1298,14 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
         const float cfac = float(sqrt(c.B * c.gam / rhoi));
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, fluid);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
                 float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
@@ -1283,9 +1323,10 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
                     for (int a = 0; a < D; ++a) ga[a] = float(s.gradRho[(int64_t)sl * D + a]);
                     tG[t] = make_float4(ga[0], ga[1], ga[2], float(s.rho[sl] - rref));
                     const float mj = float(s.m[sl]);
-                    tE[t] = make_float2(float(ax.invrho[sl]), s.b[sl] > FJSPH_PISTON ? mj : -mj);
+                    tE[t] = make_float2(fast_sqrt<float>(Bg * float(ax.invrho[sl])), s.b[sl] > FJSPH_PISTON ? mj : -mj);
                 }
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 const float4 A = tA[t];
@@ -1343,7 +1384,13 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
                             q = (cq - cos45) * (1.f - cq);
                             sure = sure && fabsf(cq - cos45) >= tolc && fabsf(1.f - cq) >= tolc;
                         }
-                        found = sure ? q > 0.f : marrone_exact<D, FR>(c, s, ax, i, tsl[with_surf ? t : 0]);
+                        if (sure) found = q > 0.f;
+                        else {
+                            /* rare: the slot goes straight to memory */
+                            if (npend < kSurfPendMax)
+                                reinterpret_cast<int32_t*>(ax.spend)[i * kSurfPendMax + npend] = tsl[with_surf ? t : 0];
+                            ++npend;
+                        }
                     }
                     const float f = A.w * (high ? (B.w - lamf) : B.w) * fg;
 #pragma unroll
@@ -1368,7 +1415,7 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
                         for (int a = 1; a < D; ++a) vdotr += ((vj[a] - vit[a]) - vlo[a]) * Rji[a];
                         if (!(vdotr > 0.f)) {
                             const float muij = Ht * vdotr * idist2;
-                            const float cbar = 0.5f * (cfac + fast_sqrt<float>(Bg * E.x));
+                            const float cbar = 0.5f * (cfac + E.x);
                             const float ff =
                                 E.y * alpha * cbar * muij * fast_rcp<float>(0.5f * (rhoif + (rreff + G.w)));
 #pragma unroll
@@ -1394,12 +1441,42 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
             frd = 0.f;
         }
         if (active) surface1_out<D, MODE>(c, s, i, fluid, bi, lam_nb, lami, found, nrm, av, rd, norms_tmp);
+        if constexpr (with_surf) {
+            /* undecided: k_surface1_resolve sets the flag from the FP64 test */
+            if (testing && !found && npend > 0) s.surf[i] = kSurfUndecided + min(npend, kSurfPendMax + 1);
+        }
     }
+}
+
+/* Surface flag of a target that k_surface1_m left undecided (an FP32
+ * point-T comparison within the margin of its threshold, and no sure hit):
+ * the FP64 test of the undecided pairs (the others are surely out), or of
+ * every pair when there were more than four (a sure FP32 decision equals the
+ * FP64 one, so the OR is that of the FP64 test), one lane per slot;
+ * src/Geometry.cpp:100-143 as in surface1_out. */
+template <int D, bool FR>
+__global__ __launch_bounds__(TPB) void k_surface1_resolve(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl,
+                                                          int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i >= n || s.surf[i] <= kSurfUndecided) return;
+    const int np = s.surf[i] - kSurfUndecided;
+    bool found = false;
+    if (np <= kSurfPendMax) {
+        const int4 pend = ax.spend[i];
+        const int32_t sl[4] = {pend.x, pend.y, pend.z, pend.w};
+        for (int k = 0; k < np && !found; ++k) found = marrone_exact<D, FR>(c, s, ax, i, sl[k]);
+    } else {
+        walk_lane<D>(tg, nl, i, [&](int32_t j) {
+            if (!found) found = marrone_exact<D, FR>(c, s, ax, i, j);
+        });
+    }
+    s.surf[i] = found ? 0 : 1;
 }
 
 template <int D, typename T, int MODE>
 static void launch_surface1_m(const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
-                              int r_from_x, double* norms_tmp, hipStream_t st)
+                              int64_t n, int r_from_x, double* norms_tmp, hipStream_t st)
 {
     const dim3 g((unsigned)tg.nblocks), b(TPB);
     if constexpr (sizeof(T) == 4) {
@@ -1407,6 +1484,11 @@ static void launch_surface1_m(const Consts& c, DevState s, Aux a, const int32_t*
             hipLaunchKernelGGL((k_surface1_m<D, MODE, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
         else
             hipLaunchKernelGGL((k_surface1_m<D, MODE, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+        if constexpr (MODE != 2) {
+            const dim3 gs(grid_of(n)), bs(TPB);
+            if (r_from_x) hipLaunchKernelGGL((k_surface1_resolve<D, false>), gs, bs, 0, st, c, s, a, tg, nl, n);
+            else hipLaunchKernelGGL((k_surface1_resolve<D, true>), gs, bs, 0, st, c, s, a, tg, nl, n);
+        }
     } else {
         if (r_from_x)
             hipLaunchKernelGGL((k_surface1_t<D, T, MODE, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
@@ -1417,24 +1499,23 @@ static void launch_surface1_m(const Consts& c, DevState s, Aux a, const int32_t*
 
 template <int D, typename T>
 static void launch_surface1_d(const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
-                              int r_from_x, double* norms_tmp, int mode, hipStream_t st)
+                              int64_t n, int r_from_x, double* norms_tmp, int mode, hipStream_t st)
 {
-    if (mode == 0) launch_surface1_m<D, T, 0>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
-    else if (mode == 1) launch_surface1_m<D, T, 1>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
-    else launch_surface1_m<D, T, 2>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, st);
+    if (mode == 0) launch_surface1_m<D, T, 0>(c, s, a, part_id, tg, nl, n, r_from_x, norms_tmp, st);
+    else if (mode == 1) launch_surface1_m<D, T, 1>(c, s, a, part_id, tg, nl, n, r_from_x, norms_tmp, st);
+    else launch_surface1_m<D, T, 2>(c, s, a, part_id, tg, nl, n, r_from_x, norms_tmp, st);
 }
 
 void launch_surface1(int dim, int prec, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg,
                      NbrList nl, int64_t n, int r_from_x, double* norms_tmp, int mode, hipStream_t st)
 {
-    (void)n;
     const bool f32 = prec == FJSPH_PREC_MIXED;
     if (dim == 2) {
-        if (f32) launch_surface1_d<2, float>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
-        else launch_surface1_d<2, double>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
+        if (f32) launch_surface1_d<2, float>(c, s, a, part_id, tg, nl, n, r_from_x, norms_tmp, mode, st);
+        else launch_surface1_d<2, double>(c, s, a, part_id, tg, nl, n, r_from_x, norms_tmp, mode, st);
     } else {
-        if (f32) launch_surface1_d<3, float>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
-        else launch_surface1_d<3, double>(c, s, a, part_id, tg, nl, r_from_x, norms_tmp, mode, st);
+        if (f32) launch_surface1_d<3, float>(c, s, a, part_id, tg, nl, n, r_from_x, norms_tmp, mode, st);
+        else launch_surface1_d<3, double>(c, s, a, part_id, tg, nl, n, r_from_x, norms_tmp, mode, st);
     }
 }
 
@@ -1479,13 +1560,14 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
         int zone = (fluid && s.surf[i] == 1) ? 1 : 0;
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, fluid);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     tp[a][t] = s.x[(int64_t)sl * D + a];
@@ -1495,6 +1577,7 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
                 tsurf[t] = s.surf[sl];
                 if constexpr (FR) tsl[t] = sl;
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 double xj[D], d2, r, Rij[D], nj[D];
@@ -1601,13 +1684,14 @@ __global__ __launch_bounds__(TPB) void k_surface2_m(Consts c, DevState s, Aux ax
         bool zone = fluid && s.surf[i] == 1;
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, fluid);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
                 float pa[3] = {0.f, 0.f, 0.f}, na[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
@@ -1619,6 +1703,7 @@ __global__ __launch_bounds__(TPB) void k_surface2_m(Consts c, DevState s, Aux ax
                 tN[t] = make_float4(na[0], na[1], na[2], s.surf[sl] == 1 ? 1.f : 0.f);
                 if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 const float4 A = tA[t], N = tN[t];
@@ -1791,13 +1876,14 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
         double maxU2 = 0.0, dmin = 2.0;
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, work);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     tp[a][t] = s.x[(int64_t)sl * D + a];
@@ -1808,6 +1894,7 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
                 tnb[t] = s.b[sl] > FJSPH_PISTON;
                 if constexpr (FR) tsl[t] = sl;
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 double xj[D], d2, r;
@@ -1889,13 +1976,14 @@ __global__ __launch_bounds__(TPB) void k_shift_m(Consts c, DevState s, Aux ax, c
         double dmin = 2.0;
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, work);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
                 float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
@@ -1909,6 +1997,7 @@ __global__ __launch_bounds__(TPB) void k_shift_m(Consts c, DevState s, Aux ax, c
                 tV[t] = make_float4(va[0], va[1], va[2], 0.f);
                 if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 const float4 A = tA[t], V = tV[t];
@@ -2342,13 +2431,14 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
         double Rrho_ = 0.0, Rrhoc_ = 0.0;
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, fluid);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     ft.P[a][t] = s.x[(int64_t)sl * D + a];
@@ -2360,6 +2450,7 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_t(Consts c, DevState
                 ft.P[7][t] = ax.P[sl];
                 ft.EA[3][t] = s.rho[sl];
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
                 double Rji[D], Vji[D], vjr[D], vpj[D], xj[D], d2;
@@ -2520,13 +2611,14 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_m(Consts c, DevState
         double Rrho_ = 0.0, Rrhoc_ = 0.0, vS = 0.0;
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, fluid);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
             for (int t = threadIdx.x; t < nt; t += TPB) {
-                const int u = tile_union_index(tk, t, NT);
-                if (u >= U) continue;
-                const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
                 float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
@@ -2540,6 +2632,7 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_m(Consts c, DevState
                 ft.X[t] = make_float4(xa[0], xa[1], xa[2], q.z);
                 if constexpr (ALE) ft.E[t] = ax.fale[sl];
             }
+            ts.load(tg, h, tk + 1, NT, U);
             __syncthreads();
             float fa[3] = {0.f, 0.f, 0.f}, fS[3] = {0.f, 0.f, 0.f}, fr = 0.f, frc = 0.f;
             FJ_TILE_ENTRIES_BEGIN(cu, nl, base)

@@ -5,11 +5,17 @@ relative after 1000 steps):
   dx = 0.02 m, 6.5k particles), 1000 Newmark-Beta steps;
 - a 3D droplet in crossflow (0.8 mm, dx = 0.05 mm, 2.1k particles) with the
   mixed-precision pair passes (FP32 pair terms, FP64 sums, positions and
-  time) against the FP64 oracle: 500 steps with the adaptive time step (the
-  droplet configuration's length), and 1000 steps at a fixed time step.
-  Beyond about 700 adaptive steps the two runs no longer take the same time
-  steps (a sub-iteration count that differs moves the CFL number), so the
-  1000-step comparison holds the step fixed.
+  time) against the FP64 oracle: 500 and 1000 steps with the adaptive time
+  step (500 is the droplet configuration's length), and 1000 steps at a
+  fixed time step.
+- sub-iteration counts of the mixed path against the FP64 engine on the
+  2D dam break at dx = 0.01 m (23k particles), 1000 steps: within one of
+  each other on every step.  The first step starts from the converged
+  state of the start-up pass and changes positions by ~1e-12 m; the FP64
+  iterates settle on an exact fixed point (13 sub-iterations), which FP32
+  pair terms cannot resolve, so the mixed path takes the force passes of
+  such a step in FP64 once the rms change is within 100 ulps of the
+  coordinates (before: the iteration limit, 21).
 Errors are field-normwise (max |engine - oracle| / max |oracle|) and are
 printed for every field."""
 import numpy as np
@@ -50,13 +56,14 @@ def test_dam_break_drift_1000_steps_fp64():
     print("  drift after 1000 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
 
 
-def test_droplet_drift_500_steps_mixed():
+@pytest.mark.parametrize("steps", [500, 1000])
+def test_droplet_drift_adaptive_mixed(steps):
     case = cases.droplet_3d(diameter=0.8e-3, dx=5e-5, precision=abi.PREC_MIXED)
-    print(f"[drift] droplet, mixed precision against the FP64 oracle: n={case.n}, 500 steps", flush=True)
-    eng, orc, dts, its = _run(case, 500)
+    print(f"[drift] droplet, mixed precision against the FP64 oracle: n={case.n}, {steps} steps", flush=True)
+    eng, orc, dts, its = _run(case, steps)
     te, to = eng.get_time()[0], orc.get_time()[0]
     print(f"  simulated time engine {te:.9e} oracle {to:.9e}; sub-iteration counts differ on "
-          f"{(its[:, 0] != its[:, 1]).sum()} of 500 steps", flush=True)
+          f"{(its[:, 0] != its[:, 1]).sum()} of {steps} steps", flush=True)
     assert abs(te - to) <= 1e-9 * to
     rho0 = case.settings.rho_rest
     errs = {}
@@ -65,7 +72,7 @@ def test_droplet_drift_500_steps_mixed():
                     ("RHO-rho0", eng.field("RHO") - rho0, orc.field("RHO") - rho0),
                     ("P", eng.field("P"), orc.field("P"))):
         errs[f] = assert_close(f, g, r, DRIFT)
-    print("  drift after 500 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    print(f"  drift after {steps} steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
 
 
 def test_droplet_drift_1000_steps_mixed_fixed_step():
@@ -83,3 +90,26 @@ def test_droplet_drift_1000_steps_mixed_fixed_step():
                     ("P", eng.field("P"), orc.field("P"))):
         errs[f] = assert_close(f, g, r, DRIFT)
     print("  drift after 1000 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+
+
+def test_dam_break_mixed_iterations_1000_steps():
+    """Sub-iteration counts of the mixed path within one of the FP64 engine's
+    (itself pinned to the oracle by test_dam_break_drift_1000_steps_fp64) on
+    every step, the time steps equal."""
+    from fjsph_amd.engine import Engine
+    runs = {}
+    for prec in (abi.PREC_FP64, abi.PREC_MIXED):
+        case = cases.dam_break_2d(dx=0.01)
+        case.settings.precision = prec
+        eng = Engine(case)
+        eng.integrate_no_update()
+        infos = [eng.integrate_step() for _ in range(1000)]
+        eng.close()
+        runs[prec] = (np.array([i["iterations"] for i in infos]), np.array([i["dt"] for i in infos]))
+    d = runs[abi.PREC_MIXED][0] - runs[abi.PREC_FP64][0]
+    print(f"[iterations] dam break dx=0.01, 1000 steps: FP64 mean {runs[abi.PREC_FP64][0].mean():.2f}, "
+          f"mixed mean {runs[abi.PREC_MIXED][0].mean():.2f}, first step {runs[abi.PREC_FP64][0][0]} / "
+          f"{runs[abi.PREC_MIXED][0][0]}, counts differ on {(d != 0).sum()} steps, max |diff| {np.abs(d).max()}",
+          flush=True)
+    assert np.abs(d).max() <= 1
+    assert np.array_equal(runs[abi.PREC_MIXED][1], runs[abi.PREC_FP64][1])
