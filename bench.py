@@ -209,6 +209,7 @@ def run_engine(case, args, label, autotune=True, block_targets=None, steps=None)
     its, group_its = [], []
     growing = bool(getattr(case, "inlets", None))
     particle_steps = 0
+    w0 = time.monotonic_ns()
     t0 = time.perf_counter()
     for _ in range(steps):
         # a growing particle set (inlets) counts the particles each step starts with
@@ -217,7 +218,9 @@ def run_engine(case, args, label, autotune=True, block_targets=None, steps=None)
         if grouped:
             group_its.append([g["iterations"] for g in eng.group_info()])
     elapsed = time.perf_counter() - t0
+    w1 = time.monotonic_ns()
     r = {"elapsed": elapsed, "value": particle_steps / elapsed, "its": float(np.mean(its)), "autotune": tune,
+         "window_ns": [w0, w1],
          "particles_end": eng.n,
          "steps": steps, "block_targets": eng.get_block_targets(),
          "timers": eng.timers(), "stats": eng.neighbour_stats(),
@@ -478,6 +481,9 @@ def main():
         "oracle_error": fp64["oracle_error"] if fp64 else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        # CLOCK_MONOTONIC bounds of the timed steps of the reported run: a kernel
+        # trace of the same command gives the GPU-busy fraction inside them
+        "timed_window_ns": run["window_ns"],
     }
     if cpu:
         out["config"]["speedup_vs_cpu"] = value / cpu["value"]

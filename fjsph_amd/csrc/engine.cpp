@@ -3591,7 +3591,7 @@ fjsph_status fjsph_pass_bytes(fjsph_ctx* ctx, int32_t pass, double* bytes_per_pa
     double b = 0.0;
     switch (pass) {
     case FJSPH_T_FORCE:
-        /* k_force_t, once per slot: reads order (4), x, v (2*8D), the FP32
+        /* k_force_t / k_force_m, once per slot: reads order (4), x, v (2*8D), the FP32
          * record {m, p/rho^2, 1/rho, m/rho} (16), rho, m, p/rho^2, 1/rho,
          * deltaD (40), part_id, b (8), ALE vPert (8D) + surfzone (4);
          * writes acc (8D), Rrho (8). */

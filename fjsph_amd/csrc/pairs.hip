@@ -29,6 +29,8 @@
  *   k_shift_t      particle_shift               src/Shifting.cpp:189-290
  *   k_bnd_*        boundary solvers             src/Resid.cpp:21-186, src/Newmark_Beta.cpp:69-132
  *   k_force_t      get_acc_and_Rrho_on_i        src/Resid.cpp:243-418
+ *   (k_force_m: the same in FP32 pair terms; the k_*_m kernels are the
+ *   mixed-precision forms of the k_*_t ones)
  *   k_nb_update    Do_NB_Iter update            src/Newmark_Beta.cpp:137-240
  */
 #include "device_common.hpp"
