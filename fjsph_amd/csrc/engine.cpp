@@ -341,7 +341,7 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->order); dfree(ctx->vorder); dfree(ctx->pos); dfree(ctx->tile_sums);
     dfree(ctx->nl.lu4); dfree(ctx->nl.count); dfree(ctx->nl.nnb); dfree(ctx->export_buf);
     dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
-    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb); dfree(ctx->aux.spend); dfree(ctx->aero_list);
+    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb); dfree(ctx->aux.spend); dfree(ctx->aux.sund); dfree(ctx->aero_list);
     dfree(ctx->aero_counts);
     ctx->aero_list_cap = 0;
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
@@ -441,6 +441,7 @@ void alloc_slots(fjsph_ctx* ctx, int64_t cap, int64_t keep)
     dgrow(ctx->aux.lnb, c * D * D, 0);
     dgrow(ctx->aux.xb, c * D, 0);
     dgrow(ctx->aux.spend, c, 0);
+    dgrow(ctx->aux.sund, c + 1, 0);
     dgrow(ctx->keep, c, 0);
     dgrow(ctx->hflag, c, 0);
     dgrow(ctx->kstart, c + 1, 0);

@@ -118,6 +118,9 @@ struct Aux {
     double* xb;      /* positions at the last neighbour build (D per slot)   */
     int4* spend;     /* Detect_Surface: the first four neighbours whose FP32
                         point-T test was undecided                          */
+    int32_t* sund;   /* the count (sund[0]) and slots (sund[1..]) of the
+                        targets with an undecided point-T test, appended by
+                        k_surface1_m                                        */
 };
 
 /* Per-block boundary settings visible to kernels. */

@@ -1445,23 +1445,21 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
         if (active) surface1_out<D, MODE>(c, s, i, fluid, bi, lam_nb, lami, found, nrm, av, rd, norms_tmp);
         if constexpr (with_surf) {
             /* undecided: k_surface1_resolve sets the flag from the FP64 test */
-            if (testing && !found && npend > 0) s.surf[i] = kSurfUndecided + min(npend, kSurfPendMax + 1);
+            if (testing && !found && npend > 0) {
+                s.surf[i] = kSurfUndecided + min(npend, kSurfPendMax + 1);
+                ax.sund[1 + atomicAdd(ax.sund, 1)] = int32_t(i);
+            }
         }
     }
 }
 
-/* Surface flag of a target that k_surface1_m left undecided (an FP32
- * point-T comparison within the margin of its threshold, and no sure hit):
- * the FP64 test of the undecided pairs (the others are surely out), or of
- * every pair when there were more than four (a sure FP32 decision equals the
- * FP64 one, so the OR is that of the FP64 test), one lane per slot;
- * src/Geometry.cpp:100-143 as in surface1_out. */
+__global__ void k_clear_count(int32_t* count) { *count = 0; }
+
+/* FP64 point-T decision of one undecided target (see k_surface1_resolve). */
 template <int D, bool FR>
-__global__ __launch_bounds__(TPB) void k_surface1_resolve(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl,
-                                                          int64_t n)
+__device__ __noinline__ void surface1_resolve_one(const Consts& c, DevState& s, const Aux& ax, const TileGrid& tg,
+                                                  const NbrList& nl, int64_t i)
 {
-    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
-    if (i >= n || s.surf[i] <= kSurfUndecided) return;
     const int np = s.surf[i] - kSurfUndecided;
     bool found = false;
     if (np <= kSurfPendMax) {
@@ -1476,20 +1474,35 @@ __global__ __launch_bounds__(TPB) void k_surface1_resolve(Consts c, DevState s, 
     s.surf[i] = found ? 0 : 1;
 }
 
+/* Surface flag of a target that k_surface1_m left undecided (an FP32
+ * point-T comparison within the margin of its threshold, and no sure hit):
+ * the FP64 test of the undecided pairs (the others are surely out), or of
+ * every pair when there were more than four (a sure FP32 decision equals the
+ * FP64 one, so the OR is that of the FP64 test), one lane per slot;
+ * src/Geometry.cpp:100-143 as in surface1_out. */
+template <int D, bool FR>
+__global__ __launch_bounds__(TPB) void k_surface1_resolve(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl)
+{
+    const int32_t cnt = ax.sund[0];
+    for (int32_t k = blockIdx.x * TPB + threadIdx.x; k < cnt; k += gridDim.x * TPB)
+        surface1_resolve_one<D, FR>(c, s, ax, tg, nl, ax.sund[1 + k]);
+}
+
 template <int D, typename T, int MODE>
 static void launch_surface1_m(const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
                               int64_t n, int r_from_x, double* norms_tmp, hipStream_t st)
 {
     const dim3 g((unsigned)tg.nblocks), b(TPB);
     if constexpr (sizeof(T) == 4) {
+        if constexpr (MODE != 2) hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(1), 0, st, a.sund);
         if (r_from_x)
             hipLaunchKernelGGL((k_surface1_m<D, MODE, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
         else
             hipLaunchKernelGGL((k_surface1_m<D, MODE, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
         if constexpr (MODE != 2) {
-            const dim3 gs(grid_of(n)), bs(TPB);
-            if (r_from_x) hipLaunchKernelGGL((k_surface1_resolve<D, false>), gs, bs, 0, st, c, s, a, tg, nl, n);
-            else hipLaunchKernelGGL((k_surface1_resolve<D, true>), gs, bs, 0, st, c, s, a, tg, nl, n);
+            const dim3 gs((unsigned)std::min<int64_t>(grid_of(n), 1024)), bs(TPB);
+            if (r_from_x) hipLaunchKernelGGL((k_surface1_resolve<D, false>), gs, bs, 0, st, c, s, a, tg, nl);
+            else hipLaunchKernelGGL((k_surface1_resolve<D, true>), gs, bs, 0, st, c, s, a, tg, nl);
         }
     } else {
         if (r_from_x)
