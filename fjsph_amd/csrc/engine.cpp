@@ -173,6 +173,7 @@ struct fjsph_ctx {
     int32_t* export_buf = nullptr;
     bool x_at_build = false;   /* pnp1.x unchanged since the last build     */
     bool f64_tail = false;     /* this step's remaining force passes in FP64 */
+    int32_t* bnd_slots = nullptr; /* one boundary block's slots (count first), per pass */
     DevScalars* sc = nullptr;
     DevScalars* sc_host = nullptr; /* pinned */
     DevBlocks bl{};
@@ -341,7 +342,7 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->order); dfree(ctx->vorder); dfree(ctx->pos); dfree(ctx->tile_sums);
     dfree(ctx->nl.lu4); dfree(ctx->nl.count); dfree(ctx->nl.nnb); dfree(ctx->export_buf);
     dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
-    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb); dfree(ctx->aux.spend); dfree(ctx->aux.sund); dfree(ctx->aero_list);
+    dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb); dfree(ctx->aux.spend); dfree(ctx->aux.sund); dfree(ctx->bnd_slots); dfree(ctx->aero_list);
     dfree(ctx->aero_counts);
     ctx->aero_list_cap = 0;
     dfree(ctx->bl.solver); dfree(ctx->bl.no_slip); dfree(ctx->bl.vel);
@@ -442,6 +443,7 @@ void alloc_slots(fjsph_ctx* ctx, int64_t cap, int64_t keep)
     dgrow(ctx->aux.xb, c * D, 0);
     dgrow(ctx->aux.spend, c, 0);
     dgrow(ctx->aux.sund, c + 1, 0);
+    dgrow(ctx->bnd_slots, c + 1, 0);
     dgrow(ctx->keep, c, 0);
     dgrow(ctx->hflag, c, 0);
     dgrow(ctx->kstart, c + 1, 0);
@@ -1079,7 +1081,7 @@ void boundary_update(fjsph_ctx* ctx)
     for (size_t k = 0; k < ctx->blocks.size(); ++k) {
         if (ctx->blocks[k].b.is_fluid) continue;
         launch_boundary(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->blk, ctx->bl, int32_t(k), ctx->tg, ctx->nl,
-                        ctx->n, ctx->near_inlet, ctx->st);
+                        ctx->n, ctx->nb, ctx->bnd_slots, ctx->near_inlet, ctx->st);
     }
 }
 
@@ -1482,7 +1484,7 @@ void rk_stage(fjsph_ctx* ctx, DevState& dst, DevState& prev, double dt, bool fin
         for (size_t k = 0; k < ctx->blocks.size(); ++k) {
             if (ctx->blocks[k].b.is_fluid) continue;
             launch_boundary(D, ctx->c, dst, ctx->aux, ctx->blk, ctx->bl, int32_t(k), ctx->tg, ctx->nl, ctx->n,
-                            ctx->near_inlet, ctx->st);
+                            ctx->nb, ctx->bnd_slots, ctx->near_inlet, ctx->st);
         }
         launch_rk_bnd_rho(D, ctx->c, dst, ctx->pn, ctx->rk1, ctx->rk2, ctx->rk3, ctx->blk, ctx->bl, ctx->near_inlet,
                           ctx->n, dt, final_stage ? 1 : 0, ctx->st);

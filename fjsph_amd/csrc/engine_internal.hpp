@@ -276,7 +276,8 @@ void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t*
 void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
                   int64_t n, int r_from_x, hipStream_t st);
 void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t* blk, DevBlocks bl, int32_t block,
-                     TileGrid tg, NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st);
+                     TileGrid tg, NbrList nl, int64_t n, int64_t nb, int32_t* slots, int32_t* near_inlet,
+                     hipStream_t st);
 /* Force pass on `s`; when next_x is non-null the Newmark update of
  * Do_NB_Iter is fused and writes next x/v/rho/p for every slot, plus one
  * residual partial sum per tile block. */

@@ -2094,28 +2094,71 @@ void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
 
 /* ------------------------------------------------------- boundary solvers */
 
-/* f(j, r) for every neighbour j of slot i with the distance frozen at the
- * last build. */
+
+/* The boundary passes run one wave per boundary slot: its 64 lanes split
+ * the slot's list rows, so the dependent loads of a neighbour walk run 64
+ * at a time instead of one after another in a single lane (a boundary
+ * layer is a few thousand slots, each with ~200 neighbours), and the lane
+ * sums meet in a fixed-order wave reduction. */
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* f(j, r) for every neighbour j of slot i, spread over the lanes of the
+ * wave (all lanes call it with the same i); r is the distance of the last
+ * build. */
 template <int D, typename F>
-__device__ __forceinline__ void walk_lane_r(const TileGrid& tg, const NbrList& nl, const double* __restrict__ xb,
+__device__ __forceinline__ void walk_wave_r(const TileGrid& tg, const NbrList& nl, const double* __restrict__ xb,
                                             int64_t i, F&& f)
 {
+    const int lane = threadIdx.x & 63;
+    int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
+    int nseg, p0, p1;
+    union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
+    const int NT = n_tiles(pre[nseg]);
+    const int64_t p = tg.pos[i];
+    const int nrows = min(nl.count[i], nl.cap >> 2);
     double xbi[D];
     load_vec<D>(xb, i, xbi);
-    walk_lane<D>(tg, nl, i, [&](int32_t j) {
-        double xbj[D];
-        load_vec<D>(xb, j, xbj);
-        f(j, sqrt(exact_dist2<D>(xbi, xbj)));
-    });
+    for (int r = lane; r < nrows; r += 64) {
+        const uint64_t u = nl.lu4[(int64_t)r * nl.npad + p];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t uq = uint32_t(u >> (16 * q)) & 0xFFFFu;
+            if (uq == NO_ENTRY) continue;
+            const int32_t j = union_slot(tg, seg_a, pre, nseg, entry_union_index(uq, NT));
+            double xbj[D];
+            load_vec<D>(xb, j, xbj);
+            f(j, sqrt(exact_dist2<D>(xbi, xbj)));
+        }
+    }
+}
+
+/* The slots of boundary block `block`, appended (count at out[0]). */
+__global__ __launch_bounds__(TPB) void k_block_slots(const int32_t* __restrict__ blk, int64_t n, int32_t block,
+                                                     int32_t* out)
+{
+    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
+    if (i < n && blk[i] == block) out[1 + atomicAdd(out, 1)] = int32_t(i);
+}
+
+/* the boundary slot of this wave, or -1 past the list */
+__device__ __forceinline__ int64_t wave_slot(const int32_t* __restrict__ slots)
+{
+    const int64_t w = (blockIdx.x * (int64_t)TPB + threadIdx.x) >> 6;
+    return w < slots[0] ? int64_t(slots[1 + w]) : int64_t(-1);
 }
 
 /* Block velocity + no-slip (src/Newmark_Beta.cpp:69-102, src/Resid.cpp:162-186). */
 template <int D>
-__global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, Aux ax, const int32_t* __restrict__ blk,
-                                                 DevBlocks bl, int32_t block, TileGrid tg, NbrList nl, int64_t n)
+__global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, Aux ax, DevBlocks bl, int32_t block,
+                                                 TileGrid tg, NbrList nl, const int32_t* __restrict__ slots)
 {
-    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
-    if (i >= n || blk[i] != block) return;
+    const int64_t i = wave_slot(slots);
+    if (i < 0) return;
     double v[D];
 #pragma unroll
     for (int a = 0; a < D; ++a) v[a] = bl.vel[block * 3 + a];
@@ -2125,7 +2168,7 @@ __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, Aux ax, c
 #pragma unroll
         for (int a = 0; a < D; ++a) vs[a] = 0.0;
         double ks = 0.0;
-        walk_lane_r<D>(tg, nl, ax.xb, i, [&](int32_t j, double r) {
+        walk_wave_r<D>(tg, nl, ax.xb, i, [&](int32_t j, double r) {
             if (s.b[j] <= FJSPH_PISTON) return;
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
@@ -2133,36 +2176,45 @@ __global__ __launch_bounds__(TPB) void k_bnd_vel(Consts c, DevState s, Aux ax, c
 #pragma unroll
             for (int a = 0; a < D; ++a) vs[a] += s.v[(int64_t)j * D + a] * W;
         });
+        ks = wave_sum(ks);
+#pragma unroll
+        for (int a = 0; a < D; ++a) vs[a] = wave_sum(vs[a]);
         if (ks > 0.0) {
 #pragma unroll
             for (int a = 0; a < D; ++a) v[a] = 2.0 * v[a] - vs[a] / ks;
         }
     }
+    if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int a = 0; a < D; ++a) s.v[i * D + a] = v[a];
+        for (int a = 0; a < D; ++a) s.v[i * D + a] = v[a];
+    }
 }
 
-/* Boundary solver of one block (src/Resid.cpp:21-160). */
+/* Boundary solvers (src/Resid.cpp:21-186): DBC (acc = 0), Adami pressure
+ * (Get_Boundary_Pressure), Ghost (boundary Rrho and the near-inlet flag). */
 template <int D>
-__global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax, const int32_t* __restrict__ blk,
-                                                   DevBlocks bl, int32_t block, TileGrid tg, NbrList nl, int64_t n,
+__global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax, DevBlocks bl, int32_t block,
+                                                   TileGrid tg, NbrList nl, const int32_t* __restrict__ slots,
                                                    int32_t* near_inlet)
 {
-    const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
-    if (i >= n || blk[i] != block) return;
+    const int64_t i = wave_slot(slots);
+    if (i < 0) return;
+    const bool lead = (threadIdx.x & 63) == 0;
     const int solver = bl.solver[block];
     const double inv_H = 1.0 / c.H;
     double xi[D];
     load_vec<D>(s.x, i, xi);
     if (solver == FJSPH_DBC) {
+        if (lead) {
 #pragma unroll
-        for (int a = 0; a < D; ++a) s.acc[i * D + a] = 0.0;
+            for (int a = 0; a < D; ++a) s.acc[i * D + a] = 0.0;
+        }
     } else if (solver == FJSPH_PRESSURE_G) {
         int near = 0;
         double ks = 0.0, pk = 0.0, ak[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) ak[a] = 0.0;
-        walk_lane_r<D>(tg, nl, ax.xb, i, [&](int32_t j, double r) {
+        walk_wave_r<D>(tg, nl, ax.xb, i, [&](int32_t j, double r) {
             if (s.b[j] <= FJSPH_PISTON) return;
             double W, fg;
             wendland<double>(r, c.H, inv_H, c.Wc, W, fg);
@@ -2174,6 +2226,11 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
             for (int a = 0; a < D; ++a) ak[a] += kr * (xi[a] - s.x[(int64_t)j * D + a]);
             if (s.surfzone[j]) near = 1;
         });
+        ks = wave_sum(ks);
+        pk = wave_sum(pk);
+#pragma unroll
+        for (int a = 0; a < D; ++a) ak[a] = wave_sum(ak[a]);
+        near = __any(near) ? 1 : 0;
         double pres = 0.0;
         if (ks > 0.0) {
             double ga = 0.0;
@@ -2182,13 +2239,15 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
             const double pp = (pk + ga) / ks;
             pres = near ? fmax(0.0, pp) : pp;
         }
-        s.p[i] = pres;
-        s.rho[i] = eos_density(c, pres);
+        if (lead) {
+            s.p[i] = pres;
+            s.rho[i] = eos_density(c, pres);
+        }
     } else if (solver == FJSPH_GHOST) {
         int ni = 1;
         double rr = 0.0, vi[D];
         load_vec<D>(s.v, i, vi);
-        walk_lane_r<D>(tg, nl, ax.xb, i, [&](int32_t j, double r) {
+        walk_wave_r<D>(tg, nl, ax.xb, i, [&](int32_t j, double r) {
             const int32_t bj = s.b[j];
             if (bj == FJSPH_PIPE || bj == FJSPH_FREE) ni = 0;
             double W, fg;
@@ -2199,21 +2258,29 @@ __global__ __launch_bounds__(TPB) void k_bnd_solve(Consts c, DevState s, Aux ax,
                 t += (s.v[(int64_t)j * D + a] - vi[a]) * (fg * (s.x[(int64_t)j * D + a] - xi[a]));
             rr -= ax.vol[j] * t;
         });
-        near_inlet[i] = ni;
-        s.Rrho[i] = rr * s.rho[i];
+        rr = wave_sum(rr);
+        ni = __any(ni == 0) ? 0 : 1;
+        if (lead) {
+            near_inlet[i] = ni;
+            s.Rrho[i] = rr * s.rho[i];
+        }
     }
 }
 
 void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t* blk, DevBlocks bl, int32_t block,
-                     TileGrid tg, NbrList nl, int64_t n, int32_t* near_inlet, hipStream_t st)
+                     TileGrid tg, NbrList nl, int64_t n, int64_t nb, int32_t* slots, int32_t* near_inlet,
+                     hipStream_t st)
 {
-    const dim3 g(grid_of(n)), b(TPB);
+    /* the block's slots (at most nb, the boundary count), then one wave each */
+    hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(1), 0, st, slots);
+    hipLaunchKernelGGL(k_block_slots, dim3(grid_of(n)), dim3(TPB), 0, st, blk, n, block, slots);
+    const dim3 g((unsigned)((nb + TPB / 64 - 1) / (TPB / 64))), b(TPB);
     if (dim == 2) {
-        hipLaunchKernelGGL(k_bnd_vel<2>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n);
-        hipLaunchKernelGGL(k_bnd_solve<2>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+        hipLaunchKernelGGL(k_bnd_vel<2>, g, b, 0, st, c, s, a, bl, block, tg, nl, slots);
+        hipLaunchKernelGGL(k_bnd_solve<2>, g, b, 0, st, c, s, a, bl, block, tg, nl, slots, near_inlet);
     } else {
-        hipLaunchKernelGGL(k_bnd_vel<3>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n);
-        hipLaunchKernelGGL(k_bnd_solve<3>, g, b, 0, st, c, s, a, blk, bl, block, tg, nl, n, near_inlet);
+        hipLaunchKernelGGL(k_bnd_vel<3>, g, b, 0, st, c, s, a, bl, block, tg, nl, slots);
+        hipLaunchKernelGGL(k_bnd_solve<3>, g, b, 0, st, c, s, a, bl, block, tg, nl, slots, near_inlet);
     }
 }
 
