@@ -904,8 +904,15 @@ void dsph(fjsph_ctx* ctx, bool with_aero)
 #ifndef FJSPH_SURFACE_FUSED
 #define FJSPH_SURFACE_FUSED 1
 #endif
+/* Detect_Surface loops 2-3 and particle_shift share one staged pass in
+ * mixed precision (k_surf2_shift_m). */
+#ifndef FJSPH_SHIFT_FUSED
+#define FJSPH_SHIFT_FUSED 1
+#endif
 
-void surface_and_dissipation(fjsph_ctx* ctx, bool with_diss)
+/* with_shift: particle_shift runs in the same pass as Detect_Surface loops
+ * 2-3 where that fused form exists (mixed precision). */
+void surface_and_dissipation(fjsph_ctx* ctx, bool with_diss, bool with_shift)
 {
     PassTimer t(ctx, FJSPH_T_SURFACE);
     if (FJSPH_SURFACE_FUSED || !with_diss) {
@@ -916,6 +923,11 @@ void surface_and_dissipation(fjsph_ctx* ctx, bool with_diss)
                         ctx->n, r_from_x(ctx), ctx->norms_tmp, 0, ctx->st);
         launch_surface1(ctx->s.dim, ctx->s.precision, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl,
                         ctx->n, r_from_x(ctx), ctx->norms_tmp, 2, ctx->st);
+    }
+    if (with_shift) {
+        launch_surface2_shift(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n,
+                              r_from_x(ctx), ctx->norms_tmp, ctx->st);
+        return;
     }
     launch_surface2(ctx->s.dim, ctx->c, ctx->pnp1, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, ctx->n, r_from_x(ctx),
                     ctx->norms_tmp, ctx->st);
@@ -1046,8 +1058,9 @@ void pipe_outlet(fjsph_ctx* ctx)
 void prepasses(fjsph_ctx* ctx)
 {
     dsph(ctx, true);
-    surface_and_dissipation(ctx, true);
-    if (ctx->s.ale) shift(ctx);
+    const bool fused_shift = FJSPH_SHIFT_FUSED && ctx->s.ale && ctx->nl.f32;
+    surface_and_dissipation(ctx, true, fused_shift);
+    if (ctx->s.ale && !fused_shift) shift(ctx);
     pipe_outlet(ctx);
 }
 
@@ -3434,7 +3447,7 @@ fjsph_status fjsph_detect_surface(fjsph_ctx* ctx)
     return guarded(ctx, [&] {
         require(ctx, true);
         refresh_aux(ctx);
-        surface_and_dissipation(ctx, false);
+        surface_and_dissipation(ctx, false, false);
         sync(ctx);
         drain_timers(ctx);
     });

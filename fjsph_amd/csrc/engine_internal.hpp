@@ -275,6 +275,9 @@ void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t*
                      int64_t n, int r_from_x, const double* norms_tmp, hipStream_t st);
 void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg, NbrList nl,
                   int64_t n, int r_from_x, hipStream_t st);
+/* Detect_Surface loops 2-3 and particle_shift in one pass (mixed precision, ALE) */
+void launch_surface2_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg,
+                           NbrList nl, int64_t n, int r_from_x, const double* norms_tmp, hipStream_t st);
 void launch_boundary(int dim, const Consts& c, DevState s, Aux a, const int32_t* blk, DevBlocks bl, int32_t block,
                      TileGrid tg, NbrList nl, int64_t n, int64_t nb, int32_t* slots, int32_t* near_inlet,
                      hipStream_t st);

@@ -667,8 +667,11 @@ __global__ __launch_bounds__(TPB) void k_dsph_t(Consts c, DevState s, Aux ax, Ti
  * fewer than 2 D neighbours takes its pairs in FP64 from the global state
  * (dsph_pair), so the rank test of its correction matrix, which decides
  * between L^-1 and the identity, sees the matrix of the FP64 path. */
+#ifndef FJ_DSPH_OCC
+#define FJ_DSPH_OCC __attribute__((amdgpu_waves_per_eu(5, 8)))
+#endif
 template <int D, bool FR>
-__global__ __launch_bounds__(TPB) void k_dsph_m(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl, double* partial)
+__global__ __launch_bounds__(TPB) FJ_DSPH_OCC void k_dsph_m(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl, double* partial)
 {
     constexpr int NS = D * (D + 1) / 2;
     __shared__ float4 tA[TILE];            /* x_j - xref, +V_j (-V_j: j is PISTON or BOUND) */
@@ -1205,7 +1208,7 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
  * the global positions, so the surface flags are those of the FP64 test.
  * acos(nhat . m) < pi/4 is evaluated as cos(pi/4) < nhat . m <= 1. */
 #ifndef FJ_SURF1_OCC
-#define FJ_SURF1_OCC __attribute__((amdgpu_waves_per_eu(4, 8)))
+#define FJ_SURF1_OCC __attribute__((amdgpu_waves_per_eu(5, 8)))
 #endif
 template <int D, int MODE, bool FR>
 __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
@@ -1643,8 +1646,11 @@ __global__ __launch_bounds__(TPB) void k_surface2_t(Consts c, DevState s, Aux ax
  * LDS tile and in FP64 over the tiles; 32 B of LDS per candidate (48 B
  * with build positions).  The surface-zone flag is exact; a zero normal
  * stays exactly zero in FP32. */
+#ifndef FJ_SURF2_OCC
+#define FJ_SURF2_OCC __attribute__((amdgpu_waves_per_eu(5, 8)))
+#endif
 template <int D, bool FR>
-__global__ __launch_bounds__(TPB) void k_surface2_m(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+__global__ __launch_bounds__(TPB) FJ_SURF2_OCC void k_surface2_m(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
                                                     TileGrid tg, NbrList nl, const double* __restrict__ norms_tmp)
 {
     __shared__ float4 tA[TILE];            /* x_j - xref, V_j */
@@ -1819,7 +1825,8 @@ void launch_surface2(int dim, const Consts& c, DevState s, Aux a, const int32_t*
  * max_j |v_j - v_i|^2, dmin = min_j n_i . n_j over the non-boundary j. */
 template <int D>
 __device__ __forceinline__ void shift_out(const Consts& c, DevState& s, int64_t i, bool work, double lam_nb,
-                                          const double* vi, const double* nhat, double* du, double maxU2, double dmin)
+                                          const double* vi, const double* nhat, double* du, double maxU2, double dmin,
+                                          int32_t zone)
 {
     double out[D];
     if (!work) {
@@ -1835,7 +1842,7 @@ __device__ __forceinline__ void shift_out(const Consts& c, DevState& s, int64_t 
         const double mag = fmin(sqrt(dotd<D>(du, du)), fmin(sqrt(maxU2) / 2.0, c.max_shift_vel));
 #pragma unroll
         for (int a = 0; a < D; ++a) du[a] = mag * dun[a];
-        if (s.surfzone[i] == 0 && lam_nb > 0.55) {
+        if (zone == 0 && lam_nb > 0.55) {
 #pragma unroll
             for (int a = 0; a < D; ++a) out[a] = du[a];
         } else if (wo < kPi / 12.0) {
@@ -1937,7 +1944,7 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
                 if (dv2 > maxU2) maxU2 = dv2;
             FJ_TILE_ENTRIES_END(cu, nl)
         }
-        if (fluid) shift_out<D>(c, s, i, work, lam_nb, vi, nhat, du, maxU2, dmin);
+        if (fluid) shift_out<D>(c, s, i, work, lam_nb, vi, nhat, du, maxU2, dmin, s.surfzone[i]);
     }
 }
 
@@ -1946,8 +1953,11 @@ __global__ __launch_bounds__(TPB) void k_shift_t(Consts c, DevState s, Aux ax, c
  * per LDS tile and in FP64 over the tiles; the normal products n_i . n_j
  * stay in FP64 (they decide between the projected and the zero shift);
  * 56 B of LDS per candidate (72 B with build positions). */
+#ifndef FJ_SHIFT_OCC
+#define FJ_SHIFT_OCC __attribute__((amdgpu_waves_per_eu(5, 8)))
+#endif
 template <int D, bool FR>
-__global__ __launch_bounds__(TPB) void k_shift_m(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
+__global__ __launch_bounds__(TPB) FJ_SHIFT_OCC void k_shift_m(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
                                                  TileGrid tg, NbrList nl)
 {
     __shared__ float4 tA[TILE];            /* x_j - xref, +V_j (-V_j: j is PISTON or BOUND) */
@@ -2064,7 +2074,7 @@ __global__ __launch_bounds__(TPB) void k_shift_m(Consts c, DevState s, Aux ax, c
         double vi[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) vi[a] = work ? s.v[i * D + a] : 0.0;
-        shift_out<D>(c, s, i, work, lam_nb, vi, nhat, du, double(maxU2), dmin);
+        shift_out<D>(c, s, i, work, lam_nb, vi, nhat, du, double(maxU2), dmin, s.surfzone[i]);
     }
 }
 
@@ -2089,6 +2099,237 @@ void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
     } else {
         if (r_from_x) hipLaunchKernelGGL((k_shift_t<3, false>), g, b, 0, st, c, s, a, part_id, tg, nl);
         else hipLaunchKernelGGL((k_shift_t<3, true>), g, b, 0, st, c, s, a, part_id, tg, nl);
+    }
+}
+
+/* Detect_Surface loops 2-3 (k_surface2_m) and particle_shift (k_shift_m)
+ * in one staged pass, for ALE builds in mixed precision: the shifting pass
+ * reads of its neighbours only what Detect_Surface loop 1 and dSPH_PreStep
+ * produced (positions, velocities, volumes and the unit normal, which is
+ * the normalised loop-1 normal) and of its target the surface-zone flag
+ * that loops 2-3 form in the same walk, so one staging and one list walk
+ * serve both (src/Geometry.cpp:145-277, src/Shifting.cpp:189-290).  The
+ * pair terms and every decision are those of the two separate kernels,
+ * which the per-pass entry points keep.  72 B of LDS per candidate (88 B
+ * with build positions). */
+#ifndef FJ_SURF2SHIFT_OCC
+#define FJ_SURF2SHIFT_OCC __attribute__((amdgpu_waves_per_eu(4, 8)))
+#endif
+template <int D, bool FR>
+__global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts c, DevState s, Aux ax,
+                                                                   const int32_t* __restrict__ part_id, TileGrid tg,
+                                                                   NbrList nl, const double* __restrict__ norms_tmp)
+{
+    __shared__ float4 tA[TILE];            /* x_j - xref, +V_j (-V_j: j is PISTON or BOUND) */
+    __shared__ float4 tN[TILE];            /* surface normal of j (Detect_Surface loop 1), surf_j */
+    __shared__ float4 tV[TILE];            /* v_j - vref */
+    __shared__ double tn[D][TILE];         /* unit normal of j */
+    __shared__ float4 tX[FR ? TILE : 1];   /* xb_j - xbref */
+    __shared__ BlockHead h;
+    block_head<D>(tg, h);
+    const int U = h.pre[h.nseg];
+    const int NT = n_tiles(U);
+    double xref[D], vref[D], xbref[D];
+    {
+        const int64_t i0 = tg.order[h.p0 < h.p1 ? h.p0 : 0];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            xref[a] = s.x[i0 * D + a];
+            vref[a] = s.v[i0 * D + a];
+            xbref[a] = FR ? ax.xb[i0 * D + a] : 0.0;
+        }
+    }
+    const float Ht = float(c.H), inv_Hf = float(1.0 / c.H), Wcf = float(c.Wc), inv_Wdx = float(1.0 / c.W_dx);
+    for (int pb = h.p0; pb < h.p1; pb += TPB) {
+        const int p = pb + threadIdx.x;
+        const bool active = p < h.p1;
+        const int64_t i = active ? tg.vorder[p] : 0;
+        const bool fluid = active && part_id[i] >= c.n_bound;
+        const double lam_nb = fluid ? s.lam_nb[i] : 0.0;
+        const bool work = fluid && !(lam_nb < 0.55 || s.b[i] == FJSPH_BUFFER);
+        float xit[D], xlo[D], xbit[D], xblo[D], vit[D], vlo[D], ni[D], Vd[D], Li[D][D], fdu[D];
+        double nhat[D], du[D];
+        {
+            double n64[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                n64[a] = fluid ? norms_tmp[i * D + a] : 0.0;
+                ni[a] = float(n64[a]);
+            }
+            normalize<D>(n64, nhat);
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            split_rel(fluid ? s.x[i * D + a] : xref[a], xref[a], xit[a], xlo[a]);
+            split_rel((FR && fluid) ? ax.xb[i * D + a] : xbref[a], xbref[a], xbit[a], xblo[a]);
+            split_rel(work ? s.v[i * D + a] : vref[a], vref[a], vit[a], vlo[a]);
+            Vd[a] = 0.f;
+            du[a] = 0.0;
+            fdu[a] = 0.f;
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) Li[a][b2] = fluid ? float(s.L[i * D * D + a * D + b2]) : 0.f;
+        }
+        const int32_t bi = fluid ? s.b[i] : 0;
+        const bool occl = fluid && (c.aero_case == FJSPH_GISSLER) && bi == FJSPH_FREE;
+        /* the gas velocity relative to the target, in FP32 for the pairs
+         * (its FP64 norm is formed again after the walk) */
+        const bool has_vd = occl && s.cellID[i] != -1;
+        if (has_vd) {
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+                Vd[a] = float((c.aero_source == FJSPH_MESH_INFL ? s.cellV[i * D + a] : c.v_inf[a]) - s.v[i * D + a]);
+        }
+        bool ni_nz = false;
+#pragma unroll
+        for (int a = 0; a < D; ++a) ni_nz = ni_nz || ni[a] != 0.f;
+        double curve = 0.0, dmin = 2.0;
+        float fcurve = 0.f, qmax = 0.f, maxU2 = 0.f;
+        bool zone = fluid && s.surf[i] == 1;
+        Rows cu;
+        cu.init(nl, p, active ? nl.count[i] : 0, fluid);
+        TileSlots ts;
+        ts.load(tg, h, 0, NT, U);
+        for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
+            const int nt = tile_len(tk, U, NT);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nt; t += TPB) {
+                const int32_t sl = ts.at(t);
+                if (sl < 0) continue;
+                float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, na[3] = {0.f, 0.f, 0.f},
+                      xa[3] = {0.f, 0.f, 0.f};
+                double nj64[D], njh[D];
+                load_vec<D>(norms_tmp, sl, nj64);
+                normalize<D>(nj64, njh);
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
+                    va[a] = float(s.v[(int64_t)sl * D + a] - vref[a]);
+                    na[a] = float(nj64[a]);
+                    tn[a][t] = njh[a];
+                    if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
+                }
+                const float v = float(ax.vol[sl]);
+                tA[t] = make_float4(pa[0], pa[1], pa[2], s.b[sl] > FJSPH_PISTON ? v : -v);
+                tN[t] = make_float4(na[0], na[1], na[2], s.surf[sl] == 1 ? 1.f : 0.f);
+                tV[t] = make_float4(va[0], va[1], va[2], 0.f);
+                if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
+            }
+            ts.load(tg, h, tk + 1, NT, U);
+            __syncthreads();
+            FJ_TILE_ENTRIES_BEGIN(cu, nl, base)
+                const float4 A = tA[t], N = tN[t];
+                const float pj[3] = {A.x, A.y, A.z}, nj[3] = {N.x, N.y, N.z};
+                const float volj = fabsf(A.w);
+                float Rij[D], rr;
+#pragma unroll
+                for (int a = 0; a < D; ++a) Rij[a] = (pj[a] - xit[a]) - xlo[a];
+                if constexpr (FR) {
+                    const float4 X = tX[FR ? t : 0];
+                    const float xbj[3] = {X.x, X.y, X.z};
+                    float Rb[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) Rb[a] = (xbj[a] - xbit[a]) - xblo[a];
+                    rr = Rb[0] * Rb[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rb[a] * Rb[a];
+                } else {
+                    rr = Rij[0] * Rij[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) rr += Rij[a] * Rij[a];
+                }
+                const float r = fast_sqrt<float>(rr);
+                float W, fg;
+                wendland<float>(r, Ht, inv_Hf, Wcf, W, fg);
+                /* Detect_Surface loops 2-3: curvature, occlusion, surface zone */
+                bool nj_nz = false;
+#pragma unroll
+                for (int a = 0; a < D; ++a) nj_nz = nj_nz || nj[a] != 0.f;
+                if (ni_nz && nj_nz) {
+                    float tt = 0.f;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        float la = 0.f;
+#pragma unroll
+                        for (int b2 = 0; b2 < D; ++b2) la += Li[a][b2] * (nj[b2] - ni[b2]);
+                        tt += la * (fg * Rij[a]);
+                    }
+                    fcurve += volj * tt;
+                }
+                if (occl) {
+                    float q = Rij[0] * Vd[0];
+#pragma unroll
+                    for (int a = 1; a < D; ++a) q += Rij[a] * Vd[a];
+                    q = -q / r;
+                    if (q > qmax) qmax = q;
+                }
+                zone = zone || N.w != 0.f;
+                /* particle_shift */
+                if (work) {
+                    const float4 V = tV[t];
+                    const float vj[3] = {V.x, V.y, V.z};
+                    const float w = W * inv_Wdx;
+                    const float w2 = w * w;
+                    const float f = (1.f + 0.2f * (w2 * w2)) * fg * volj;
+                    float dv2 = 0.f;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        fdu[a] += f * Rij[a];
+                        const float dv = (vj[a] - vit[a]) - vlo[a];
+                        dv2 += dv * dv;
+                    }
+                    if (A.w > 0.f) {
+                        double d = 0.0;
+#pragma unroll
+                        for (int a = 0; a < D; ++a) d += nhat[a] * tn[a][t];
+                        if (d >= -1.0 && d <= 1.0 && d < dmin) dmin = d;
+                    }
+                    maxU2 = fmaxf(maxU2, dv2);
+                }
+            FJ_TILE_ENTRIES_END(cu, nl)
+            curve += double(fcurve);
+            fcurve = 0.f;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                du[a] += double(fdu[a]);
+                fdu[a] = 0.f;
+            }
+        }
+        if (!fluid) continue;
+        double Vd64[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+            Vd64[a] = has_vd ? (c.aero_source == FJSPH_MESH_INFL ? s.cellV[i * D + a] : c.v_inf[a]) - s.v[i * D + a]
+                             : 0.0;
+        const double Vdn = sqrt(dotd<D>(Vd64, Vd64));
+        const double wo = (occl && Vdn > 0.0) ? double(qmax) / Vdn : 0.0;
+        s.woccl[i] = lam_nb < c.lam_cutoff ? fmax(0.0, fmin(wo, 1.0)) : 1.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            s.norm[i * D + a] = norms_tmp[i * D + a];
+            ax.nhat[i * D + a] = nhat[a];
+        }
+        s.curve[i] = curve;
+        s.norm_curve[i] = c.dx * curve;
+        s.pDist[i] = s.lam[i];
+        s.surfzone[i] = zone ? 1 : 0;
+        double vi[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) vi[a] = work ? s.v[i * D + a] : 0.0;
+        shift_out<D>(c, s, i, work, lam_nb, vi, nhat, du, double(maxU2), dmin, zone ? 1 : 0);
+    }
+}
+
+void launch_surface2_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* part_id, TileGrid tg,
+                           NbrList nl, int64_t n, int r_from_x, const double* norms_tmp, hipStream_t st)
+{
+    (void)n;
+    const dim3 g((unsigned)tg.nblocks), b(TPB);
+    if (dim == 2) {
+        if (r_from_x) hipLaunchKernelGGL((k_surf2_shift_m<2, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+        else hipLaunchKernelGGL((k_surf2_shift_m<2, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+    } else {
+        if (r_from_x) hipLaunchKernelGGL((k_surf2_shift_m<3, false>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
+        else hipLaunchKernelGGL((k_surf2_shift_m<3, true>), g, b, 0, st, c, s, a, part_id, tg, nl, norms_tmp);
     }
 }
 
