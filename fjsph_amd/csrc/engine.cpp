@@ -3629,6 +3629,8 @@ fjsph_status fjsph_pass_bytes(fjsph_ctx* ctx, int32_t pass, double* bytes_per_pa
         break;
     case FJSPH_T_SURFACE:
         b = 2 * (8 * D + 8 * D * D + 8 * D + 48) + 8 * D * 3 + 24;
+        /* particle_shift's 48 B when it runs inside loops 2-3 */
+        if (FJSPH_SHIFT_FUSED && ctx->s.ale && ctx->s.precision == FJSPH_PREC_MIXED) b += 48;
         break;
     case FJSPH_T_NEIGHBOUR:
         b = 8 * D + 4 * 4 + 4;
