@@ -1458,37 +1458,49 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
 
 __global__ void k_clear_count(int32_t* count) { *count = 0; }
 
-/* FP64 point-T decision of one undecided target (see k_surface1_resolve). */
-template <int D, bool FR>
-__device__ __noinline__ void surface1_resolve_one(const Consts& c, DevState& s, const Aux& ax, const TileGrid& tg,
-                                                  const NbrList& nl, int64_t i)
-{
-    const int np = s.surf[i] - kSurfUndecided;
-    bool found = false;
-    if (np <= kSurfPendMax) {
-        const int4 pend = ax.spend[i];
-        const int32_t sl[4] = {pend.x, pend.y, pend.z, pend.w};
-        for (int k = 0; k < np && !found; ++k) found = marrone_exact<D, FR>(c, s, ax, i, sl[k]);
-    } else {
-        walk_lane<D>(tg, nl, i, [&](int32_t j) {
-            if (!found) found = marrone_exact<D, FR>(c, s, ax, i, j);
-        });
-    }
-    s.surf[i] = found ? 0 : 1;
-}
-
 /* Surface flag of a target that k_surface1_m left undecided (an FP32
  * point-T comparison within the margin of its threshold, and no sure hit):
  * the FP64 test of the undecided pairs (the others are surely out), or of
  * every pair when there were more than four (a sure FP32 decision equals the
- * FP64 one, so the OR is that of the FP64 test), one lane per slot;
- * src/Geometry.cpp:100-143 as in surface1_out. */
+ * FP64 one, so the OR is that of the FP64 test); src/Geometry.cpp:100-143
+ * as in surface1_out.  One wave per undecided target: its lanes take the
+ * pending slots, or split the list rows, and the flag is the OR over the
+ * wave, so the dependent loads of a list walk run 64 at a time. */
 template <int D, bool FR>
 __global__ __launch_bounds__(TPB) void k_surface1_resolve(Consts c, DevState s, Aux ax, TileGrid tg, NbrList nl)
 {
     const int32_t cnt = ax.sund[0];
-    for (int32_t k = blockIdx.x * TPB + threadIdx.x; k < cnt; k += gridDim.x * TPB)
-        surface1_resolve_one<D, FR>(c, s, ax, tg, nl, ax.sund[1 + k]);
+    const int lane = threadIdx.x & 63;
+    const int64_t nwave = int64_t(gridDim.x) * (TPB / 64);
+    for (int64_t w = (blockIdx.x * (int64_t)TPB + threadIdx.x) >> 6; w < cnt; w += nwave) {
+        const int64_t i = ax.sund[1 + w];
+        const int np = s.surf[i] - kSurfUndecided;
+        bool found = false;
+        if (np <= kSurfPendMax) {
+            if (lane < np) {
+                const int4 pend = ax.spend[i];
+                const int32_t sl = lane == 0 ? pend.x : lane == 1 ? pend.y : lane == 2 ? pend.z : pend.w;
+                found = marrone_exact<D, FR>(c, s, ax, i, sl);
+            }
+        } else {
+            int32_t seg_a[MAXSEG], pre[MAXSEG + 1];
+            int nseg, p0, p1;
+            union_of_block<D>(tg, block_of_slot(tg, i), seg_a, pre, nseg, p0, p1);
+            const int NT = n_tiles(pre[nseg]);
+            const int64_t p = tg.pos[i];
+            const int nrows = min(nl.count[i], nl.cap >> 2);
+            for (int r = lane; r < nrows && !found; r += 64) {
+                const uint64_t u = nl.lu4[(int64_t)r * nl.npad + p];
+                for (int q = 0; q < 4 && !found; ++q) {
+                    const uint32_t uq = uint32_t(u >> (16 * q)) & 0xFFFFu;
+                    if (uq != NO_ENTRY)
+                        found = marrone_exact<D, FR>(c, s, ax, i, union_slot(tg, seg_a, pre, nseg, entry_union_index(uq, NT)));
+                }
+            }
+        }
+        found = __any(found);
+        if (lane == 0) s.surf[i] = found ? 0 : 1;
+    }
 }
 
 template <int D, typename T, int MODE>
