@@ -6,7 +6,7 @@
  * the cell-sorted order[] array, recorded as the target order vorder[]) and
  * its candidate union is the 3^(D-1) neighbour rows over cells
  * [cx0 - 1, cx1].  The union is staged in LDS in tiles of TILE records,
- * dealt to the tiles round robin in groups of 4 (tile_union_index); each
+ * dealt to the tiles round robin in groups of TGRP = 8 (tile_union_index); each
  * lane walks its own neighbour list tile by tile.  Pair terms never
  * scatter: every pass is owner-computes.
  *
