@@ -313,6 +313,34 @@ def test_step_parity_mixed_droplet():
     print("mixed per-step errors:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
 
 
+def test_step_parity_mixed_ale_2d():
+    """A 2D droplet with ALE shifting in mixed precision: two steps, whose
+    pre-passes run Detect_Surface loops 2-3 and particle_shift as one fused
+    2D kernel, against the FP64 oracle."""
+    case = _perturb(cases.droplet_2d(diameter=2e-3, dx=5e-5, precision=abi.PREC_MIXED), seed=9, dv=0.01, drho=1e-4)
+    case.settings.ale = 1
+    case.settings.cfl = 0.3  # Newmark-Beta converges (10-11 sub-iterations) instead of running to the limit
+    print(f"[mixed step, 2D ALE] droplet n={case.n}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    eng.integrate_no_update()
+    orc.integrate_no_update()
+    for _ in range(2):
+        ie, io = eng.integrate_step(), orc.integrate_step()
+        print(f"sub-iterations: engine {ie['iterations']}, oracle {io['iterations']}", flush=True)
+    rho0 = case.settings.rho_rest
+    errs = {}
+    for f, g, r in (("X-x0", eng.field("X") - case.x, orc.field("X") - case.x),
+                    ("V", eng.field("V"), orc.field("V")),
+                    ("RHO-rho0", eng.field("RHO") - rho0, orc.field("RHO") - rho0),
+                    ("ACC", eng.field("ACC"), orc.field("ACC")),
+                    ("CURVE", eng.field("CURVE"), orc.field("CURVE")),
+                    ("WOCCL", eng.field("WOCCL"), orc.field("WOCCL"))):
+        errs[f] = assert_close(f, g, r, MIXED_STEP)
+    assert np.array_equal(eng.field("SURFZONE"), orc.field("SURFZONE"))
+    errs["VPERT"] = assert_close("VPERT", eng.field("VPERT"), orc.field("VPERT"), MIXED_VPERT)
+    print("mixed 2D ALE errors:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+
+
 def test_neighbours_full_droplet():
     """Bit-exact sorted neighbour sets on the 268k-particle droplet IC."""
     case = cases.droplet_3d()
