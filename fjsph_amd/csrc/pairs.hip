@@ -284,7 +284,6 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
 {
     __shared__ double tp[D][TILE];
     __shared__ float4 tf[(TILE + 31) / 32 * 32]; /* read in chunks of 32 */
-    __shared__ int32_t tsl[TILE];
     __shared__ BlockHead h;
     block_head<D>(tg, h);
     const int U = h.pre[h.nseg];
@@ -330,6 +329,17 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
         const int cxi = active ? (int)(tg.cell[i] % tg.dims[0]) : 0;
         const int wlo = wave_min(active ? cxi : 0x7fffffff) - 1;
         const int whi = wave_max(active ? cxi : -0x7fffffff) + 1;
+        /* the target's own slot in the union (tile k_self, slot t_self):
+         * the segment of the block's own cell row holds its order position */
+        int k_self = -1, t_self = 0;
+        if (active) {
+            int sg = 0;
+            while (sg + 1 < h.nseg && !(p >= h.seg_a[sg] && p < h.seg_a[sg] + (h.pre[sg + 1] - h.pre[sg]))) ++sg;
+            const int u = h.pre[sg] + (p - h.seg_a[sg]);
+            const int g = u / TGRP;
+            k_self = g % NT;
+            t_self = (g / NT) * TGRP + u % TGRP;
+        }
         int nrow = 0, nreal = 0;
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
@@ -338,12 +348,10 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
                 const int u = tile_union_index(tk, t, NT);
                 if (u >= U) {
                     /* padding: far away, never a neighbour */
-                    tsl[t] = -1;
                     tf[t] = make_float4(1e30f, 1e30f, 1e30f, -2.f);
                     continue;
                 }
                 const int32_t sl = union_slot(tg, h.seg_a, h.pre, h.nseg, u);
-                tsl[t] = sl;
                 float pa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
@@ -373,45 +381,54 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
             if (active) {
                 int q = 0;
                 uint64_t cu = ~0ull;
-                /* 32 candidates at a time: a branch-free FP32 test into a
-                 * bit mask of those not surely out, then per marked
-                 * candidate (ascending) surely in, or the FP64 test */
+                /* 32 candidates at a time: a branch-free FP32 test into two
+                 * bit masks, those not surely out (near) and those surely
+                 * in; the FP64 test decides the rest of near (the shell),
+                 * and the target's own slot is masked out; the neighbours
+                 * are then emitted in ascending order */
                 for (int c0 = 0; c0 < nt; c0 += 32) {
                     const uint32_t gm = uint32_t(gmask[c0 >> 8] >> ((c0 >> 2) & 63)) & 0xFFu;
                     if (gm == 0u) continue;
-                    uint32_t near = 0u;
+                    uint32_t near = 0u, in = 0u;
+                    /* candidates in descending order, each shifted in at
+                     * bit 0, so candidate k ends at bit k */
 #pragma unroll
-                    for (int gj = 0; gj < 8; ++gj) {
-                        if (!((gm >> gj) & 1u)) continue;
+                    for (int gj = 7; gj >= 0; --gj) {
+                        if (!((gm >> gj) & 1u)) {
+                            near <<= 4;
+                            in <<= 4;
+                            continue;
+                        }
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            const int k = 4 * gj + e;
-                            const float4 P = tf[c0 + k];
+                        for (int e = 3; e >= 0; --e) {
+                            const float4 P = tf[c0 + 4 * gj + e];
                             const float d0 = P.x - xit[0], d1 = P.y - xit[1], d2f = P.z - xit[2];
                             const float rr = d0 * d0 + d1 * d1 + d2f * d2f;
-                            near |= (rr <= sr_hi ? 1u : 0u) << k;
+                            near = near + near + uint32_t(rr <= sr_hi);
+                            in = in + in + uint32_t(rr < sr_lo);
                         }
                     }
                     const int rem = nt - c0;
                     near &= rem >= 32 ? ~0u : ((1u << rem) - 1u);
-                    while (near) {
-                        const int k = __builtin_ctz(near);
-                        near &= near - 1u;
-                        const int t = c0 + k;
-                        {
-                            const float4 P = tf[t];
-                            const float d0 = P.x - xit[0], d1 = P.y - xit[1], d2f = P.z - xit[2];
-                            if (!(d0 * d0 + d1 * d1 + d2f * d2f < sr_lo)) {
-                                /* the shell: the FP64 test decides */
-                                double xj[D];
+                    in &= near;
+                    if (tk == k_self && unsigned(t_self - c0) < 32u) {
+                        const uint32_t sb = 1u << (t_self - c0);
+                        near &= ~sb;
+                        in &= ~sb;
+                    }
+                    for (uint32_t shell = near & ~in; shell;) {
+                        const int k = __builtin_ctz(shell);
+                        shell &= shell - 1u;
+                        double xj[D];
 #pragma unroll
-                                for (int a = 0; a < D; ++a) xj[a] = tp[a][t];
-                                if (!(exact_dist2<D>(xi, xj) < sr)) continue;
-                            }
-                        }
-                        if (tsl[t] == (int32_t)i) continue;
-                        cu = (cu >> 16) | (uint64_t(base + t) << 48);
-                        ++nreal;
+                        for (int a = 0; a < D; ++a) xj[a] = tp[a][c0 + k];
+                        if (exact_dist2<D>(xi, xj) < sr) in |= 1u << k;
+                    }
+                    nreal += __builtin_popcount(in);
+                    while (in) {
+                        const int k = __builtin_ctz(in);
+                        in &= in - 1u;
+                        cu = (cu >> 16) | (uint64_t(base + c0 + k) << 48);
                         if (++q == 4) {
                             if (nrow < rcap) nl.lu4[(int64_t)nrow * nl.npad + p] = cu;
                             ++nrow;
