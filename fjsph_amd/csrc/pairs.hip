@@ -257,12 +257,20 @@ __device__ __forceinline__ int wave_max(int v)
     return v;
 }
 
-__device__ __forceinline__ int wave_min(int v)
+__device__ __forceinline__ float wave_minf(float v)
 {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
     return v;
 }
+
+__device__ __forceinline__ float wave_maxf(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 
 /* ------------------------------------------------------------ list build */
 
@@ -323,12 +331,15 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
             xi[a] = active ? x[i * D + a] : xref[a];
             xit[a] = float(xi[a] - xref[a]);
         }
-        /* x-cells within reach of the wave's targets (the block's targets
-         * share one cell row; a candidate two cells away in x is farther
-         * than the cell size 2H) */
-        const int cxi = active ? (int)(tg.cell[i] % tg.dims[0]) : 0;
-        const int wlo = wave_min(active ? cxi : 0x7fffffff) - 1;
-        const int whi = wave_max(active ? cxi : -0x7fffffff) + 1;
+        /* bounding box of the wave's targets (FP32, relative to xref): a
+         * candidate farther than the cut-off from it is no neighbour of any
+         * of them */
+        float blo[3] = {0.f, 0.f, 0.f}, bhi[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            blo[a] = wave_minf(active ? xit[a] : HUGE_VALF);
+            bhi[a] = wave_maxf(active ? xit[a] : -HUGE_VALF);
+        }
         /* the target's own slot in the union (tile k_self, slot t_self):
          * the segment of the block's own cell row holds its order position */
         int k_self = -1, t_self = 0;
@@ -341,6 +352,7 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
             t_self = (g / NT) * TGRP + u % TGRP;
         }
         int nrow = 0, nreal = 0;
+        uint64_t* rowp = nl.lu4 + p; /* row nrow of the target */
         for (int tk = 0, base = 0; tk < NT; ++tk, base += TILE) {
             const int nt = tile_len(tk, U, NT);
             __syncthreads();
@@ -358,11 +370,13 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
                     tp[a][t] = x[(int64_t)sl * D + a];
                     pa[a] = float(tp[a][t] - xref[a]);
                 }
-                tf[t] = make_float4(pa[0], pa[1], pa[2], float(tg.cell[sl] % tg.dims[0]));
+                tf[t] = make_float4(pa[0], pa[1], pa[2], 0.f);
             }
             __syncthreads();
-            /* groups of 4 slots with a candidate in reach: one bit per
-             * group, wave-uniform (tiles hold at most 128 groups) */
+            /* groups of 4 slots with a candidate in reach of the wave's
+             * box (squared distance to it at most sr_hi, which every
+             * neighbour's FP32 squared distance to its target meets): one
+             * bit per group, wave-uniform (tiles hold at most 128 groups) */
             uint64_t gmask[2];
 #pragma unroll
             for (int hgrp = 0; hgrp < 2; ++hgrp) {
@@ -371,8 +385,15 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
                 if (4 * g < nt) {
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const float cx = tf[4 * g + e].w;
-                        reach = reach || (cx >= float(wlo) && cx <= float(whi));
+                        const float4 P = tf[4 * g + e];
+                        const float pp[3] = {P.x, P.y, P.z};
+                        float d2b = 0.f;
+#pragma unroll
+                        for (int a = 0; a < D; ++a) {
+                            const float da = fmaxf(fmaxf(blo[a] - pp[a], pp[a] - bhi[a]), 0.f);
+                            d2b += da * da;
+                        }
+                        reach = reach || d2b <= sr_hi;
                     }
                 }
                 gmask[hgrp] = __ballot(reach);
@@ -430,8 +451,9 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
                         in &= in - 1u;
                         cu = (cu >> 16) | (uint64_t(base + c0 + k) << 48);
                         if (++q == 4) {
-                            if (nrow < rcap) nl.lu4[(int64_t)nrow * nl.npad + p] = cu;
+                            if (nrow < rcap) *rowp = cu;
                             ++nrow;
+                            rowp += nl.npad;
                             ++rows_t;
                             q = 0;
                             cu = ~0ull;
@@ -441,8 +463,9 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
                 if (q > 0) {
                     /* close the tile's last row: entries to the low slots */
                     cu = (cu >> (16 * (4 - q))) | (~0ull << (16 * q));
-                    if (nrow < rcap) nl.lu4[(int64_t)nrow * nl.npad + p] = cu;
+                    if (nrow < rcap) *rowp = cu;
                     ++nrow;
+                    rowp += nl.npad;
                     ++rows_t;
                 }
             }
@@ -450,8 +473,9 @@ __global__ __launch_bounds__(TPB) void k_build_t(const double* __restrict__ x, T
             const int wrows = wave_max(rows_t);
             if (active) {
                 for (int k = rows_t; k < wrows; ++k) {
-                    if (nrow < rcap) nl.lu4[(int64_t)nrow * nl.npad + p] = ~0ull;
+                    if (nrow < rcap) *rowp = ~0ull;
                     ++nrow;
+                    rowp += nl.npad;
                 }
             }
         }
