@@ -201,24 +201,28 @@ __device__ __forceinline__ void split_rel(double x, double ref, float& hi, float
  * leaves a tile only when no lane has an entry of it left, so a lane that
  * dropped out could leave the others on padding rows of the tile. */
 struct Rows {
-    int64_t p;
-    int row, nrows;
+    const uint64_t* rp; /* the next row (row nxt of the target) */
+    int64_t stride;
+    int nxt, nrows;
     bool on;
     uint64_t u, un;
-    __device__ __forceinline__ void init(const NbrList& nl, int64_t p_, int nrows_, bool on_)
+    __device__ __forceinline__ void init(const NbrList& nl, int64_t p, int nrows_, bool on_)
     {
-        p = p_;
-        row = 0;
         on = on_;
         nrows = nrows_;
-        u = nrows > 0 ? nl.lu4[p] : ~0ull;
-        un = nrows > 1 ? nl.lu4[nl.npad + p] : ~0ull;
+        stride = nl.npad;
+        rp = nl.lu4 + p;
+        u = nrows > 0 ? *rp : ~0ull;
+        rp += stride;
+        nxt = 1;
+        un = nrows > 1 ? *rp : ~0ull;
     }
-    __device__ __forceinline__ void advance(const NbrList& nl)
+    __device__ __forceinline__ void advance(const NbrList&)
     {
-        ++row;
         u = un;
-        un = (row + 1 < nrows) ? nl.lu4[(int64_t)(row + 1) * nl.npad + p] : ~0ull;
+        rp += stride;
+        ++nxt;
+        un = nxt < nrows ? *rp : ~0ull;
     }
 };
 
