@@ -342,6 +342,7 @@ void free_all(fjsph_ctx* ctx)
     dfree(ctx->order); dfree(ctx->vorder); dfree(ctx->pos); dfree(ctx->tile_sums);
     dfree(ctx->nl.lu4); dfree(ctx->nl.count); dfree(ctx->nl.nnb); dfree(ctx->export_buf);
     dfree(ctx->aux.vol); dfree(ctx->aux.P); dfree(ctx->aux.invrho); dfree(ctx->aux.fq); dfree(ctx->aux.fale);
+    dfree(ctx->aux.fs1); dfree(ctx->aux.fs2);
     dfree(ctx->aux.nhat); dfree(ctx->aux.lnb); dfree(ctx->aux.xb); dfree(ctx->aux.spend); dfree(ctx->aux.sund); dfree(ctx->bnd_slots); dfree(ctx->aero_list);
     dfree(ctx->aero_counts);
     ctx->aero_list_cap = 0;
@@ -437,6 +438,10 @@ void alloc_slots(fjsph_ctx* ctx, int64_t cap, int64_t keep)
     dgrow(ctx->aux.invrho, c, 0);
     dgrow(ctx->aux.fq, c, 0);
     dgrow(ctx->aux.fale, c, 0);
+    dgrow(ctx->aux.fs1, c, 0);
+    dgrow(ctx->aux.fs2, c, 0);
+    HIPCHK(hipMemset(ctx->aux.fs1, 0, c * sizeof(float4)));
+    HIPCHK(hipMemset(ctx->aux.fs2, 0, c * sizeof(float4)));
     dgrow(ctx->aux.nhat, c * D, k * D);
     HIPCHK(hipMemset(ctx->aux.nhat + k * D, 0, (c - k) * D * 8));
     dgrow(ctx->aux.lnb, c * D * D, 0);
@@ -785,7 +790,8 @@ void require(fjsph_ctx* ctx, bool nbrs)
 
 void refresh_aux(fjsph_ctx* ctx)
 {
-    launch_aux(ctx->s.dim, ctx->pnp1, ctx->aux, ctx->n, ctx->s.ale, ctx->s.precision == FJSPH_PREC_MIXED, ctx->st);
+    launch_aux(ctx->s.dim, ctx->pnp1, ctx->aux, ctx->n, ctx->s.ale, ctx->s.precision == FJSPH_PREC_MIXED, ctx->c.B * ctx->c.gam,
+               ctx->st);
 }
 
 int r_from_x(const fjsph_ctx* ctx) { return ctx->x_at_build ? 1 : 0; }
@@ -1493,7 +1499,7 @@ void rk_stage(fjsph_ctx* ctx, DevState& dst, DevState& prev, double dt, bool fin
     if (ctx->nb > 0) {
         PassTimer t(ctx, FJSPH_T_BOUNDARY);
         upload_block_velocities(ctx, true);
-        launch_aux(D, dst, ctx->aux, ctx->n, ctx->s.ale, mixed, ctx->st);
+        launch_aux(D, dst, ctx->aux, ctx->n, ctx->s.ale, mixed, ctx->c.B * ctx->c.gam, ctx->st);
         for (size_t k = 0; k < ctx->blocks.size(); ++k) {
             if (ctx->blocks[k].b.is_fluid) continue;
             launch_boundary(D, ctx->c, dst, ctx->aux, ctx->blk, ctx->bl, int32_t(k), ctx->tg, ctx->nl, ctx->n,
@@ -1502,7 +1508,7 @@ void rk_stage(fjsph_ctx* ctx, DevState& dst, DevState& prev, double dt, bool fin
         launch_rk_bnd_rho(D, ctx->c, dst, ctx->pn, ctx->rk1, ctx->rk2, ctx->rk3, ctx->blk, ctx->bl, ctx->near_inlet,
                           ctx->n, dt, final_stage ? 1 : 0, ctx->st);
     }
-    launch_aux(D, dst, ctx->aux, ctx->n, ctx->s.ale, mixed, ctx->st);
+    launch_aux(D, dst, ctx->aux, ctx->n, ctx->s.ale, mixed, ctx->c.B * ctx->c.gam, ctx->st);
     {
         PassTimer t(ctx, FJSPH_T_FORCE);
         launch_force(D, ctx->c, dst, ctx->aux, ctx->part_id, ctx->tg, ctx->nl, 0, ctx->sc, ctx->st);

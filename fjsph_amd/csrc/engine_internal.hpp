@@ -112,6 +112,9 @@ struct Aux {
     double* invrho;  /* 1 / rho            */
     float4* fq;      /* {m (negative for BOUND), p/rho^2, 1/rho, m/rho} in FP32 */
     float4* fale;    /* {vPert, rho} in FP32 (ALE)                          */
+    float4* fs1;     /* {gradRho, lambda} in FP32 (mixed), by dSPH_PreStep   */
+    float4* fs2;     /* {m/rho, sqrt(B gamma / rho), m (negative for PISTON
+                        and BOUND), 0} in FP32 (mixed)                      */
     double* nhat;    /* normalised surface normal (after Detect_Surface)     */
     double* lnb;     /* dSPH_PreStep: correction matrix over non-boundary
                         neighbours (D x D per slot), for its eigenvalue     */
@@ -257,7 +260,7 @@ void launch_sort_cells(const int32_t* start, int64_t ncell, int32_t* order, hipS
 void launch_build_lists(int dim, const double* x, int64_t n, TileGrid tg, double sr, NbrList nl, DevScalars* sc,
                         hipStream_t st);
 void launch_export_lists(int dim, TileGrid tg, NbrList nl, int64_t n, int32_t* out, hipStream_t st);
-void launch_aux(int dim, DevState s, Aux a, int64_t n, int ale, int mixed, hipStream_t st);
+void launch_aux(int dim, DevState s, Aux a, int64_t n, int ale, int mixed, double bgam, hipStream_t st);
 void launch_gather_d(const double* src, double* dst, const int32_t* perm, int64_t n, int width, hipStream_t st);
 void launch_gather_i(const int32_t* src, int32_t* dst, const int32_t* perm, int64_t n, hipStream_t st);
 void launch_iota(int32_t* a, int64_t n, hipStream_t st);

@@ -301,7 +301,7 @@ void launch_sort_cells(const int32_t* start, int64_t ncell, int32_t* order, hipS
 
 /* Per-particle derived quantities (see Aux). */
 template <int D>
-__global__ __launch_bounds__(TPB) void k_aux(DevState s, Aux a, int64_t n, int ale, int mixed)
+__global__ __launch_bounds__(TPB) void k_aux(DevState s, Aux a, int64_t n, int ale, int mixed, float bgam)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n) return;
@@ -311,8 +311,11 @@ __global__ __launch_bounds__(TPB) void k_aux(DevState s, Aux a, int64_t n, int a
     a.P[i] = P;
     a.invrho[i] = ir;
     if (mixed) {
-        const float ms = s.b[i] == FJSPH_BOUND ? -float(m) : float(m);
+        const int32_t b = s.b[i];
+        const float ms = b == FJSPH_BOUND ? -float(m) : float(m);
         a.fq[i] = make_float4(ms, float(P), float(ir), float(vol));
+        a.fs2[i] = make_float4(float(vol), __builtin_amdgcn_sqrtf(bgam * float(ir)),
+                               b > FJSPH_PISTON ? float(m) : -float(m), 0.f);
         if (ale) {
             float vp[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -322,10 +325,11 @@ __global__ __launch_bounds__(TPB) void k_aux(DevState s, Aux a, int64_t n, int a
     }
 }
 
-void launch_aux(int dim, DevState s, Aux a, int64_t n, int ale, int mixed, hipStream_t st)
+void launch_aux(int dim, DevState s, Aux a, int64_t n, int ale, int mixed, double bgam, hipStream_t st)
 {
-    if (dim == 2) hipLaunchKernelGGL(k_aux<2>, dim3(grid_of(n)), dim3(TPB), 0, st, s, a, n, ale, mixed);
-    else hipLaunchKernelGGL(k_aux<3>, dim3(grid_of(n)), dim3(TPB), 0, st, s, a, n, ale, mixed);
+    const float bg = float(bgam);
+    if (dim == 2) hipLaunchKernelGGL(k_aux<2>, dim3(grid_of(n)), dim3(TPB), 0, st, s, a, n, ale, mixed, bg);
+    else hipLaunchKernelGGL(k_aux<3>, dim3(grid_of(n)), dim3(TPB), 0, st, s, a, n, ale, mixed, bg);
 }
 
 __global__ __launch_bounds__(TPB) void k_gather_d(const double* __restrict__ src, double* __restrict__ dst,

@@ -776,8 +776,8 @@ __global__ __launch_bounds__(TPB) FJ_DSPH_OCC void k_dsph_m(Consts c, DevState s
                     pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
                     if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
                 }
-                const float v = float(ax.vol[sl]);
-                tA[t] = make_float4(pa[0], pa[1], pa[2], s.b[sl] > FJSPH_PISTON ? v : -v);
+                const float4 r2 = ax.fs2[sl]; /* V, .., m negative for PISTON and BOUND */
+                tA[t] = make_float4(pa[0], pa[1], pa[2], copysignf(r2.x, r2.z));
                 tR[t] = float(s.rho[sl] - rref);
                 if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
             }
@@ -884,7 +884,8 @@ __global__ __launch_bounds__(TPB) FJ_DSPH_OCC void k_dsph_m(Consts c, DevState s
  * for the constant-velocity source. */
 template <int D>
 __global__ __launch_bounds__(TPB) void k_dsph_fin(Consts c, DevState s, Aux ax, const int32_t* __restrict__ part_id,
-                                                  const int32_t* __restrict__ nnb, int64_t n, int with_aero)
+                                                  const int32_t* __restrict__ nnb, int64_t n, int with_aero,
+                                                  int mixed)
 {
     const int64_t i = blockIdx.x * (int64_t)TPB + threadIdx.x;
     if (i >= n) return;
@@ -921,6 +922,7 @@ __global__ __launch_bounds__(TPB) void k_dsph_fin(Consts c, DevState s, Aux ax, 
     double gR[D], nr[D];
     load_vec<D>(s.gradRho, i, gR);
     load_vec<D>(s.norm, i, nr);
+    float gf[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         double g = 0.0, q = 0.0;
@@ -932,7 +934,10 @@ __global__ __launch_bounds__(TPB) void k_dsph_fin(Consts c, DevState s, Aux ax, 
         }
         s.gradRho[i * D + a] = g;
         s.norm[i * D + a] = q;
+        gf[a] = float(g);
     }
+    /* the FP32 staging record of the surface and dissipation pass */
+    if (mixed) ax.fs1[i] = make_float4(gf[0], gf[1], gf[2], float(lam));
     const double colour = s.colour[i];
     s.colourG[i] = lam > 0.7 ? 2.0 : 2.0 * fmax(1.0, 1.0 / (2.0 * colour));
     if (with_aero && c.aero_source == FJSPH_CONST_VEL && part_id[i] >= c.n_bound)
@@ -960,9 +965,11 @@ void launch_dsph(int dim, const Consts& c, DevState s, Aux a, const int32_t* par
     }
     launch_sum_partials_scaled(partial, (int)tg.nblocks, 1, 1.0 / double(n), &sc->npd, st);
     if (dim == 2)
-        hipLaunchKernelGGL(k_dsph_fin<2>, dim3(grid_of(n)), b, 0, st, c, s, a, part_id, nl.nnb, n, with_aero_vel);
+        hipLaunchKernelGGL(k_dsph_fin<2>, dim3(grid_of(n)), b, 0, st, c, s, a, part_id, nl.nnb, n, with_aero_vel,
+                           nl.f32);
     else
-        hipLaunchKernelGGL(k_dsph_fin<3>, dim3(grid_of(n)), b, 0, st, c, s, a, part_id, nl.nnb, n, with_aero_vel);
+        hipLaunchKernelGGL(k_dsph_fin<3>, dim3(grid_of(n)), b, 0, st, c, s, a, part_id, nl.nnb, n, with_aero_vel,
+                           nl.f32);
 }
 
 template <int D>
@@ -1052,14 +1059,17 @@ __device__ __noinline__ bool marrone_exact(const Consts& c, const DevState& s, c
 template <int D, int MODE>
 __device__ __forceinline__ void surface1_out(const Consts& c, DevState& s, int64_t i, bool fluid, int32_t bi,
                                              double lam_nb, double lami, bool found, const double* nrm,
-                                             const double* av, double rd, double* norms_tmp)
+                                             const double* av, double rd, double* norms_tmp, double* nhat)
 {
     constexpr bool with_surf = MODE != 2;
     constexpr bool with_diss = MODE != 0;
     if (!fluid) {
         if constexpr (with_surf) {
 #pragma unroll
-            for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = 0.0;
+            for (int a = 0; a < D; ++a) {
+                norms_tmp[i * D + a] = 0.0;
+                nhat[i * D + a] = 0.0;
+            }
         }
         return;
     }
@@ -1090,8 +1100,15 @@ __device__ __forceinline__ void surface1_out(const Consts& c, DevState& s, int64
 #pragma unroll
             for (int a = 0; a < D; ++a) out[a] = 0.0;
         }
+        /* the unit normal of loops 2-3 and particle_shift (the normalised
+         * loop-1 normal, as Detect_Surface forms it after its loops) */
+        double nh[D];
+        normalize<D>(out, nh);
 #pragma unroll
-        for (int a = 0; a < D; ++a) norms_tmp[i * D + a] = out[a];
+        for (int a = 0; a < D; ++a) {
+            norms_tmp[i * D + a] = out[a];
+            nhat[i * D + a] = nh[a];
+        }
     }
 }
 
@@ -1236,7 +1253,7 @@ __global__ __launch_bounds__(TPB) void k_surface1_t(Consts c, DevState s, Aux ax
                 }
             FJ_TILE_ENTRIES_END(cu, nl)
         }
-        if (active) surface1_out<D, MODE>(c, s, i, fluid, bi, lam_nb, lami, found, nrm, av, rd, norms_tmp);
+        if (active) surface1_out<D, MODE>(c, s, i, fluid, bi, lam_nb, lami, found, nrm, av, rd, norms_tmp, ax.nhat);
     }
 }
 
@@ -1288,7 +1305,7 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
     const float h133f = float(h133), r_limf = float(r_lim);
     const float tol = float(1e-3 * h133), tolc = 1e-3f;
     const float cos45 = float(cos(kPi / 4.0));
-    const float Bg = float(c.B * c.gam), Ht = float(c.H), inv_H = float(1.0 / c.H), Wc = float(c.Wc);
+    const float Ht = float(c.H), inv_H = float(1.0 / c.H), Wc = float(c.Wc);
     const float eps2 = float(0.0001 * c.H_sq), alpha = float(c.visc_alpha);
     const float rreff = float(rref);
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
@@ -1363,17 +1380,16 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
                     if constexpr (with_diss) va[a] = float(s.v[(int64_t)sl * D + a] - vref[a]);
                     if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
                 }
-                tA[t] = make_float4(pa[0], pa[1], pa[2], float(ax.vol[sl]));
-                tB[t] = make_float4(va[0], va[1], va[2], with_surf ? float(s.lam[sl]) : 0.f);
+                /* the slot's FP32 records: {gradRho, lambda} from dSPH_PreStep,
+                 * {V, sqrt(B gamma / rho), +-m} from the aux refresh */
+                const float4 r1 = ax.fs1[sl], r2 = ax.fs2[sl];
+                tA[t] = make_float4(pa[0], pa[1], pa[2], r2.x);
+                tB[t] = make_float4(va[0], va[1], va[2], with_surf ? r1.w : 0.f);
                 if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
                 if constexpr (with_surf) tsl[t] = sl;
                 if constexpr (with_diss) {
-                    float ga[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int a = 0; a < D; ++a) ga[a] = float(s.gradRho[(int64_t)sl * D + a]);
-                    tG[t] = make_float4(ga[0], ga[1], ga[2], float(s.rho[sl] - rref));
-                    const float mj = float(s.m[sl]);
-                    tE[t] = make_float2(fast_sqrt<float>(Bg * float(ax.invrho[sl])), s.b[sl] > FJSPH_PISTON ? mj : -mj);
+                    tG[t] = make_float4(r1.x, r1.y, r1.z, float(s.rho[sl] - rref));
+                    tE[t] = make_float2(r2.y, r2.z);
                 }
             }
             ts.load(tg, h, tk + 1, NT, U);
@@ -1490,7 +1506,7 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
             rd += double(frd);
             frd = 0.f;
         }
-        if (active) surface1_out<D, MODE>(c, s, i, fluid, bi, lam_nb, lami, found, nrm, av, rd, norms_tmp);
+        if (active) surface1_out<D, MODE>(c, s, i, fluid, bi, lam_nb, lami, found, nrm, av, rd, norms_tmp, ax.nhat);
         if constexpr (with_surf) {
             /* undecided: k_surface1_resolve sets the flag from the FP64 test */
             if (testing && !found && npend > 0) {
@@ -2162,13 +2178,15 @@ void launch_shift(int dim, const Consts& c, DevState s, Aux a, const int32_t* pa
 /* Detect_Surface loops 2-3 (k_surface2_m) and particle_shift (k_shift_m)
  * in one staged pass, for ALE builds in mixed precision: the shifting pass
  * reads of its neighbours only what Detect_Surface loop 1 and dSPH_PreStep
- * produced (positions, velocities, volumes and the unit normal, which is
- * the normalised loop-1 normal) and of its target the surface-zone flag
- * that loops 2-3 form in the same walk, so one staging and one list walk
- * serve both (src/Geometry.cpp:145-277, src/Shifting.cpp:189-290).  The
- * pair terms and every decision are those of the two separate kernels,
- * which the per-pass entry points keep.  72 B of LDS per candidate (88 B
- * with build positions). */
+ * produced (positions, velocities, volumes and the unit normal, the
+ * normalised loop-1 normal that loop 1 writes to Aux::nhat) and of its
+ * target the surface-zone flag that loops 2-3 form in the same walk, so one
+ * staging and one list walk serve both (src/Geometry.cpp:145-277,
+ * src/Shifting.cpp:189-290).  The pair terms and every decision are those
+ * of the two separate kernels, which the per-pass entry points keep; the
+ * FP32 normals of the curvature sum are the unit normals (the loop-1
+ * normals are unit vectors or zero, so they differ in the last bit at
+ * most).  72 B of LDS per candidate (88 B with build positions). */
 #ifndef FJ_SURF2SHIFT_OCC
 #define FJ_SURF2SHIFT_OCC __attribute__((amdgpu_waves_per_eu(4, 8)))
 #endif
@@ -2206,14 +2224,10 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
         const bool work = fluid && !(lam_nb < 0.55 || s.b[i] == FJSPH_BUFFER);
         float xit[D], xlo[D], xbit[D], xblo[D], vit[D], vlo[D], ni[D], Vd[D], Li[D][D], fdu[D];
         double nhat[D], du[D];
-        {
-            double n64[D];
 #pragma unroll
-            for (int a = 0; a < D; ++a) {
-                n64[a] = fluid ? norms_tmp[i * D + a] : 0.0;
-                ni[a] = float(n64[a]);
-            }
-            normalize<D>(n64, nhat);
+        for (int a = 0; a < D; ++a) {
+            nhat[a] = fluid ? ax.nhat[i * D + a] : 0.0;
+            ni[a] = float(nhat[a]);
         }
 #pragma unroll
         for (int a = 0; a < D; ++a) {
@@ -2254,19 +2268,17 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
                 if (sl < 0) continue;
                 float pa[3] = {0.f, 0.f, 0.f}, va[3] = {0.f, 0.f, 0.f}, na[3] = {0.f, 0.f, 0.f},
                       xa[3] = {0.f, 0.f, 0.f};
-                double nj64[D], njh[D];
-                load_vec<D>(norms_tmp, sl, nj64);
-                normalize<D>(nj64, njh);
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
+                    const double nj = ax.nhat[(int64_t)sl * D + a];
                     pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
                     va[a] = float(s.v[(int64_t)sl * D + a] - vref[a]);
-                    na[a] = float(nj64[a]);
-                    tn[a][t] = njh[a];
+                    na[a] = float(nj);
+                    tn[a][t] = nj;
                     if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
                 }
-                const float v = float(ax.vol[sl]);
-                tA[t] = make_float4(pa[0], pa[1], pa[2], s.b[sl] > FJSPH_PISTON ? v : -v);
+                const float4 r2 = ax.fs2[sl]; /* V, .., m negative for PISTON and BOUND */
+                tA[t] = make_float4(pa[0], pa[1], pa[2], copysignf(r2.x, r2.z));
                 tN[t] = make_float4(na[0], na[1], na[2], s.surf[sl] == 1 ? 1.f : 0.f);
                 tV[t] = make_float4(va[0], va[1], va[2], 0.f);
                 if constexpr (FR) tX[t] = make_float4(xa[0], xa[1], xa[2], 0.f);
@@ -2361,10 +2373,7 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
         const double wo = (occl && Vdn > 0.0) ? double(qmax) / Vdn : 0.0;
         s.woccl[i] = lam_nb < c.lam_cutoff ? fmax(0.0, fmin(wo, 1.0)) : 1.0;
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            s.norm[i * D + a] = norms_tmp[i * D + a];
-            ax.nhat[i * D + a] = nhat[a];
-        }
+        for (int a = 0; a < D; ++a) s.norm[i * D + a] = norms_tmp[i * D + a];
         s.curve[i] = curve;
         s.norm_curve[i] = c.dx * curve;
         s.pDist[i] = s.lam[i];
