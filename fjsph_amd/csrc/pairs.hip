@@ -3054,7 +3054,10 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_m(Consts c, DevState
                 const float idist2 = fast_rcp<float>(rr + eps2);
                 const float qh = r * inv_H;
                 const float tq = 1.f - 0.5f * qh;
-                const float fg = (qh < 1e-12f) ? 0.f : fgc * (tq * tq * tq);
+                /* W'(r)/r is zero below r = 1e-12 H (src/Kernel.h:48-61); with r
+                 * from the current positions such an FP32 distance is zero and so
+                 * is R_ji, which carries every term of the pair */
+                const float fg = (FR && qh < 1e-12f) ? 0.f : fgc * (tq * tq * tq);
                 float g[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) g[a] = fg * Rji[a];
