@@ -3069,10 +3069,18 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_m(Consts c, DevState
                     vg += Vji[a] * g[a];
                 }
                 float vf = mj * nu * (iri + irj) * rdotg * idist2;
+                /* surface tension: with ALE only the surface zone has it, so
+                 * its terms go to the sums inside the branch, which most waves
+                 * skip; without ALE every fluid target has it and the terms
+                 * join the momentum sum below */
                 float sf = 0.f;
                 if (st_on) {
                     const float fac = (bound_i || bound_j) ? contang_fac : 1.f;
                     sf = stm * fac * fast_cos<float>(pi3o4H * r) * fast_rcp<float>(r);
+                    if constexpr (ALE) {
+#pragma unroll
+                        for (int a = 0; a < D; ++a) fa[a] += sf * Rji[a];
+                    }
                 }
                 if constexpr (ALE) {
                     const float4 E = ft.E[t];
@@ -3088,8 +3096,13 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_m(Consts c, DevState
                 } else {
                     fr += volj * vg;
                 }
+                if constexpr (ALE) {
 #pragma unroll
-                for (int a = 0; a < D; ++a) fa[a] += vf * Vji[a] + sf * Rji[a] - pc * g[a];
+                    for (int a = 0; a < D; ++a) fa[a] += vf * Vji[a] - pc * g[a];
+                } else {
+#pragma unroll
+                    for (int a = 0; a < D; ++a) fa[a] += vf * Vji[a] + sf * Rji[a] - pc * g[a];
+                }
             FJ_TILE_ENTRIES_END(cu, nl)
 #pragma unroll
             for (int a = 0; a < D; ++a) acc[a] += double(fa[a]);
