@@ -2968,6 +2968,10 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_m(Consts c, DevState
     const float pi3o4H = float(3.0 * kPi / 4.0 / c.H);
     const float inv_H = float(1.0 / c.H);
     const float fgc = float(5.0 * c.Wc / c.H_sq);
+    /* ALE form of W'(r)/r = 5 Wc / H^2 (1 - r / 2H)^3: (kc - kc r / 2H)^3,
+     * kc the cube root of 5 Wc / H^2 */
+    const double kcd = cbrt(5.0 * c.Wc / c.H_sq);
+    const float kc = float(kcd), kq = float(-0.5 * kcd / c.H), rmin = float(1e-12 * c.H);
     const float eps2 = float(0.001 * c.H_sq);
     const float nu = float(c.nu);
     const float contang_fac = float(1.0 + 0.5 * cos(0.5 * kPi * 7.0 / 9.0));
@@ -3052,20 +3056,29 @@ __global__ __launch_bounds__(TPB) FJ_FORCE_OCC void k_force_m(Consts c, DevState
                 const float mj = fabsf(msg);
                 const float volj = mj * irj;
                 const float idist2 = fast_rcp<float>(rr + eps2);
-                const float qh = r * inv_H;
-                const float tq = 1.f - 0.5f * qh;
                 /* W'(r)/r is zero below r = 1e-12 H (src/Kernel.h:48-61); with r
                  * from the current positions such an FP32 distance is zero and so
                  * is R_ji, which carries every term of the pair */
-                const float fg = (FR && qh < 1e-12f) ? 0.f : fgc * (tq * tq * tq);
+                float fg;
+                if constexpr (ALE) {
+                    const float tq = fmaf(r, kq, kc);
+                    fg = (FR && r < rmin) ? 0.f : tq * tq * tq;
+                } else {
+                    const float qh = r * inv_H;
+                    const float tq = 1.f - 0.5f * qh;
+                    fg = (FR && qh < 1e-12f) ? 0.f : fgc * (tq * tq * tq);
+                }
                 float g[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) g[a] = fg * Rji[a];
                 const float pc = mj * (Pi + Pj);
-                float rdotg = Rji[0] * g[0], vg = Vji[0] * g[0];
+                /* with ALE and r from the current positions, R_ji . g is
+                 * W'(r)/r r^2 */
+                constexpr bool rdot_rr = ALE && !FR;
+                float rdotg = rdot_rr ? fg * rr : Rji[0] * g[0], vg = Vji[0] * g[0];
 #pragma unroll
                 for (int a = 1; a < D; ++a) {
-                    rdotg += Rji[a] * g[a];
+                    if constexpr (!rdot_rr) rdotg += Rji[a] * g[a];
                     vg += Vji[a] * g[a];
                 }
                 float vf = mj * nu * (iri + irj) * rdotg * idist2;
