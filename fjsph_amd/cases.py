@@ -568,6 +568,20 @@ def fluid_block(dim, n_side, dx=1e-3, jitter=0.0, seed=0, precision=abi.PREC_FP6
     return _finish(f"block_{dim}d", s, xb, xf, 0.0, f"{dim}D block {n_side}^{dim}")
 
 
+def irregular_cloud(dim, H_fac=1.3, dx=1e-3, seed=0, precision=abi.PREC_FP64):
+    """A block jittered by up to half a spacing per component beside a denser
+    Gaussian blob: cells of uneven counts and no lattice order (the test
+    workload of the neighbour search on irregular geometry)."""
+    n_side = 40 if dim == 2 else 18
+    blk = fluid_block(dim, n_side, dx=dx, jitter=0.5, seed=seed, H_fac=H_fac, precision=precision)
+    rng = np.random.default_rng(seed + 1)
+    n_blob = 1500 if dim == 2 else 2500
+    blob = rng.normal(0.0, 3.0 * dx, size=(n_blob, dim)) + blk.x.max(axis=0)
+    xf = np.concatenate([blk.x, blob], axis=0)
+    return _finish(f"cloud_{dim}d", blk.settings, np.zeros((0, dim)), xf, 0.0,
+                   f"{dim}D jittered block + Gaussian blob: {xf.shape[0]} particles, seed {seed}")
+
+
 def batched_droplets(n_drops=256, dx=5e-5, seed=99, precision=abi.PREC_MIXED, d_range=(0.5e-3, 2e-3),
                      u_range=(5.0, 40.0), spacing_factor=1.5):
     """Many independent 3D droplets of uneven size in one particle set.

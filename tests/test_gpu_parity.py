@@ -353,6 +353,22 @@ def test_neighbours_full_droplet():
     assert same_neighbour_sets(off_e, idx_e, off_o, idx_o)
 
 
+@pytest.mark.parametrize("dim,H_fac,targets", [(2, 1.3, 256), (2, 2.0, 64), (3, 1.3, 256), (3, 2.0, 1024)])
+def test_neighbours_irregular(dim, H_fac, targets):
+    """Bit-exact neighbour sets on irregular clouds, at several tile-block
+    sizes (the wave's reach box, the two FP32 masks and the target's own
+    slot depend on how targets and candidates fall into blocks and tiles)."""
+    case = cases.irregular_cloud(dim, H_fac=H_fac, seed=dim * 10 + int(H_fac * 10))
+    print(f"[neighbours irregular] dim={dim} H_fac={H_fac} targets={targets} n={case.n}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    eng.set_block_targets(targets)
+    eng.build_neighbours()
+    orc.build_neighbours()
+    off_e, idx_e = eng.neighbours()
+    off_o, idx_o, _ = orc.neighbours()
+    assert same_neighbour_sets(off_e, idx_e, off_o, idx_o)
+
+
 def test_dam_break_drift_fp64():
     """2D dam break (dx = 0.04): 40 steps, FP64 engine vs FP64 oracle."""
     case = cases.dam_break_2d(dx=0.04)
