@@ -2325,10 +2325,12 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
                     fcurve += volj * tt;
                 }
                 if (occl) {
+                    /* -R_ij . Vd / r with the hardware reciprocal (1 ulp) in
+                     * place of the IEEE division */
                     float q = Rij[0] * Vd[0];
 #pragma unroll
                     for (int a = 1; a < D; ++a) q += Rij[a] * Vd[a];
-                    q = -q / r;
+                    q = -q * fast_rcp<float>(r);
                     if (q > qmax) qmax = q;
                 }
                 zone = zone || N.w != 0.f;
