@@ -38,13 +38,17 @@ def force_flops_per_pair(dim, ale):
     """FP32 operations of one pair in k_force_m (FMA = 2) as it evaluates
     them, frozen-distance form: R_ji and v_ji with the target's rounding
     residual (4D), build-position difference (2D), r^2 (2D-1), sqrt (1), V_j
-    (1), 1/(r^2+eps) (2), W'/r (6), gradient (D), pressure factor (2), r.g and
-    v.g (2(2D-1)), viscous factor (5), surface tension (6), momentum sums
-    (6D); with ALE vPert_j.g (2D-1), its share of the viscous factor (2), the
-    two density sums (6) and sum_j V_j g (2D); without it the continuity sum
-    (2)."""
-    base = 4 * dim + 2 * dim + (2 * dim - 1) + 1 + 1 + 2 + 6 + dim + 2 + 2 * (2 * dim - 1) + 5 + 6 + 6 * dim
-    return base + ((2 * dim - 1) + 2 + 6 + 2 * dim if ale else 2)
+    (1), 1/(r^2+eps) (2), gradient (D), pressure factor (2), r.g and v.g
+    (2(2D-1)), viscous factor (5).  Without ALE: W'/r (6), surface tension
+    (6, every fluid pair), momentum sums (6D) and the continuity sum (2).
+    With ALE: W'/r folded to (kc - kc r/2H)^3 (4), momentum sums without the
+    surface-tension term (4D; surface tension, 6 + 2D, runs only in the thin
+    surface zone and is not counted), vPert_j.g (2D-1), its share of the
+    viscous factor (2), the two density sums (6) and sum_j V_j g (2D)."""
+    base = 4 * dim + 2 * dim + (2 * dim - 1) + 1 + 1 + 2 + dim + 2 + 2 * (2 * dim - 1) + 5
+    if ale:
+        return base + 4 + 4 * dim + (2 * dim - 1) + 2 + 6 + 2 * dim
+    return base + 6 + 6 + 6 * dim + 2
 
 
 METRIC = "particle-steps/s and %HBM roofline at 4M 3D particles; surrogate vs mesh aero"
