@@ -341,6 +341,37 @@ def test_step_parity_mixed_ale_2d():
     print("mixed 2D ALE errors:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
 
 
+def test_steps_parity_mixed_jet3d_ale():
+    """The 3D jet column as the throughput runs it (mixed precision, ALE
+    shifting, fused surface/shift pre-pass, surface tension), unperturbed,
+    over six steps (start-up pass + 5) against the FP64 oracle: the per-step
+    FP32 pair noise must not grow over the steps beyond MIXED_STEP."""
+    case = cases.jet_column_3d(n_target=6000, diameter=0.6e-3, dx=5e-5, precision=abi.PREC_MIXED)
+    assert case.settings.ale
+    print(f"[mixed steps, 3D ALE] jet n={case.n}", flush=True)
+    eng, orc = _engine(case), _oracle(case)
+    eng.integrate_no_update()
+    orc.integrate_no_update()
+    for k in range(5):
+        ie, io = eng.integrate_step(), orc.integrate_step()
+        print(f"step {k}: dt engine {ie['dt']:.6e} oracle {io['dt']:.6e}, sub-iterations: engine "
+              f"{ie['iterations']}, oracle {io['iterations']}", flush=True)
+        assert ie["dt"] == pytest.approx(io["dt"], rel=MIXED_STEP)
+    rho0 = case.settings.rho_rest
+    fluid = slice(case.n_bound, case.n)
+    errs = {}
+    for f, g, r in (("X-x0", eng.field("X") - case.x, orc.field("X") - case.x),
+                    ("V", eng.field("V"), orc.field("V")),
+                    ("RHO-rho0", eng.field("RHO") - rho0, orc.field("RHO") - rho0),
+                    ("ACC", eng.field("ACC"), orc.field("ACC")),
+                    ("CURVE", eng.field("CURVE")[fluid], orc.field("CURVE")[fluid]),
+                    ("WOCCL", eng.field("WOCCL")[fluid], orc.field("WOCCL")[fluid])):
+        errs[f] = assert_close(f, g, r, MIXED_STEP)
+    assert np.array_equal(eng.field("SURFZONE"), orc.field("SURFZONE"))
+    errs["VPERT"] = assert_close("VPERT", eng.field("VPERT")[fluid], orc.field("VPERT")[fluid], MIXED_VPERT)
+    print("mixed 3D ALE errors after 5 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+
+
 def test_neighbours_full_droplet():
     """Bit-exact sorted neighbour sets on the 268k-particle droplet IC."""
     case = cases.droplet_3d()
