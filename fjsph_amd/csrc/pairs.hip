@@ -1445,8 +1445,10 @@ __global__ __launch_bounds__(TPB) FJ_SURF1_OCC void k_surface1_m(Consts c, DevSt
 #pragma unroll
                             for (int a = 1; a < D; ++a) z += nhf[a] * Rji[a];
                             /* acos of a cosine past 1 (moved positions,
-                             * frozen r) is NaN: no hit */
-                            const float cq = z / r;
+                             * frozen r) is NaN: no hit.  The hardware
+                             * reciprocal (1 ulp) sits far inside tolc; r = 0
+                             * gives NaN either way, which is undecided */
+                            const float cq = z * fast_rcp<float>(r);
                             q = (cq - cos45) * (1.f - cq);
                             sure = sure && fabsf(cq - cos45) >= tolc && fabsf(1.f - cq) >= tolc;
                         }
@@ -2198,7 +2200,7 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
     __shared__ float4 tA[TILE];            /* x_j - xref, +V_j (-V_j: j is PISTON or BOUND) */
     __shared__ float4 tN[TILE];            /* surface normal of j (Detect_Surface loop 1), surf_j */
     __shared__ float4 tV[TILE];            /* v_j - vref */
-    __shared__ double tn[D][TILE];         /* unit normal of j */
+    __shared__ int32_t tsl[TILE];          /* particle slot of j (FP64 normal of an undecided angle test) */
     __shared__ float4 tX[FR ? TILE : 1];   /* xb_j - xbref */
     __shared__ BlockHead h;
     block_head<D>(tg, h);
@@ -2215,6 +2217,12 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
         }
     }
     const float Ht = float(c.H), inv_Hf = float(1.0 / c.H), Wcf = float(c.Wc), inv_Wdx = float(1.0 / c.W_dx);
+    /* particle_shift's angle test, acos(n_i . n_j) >= pi/12 over the fluid
+     * j whose dot lies in [-1, 1], decided on the FP32 dot away from its two
+     * thresholds: the FP32 dot of the rounded unit normals is within 4e-7 of
+     * the FP64 one, the margin is 1e-5 */
+    const double cos15 = cos(kPi / 12.0);
+    const float ang_yes_lo = float(-1.0 + 1e-5), ang_yes_hi = float(cos15 - 1e-5), ang_no = float(cos15 + 1e-5);
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
         const int p = pb + threadIdx.x;
         const bool active = p < h.p1;
@@ -2253,9 +2261,9 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
         bool ni_nz = false;
 #pragma unroll
         for (int a = 0; a < D; ++a) ni_nz = ni_nz || ni[a] != 0.f;
-        double curve = 0.0, dmin = 2.0;
+        double curve = 0.0;
         float fcurve = 0.f, qmax = 0.f, maxU2 = 0.f;
-        bool zone = fluid && s.surf[i] == 1;
+        bool zone = fluid && s.surf[i] == 1, wide = false;
         Rows cu;
         cu.init(nl, p, active ? nl.count[i] : 0, fluid);
         TileSlots ts;
@@ -2270,13 +2278,12 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
                       xa[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    const double nj = ax.nhat[(int64_t)sl * D + a];
                     pa[a] = float(s.x[(int64_t)sl * D + a] - xref[a]);
                     va[a] = float(s.v[(int64_t)sl * D + a] - vref[a]);
-                    na[a] = float(nj);
-                    tn[a][t] = nj;
+                    na[a] = float(ax.nhat[(int64_t)sl * D + a]);
                     if constexpr (FR) xa[a] = float(ax.xb[(int64_t)sl * D + a] - xbref[a]);
                 }
+                tsl[t] = sl;
                 const float4 r2 = ax.fs2[sl]; /* V, .., m negative for PISTON and BOUND */
                 tA[t] = make_float4(pa[0], pa[1], pa[2], copysignf(r2.x, r2.z));
                 tN[t] = make_float4(na[0], na[1], na[2], s.surf[sl] == 1 ? 1.f : 0.f);
@@ -2348,11 +2355,19 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
                         const float dv = (vj[a] - vit[a]) - vlo[a];
                         dv2 += dv * dv;
                     }
-                    if (A.w > 0.f) {
-                        double d = 0.0;
+                    if (A.w > 0.f && !wide) {
+                        float d = ni[0] * nj[0];
 #pragma unroll
-                        for (int a = 0; a < D; ++a) d += nhat[a] * tn[a][t];
-                        if (d >= -1.0 && d <= 1.0 && d < dmin) dmin = d;
+                        for (int a = 1; a < D; ++a) d += ni[a] * nj[a];
+                        if (d >= ang_yes_lo && d <= ang_yes_hi) wide = true;
+                        else if (!(d >= ang_no)) {
+                            /* rare: within the margin of cos(pi/12) or -1 */
+                            const int32_t sl = tsl[t];
+                            double d64 = 0.0;
+#pragma unroll
+                            for (int a = 0; a < D; ++a) d64 += nhat[a] * ax.nhat[(int64_t)sl * D + a];
+                            if (d64 >= -1.0 && d64 <= 1.0 && acos(d64) >= kPi / 12.0) wide = true;
+                        }
                     }
                     maxU2 = fmaxf(maxU2, dv2);
                 }
@@ -2383,7 +2398,9 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
         double vi[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) vi[a] = work ? s.v[i * D + a] : 0.0;
-        shift_out<D>(c, s, i, work, lam_nb, vi, nhat, du, double(maxU2), dmin, zone ? 1 : 0);
+        /* min dot 0 (acos = pi/2) when some angle reached pi/12, else 2
+         * (no dot in range) */
+        shift_out<D>(c, s, i, work, lam_nb, vi, nhat, du, double(maxU2), wide ? 0.0 : 2.0, zone ? 1 : 0);
     }
 }
 
