@@ -2217,10 +2217,11 @@ __global__ __launch_bounds__(TPB) FJ_SURF2SHIFT_OCC void k_surf2_shift_m(Consts 
         }
     }
     const float Ht = float(c.H), inv_Hf = float(1.0 / c.H), Wcf = float(c.Wc), inv_Wdx = float(1.0 / c.W_dx);
-    /* particle_shift's angle test, acos(n_i . n_j) >= pi/12 over the fluid
-     * j whose dot lies in [-1, 1], decided on the FP32 dot away from its two
-     * thresholds: the FP32 dot of the rounded unit normals is within 4e-7 of
-     * the FP64 one, the margin is 1e-5 */
+    /* particle_shift's angle test (src/Shifting.cpp:241-243, 273),
+     * acos(n_i . n_j) >= pi/12 for some fluid j whose dot lies in [-1, 1],
+     * decided on the FP32 dot away from its two thresholds: the FP32 dot of
+     * the rounded unit normals is within 4e-7 of the FP64 one, the margin
+     * is 1e-5 */
     const double cos15 = cos(kPi / 12.0);
     const float ang_yes_lo = float(-1.0 + 1e-5), ang_yes_hi = float(cos15 - 1e-5), ang_no = float(cos15 + 1e-5);
     for (int pb = h.p0; pb < h.p1; pb += TPB) {
